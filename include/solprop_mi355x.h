@@ -1,0 +1,225 @@
+/*
+ * C ABI of the MI355X propagation library (libsolprop_mi355x.so).
+ *
+ * The library replaces the compiled half of solprop's native backend: the nanobind
+ * module `_core` (native/src/bindings/module.cpp:786-832; Python stub
+ * native/src/solprop_native/_core.pyi). Its Problem record, Workspace, Session and
+ * Probe become the plain structs and entry points below: plain pointers and sizes, no
+ * Python or torch types. Host arrays are copied in at create/load and copied out at
+ * export, as the reference's are (native/src/general/trial.cpp:356-390; backends.md
+ * "Outputs: host arrays only"). Device buffers are owned by the workspace for its life.
+ *
+ * Every entry point returns 0 on success or a status (SP_VALUE, SP_BACKEND,
+ * SP_CONVERGENCE, SP_STATE) and writes a message into `msg` (at most `msglen` bytes,
+ * NUL-terminated) -- the reference's refusals and errors
+ * (native/src/solprop_native/_core.pyi:23-28 BackendRefusal; kernel-contract.md
+ * "Native errors carry a stage, a category and a message").
+ *
+ * Complex arrays are interleaved float64 pairs (numpy complex128 layout).
+ */
+#ifndef SOLPROP_MI355X_H
+#define SOLPROP_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SP_OK 0
+#define SP_VALUE 1
+#define SP_BACKEND 2
+#define SP_CONVERGENCE 3
+#define SP_STATE 4
+
+#define SP_MAX_TERMS 4
+
+/* A scalar function of z (the reference's stage tables, translate/stage.py:170 and
+ * native/include/solprop_native/march/stage_tables.hpp): kind 0 constant `value`,
+ * kind 1 the position z itself, kind 2 piecewise Chebyshev on `n_intervals` intervals
+ * with `edges` (n_intervals+1) and `coefficients` (n_intervals, degree+1), evaluated by
+ * Clenshaw's recurrence. */
+typedef struct sp_scalar_path {
+  int kind;
+  double value;
+  int n_intervals;
+  int degree;
+  const double* edges;
+  const double* coefficients;
+} sp_scalar_path;
+
+/* The problem record: the subset of `_core.Problem` (_core.pyi:51-466) that the
+ * envelope-GNLSE and modal field-grid routes read. Shapes: C = n_modes, N = n_omega,
+ * T = n_to, Q = n_nodes. */
+typedef struct sp_problem_desc {
+  /* Dims (_core.pyi Problem.Dims) */
+  int kind; /* 0 envelope (complex time samples), 1 field (real) */
+  int n_t, n_to, n_modes, n_nodes, n_omega;
+  /* Grid (Problem.Grid) */
+  const int64_t* crop_index;        /* (N) */
+  const uint8_t* active_mask;       /* (N) */
+  const uint8_t* norm_mask;         /* (N) */
+  const double* omega_window;       /* (N) */
+  const double* omega;              /* (N) absolute angular frequency */
+  const double* time_window;        /* (T) */
+  const double* time_window_derivative; /* (T) 1/s */
+  double dt_fine;
+  double z_end;
+  /* Linear (Problem.Linear): ΔΦ = Σ_t (W_t(z1) - W_t(z0)) Φ_t, prefactor Σ p_t(z) P_t */
+  int n_phi;
+  const double* phi_terms;          /* complex (n_phi, C, N) */
+  sp_scalar_path phi_paths[SP_MAX_TERMS];
+  int n_prefactor;
+  const double* prefactor_terms;    /* complex (n_prefactor, C, N) */
+  sp_scalar_path prefactor_paths[SP_MAX_TERMS];
+  /* Transform (Problem.Transform) */
+  const double* synthesis;          /* (Q, C) */
+  const double* projection;         /* (C, Q) */
+  sp_scalar_path density;
+  sp_scalar_path preionisation;
+  /* Kerr (Problem.Kerr) */
+  int has_kerr;
+  int kerr_real;                    /* 1: real cubic form, 0: analytic/envelope form */
+  double kerr_coefficient, kerr_self, kerr_conjugate;
+  /* Raman (Problem.Raman) */
+  int n_raman;                      /* rows; 0 for none */
+  const double* raman_undamped;     /* (n_raman, 2T) */
+  double raman_fraction, raman_epsilon0;
+  int raman_thg;
+  sp_scalar_path raman_rates[SP_MAX_TERMS];
+  /* Plasma (Problem.Plasma), one channel */
+  int has_plasma;
+  double charge_mass_ratio, ionisation_potential;
+  double log_field_min, d_log_field, field_min, field_max, rate_max;
+  int rate_intervals;
+  const double* rate_coefficients;  /* (4, rate_intervals) highest power first */
+  int n_taps;                       /* 1: trapezoid; >1: band rule taps c_0..c_K */
+  const double* taps;
+  /* Stepper (Problem.Stepper) */
+  int n_stages;
+  const double* tableau_a;          /* (n_stages, n_stages) */
+  const double* tableau_c;          /* (n_stages) */
+  const double* tableau_b;          /* (n_stages) */
+  const double* tableau_e;          /* (n_stages) b - bhat */
+  int first_same_as_last;
+  /* Noise: 0 none, 1 input (already in the launch field), 2 modified (per stage) */
+  int noise_model;
+  const double* noise;              /* complex (C, N) or NULL */
+  /* Controller (PIController, src/solprop/stepper/controller.py:262) and landings */
+  double rtol, atol, safety, max_factor, min_factor, beta1, beta2;
+  int lag, reject_free, estimator_order, auto_init;
+  double dz_min, dz_max, stretch;
+  double dz_init;                   /* the first proposal, m */
+} sp_problem_desc;
+
+typedef struct sp_workspace sp_workspace;
+
+/* `Workspace(problem)` (_core.pyi:595-607; native/include/.../workspace.hpp:139):
+ * validates, allocates every buffer and plans every transform once. */
+int sp_workspace_create(const sp_problem_desc* desc, sp_workspace** out, char* msg,
+                        size_t msglen);
+/* `Workspace.close()`; NULL is accepted. */
+void sp_workspace_destroy(sp_workspace* ws);
+
+/* Load a committed state at z (the reference's upload at start, trial.cpp:356-390).
+ * `field` is complex (C, N). Resets the march and any held trial. */
+int sp_workspace_load(sp_workspace* ws, double z, const double* field, char* msg,
+                      size_t msglen);
+
+/* `Probe.trial(z, dz) -> float` (_core.pyi:36-49; native probe, workspace.hpp:209-214):
+ * one embedded Runge-Kutta trial from the committed state, every stage evaluated,
+ * committing nothing; returns the scaled error norm. */
+int sp_workspace_trial(sp_workspace* ws, double z, double dz, double* norm, char* msg,
+                       size_t msglen);
+/* `Probe.field()`: the held trial's end state, complex (C, N). */
+int sp_workspace_trial_field(sp_workspace* ws, double* out, char* msg, size_t msglen);
+/* The committed state, complex (C, N) (the raw state before the export recipe). */
+int sp_workspace_state(sp_workspace* ws, double* out, char* msg, size_t msglen);
+
+/* One march call (`Session.advance(work_limit, deadline)`, _core.pyi:468-499;
+ * native/src/session/session.cpp:695): step adaptively towards `target`, at most
+ * `work_limit` attempts. Writes the attempt rows (z_end, dz, norm, accepted) into
+ * `rows` (4 doubles each, up to `max_rows`), sets `n_rows`, the new position `z` and
+ * `done` (1 when the target was reached). SP_CONVERGENCE on the step floor. */
+typedef struct sp_march_result {
+  double z;
+  double proposal;
+  int done;
+  int n_rows;
+  long attempts;
+  long accepted;
+  long evaluations;
+} sp_march_result;
+int sp_workspace_advance(sp_workspace* ws, double target, long work_limit, double* rows,
+                         long max_rows, sp_march_result* out, char* msg, size_t msglen);
+
+/* `Session.export(ticket, recipe, out)` (native/src/solprop_native/session.py:960;
+ * src/solprop/backends/export.py:52-207): the committed state converted by the recipe,
+ * to the bit: first the product with sqrt(w) when `power_weight` (complex-state shape
+ * (C, N), float64) is given, then, when `optics` is set, conj(.) times (-1)^k on the
+ * storage index k. `out` is complex (C, N). */
+int sp_workspace_export(sp_workspace* ws, const double* power_weight, int optics,
+                        double* out, char* msg, size_t msglen);
+
+/* Counters (`Workspace.counters()`): trials, evaluations, accepted steps, graph
+ * replays, device bytes held. */
+typedef struct sp_counters {
+  long trials;
+  long evaluations;
+  long accepted;
+  long graph_launches;
+  long device_bytes;
+} sp_counters;
+int sp_workspace_counters(sp_workspace* ws, sp_counters* out);
+
+/* Device timing of the last trials: the average duration in microseconds of the
+ * whole trial and of each kernel class since the last reset, from HIP events on the
+ * workspace's stream. Enabled by `sp_workspace_profile(ws, 1)`. */
+int sp_workspace_profile(sp_workspace* ws, int enable);
+int sp_workspace_timing(sp_workspace* ws, double* trial_us, long* trials);
+
+/* -------------------------------------------------------------------- traces */
+
+/* A scripted Runge-Kutta march (tests/backends/rk_traces.py:63-141 Scenario). */
+typedef struct sp_rk_scenario {
+  double rtol, atol, safety, max_factor, min_factor, beta1, beta2;
+  int lag, reject_free, estimator_order;
+  double z0, proposal, dz_min, dz_max, z_end, stretch;
+  int auto_init, interval_cap, step_planes;
+  int has_h0;
+  double h0;
+  const double* norms; long n_norms;
+  const double* save_planes; int n_planes;
+  const double* breakpoints; int n_breakpoints;
+  const double* kinks; int n_kinks;
+  const double* targets; int n_targets;
+  const double* schedule; long n_schedule;
+  const long* quiet_keys; const int* quiet_values; int n_quiet;
+  int has_quiet_interval;
+  double quiet_interval;
+  int has_ceiling;
+  double ceiling;
+  const long* split_keys; const double* split_values; int n_split;
+} sp_rk_scenario;
+
+/* Run the library's Runge-Kutta march over a scripted engine and write its decision
+ * trace as JSON (the layout of the reference's tests/data/rk_traces.json.gz) into
+ * `out`. Returns SP_VALUE if `outlen` is too small. */
+int sp_rk_trace(const sp_rk_scenario* scenario, char* out, size_t outlen);
+
+/* Testing entry (the counterpart of the reference's `_core._testing`,
+ * native/src/bindings/testing.cpp): one batched complex FFT of `n` points over
+ * `batch` contiguous rows, unnormalised, sign -1 forward or +1 inverse, through the
+ * library's own Stockham plan; host arrays in and out (complex, batch x n). */
+int sp_testing_fft(const double* in, double* out, long n, long batch, int sign, char* msg,
+                   size_t msglen);
+
+/* Library identification: the gfx target and build string. */
+const char* sp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOLPROP_MI355X_H */

@@ -1,0 +1,165 @@
+// The extern "C" boundary (include/solprop_mi355x.h): status codes and messages in
+// place of exceptions, host arrays in and out.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "trace.hpp"
+#include "workspace.hpp"
+
+struct sp_workspace {
+  sp::Workspace impl;
+  explicit sp_workspace(const sp_problem_desc& d) : impl(d) {}
+};
+
+namespace {
+
+void write_message(char* msg, std::size_t len, const char* text) {
+  if (!msg || len == 0) return;
+  std::snprintf(msg, len, "%s", text);
+}
+
+template <typename F>
+int guarded(char* msg, std::size_t len, F&& body) {
+  try {
+    body();
+    write_message(msg, len, "");
+    return SP_OK;
+  } catch (const sp::Error& e) {
+    write_message(msg, len, e.what());
+    return e.code;
+  } catch (const sp::march::ConvergenceError& e) {
+    write_message(msg, len, e.what());
+    return SP_CONVERGENCE;
+  } catch (const std::invalid_argument& e) {
+    write_message(msg, len, e.what());
+    return SP_VALUE;
+  } catch (const std::bad_alloc&) {
+    write_message(msg, len, "host allocation failed");
+    return SP_BACKEND;
+  } catch (const std::exception& e) {
+    write_message(msg, len, e.what());
+    return SP_BACKEND;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sp_workspace_create(const sp_problem_desc* desc, sp_workspace** out, char* msg,
+                        size_t msglen) {
+  if (!desc || !out) {
+    write_message(msg, msglen, "create needs a problem and an output handle");
+    return SP_VALUE;
+  }
+  *out = nullptr;
+  return guarded(msg, msglen, [&] { *out = new sp_workspace(*desc); });
+}
+
+void sp_workspace_destroy(sp_workspace* ws) { delete ws; }
+
+int sp_workspace_load(sp_workspace* ws, double z, const double* field, char* msg,
+                      size_t msglen) {
+  if (!ws) return SP_VALUE;
+  return guarded(msg, msglen, [&] { ws->impl.load(z, field); });
+}
+
+int sp_workspace_trial(sp_workspace* ws, double z, double dz, double* norm, char* msg,
+                       size_t msglen) {
+  if (!ws || !norm) return SP_VALUE;
+  return guarded(msg, msglen, [&] { *norm = ws->impl.trial(z, dz, true); });
+}
+
+int sp_workspace_trial_field(sp_workspace* ws, double* out, char* msg, size_t msglen) {
+  if (!ws || !out) return SP_VALUE;
+  return guarded(msg, msglen, [&] { ws->impl.read_trial(out); });
+}
+
+int sp_workspace_state(sp_workspace* ws, double* out, char* msg, size_t msglen) {
+  if (!ws || !out) return SP_VALUE;
+  return guarded(msg, msglen, [&] { ws->impl.read_state(out); });
+}
+
+int sp_workspace_advance(sp_workspace* ws, double target, long work_limit, double* rows,
+                         long max_rows, sp_march_result* out, char* msg, size_t msglen) {
+  if (!ws || !out) return SP_VALUE;
+  std::memset(out, 0, sizeof(*out));
+  std::vector<double> held;
+  const int status = guarded(msg, msglen, [&] {
+    ws->impl.advance(target, work_limit > 0 ? work_limit : 1, held, out);
+  });
+  const long n = static_cast<long>(held.size() / 4);
+  const long kept = rows ? (n < max_rows ? n : max_rows) : 0;
+  if (kept > 0) std::memcpy(rows, held.data(), sizeof(double) * 4 * kept);
+  out->n_rows = static_cast<int>(kept);
+  return status;
+}
+
+int sp_workspace_export(sp_workspace* ws, const double* power_weight, int optics,
+                        double* out, char* msg, size_t msglen) {
+  if (!ws || !out) return SP_VALUE;
+  return guarded(msg, msglen, [&] { ws->impl.export_state(power_weight, optics != 0, out); });
+}
+
+int sp_workspace_counters(sp_workspace* ws, sp_counters* out) {
+  if (!ws || !out) return SP_VALUE;
+  *out = ws->impl.counters();
+  return SP_OK;
+}
+
+int sp_workspace_profile(sp_workspace* ws, int enable) {
+  if (!ws) return SP_VALUE;
+  ws->impl.set_profile(enable != 0);
+  return SP_OK;
+}
+
+int sp_workspace_timing(sp_workspace* ws, double* trial_us, long* trials) {
+  if (!ws || !trial_us || !trials) return SP_VALUE;
+  *trial_us = ws->impl.trial_us();
+  *trials = ws->impl.timed();
+  return SP_OK;
+}
+
+int sp_rk_trace(const sp_rk_scenario* scenario, char* out, size_t outlen) {
+  if (!scenario || !out) return SP_VALUE;
+  std::string trace;
+  try {
+    trace = sp::march::run_trace(*scenario);
+  } catch (const std::exception& e) {
+    std::snprintf(out, outlen, "%s", e.what());
+    return SP_BACKEND;
+  }
+  if (trace.size() + 1 > outlen) return SP_VALUE;
+  std::memcpy(out, trace.c_str(), trace.size() + 1);
+  return SP_OK;
+}
+
+int sp_testing_fft(const double* in, double* out, long n, long batch, int sign, char* msg,
+                   size_t msglen) {
+  if (!in || !out) return SP_VALUE;
+  return guarded(msg, msglen, [&] {
+    SP_CHECK(sign == -1 || sign == 1, "sign must be -1 or +1");
+    SP_CHECK(n > 0 && batch > 0, "n and batch must be positive");
+    sp::FftPlan plan;
+    plan.init(static_cast<std::size_t>(n), static_cast<std::size_t>(batch));
+    const std::size_t bytes = sizeof(sp::cplx) * static_cast<std::size_t>(n * batch);
+    sp::cplx *data = nullptr, *scratch = nullptr;
+    hipStream_t s = nullptr;
+    SP_HIP(hipStreamCreate(&s));
+    SP_HIP(hipMalloc(&data, bytes));
+    SP_HIP(hipMalloc(&scratch, bytes));
+    SP_HIP(hipMemcpy(data, in, bytes, hipMemcpyHostToDevice));
+    plan.exec(data, scratch, sign, s);
+    SP_HIP(hipStreamSynchronize(s));
+    SP_HIP(hipMemcpy(out, data, bytes, hipMemcpyDeviceToHost));
+    SP_HIP(hipFree(data));
+    SP_HIP(hipFree(scratch));
+    SP_HIP(hipStreamDestroy(s));
+  });
+}
+
+const char* sp_version(void) { return "solprop_mi355x 0.1 gfx950 fp64"; }
+
+}  // extern "C"

@@ -1,0 +1,336 @@
+// Stockham radix-4/2 FFT passes staged in LDS, FP64, for gfx950.
+#include "fft.hpp"
+
+#include <cmath>
+
+namespace sp {
+namespace {
+
+constexpr int kMaxElems = 4096;  // complex doubles a workgroup holds in LDS (64 KiB)
+
+struct PassArgs {
+  long in_sj, in_ss, out_sk, out_ss, batch_in, batch_out;
+  int global_twiddle;
+  int mask_n;  // N - 1 for the global twiddle index
+};
+
+template <int LOG2L, int G>
+struct Shape {
+  static constexpr int L = 1 << LOG2L;
+  static constexpr int E = L * G;
+  static constexpr int NT = (E / 4 >= 256) ? 256 : (E / 4 >= 1 ? E / 4 : 1);
+  static constexpr int STRIDE = L + (G > 1 ? 1 : 0);  // pad rows against bank conflicts
+  static constexpr int PER4 = (E / 4 + NT - 1) / NT;
+  static constexpr int PER2 = (E / 2 + NT - 1) / NT;
+};
+
+__device__ inline cplx twiddle(const cplx* table, int index, int sign) {
+  const cplx w = table[index];
+  return sign > 0 ? make(w.x, -w.y) : w;
+}
+
+template <int LOG2L, int G>
+__global__ void __launch_bounds__((Shape<LOG2L, G>::NT))
+    fft_pass_kernel(const cplx* in, cplx* out,
+                    const cplx* __restrict__ tw_l, const cplx* __restrict__ tw_n,
+                    PassArgs a, int sign) {
+  using S = Shape<LOG2L, G>;
+  constexpr int L = S::L, E = S::E, NT = S::NT, STRIDE = S::STRIDE;
+  __shared__ cplx buf[G * STRIDE];
+  const int tid = threadIdx.x;
+  const long s0 = static_cast<long>(blockIdx.x) * G;
+  const cplx* src = in + static_cast<long>(blockIdx.y) * a.batch_in;
+  cplx* dst = out + static_cast<long>(blockIdx.y) * a.batch_out;
+
+  // Load G sequences, the contiguous axis fastest across the threads.
+  for (int e = tid; e < E; e += NT) {
+    int s, j;
+    if (a.in_ss == 1) {
+      s = e % G;
+      j = e / G;
+    } else {
+      j = e % L;
+      s = e / L;
+    }
+    buf[s * STRIDE + j] = src[j * a.in_sj + (s0 + s) * a.in_ss];
+  }
+  __syncthreads();
+
+  int p = 1;
+  // Radix-4 Stockham stages: inputs at i + m L/4, outputs at 4(i-k) + k + m p.
+  for (; 4 * p <= L; p *= 4) {
+    cplx v[S::PER4][4];
+    const int step = L / (4 * p);
+#pragma unroll
+    for (int r = 0; r < S::PER4; ++r) {
+      const int b = tid + r * NT;
+      if (b < E / 4) {
+        const int s = b / (L / 4), i = b % (L / 4);
+        const int k = i & (p - 1);
+        const cplx* x = buf + s * STRIDE;
+        cplx u0 = x[i], u1 = x[i + L / 4], u2 = x[i + L / 2], u3 = x[i + 3 * L / 4];
+        if (p > 1) {
+          u1 = mul(u1, twiddle(tw_l, k * step, sign));
+          u2 = mul(u2, twiddle(tw_l, 2 * k * step, sign));
+          u3 = mul(u3, twiddle(tw_l, 3 * k * step, sign));
+        }
+        const cplx a0 = add(u0, u2), a1 = sub(u0, u2), a2 = add(u1, u3);
+        const cplx a3 = rot(sub(u1, u3), sign);
+        v[r][0] = add(a0, a2);
+        v[r][1] = add(a1, a3);
+        v[r][2] = sub(a0, a2);
+        v[r][3] = sub(a1, a3);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < S::PER4; ++r) {
+      const int b = tid + r * NT;
+      if (b < E / 4) {
+        const int s = b / (L / 4), i = b % (L / 4);
+        const int k = i & (p - 1);
+        cplx* y = buf + s * STRIDE + ((i - k) << 2) + k;
+        y[0] = v[r][0];
+        y[p] = v[r][1];
+        y[2 * p] = v[r][2];
+        y[3 * p] = v[r][3];
+      }
+    }
+    __syncthreads();
+  }
+  if (p < L) {  // one radix-2 stage with p = L/2: output j = i
+    cplx v[S::PER2][2];
+#pragma unroll
+    for (int r = 0; r < S::PER2; ++r) {
+      const int b = tid + r * NT;
+      if (b < E / 2) {
+        const int s = b / (L / 2), i = b % (L / 2);
+        const cplx* x = buf + s * STRIDE;
+        const cplx u0 = x[i];
+        const cplx u1 = mul(x[i + L / 2], twiddle(tw_l, i, sign));
+        v[r][0] = add(u0, u1);
+        v[r][1] = sub(u0, u1);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < S::PER2; ++r) {
+      const int b = tid + r * NT;
+      if (b < E / 2) {
+        const int s = b / (L / 2), i = b % (L / 2);
+        cplx* y = buf + s * STRIDE;
+        y[i] = v[r][0];
+        y[i + L / 2] = v[r][1];
+      }
+    }
+    __syncthreads();
+  }
+
+  // Store, with the four-step twiddle W_N^{s k} where asked.
+  for (int e = tid; e < E; e += NT) {
+    int s, k;
+    if (a.out_ss == 1) {
+      s = e % G;
+      k = e / G;
+    } else {
+      k = e % L;
+      s = e / L;
+    }
+    cplx value = buf[s * STRIDE + k];
+    if (a.global_twiddle) {
+      const long idx = ((s0 + s) * static_cast<long>(k)) & a.mask_n;
+      value = mul(value, twiddle(tw_n, static_cast<int>(idx), sign));
+    }
+    dst[k * a.out_sk + (s0 + s) * a.out_ss] = value;
+  }
+}
+
+using KernelFn = void (*)(const cplx*, cplx*, const cplx*, const cplx*, PassArgs, int);
+
+template <int LOG2L, int G>
+constexpr KernelFn pick() {
+  if constexpr ((1 << LOG2L) * G <= kMaxElems)
+    return fft_pass_kernel<LOG2L, G>;
+  else
+    return nullptr;
+}
+
+template <int LOG2L>
+KernelFn kernel_for_g(int log2g) {
+  switch (log2g) {
+    case 0: return pick<LOG2L, 1>();
+    case 1: return pick<LOG2L, 2>();
+    case 2: return pick<LOG2L, 4>();
+    case 3: return pick<LOG2L, 8>();
+    case 4: return pick<LOG2L, 16>();
+    case 5: return pick<LOG2L, 32>();
+    case 6: return pick<LOG2L, 64>();
+    default: return nullptr;
+  }
+}
+
+KernelFn kernel_for(int log2l, int log2g) {
+  switch (log2l) {
+    case 2: return kernel_for_g<2>(log2g);
+    case 3: return kernel_for_g<3>(log2g);
+    case 4: return kernel_for_g<4>(log2g);
+    case 5: return kernel_for_g<5>(log2g);
+    case 6: return kernel_for_g<6>(log2g);
+    case 7: return kernel_for_g<7>(log2g);
+    case 8: return kernel_for_g<8>(log2g);
+    case 9: return kernel_for_g<9>(log2g);
+    case 10: return kernel_for_g<10>(log2g);
+    case 11: return kernel_for_g<11>(log2g);
+    case 12: return kernel_for_g<12>(log2g);
+    default: return nullptr;
+  }
+}
+
+int threads_for(int log2l, int log2g) {
+  const int e = 1 << (log2l + log2g);
+  const int nt = e / 4;
+  return nt >= 256 ? 256 : (nt >= 1 ? nt : 1);
+}
+
+int ilog2(std::size_t n) {
+  int m = 0;
+  while ((std::size_t{1} << m) < n) ++m;
+  return m;
+}
+
+// Sequences per workgroup: aim for 1024 points, never more than the sequences there
+// are, and a divisor of their count.
+int pick_log2g(int log2l, std::size_t n_seq) {
+  int g = 0;
+  while (g < 6 && (1 << (log2l + g + 1)) <= 1024 && (std::size_t{1} << (g + 1)) <= n_seq &&
+         n_seq % (std::size_t{1} << (g + 1)) == 0)
+    ++g;
+  return g;
+}
+
+}  // namespace
+
+std::vector<cplx> unit_roots(std::size_t n) {
+  // exp(-2 pi i t/n): the angle is reduced to an octant offset in [0, pi/4] in long
+  // double (exact, n a power of two) and mapped by the octant symmetries, reflecting
+  // the odd octants so that no cancellation occurs.
+  std::vector<cplx> w(n);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (std::size_t t = 0; t < n; ++t) {
+    const std::size_t q = (8 * t) / n;  // octant 0..7
+    const long double frac = static_cast<long double>(t) / static_cast<long double>(n);
+    long double c, s;
+    if (q % 2 == 0) {
+      const long double r = two_pi * (frac - static_cast<long double>(q) / 8.0L);
+      const long double cr = std::cos(r), sr = std::sin(r);
+      switch (q) {
+        case 0: c = cr; s = sr; break;
+        case 2: c = -sr; s = cr; break;
+        case 4: c = -cr; s = -sr; break;
+        default: c = sr; s = -cr; break;
+      }
+    } else {
+      const long double r = two_pi * (static_cast<long double>(q + 1) / 8.0L - frac);
+      const long double cr = std::cos(r), sr = std::sin(r);
+      switch (q) {
+        case 1: c = sr; s = cr; break;
+        case 3: c = -cr; s = sr; break;
+        case 5: c = -sr; s = -cr; break;
+        default: c = cr; s = -sr; break;
+      }
+    }
+    w[t] = make(static_cast<double>(c), -static_cast<double>(s));
+  }
+  return w;
+}
+
+FftPlan::~FftPlan() {
+  if (roots_n_) (void)hipFree(roots_n_);
+  for (auto* p : roots_l_)
+    if (p) (void)hipFree(p);
+}
+
+void FftPlan::init(std::size_t n, std::size_t batch) {
+  SP_CHECK(n >= 4 && (n & (n - 1)) == 0, "FFT length must be a power of two >= 4");
+  SP_CHECK(n <= (std::size_t{1} << 24), "FFT length above 2^24 is not supported");
+  SP_CHECK(batch >= 1, "FFT batch must be positive");
+  n_ = n;
+  batch_ = batch;
+  const int m = ilog2(n);
+  passes_.clear();
+  auto upload = [&](std::size_t len, cplx** dst) {
+    const auto host = unit_roots(len);
+    SP_HIP(hipMalloc(dst, len * sizeof(cplx)));
+    SP_HIP(hipMemcpy(*dst, host.data(), len * sizeof(cplx), hipMemcpyHostToDevice));
+    bytes_ += len * sizeof(cplx);
+  };
+  upload(n, &roots_n_);
+  if (n <= static_cast<std::size_t>(kMaxElems)) {
+    FftPass p;
+    p.log2L = m;
+    p.log2G = pick_log2g(m, batch);
+    p.in_sj = 1;
+    p.in_ss = static_cast<long>(n);
+    p.out_sk = 1;
+    p.out_ss = static_cast<long>(n);
+    p.batch_in = p.batch_out = 0;
+    p.n_seq = static_cast<int>(batch);
+    passes_.push_back(p);
+    upload(n, &roots_l_[0]);
+  } else {
+    const int m1 = m / 2, m2 = m - m1;
+    const long n1 = 1L << m1, n2 = 1L << m2;
+    FftPass p1;  // columns: length n1 over j1 (stride n2), sequences j2
+    p1.log2L = m1;
+    p1.log2G = pick_log2g(m1, static_cast<std::size_t>(n2));
+    p1.in_sj = n2;
+    p1.in_ss = 1;
+    p1.out_sk = n2;
+    p1.out_ss = 1;
+    p1.batch_in = p1.batch_out = static_cast<long>(n);
+    p1.n_seq = static_cast<int>(n2);
+    p1.global_twiddle = true;
+    p1.log2N = m;
+    FftPass p2;  // rows: length n2 over j2, sequences k1 (stride n2), transposed store
+    p2.log2L = m2;
+    p2.log2G = pick_log2g(m2, static_cast<std::size_t>(n1));
+    p2.in_sj = 1;
+    p2.in_ss = n2;
+    p2.out_sk = n1;
+    p2.out_ss = 1;
+    p2.batch_in = p2.batch_out = static_cast<long>(n);
+    p2.n_seq = static_cast<int>(n1);
+    passes_.push_back(p1);
+    passes_.push_back(p2);
+    upload(static_cast<std::size_t>(n1), &roots_l_[0]);
+    upload(static_cast<std::size_t>(n2), &roots_l_[1]);
+  }
+  for (const auto& p : passes_)
+    SP_CHECK(kernel_for(p.log2L, p.log2G) != nullptr, "no FFT pass kernel for this shape");
+}
+
+void FftPlan::launch(const FftPass& p, const cplx* in, cplx* out, int sign,
+                     hipStream_t stream) const {
+  const int log2g = p.log2G;
+  KernelFn fn = kernel_for(p.log2L, log2g);
+  const int g = 1 << log2g;
+  PassArgs a{p.in_sj, p.in_ss, p.out_sk, p.out_ss, p.batch_in, p.batch_out,
+             p.global_twiddle ? 1 : 0, static_cast<int>((1L << p.log2N) - 1)};
+  const unsigned groups = static_cast<unsigned>(p.n_seq / g);
+  const unsigned rows = p.batch_in == 0 && p.batch_out == 0 ? 1u : static_cast<unsigned>(batch_);
+  const cplx* tw_l = (&p == &passes_[0]) ? roots_l_[0] : roots_l_[1];
+  hipLaunchKernelGGL(fn, dim3(groups, rows), dim3(threads_for(p.log2L, log2g)), 0, stream,
+                     in, out, tw_l, roots_n_, a, sign);
+  SP_HIP(hipGetLastError());
+}
+
+void FftPlan::exec(cplx* data, cplx* scratch, int sign, hipStream_t stream) const {
+  if (passes_.size() == 1) {
+    launch(passes_[0], data, data, sign, stream);
+    return;
+  }
+  launch(passes_[0], data, scratch, sign, stream);
+  launch(passes_[1], scratch, data, sign, stream);
+}
+
+}  // namespace sp

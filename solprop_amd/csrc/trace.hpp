@@ -1,0 +1,15 @@
+// Scripted Runge-Kutta decision traces (see trace.cpp).
+#pragma once
+
+#include <string>
+
+extern "C" {
+#include "../../include/solprop_mi355x.h"
+}
+
+namespace sp::march {
+
+using TraceScenario = sp_rk_scenario;
+std::string run_trace(const TraceScenario& scenario);
+
+}  // namespace sp::march

@@ -1,0 +1,88 @@
+// The device workspace: the problem's arrays resident in HBM, the stage buffers, the
+// transform plans and the captured trial graphs (the reference's GeneralWorkspace,
+// native/src/general/general_workspace.hpp, and native::Workspace,
+// native/include/solprop_native/workspace/workspace.hpp:139).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "device.hpp"
+#include "fft.hpp"
+#include "march.hpp"
+#include "scalars.hpp"
+
+namespace sp {
+
+class Workspace {
+ public:
+  explicit Workspace(const sp_problem_desc& desc);
+  ~Workspace();
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
+
+  void load(double z, const double* field);
+  // One trial from the committed state at z; every stage evaluated when `full`.
+  double trial(double z, double dz, bool full);
+  void accept(double dz);
+  void read_state(double* out);
+  void read_trial(double* out);
+  void export_state(const double* weight, bool optics, double* out);
+  bool advance(double target, long work_limit, std::vector<double>& rows, sp_march_result* r);
+
+  void set_profile(bool on) { profile_ = on; }
+  double trial_us() const { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
+  long timed() const { return timed_; }
+  sp_counters counters() const;
+
+ private:
+  enum Stage0 { kEvaluate = 0, kCopyRecycled = 1, kKeep = 2 };
+  template <typename T>
+  T* device_copy(const T* host, std::size_t n);
+  template <typename T>
+  T* device_alloc(std::size_t n);
+  void fill_record(double z, double dz);
+  void enqueue_trial(bool commit, Stage0 mode);
+  void enqueue_stage(int stage);
+  void enqueue_raman_kernel(int stage);
+  void ensure_committed();
+  double finish_trial();
+
+  DeviceProblem P_{};
+  std::vector<void*> allocations_;
+  std::size_t device_bytes_ = 0;
+  hipStream_t stream_ = nullptr;
+  FftPlan fine_plan_, raman_plan_;
+  // Host scalar paths.
+  std::vector<ScalarPath> phi_paths_, pref_paths_, raman_paths_;
+  ScalarPath density_, preion_;
+  std::vector<double> tab_c_;
+  bool fsal_ = false, raman_dynamic_ = false;
+  // Device buffers.
+  cplx *y_ = nullptr, *adv_ = nullptr, *recycled_ = nullptr, *einv_ = nullptr, *k_ = nullptr;
+  cplx *fine_ = nullptr, *scratch_ = nullptr, *drive_ = nullptr, *H_ = nullptr;
+  cplx* roots2T_ = nullptr;
+  double* partials_ = nullptr;
+  TrialRecord* rec_dev_ = nullptr;
+  TrialRecord* rec_host_ = nullptr;
+  double* partials_host_ = nullptr;
+  int n_partial_blocks_ = 0;
+  // Graphs keyed by (commit, stage-0 mode).
+  std::map<int, hipGraphExec_t> graphs_;
+  bool use_graphs_ = true;
+  // March state.
+  march::Policy policy_;
+  march::State state_;
+  double dz_init_ = 0.0;
+  bool loaded_ = false, commit_pending_ = false, k0_valid_ = false, trial_held_ = false;
+  double k0_z_ = 0.0;
+  // Counters and timing.
+  long trials_ = 0, evaluations_ = 0, accepted_ = 0, graph_launches_ = 0;
+  bool profile_ = false;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  double time_total_ms_ = 0.0;
+  long timed_ = 0;
+};
+
+}  // namespace sp

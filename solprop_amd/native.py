@@ -1,0 +1,441 @@
+"""Host layer over the C ABI: the reference's native adapter shape.
+
+Mirrors ``native/src/solprop_native`` -- ``_lowering._lower`` (``_lowering.py:334-460``)
+fills the problem record, :class:`Workspace` is ``_core.Workspace``
+(``_core.pyi:595-607``), :meth:`Workspace.probe` returns a :class:`Probe`
+(``_core.pyi:36-49``, ``_probe.py:33-70``) and :meth:`Workspace.start` a
+:class:`Session` whose :meth:`Session.advance` marches natively until its target or
+its work bound (``_core.pyi:468-499``). Errors map onto the reference's classes:
+a refusal before allocation is :class:`BackendRefusal`, the step floor is
+:class:`ConvergenceError`, everything else :class:`BackendError`.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from solprop_amd import _lib
+from solprop_amd.problem import Problem, ScalarPath
+
+
+class BackendError(RuntimeError):
+    """A device or runtime failure (``src/solprop/exceptions.py:260``)."""
+
+
+class BackendRefusal(BackendError):
+    """The problem is outside what the library runs; raised before allocation."""
+
+
+class ConvergenceError(RuntimeError):
+    """The controller asked for a step below the floor (``controller.py:34-43``)."""
+
+
+class LifecycleError(BackendError):
+    """A call out of order (``_core.pyi`` SessionLifecycleError)."""
+
+
+def _raise(status: int, msg: bytes) -> None:
+    text = msg.value.decode(errors="replace") if hasattr(msg, "value") else str(msg)
+    if status == _lib.SP_VALUE:
+        raise BackendRefusal(text)
+    if status == _lib.SP_CONVERGENCE:
+        raise ConvergenceError(text)
+    if status == _lib.SP_STATE:
+        raise LifecycleError(text)
+    raise BackendError(text)
+
+
+def _check(status: int, msg) -> None:
+    if status != _lib.SP_OK:
+        _raise(status, msg)
+
+
+def _ptr(array: np.ndarray, ctype=C.c_double):
+    return array.ctypes.data_as(C.POINTER(ctype))
+
+
+class _Keep:
+    """Holds the arrays a lowered record points into for the record's life."""
+
+    def __init__(self):
+        self.arrays = []
+
+    def f64(self, a) -> C.POINTER(C.c_double):
+        arr = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+        self.arrays.append(arr)
+        return _ptr(arr)
+
+    def c128(self, a) -> C.POINTER(C.c_double):
+        arr = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+        self.arrays.append(arr)
+        return arr.ctypes.data_as(C.POINTER(C.c_double))
+
+    def u8(self, a):
+        arr = np.ascontiguousarray(np.asarray(a, dtype=np.uint8))
+        self.arrays.append(arr)
+        return _ptr(arr, C.c_uint8)
+
+    def i64(self, a):
+        arr = np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+        self.arrays.append(arr)
+        return _ptr(arr, C.c_int64)
+
+    def path(self, p: ScalarPath) -> _lib.ScalarPathC:
+        out = _lib.ScalarPathC()
+        out.kind = p.kind
+        out.value = float(p.value)
+        if p.kind == 2:
+            out.n_intervals = len(p.edges) - 1
+            out.degree = p.coefficients.shape[1] - 1
+            out.edges = self.f64(p.edges)
+            out.coefficients = self.f64(p.coefficients)
+        return out
+
+
+@dataclass
+class Settings:
+    """The stepper and controller settings a run lowers (``PIController`` defaults)."""
+
+    safety: float = 0.8
+    max_factor: float = 1.1
+    min_factor: float = 0.2
+    beta1: float = 3 / 25
+    beta2: float = -1 / 25
+    lag: int = 1
+    reject_free: bool = False
+    estimator_order: int = 4
+    auto_init: bool = True
+    dz_min: float | None = None
+    dz_max: float | None = None
+    stretch: float = 1.1
+    dz_init: float | None = None
+    first_same_as_last: bool = True
+
+
+def lower(problem: Problem, settings: Settings | None = None) -> tuple[_lib.ProblemDescC, _Keep]:
+    """Fill the C record from the problem (``_lowering._lower``, ``_lowering.py:334``)."""
+    problem.validate()
+    s = settings or Settings()
+    k = _Keep()
+    d = _lib.ProblemDescC()
+    d.kind = 0 if problem.kind == "envelope" else 1
+    d.n_t, d.n_to = problem.n_t, problem.n_to
+    d.n_modes, d.n_nodes, d.n_omega = problem.n_modes, problem.n_nodes, problem.n_omega
+    d.crop_index = k.i64(problem.crop_index)
+    d.active_mask = k.u8(problem.active_mask)
+    d.norm_mask = k.u8(problem.norm_mask)
+    d.omega_window = k.f64(problem.omega_window)
+    d.omega = k.f64(problem.omega)
+    d.time_window = k.f64(problem.time_window)
+    d.time_window_derivative = k.f64(problem.time_window_derivative)
+    d.dt_fine = problem.dt_fine
+    d.z_end = problem.z_end
+    if len(problem.phi_paths) > _lib.SP_MAX_TERMS or len(problem.prefactor_paths) > 4:
+        raise BackendRefusal("at most four operator and prefactor terms")
+    d.n_phi = len(problem.phi_paths)
+    d.phi_terms = k.c128(problem.phi_terms)
+    for i, p in enumerate(problem.phi_paths):
+        d.phi_paths[i] = k.path(p)
+    d.n_prefactor = len(problem.prefactor_paths)
+    d.prefactor_terms = k.c128(problem.prefactor_terms)
+    for i, p in enumerate(problem.prefactor_paths):
+        d.prefactor_paths[i] = k.path(p)
+    d.synthesis = k.f64(problem.synthesis)
+    d.projection = k.f64(problem.projection)
+    d.density = k.path(problem.density)
+    d.preionisation = k.path(problem.preionisation)
+    if problem.kerr is not None:
+        d.has_kerr = 1
+        d.kerr_real = 1 if problem.kerr.cubic_form == "real" else 0
+        d.kerr_coefficient = problem.kerr.coefficient
+        d.kerr_self = problem.kerr.self_factor
+        d.kerr_conjugate = problem.kerr.conjugate_factor
+    if problem.raman is not None:
+        r = problem.raman
+        if len(r.rates) > _lib.SP_MAX_TERMS:
+            raise BackendRefusal("at most four Raman rows")
+        d.n_raman = r.undamped.shape[0]
+        d.raman_undamped = k.f64(r.undamped)
+        d.raman_fraction = r.fraction
+        d.raman_epsilon0 = r.epsilon0
+        d.raman_thg = int(r.thg)
+        for i, p in enumerate(r.rates):
+            d.raman_rates[i] = k.path(p)
+    if problem.plasma is not None:
+        g = problem.plasma
+        d.has_plasma = 1
+        d.charge_mass_ratio = g.charge_mass_ratio
+        d.ionisation_potential = g.ionisation_potential
+        d.log_field_min, d.d_log_field = g.log_field_min, g.d_log_field
+        d.field_min, d.field_max, d.rate_max = g.field_min, g.field_max, g.rate_max
+        d.rate_intervals = g.rate_coefficients.shape[1]
+        d.rate_coefficients = k.f64(g.rate_coefficients)
+        d.n_taps = len(g.taps)
+        d.taps = k.f64(g.taps)
+    tab = problem.tableau
+    d.n_stages = len(tab.nodes)
+    d.tableau_a = k.f64(tab.matrix)
+    d.tableau_c = k.f64(tab.nodes)
+    d.tableau_b = k.f64(tab.solution_weights)
+    d.tableau_e = k.f64(tab.error_weights)
+    d.first_same_as_last = int(tab.first_same_as_last and s.first_same_as_last)
+    d.noise_model = {"none": 0, "input": 1, "modified": 2}[problem.noise_model]
+    if problem.noise_model == "modified" and problem.noise0 is not None:
+        d.noise = k.c128(problem.noise0)
+    d.rtol, d.atol = problem.rtol, problem.atol
+    d.safety, d.max_factor, d.min_factor = s.safety, s.max_factor, s.min_factor
+    d.beta1, d.beta2 = s.beta1, s.beta2
+    d.lag, d.reject_free, d.estimator_order = s.lag, int(s.reject_free), s.estimator_order
+    d.auto_init = int(s.auto_init)
+    d.dz_min = 1e-9 * problem.z_end if s.dz_min is None else s.dz_min
+    d.dz_max = problem.z_end if s.dz_max is None else s.dz_max
+    d.stretch = s.stretch
+    d.dz_init = 1e-4 * problem.z_end if s.dz_init is None else s.dz_init
+    return d, k
+
+
+class Workspace:
+    """``_core.Workspace``: everything allocated and planned at construction."""
+
+    def __init__(self, problem: Problem, settings: Settings | None = None):
+        self._lib = _lib.lib()
+        self.problem = problem
+        desc, keep = lower(problem, settings)
+        self._keep = keep
+        handle = C.c_void_p()
+        msg = C.create_string_buffer(1024)
+        _check(self._lib.sp_workspace_create(C.byref(desc), C.byref(handle), msg, 1024), msg)
+        self._handle = handle
+        self._shape = (problem.n_modes, problem.n_omega)
+
+    @property
+    def closed(self) -> bool:
+        return self._handle is None
+
+    def close(self) -> None:
+        if self._handle is not None:
+            self._lib.sp_workspace_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        if self._handle is None:
+            raise LifecycleError("the workspace is closed")
+        msg = C.create_string_buffer(1024)
+        _check(getattr(self._lib, name)(self._handle, *args, msg, 1024), msg)
+
+    def load(self, z: float, field: np.ndarray) -> None:
+        arr = np.ascontiguousarray(np.asarray(field, dtype=np.complex128).reshape(self._shape))
+        self._call("sp_workspace_load", float(z), arr.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def probe(self, z: float, field: np.ndarray) -> "Probe":
+        self.load(z, field)
+        return Probe(self, z)
+
+    def start(self, z: float, field: np.ndarray) -> "Session":
+        self.load(z, field)
+        return Session(self, z)
+
+    def trial(self, z: float, dz: float) -> float:
+        norm = C.c_double()
+        self._call("sp_workspace_trial", float(z), float(dz), C.byref(norm))
+        return norm.value
+
+    def trial_field(self) -> np.ndarray:
+        out = np.empty(self._shape, dtype=np.complex128)
+        self._call("sp_workspace_trial_field", out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def state(self) -> np.ndarray:
+        out = np.empty(self._shape, dtype=np.complex128)
+        self._call("sp_workspace_state", out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def export(self, weight: np.ndarray | None = None, optics: bool = True) -> np.ndarray:
+        out = np.empty(self._shape, dtype=np.complex128)
+        w = None
+        if weight is not None:
+            w = np.ascontiguousarray(np.broadcast_to(weight, self._shape), dtype=np.float64)
+        self._call(
+            "sp_workspace_export",
+            None if w is None else _ptr(w),
+            int(optics),
+            out.ctypes.data_as(C.POINTER(C.c_double)),
+        )
+        return out
+
+    def counters(self) -> dict:
+        c = _lib.CountersC()
+        self._lib.sp_workspace_counters(self._handle, C.byref(c))
+        return {name: getattr(c, name) for name, _ in _lib.CountersC._fields_}
+
+    def profile(self, enable: bool = True) -> None:
+        self._lib.sp_workspace_profile(self._handle, int(enable))
+
+    def timing(self) -> tuple[float, int]:
+        us = C.c_double()
+        n = C.c_long()
+        self._lib.sp_workspace_timing(self._handle, C.byref(us), C.byref(n))
+        return us.value, n.value
+
+
+class Probe:
+    """One trial from a loaded state, committing nothing (``_probe.py:33-70``)."""
+
+    def __init__(self, workspace: Workspace, z: float):
+        self._ws = workspace
+        self.z = z
+        self.trials = 0
+
+    def trial(self, z: float, dz: float) -> float:
+        self.trials += 1
+        return self._ws.trial(z, dz)
+
+    def field(self) -> np.ndarray:
+        return self._ws.trial_field()
+
+
+@dataclass
+class Batch:
+    """What one ``advance`` returns: the attempts rows and where the march stands."""
+
+    z: float
+    done: bool
+    attempts: np.ndarray  # (n, 4): end, step, norm, accepted
+    evaluations: int
+
+
+class Session:
+    """A native march over the workspace (``_core.pyi`` Session)."""
+
+    def __init__(self, workspace: Workspace, z: float):
+        self._ws = workspace
+        self.z = z
+        self.finished = False
+
+    def advance(self, target: float, work_limit: int = 256) -> Batch:
+        rows = np.zeros((work_limit, 4))
+        res = _lib.MarchResultC()
+        msg = C.create_string_buffer(1024)
+        status = self._ws._lib.sp_workspace_advance(
+            self._ws._handle, float(target), int(work_limit), _ptr(rows), int(work_limit),
+            C.byref(res), msg, 1024,
+        )
+        _check(status, msg)
+        self.z = res.z
+        return Batch(res.z, bool(res.done), rows[: res.n_rows].copy(), res.evaluations)
+
+    def run(self, target: float, work_limit: int = 256) -> np.ndarray:
+        """Advance until the target; return every attempt row."""
+        out = []
+        while True:
+            batch = self.advance(target, work_limit)
+            out.append(batch.attempts)
+            if batch.done:
+                break
+        self.finished = True
+        return np.concatenate(out) if out else np.zeros((0, 4))
+
+    def state(self) -> np.ndarray:
+        return self._ws.state()
+
+    def export(self, weight=None, optics=True) -> np.ndarray:
+        return self._ws.export(weight, optics)
+
+
+def testing_fft(x: np.ndarray, sign: int) -> np.ndarray:
+    """The library's batched complex FFT over the last axis (unnormalised)."""
+    arr = np.ascontiguousarray(np.asarray(x, dtype=np.complex128))
+    n = arr.shape[-1]
+    batch = arr.size // n
+    out = np.empty_like(arr)
+    msg = C.create_string_buffer(1024)
+    status = _lib.lib().sp_testing_fft(
+        arr.ctypes.data_as(C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)),
+        n, batch, int(sign), msg, 1024,
+    )
+    _check(status, msg)
+    return out
+
+
+# ---------------------------------------------------------------------- traces
+
+
+def rk_trace(scenario: dict) -> dict:
+    """Run a scripted scenario through the library's march; return its trace dict."""
+    import json
+
+    lib = _lib.lib()
+    keep = _Keep()
+    s = _lib.RKScenarioC()
+    c = scenario["controller"]
+    s.rtol = c.get("rtol", 1e-6)
+    s.atol = c.get("atol", 0.0)
+    s.safety = c.get("safety", 0.8)
+    s.max_factor = c.get("max_factor", 1.1)
+    s.min_factor = c.get("min_factor", 0.2)
+    s.beta1 = c.get("beta1", 3 / 25)
+    s.beta2 = c.get("beta2", -1 / 25)
+    s.lag = c.get("lag", 1)
+    s.reject_free = int(c.get("reject_free", False))
+    s.estimator_order = c.get("estimator_order", 4)
+    for name in ("z0", "proposal", "dz_min", "dz_max", "z_end", "stretch"):
+        setattr(s, name, float(scenario[name]))
+    s.auto_init = int(scenario["auto_init"])
+    s.interval_cap = int(scenario["interval_cap"])
+    s.step_planes = int(scenario["step_planes"])
+    s.has_h0 = int(scenario["h0"] is not None)
+    s.h0 = 0.0 if scenario["h0"] is None else float(scenario["h0"])
+    s.norms = keep.f64(scenario["norms"])
+    s.n_norms = len(scenario["norms"])
+    s.save_planes = keep.f64(scenario["save_planes"] or [0.0])
+    s.n_planes = len(scenario["save_planes"])
+    s.breakpoints = keep.f64(scenario["breakpoints"] or [0.0])
+    s.n_breakpoints = len(scenario["breakpoints"])
+    s.kinks = keep.f64(scenario["kinks"] or [0.0])
+    s.n_kinks = len(scenario["kinks"])
+    targets = scenario["targets"]
+    s.targets = keep.f64(targets or [0.0])
+    s.n_targets = 0 if targets is None else len(targets)
+    schedule = scenario["schedule"]
+    s.schedule = keep.f64(schedule or [0.0])
+    s.n_schedule = 0 if schedule is None else len(schedule)
+    quiet = {int(a): bool(b) for a, b in scenario["quiet"].items()}
+    qk = np.ascontiguousarray(np.array(list(quiet) or [0], dtype=np.int64))
+    qv = np.ascontiguousarray(np.array([int(v) for v in quiet.values()] or [0], dtype=np.int32))
+    keep.arrays += [qk, qv]
+    s.quiet_keys = qk.ctypes.data_as(C.POINTER(C.c_long))
+    s.quiet_values = qv.ctypes.data_as(C.POINTER(C.c_int))
+    s.n_quiet = len(quiet)
+    s.has_quiet_interval = int(scenario["quiet_interval"] is not None)
+    s.quiet_interval = float(scenario["quiet_interval"] or 0.0)
+    s.has_ceiling = int(bool(scenario["ceiling"]))
+    s.ceiling = float(scenario["ceiling"] or 0.0)
+    split = {int(a): float(b) for a, b in scenario["split"].items()}
+    sk = np.ascontiguousarray(np.array(list(split) or [0], dtype=np.int64))
+    keep.arrays.append(sk)
+    s.split_keys = sk.ctypes.data_as(C.POINTER(C.c_long))
+    s.split_values = keep.f64(list(split.values()) or [0.0])
+    s.n_split = len(split)
+    size = 1 << 22
+    buf = C.create_string_buffer(size)
+    status = lib.sp_rk_trace(C.byref(s), buf, size)
+    if status != _lib.SP_OK:
+        raise BackendError(buf.value.decode(errors="replace"))
+    return json.loads(buf.value.decode())
+
+
+__all__ = [
+    "BackendError", "BackendRefusal", "Batch", "ConvergenceError", "LifecycleError",
+    "Probe", "Session", "Settings", "Workspace", "lower", "rk_trace",
+]
+_ = math

@@ -1,0 +1,427 @@
+"""The problem record that crosses into the device library, and its builders.
+
+The record mirrors the subset of solprop-native's ``_core.Problem``
+(``native/src/solprop_native/_core.pyi:51-466``) that the envelope-GNLSE and the modal
+field-grid routes read: dims, grid, stepper tableau, linear terms, transform matrices,
+Kerr, Raman and plasma groups. Every array is C-contiguous with a fixed dtype, as the
+reference's lowering makes them (``native/src/solprop_native/_lowering.py:334-460``).
+
+The z-dependence of a stage is carried, as in the reference, by host scalars against
+arrays uploaded once (``docs/dev/backends.md:144-163``): ``ΔΦ_i = Σ_t w_{i,t} Φ_t`` with
+``w_{i,t} = W_t(z_i) - W_t(z_anchor)`` from cumulative scalar paths, and the prefactor,
+density, pre-ionisation and Raman rates read from scalar paths at the stage position.
+A scalar path is a constant, the position itself, or a piecewise Chebyshev series
+evaluated by Clenshaw's recurrence (``src/solprop/translate/stage.py:170``), so the
+library evaluates exactly the same floats as the host.
+
+The builders at the bottom make the synthetic inputs of the benchmarked workloads
+(solprop's translation layer, which would make them from materials and mode solvers, is
+outside this package): an envelope GNLSE (sech soliton, Taylor dispersion, damped
+oscillator Raman, shot noise), and a gas-filled capillary on a real field grid with
+modal synthesis/projection over a radial quadrature, Kerr and tunnelling ionisation.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+C_LIGHT = 299792458.0
+EPS0 = 8.8541878188e-12
+HBAR = 1.054571817e-34
+E_CHARGE = 1.602176634e-19
+M_E = 9.1093837139e-31
+KB = 1.380649e-23
+
+
+# ------------------------------------------------------------------- scalar paths
+
+
+@dataclass(frozen=True)
+class ScalarPath:
+    """A scalar function of z: constant, the position, or piecewise Chebyshev.
+
+    ``kind`` is 0 (``value`` everywhere), 1 (``z`` itself, exact) or 2 (piecewise
+    Chebyshev: ``edges`` of shape (n+1,), ``coefficients`` of shape (n, degree+1),
+    interval ``j`` mapping ``[edges[j], edges[j+1]]`` onto ``[-1, 1]``).
+    """
+
+    kind: int = 0
+    value: float = 0.0
+    edges: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    coefficients: np.ndarray = field(default_factory=lambda: np.zeros((0, 0)))
+
+    @staticmethod
+    def constant(value: float) -> "ScalarPath":
+        return ScalarPath(0, float(value))
+
+    @staticmethod
+    def position() -> "ScalarPath":
+        return ScalarPath(1, 0.0)
+
+    @staticmethod
+    def chebyshev(function, z0: float, z1: float, intervals: int = 8, degree: int = 24):
+        """Interpolate ``function`` on ``intervals`` equal pieces of ``[z0, z1]``."""
+        edges = np.linspace(z0, z1, intervals + 1)
+        rows = []
+        for a, b in zip(edges[:-1], edges[1:], strict=True):
+            rows.append(
+                np.polynomial.chebyshev.chebinterpolate(
+                    lambda x, a=a, b=b: function(0.5 * (b - a) * x + 0.5 * (b + a)),
+                    degree,
+                )
+            )
+        return ScalarPath(2, 0.0, np.ascontiguousarray(edges), np.ascontiguousarray(rows))
+
+    def __call__(self, z: float) -> float:
+        """Evaluate exactly as the library does (same operations, same order)."""
+        if self.kind == 0:
+            return self.value
+        if self.kind == 1:
+            return float(z)
+        edges = self.edges
+        n = len(edges) - 1
+        j = 0
+        while j < n - 1 and z >= edges[j + 1]:
+            j += 1
+        a, b = float(edges[j]), float(edges[j + 1])
+        x = (2.0 * z - a - b) / (b - a)
+        c = self.coefficients[j]
+        b1 = 0.0
+        b2 = 0.0
+        for k in range(len(c) - 1, 0, -1):
+            t = 2.0 * x * b1 - b2 + float(c[k])
+            b2 = b1
+            b1 = t
+        return x * b1 - b2 + float(c[0])
+
+
+# ------------------------------------------------------------------------ records
+
+
+@dataclass(frozen=True)
+class Tableau:
+    """An embedded explicit Runge-Kutta pair (``src/solprop/stepper/tableau.py:104``)."""
+
+    nodes: np.ndarray
+    matrix: np.ndarray
+    solution_weights: np.ndarray
+    error_weights: np.ndarray
+    first_same_as_last: bool
+
+
+def dopri5() -> Tableau:
+    """Dormand and Prince's RK5(4)7M pair, error weights as solution less embedded.
+
+    The published coefficients (J. R. Dormand and P. J. Prince, J. Comput. Appl. Math.
+    6, 19 (1980)); the reference holds the same table at
+    ``src/solprop/stepper/tableau.py:475-506``.
+    """
+    nodes = np.array([0.0, 1 / 5, 3 / 10, 4 / 5, 8 / 9, 1.0, 1.0])
+    matrix = np.array(
+        [
+            [0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0],
+            [1 / 5, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0],
+            [3 / 40, 9 / 40, 0.0, 0.0, 0.0, 0.0, 0.0],
+            [44 / 45, -56 / 15, 32 / 9, 0.0, 0.0, 0.0, 0.0],
+            [19372 / 6561, -25360 / 2187, 64448 / 6561, -212 / 729, 0.0, 0.0, 0.0],
+            [9017 / 3168, -355 / 33, 46732 / 5247, 49 / 176, -5103 / 18656, 0.0, 0.0],
+            [35 / 384, 0.0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84, 0.0],
+        ]
+    )
+    solution = np.array([35 / 384, 0.0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84, 0.0])
+    embedded = np.array(
+        [5179 / 57600, 0.0, 7571 / 16695, 393 / 640, -92097 / 339200, 187 / 2100, 1 / 40]
+    )
+    return Tableau(nodes, matrix, solution, solution - embedded, True)
+
+
+@dataclass(frozen=True)
+class KerrGroup:
+    """Kerr coefficients (``_core.pyi`` ``Problem.Kerr``)."""
+
+    coefficient: float
+    cubic_form: str  # "real" or "analytic"
+    self_factor: float = 0.75
+    conjugate_factor: float = 0.0
+    harmonic_factor: float = 0.0
+    take_real: bool = False
+    analytic_signal: bool = False
+
+
+@dataclass(frozen=True)
+class RamanGroup:
+    """Scalar Raman: undamped rows on the doubled grid (``Problem.Raman``)."""
+
+    undamped: np.ndarray  # (n_terms, 2 n_to) float64
+    fraction: float
+    epsilon0: float
+    rates: tuple  # one ScalarPath per row: decay rate at z, 1/s
+    thg: bool = False
+
+
+@dataclass(frozen=True)
+class PlasmaGroup:
+    """One ionisation channel and its rate table (``Problem.Plasma``)."""
+
+    charge_mass_ratio: float
+    ionisation_potential: float
+    log_field_min: float
+    d_log_field: float
+    field_min: float
+    field_max: float
+    rate_max: float
+    rate_coefficients: np.ndarray  # (4, n_intervals) highest power first
+    taps: np.ndarray = field(default_factory=lambda: np.ones(1))
+
+
+@dataclass
+class Problem:
+    """Everything the library needs to run one propagation (the seam's arrays).
+
+    Shapes use ``C = n_modes`` (one polarisation component), ``N = n_omega`` and
+    ``T = n_to``.
+    """
+
+    kind: str  # "envelope" (complex time samples) or "field" (real)
+    n_t: int
+    n_to: int
+    n_modes: int
+    n_nodes: int
+    crop_index: np.ndarray  # (N,) int64
+    active_mask: np.ndarray  # (N,) bool
+    norm_mask: np.ndarray  # (N,) bool
+    omega_window: np.ndarray  # (N,) float64
+    omega: np.ndarray  # (N,) float64, absolute angular frequency of each bin
+    time_window: np.ndarray  # (T,) float64, the temporal collar g
+    time_window_derivative: np.ndarray  # (T,) float64, g', 1/s
+    dt_fine: float
+    z_end: float
+    phi_terms: np.ndarray  # (n_phi, C, N) complex128
+    phi_paths: tuple  # n_phi cumulative ScalarPaths W_t(z)
+    prefactor_terms: np.ndarray  # (n_pref, C, N) complex128
+    prefactor_paths: tuple  # n_pref ScalarPaths p_t(z)
+    synthesis: np.ndarray  # (Q, M) float64, carries the pad scale
+    projection: np.ndarray  # (M, Q) float64, weights folded in
+    density: ScalarPath
+    preionisation: ScalarPath
+    kerr: KerrGroup | None = None
+    raman: RamanGroup | None = None
+    plasma: PlasmaGroup | None = None
+    tableau: Tableau = field(default_factory=dopri5)
+    field0: np.ndarray | None = None  # (C, N) complex128 launch spectrum
+    noise0: np.ndarray | None = None  # (C, N) complex128, shot-noise spectrum
+    noise_model: str = "none"  # "none", "input" or "modified"
+    rtol: float = 1e-6
+    atol: float = 0.0
+    description: dict = field(default_factory=dict)
+
+    @property
+    def n_omega(self) -> int:
+        return int(self.crop_index.size)
+
+    @property
+    def shape(self) -> tuple[int, int, int]:
+        return (self.n_modes, 1, self.n_omega)
+
+    @property
+    def n_fine_bins(self) -> int:
+        return self.n_to // 2 + 1 if self.kind == "field" else self.n_to
+
+    # -- host scalars of a stage (the reference's StageScalars, translate/stage.py:170)
+
+    def phi_weights(self, z0: float, z1: float) -> np.ndarray:
+        return np.array([p(z1) - p(z0) for p in self.phi_paths])
+
+    def phi_at(self, z: float) -> np.ndarray:
+        return np.array([p(z) for p in self.phi_paths])
+
+    def prefactor_weights(self, z: float) -> np.ndarray:
+        return np.array([p(z) for p in self.prefactor_paths])
+
+    def raman_rates(self, z: float) -> np.ndarray:
+        if self.raman is None:
+            return np.zeros(0)
+        return np.array([p(z) for p in self.raman.rates])
+
+    def validate(self) -> None:
+        """Refuse shapes and dtypes the library cannot run (before any allocation)."""
+        n, t, c = self.n_omega, self.n_to, self.n_modes
+        if self.kind not in ("field", "envelope"):
+            raise ValueError(f"kind must be 'field' or 'envelope', got {self.kind!r}")
+        if t & (t - 1) or t < 16:
+            raise ValueError(f"n_to must be a power of two of at least 16, got {t}")
+        if self.n_t > t or t % self.n_t:
+            raise ValueError("n_to must be an integer multiple of n_t")
+        expected = self.n_t // 2 + 1 if self.kind == "field" else self.n_t
+        if n != expected:
+            raise ValueError(f"crop_index has {n} bins, the grid needs {expected}")
+        for name, arr, shape, dtype in (
+            ("active_mask", self.active_mask, (n,), np.bool_),
+            ("norm_mask", self.norm_mask, (n,), np.bool_),
+            ("omega_window", self.omega_window, (n,), np.float64),
+            ("omega", self.omega, (n,), np.float64),
+            ("time_window", self.time_window, (t,), np.float64),
+            ("time_window_derivative", self.time_window_derivative, (t,), np.float64),
+            ("phi_terms", self.phi_terms, (len(self.phi_paths), c, n), np.complex128),
+            (
+                "prefactor_terms",
+                self.prefactor_terms,
+                (len(self.prefactor_paths), c, n),
+                np.complex128,
+            ),
+            ("synthesis", self.synthesis, (self.n_nodes, c), np.float64),
+            ("projection", self.projection, (c, self.n_nodes), np.float64),
+        ):
+            if arr.shape != shape or arr.dtype != dtype:
+                raise ValueError(f"{name}: expected {shape} {np.dtype(dtype)}, got "
+                                 f"{arr.shape} {arr.dtype}")
+        if self.raman is not None and self.raman.undamped.shape[1] != 2 * t:
+            raise ValueError("Raman rows must span the doubled grid, 2 n_to samples")
+        if self.plasma is not None and self.kind != "field":
+            raise ValueError("the field-grid plasma needs a real field grid")
+
+
+# ----------------------------------------------------------------------- builders
+
+
+def _collar(n: int, width: int) -> np.ndarray:
+    """A smooth 0..1..0 collar: raised-cosine edges of ``width`` samples."""
+    w = np.ones(n)
+    if width > 0:
+        ramp = 0.5 - 0.5 * np.cos(np.pi * np.arange(width) / width)
+        w[:width] = ramp
+        w[n - width :] = ramp[::-1]
+    return w
+
+
+def gnlse_problem(
+    *,
+    n_t: int = 8192,
+    ffac: int = 2,
+    window: float = 10e-12,
+    wavelength: float = 1550e-9,
+    fwhm: float = 50e-15,
+    soliton_order: float = 4.0,
+    betas=(-20e-27, 0.1e-39, -0.0001e-51, 0.0, 0.0),
+    gamma: float = 1.1e-3,
+    raman_fraction: float = 0.18,
+    tau1: float = 12.2e-15,
+    tau2: float = 32e-15,
+    shock: bool = True,
+    length: float = 1.0,
+    rtol: float = 1e-6,
+    noise_seed: int | None = 1,
+    noise_model: str = "input",
+    time_collar: float = 0.05,
+) -> Problem:
+    """Single-mode envelope GNLSE: sech pulse at soliton order N, Kerr + Raman.
+
+    In the internal (unnormalised numpy) convention the envelope bin ``k`` sits at
+    ``ω_0 + Δω_k``; ``Φ = -i Σ_{n≥2} β_n Δω^n / n! · z``; the prefactor is
+    ``-i γ ω / ω_0`` (``ω/ω_0`` frozen without the shock) times the crop scale; Kerr is
+    ``(1 - f_R)|A|^2 A`` and Raman ``f_R A (h * |A|^2)`` with ``∫h = 1`` (the row holds
+    ``2 h`` against the ``|A|^2/2`` drive; ``ε_0 = 1`` in these units).
+    ``betas`` are β_2..β_6 in s^n/m (β_2 = -20 ps^2/km by default).
+    """
+    n_to = ffac * n_t
+    dt = window / n_t
+    dt_fine = window / n_to
+    omega0 = 2 * np.pi * C_LIGHT / wavelength
+    dw = 2 * np.pi * np.fft.fftfreq(n_t, dt)
+    omega = omega0 + dw
+    beta_poly = np.zeros_like(dw)
+    for order, b in enumerate(betas, start=2):
+        beta_poly += b * dw**order / math.factorial(order)
+    phi = (-1j * beta_poly).reshape(1, 1, n_t)
+    ratio = omega / omega0 if shock else np.ones_like(omega)
+    crop_scale = n_t / n_to
+    prefactor = (-1j * gamma * ratio * crop_scale).reshape(1, 1, n_t)
+    # Active band: everything but a thin spectral margin; the collar rolls off there.
+    active = np.ones(n_t, dtype=bool)
+    omega_window = np.ones(n_t)
+    edge = int(0.02 * n_t)
+    order_idx = np.argsort(dw)
+    collar = _collar(n_t, edge)
+    omega_window[order_idx] = collar
+    t_window = _collar(n_to, int(time_collar * n_to))
+    t_deriv = np.gradient(t_window, dt_fine)
+    # Launch field: sech at soliton order N, P0 = N^2 |β2| / (γ T0^2).
+    t0 = fwhm / (2 * np.arccosh(np.sqrt(2.0)))
+    p0 = soliton_order**2 * abs(betas[0]) / (gamma * t0**2)
+    t = (np.arange(n_t) - n_t // 2) * dt
+    a_t = np.sqrt(p0) / np.cosh(t / t0)
+    spectrum = np.fft.fft(a_t).reshape(1, n_t).astype(np.complex128)
+    noise = None
+    if noise_seed is not None and noise_model != "none":
+        rng = np.random.default_rng(noise_seed)
+        # One photon per bin: |A_k| = sqrt(hbar ω N / dt) (src/solprop/noise.py:195-246).
+        phase = rng.uniform(0.0, 2.0 * np.pi, size=(1, 1, n_t))
+        amplitude = np.sqrt(HBAR * omega * n_t / dt)
+        noise = (amplitude * np.exp(1j * phase)).reshape(1, n_t)
+        if noise_model == "input":
+            spectrum = spectrum + noise
+    raman = None
+    kerr_coefficient = 1.0
+    if raman_fraction > 0.0:
+        kerr_coefficient = 1.0 - raman_fraction
+        delays = np.arange(2 * n_to) * dt_fine
+        frequency = 1.0 / tau1
+        amplitude = (frequency**2 + 1.0 / tau2**2) / frequency
+        row = np.where(delays > 0.0, amplitude * np.sin(frequency * delays), 0.0)
+        row[n_to:] = 0.0  # the padding half of the doubled grid stays zero
+        raman = RamanGroup(
+            undamped=np.ascontiguousarray(2.0 * row.reshape(1, -1)),
+            fraction=raman_fraction,
+            epsilon0=1.0,
+            rates=(ScalarPath.constant(1.0 / tau2),),
+        )
+    kerr = KerrGroup(kerr_coefficient, "analytic", self_factor=1.0)
+    return Problem(
+        kind="envelope",
+        n_t=n_t,
+        n_to=n_to,
+        n_modes=1,
+        n_nodes=1,
+        crop_index=np.ascontiguousarray(
+            np.concatenate((np.arange(n_t // 2), np.arange(n_to - n_t // 2, n_to)))
+        ).astype(np.int64),
+        active_mask=active,
+        norm_mask=active.copy(),
+        omega_window=omega_window,
+        omega=omega,
+        time_window=t_window,
+        time_window_derivative=t_deriv,
+        dt_fine=dt_fine,
+        z_end=length,
+        phi_terms=np.ascontiguousarray(phi),
+        phi_paths=(ScalarPath.position(),),
+        prefactor_terms=np.ascontiguousarray(prefactor),
+        prefactor_paths=(ScalarPath.constant(1.0),),
+        synthesis=np.array([[n_to / n_t]]),
+        projection=np.array([[1.0]]),
+        density=ScalarPath.constant(1.0),
+        preionisation=ScalarPath.constant(0.0),
+        kerr=kerr,
+        raman=raman,
+        field0=np.ascontiguousarray(spectrum),
+        noise0=None if noise is None else np.ascontiguousarray(noise),
+        noise_model=noise_model if noise is not None else "none",
+        rtol=rtol,
+        description={
+            "route": "envelope GNLSE",
+            "n_t": n_t,
+            "n_to": n_to,
+            "window_s": window,
+            "wavelength_m": wavelength,
+            "fwhm_s": fwhm,
+            "soliton_order": soliton_order,
+            "gamma_per_W_m": gamma,
+            "raman_fraction": raman_fraction,
+            "length_m": length,
+            "rtol": rtol,
+            "noise_seed": noise_seed,
+        },
+    )

@@ -1,0 +1,50 @@
+"""Write the golden fixtures from the data files the reference's own tests hold.
+
+Run in a checkout that has the reference at ``/root/reference`` (read-only). Nothing
+of the reference is imported or executed: the files below are JSON data, read as
+data.
+
+- ``rk_traces.json.gz``: the Runge-Kutta decision corpus
+  (``tests/data/rk_traces.json.gz``, 29 scripted scenarios), copied unchanged.
+- ``adams_traces.json.gz``: the Adams decision corpus (``tests/data/adams_traces.json.gz``),
+  copied unchanged.
+- ``luna_plasma_witness.npz``: the input signals, parameters and the Luna plasma stages
+  of ``tests/data/luna_plasma.json`` that the plasma column is compared with (the
+  SCALAR, ZEROS, PPT and VECTOR cases and their free-current halves).
+"""
+
+from __future__ import annotations
+
+import json
+import shutil
+from pathlib import Path
+
+import numpy as np
+
+REFERENCE = Path("/root/reference/tests/data")
+HERE = Path(__file__).resolve().parent
+
+
+def main() -> None:
+    for name in ("rk_traces.json.gz", "adams_traces.json.gz"):
+        shutil.copyfile(REFERENCE / name, HERE / name)
+    data = json.loads((REFERENCE / "luna_plasma.json").read_text())
+    luna, signals, parameters = data["luna"], data["signals"], data["parameters"]
+    keep = {}
+    for key in ("t", "field", "zeroed", "field_x", "field_y"):
+        keep["signal_" + key] = np.array(signals[key])
+    for case in ("SCALAR", "ZEROS", "PPT", "VECTOR"):
+        for suffix in ("", "_FREE"):
+            for stage in ("RATE", "FRACTION", "J", "P", "J_X", "J_Y", "P_X", "P_Y"):
+                key = f"{case}{suffix}_{stage}"
+                if key in luna:
+                    keep["luna_" + key] = np.array(luna[key])
+    keep["luna_CONSTANTS"] = np.array(luna["CONSTANTS"])
+    for key, value in parameters.items():
+        if isinstance(value, int | float):
+            keep["param_" + key] = np.array(value)
+    np.savez_compressed(HERE / "luna_plasma_witness.npz", **keep)
+
+
+if __name__ == "__main__":
+    main()

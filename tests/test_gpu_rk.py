@@ -1,0 +1,149 @@
+"""GPU parity of the envelope Runge-Kutta route against the oracle, through the C ABI.
+
+Tolerances (class R, fixed before the run): an FP64 FFT of n points carries a
+relative L2 error of a few eps log2 n, so a stage chain of three transforms and a
+few products is held to 1e-12 relative L2 at the trial level; the scaled norm to
+1e-10 relative (it is a ratio of masked sums of such fields); a short adaptive march
+must make the same accept/reject decisions and end within 1e-9 relative L2.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import rk
+from solprop_amd.native import Settings, Workspace, testing_fft
+from solprop_amd.problem import gnlse_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("n", [16, 64, 512, 4096, 8192, 16384, 32768])
+@pytest.mark.parametrize("batch", [1, 3])
+def test_fft_matches_numpy(n, batch):
+    print(f"fft n={n} batch={batch}", flush=True)
+    rng = np.random.default_rng(n + batch)
+    x = rng.normal(size=(batch, n)) + 1j * rng.normal(size=(batch, n))
+    fwd = testing_fft(x, -1)
+    assert rel_l2(fwd, np.fft.fft(x, axis=-1)) < 4e-16 * np.log2(n) * 4
+    inv = testing_fft(x, +1)
+    assert rel_l2(inv, np.fft.ifft(x, axis=-1) * n) < 4e-16 * np.log2(n) * 4
+
+
+def small(**kw):
+    args = dict(n_t=512, ffac=2, window=2e-12, fwhm=50e-15, soliton_order=2.0,
+                raman_fraction=0.18, shock=True, noise_seed=7, noise_model="input",
+                length=0.02)
+    args.update(kw)
+    return gnlse_problem(**args)
+
+
+def oracle_trial(problem, dz, z=0.0):
+    y0 = problem.field0.reshape(problem.shape)
+    return rk.rk_trial(problem, rk.StageSource(problem), y0, z, dz)
+
+
+@pytest.mark.parametrize("case", ["kerr_raman", "kerr_only", "modified_noise", "linear"])
+def test_probe_trial_matches_oracle(case):
+    print(f"probe trial {case}", flush=True)
+    if case == "kerr_only":
+        problem = small(raman_fraction=0.0)
+    elif case == "modified_noise":
+        problem = small(noise_model="modified")
+    elif case == "linear":
+        problem = small()
+        problem.kerr = None
+        problem.raman = None
+    else:
+        problem = small()
+    dz = 2e-4
+    advanced, norm, _ = oracle_trial(problem, dz)
+    ws = Workspace(problem)
+    probe = ws.probe(0.0, problem.field0)
+    got = probe.trial(0.0, dz)
+    field = probe.field()
+    ws.close()
+    assert rel_l2(field, advanced.reshape(field.shape)) < 1e-12
+    if case == "linear":
+        assert got == 0.0 and norm == 0.0
+    else:
+        assert abs(got - norm) / norm < 1e-10
+
+
+def test_probe_trial_matches_oracle_at_full_size():
+    print("probe trial n_t=8192 n_to=16384", flush=True)
+    problem = gnlse_problem()  # N = 8192 bins, T = 16384 samples
+    dz = 1e-5
+    advanced, norm, _ = oracle_trial(problem, dz)
+    ws = Workspace(problem)
+    probe = ws.probe(0.0, problem.field0)
+    got = probe.trial(0.0, dz)
+    field = probe.field()
+    ws.close()
+    assert rel_l2(field, advanced.reshape(field.shape)) < 1e-12
+    assert abs(got - norm) / norm < 1e-10
+
+
+def test_march_matches_oracle_decisions():
+    print("march n_t=512 to z=5e-3", flush=True)
+    problem = small(length=5e-3)
+    y0 = problem.field0.reshape(problem.shape)
+    expected, records = rk.march(problem, y0)
+    ws = Workspace(problem)
+    session = ws.start(0.0, problem.field0)
+    rows = session.run(problem.z_end, work_limit=64)
+    state = session.state()
+    ws.close()
+    assert len(rows) == len(records)
+    assert [bool(r[3]) for r in rows] == [bool(r[3]) for r in records]
+    steps = np.array([r[1] for r in records])
+    assert np.allclose(rows[:, 1], steps, rtol=1e-9, atol=0)
+    assert rel_l2(state, expected.reshape(state.shape)) < 1e-9
+
+
+def test_repeats_agree_to_the_bit():
+    print("determinism", flush=True)
+    problem = small(length=2e-3)
+    outs = []
+    for _ in range(2):
+        ws = Workspace(problem)
+        session = ws.start(0.0, problem.field0)
+        rows = session.run(problem.z_end)
+        outs.append((rows.copy(), session.state()))
+        ws.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_export_rule_is_bitwise():
+    print("export", flush=True)
+    problem = small()
+    ws = Workspace(problem)
+    ws.load(0.0, problem.field0)
+    weight = np.linspace(0.5, 2.0, problem.n_omega)
+    got = ws.export(weight, optics=True)
+    state = ws.state()
+    ws.close()
+    k = np.arange(problem.n_omega)
+    expected = np.conj(state * np.sqrt(weight)) * np.where(k % 2, -1.0, 1.0)
+    assert np.array_equal(got.view(np.float64), expected.view(np.float64))
+
+
+def test_graphs_and_direct_launches_agree():
+    import os
+
+    print("graph vs direct", flush=True)
+    problem = small(length=1e-3)
+    results = []
+    for flag in ("1", "0"):
+        os.environ["SOLPROP_MI355X_GRAPH"] = flag
+        ws = Workspace(problem, Settings())
+        session = ws.start(0.0, problem.field0)
+        session.run(problem.z_end)
+        results.append(session.state())
+        ws.close()
+    os.environ.pop("SOLPROP_MI355X_GRAPH", None)
+    assert np.array_equal(results[0], results[1])

@@ -1,0 +1,223 @@
+"""Pins of the oracle: known answers, closed forms and the Luna plasma witness.
+
+These check the CPU restatement itself, so that the GPU parity tests compare against a
+trusted oracle:
+
+- every pass's docstring example in the reference (``passes/transforms.py``,
+  ``passes/stepping.py``, ``passes/responses.py``, ``passes/plasma.py``,
+  ``src/solprop/fourier.py``), which a dropped term or a transposed operand fails;
+- the DFT definition on small sizes and Parseval;
+- the linear-only problem, whose trial must be exactly ``exp(ΔΦ) y0``;
+- the fundamental soliton, stationary under the full envelope stage (a wrong sign
+  between the dispersion and the Kerr prefactor makes it spread or collapse);
+- the damped oscillator's integral (``src/solprop/nonlinear/raman.py:243-310``);
+- the plasma column against Luna's stages (``tests/data/luna_plasma.json`` of the
+  reference), with the analytic eighth-power rate as a rate table, at the witness's own
+  tolerances.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import passes as P
+from oracle import rk
+from solprop_amd.problem import E_CHARGE, M_E, gnlse_problem
+from tests.conftest import GOLDEN
+
+# ------------------------------------------------------------- docstring answers
+
+
+def test_transform_docstring_answers():
+    spectrum = np.array([[[1.0 + 0.0j, 0.0, 0.0]]])
+    out = P.to_time(spectrum, np.arange(3), 8, "field")
+    assert np.array_equal(out, np.full((1, 1, 8), 0.125))
+    samples = np.ones((1, 1, 8))
+    assert np.array_equal(P.to_frequency(samples, np.arange(3), "field").real,
+                          [[[8.0, 0.0, 0.0]]])
+    assert P.synthesise(np.ones((1, 1, 4)), np.array([[2.0], [3.0]]), 1.0, 1).shape == (2, 1, 4)
+    assert P.project(np.ones((2, 1, 4)), np.array([[1.0, 3.0]]), 1.0, 1).shape == (1, 1, 4)
+    assert P.crop_index(4, 8, "field").tolist() == [0, 1, 2]
+    assert P.crop_index(4, 8, "envelope").tolist() == [0, 1, 6, 7]
+    assert (P.n_omega_for(8, "field"), P.n_omega_for(8, "envelope")) == (5, 8)
+
+
+def test_stepping_docstring_answers():
+    state = np.ones((1, 1, 2), dtype=complex)
+    zero = np.zeros_like(state)
+    assert P.stage_input(state, zero, 0.1, 2.0 * np.ones_like(state), None).ravel().tolist() \
+        == [2 + 0j, 2 + 0j]
+    pol = np.ones((1, 1, 2), dtype=complex)
+    out = P.stage_output(pol, np.array([[[-2.0j, 0.0j]]]), np.ones(2), np.ones_like(pol))
+    assert out.ravel().tolist() == [-2j, 0j]
+    assert P.accumulate(np.zeros((1, 1, 2), complex), np.ones((1, 1, 2)), 0.5).ravel().tolist() \
+        == [0.5, 0.5]
+    assert P.accept_step(state, 2.0 * np.ones_like(state), 0.5, state).ravel().tolist() \
+        == [3 + 0j, 3 + 0j]
+    assert round(P.error_norm(1e-2 * state, state, state, np.ones(2, bool), 1e-2, 0.0), 12) \
+        == 1.0
+    masked = P.apply_hard_mask(np.ones((1, 1, 3), complex), np.array([True, True, False]))
+    assert masked.ravel().tolist() == [1, 1, 0]
+
+
+def test_response_docstring_answers():
+    kerr = P.Kerr(1.0, "analytic", 0.5, 0.25)
+    field = np.array([[[2.0 + 0.0j]]])
+    assert np.allclose(P.kerr_polarisation(field, kerr, 1.0), 0.75 * 4.0 * 2.0)
+    rows = np.zeros((1, 8))
+    rows[0, 1] = 1.0
+    spectrum = P.raman_spectrum(rows, [0.0], 1.0, 2.0)
+    assert spectrum.shape == (5,) and np.isclose(spectrum[0], 2.0)
+    assert P.raman_drive(np.array([3.0, -2.0]), "field", True).tolist() == [9.0, 4.0]
+    assert P.raman_drive(np.array([2.0 + 0j]), "envelope", True).tolist() == [2.0]
+    flat = np.zeros(5, dtype=complex)
+    assert np.all(P.raman_polarisation(np.ones((1, 1, 4)), flat, "field", True, 1.0) == 0.0)
+
+
+def test_plasma_scan_docstring_answer():
+    table = P.RateTable(np.zeros((4, 1)), 0.0, 1.0, 1.0, 2.0, 0.0)
+    plasma = P.Plasma(table, E_CHARGE**2 / M_E, 2.5e-18)
+    fraction, current = P.plasma_scan(np.zeros((1, 1, 4)), np.zeros((1, 4)), plasma, 0.25, 1e-16)
+    assert fraction.tolist() == [[0.25, 0.25, 0.25, 0.25]] and np.all(current == 0.0)
+
+
+# ------------------------------------------------------------------ transforms
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32, 64])
+def test_transforms_are_the_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    k = np.arange(n)
+    dft = np.exp(-2j * np.pi * np.outer(k, k) / n) @ x
+    assert np.allclose(P.forward(x, "envelope"), dft, atol=1e-12)
+    assert np.allclose(P.inverse(dft, "envelope"), x, atol=1e-12)
+    real = rng.normal(size=n)
+    assert np.allclose(P.forward(real, "field"), (np.exp(-2j * np.pi * np.outer(k, k) / n)
+                                                  @ real)[: n // 2 + 1], atol=1e-12)
+
+
+def test_parseval():
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=4096) + 1j * rng.normal(size=4096)
+    X = P.forward(x, "envelope")
+    assert math.isclose(np.sum(np.abs(x) ** 2), np.sum(np.abs(X) ** 2) / x.size, rel_tol=1e-13)
+
+
+# -------------------------------------------------------------- closed forms
+
+
+def small_gnlse(**kw):
+    args = dict(n_t=256, ffac=2, window=2e-12, fwhm=50e-15, soliton_order=1.0,
+                betas=(-20e-27,), raman_fraction=0.0, shock=False, noise_seed=None,
+                length=0.05)
+    args.update(kw)
+    return gnlse_problem(**args)
+
+
+def test_linear_only_trial_is_the_exact_flow():
+    problem = small_gnlse()
+    problem.kerr = None
+    source = rk.StageSource(problem)
+    y0 = problem.field0.reshape(problem.shape)
+    advanced, norm, _ = rk.rk_trial(problem, source, y0, 0.0, 1e-3)
+    assert np.array_equal(advanced, rk.linear_solution(problem, y0, 1e-3))
+    assert norm == 0.0
+
+
+def test_damped_oscillator_integrates_to_one():
+    problem = gnlse_problem(n_t=2048, ffac=2, window=4e-12, noise_seed=None)
+    row = problem.raman.undamped[0] / 2.0  # the row holds 2 h against the |A|^2/2 drive
+    delays = np.arange(row.size) * problem.dt_fine
+    h = row * np.exp(-delays / 32e-15)
+    # The trapezoid's own error at dt = 1 fs on a 77 fs period is 6e-4.
+    assert math.isclose(np.trapezoid(h, dx=problem.dt_fine), 1.0, rel_tol=2e-3)
+
+
+@pytest.mark.slow
+def test_fundamental_soliton_is_stationary():
+    problem = small_gnlse(length=0.0636, rtol=1e-7)
+    y0 = problem.field0.reshape(problem.shape)
+    state, records = rk.march(problem, y0)
+    assert any(r[3] for r in records)
+    a0 = np.abs(np.fft.ifft(y0[0, 0]))
+    a1 = np.abs(np.fft.ifft(state[0, 0]))
+    assert np.max(np.abs(a1 - a0)) / np.max(a0) < 1e-6  # measured 3.3e-8
+    # The same march with the prefactor's sign flipped is far from stationary (0.35).
+    problem.prefactor_terms = -problem.prefactor_terms
+    flipped, _ = rk.march(problem, y0)
+    a2 = np.abs(np.fft.ifft(flipped[0, 0]))
+    assert np.max(np.abs(a2 - a0)) / np.max(a0) > 0.1
+
+
+# ----------------------------------------------------------- the Luna witness
+
+W = np.load(GOLDEN / "luna_plasma_witness.npz")
+CMR = E_CHARGE**2 / M_E
+RESCALE = CMR / float(W["luna_CONSTANTS"][0])
+
+
+def _rescaled(case, stage):
+    free = W[f"luna_{case}_FREE_{stage}"]
+    return free * RESCALE + (W[f"luna_{case}_{stage}"] - free)
+
+
+def _deviation(got, expected):
+    return float(np.max(np.abs(got - expected)) / np.max(np.abs(expected)))
+
+
+def analytic_table(field):
+    """The witness's eighth-power rate as a log-log table: exactly linear per interval."""
+    threshold = float(W["param_rate_threshold"])
+    scale = float(W["param_rate_scale"])
+    reference = float(W["param_rate_reference"])
+    field_max = 4.0 * float(np.max(np.abs(field)))
+    log_min = math.log(threshold)
+    n = 64
+    d = (math.log(field_max) - log_min) / n
+    knots = log_min + d * np.arange(n)
+    coefficients = np.zeros((4, n))
+    coefficients[2] = 8.0
+    coefficients[3] = math.log(scale) + 8.0 * (knots - math.log(reference))
+    rate_max = scale * (field_max / reference) ** 8
+    return P.RateTable(coefficients, log_min, d, threshold, field_max, rate_max)
+
+
+@pytest.mark.parametrize("case,signal", [("SCALAR", "field"), ("ZEROS", "zeroed")])
+def test_plasma_column_matches_luna(case, signal):
+    field = W[f"signal_{signal}"]
+    dt = float(W["param_dt"])
+    ip = float(W["param_ionisation_potential"])
+    table = analytic_table(field)
+    plasma = P.Plasma(table, CMR, ip)
+    f3 = field[None, None, :]
+    rate = P.ionisation_rate(f3, table)
+    assert _deviation(rate[0], W[f"luna_{case}_RATE"]) < 1e-13
+    fraction, free = P.plasma_scan(f3, rate, plasma, 0.0, dt)
+    assert _deviation(fraction[0], W[f"luna_{case}_FRACTION"]) < 1e-13
+    pol = P.plasma_current(f3, rate, fraction, free, plasma, 1.0, dt)
+    assert _deviation(pol[0, 0], _rescaled(case, "P")) < 2e-13
+
+
+def test_plasma_column_on_a_real_rate_matches_luna():
+    field = W["signal_field"]
+    dt = float(W["param_dt"])
+    table = analytic_table(field)  # unused for the rate: the recorded PPT rate is given
+    plasma = P.Plasma(table, CMR, float(W["param_ionisation_potential"]))
+    rate = W["luna_PPT_RATE"][None, :]
+    f3 = field[None, None, :]
+    fraction, free = P.plasma_scan(f3, rate, plasma, 0.0, dt)
+    pol = P.plasma_current(f3, rate, fraction, free, plasma, 1.0, dt)
+    assert _deviation(pol[0, 0], _rescaled("PPT", "P")) < 2e-13
+
+
+def test_preionised_fraction_stays_below_one():
+    field = W["signal_field"]
+    table = analytic_table(field)
+    plasma = P.Plasma(table, CMR, float(W["param_ionisation_potential"]))
+    f3 = field[None, None, :]
+    rate = P.ionisation_rate(f3, table)
+    fraction, _ = P.plasma_scan(f3, rate, plasma, 0.2, float(W["param_dt"]))
+    assert np.all(np.diff(fraction[0]) >= -1e-15)
+    assert fraction.min() >= 0.2 - 1e-15 and fraction.max() <= 1.0
