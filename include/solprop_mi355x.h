@@ -208,6 +208,30 @@ typedef struct sp_rk_scenario {
  * `out`. Returns SP_VALUE if `outlen` is too small. */
 int sp_rk_trace(const sp_rk_scenario* scenario, char* out, size_t outlen);
 
+/* A scripted exponential Adams march (tests/backends/adams_traces.py:50-118 Scenario):
+ * the method's settings (Multistep, src/solprop/stepper/multistep.py:136-214), the
+ * stepper settings, landings and caps, and per attempt three norms, of which the march
+ * reads one per order it scores. `max_calls` < 0 is no budget. */
+typedef struct sp_adams_scenario {
+  int order, variable_order, equal_approach, interval_cap;
+  double safety, max_growth, min_factor, collar_rate;
+  long max_calls;
+  double z0, proposal, dz_min, dz_max, z_end, stretch;
+  const double* norms; long n_attempts;     /* (n_attempts, 3) */
+  const double* save_planes; int n_planes;
+  const double* breakpoints; int n_breakpoints;
+  const double* kinks; int n_kinks;
+  const double* targets; int n_targets;
+  const double* schedule; long n_schedule;
+  const long* quiet_keys; const int* quiet_values; int n_quiet;
+  int has_quiet_interval;
+  double quiet_interval;
+} sp_adams_scenario;
+
+/* Run the library's Adams march over a scripted engine and write its decision trace
+ * as JSON (the layout of the reference's tests/data/adams_traces.json.gz). */
+int sp_adams_trace(const sp_adams_scenario* scenario, char* out, size_t outlen);
+
 /* Testing entry (the counterpart of the reference's `_core._testing`,
  * native/src/bindings/testing.cpp): one batched complex FFT of `n` points over
  * `batch` contiguous rows, unnormalised, sign -1 forward or +1 inverse, through the

@@ -106,6 +106,25 @@ class RKScenarioC(C.Structure):
     ]
 
 
+class AdamsScenarioC(C.Structure):
+    _fields_ = [
+        ("order", _I), ("variable_order", _I), ("equal_approach", _I), ("interval_cap", _I),
+        ("safety", _D), ("max_growth", _D), ("min_factor", _D), ("collar_rate", _D),
+        ("max_calls", C.c_long),
+        ("z0", _D), ("proposal", _D), ("dz_min", _D), ("dz_max", _D), ("z_end", _D),
+        ("stretch", _D),
+        ("norms", _PD), ("n_attempts", C.c_long),
+        ("save_planes", _PD), ("n_planes", _I),
+        ("breakpoints", _PD), ("n_breakpoints", _I),
+        ("kinks", _PD), ("n_kinks", _I),
+        ("targets", _PD), ("n_targets", _I),
+        ("schedule", _PD), ("n_schedule", C.c_long),
+        ("quiet_keys", C.POINTER(C.c_long)), ("quiet_values", C.POINTER(C.c_int)),
+        ("n_quiet", _I),
+        ("has_quiet_interval", _I), ("quiet_interval", _D),
+    ]
+
+
 # The exported entry points and their signatures (include/solprop_mi355x.h).
 SIGNATURES = {
     "sp_workspace_create": (_I, [C.POINTER(ProblemDescC), C.POINTER(C.c_void_p), C.c_char_p,
@@ -122,6 +141,7 @@ SIGNATURES = {
     "sp_workspace_profile": (_I, [C.c_void_p, _I]),
     "sp_workspace_timing": (_I, [C.c_void_p, _PD, C.POINTER(C.c_long)]),
     "sp_rk_trace": (_I, [C.POINTER(RKScenarioC), C.c_char_p, C.c_size_t]),
+    "sp_adams_trace": (_I, [C.POINTER(AdamsScenarioC), C.c_char_p, C.c_size_t]),
     "sp_testing_fft": (_I, [_PD, _PD, C.c_long, C.c_long, _I, C.c_char_p, C.c_size_t]),
     "sp_version": (C.c_char_p, []),
 }

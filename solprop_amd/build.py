@@ -23,7 +23,8 @@ ARCH = "gfx950"
 HIP_SOURCES = ["fft.hip", "stage.hip", "field.hip", "adams.hip", "workspace.hip",
                "field_ws.hip"]
 HOST_SOURCES = ["capi.cpp"]
-STRICT_SOURCES = ["march.cpp", "trace.cpp", "scalars.cpp"]
+STRICT_SOURCES = ["march.cpp", "trace.cpp", "scalars.cpp", "adams_march.cpp",
+                  "adams_trace.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -136,6 +136,20 @@ int sp_rk_trace(const sp_rk_scenario* scenario, char* out, size_t outlen) {
   return SP_OK;
 }
 
+int sp_adams_trace(const sp_adams_scenario* scenario, char* out, size_t outlen) {
+  if (!scenario || !out) return SP_VALUE;
+  std::string trace;
+  try {
+    trace = sp::adams::run_adams_trace(*scenario);
+  } catch (const std::exception& e) {
+    std::snprintf(out, outlen, "%s", e.what());
+    return SP_BACKEND;
+  }
+  if (trace.size() + 1 > outlen) return SP_VALUE;
+  std::memcpy(out, trace.c_str(), trace.size() + 1);
+  return SP_OK;
+}
+
 int sp_testing_fft(const double* in, double* out, long n, long batch, int sign, char* msg,
                    size_t msglen) {
   if (!in || !out) return SP_VALUE;

@@ -434,8 +434,62 @@ def rk_trace(scenario: dict) -> dict:
     return json.loads(buf.value.decode())
 
 
+def adams_trace(scenario: dict) -> dict:
+    """Run a scripted scenario through the library's Adams march; return its trace."""
+    import json
+
+    lib = _lib.lib()
+    keep = _Keep()
+    s = _lib.AdamsScenarioC()
+    m = scenario["method"]
+    s.order = m.get("order", 5)
+    s.variable_order = int(m.get("variable_order", True))
+    s.equal_approach = int(m.get("equal_approach", True))
+    s.interval_cap = int(m.get("interval_cap", False))
+    s.safety = m.get("safety", 0.8)
+    s.max_growth = m.get("max_growth", 2.0)
+    s.min_factor = m.get("min_factor", 0.2)
+    s.collar_rate = float(scenario["collar_rate"])
+    s.max_calls = -1 if m.get("max_calls") is None else int(m["max_calls"])
+    for name in ("z0", "proposal", "dz_min", "dz_max", "z_end", "stretch"):
+        setattr(s, name, float(scenario[name]))
+    norms = np.zeros((max(len(scenario["norms"]), 1), 3))
+    for i, row in enumerate(scenario["norms"]):
+        norms[i, : len(row)] = row
+    s.norms = keep.f64(norms)
+    s.n_attempts = len(scenario["norms"])
+    planes = [] if scenario["interpolate"] else scenario["save_planes"]
+    s.save_planes = keep.f64(planes or [0.0])
+    s.n_planes = len(planes)
+    s.breakpoints = keep.f64(scenario["breakpoints"] or [0.0])
+    s.n_breakpoints = len(scenario["breakpoints"])
+    s.kinks = keep.f64(scenario["kinks"] or [0.0])
+    s.n_kinks = len(scenario["kinks"])
+    targets = scenario["targets"]
+    s.targets = keep.f64(targets or [0.0])
+    s.n_targets = 0 if targets is None else len(targets)
+    schedule = scenario["schedule"]
+    s.schedule = keep.f64(schedule or [0.0])
+    s.n_schedule = 0 if schedule is None else len(schedule)
+    quiet = {int(a): bool(b) for a, b in scenario["quiet"].items()}
+    qk = np.ascontiguousarray(np.array(list(quiet) or [0], dtype=np.int64))
+    qv = np.ascontiguousarray(np.array([int(v) for v in quiet.values()] or [0], dtype=np.int32))
+    keep.arrays += [qk, qv]
+    s.quiet_keys = qk.ctypes.data_as(C.POINTER(C.c_long))
+    s.quiet_values = qv.ctypes.data_as(C.POINTER(C.c_int))
+    s.n_quiet = len(quiet)
+    s.has_quiet_interval = int(scenario["quiet_interval"] is not None)
+    s.quiet_interval = float(scenario["quiet_interval"] or 0.0)
+    size = 1 << 23
+    buf = C.create_string_buffer(size)
+    status = lib.sp_adams_trace(C.byref(s), buf, size)
+    if status != _lib.SP_OK:
+        raise BackendError(buf.value.decode(errors="replace"))
+    return json.loads(buf.value.decode())
+
+
 __all__ = [
     "BackendError", "BackendRefusal", "Batch", "ConvergenceError", "LifecycleError",
-    "Probe", "Session", "Settings", "Workspace", "lower", "rk_trace",
+    "Probe", "Session", "Settings", "Workspace", "adams_trace", "lower", "rk_trace",
 ]
 _ = math
