@@ -70,6 +70,9 @@ typedef struct sp_problem_desc {
   int n_phi;
   const double* phi_terms;          /* complex (n_phi, C, N) */
   sp_scalar_path phi_paths[SP_MAX_TERMS];
+  /* dW_t/dz: the operator's rate D(z) = Σ_t W_t'(z) Φ_t (translate/linop.py phi_rate),
+   * read by the exponential Adams compensation of a varying operator */
+  sp_scalar_path phi_rate_paths[SP_MAX_TERMS];
   int n_prefactor;
   const double* prefactor_terms;    /* complex (n_prefactor, C, N) */
   sp_scalar_path prefactor_paths[SP_MAX_TERMS];
@@ -111,6 +114,12 @@ typedef struct sp_problem_desc {
   int lag, reject_free, estimator_order, auto_init;
   double dz_min, dz_max, stretch;
   double dz_init;                   /* the first proposal, m */
+  /* Method: 0 the embedded Runge-Kutta march (envelope route), 1 the exponential Adams
+   * march (field route), with the settings of Multistep
+   * (src/solprop/stepper/multistep.py:136-214). */
+  int method;
+  int adams_order, adams_variable_order, adams_equal_approach;
+  double adams_safety, adams_max_growth, adams_min_factor;
 } sp_problem_desc;
 
 typedef struct sp_workspace sp_workspace;

@@ -613,8 +613,201 @@ def run_scenario(scenario: dict) -> dict:
     return {"events": events, "end": end}
 
 
+# ----------------------------------------------------- the exponential engine
+
+
+def moments(exponent, count):
+    """``M_k = k! phi_{k+1}(x)``, ``k < count`` (``adams_kernels.py:421-447``)."""
+    phi = phi_functions(exponent, count)
+    factors = np.array([math.factorial(k) for k in range(count)], dtype=np.float64)
+    return phi[1:] * factors.reshape((count, *([1] * exponent.ndim)))
+
+
+def apply(M, coefficients, values, dz):
+    """``dz Σ_k M_k (Σ_j c_jk X_j)`` (``adams_kernels.py:521-551``)."""
+    total = np.zeros_like(values[0])
+    for k in range(coefficients.shape[1]):
+        combined = np.zeros_like(values[0])
+        for weight, value in zip(coefficients[:, k], values, strict=True):
+            if weight != 0.0:
+                combined = P.accumulate(combined, value, float(weight))
+        total += M[k] * combined
+    return dz * total
+
+
+class ExponentialEngine:
+    """``ExponentialHistory`` (``numpy_ref/multistep.py:806-1131``) over a problem.
+
+    Untransported source rows; a varying operator (more than one Φ term with a
+    non-constant rate) adds the transported compensation ``(D(z_j) - x/h) u_j``.
+    """
+
+    def __init__(self, problem, source, n_slots):
+        from oracle import rk
+
+        self.p = problem
+        self.source = source
+        self.rk = rk
+        self.n_slots = n_slots
+        self.state = None
+        self.store = None
+        self.states = None
+        self.rates = None
+        self.compensated = len(problem.phi_paths) > 1
+        self.held = None
+
+    def lab_source(self, lab, z):
+        """The source at a lab-frame field, unit frame factor (``workspace.py:819``)."""
+        return self.source.evaluate(lab, z, 1.0)
+
+    def start(self, z):
+        value = self.lab_source(self.state, z)
+        if self.store is None:
+            self.store = np.zeros((self.n_slots, *value.shape), dtype=np.complex128)
+            if self.compensated:
+                self.states = np.zeros_like(self.store)
+                self.rates = np.zeros_like(self.store)
+        self.store[0] = value
+        if self.compensated:
+            self.states[0] = self.state
+            self.rates[0] = self.p.phi_rate(z)
+
+    def trial(self, z, dz, st: Stencil):
+        p = self.p
+        state = self.state
+        nodes, slots, order, orders = st.nodes, st.rows, st.order, st.orders
+        width = 1 + max(orders)
+        exponent = self.rk.delta_phi(p, z, z + dz)
+        M = moments(exponent, width)
+        flow = np.exp(exponent)
+        rows_read = slots[: width - 1]
+        values = [self.store[s] for s in rows_read[:order]]
+        lab = flow * state + apply(M, lagrange_coefficients(nodes[:order]), values, dz)
+        carried = None
+        if self.compensated:
+            reference = exponent / dz
+            used = list(nodes[: len(rows_read)])
+            carried = np.stack([
+                np.exp((1.0 - node) * exponent)
+                * ((self.rates[slot] - reference) * self.states[slot])
+                for slot, node in zip(rows_read, used, strict=True)
+            ])
+
+            def csum(ns, first, count):
+                w = integrated_weights(np.asarray(ns))
+                total = np.tensordot(w[len(ns) - count:], carried[:count], axes=1)
+                if first is not None:
+                    total = total + w[0] * first
+                return dz * total
+
+            lab = lab + csum(used[:order], None, order)
+        source = self.lab_source(lab, z + dz)
+        X = [source, *[self.store[s] for s in rows_read]]
+        corrector = exponential_corrector(nodes, order, width)
+        drows = exponential_rows(nodes, orders, width)
+        extra = None
+        endpoint_rate = None
+        if self.compensated:
+            endpoint_rate = p.phi_rate(z + dz)
+            endpoint = (endpoint_rate - exponent / dz) * lab
+
+            def corr(k):
+                return csum([1.0, *used[:k]], endpoint, k)
+
+            def pred(k):
+                return csum(used[:k], None, k)
+
+            extra = (corr(order), [corr(k) - pred(k) for k in orders])
+        advanced = flow * state + apply(M, corrector, X, dz)
+        if extra is not None:
+            advanced = advanced + extra[0]
+        mask = p.norm_mask
+        magnitude = float(np.linalg.norm(state[..., mask]))
+        divisor = p.rtol * max(magnitude, magnitude) + p.atol
+        norms = np.empty(len(orders))
+        for index, row in enumerate(drows):
+            defect = apply(M, row, X, dz)
+            if extra is not None:
+                defect = defect + extra[1][index]
+            norms[index] = float(np.linalg.norm(defect[..., mask] / divisor))
+        self.held = (source, lab, advanced, endpoint_rate)
+        return norms
+
+    def accept(self, dz, free, st, kept):
+        source, lab, advanced, endpoint_rate = self.held
+        self.held = None
+        self.store[free] = source
+        if self.compensated:
+            self.states[free] = lab
+            self.rates[free] = endpoint_rate
+        self.state = P.apply_hard_mask(advanced, self.p.active_mask)
+
+
+class _PlainHost:
+    collar_quiet = False
+
+    def __init__(self):
+        self.rows = []
+
+    def cap(self, proposal, z):
+        return proposal
+
+    def record(self, end, dz, norm, accepted):
+        self.rows.append((end, dz, norm, accepted))
+
+    def stepped(self):
+        pass
+
+    def natural_stop(self, target):
+        return False
+
+
+def adams_policy_for(problem, **overrides) -> MarchPolicy:
+    values = dict(
+        order=problem.adams_order,
+        variable_order=problem.adams_variable_order,
+        dz_min=1e-9 * problem.z_end,
+        dz_max=problem.z_end,
+        z_end=problem.z_end,
+        breakpoints=(0.0, problem.z_end),
+    )
+    values.update(overrides)
+    return MarchPolicy(**values)
+
+
+def march(problem, state, *, target=None, proposal=None, max_attempts=None, policy=None,
+          on_attempt=None):
+    """Adaptive exponential Adams march of the problem (``HostAdamsMarch.march_to``).
+
+    Returns ``(state, rows)`` with rows ``(end, dz, norm, accepted)`` per attempt.
+    """
+    from oracle import rk
+
+    policy = policy or adams_policy_for(problem)
+    engine = ExponentialEngine(problem, rk.StageSource(problem), policy.order + 2)
+    engine.state = state
+    host = _PlainHost()
+    if on_attempt is not None or max_attempts is not None:
+        inner = host.record
+
+        def record(end, dz, norm, accepted):
+            inner(end, dz, norm, accepted)
+            if on_attempt is not None:
+                on_attempt(end, dz, norm, accepted)
+            if max_attempts is not None and len(host.rows) >= max_attempts:
+                raise StopIteration
+
+        host.record = record
+    am = AdamsMarch(engine, policy, host, proposal=proposal or 1e-4 * problem.z_end)
+    try:
+        am.march_to(problem.z_end if target is None else target)
+    except StopIteration:
+        pass
+    return engine.state, host.rows
+
+
 __all__ = [
-    "AdamsMarch", "MarchPolicy", "exponential_corrector", "exponential_rows",
-    "integrated_weights", "lagrange_coefficients", "phi_functions", "run_scenario",
+    "AdamsMarch", "ExponentialEngine", "MarchPolicy", "exponential_corrector",
+    "exponential_rows", "integrated_weights", "lagrange_coefficients", "march",
+    "phi_functions", "run_scenario",
 ]
-_ = P

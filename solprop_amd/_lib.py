@@ -46,6 +46,7 @@ class ProblemDescC(C.Structure):
         ("omega_window", _PD), ("omega", _PD), ("time_window", _PD),
         ("time_window_derivative", _PD), ("dt_fine", _D), ("z_end", _D),
         ("n_phi", _I), ("phi_terms", _PD), ("phi_paths", ScalarPathC * SP_MAX_TERMS),
+        ("phi_rate_paths", ScalarPathC * SP_MAX_TERMS),
         ("n_prefactor", _I), ("prefactor_terms", _PD),
         ("prefactor_paths", ScalarPathC * SP_MAX_TERMS),
         ("synthesis", _PD), ("projection", _PD),
@@ -66,6 +67,9 @@ class ProblemDescC(C.Structure):
         ("beta1", _D), ("beta2", _D),
         ("lag", _I), ("reject_free", _I), ("estimator_order", _I), ("auto_init", _I),
         ("dz_min", _D), ("dz_max", _D), ("stretch", _D), ("dz_init", _D),
+        ("method", _I), ("adams_order", _I), ("adams_variable_order", _I),
+        ("adams_equal_approach", _I), ("adams_safety", _D), ("adams_max_growth", _D),
+        ("adams_min_factor", _D),
     ]
 
 

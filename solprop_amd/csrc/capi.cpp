@@ -2,15 +2,27 @@
 // place of exceptions, host arrays in and out.
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
+#include "field_ws.hpp"
 #include "trace.hpp"
 #include "workspace.hpp"
 
+namespace {
+// The envelope route runs the embedded Runge-Kutta march; the modal field route the
+// exponential Adams march.
+std::unique_ptr<sp::Propagator> make_propagator(const sp_problem_desc& d) {
+  if (d.kind == 1) return std::make_unique<sp::FieldWorkspace>(d);
+  return std::make_unique<sp::Workspace>(d);
+}
+}  // namespace
+
 struct sp_workspace {
-  sp::Workspace impl;
-  explicit sp_workspace(const sp_problem_desc& d) : impl(d) {}
+  std::unique_ptr<sp::Propagator> owner;
+  sp::Propagator& impl;
+  explicit sp_workspace(const sp_problem_desc& d) : owner(make_propagator(d)), impl(*owner) {}
 };
 
 namespace {
@@ -69,7 +81,7 @@ int sp_workspace_load(sp_workspace* ws, double z, const double* field, char* msg
 int sp_workspace_trial(sp_workspace* ws, double z, double dz, double* norm, char* msg,
                        size_t msglen) {
   if (!ws || !norm) return SP_VALUE;
-  return guarded(msg, msglen, [&] { *norm = ws->impl.trial(z, dz, true); });
+  return guarded(msg, msglen, [&] { *norm = ws->impl.probe_trial(z, dz); });
 }
 
 int sp_workspace_trial_field(sp_workspace* ws, double* out, char* msg, size_t msglen) {

@@ -44,6 +44,7 @@ struct DeviceProblem {
   int has_kerr, kerr_real, has_plasma, noise_modified;
   double kerr_coefficient, kerr_self, kerr_conjugate;
   double raman_fraction, raman_epsilon0;
+  double plasma_cmr, plasma_ip;  // e^2/m_e and the ionisation potential, J
   double dt_fine, inv_T, inv_2T;
   const int* fine_to_coarse;   // (NF) coarse bin of each fine bin, -1 in the gaps
   const int* crop;             // (N) fine bin of each coarse bin

@@ -1,0 +1,85 @@
+// The modal field-grid route under the exponential Adams march: device record and
+// launchers.
+//
+// One source evaluation is: scatter the lab-frame modal spectrum into the packed
+// real-transform input (1/T folded in) -> inverse FFT of T/2 packed points -> the node
+// passes (synthesis over the radial nodes, Kerr, the photoionisation rate and the two
+// cumulative plasma integrals as tile scans, the collars, and the projection of the
+// local polarisation and both held current streams back onto the modes) -> forward
+// FFT of the 3C packed streams -> finish (unpack, held-current assembly on the active
+// bins, prefactor and spectral collar).
+#pragma once
+
+#include "device.hpp"
+
+namespace sp {
+
+constexpr int kMaxAdamsWidth = 8;  // 1 + the highest scored order (MAX_ADAMS_ORDER 6)
+constexpr int kMaxScored = 3;      // the step's own order and two neighbours
+constexpr int kScanTile = 128;     // samples per plasma-scan tile (one per thread)
+constexpr int kNodeBlock = 64;     // radial nodes per workgroup of the node passes
+
+// What one Adams trial uploads (host scalars and the Lagrange tables of its stencil).
+struct AdamsRecord {
+  double dz;
+  double x_w[kMaxTerms];                 // whole-step Φ weights
+  double rate_end_w[kMaxTerms];          // D(z + dz) weights
+  double pref_w[kMaxTerms];              // prefactor at z + dz
+  double density, preionisation;         // at z + dz
+  double slot_rate_w[kMaxAdamsWidth][kMaxTerms];  // D(z_j) weights of each store slot
+  double nodes[kMaxAdamsWidth];          // θ_j of the rows read, newest first
+  double pred_c[kMaxAdamsWidth][kMaxAdamsWidth];  // [j][k], j,k < order
+  double corr_c[kMaxAdamsWidth][kMaxAdamsWidth];  // [j][k] over X = (source, rows)
+  double rows_c[kMaxScored][kMaxAdamsWidth][kMaxAdamsWidth];
+  double comp_pred_w[kMaxScored][kMaxAdamsWidth];      // weights of nodes[:k]
+  double comp_corr_w[kMaxScored][kMaxAdamsWidth + 1];  // weights of [1, nodes[:k]]
+  int slots[kMaxAdamsWidth];             // store rows read, newest first
+  int order, width, n_orders, n_read, compensated, free_slot, pad0, pad1;
+  int orders[kMaxScored];
+  int pad2;
+};
+
+struct FieldBuffers {
+  cplx* y;          // (C, N) committed state
+  cplx* store;      // (S, C, N) untransported source rows
+  cplx* states;     // (S, C, N) the state each row was evaluated at
+  cplx* moments;    // (W, C, N) k! phi_{k+1}(x)
+  cplx* flow;       // (C, N) exp(x)
+  cplx* carried;    // (W-1, C, N) transported compensation of the rows read
+  cplx* lab;        // (C, N) predicted lab-frame field
+  cplx* source;     // (C, N) source at the prediction
+  cplx* advanced;   // (C, N) corrected field
+  cplx* modal_t;    // (C, T/2) packed real modal fields
+  cplx* streams;    // (3, C, T/2) packed polarisation, gJ, g'J
+  double* partial;  // (node blocks, 3, C, T) projected stream partials
+  double* tile_x;   // (Q, tiles) exposure tile sums, then offsets
+  double* tile_f;   // (Q, tiles) free-current tile sums, then offsets
+  double* partials; // (1 + kMaxScored, blocks) masked squared sums
+  const cplx* roots_T;  // unit roots of T
+  const double* rate_c; // (4, n_intervals) rate table
+};
+
+struct PlasmaTable {
+  double log_field_min, d_log_field, field_min, field_max, rate_max;
+  double charge_mass_ratio, ionisation_potential;
+  int n_intervals;
+};
+
+unsigned correct_blocks(const DeviceProblem& P);
+
+void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                          hipStream_t s);
+void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuffers B,
+                          hipStream_t s);
+void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,
+                        FieldBuffers B, hipStream_t s);
+void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                         cplx* out, hipStream_t s);
+void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                          hipStream_t s);
+void launch_adams_accept(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                         hipStream_t s);
+void launch_adams_start(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                        hipStream_t s);
+
+}  // namespace sp

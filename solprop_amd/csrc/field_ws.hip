@@ -1,0 +1,448 @@
+// The modal field-grid workspace under the exponential Adams march (see field_ws.hpp).
+#include "field_ws.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+namespace sp {
+
+template <typename T>
+T* FieldWorkspace::alloc(std::size_t n) {
+  T* p = nullptr;
+  if (n == 0) n = 1;
+  SP_HIP(hipMalloc(&p, n * sizeof(T)));
+  allocations_.push_back(p);
+  device_bytes_ += n * sizeof(T);
+  return p;
+}
+
+template <typename T>
+T* FieldWorkspace::upload(const T* host, std::size_t n) {
+  T* p = alloc<T>(n);
+  if (host && n) SP_HIP(hipMemcpy(p, host, n * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
+  // Refusals first, before anything is allocated.
+  SP_CHECK(d.kind == 1 && d.method == 1,
+           "the field workspace runs the real field grid under the exponential Adams march");
+  const int C = d.n_modes, N = d.n_omega, T = d.n_to, Q = d.n_nodes;
+  SP_CHECK(T >= 2 * kScanTile && (T & (T - 1)) == 0, "n_to must be a power of two >= 256");
+  SP_CHECK(d.n_t >= 4 && T % d.n_t == 0, "n_to must be a multiple of n_t");
+  SP_CHECK(N == d.n_t / 2 + 1, "a field grid has n_omega = n_t/2 + 1 bins");
+  SP_CHECK(C >= 1 && C <= 8, "1..8 modes");
+  SP_CHECK(Q >= 1, "at least one quadrature node");
+  SP_CHECK(d.n_phi >= 1 && d.n_phi <= kMaxTerms, "1..4 phi terms");
+  SP_CHECK(d.n_prefactor >= 1 && d.n_prefactor <= kMaxTerms, "1..4 prefactor terms");
+  SP_CHECK(d.n_raman == 0, "the field route runs Kerr and photoionisation (no Raman)");
+  SP_CHECK(!d.has_plasma || d.n_taps <= 1,
+           "the band running-integral rule is not built; the trapezoid rule is");
+  SP_CHECK(d.adams_order >= 1 && d.adams_order + 1 < kMaxAdamsWidth, "Adams order 1..6");
+  SP_CHECK(d.noise_model != 2, "the modified noise model is not built on the field route");
+  SP_CHECK(d.crop_index && d.active_mask && d.norm_mask && d.omega_window && d.omega &&
+               d.time_window && d.time_window_derivative && d.phi_terms &&
+               d.prefactor_terms && d.synthesis && d.projection,
+           "a required problem array is missing");
+  for (int b = 0; b < N; ++b)
+    SP_CHECK(d.crop_index[b] == b, "a field grid's coarse bins lead the fine spectrum");
+
+  for (int t = 0; t < d.n_phi; ++t) {
+    phi_paths_.emplace_back();
+    phi_paths_.back().assign(d.phi_paths[t]);
+    rate_paths_.emplace_back();
+    rate_paths_.back().assign(d.phi_rate_paths[t]);
+  }
+  for (int t = 0; t < d.n_prefactor; ++t) {
+    pref_paths_.emplace_back();
+    pref_paths_.back().assign(d.prefactor_paths[t]);
+  }
+  density_.assign(d.density);
+  preion_.assign(d.preionisation);
+  rtol_ = d.rtol;
+  atol_ = d.atol;
+  compensated_ = d.n_phi > 1;
+
+  policy_.order = d.adams_order;
+  policy_.variable_order = d.adams_variable_order != 0;
+  policy_.safety = d.adams_safety;
+  policy_.max_growth = d.adams_max_growth;
+  policy_.min_factor = d.adams_min_factor;
+  policy_.equal_approach = d.adams_equal_approach != 0;
+  policy_.dz_min = d.dz_min;
+  policy_.dz_max = d.dz_max;
+  policy_.stretch = d.stretch;
+  policy_.z_end = d.z_end;
+  policy_.breakpoints = {0.0, d.z_end};
+  dz_init_ = d.dz_init > 0.0 ? d.dz_init : 1e-4 * d.z_end;
+
+  SP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  const char* env = std::getenv("SOLPROP_MI355X_GRAPH");
+  use_graphs_ = !(env && env[0] == '0');
+
+  P_.kind = 1;
+  P_.C = C;
+  P_.N = N;
+  P_.T = T;
+  P_.NF = T / 2 + 1;
+  P_.n_phi = d.n_phi;
+  P_.n_pref = d.n_prefactor;
+  P_.Q = Q;
+  P_.has_kerr = d.has_kerr;
+  P_.kerr_real = 1;
+  P_.has_plasma = d.has_plasma;
+  P_.kerr_coefficient = d.kerr_coefficient;
+  P_.plasma_cmr = d.charge_mass_ratio;
+  P_.plasma_ip = d.ionisation_potential;
+  P_.dt_fine = d.dt_fine;
+  P_.inv_T = 1.0 / T;
+  std::vector<int> crop(N);
+  for (int b = 0; b < N; ++b) crop[b] = b;
+  P_.crop = upload(crop.data(), crop.size());
+  P_.active = upload(d.active_mask, N);
+  P_.normmask = upload(d.norm_mask, N);
+  P_.omega_window = upload(d.omega_window, N);
+  P_.omega = upload(d.omega, N);
+  P_.t_window = upload(d.time_window, T);
+  P_.t_deriv = upload(d.time_window_derivative, T);
+  const std::size_t CN = static_cast<std::size_t>(C) * N;
+  P_.phi = upload(reinterpret_cast<const cplx*>(d.phi_terms), d.n_phi * CN);
+  P_.pref = upload(reinterpret_cast<const cplx*>(d.prefactor_terms), d.n_prefactor * CN);
+  P_.synthesis = upload(d.synthesis, static_cast<std::size_t>(Q) * C);
+  P_.projection = upload(d.projection, static_cast<std::size_t>(Q) * C);
+  if (d.has_plasma) {
+    SP_CHECK(d.rate_intervals >= 1 && d.rate_coefficients, "the rate table is missing");
+    pt_.log_field_min = d.log_field_min;
+    pt_.d_log_field = d.d_log_field;
+    pt_.field_min = d.field_min;
+    pt_.field_max = d.field_max;
+    pt_.rate_max = d.rate_max;
+    pt_.charge_mass_ratio = d.charge_mass_ratio;
+    pt_.ionisation_potential = d.ionisation_potential;
+    pt_.n_intervals = d.rate_intervals;
+    B_.rate_c = upload(d.rate_coefficients, 4 * static_cast<std::size_t>(d.rate_intervals));
+  }
+
+  // Store slots, and the widest stencil's moments and carried rows.
+  const int S = policy_.order + 2;
+  const int W = S;
+  B_.y = alloc<cplx>(CN);
+  B_.store = alloc<cplx>(S * CN);
+  B_.states = compensated_ ? alloc<cplx>(S * CN) : nullptr;
+  B_.moments = alloc<cplx>(W * CN);
+  B_.flow = alloc<cplx>(CN);
+  B_.carried = compensated_ ? alloc<cplx>(W * CN) : nullptr;
+  B_.lab = alloc<cplx>(CN);
+  B_.source = alloc<cplx>(CN);
+  B_.advanced = alloc<cplx>(CN);
+  B_.modal_t = alloc<cplx>(static_cast<std::size_t>(C) * (T / 2));
+  B_.streams = alloc<cplx>(3UL * C * (T / 2));
+  const int node_blocks = (Q + kNodeBlock - 1) / kNodeBlock;
+  B_.partial = alloc<double>(static_cast<std::size_t>(node_blocks) * 3 * C * T);
+  B_.tile_x = alloc<double>(static_cast<std::size_t>(Q) * (T / kScanTile));
+  B_.tile_f = alloc<double>(static_cast<std::size_t>(Q) * (T / kScanTile));
+  n_blocks_ = static_cast<int>(correct_blocks(P_));
+  B_.partials = alloc<double>((1 + kMaxScored) * static_cast<std::size_t>(n_blocks_));
+  const auto roots = unit_roots(static_cast<std::size_t>(T));
+  B_.roots_T = upload(roots.data(), roots.size());
+  scratch_ = alloc<cplx>(3UL * C * (T / 2));
+  inv_plan_.init(T / 2, C);
+  fwd_plan_.init(T / 2, 3 * C);
+  rec_dev_ = alloc<AdamsRecord>(1);
+  SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&rec_host_), sizeof(AdamsRecord)));
+  std::memset(rec_host_, 0, sizeof(AdamsRecord));
+  SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&partials_host_),
+                       sizeof(double) * (1 + kMaxScored) * n_blocks_));
+  SP_HIP(hipEventCreate(&ev0_));
+  SP_HIP(hipEventCreate(&ev1_));
+  slot_rates_.assign(S, std::vector<double>(kMaxTerms, 0.0));
+}
+
+FieldWorkspace::~FieldWorkspace() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  for (void* p : allocations_) (void)hipFree(p);
+  if (rec_host_) (void)hipHostFree(rec_host_);
+  if (partials_host_) (void)hipHostFree(partials_host_);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// ------------------------------------------------------------- device sequences
+
+void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
+  launch_field_scatter(P_, spectrum, B_, stream_);
+  inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
+  launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
+  fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
+  launch_field_finish(P_, rec_dev_, B_, out, stream_);
+}
+
+void FieldWorkspace::enqueue_trial() {
+  launch_adams_predict(P_, rec_dev_, B_, stream_);
+  enqueue_source(B_.lab, B_.source);
+  launch_adams_correct(P_, rec_dev_, B_, stream_);
+}
+
+void FieldWorkspace::enqueue_start() {
+  enqueue_source(B_.y, B_.source);
+  launch_adams_start(P_, rec_dev_, B_, stream_);
+}
+
+void FieldWorkspace::run(int key) {
+  SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
+                        stream_));
+  if (profile_ && key == 1) SP_HIP(hipEventRecord(ev0_, stream_));
+  if (!use_graphs_) {
+    if (key == 1)
+      enqueue_trial();
+    else
+      enqueue_start();
+  } else {
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+      hipGraph_t graph;
+      SP_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      if (key == 1)
+        enqueue_trial();
+      else
+        enqueue_start();
+      SP_HIP(hipStreamEndCapture(stream_, &graph));
+      hipGraphExec_t exec;
+      SP_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      SP_HIP(hipGraphDestroy(graph));
+      it = graphs_.emplace(key, exec).first;
+    }
+    SP_HIP(hipGraphLaunch(it->second, stream_));
+    graph_launches_ += 1;
+  }
+}
+
+// -------------------------------------------------------------- host scalars
+
+void FieldWorkspace::fill_position(double zp) {
+  AdamsRecord& r = *rec_host_;
+  for (int t = 0; t < P_.n_pref; ++t) r.pref_w[t] = pref_paths_[t](zp);
+  r.density = density_(zp);
+  r.preionisation = preion_(zp);
+  r.compensated = compensated_ ? 1 : 0;
+}
+
+// ------------------------------------------------------------------ the engine
+
+void FieldWorkspace::start(double z) {
+  fill_position(z);
+  for (int t = 0; t < P_.n_phi; ++t) slot_rates_[0][t] = rate_paths_[t](z);
+  run(0);
+  SP_HIP(hipStreamSynchronize(stream_));
+  evaluations_ += 1;
+}
+
+std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Stencil& s) {
+  AdamsRecord& r = *rec_host_;
+  const double end = z + dz;
+  r.dz = dz;
+  for (int t = 0; t < P_.n_phi; ++t) {
+    r.x_w[t] = path_increment(phi_paths_[t], z, end);
+    r.rate_end_w[t] = rate_paths_[t](end);
+  }
+  fill_position(end);
+  r.order = s.order;
+  r.n_orders = static_cast<int>(s.orders.size());
+  int top = 0;
+  for (int i = 0; i < r.n_orders; ++i) {
+    r.orders[i] = s.orders[i];
+    top = std::max(top, s.orders[i]);
+  }
+  r.width = 1 + top;
+  r.n_read = r.width - 1;
+  SP_CHECK(r.width <= kMaxAdamsWidth, "the stencil is wider than the device record");
+  for (int j = 0; j < r.n_read; ++j) {
+    r.slots[j] = s.rows[j];
+    r.nodes[j] = s.nodes[j];
+    for (int t = 0; t < kMaxTerms; ++t) r.slot_rate_w[j][t] = slot_rates_[s.rows[j]][t];
+  }
+  double rows[kMaxAdamsWidth * kMaxAdamsWidth];
+  double points[kMaxAdamsWidth + 1];
+  double weights[kMaxAdamsWidth + 1];
+  std::memset(r.pred_c, 0, sizeof r.pred_c);
+  std::memset(r.corr_c, 0, sizeof r.corr_c);
+  std::memset(r.rows_c, 0, sizeof r.rows_c);
+  std::memset(r.comp_pred_w, 0, sizeof r.comp_pred_w);
+  std::memset(r.comp_corr_w, 0, sizeof r.comp_corr_w);
+  // The predictor's basis on the order newest nodes.
+  lagrange_basis(s.nodes.data(), s.order, 1.0, false, rows, nullptr);
+  for (int j = 0; j < s.order; ++j)
+    for (int k = 0; k < s.order; ++k) r.pred_c[j][k] = rows[j * s.order + k];
+  // The corrector through the new sample, [1, nodes[:order]].
+  points[0] = 1.0;
+  for (int j = 0; j < s.order; ++j) points[1 + j] = s.nodes[j];
+  lagrange_basis(points, s.order + 1, 1.0, false, rows, nullptr);
+  for (int j = 0; j <= s.order; ++j)
+    for (int k = 0; k <= s.order; ++k) r.corr_c[j][k] = rows[j * (s.order + 1) + k];
+  // Each scored order's defect: its corrector less its predictor; the compensation's
+  // ordinary Adams weights of both.
+  for (int i = 0; i < r.n_orders; ++i) {
+    const int k = r.orders[i];
+    points[0] = 1.0;
+    for (int j = 0; j < k; ++j) points[1 + j] = s.nodes[j];
+    lagrange_basis(points, k + 1, 1.0, false, rows, nullptr);
+    for (int j = 0; j <= k; ++j)
+      for (int c = 0; c <= k; ++c) r.rows_c[i][j][c] = rows[j * (k + 1) + c];
+    lagrange_basis(s.nodes.data(), k, 1.0, false, rows, nullptr);
+    for (int j = 0; j < k; ++j)
+      for (int c = 0; c < k; ++c) r.rows_c[i][1 + j][c] -= rows[j * k + c];
+    lagrange_basis(s.nodes.data(), k, 1.0, true, nullptr, weights);
+    for (int j = 0; j < k; ++j) r.comp_pred_w[i][j] = weights[j];
+    lagrange_basis(points, k + 1, 1.0, true, nullptr, weights);
+    for (int j = 0; j <= k; ++j) r.comp_corr_w[i][j] = weights[j];
+  }
+  run(1);
+  SP_HIP(hipMemcpyAsync(partials_host_, B_.partials,
+                        sizeof(double) * (1 + kMaxScored) * n_blocks_, hipMemcpyDeviceToHost,
+                        stream_));
+  if (profile_) SP_HIP(hipEventRecord(ev1_, stream_));
+  SP_HIP(hipStreamSynchronize(stream_));
+  if (profile_) {
+    float ms = 0.0f;
+    SP_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    time_total_ms_ += ms;
+    timed_ += 1;
+  }
+  held_rate_end_.assign(r.rate_end_w, r.rate_end_w + kMaxTerms);
+  const int nb = n_blocks_;
+  double s0 = 0.0;
+  for (int b = 0; b < nb; ++b) s0 = s0 + partials_host_[b];
+  const double magnitude = std::sqrt(s0);
+  const double divisor = rtol_ * magnitude + atol_;
+  std::vector<double> norms(r.n_orders);
+  for (int i = 0; i < r.n_orders; ++i) {
+    double si = 0.0;
+    for (int b = 0; b < nb; ++b) si = si + partials_host_[(1 + i) * nb + b];
+    norms[i] = std::sqrt(si) / divisor;
+  }
+  trials_ += 1;
+  evaluations_ += 1;
+  return norms;
+}
+
+void FieldWorkspace::accept(double, int free, const adams::Stencil&,
+                            const std::vector<adams::Entry>&) {
+  rec_host_->free_slot = free;
+  SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
+                        stream_));
+  launch_adams_accept(P_, rec_dev_, B_, stream_);
+  SP_HIP(hipStreamSynchronize(stream_));
+  slot_rates_[free] = held_rate_end_;
+  accepted_ += 1;
+}
+
+// -------------------------------------------------------------- the interface
+
+void FieldWorkspace::load(double z, const double* field) {
+  SP_CHECK(field != nullptr, "load needs a field");
+  const std::size_t CN = static_cast<std::size_t>(P_.C) * P_.N;
+  SP_HIP(hipMemcpy(B_.y, field, CN * sizeof(cplx), hipMemcpyHostToDevice));
+  adams::Engine& engine = *this;
+  march_ = std::make_unique<adams::March>(policy_, engine, host_, z, dz_init_);
+  z_ = z;
+  loaded_ = true;
+}
+
+double FieldWorkspace::probe_trial(double z, double dz) {
+  if (!loaded_) throw Error(kState, "load a state before taking a trial");
+  if (march_->history.empty()) {
+    march_->z = z;
+    start(z);
+    march_->history.assign(1, adams::Entry{z, 0, 0});
+    march_->ledger.startup += 1;
+  }
+  const adams::Stencil s = adams::stencil(policy_, z, dz, march_->history, march_->order,
+                                          march_->last_rejected);
+  return trial(z, dz, s)[0];
+}
+
+void FieldWorkspace::read_trial(double* out) {
+  SP_HIP(hipStreamSynchronize(stream_));
+  SP_HIP(hipMemcpy(out, B_.advanced, sizeof(cplx) * P_.C * P_.N, hipMemcpyDeviceToHost));
+}
+
+void FieldWorkspace::read_state(double* out) {
+  if (!loaded_) throw Error(kState, "no state is loaded");
+  SP_HIP(hipStreamSynchronize(stream_));
+  SP_HIP(hipMemcpy(out, B_.y, sizeof(cplx) * P_.C * P_.N, hipMemcpyDeviceToHost));
+}
+
+void FieldWorkspace::export_state(const double* weight, bool optics, double* out) {
+  read_state(out);
+  const long n = static_cast<long>(P_.C) * P_.N;
+  for (long i = 0; i < n; ++i) {
+    double re = out[2 * i], im = out[2 * i + 1];
+    if (weight) {
+      const double s = std::sqrt(weight[i]);
+      re = re * s;
+      im = im * s;
+    }
+    if (optics) {
+      im = -im;
+      if ((i % P_.N) & 1) {
+        re = -re;
+        im = -im;
+      }
+    }
+    out[2 * i] = re;
+    out[2 * i + 1] = im;
+  }
+}
+
+void FieldWorkspace::CallHost::record(double end, double dz, double norm, bool acc) {
+  if (rows) {
+    rows->push_back(end);
+    rows->push_back(dz);
+    rows->push_back(norm);
+    rows->push_back(acc ? 1.0 : 0.0);
+  }
+  attempts += 1;
+  accepted += acc ? 1 : 0;
+}
+
+bool FieldWorkspace::advance(double target, long work_limit, std::vector<double>& rows,
+                             sp_march_result* r) {
+  if (!loaded_) throw Error(kState, "load a state before advancing");
+  host_.rows = &rows;
+  host_.limit = work_limit;
+  host_.attempts = 0;
+  host_.accepted = 0;
+  bool done = false;
+  try {
+    done = march_->march_to(target) && march::reached(march_->z, target);
+  } catch (...) {
+    host_.rows = nullptr;
+    throw;
+  }
+  host_.rows = nullptr;
+  z_ = march_->z;
+  r->z = march_->z;
+  r->proposal = march_->proposal;
+  r->done = done ? 1 : 0;
+  r->attempts = host_.attempts;
+  r->accepted = host_.accepted;
+  r->evaluations = evaluations_;
+  return done;
+}
+
+sp_counters FieldWorkspace::counters() const {
+  sp_counters c{};
+  c.trials = trials_;
+  c.evaluations = evaluations_;
+  c.accepted = accepted_;
+  c.graph_launches = graph_launches_;
+  c.device_bytes =
+      static_cast<long>(device_bytes_ + inv_plan_.bytes() + fwd_plan_.bytes());
+  return c;
+}
+
+}  // namespace sp

@@ -1,0 +1,97 @@
+// The modal field-grid workspace under the exponential Adams march: the history's rows
+// resident in HBM (the reference's resident engine, native/src/march/adams_march.cpp,
+// and ExponentialHistory, src/solprop/backends/numpy_ref/multistep.py:806-1131), the
+// trial captured as one HIP graph, the decisions on the host (adams_march.hpp).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "adams_march.hpp"
+#include "device.hpp"
+#include "fft.hpp"
+#include "field.hpp"
+#include "propagator.hpp"
+#include "scalars.hpp"
+
+namespace sp {
+
+class FieldWorkspace final : public Propagator, private adams::Engine {
+ public:
+  explicit FieldWorkspace(const sp_problem_desc& desc);
+  ~FieldWorkspace() override;
+  FieldWorkspace(const FieldWorkspace&) = delete;
+  FieldWorkspace& operator=(const FieldWorkspace&) = delete;
+
+  void load(double z, const double* field) override;
+  double probe_trial(double z, double dz) override;
+  void read_trial(double* out) override;
+  void read_state(double* out) override;
+  void export_state(const double* weight, bool optics, double* out) override;
+  bool advance(double target, long work_limit, std::vector<double>& rows,
+               sp_march_result* r) override;
+  sp_counters counters() const override;
+  void set_profile(bool on) override { profile_ = on; }
+  double trial_us() const override { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
+  long timed() const override { return timed_; }
+
+ private:
+  // adams::Engine
+  void start(double z) override;
+  std::vector<double> trial(double z, double dz, const adams::Stencil& s) override;
+  void accept(double dz, int free, const adams::Stencil& s,
+              const std::vector<adams::Entry>& kept) override;
+
+  template <typename T>
+  T* alloc(std::size_t n);
+  template <typename T>
+  T* upload(const T* host, std::size_t n);
+  void fill_position(double z);
+  void enqueue_source(const cplx* spectrum, cplx* out);
+  void enqueue_trial();
+  void enqueue_start();
+  void run(int key);
+
+  DeviceProblem P_{};
+  PlasmaTable pt_{};
+  FieldBuffers B_{};
+  std::vector<void*> allocations_;
+  std::size_t device_bytes_ = 0;
+  hipStream_t stream_ = nullptr;
+  FftPlan inv_plan_, fwd_plan_;
+  cplx* scratch_ = nullptr;
+  AdamsRecord* rec_dev_ = nullptr;
+  AdamsRecord* rec_host_ = nullptr;
+  double* partials_host_ = nullptr;
+  int n_blocks_ = 0;
+  std::map<int, hipGraphExec_t> graphs_;
+  bool use_graphs_ = true;
+  // Host scalar paths.
+  std::vector<ScalarPath> phi_paths_, rate_paths_, pref_paths_;
+  ScalarPath density_, preion_;
+  double rtol_ = 1e-6, atol_ = 0.0;
+  bool compensated_ = false;
+  // The march, and the host it reports to: rows and the attempt limit of one call.
+  struct CallHost final : adams::Host {
+    std::vector<double>* rows = nullptr;
+    long limit = -1, attempts = 0, accepted = 0;
+    void record(double end, double dz, double norm, bool acc) override;
+    bool halt() override { return limit >= 0 && attempts >= limit; }
+  };
+  adams::Policy policy_;
+  CallHost host_;
+  std::unique_ptr<adams::March> march_;
+  std::vector<std::vector<double>> slot_rates_;  // D(z_j) weights by store slot
+  std::vector<double> held_rate_end_;
+  double z_ = 0.0, dz_init_ = 0.0;
+  bool loaded_ = false;
+  // Counters and timing.
+  long trials_ = 0, evaluations_ = 0, accepted_ = 0, graph_launches_ = 0;
+  bool profile_ = false;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  double time_total_ms_ = 0.0;
+  long timed_ = 0;
+};
+
+}  // namespace sp

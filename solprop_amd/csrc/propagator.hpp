@@ -1,0 +1,31 @@
+// What the C ABI asks of a workspace, whichever route and march it runs: the
+// envelope route under the embedded Runge-Kutta march (workspace.hpp) or the modal
+// field route under the exponential Adams march (field_ws.hpp).
+#pragma once
+
+#include <vector>
+
+extern "C" {
+#include "../../include/solprop_mi355x.h"
+}
+
+namespace sp {
+
+class Propagator {
+ public:
+  virtual ~Propagator() = default;
+  virtual void load(double z, const double* field) = 0;
+  // One trial from the committed state (a probe): the scaled norm.
+  virtual double probe_trial(double z, double dz) = 0;
+  virtual void read_trial(double* out) = 0;
+  virtual void read_state(double* out) = 0;
+  virtual void export_state(const double* weight, bool optics, double* out) = 0;
+  virtual bool advance(double target, long work_limit, std::vector<double>& rows,
+                       sp_march_result* r) = 0;
+  virtual sp_counters counters() const = 0;
+  virtual void set_profile(bool on) = 0;
+  virtual double trial_us() const = 0;
+  virtual long timed() const = 0;
+};
+
+}  // namespace sp

@@ -42,6 +42,38 @@ double ScalarPath::operator()(double z) const {
 
 double stage_position(double z, double c, double dz) { return z + c * dz; }
 
+void lagrange_basis(const double* points, int n, double upper, bool integrate, double* rows,
+                    double* weights) {
+  std::vector<double> coefficients(n), product(n);
+  if (!integrate)
+    for (int i = 0; i < n * n; ++i) rows[i] = 0.0;
+  for (int index = 0; index < n; ++index) {
+    const double node = points[index];
+    int length = 1;
+    coefficients[0] = 1.0;
+    double denominator = 1.0;
+    for (int other = 0; other < n; ++other) {
+      if (other == index) continue;
+      const double root = points[other];
+      product[0] = 0.0;
+      for (int power = 0; power < length; ++power) product[power + 1] = coefficients[power];
+      for (int power = 0; power < length; ++power) product[power] -= root * coefficients[power];
+      length += 1;
+      for (int power = 0; power < length; ++power) coefficients[power] = product[power];
+      denominator *= node - root;
+    }
+    if (integrate) {
+      double integral = 0.0;
+      for (int power = length - 1; power >= 0; --power)
+        integral = (integral + coefficients[power] / (power + 1)) * upper;
+      weights[index] = integral / denominator;
+    } else {
+      for (int power = 0; power < length; ++power)
+        rows[index * n + power] = coefficients[power] / denominator;
+    }
+  }
+}
+
 double path_increment(const ScalarPath& path, double z0, double z1) {
   return path(z1) - path(z0);
 }

@@ -25,4 +25,11 @@ struct ScalarPath {
 double stage_position(double z, double c, double dz);
 double path_increment(const ScalarPath& path, double z0, double z1);
 
+// The Lagrange basis on n distinct points, as the reference forms it
+// (src/solprop/stepper/multistep.py:515-564): each numerator built from its own roots,
+// ascending; its monomial coefficients (`rows`, n x n, when !integrate) or its integrals
+// from 0 to `upper` by Horner's rule (`weights`, n, when integrate).
+void lagrange_basis(const double* points, int n, double upper, bool integrate, double* rows,
+                    double* weights);
+
 }  // namespace sp

@@ -31,8 +31,8 @@ T* Workspace::device_copy(const T* host, std::size_t n) {
 
 Workspace::Workspace(const sp_problem_desc& d) {
   // Refusals first, before anything is allocated.
-  SP_CHECK(d.kind == 0, "this build runs the envelope route through the workspace; the "
-                        "field route runs through its own entry points");
+  SP_CHECK(d.kind == 0, "the Runge-Kutta workspace runs the envelope route");
+  SP_CHECK(d.method == 0, "the envelope route marches with the embedded Runge-Kutta pair");
   SP_CHECK(d.n_to >= 16 && (d.n_to & (d.n_to - 1)) == 0, "n_to must be a power of two >= 16");
   SP_CHECK(d.n_t >= 4 && d.n_to % d.n_t == 0, "n_to must be a multiple of n_t");
   SP_CHECK(d.n_omega == d.n_t, "an envelope grid has n_omega = n_t bins");

@@ -11,30 +11,33 @@
 #include "device.hpp"
 #include "fft.hpp"
 #include "march.hpp"
+#include "propagator.hpp"
 #include "scalars.hpp"
 
 namespace sp {
 
-class Workspace {
+class Workspace final : public Propagator {
  public:
   explicit Workspace(const sp_problem_desc& desc);
-  ~Workspace();
+  ~Workspace() override;
   Workspace(const Workspace&) = delete;
   Workspace& operator=(const Workspace&) = delete;
 
-  void load(double z, const double* field);
+  void load(double z, const double* field) override;
   // One trial from the committed state at z; every stage evaluated when `full`.
   double trial(double z, double dz, bool full);
+  double probe_trial(double z, double dz) override { return trial(z, dz, true); }
   void accept(double dz);
-  void read_state(double* out);
-  void read_trial(double* out);
-  void export_state(const double* weight, bool optics, double* out);
-  bool advance(double target, long work_limit, std::vector<double>& rows, sp_march_result* r);
+  void read_state(double* out) override;
+  void read_trial(double* out) override;
+  void export_state(const double* weight, bool optics, double* out) override;
+  bool advance(double target, long work_limit, std::vector<double>& rows,
+               sp_march_result* r) override;
 
-  void set_profile(bool on) { profile_ = on; }
-  double trial_us() const { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
-  long timed() const { return timed_; }
-  sp_counters counters() const;
+  void set_profile(bool on) override { profile_ = on; }
+  double trial_us() const override { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
+  long timed() const override { return timed_; }
+  sp_counters counters() const override;
 
  private:
   enum Stage0 { kEvaluate = 0, kCopyRecycled = 1, kKeep = 2 };

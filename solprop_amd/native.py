@@ -140,6 +140,8 @@ def lower(problem: Problem, settings: Settings | None = None) -> tuple[_lib.Prob
     d.phi_terms = k.c128(problem.phi_terms)
     for i, p in enumerate(problem.phi_paths):
         d.phi_paths[i] = k.path(p)
+    for i, p in enumerate(problem.phi_rate_paths):
+        d.phi_rate_paths[i] = k.path(p)
     d.n_prefactor = len(problem.prefactor_paths)
     d.prefactor_terms = k.c128(problem.prefactor_terms)
     for i, p in enumerate(problem.prefactor_paths):
@@ -195,6 +197,11 @@ def lower(problem: Problem, settings: Settings | None = None) -> tuple[_lib.Prob
     d.dz_max = problem.z_end if s.dz_max is None else s.dz_max
     d.stretch = s.stretch
     d.dz_init = 1e-4 * problem.z_end if s.dz_init is None else s.dz_init
+    d.method = 1 if problem.method == "adams" else 0
+    d.adams_order = problem.adams_order
+    d.adams_variable_order = int(problem.adams_variable_order)
+    d.adams_equal_approach = 1
+    d.adams_safety, d.adams_max_growth, d.adams_min_factor = 0.8, 2.0, 0.2
     return d, k
 
 

@@ -62,9 +62,11 @@ class ScalarPath:
         return ScalarPath(1, 0.0)
 
     @staticmethod
-    def chebyshev(function, z0: float, z1: float, intervals: int = 8, degree: int = 24):
-        """Interpolate ``function`` on ``intervals`` equal pieces of ``[z0, z1]``."""
-        edges = np.linspace(z0, z1, intervals + 1)
+    def chebyshev(function, z0: float, z1: float, intervals: int = 8, degree: int = 24,
+                  edges=None):
+        """Interpolate ``function`` on ``intervals`` equal pieces of ``[z0, z1]``
+        (or on the given ``edges``)."""
+        edges = np.linspace(z0, z1, intervals + 1) if edges is None else np.asarray(edges)
         rows = []
         for a, b in zip(edges[:-1], edges[1:], strict=True):
             rows.append(
@@ -74,6 +76,19 @@ class ScalarPath:
                 )
             )
         return ScalarPath(2, 0.0, np.ascontiguousarray(edges), np.ascontiguousarray(rows))
+
+    def derivative(self) -> "ScalarPath":
+        """The path's derivative in z: the rate of a cumulative operator term."""
+        if self.kind == 0:
+            return ScalarPath.constant(0.0)
+        if self.kind == 1:
+            return ScalarPath.constant(1.0)
+        rows = []
+        for j in range(len(self.edges) - 1):
+            a, b = float(self.edges[j]), float(self.edges[j + 1])
+            d = np.polynomial.chebyshev.chebder(self.coefficients[j]) * (2.0 / (b - a))
+            rows.append(d)
+        return ScalarPath(2, 0.0, self.edges.copy(), np.ascontiguousarray(rows))
 
     def __call__(self, z: float) -> float:
         """Evaluate exactly as the library does (same operations, same order)."""
@@ -216,11 +231,31 @@ class Problem:
     noise_model: str = "none"  # "none", "input" or "modified"
     rtol: float = 1e-6
     atol: float = 0.0
+    method: str = "rk"  # "rk" (embedded Runge-Kutta) or "adams" (exponential Adams)
+    adams_order: int = 5
+    adams_variable_order: bool = True
     description: dict = field(default_factory=dict)
 
     @property
     def n_omega(self) -> int:
         return int(self.crop_index.size)
+
+    @property
+    def phi_rate_paths(self) -> tuple:
+        """``dW_t/dz``: the operator's rate ``D(z) = Σ_t W_t'(z) Φ_t`` (cached)."""
+        cached = self.__dict__.get("_rate_paths")
+        if cached is None or cached[0] is not self.phi_paths:
+            cached = (self.phi_paths, tuple(p.derivative() for p in self.phi_paths))
+            self.__dict__["_rate_paths"] = cached
+        return cached[1]
+
+    def phi_rate(self, z: float) -> np.ndarray:
+        """``D(z)`` in the state's layout (``translate/linop.py`` ``phi_rate``)."""
+        w = [p(z) for p in self.phi_rate_paths]
+        total = np.zeros(self.phi_terms.shape[1:], dtype=np.complex128)
+        for weight, term in zip(w, self.phi_terms, strict=True):
+            total = total + weight * term
+        return total.reshape(self.shape)
 
     @property
     def shape(self) -> tuple[int, int, int]:
@@ -423,5 +458,239 @@ def gnlse_problem(
             "length_m": length,
             "rtol": rtol,
             "noise_seed": noise_seed,
+        },
+    )
+
+
+# ------------------------------------------------------------ gas-filled capillary
+
+HE_SELLMEIER = ((4977.77e-8, 28.54e-6), (1856.94e-8, 7.76e-3))
+"""Helium's refractivity at 273.15 K and 1 bar, ``n - 1 = Σ B λ²/(λ² - C)`` with λ in
+µm (A. Börzsönyi et al., Appl. Opt. 47, 4856 (2008))."""
+HE_IONISATION_EV = 24.587387
+HE_N2_PER_BAR = 3.4e-25  # m^2/W at 1 bar (the Kerr index of helium; Wahlstrand et al.)
+
+
+def adk_rate(field, ip_ev: float, z: int = 1):
+    """Tunnelling rate of an s-state by the ADK formula, 1/s, for a field in V/m.
+
+    M. V. Ammosov, N. B. Delone and V. P. Krainov, Sov. Phys. JETP 64, 1191 (1986),
+    with l = m = 0: in atomic units ``W = C² Ip (2 F0/F)^(2n*-1) exp(-2 F0/(3F))``,
+    ``F0 = (2 Ip)^(3/2)``, ``n* = Z / sqrt(2 Ip)``, ``C² = 2^(2n*) / (n* Γ(n*+1) Γ(n*))``.
+    """
+    from math import gamma
+
+    hartree_ev = 27.211386245988
+    au_field = 5.14220674763e11
+    au_time = 2.4188843265857e-17
+    ip = ip_ev / hartree_ev
+    n_star = z / math.sqrt(2.0 * ip)
+    c2 = 2.0 ** (2 * n_star) / (n_star * gamma(n_star + 1.0) * gamma(n_star))
+    f0 = (2.0 * ip) ** 1.5
+    f = np.abs(np.asarray(field, dtype=float)) / au_field
+    with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+        w = c2 * ip * (2.0 * f0 / f) ** (2 * n_star - 1) * np.exp(-2.0 * f0 / (3.0 * f))
+    return np.where(f > 0, w / au_time, 0.0)
+
+
+def rate_table(rate, field_min: float, field_max: float, n_intervals: int = 512):
+    """A cubic spline of ``log W`` in ``log |E|`` on uniform knots, as the reference's
+    ``translate/ionisation.py:95`` RateTable holds it (coefficients highest first)."""
+    from scipy.interpolate import CubicSpline
+
+    log_min, log_max = math.log(field_min), math.log(field_max)
+    knots = np.linspace(log_min, log_max, n_intervals + 1)
+    log_rate = np.log(rate(np.exp(knots)))
+    spline = CubicSpline(knots, log_rate)
+    coefficients = np.ascontiguousarray(spline.c)  # (4, n_intervals), highest first
+    return PlasmaGroup(
+        charge_mass_ratio=E_CHARGE**2 / M_E,
+        ionisation_potential=HE_IONISATION_EV * E_CHARGE,
+        log_field_min=log_min,
+        d_log_field=(log_max - log_min) / n_intervals,
+        field_min=field_min,
+        field_max=field_max,
+        rate_max=float(rate(np.array([field_max]))[0]),
+        rate_coefficients=coefficients,
+    )
+
+
+def capillary_problem(
+    *,
+    n_t: int = 2**14,
+    window: float = 400e-15,
+    radius: float = 125e-6,
+    length: float = 1.5,
+    n_modes: int = 4,
+    n_nodes: int = 128,
+    p0: float = 0.8,
+    p1: float = 0.0,
+    wavelength: float = 800e-9,
+    fwhm: float = 7.5e-15,
+    energy: float = 275e-6,
+    band=(100e-9, 4e-6),
+    temperature: float = 294.0,
+    rtol: float = 1e-6,
+    plasma: bool = True,
+    kerr: bool = True,
+    cladding_index: float = 1.45,
+    time_collar: float = 0.06,
+) -> Problem:
+    """Gas-filled capillary on a real field grid: HE_1m modes, a pressure gradient.
+
+    Modes of a hollow capillary in Marcatili's model (E. A. J. Marcatili and R. A.
+    Schmeltzer, Bell Syst. Tech. J. 43, 1783 (1964)): profile ``J0(u_m r/a)``,
+    ``β_m = k n_gas - u_m² / (2 k a²)``, loss ``α_m = (u_m/2π)² λ²/a³ (ν²+1)/√(ν²-1)``.
+    The state is the modal spectrum (sqrt(W)); the synthesis samples power-normalised
+    profiles at an n_nodes-point Gauss-Legendre rule on [0, a]; the projection carries
+    the rule's weights times ``2π r``; the prefactor is ``-i ω/4`` (internal
+    convention). Helium's refractivity is linear in the density, so the operator is
+    ``Φ(z) = z Φ_0 + (∫ρ dz) Φ_1`` with the steady-flow profile
+    ``p(z) = sqrt(p0² + (z/L)(p1² - p0²))``. Kerr: ``P = c ρ E³`` (real form);
+    tunnelling ionisation by the ADK rate tabulated as a log-log cubic spline.
+    """
+    from scipy.special import jn_zeros, j0, j1
+
+    n_to = n_t
+    dt = window / n_t
+    n = n_t // 2 + 1
+    omega = 2 * np.pi * np.arange(n) / (n_t * dt)
+    c = C_LIGHT
+    w_lo = 2 * np.pi * c / band[1]
+    w_hi = 2 * np.pi * c / band[0]
+    active = (omega > 0.5 * w_lo) & (omega < 1.1 * w_hi)
+    norm_mask = active.copy()
+    # Spectral collar: one inside the band, raised-cosine to zero at the active edges.
+    omega_window = np.zeros(n)
+    inside = (omega >= w_lo) & (omega <= w_hi)
+    omega_window[inside] = 1.0
+    lo = (omega > 0.5 * w_lo) & (omega < w_lo)
+    omega_window[lo] = 0.5 - 0.5 * np.cos(np.pi * (omega[lo] - 0.5 * w_lo) / (0.5 * w_lo))
+    hi = (omega > w_hi) & (omega < 1.1 * w_hi)
+    omega_window[hi] = 0.5 + 0.5 * np.cos(np.pi * (omega[hi] - w_hi) / (0.1 * w_hi))
+    # Temporal collar and its derivative.
+    t = (np.arange(n_to) - n_to // 2) * dt
+    width = time_collar * window
+    edge = 0.5 * window - width
+    g = np.ones(n_to)
+    gp = np.zeros(n_to)
+    a = np.abs(t) > edge
+    s = (np.abs(t[a]) - edge) / width
+    g[a] = np.where(s < 1, 0.5 + 0.5 * np.cos(np.pi * s), 0.0)
+    gp[a] = np.where(s < 1, -0.5 * np.pi / width * np.sin(np.pi * s) * np.sign(t[a]), 0.0)
+    # Modes.
+    u = jn_zeros(0, n_modes)
+    k = omega / c
+    safe = np.where(omega > 0, omega, 1.0)
+    lam = 2 * np.pi * c / safe
+    nu = cladding_index
+    beta_vac = np.zeros((n_modes, n))
+    alpha = np.zeros((n_modes, n))
+    for m in range(n_modes):
+        beta_vac[m] = np.where(omega > 0, k - u[m] ** 2 / (2 * np.where(k > 0, k, 1.0) * radius**2), 0.0)
+        alpha[m] = np.where(
+            omega > 0, (u[m] / (2 * np.pi)) ** 2 * lam**2 / radius**3 * (nu**2 + 1) / np.sqrt(nu**2 - 1), 0.0
+        )
+    # Refractivity per unit density (m^3): (n - 1)/ρ_ref at 273.15 K, 1 bar.
+    lam_um = lam * 1e6
+    refr = np.zeros(n)
+    for b_coef, c_coef in HE_SELLMEIER:
+        refr += np.where(omega > 0, b_coef * lam_um**2 / (lam_um**2 - c_coef), 0.0)
+    rho_ref = 1e5 / (KB * 273.15)
+    beta_rho = k * refr / rho_ref  # dβ/dρ
+    v_frame = c  # the frame moves at the speed of light
+    phi0 = np.zeros((n_modes, n), dtype=np.complex128)
+    phi1 = np.zeros((n_modes, n), dtype=np.complex128)
+    for m in range(n_modes):
+        phi0[m] = np.where(active, -1j * (beta_vac[m] - omega / v_frame) - 0.5 * alpha[m], 0.0)
+        phi1[m] = np.where(active, -1j * beta_rho, 0.0)
+    phi_terms = np.ascontiguousarray(np.stack([phi0, phi1]))
+    rho_unit = 1e5 / (KB * temperature)  # density per bar
+
+    def pressure(z):
+        return np.sqrt(np.maximum(p0**2 + (z / length) * (p1**2 - p0**2), 0.0))
+
+    def density(z):
+        return rho_unit * pressure(z)
+
+    def cumulative(z):
+        # ∫_0^z p dz' for p = sqrt(p0² + s (p1² - p0²)), s = z/L.
+        d = p1**2 - p0**2
+        if abs(d) < 1e-300:
+            return rho_unit * p0 * z
+        return rho_unit * (2 * length / (3 * d)) * (
+            (p0**2 + (z / length) * d) ** 1.5 - p0**3
+        )
+
+    # Geometric refinement towards the end, where sqrt(1 - z/L) is steepest.
+    fractions = 1.0 - 0.5 ** np.arange(0, 14)
+    edges = np.unique(np.concatenate([np.linspace(0, 0.5, 5), fractions, [1.0]])) * length
+    w_path = ScalarPath.chebyshev(lambda z: cumulative(z), 0.0, length, degree=24, edges=edges)
+    rho_path = ScalarPath.chebyshev(lambda z: density(z), 0.0, length, degree=24, edges=edges)
+    # Synthesis and projection on a Gauss-Legendre rule in r.
+    x, wq = np.polynomial.legendre.leggauss(n_nodes)
+    r = 0.5 * radius * (x + 1.0)
+    wr = 0.5 * radius * wq * 2 * np.pi * r
+    synthesis = np.zeros((n_nodes, n_modes))
+    for m in range(n_modes):
+        norm = math.sqrt(2.0 / (EPS0 * c)) / (math.sqrt(math.pi) * radius * abs(j1(u[m])))
+        synthesis[:, m] = norm * j0(u[m] * r / radius)
+    projection = np.ascontiguousarray((synthesis * wr[:, None]).T)
+    prefactor = np.zeros((1, n_modes, n), dtype=np.complex128)
+    for m in range(n_modes):
+        prefactor[0, m] = np.where(active, -1j * omega / 4.0, 0.0)
+    # Launch: a Gaussian at the carrier in HE11, cycle-averaged power P(t).
+    omega0 = 2 * np.pi * c / wavelength
+    p_peak = energy / (fwhm * math.sqrt(math.pi / (4 * math.log(2))))
+    power = p_peak * np.exp(-4 * math.log(2) * t**2 / fwhm**2)
+    a_t = np.sqrt(2 * power) * np.cos(omega0 * t)
+    field0 = np.zeros((n_modes, n), dtype=np.complex128)
+    field0[0] = np.fft.rfft(a_t) * active
+    kerr_group = None
+    if kerr:
+        chi3_per_rho = (4.0 / 3.0) * HE_N2_PER_BAR * EPS0 * c / (1e5 / (KB * 273.15))
+        kerr_group = KerrGroup(EPS0 * chi3_per_rho, "real")
+    plasma_group = None
+    if plasma:
+        # Below 3 GV/m helium's rate is under 1e-150 /s: the table is exactly zero there.
+        plasma_group = rate_table(lambda f: adk_rate(f, HE_IONISATION_EV), 3e9, 2e12)
+    return Problem(
+        kind="field",
+        n_t=n_t,
+        n_to=n_to,
+        n_modes=n_modes,
+        n_nodes=n_nodes,
+        crop_index=np.arange(n, dtype=np.int64),
+        active_mask=active,
+        norm_mask=norm_mask,
+        omega_window=omega_window,
+        omega=omega,
+        time_window=g,
+        time_window_derivative=gp,
+        dt_fine=dt,
+        z_end=length,
+        phi_terms=phi_terms,
+        phi_paths=(ScalarPath.position(), w_path),
+        prefactor_terms=np.ascontiguousarray(prefactor),
+        prefactor_paths=(ScalarPath.constant(1.0),),
+        synthesis=np.ascontiguousarray(synthesis),
+        projection=projection,
+        density=rho_path,
+        preionisation=ScalarPath.constant(0.0),
+        kerr=kerr_group,
+        raman=None,
+        plasma=plasma_group,
+        field0=np.ascontiguousarray(field0),
+        noise0=None,
+        noise_model="none",
+        rtol=rtol,
+        method="adams",
+        description={
+            "route": "capillary UPPE, modal field grid",
+            "n_t": n_t, "n_omega": n, "window_s": window, "radius_m": radius,
+            "length_m": length, "modes": n_modes, "nodes": n_nodes,
+            "pressure_bar": [p0, p1], "gas": "He", "wavelength_m": wavelength,
+            "fwhm_s": fwhm, "energy_J": energy, "rtol": rtol,
+            "stepper": "exponential Adams, variable order <= 5",
         },
     )

@@ -1,0 +1,119 @@
+"""GPU parity of the modal field route under the exponential Adams march, through the C ABI.
+
+A small gas-filled capillary (2048 samples, 32 radial nodes, 4 modes, helium with a
+pressure gradient, Kerr and tunnelling ionisation) against the oracle's
+``ExponentialEngine`` and ``march``.
+
+Tolerances (class R, fixed before the run): the field passes through two FP64 real
+transforms, a node synthesis, the plasma scans (a fixed-order tile scan against numpy's
+sequential cumsum) and the projection, so the corrected field is held to 1e-11
+relative L2. The scaled norm is a defect of order rtol times the field, so the field's
+rounding reaches it amplified by about 1/rtol: 1e-8 relative. A short adaptive march
+must make the same accept/reject decisions, its steps within 1e-7 relative, and end
+within 1e-8 relative L2.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import adams, rk
+from solprop_amd.native import Settings, Workspace
+from solprop_amd.problem import capillary_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def small(**kw):
+    args = dict(n_t=2048, n_nodes=32, length=0.05)
+    args.update(kw)
+    return capillary_problem(**args)
+
+
+CASES = {
+    "kerr_plasma": {},
+    "kerr_only": {"plasma": False},
+    "plasma_only": {"kerr": False},
+    "constant_pressure": {"p1": 0.8},
+}
+
+
+def oracle_first_trial(problem, dz):
+    policy = adams.adams_policy_for(problem)
+    engine = adams.ExponentialEngine(problem, rk.StageSource(problem), policy.order + 2)
+    engine.state = problem.field0.reshape(problem.shape).copy()
+    engine.start(0.0)
+    st = adams.stencil(policy, 0.0, dz, [0.0], [0], 1, last_rejected=False)
+    norms = engine.trial(0.0, dz, st)
+    return engine.held[2], norms
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_first_trial_matches_oracle(case):
+    print(f"field probe trial {case}", flush=True)
+    problem = small(**CASES[case])
+    dz = 2e-4
+    advanced, norms = oracle_first_trial(problem, dz)
+    ws = Workspace(problem)
+    probe = ws.probe(0.0, problem.field0)
+    got = probe.trial(0.0, dz)
+    field = probe.field()
+    ws.close()
+    assert rel_l2(field, advanced.reshape(field.shape)) < 1e-11
+    assert abs(got - norms[0]) / norms[0] < 1e-8
+
+
+@pytest.mark.parametrize("case,proposal", [("kerr_plasma", None), ("constant_pressure", None),
+                                           ("kerr_plasma", 2e-3)])
+def test_march_matches_oracle_decisions(case, proposal):
+    print(f"field march {case} proposal={proposal}", flush=True)
+    problem = small(**CASES[case])
+    y0 = problem.field0.reshape(problem.shape)
+    expected, records = adams.march(problem, y0.copy(), proposal=proposal, max_attempts=40)
+    if proposal is not None:
+        assert not all(r[3] for r in records)  # the large first step is rejected
+    ws = Workspace(problem, Settings(dz_init=proposal))
+    session = ws.start(0.0, problem.field0)
+    batch = session.advance(problem.z_end, work_limit=len(records))
+    rows = batch.attempts
+    state = session.state()
+    ws.close()
+    assert len(rows) == len(records)
+    assert [bool(r[3]) for r in rows] == [bool(r[3]) for r in records]
+    steps = np.array([r[1] for r in records])
+    assert np.allclose(rows[:, 1], steps, rtol=1e-7, atol=0)
+    assert rel_l2(state, expected.reshape(state.shape)) < 1e-8
+
+
+def test_field_repeats_agree_to_the_bit():
+    print("field determinism", flush=True)
+    problem = small(length=0.01)
+    outs = []
+    for _ in range(2):
+        ws = Workspace(problem)
+        session = ws.start(0.0, problem.field0)
+        batch = session.advance(problem.z_end, work_limit=12)
+        outs.append((batch.attempts.copy(), session.state()))
+        ws.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_field_graphs_and_direct_launches_agree():
+    import os
+
+    print("field graph vs direct", flush=True)
+    problem = small(length=0.01)
+    results = []
+    for flag in ("1", "0"):
+        os.environ["SOLPROP_MI355X_GRAPH"] = flag
+        ws = Workspace(problem)
+        session = ws.start(0.0, problem.field0)
+        session.advance(problem.z_end, work_limit=12)
+        results.append(session.state())
+        ws.close()
+    os.environ.pop("SOLPROP_MI355X_GRAPH", None)
+    assert np.array_equal(results[0], results[1])
