@@ -1,14 +1,21 @@
 """Propagation throughput on one MI355X: accepted steps per second of the native march.
 
-``python bench.py --gpus 1 --steps K --warmup W`` builds the workload's problem from a
-fixed seed, loads it into the device workspace, takes W accepted steps untimed, then
-times exactly K accepted steps of the adaptive march (every trial, rejected ones
+``python bench.py --gpus 1 --steps K --warmup W`` builds the workload's problem from
+fixed parameters, loads it into the device workspace, takes W accepted steps untimed,
+then times exactly K accepted steps of the adaptive march (every trial, rejected ones
 included, is inside the timed region) with ``torch.cuda.synchronize`` on both sides,
 and prints one JSON line. The inputs are resident in HBM before the timed region.
 
-Beside it: the roofline figure of the dominant unit (algorithmic bytes per trial over
-its HIP-event duration on the workspace's stream) and the CPU oracle timed on a bounded
-sample of the same workload on this host's cores.
+The default workload is the gas-filled capillary: 4 coupled HE_1m modes of a 125 um
+bore, 1.5 m of helium at 0.8 -> 0 bar, a real field of 2^14 samples over 400 fs (8193
+bins), a 7.5 fs 800 nm 275 uJ Gaussian, a 4-mode x 128-node radial quadrature, Kerr and
+tunnelling ionisation, the exponential Adams march at rtol 1e-6. ``--workload envelope``
+runs the single-mode GNLSE fibre case (2^13 bins, DOPRI5) instead.
+
+Beside the rate: the roofline figure of the trial's dominant kernel (its algorithmic
+bytes over its average duration, from HIP events recorded inside the captured trial on
+the workspace's stream during the timed region), every kernel's duration, and the CPU
+oracle timed on a bounded sample of the same workload on this host's cores.
 """
 
 from __future__ import annotations
@@ -24,22 +31,66 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 
 
+def capillary_bytes(problem, order):
+    """Algorithmic HBM bytes of each kernel of one Adams trial at order `order`.
+
+    Each array the kernel must read or write once, at its stored width: complex
+    (C, N) rows of 16 bytes per element, real (C, T) time rows of 8, the (Q, C)
+    synthesis and (C, Q) projection tables, the rate-table lookups from cache. W =
+    order + 1 moments; the corrector reads W - 1 stored source rows and the predictor
+    `order` of them; a varying operator adds the transported states of the rows read.
+    """
+    C, N, T, Q = problem.n_modes, problem.n_omega, problem.n_to, problem.n_nodes
+    n_phi = len(problem.phi_paths)
+    W = order + 1
+    cn = 16 * C * N
+    ct = 8 * C * T  # one real time row per mode = C * T/2 complex
+    tiles = T // 128
+    nb = (Q + 63) // 64
+    table = 8 * Q * C
+    comp = len(problem.phi_paths) > 1
+    return {
+        "adams_predict": cn * (n_phi + 1 + order + (order if comp else 0))
+        + cn * (W + 2 + (order if comp else 0)),
+        "field_scatter": cn + 8 * T + ct,
+        "inverse_fft": 2 * ct,
+        "plasma_exposure_tiles": ct + table + 8 * Q * tiles,
+        "plasma_tile_prefix": 2 * 8 * Q * tiles,
+        "plasma_current_tiles": ct + table + 8 * Q * tiles * 2,
+        "plasma_tile_prefix_current": 2 * 8 * Q * tiles,
+        "node_response": ct + 2 * 8 * T + 2 * table + 2 * 8 * Q * tiles + nb * 3 * ct,
+        "stream_reduce": nb * 3 * ct + 3 * ct,
+        "forward_fft": 2 * 3 * ct,
+        "field_finish": 3 * ct + 16 * C * N * len(problem.prefactor_paths) + cn,
+        "adams_correct": cn * (W + 1 + (W - 1) + 2) + cn,
+    }
+
+
+def workload_capillary():
+    from solprop_amd.problem import capillary_problem
+
+    problem = capillary_problem()
+    config = {
+        "workload": "gas-filled capillary UPPE, 4 HE_1m modes, 125 um bore, 1.5 m helium "
+                    "0.8->0 bar, real field 2^14 samples (8193 bins) over 400 fs, 7.5 fs "
+                    "800 nm 275 uJ Gaussian, 4x128 radial quadrature, Kerr + tunnelling "
+                    "plasma, exponential Adams (order <= 5) rtol 1e-6",
+        "n_t": problem.n_t, "n_modes": problem.n_modes, "n_nodes": problem.n_nodes,
+        "stepper": "exponential Adams PEC, variable order", "parallelism": "single GPU",
+    }
+    return problem, config
+
+
 def workload_envelope():
     from solprop_amd.problem import gnlse_problem
 
-    problem = gnlse_problem()  # n_t = 2^13 over 10 ps, n_to = 2^14, sech N=4, Raman
-    n, t = problem.n_omega, problem.n_to
-    # Pass-level algorithmic bytes of one source evaluation and of one accepted
-    # DOPRI5 step (6 evaluations with FSAL + the accept pass): 232 N + 296 T per
-    # evaluation, 96 N for the accept.
-    per_eval = 232 * n + 296 * t
-    per_step = 6 * per_eval + 96 * n
+    problem = gnlse_problem()
     config = {"workload": "envelope GNLSE, N=4 sech 50 fs 1550 nm, 2^13 bins over 10 ps, "
                           "n_to=2^14, Kerr + damped-oscillator Raman (f_R=0.18), shot noise "
                           "seed 1, DOPRI5 rtol 1e-6, 1 m",
-              "n_t": problem.n_t, "n_to": t, "n_modes": 1, "stepper": "DOPRI5 (FSAL)",
-              "parallelism": "single GPU"}
-    return problem, per_step, per_eval, config
+              "n_t": problem.n_t, "n_to": problem.n_to, "n_modes": 1,
+              "stepper": "DOPRI5 (FSAL)", "parallelism": "single GPU"}
+    return problem, config
 
 
 def run_gpu(args):
@@ -47,58 +98,79 @@ def run_gpu(args):
 
     from solprop_amd.native import Workspace
 
-    problem, bytes_per_step, bytes_per_eval, config = workload_envelope()
+    if args.workload == "capillary":
+        problem, config = workload_capillary()
+    else:
+        problem, config = workload_envelope()
     ws = Workspace(problem)
     session = ws.start(0.0, problem.field0)
-    accepted = 0
-    while accepted < args.warmup:
-        batch = session.advance(problem.z_end, work_limit=1)
-        accepted += int(batch.attempts[:, 3].sum())
+
+    def take(count):
+        accepted = attempts = 0
+        done = False
+        while accepted < count and not done:
+            batch = session.advance(problem.z_end, work_limit=count - accepted)
+            accepted += int(batch.attempts[:, 3].sum())
+            attempts += len(batch.attempts)
+            done = batch.done
+        return accepted, attempts
+
+    take(args.warmup)
     torch.cuda.synchronize()
     ws.profile(True)
     c0 = ws.counters()
+    z0 = session.z
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    accepted = 0
-    attempts = 0
-    while accepted < args.steps:
-        batch = session.advance(problem.z_end, work_limit=1)
-        accepted += int(batch.attempts[:, 3].sum())
-        attempts += len(batch.attempts)
-        if batch.done:
-            break
+    steps, attempts = take(args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     c1 = ws.counters()
     trial_us, trials = ws.timing()
-    evals = c1["evaluations"] - c0["evaluations"]
+    phases = ws.phases()
+    z1 = session.z
     ws.close()
-    steps = accepted
+    config["z_span_m"] = [z0, z1]
     value = steps / elapsed
-    # The trial graph is the unit timed by events: its bytes are the evaluations it
-    # made plus the accept pass.
-    bytes_per_trial = (evals / max(trials, 1)) * bytes_per_eval + 96 * problem.n_omega
-    achieved = bytes_per_trial / (trial_us * 1e-6) / 1e9 if trial_us > 0 else None
-    roofline = {
-        "unit": "GB/s",
-        "bound": "hbm",
-        "kernel": "trial graph (all stage kernels of one DOPRI5 trial)",
-        "achieved": achieved,
-        "peak": HBM_PEAK_GBS,
-        "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
-        "traffic": None,
-        "bytes_per_launch": bytes_per_trial,
-        "avg_launch_us": trial_us,
-    }
-    return value, elapsed * 1e3 / max(steps, 1), steps, attempts, config, roofline
+    if phases:
+        algorithmic = capillary_bytes(problem, problem.adams_order)
+        kernels = {
+            name: {"avg_us": us, "bytes": algorithmic[name],
+                   "GB/s": algorithmic[name] / (us * 1e-6) / 1e9 if us > 0 else None}
+            for name, us in phases.items()
+        }
+        dominant = max(phases, key=phases.get)
+        achieved = kernels[dominant]["GB/s"]
+        roofline = {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
+            "traffic": None, "kernel": dominant,
+            "bytes_per_launch": algorithmic[dominant], "avg_launch_us": phases[dominant],
+            "trial_us": trial_us, "kernels": kernels,
+        }
+    else:
+        roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": None, "traffic": None, "kernel": "trial graph",
+                    "trial_us": trial_us}
+    extra = {"attempts": attempts, "trials_timed": trials,
+             "evaluations": c1["evaluations"] - c0["evaluations"]}
+    return value, elapsed * 1e3 / max(steps, 1), steps, config, roofline, extra, problem
 
 
-def cpu_baseline(seconds: float = 12.0):
-    """The oracle's march on this host, on the first steps of the same workload."""
-    from oracle import rk
-    from solprop_amd.problem import gnlse_problem
+def cpu_baseline(problem, seconds: float = 15.0):
+    """The oracle's march on this host, on the first steps of the same workload.
 
-    problem = gnlse_problem()
-    y0 = problem.field0.reshape(problem.shape)
+    One thread: scipy.fft runs one worker by default, and the BLAS pool is held to one
+    thread for the duration.
+    """
+    from threadpoolctl import threadpool_limits
+
+    with threadpool_limits(limits=1):
+        return _cpu_baseline(problem, seconds)
+
+
+def _cpu_baseline(problem, seconds):
+    y0 = problem.field0.reshape(problem.shape).copy()
     steps = [0]
     t0 = time.perf_counter()
 
@@ -108,34 +180,44 @@ def cpu_baseline(seconds: float = 12.0):
         if time.perf_counter() - t0 > seconds:
             raise StopIteration
 
-    try:
-        rk.march(problem, y0, on_attempt=on_attempt)
-    except StopIteration:
-        pass
+    if problem.method == "adams":
+        from oracle import adams
+
+        adams.march(problem, y0, on_attempt=on_attempt)
+        what = "exponential Adams march"
+    else:
+        from oracle import rk
+
+        try:
+            rk.march(problem, y0, on_attempt=on_attempt)
+        except StopIteration:
+            pass
+        what = "DOPRI5 march (every stage, no FSAL)"
     elapsed = time.perf_counter() - t0
     return {
         "value": steps[0] / elapsed,
-        "unit": "steps/s",
+        "unit": "accepted steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"the oracle's NumPy/SciPy march (every stage, no FSAL) from z=0 for "
-                  f"{elapsed:.1f} s: {steps[0]} accepted steps",
+        "sample": f"the oracle's NumPy/SciPy {what} from z=0 for {elapsed:.1f} s: "
+                  f"{steps[0]} accepted steps, one thread",
     }
 
 
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=400)
-    parser.add_argument("--warmup", type=int, default=20)
+    parser.add_argument("--steps", type=int, default=300)
+    parser.add_argument("--warmup", type=int, default=30)
+    parser.add_argument("--workload", choices=["capillary", "envelope"], default="capillary")
     parser.add_argument("--no-cpu", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1:
         print("bench.py runs one GPU: one propagation problem runs on one device, and this "
               "build measures the single-GPU path only (--gpus 1).", file=sys.stderr)
         sys.exit(2)
-    value, ms, steps, attempts, config, roofline = run_gpu(args)
-    baseline = None if args.no_cpu else cpu_baseline()
+    value, ms, steps, config, roofline, extra, problem = run_gpu(args)
+    baseline = None if args.no_cpu else cpu_baseline(problem)
     line = {
         "metric": "propagation steps/s per GPU",
         "value": value,
@@ -148,9 +230,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded builders; no material database)",
+        "data": "synthetic (parametric builders: Marcatili modes, ADK helium rate, "
+                "Gaussian input; no material database)",
         "config": config,
-        "attempts": attempts,
+        **extra,
         "roofline": roofline,
         "cpu_baseline": baseline,
     }

@@ -187,6 +187,21 @@ int sp_workspace_counters(sp_workspace* ws, sp_counters* out);
  * workspace's stream. Enabled by `sp_workspace_profile(ws, 1)`. */
 int sp_workspace_profile(sp_workspace* ws, int enable);
 int sp_workspace_timing(sp_workspace* ws, double* trial_us, long* trials);
+/* The same events split by kernel: `*count` phases (at most `capacity`), the average
+ * microseconds of each in `us`, their kernel names comma-separated in `names`. A route
+ * that times only whole trials reports no phases. */
+int sp_workspace_phases(sp_workspace* ws, double* us, int capacity, int* count, char* names,
+                        size_t names_len);
+
+/* The source (the stage derivative before any propagator) at a lab-frame field
+ * `lab` (complex (C, N)) at z, into `out` (complex (C, N)); when given, the
+ * time-domain intermediates: `streams` (3, C, T) real, the projected polarisation,
+ * gated current gJ and edge current g'J, and `modal` (C, T) real, the modal fields.
+ * The reference's engine-level testing entry points
+ * (native/src/bindings/testing.cpp:599-700, `Engines.stage`). Field route only. It
+ * leaves the march's history alone but overwrites the last trial's buffers. */
+int sp_testing_source(sp_workspace* ws, double z, const double* lab, double* out,
+                      double* streams, double* modal, char* msg, size_t msglen);
 
 /* -------------------------------------------------------------------- traces */
 

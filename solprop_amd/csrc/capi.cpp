@@ -134,6 +134,29 @@ int sp_workspace_timing(sp_workspace* ws, double* trial_us, long* trials) {
   return SP_OK;
 }
 
+int sp_testing_source(sp_workspace* ws, double z, const double* lab, double* out,
+                      double* streams, double* modal, char* msg, size_t msglen) {
+  if (!ws) return SP_VALUE;
+  return guarded(msg, msglen, [&] { ws->impl.source(z, lab, out, streams, modal); });
+}
+
+int sp_workspace_phases(sp_workspace* ws, double* us, int capacity, int* count, char* names,
+                        size_t names_len) {
+  if (!ws || !count || (capacity > 0 && !us)) return SP_VALUE;
+  const auto phases = ws->impl.phases();
+  std::string joined;
+  int n = 0;
+  for (const auto& ph : phases) {
+    if (n < capacity) us[n] = ph.second;
+    if (!joined.empty()) joined += ",";
+    joined += ph.first;
+    n += 1;
+  }
+  *count = n;
+  write_message(names, names_len, joined.c_str());
+  return n > capacity ? SP_VALUE : SP_OK;
+}
+
 int sp_rk_trace(const sp_rk_scenario* scenario, char* out, size_t outlen) {
   if (!scenario || !out) return SP_VALUE;
   std::string trace;

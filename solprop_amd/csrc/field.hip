@@ -490,26 +490,36 @@ void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuf
 }
 
 void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,
-                        FieldBuffers B, hipStream_t s) {
+                        FieldBuffers B, hipStream_t s, hipEvent_t* marks) {
   SP_CHECK(P.C <= kMaxModes, "the node passes project at most 8 modes");
   SP_CHECK(P.T % kScanTile == 0, "the fine grid must hold whole scan tiles");
   const int tiles = P.T / kScanTile;
   const int node_blocks = (P.Q + kNodeBlock - 1) / kNodeBlock;
   const dim3 grid(tiles, node_blocks);
+  auto mark = [&](int i) {
+    if (marks) SP_HIP(hipEventRecord(marks[i], s));
+  };
   if (P.has_plasma) {
     hipLaunchKernelGGL(plasma_exposure_tiles, grid, dim3(kScanTile), 0, s, P, pt, B);
     check();
+    mark(0);
     hipLaunchKernelGGL(plasma_tile_prefix, dim3(blocks_for(P.Q, 64)), dim3(64), 0, s, P.Q,
                        tiles, B.tile_x);
     check();
+    mark(1);
     hipLaunchKernelGGL(node_response<false>, grid, dim3(kScanTile), 0, s, P, R, pt, B);
     check();
+    mark(2);
     hipLaunchKernelGGL(plasma_tile_prefix, dim3(blocks_for(P.Q, 64)), dim3(64), 0, s, P.Q,
                        tiles, B.tile_f);
     check();
+    mark(3);
+  } else {
+    for (int i = 0; i < 4; ++i) mark(i);
   }
   hipLaunchKernelGGL(node_response<true>, grid, dim3(kScanTile), 0, s, P, R, pt, B);
   check();
+  mark(4);
   const std::size_t n = 3UL * P.C * P.T;
   hipLaunchKernelGGL(stream_reduce, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, P, B,
                      node_blocks);

@@ -71,8 +71,11 @@ void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuf
                           hipStream_t s);
 void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuffers B,
                           hipStream_t s);
+// `marks`, when given, holds five events recorded after the exposure tiles, its tile
+// prefix, the free-current tiles, its prefix and the node response (the last four
+// passes of a plain Kerr run record back to back).
 void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,
-                        FieldBuffers B, hipStream_t s);
+                        FieldBuffers B, hipStream_t s, hipEvent_t* marks = nullptr);
 void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                          cplx* out, hipStream_t s);
 void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,

@@ -155,8 +155,7 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   std::memset(rec_host_, 0, sizeof(AdamsRecord));
   SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&partials_host_),
                        sizeof(double) * (1 + kMaxScored) * n_blocks_));
-  SP_HIP(hipEventCreate(&ev0_));
-  SP_HIP(hipEventCreate(&ev1_));
+  for (hipEvent_t& e : marks_) SP_HIP(hipEventCreate(&e));
   slot_rates_.assign(S, std::vector<double>(kMaxTerms, 0.0));
 }
 
@@ -166,25 +165,46 @@ FieldWorkspace::~FieldWorkspace() {
   for (void* p : allocations_) (void)hipFree(p);
   if (rec_host_) (void)hipHostFree(rec_host_);
   if (partials_host_) (void)hipHostFree(partials_host_);
-  if (ev0_) (void)hipEventDestroy(ev0_);
-  if (ev1_) (void)hipEventDestroy(ev1_);
+  for (hipEvent_t e : marks_)
+    if (e) (void)hipEventDestroy(e);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 // ------------------------------------------------------------- device sequences
 
+namespace {
+// The kernels (or transform plans) of one trial, in launch order.
+constexpr const char* kPhaseNames[FieldWorkspace::kPhases] = {
+    "adams_predict",         "field_scatter",  "inverse_fft",        "plasma_exposure_tiles",
+    "plasma_tile_prefix",    "plasma_current_tiles", "plasma_tile_prefix_current",
+    "node_response",         "stream_reduce",  "forward_fft",        "field_finish",
+    "adams_correct"};
+}  // namespace
+
+void FieldWorkspace::mark(int i) {
+  if (marking_) SP_HIP(hipEventRecord(marks_[i], stream_));
+}
+
 void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
   launch_field_scatter(P_, spectrum, B_, stream_);
+  mark(2);
   inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
-  launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
+  mark(3);
+  launch_node_passes(P_, rec_dev_, pt_, B_, stream_, marking_ ? marks_ + 4 : nullptr);
+  mark(9);
   fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
+  mark(10);
   launch_field_finish(P_, rec_dev_, B_, out, stream_);
+  mark(11);
 }
 
 void FieldWorkspace::enqueue_trial() {
+  mark(0);
   launch_adams_predict(P_, rec_dev_, B_, stream_);
+  mark(1);
   enqueue_source(B_.lab, B_.source);
   launch_adams_correct(P_, rec_dev_, B_, stream_);
+  mark(12);
 }
 
 void FieldWorkspace::enqueue_start() {
@@ -195,14 +215,16 @@ void FieldWorkspace::enqueue_start() {
 void FieldWorkspace::run(int key) {
   SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
                         stream_));
-  if (profile_ && key == 1) SP_HIP(hipEventRecord(ev0_, stream_));
+  // Event marks go into the trial only while profiling (a graph of their own).
+  marking_ = profile_ && key == 1;
+  const int graph_key = key + (marking_ ? 8 : 0);
   if (!use_graphs_) {
     if (key == 1)
       enqueue_trial();
     else
       enqueue_start();
   } else {
-    auto it = graphs_.find(key);
+    auto it = graphs_.find(graph_key);
     if (it == graphs_.end()) {
       hipGraph_t graph;
       SP_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
@@ -214,11 +236,18 @@ void FieldWorkspace::run(int key) {
       hipGraphExec_t exec;
       SP_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
       SP_HIP(hipGraphDestroy(graph));
-      it = graphs_.emplace(key, exec).first;
+      it = graphs_.emplace(graph_key, exec).first;
     }
     SP_HIP(hipGraphLaunch(it->second, stream_));
     graph_launches_ += 1;
   }
+}
+
+std::vector<std::pair<const char*, double>> FieldWorkspace::phases() const {
+  std::vector<std::pair<const char*, double>> out;
+  for (int i = 0; i < kPhases; ++i)
+    out.emplace_back(kPhaseNames[i], timed_ ? phase_ms_[i] * 1e3 / timed_ : 0.0);
+  return out;
 }
 
 // -------------------------------------------------------------- host scalars
@@ -304,11 +333,14 @@ std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Sten
   SP_HIP(hipMemcpyAsync(partials_host_, B_.partials,
                         sizeof(double) * (1 + kMaxScored) * n_blocks_, hipMemcpyDeviceToHost,
                         stream_));
-  if (profile_) SP_HIP(hipEventRecord(ev1_, stream_));
   SP_HIP(hipStreamSynchronize(stream_));
-  if (profile_) {
+  if (marking_) {
     float ms = 0.0f;
-    SP_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    for (int i = 0; i < kPhases; ++i) {
+      SP_HIP(hipEventElapsedTime(&ms, marks_[i], marks_[i + 1]));
+      phase_ms_[i] += ms;
+    }
+    SP_HIP(hipEventElapsedTime(&ms, marks_[0], marks_[kPhases]));
     time_total_ms_ += ms;
     timed_ += 1;
   }
@@ -432,6 +464,30 @@ bool FieldWorkspace::advance(double target, long work_limit, std::vector<double>
   r->accepted = host_.accepted;
   r->evaluations = evaluations_;
   return done;
+}
+
+void FieldWorkspace::source(double z, const double* lab, double* out, double* streams,
+                            double* modal) {
+  SP_CHECK(lab && out, "the source needs a field and an output");
+  const std::size_t CN = static_cast<std::size_t>(P_.C) * P_.N;
+  const std::size_t CT = static_cast<std::size_t>(P_.C) * P_.T;
+  SP_HIP(hipStreamSynchronize(stream_));
+  SP_HIP(hipMemcpy(B_.lab, lab, CN * sizeof(cplx), hipMemcpyHostToDevice));
+  fill_position(z);
+  SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
+                        stream_));
+  marking_ = false;
+  launch_field_scatter(P_, B_.lab, B_, stream_);
+  inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
+  launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
+  SP_HIP(hipStreamSynchronize(stream_));
+  if (modal) SP_HIP(hipMemcpy(modal, B_.modal_t, CT * sizeof(double), hipMemcpyDeviceToHost));
+  if (streams)
+    SP_HIP(hipMemcpy(streams, B_.streams, 3 * CT * sizeof(double), hipMemcpyDeviceToHost));
+  fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
+  launch_field_finish(P_, rec_dev_, B_, B_.source, stream_);
+  SP_HIP(hipStreamSynchronize(stream_));
+  SP_HIP(hipMemcpy(out, B_.source, CN * sizeof(cplx), hipMemcpyDeviceToHost));
 }
 
 sp_counters FieldWorkspace::counters() const {

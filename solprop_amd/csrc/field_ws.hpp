@@ -32,7 +32,11 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   bool advance(double target, long work_limit, std::vector<double>& rows,
                sp_march_result* r) override;
   sp_counters counters() const override;
+  void source(double z, const double* lab, double* out, double* streams,
+              double* modal) override;
   void set_profile(bool on) override { profile_ = on; }
+  std::vector<std::pair<const char*, double>> phases() const override;
+  static constexpr int kPhases = 12;
   double trial_us() const override { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
   long timed() const override { return timed_; }
 
@@ -52,6 +56,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   void enqueue_trial();
   void enqueue_start();
   void run(int key);
+  void mark(int i);
 
   DeviceProblem P_{};
   PlasmaTable pt_{};
@@ -88,8 +93,9 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   bool loaded_ = false;
   // Counters and timing.
   long trials_ = 0, evaluations_ = 0, accepted_ = 0, graph_launches_ = 0;
-  bool profile_ = false;
-  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  bool profile_ = false, marking_ = false;
+  hipEvent_t marks_[kPhases + 1] = {};
+  double phase_ms_[kPhases] = {};
   double time_total_ms_ = 0.0;
   long timed_ = 0;
 };

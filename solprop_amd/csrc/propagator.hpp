@@ -3,6 +3,8 @@
 // field route under the exponential Adams march (field_ws.hpp).
 #pragma once
 
+#include <stdexcept>
+#include <utility>
 #include <vector>
 
 extern "C" {
@@ -26,6 +28,15 @@ class Propagator {
   virtual void set_profile(bool on) = 0;
   virtual double trial_us() const = 0;
   virtual long timed() const = 0;
+  // The source at a lab-frame field at z (complex (C, N)); with `streams` (3, C, T)
+  // and `modal` (C, T) given, the route's time-domain intermediates as well.
+  virtual void source(double z, const double* lab, double* out, double* streams,
+                      double* modal) {
+    (void)z; (void)lab; (void)out; (void)streams; (void)modal;
+    throw std::invalid_argument("source evaluation is built on the field route only");
+  }
+  // The average device time of each kernel (or transform) of a trial, microseconds.
+  virtual std::vector<std::pair<const char*, double>> phases() const { return {}; }
 };
 
 }  // namespace sp

@@ -294,6 +294,31 @@ class Workspace:
         self._lib.sp_workspace_timing(self._handle, C.byref(us), C.byref(n))
         return us.value, n.value
 
+    def source(self, z: float, lab: np.ndarray, intermediates: bool = False):
+        """The source at a lab-frame field (the engines' stage evaluation); with
+        `intermediates`, also the (3, C, T) time-domain streams and (C, T) modal fields."""
+        lab = np.ascontiguousarray(lab, dtype=np.complex128).reshape(self._shape)
+        out = np.zeros(self._shape, dtype=np.complex128)
+        C_, T = self.problem.n_modes, self.problem.n_to
+        streams = np.zeros((3, C_, T)) if intermediates else None
+        modal = np.zeros((C_, T)) if intermediates else None
+        msg = C.create_string_buffer(1024)
+        _check(self._lib.sp_testing_source(
+            self._handle, float(z), _ptr(lab.view(np.float64)), _ptr(out.view(np.float64)),
+            None if streams is None else _ptr(streams),
+            None if modal is None else _ptr(modal), msg, 1024), msg)
+        return (out, streams, modal) if intermediates else out
+
+    def phases(self) -> dict:
+        """Average device microseconds of each kernel of a trial, in launch order."""
+        us = np.zeros(32)
+        count = C.c_int()
+        names = C.create_string_buffer(2048)
+        _check(self._lib.sp_workspace_phases(self._handle, _ptr(us), len(us), C.byref(count),
+                                             names, 2048), names)
+        labels = names.value.decode().split(",") if count.value else []
+        return dict(zip(labels, us[: count.value].tolist(), strict=True))
+
 
 class Probe:
     """One trial from a loaded state, committing nothing (``_probe.py:33-70``)."""

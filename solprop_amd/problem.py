@@ -332,6 +332,24 @@ def _collar(n: int, width: int) -> np.ndarray:
     return w
 
 
+def _band(omega: np.ndarray, band) -> tuple[np.ndarray, np.ndarray]:
+    """The active mask and spectral collar of a wavelength band (``grid.py:1359-1387``).
+
+    Active: ``ω_min/2 < ω < 1.1 ω_max``; the collar is one inside ``[ω_min, ω_max]`` and
+    a raised cosine to zero at the active edges.
+    """
+    w_lo = 2 * np.pi * C_LIGHT / band[1]
+    w_hi = 2 * np.pi * C_LIGHT / band[0]
+    active = (omega > 0.5 * w_lo) & (omega < 1.1 * w_hi)
+    window = np.zeros(omega.shape)
+    window[(omega >= w_lo) & (omega <= w_hi)] = 1.0
+    lo = (omega > 0.5 * w_lo) & (omega < w_lo)
+    window[lo] = 0.5 - 0.5 * np.cos(np.pi * (omega[lo] - 0.5 * w_lo) / (0.5 * w_lo))
+    hi = (omega > w_hi) & (omega < 1.1 * w_hi)
+    window[hi] = 0.5 + 0.5 * np.cos(np.pi * (omega[hi] - w_hi) / (0.1 * w_hi))
+    return active, window
+
+
 def gnlse_problem(
     *,
     n_t: int = 8192,
@@ -351,8 +369,13 @@ def gnlse_problem(
     noise_seed: int | None = 1,
     noise_model: str = "input",
     time_collar: float = 0.05,
+    band=(700e-9, 5e-6),
 ) -> Problem:
     """Single-mode envelope GNLSE: sech pulse at soliton order N, Kerr + Raman.
+
+    The physics lives on the active bins of the wavelength `band`: the dispersion, the
+    prefactor, the noise and the launch field are zero outside it, as the reference
+    evaluates them on ``active_mask`` only (``grid.py:1359``, ``noise.py:195-214``).
 
     In the internal (unnormalised numpy) convention the envelope bin ``k`` sits at
     ``ω_0 + Δω_k``; ``Φ = -i Σ_{n≥2} β_n Δω^n / n! · z``; the prefactor is
@@ -367,20 +390,14 @@ def gnlse_problem(
     omega0 = 2 * np.pi * C_LIGHT / wavelength
     dw = 2 * np.pi * np.fft.fftfreq(n_t, dt)
     omega = omega0 + dw
+    active, omega_window = _band(omega, band)
     beta_poly = np.zeros_like(dw)
     for order, b in enumerate(betas, start=2):
         beta_poly += b * dw**order / math.factorial(order)
-    phi = (-1j * beta_poly).reshape(1, 1, n_t)
+    phi = np.where(active, -1j * beta_poly, 0.0).reshape(1, 1, n_t)
     ratio = omega / omega0 if shock else np.ones_like(omega)
     crop_scale = n_t / n_to
-    prefactor = (-1j * gamma * ratio * crop_scale).reshape(1, 1, n_t)
-    # Active band: everything but a thin spectral margin; the collar rolls off there.
-    active = np.ones(n_t, dtype=bool)
-    omega_window = np.ones(n_t)
-    edge = int(0.02 * n_t)
-    order_idx = np.argsort(dw)
-    collar = _collar(n_t, edge)
-    omega_window[order_idx] = collar
+    prefactor = np.where(active, -1j * gamma * ratio * crop_scale, 0.0).reshape(1, 1, n_t)
     t_window = _collar(n_to, int(time_collar * n_to))
     t_deriv = np.gradient(t_window, dt_fine)
     # Launch field: sech at soliton order N, P0 = N^2 |β2| / (γ T0^2).
@@ -388,13 +405,14 @@ def gnlse_problem(
     p0 = soliton_order**2 * abs(betas[0]) / (gamma * t0**2)
     t = (np.arange(n_t) - n_t // 2) * dt
     a_t = np.sqrt(p0) / np.cosh(t / t0)
-    spectrum = np.fft.fft(a_t).reshape(1, n_t).astype(np.complex128)
+    spectrum = (np.fft.fft(a_t) * active).reshape(1, n_t).astype(np.complex128)
     noise = None
     if noise_seed is not None and noise_model != "none":
         rng = np.random.default_rng(noise_seed)
         # One photon per bin: |A_k| = sqrt(hbar ω N / dt) (src/solprop/noise.py:195-246).
         phase = rng.uniform(0.0, 2.0 * np.pi, size=(1, 1, n_t))
-        amplitude = np.sqrt(HBAR * omega * n_t / dt)
+        amplitude = np.zeros(n_t)
+        amplitude[active] = np.sqrt(HBAR * omega[active] * n_t / dt)
         noise = (amplitude * np.exp(1j * phase)).reshape(1, n_t)
         if noise_model == "input":
             spectrum = spectrum + noise
@@ -458,6 +476,7 @@ def gnlse_problem(
             "length_m": length,
             "rtol": rtol,
             "noise_seed": noise_seed,
+            "band_m": tuple(band),
         },
     )
 
@@ -556,18 +575,8 @@ def capillary_problem(
     n = n_t // 2 + 1
     omega = 2 * np.pi * np.arange(n) / (n_t * dt)
     c = C_LIGHT
-    w_lo = 2 * np.pi * c / band[1]
-    w_hi = 2 * np.pi * c / band[0]
-    active = (omega > 0.5 * w_lo) & (omega < 1.1 * w_hi)
+    active, omega_window = _band(omega, band)
     norm_mask = active.copy()
-    # Spectral collar: one inside the band, raised-cosine to zero at the active edges.
-    omega_window = np.zeros(n)
-    inside = (omega >= w_lo) & (omega <= w_hi)
-    omega_window[inside] = 1.0
-    lo = (omega > 0.5 * w_lo) & (omega < w_lo)
-    omega_window[lo] = 0.5 - 0.5 * np.cos(np.pi * (omega[lo] - 0.5 * w_lo) / (0.5 * w_lo))
-    hi = (omega > w_hi) & (omega < 1.1 * w_hi)
-    omega_window[hi] = 0.5 + 0.5 * np.cos(np.pi * (omega[hi] - w_hi) / (0.1 * w_hi))
     # Temporal collar and its derivative.
     t = (np.arange(n_to) - n_to // 2) * dt
     width = time_collar * window

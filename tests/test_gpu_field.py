@@ -41,6 +41,48 @@ CASES = {
 }
 
 
+def oracle_source_stages(problem, lab, z):
+    """The oracle's modal fields, its three projected time streams and its source."""
+    from oracle import passes as P
+
+    src = rk.StageSource(problem)
+    crop = problem.crop_index.astype(np.intp)
+    modal = P.to_time(lab, crop, problem.n_to, problem.kind)
+    field_t = P.synthesise(modal, problem.synthesis, 1.0, 1)
+    s = P.nonlinear_sources(
+        field_t, kerr=src.kerr, raman=None, raman_thg=False, plasma=src.plasma,
+        t_window_fine=problem.time_window, kind=problem.kind, density=problem.density(z),
+        preionisation=problem.preionisation(z), dt=problem.dt_fine)
+    streams = np.zeros((3, problem.n_modes, problem.n_to))
+    streams[0] = P.project(s.polarisation, problem.projection, 1.0, 1)[:, 0]
+    if s.current is not None:
+        streams[1] = P.project(problem.time_window * s.current, problem.projection, 1.0,
+                               1)[:, 0]
+        streams[2] = P.project(problem.time_window_derivative * s.current,
+                               problem.projection, 1.0, 1)[:, 0]
+    return modal[:, 0], streams, src.evaluate(lab, z, 1.0)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_source_stages_match_oracle(case):
+    print(f"field source stages {case}", flush=True)
+    problem = small(**CASES[case])
+    lab = problem.field0.reshape(problem.shape)
+    z = 2e-4
+    modal, streams, source = oracle_source_stages(problem, lab, z)
+    ws = Workspace(problem)
+    ws.load(0.0, problem.field0)
+    got, got_streams, got_modal = ws.source(z, lab, intermediates=True)
+    ws.close()
+    errors = {"modal": rel_l2(got_modal, modal), "source": rel_l2(got, source.reshape(got.shape))}
+    for i, name in enumerate(("polarisation", "gated current", "edge current")):
+        if np.any(streams[i]):
+            errors[name] = rel_l2(got_streams[i], streams[i])
+    print(errors, flush=True)
+    assert errors["modal"] < 1e-14
+    assert all(e < 1e-12 for e in errors.values()), errors
+
+
 def oracle_first_trial(problem, dz):
     policy = adams.adams_policy_for(problem)
     engine = adams.ExponentialEngine(problem, rk.StageSource(problem), policy.order + 2)

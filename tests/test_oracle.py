@@ -151,6 +151,34 @@ def test_fundamental_soliton_is_stationary():
     assert np.max(np.abs(a2 - a0)) / np.max(a0) > 0.1
 
 
+def test_held_spectrum_is_the_gated_current_derivative_over_iw_squared():
+    """``DFT[gJ]/(iω) - DFT[g'J]/(iω)^2 = DFT[g J']/(iω)^2`` on band-limited signals.
+
+    Multiplying by ``(iω)^2``: ``iω DFT[gJ] - DFT[g'J] = DFT[(gJ)' - g'J] = DFT[g J']``,
+    exact for periodic signals whose products stay below Nyquist (the spectral
+    derivative is then the derivative). A sign or term error fails this.
+    """
+    n, dt = 256, 1e-15
+    t = np.arange(n) * dt
+    w = 2 * np.pi * np.fft.rfftfreq(n, dt)
+    rng = np.random.default_rng(3)
+    J = np.zeros(n)
+    dJ = np.zeros(n)
+    for k in range(3, 11):
+        a, ph = rng.normal(), rng.uniform(0, 2 * np.pi)
+        J += a * np.cos(w[k] * t + ph)
+        dJ -= a * w[k] * np.sin(w[k] * t + ph)
+    g = 1.0 + 0.3 * np.cos(w[2] * t)
+    dg = -0.3 * w[2] * np.sin(w[2] * t)
+    active = w > 0
+    crop = np.arange(w.size)
+    held = P.held_plasma_spectrum(g * J, dg * J, w, crop, active)
+    expected = np.zeros_like(held)
+    expected[active] = np.fft.rfft(g * dJ)[active] / (1j * w[active]) ** 2
+    assert np.all(held[~active] == 0)
+    assert np.max(np.abs(held - expected)) < 1e-12 * np.max(np.abs(expected))
+
+
 # ----------------------------------------------------------- the Luna witness
 
 W = np.load(GOLDEN / "luna_plasma_witness.npz")
