@@ -203,6 +203,13 @@ int sp_workspace_phases(sp_workspace* ws, double* us, int capacity, int* count, 
 int sp_testing_source(sp_workspace* ws, double z, const double* lab, double* out,
                       double* streams, double* modal, char* msg, size_t msglen);
 
+/* A named device buffer of the field route ("y", "lab", "source", "advanced", "flow",
+ * "store", "moments", "states", "carried") copied into `out` (at most `count`
+ * doubles); `*size` receives the buffer's length in doubles. The reference's
+ * `Engines.field` / `Engines.stage` readbacks (native/src/bindings/testing.cpp:601-602). */
+int sp_testing_buffer(sp_workspace* ws, const char* name, double* out, long count, long* size,
+                      char* msg, size_t msglen);
+
 /* -------------------------------------------------------------------- traces */
 
 /* A scripted Runge-Kutta march (tests/backends/rk_traces.py:63-141 Scenario). */

@@ -146,6 +146,8 @@ SIGNATURES = {
     "sp_workspace_timing": (_I, [C.c_void_p, _PD, C.POINTER(C.c_long)]),
     "sp_testing_source": (_I, [C.c_void_p, C.c_double, _PD, _PD, _PD, _PD, C.c_char_p,
                                C.c_size_t]),
+    "sp_testing_buffer": (_I, [C.c_void_p, C.c_char_p, _PD, C.c_long, C.POINTER(C.c_long),
+                               C.c_char_p, C.c_size_t]),
     "sp_workspace_phases": (_I, [C.c_void_p, _PD, _I, C.POINTER(_I), C.c_char_p, C.c_size_t]),
     "sp_rk_trace": (_I, [C.POINTER(RKScenarioC), C.c_char_p, C.c_size_t]),
     "sp_adams_trace": (_I, [C.POINTER(AdamsScenarioC), C.c_char_p, C.c_size_t]),

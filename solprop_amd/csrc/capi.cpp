@@ -140,6 +140,12 @@ int sp_testing_source(sp_workspace* ws, double z, const double* lab, double* out
   return guarded(msg, msglen, [&] { ws->impl.source(z, lab, out, streams, modal); });
 }
 
+int sp_testing_buffer(sp_workspace* ws, const char* name, double* out, long count, long* size,
+                      char* msg, size_t msglen) {
+  if (!ws || !name || !size) return SP_VALUE;
+  return guarded(msg, msglen, [&] { *size = ws->impl.buffer(name, out, count); });
+}
+
 int sp_workspace_phases(sp_workspace* ws, double* us, int capacity, int* count, char* names,
                         size_t names_len) {
   if (!ws || !count || (capacity > 0 && !us)) return SP_VALUE;

@@ -102,8 +102,11 @@ __device__ inline cplx c2r_pre(cplx ya, cplx yb, cplx w) {
 }
 
 // The rate table (src/solprop/translate/ionisation.py:242-313): exactly zero below
-// field_min, held above field_max, exp(cubic in log|E|) between.
+// field_min, held above field_max, exp(cubic in log|E|) between. Evaluated without
+// contraction, in the table's own order of roundings: the rate's log-slope (tens to
+// hundreds near threshold) multiplies any rounding of the cubic.
 __device__ inline double rate_of(double magnitude, const PlasmaTable& pt, const double* c) {
+#pragma clang fp contract(off)
   if (magnitude > pt.field_max) return pt.rate_max;
   if (!(magnitude >= pt.field_min)) return 0.0;
   const double lf = log(magnitude);
@@ -114,6 +117,13 @@ __device__ inline double rate_of(double magnitude, const PlasmaTable& pt, const 
   const double v = ((c[index] * local + c[n + index]) * local + c[2 * n + index]) * local +
                    c[3 * n + index];
   return exp(v);
+}
+
+// The ionised fraction after an exposure x from f0, 1 - (1 - f0) e^{-x}, without
+// contraction (src/solprop/backends/numpy_ref/passes/plasma.py:104-191).
+__device__ inline double fraction_of(double f0, double x) {
+#pragma clang fp contract(off)
+  return 1.0 - (1.0 - f0) * exp(-x);
 }
 
 // Scan state of one workgroup of kScanTile threads (two waves).
@@ -226,8 +236,14 @@ __global__ void field_scatter(DeviceProblem P, const cplx* spectrum, cplx* modal
   const int c = static_cast<int>(idx / half), k = static_cast<int>(idx % half);
   const long row = static_cast<long>(c) * P.N;
   const int kb = half - k;
-  const cplx xa = k < P.N ? scale(P.inv_T, spectrum[row + k]) : make(0.0, 0.0);
-  const cplx xb = kb < P.N ? scale(P.inv_T, spectrum[row + kb]) : make(0.0, 0.0);
+  cplx xa = k < P.N ? scale(P.inv_T, spectrum[row + k]) : make(0.0, 0.0);
+  cplx xb = kb < P.N ? scale(P.inv_T, spectrum[row + kb]) : make(0.0, 0.0);
+  // A real transform's zero-frequency and Nyquist bins are real: their imaginary parts
+  // are discarded, as a c2r transform does (numpy.fft.irfft, FFTW, rocFFT).
+  if (k == 0) {
+    xa.y = 0.0;
+    xb.y = 0.0;
+  }
   modal_t[idx] = c2r_pre(xa, xb, roots_T[k]);
 }
 
@@ -307,8 +323,8 @@ __global__ void __launch_bounds__(kScanTile)
       const double off = B.tile_x[static_cast<long>(q) * tiles + tile];
       const double x = off + excl;
       const double x_next = off + incl;
-      f = 1.0 - (1.0 - f0) * exp(-x);
-      const double f_next = 1.0 - (1.0 - f0) * exp(-x_next);
+      f = fraction_of(f0, x);
+      const double f_next = fraction_of(f0, x_next);
       const double gval = (f * P.plasma_cmr) * e;
       const double g_next = (f_next * P.plasma_cmr) * e_next;
       const double jinc = t + 1 < T ? (dt * (g_next + gval)) / 2.0 : 0.0;

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 namespace sp {
 
@@ -215,10 +216,11 @@ void FieldWorkspace::enqueue_start() {
 void FieldWorkspace::run(int key) {
   SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
                         stream_));
-  // Event marks go into the trial only while profiling (a graph of their own).
+  // While profiling, the trial runs as direct launches with an event between kernels
+  // (events recorded by graph nodes cannot be timed); the kernels are the same.
   marking_ = profile_ && key == 1;
-  const int graph_key = key + (marking_ ? 8 : 0);
-  if (!use_graphs_) {
+  const int graph_key = key;
+  if (!use_graphs_ || marking_) {
     if (key == 1)
       enqueue_trial();
     else
@@ -488,6 +490,34 @@ void FieldWorkspace::source(double z, const double* lab, double* out, double* st
   launch_field_finish(P_, rec_dev_, B_, B_.source, stream_);
   SP_HIP(hipStreamSynchronize(stream_));
   SP_HIP(hipMemcpy(out, B_.source, CN * sizeof(cplx), hipMemcpyDeviceToHost));
+}
+
+long FieldWorkspace::buffer(const char* name, double* out, long count) {
+  SP_CHECK(name != nullptr, "a buffer name is required");
+  const long CN2 = 2L * P_.C * P_.N;
+  const long S = policy_.order + 2;
+  const struct {
+    const char* name;
+    const void* ptr;
+    long doubles;
+  } table[] = {
+      {"y", B_.y, CN2},           {"lab", B_.lab, CN2},
+      {"source", B_.source, CN2}, {"advanced", B_.advanced, CN2},
+      {"flow", B_.flow, CN2},     {"store", B_.store, S * CN2},
+      {"moments", B_.moments, S * CN2},
+      {"states", B_.states, B_.states ? S * CN2 : 0},
+      {"carried", B_.carried, B_.carried ? S * CN2 : 0},
+  };
+  for (const auto& row : table) {
+    if (std::strcmp(row.name, name) != 0) continue;
+    if (out && count > 0 && row.ptr) {
+      SP_HIP(hipStreamSynchronize(stream_));
+      SP_HIP(hipMemcpy(out, row.ptr, sizeof(double) * std::min(count, row.doubles),
+                       hipMemcpyDeviceToHost));
+    }
+    return row.doubles;
+  }
+  throw std::invalid_argument(std::string("no device buffer named ") + name);
 }
 
 sp_counters FieldWorkspace::counters() const {

@@ -34,6 +34,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   sp_counters counters() const override;
   void source(double z, const double* lab, double* out, double* streams,
               double* modal) override;
+  long buffer(const char* name, double* out, long count) override;
   void set_profile(bool on) override { profile_ = on; }
   std::vector<std::pair<const char*, double>> phases() const override;
   static constexpr int kPhases = 12;

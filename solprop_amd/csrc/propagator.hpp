@@ -35,6 +35,12 @@ class Propagator {
     (void)z; (void)lab; (void)out; (void)streams; (void)modal;
     throw std::invalid_argument("source evaluation is built on the field route only");
   }
+  // A named device buffer copied to the host (`count` doubles at most); the number of
+  // doubles it holds.
+  virtual long buffer(const char* name, double* out, long count) {
+    (void)name; (void)out; (void)count;
+    throw std::invalid_argument("buffer readback is built on the field route only");
+  }
   // The average device time of each kernel (or transform) of a trial, microseconds.
   virtual std::vector<std::pair<const char*, double>> phases() const { return {}; }
 };

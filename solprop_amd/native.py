@@ -309,6 +309,17 @@ class Workspace:
             None if modal is None else _ptr(modal), msg, 1024), msg)
         return (out, streams, modal) if intermediates else out
 
+    def buffer(self, name: str) -> np.ndarray:
+        """A named device buffer of the field route, as complex rows of (C, N)."""
+        size = C.c_long()
+        msg = C.create_string_buffer(1024)
+        _check(self._lib.sp_testing_buffer(self._handle, name.encode(), None, 0,
+                                           C.byref(size), msg, 1024), msg)
+        out = np.zeros(size.value)
+        _check(self._lib.sp_testing_buffer(self._handle, name.encode(), _ptr(out), size.value,
+                                           C.byref(size), msg, 1024), msg)
+        return out.view(np.complex128).reshape(-1, *self._shape)
+
     def phases(self) -> dict:
         """Average device microseconds of each kernel of a trial, in launch order."""
         us = np.zeros(32)

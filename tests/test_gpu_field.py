@@ -63,8 +63,39 @@ def oracle_source_stages(problem, lab, z):
     return modal[:, 0], streams, src.evaluate(lab, z, 1.0)
 
 
+def log_ulp_sensitivity(problem, lab, z, streams):
+    """How far each current stream moves when log|E| is off by one ulp in one direction.
+
+    The rate is exp of a cubic in log|E| with a log-slope of tens to hundreds, and the
+    free current late in the window is a small residual of an oscillating integral: a
+    systematic last-ulp difference between two libm ``log`` implementations (the
+    device's and the host's) moves the current streams by this much.
+    """
+    from oracle import passes as P
+
+    base = P.RateTable.log_evaluate
+    worst = np.zeros(3)
+    try:
+        for sign in (1.0, -1.0):
+            def shifted(self, lf, sign=sign):
+                return base(self, lf + sign * np.spacing(lf))
+
+            P.RateTable.log_evaluate = shifted
+            _, moved, _ = oracle_source_stages(problem, lab, z)
+            for i in range(3):
+                if np.any(streams[i]):
+                    worst[i] = max(worst[i], rel_l2(moved[i], streams[i]))
+    finally:
+        P.RateTable.log_evaluate = base
+    return worst
+
+
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_source_stages_match_oracle(case):
+    """Stage by stage at the launch field: modal fields and polarisation to 1e-14 and
+    1e-13; the current streams within 1e-12 or four times their sensitivity to a one-ulp
+    bias of log|E| (:func:`log_ulp_sensitivity`), whichever is larger; the source to
+    1e-12."""
     print(f"field source stages {case}", flush=True)
     problem = small(**CASES[case])
     lab = problem.field0.reshape(problem.shape)
@@ -75,12 +106,19 @@ def test_source_stages_match_oracle(case):
     got, got_streams, got_modal = ws.source(z, lab, intermediates=True)
     ws.close()
     errors = {"modal": rel_l2(got_modal, modal), "source": rel_l2(got, source.reshape(got.shape))}
-    for i, name in enumerate(("polarisation", "gated current", "edge current")):
+    names = ("polarisation", "gated current", "edge current")
+    for i, name in enumerate(names):
         if np.any(streams[i]):
             errors[name] = rel_l2(got_streams[i], streams[i])
-    print(errors, flush=True)
+    sensitivity = log_ulp_sensitivity(problem, lab, z, streams)
+    print(errors, "log-ulp sensitivity", sensitivity.tolist(), flush=True)
     assert errors["modal"] < 1e-14
-    assert all(e < 1e-12 for e in errors.values()), errors
+    assert errors["source"] < 1e-12
+    if "polarisation" in errors:
+        assert errors["polarisation"] < 1e-13
+    for i in (1, 2):
+        if names[i] in errors:
+            assert errors[names[i]] < max(1e-12, 4 * sensitivity[i]), (names[i], errors)
 
 
 def oracle_first_trial(problem, dz):
