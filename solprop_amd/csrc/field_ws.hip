@@ -49,6 +49,8 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
            "a required problem array is missing");
   for (int b = 0; b < N; ++b)
     SP_CHECK(d.crop_index[b] == b, "a field grid's coarse bins lead the fine spectrum");
+  SP_CHECK(!d.has_plasma || (d.rate_intervals >= 1 && d.rate_coefficients),
+           "the rate table is missing");
 
   for (int t = 0; t < d.n_phi; ++t) {
     phi_paths_.emplace_back();
@@ -79,9 +81,19 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   policy_.breakpoints = {0.0, d.z_end};
   dz_init_ = d.dz_init > 0.0 ? d.dz_init : 1e-4 * d.z_end;
 
-  SP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   const char* env = std::getenv("SOLPROP_MI355X_GRAPH");
   use_graphs_ = !(env && env[0] == '0');
+  try {
+    allocate(d);
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+void FieldWorkspace::allocate(const sp_problem_desc& d) {
+  const int C = d.n_modes, N = d.n_omega, T = d.n_to, Q = d.n_nodes;
+  SP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 
   P_.kind = 1;
   P_.C = C;
@@ -114,7 +126,6 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   P_.synthesis = upload(d.synthesis, static_cast<std::size_t>(Q) * C);
   P_.projection = upload(d.projection, static_cast<std::size_t>(Q) * C);
   if (d.has_plasma) {
-    SP_CHECK(d.rate_intervals >= 1 && d.rate_coefficients, "the rate table is missing");
     pt_.log_field_min = d.log_field_min;
     pt_.d_log_field = d.d_log_field;
     pt_.field_min = d.field_min;
@@ -160,15 +171,24 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   slot_rates_.assign(S, std::vector<double>(kMaxTerms, 0.0));
 }
 
-FieldWorkspace::~FieldWorkspace() {
+FieldWorkspace::~FieldWorkspace() { release(); }
+
+void FieldWorkspace::release() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   for (void* p : allocations_) (void)hipFree(p);
   if (rec_host_) (void)hipHostFree(rec_host_);
   if (partials_host_) (void)hipHostFree(partials_host_);
-  for (hipEvent_t e : marks_)
+  for (hipEvent_t& e : marks_) {
     if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
   if (stream_) (void)hipStreamDestroy(stream_);
+  graphs_.clear();
+  allocations_.clear();
+  rec_host_ = nullptr;
+  partials_host_ = nullptr;
+  stream_ = nullptr;
 }
 
 // ------------------------------------------------------------- device sequences

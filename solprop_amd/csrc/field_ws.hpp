@@ -48,6 +48,8 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   void accept(double dz, int free, const adams::Stencil& s,
               const std::vector<adams::Entry>& kept) override;
 
+  void allocate(const sp_problem_desc& d);
+  void release();
   template <typename T>
   T* alloc(std::size_t n);
   template <typename T>

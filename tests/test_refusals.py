@@ -1,0 +1,56 @@
+"""The boundary refuses problems outside what it runs, before touching the device.
+
+Refusals are the reference's ``BackendRefusal`` (``_core.pyi:23-28``,
+``backend.py:1165-1199``): raised at workspace construction, before any allocation.
+They run here without a GPU because the library checks the problem before its first
+HIP call.
+"""
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+from solprop_amd.native import BackendRefusal, Workspace
+from solprop_amd.problem import capillary_problem, gnlse_problem
+
+
+def refused(problem, match):
+    with pytest.raises(BackendRefusal, match=match):
+        Workspace(problem)
+
+
+def test_envelope_under_adams_is_refused():
+    refused(dataclasses.replace(gnlse_problem(n_t=64, window=1e-12), method="adams"),
+            "embedded Runge-Kutta")
+
+
+def test_field_route_under_rk_is_refused():
+    refused(dataclasses.replace(capillary_problem(n_t=256, n_nodes=4), method="rk"),
+            "exponential Adams")
+
+
+def test_band_running_integral_is_refused():
+    p = capillary_problem(n_t=256, n_nodes=4)
+    plasma = dataclasses.replace(p.plasma, taps=(0.5, 0.25))
+    refused(dataclasses.replace(p, plasma=plasma), "band running-integral")
+
+
+def test_too_many_modes_are_refused():
+    p = capillary_problem(n_t=256, n_nodes=4, n_modes=9)
+    refused(p, "1..8 modes")
+
+
+def test_short_field_grid_is_refused():
+    refused(capillary_problem(n_t=128, n_nodes=4), "power of two >= 256")
+
+
+def test_envelope_with_several_modes_is_refused():
+    p = gnlse_problem(n_t=64, window=1e-12)
+    two = dataclasses.replace(
+        p, n_modes=2, phi_terms=np.repeat(p.phi_terms, 2, axis=1),
+        prefactor_terms=np.repeat(p.prefactor_terms, 2, axis=1),
+        field0=np.repeat(p.field0, 2, axis=0), synthesis=np.ones((1, 2)),
+        projection=np.ones((2, 1)))
+    with pytest.raises((BackendRefusal, ValueError)):
+        Workspace(two)
