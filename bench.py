@@ -36,34 +36,35 @@ HBM_PEAK_GBS = 8000.0
 def capillary_bytes(problem, order):
     """Algorithmic HBM bytes of each kernel of one Adams trial at order `order`.
 
-    Each array the kernel must read or write once, at its stored width: complex
-    (C, N) rows of 16 bytes per element, real (C, T) time rows of 8, the (Q, C)
-    synthesis and (C, Q) projection tables, the rate-table lookups from cache. W =
-    order + 1 moments; the corrector reads W - 1 stored source rows and the predictor
-    `order` of them; a varying operator adds the transported states of the rows read.
+    Each array the kernel must read or write once, at its stored width: complex (C, N)
+    rows of 16 bytes per element, real (C, T) time rows and (Q, T) node rows of 8, the
+    scan tiles' sums, the (Q, C) synthesis and (C, Q) projection tables; the rate-table
+    lookups come from cache. W = order + 1 moments; the corrector reads W - 1 stored
+    source rows and the predictor `order` of them; a varying operator adds the
+    transported states of the rows read.
     """
     C, N, T, Q = problem.n_modes, problem.n_omega, problem.n_to, problem.n_nodes
     n_phi = len(problem.phi_paths)
     W = order + 1
     cn = 16 * C * N
     ct = 8 * C * T  # one real time row per mode = C * T/2 complex
-    tiles = T // 128
-    nb = (Q + 63) // 64
+    qt = 8 * Q * T
+    tiles = (T - 1 + 62) // 63
+    qk = 8 * Q * tiles
     table = 8 * Q * C
-    comp = len(problem.phi_paths) > 1
+    comp = n_phi > 1
     return {
         "adams_predict": cn * (n_phi + 1 + order + (order if comp else 0))
         + cn * (W + 2 + (order if comp else 0)),
         "field_scatter": cn + 8 * T + ct,
         "inverse_fft": 2 * ct,
-        "plasma_exposure_tiles": ct + table + 8 * Q * tiles,
-        "plasma_tile_prefix": 2 * 8 * Q * tiles,
-        "plasma_current_tiles": ct + table + 8 * Q * tiles * 2,
-        "plasma_tile_prefix_current": 2 * 8 * Q * tiles,
-        "node_response": ct + 2 * 8 * T + 2 * table + 2 * 8 * Q * tiles + nb * 3 * ct,
-        "stream_reduce": nb * 3 * ct + 3 * ct,
+        "plasma_rate_tiles": ct + table + qt + qk,
+        "plasma_tile_prefix": 2 * qk,
+        "plasma_current_tiles": ct + table + qt + 2 * qk,
+        "plasma_tile_prefix_current": 2 * qk,
+        "node_response": ct + 2 * table + qt + 2 * qk + 2 * 8 * T + 3 * ct,
         "forward_fft": 2 * 3 * ct,
-        "field_finish": 3 * ct + 16 * C * N * len(problem.prefactor_paths) + cn,
+        "field_finish": 3 * ct + cn * len(problem.prefactor_paths) + cn,
         "adams_correct": cn * (W + 1 + (W - 1) + 2) + cn,
     }
 

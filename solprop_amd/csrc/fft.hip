@@ -198,11 +198,12 @@ int ilog2(std::size_t n) {
   return m;
 }
 
-// Sequences per workgroup: aim for 1024 points, never more than the sequences there
-// are, and a divisor of their count.
+// Sequences per workgroup: aim for 256 points (one wavefront of radix-4 butterflies),
+// never more than the sequences there are, and a divisor of their count. Small
+// workgroups keep the short transforms of a few modes spread over many CUs.
 int pick_log2g(int log2l, std::size_t n_seq) {
   int g = 0;
-  while (g < 6 && (1 << (log2l + g + 1)) <= 1024 && (std::size_t{1} << (g + 1)) <= n_seq &&
+  while (g < 6 && (1 << (log2l + g + 1)) <= 256 && (std::size_t{1} << (g + 1)) <= n_seq &&
          n_seq % (std::size_t{1} << (g + 1)) == 0)
     ++g;
   return g;

@@ -29,13 +29,15 @@ __device__ inline cplx cdiv(cplx a, cplx b) {
 
 // phi_0..phi_order at one argument (src/solprop/backends/exponential.py:131-226):
 // scaled Taylor series, downward recurrence and doubling near zero; upward recurrence
-// from exp(x) beyond |x| = 3.
-__device__ void phi_functions(cplx x, int order, cplx* out) {
+// from exp(x) beyond |x| = 3. Every loop over the entries is unrolled to the largest
+// order with the runtime order as a guard, so that `out` stays in registers.
+__device__ void phi_functions(cplx x, int order, cplx (&out)[kMaxAdamsWidth + 1]) {
   const double magnitude = hypot(x.x, x.y);
   if (magnitude > 3.0) {
     out[0] = cexp(x);
-    for (int k = 1; k <= order; ++k)
-      out[k] = cdiv(sub(out[k - 1], make(kRecipFactorial[k - 1], 0.0)), x);
+#pragma unroll
+    for (int k = 1; k <= kMaxAdamsWidth; ++k)
+      if (k <= order) out[k] = cdiv(sub(out[k - 1], make(kRecipFactorial[k - 1], 0.0)), x);
     return;
   }
   int squarings = 0;
@@ -50,20 +52,29 @@ __device__ void phi_functions(cplx x, int order, cplx* out) {
     term = make(ty.x / d, ty.y / d);
     total = add(total, term);
   }
-  out[order] = total;
-  for (int k = order - 1; k >= 1; --k)
-    out[k] = add(make(kRecipFactorial[k], 0.0), mul(y, out[k + 1]));
+#pragma unroll
+  for (int k = 0; k <= kMaxAdamsWidth; ++k)
+    if (k == order) out[k] = total;
+#pragma unroll
+  for (int k = kMaxAdamsWidth - 1; k >= 1; --k)
+    if (k < order) out[k] = add(make(kRecipFactorial[k], 0.0), mul(y, out[k + 1]));
   out[0] = cexp(y);
   for (int level = 0; level < squarings; ++level) {
     cplx doubled[kMaxAdamsWidth + 1];
     doubled[0] = mul(out[0], out[0]);
-    for (int k = 1; k <= order; ++k) {
-      cplx t = mul(out[0], out[k]);
-      for (int j = 1; j <= k; ++j) t = add(t, scale(kRecipFactorial[k - j], out[j]));
-      const double p2 = exp2(static_cast<double>(k));
-      doubled[k] = make(t.x / p2, t.y / p2);
+#pragma unroll
+    for (int k = 1; k <= kMaxAdamsWidth; ++k) {
+      if (k <= order) {
+        cplx t = mul(out[0], out[k]);
+#pragma unroll
+        for (int j = 1; j <= k; ++j) t = add(t, scale(kRecipFactorial[k - j], out[j]));
+        const double p2 = exp2(static_cast<double>(k));
+        doubled[k] = make(t.x / p2, t.y / p2);
+      }
     }
-    for (int k = 0; k <= order; ++k) out[k] = doubled[k];
+#pragma unroll
+    for (int k = 0; k <= kMaxAdamsWidth; ++k)
+      if (k <= order) out[k] = doubled[k];
   }
 }
 
@@ -74,17 +85,26 @@ __device__ inline cplx weighted(const double* w, const cplx* terms, int n, long 
   return total;
 }
 
-// dz Σ_k M_k (Σ_j c_jk X_j) over n_k columns and n_j rows (coefficients [j][k]).
-__device__ inline cplx apply_moments(const cplx* M, const double (*c)[kMaxAdamsWidth],
-                                     const cplx* X, int n_j, int n_k, double dz) {
+// dz Σ_k M_k (Σ_j c_jk X_j) over n_k columns and n_j rows (coefficients [j][k]), in
+// increasing k and j, zero coefficients skipped; unrolled to the largest width.
+__device__ inline cplx apply_moments(const cplx (&M)[kMaxAdamsWidth],
+                                     const double (*c)[kMaxAdamsWidth],
+                                     const cplx (&X)[kMaxAdamsWidth], int n_j, int n_k,
+                                     double dz) {
   cplx total = make(0.0, 0.0);
-  for (int k = 0; k < n_k; ++k) {
-    cplx combined = make(0.0, 0.0);
-    for (int j = 0; j < n_j; ++j) {
-      const double w = c[j][k];
-      if (w != 0.0) combined = add(combined, scale(w, X[j]));
+#pragma unroll
+  for (int k = 0; k < kMaxAdamsWidth; ++k) {
+    if (k < n_k) {
+      cplx combined = make(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < kMaxAdamsWidth; ++j) {
+        if (j < n_j) {
+          const double w = c[j][k];
+          if (w != 0.0) combined = add(combined, scale(w, X[j]));
+        }
+      }
+      total = add(total, mul(M[k], combined));
     }
-    total = add(total, mul(M[k], combined));
   }
   return scale(dz, total);
 }
@@ -126,55 +146,14 @@ __device__ inline double fraction_of(double f0, double x) {
   return 1.0 - (1.0 - f0) * exp(-x);
 }
 
-// Scan state of one workgroup of kScanTile threads (two waves).
-struct ScanLds {
-  double waves[2];
-  double incl[kScanTile];
-  double edge[2][2];  // the first lane's values of each wave, for the neighbour exchange
-};
-
-// Inclusive scan in a fixed order (a shuffle ladder in each wave, then the wave
-// totals); the exclusive value is the previous thread's inclusive one, bit for bit.
-__device__ inline double tile_scan(double v, ScanLds& s, double* exclusive) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int d = 1; d < 64; d <<= 1) {
-    const double n = __shfl_up(v, d, 64);
+// Inclusive scan across one wavefront in a fixed order (a shuffle ladder), so that the
+// result is the same on every run.
+__device__ inline double wave_scan(double v, int lane) {
+  for (int d = 1; d < kWave; d <<= 1) {
+    const double n = __shfl_up(v, d, kWave);
     if (lane >= d) v = v + n;
   }
-  if (lane == 63) s.waves[wave] = v;
-  __syncthreads();
-  double offset = 0.0;
-  for (int w = 0; w < wave; ++w) offset = offset + s.waves[w];
-  const double incl = wave ? offset + v : v;
-  s.incl[threadIdx.x] = incl;
-  __syncthreads();
-  *exclusive = threadIdx.x ? s.incl[threadIdx.x - 1] : 0.0;
-  __syncthreads();
-  return incl;
-}
-
-// Two values of thread + 1 within the tile; the tile's last thread takes `tail_*`.
-__device__ inline void next_of(double a, double b, ScanLds& s, double tail_a, double tail_b,
-                               double* na, double* nb) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double xa = __shfl_down(a, 1, 64), xb = __shfl_down(b, 1, 64);
-  if (lane == 0) {
-    s.edge[wave][0] = a;
-    s.edge[wave][1] = b;
-  }
-  __syncthreads();
-  if (lane == 63) {
-    if (threadIdx.x == kScanTile - 1) {
-      xa = tail_a;
-      xb = tail_b;
-    } else {
-      xa = s.edge[wave + 1][0];
-      xb = s.edge[wave + 1][1];
-    }
-  }
-  __syncthreads();
-  *na = xa;
-  *nb = xb;
+  return v;
 }
 
 __device__ inline double node_field(const double* S, const double* a, int C, int T, int t) {
@@ -183,6 +162,39 @@ __device__ inline double node_field(const double* S, const double* a, int C, int
   return e;
 }
 
+// One node's exposure and free current over one scan tile (see field.hpp): the rate
+// read back, the exposure increments' scan plus the tile's offset, the fraction, the
+// free-current increments' scan. Lanes past the grid's end hold zeros.
+struct TileColumn {
+  double e, w, f;      // field, rate and fraction at this lane's sample
+  double j_incl;       // inclusive free-current scan within the tile
+  double j_excl;       // exclusive free-current scan within the tile
+};
+
+__device__ inline TileColumn tile_column(const DeviceProblem& P, const double* S,
+                                         const double* a, const double* w_row, double x_off,
+                                         double f0, int t, int lane) {
+  const int T = P.T;
+  const double dt = P.dt_fine;
+  TileColumn c;
+  const bool inside = t < T;
+  c.e = inside ? node_field(S, a, P.C, T, t) : 0.0;
+  c.w = inside ? w_row[t] : 0.0;
+  const bool step = lane < kTileSpan && t + 1 < T;  // this lane's increment reaches t + 1
+  const double w_next = __shfl_down(c.w, 1, kWave);
+  const double inc = step ? (dt * (w_next + c.w)) / 2.0 : 0.0;
+  const double incl = wave_scan(inc, lane);
+  double excl = __shfl_up(incl, 1, kWave);
+  if (lane == 0) excl = 0.0;
+  c.f = fraction_of(f0, x_off + excl);
+  const double g = (c.f * P.plasma_cmr) * c.e;
+  const double g_next = __shfl_down(g, 1, kWave);
+  const double jinc = step ? (dt * (g_next + g)) / 2.0 : 0.0;
+  c.j_incl = wave_scan(jinc, lane);
+  c.j_excl = __shfl_up(c.j_incl, 1, kWave);
+  if (lane == 0) c.j_excl = 0.0;
+  return c;
+}
 }  // namespace
 
 // ------------------------------------------------------------- Adams predictor
@@ -193,32 +205,41 @@ __global__ void adams_predict(DeviceProblem P, const AdamsRecord* R, FieldBuffer
   if (idx >= CN) return;
   const AdamsRecord& r = *R;
   const double dz = r.dz;
+  const int width = r.width, order = r.order, n_read = r.n_read;
   const cplx x = weighted(r.x_w, P.phi, P.n_phi, CN, idx);
   cplx phi[kMaxAdamsWidth + 1];
-  phi_functions(x, r.width, phi);
+  phi_functions(x, width, phi);
   cplx M[kMaxAdamsWidth];
-  for (int k = 0; k < r.width; ++k) {
-    M[k] = scale(kFactorial[k], phi[k + 1]);
-    B.moments[k * CN + idx] = M[k];
+#pragma unroll
+  for (int k = 0; k < kMaxAdamsWidth; ++k) {
+    M[k] = make(0.0, 0.0);
+    if (k < width) {
+      M[k] = scale(kFactorial[k], phi[k + 1]);
+      B.moments[k * CN + idx] = M[k];
+    }
   }
   const cplx flow = cexp(x);
   B.flow[idx] = flow;
   cplx F[kMaxAdamsWidth];
-  for (int j = 0; j < r.order; ++j) F[j] = B.store[r.slots[j] * CN + idx];
-  cplx lab = add(mul(flow, B.y[idx]), apply_moments(M, r.pred_c, F, r.order, r.order, dz));
+#pragma unroll
+  for (int j = 0; j < kMaxAdamsWidth; ++j)
+    F[j] = j < order ? B.store[r.slots[j] * CN + idx] : make(0.0, 0.0);
+  cplx lab = add(mul(flow, B.y[idx]), apply_moments(M, r.pred_c, F, order, order, dz));
   if (r.compensated) {
     const cplx reference = make(x.x / dz, x.y / dz);
-    cplx carried[kMaxAdamsWidth];
-    for (int j = 0; j < r.n_read; ++j) {
-      const int slot = r.slots[j];
-      const cplx rate = weighted(r.slot_rate_w[j], P.phi, P.n_phi, CN, idx);
-      const cplx term = mul(sub(rate, reference), B.states[slot * CN + idx]);
-      const cplx span = scale(1.0 - r.nodes[j], x);
-      carried[j] = mul(cexp(span), term);
-      B.carried[j * CN + idx] = carried[j];
-    }
     cplx total = make(0.0, 0.0);
-    for (int j = 0; j < r.order; ++j) total = add(total, scale(r.comp_pred_w[0][j], carried[j]));
+#pragma unroll
+    for (int j = 0; j < kMaxAdamsWidth; ++j) {
+      if (j < n_read) {
+        const int slot = r.slots[j];
+        const cplx rate = weighted(r.slot_rate_w[j], P.phi, P.n_phi, CN, idx);
+        const cplx term = mul(sub(rate, reference), B.states[slot * CN + idx]);
+        const cplx span = scale(1.0 - r.nodes[j], x);
+        const cplx carried = mul(cexp(span), term);
+        B.carried[j * CN + idx] = carried;
+        if (j < order) total = add(total, scale(r.comp_pred_w[0][j], carried));
+      }
+    }
     lab = add(lab, scale(dz, total));
   }
   B.lab[idx] = lab;
@@ -247,133 +268,140 @@ __global__ void field_scatter(DeviceProblem P, const cplx* spectrum, cplx* modal
   modal_t[idx] = c2r_pre(xa, xb, roots_T[k]);
 }
 
-// Pass A: the exposure's trapezoid increments over each tile of each node; the tile's
-// total, its last inclusive value.
-__global__ void __launch_bounds__(kScanTile)
-    plasma_exposure_tiles(DeviceProblem P, PlasmaTable pt, FieldBuffers B) {
-  __shared__ ScanLds lds;
-  const int T = P.T, tiles = T / kScanTile;
-  const int tile = blockIdx.x, q0 = blockIdx.y * kNodeBlock;
-  const int t = tile * kScanTile + threadIdx.x;
-  const bool last = threadIdx.x == kScanTile - 1;
+// Pass A: the rate at every node and sample, stored, and the exposure increments' sum
+// over each tile. One wave per (tile, node).
+__global__ void __launch_bounds__(kWave * kNodeWaves)
+    plasma_rate_tiles(DeviceProblem P, PlasmaTable pt, FieldBuffers B) {
+  const int T = P.T, tiles = scan_tiles(T);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int q = blockIdx.y * kNodeWaves + wave, tile = blockIdx.x;
+  if (q >= P.Q) return;  // whole waves leave; this kernel has no barrier
+  const int t = tile * kTileSpan + lane;
   const double* a = reinterpret_cast<const double*>(B.modal_t);
-  const double dt = P.dt_fine;
-  for (int q = q0; q < q0 + kNodeBlock && q < P.Q; ++q) {
-    const double* S = P.synthesis + static_cast<long>(q) * P.C;
-    const double e = node_field(S, a, P.C, T, t);
-    const double w = rate_of(fabs(e), pt, B.rate_c);
-    double tail = 0.0;
-    if (last && t + 1 < T) tail = rate_of(fabs(node_field(S, a, P.C, T, t + 1)), pt, B.rate_c);
-    double w_next, unused;
-    next_of(w, 0.0, lds, tail, 0.0, &w_next, &unused);
-    const double inc = t + 1 < T ? (dt * (w_next + w)) / 2.0 : 0.0;
-    double excl;
-    const double incl = tile_scan(inc, lds, &excl);
-    if (last) B.tile_x[static_cast<long>(q) * tiles + tile] = incl;
-  }
+  const double* S = P.synthesis + static_cast<long>(q) * P.C;
+  const bool inside = t < T;
+  const double e = inside ? node_field(S, a, P.C, T, t) : 0.0;
+  const double w = inside ? rate_of(fabs(e), pt, B.rate_c) : 0.0;
+  // The shared last sample is stored by the next tile's first lane.
+  if (inside && (lane < kTileSpan || t == T - 1)) B.rate[static_cast<long>(q) * T + t] = w;
+  const bool step = lane < kTileSpan && t + 1 < T;
+  const double w_next = __shfl_down(w, 1, kWave);
+  const double inc = step ? (P.dt_fine * (w_next + w)) / 2.0 : 0.0;
+  const double incl = wave_scan(inc, lane);
+  if (lane == kWave - 1) B.tile_x[static_cast<long>(q) * tiles + tile] = incl;
 }
 
-// Passes B and D: exclusive prefix of the tile sums of each node, in tile order.
-__global__ void plasma_tile_prefix(int Q, int tiles, double* sums) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= Q) return;
-  double running = 0.0;
-  double* row = sums + static_cast<long>(q) * tiles;
-  for (int k = 0; k < tiles; ++k) {
-    const double s = row[k];
+// Passes B and D: exclusive prefix of the tile sums of each node, one wavefront per
+// node. Each lane sums a run of consecutive tiles in order, the wave scans the run
+// totals, and each lane then writes its run's offsets in order: a fixed order, so the
+// offsets are the same on every run.
+__global__ void __launch_bounds__(kWave) plasma_tile_prefix(int tiles, double* sums) {
+  const int lane = threadIdx.x;
+  double* row = sums + static_cast<long>(blockIdx.x) * tiles;
+  const int run = (tiles + kWave - 1) / kWave;
+  const int k0 = lane * run;
+  const int k1 = k0 + run < tiles ? k0 + run : tiles;
+  double total = 0.0;
+  for (int k = k0; k < k1; ++k) total = total + row[k];
+  const double incl = wave_scan(total, lane);
+  double running = __shfl_up(incl, 1, kWave);
+  if (lane == 0) running = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const double here = row[k];
     row[k] = running;
-    running = running + s;
+    running = running + here;
   }
 }
 
-// Pass C (kFinal false): the free-current increments' tile sums. Pass E (kFinal true):
-// the whole column -- Kerr, fraction, free and loss currents, the collars -- projected
-// onto the modes over this workgroup's block of nodes.
-template <bool kFinal>
-__global__ void __launch_bounds__(kScanTile)
-    node_response(DeviceProblem P, const AdamsRecord* R, PlasmaTable pt, FieldBuffers B) {
-  __shared__ ScanLds lds;
-  const int T = P.T, tiles = T / kScanTile;
-  const int tile = blockIdx.x, q0 = blockIdx.y * kNodeBlock;
-  const int t = tile * kScanTile + threadIdx.x;
-  const bool last = threadIdx.x == kScanTile - 1;
+// Pass C: the free-current increments' sum over each tile. One wave per (tile, node).
+__global__ void __launch_bounds__(kWave * kNodeWaves)
+    plasma_current_tiles(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
+  const int T = P.T, tiles = scan_tiles(T);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int q = blockIdx.y * kNodeWaves + wave, tile = blockIdx.x;
+  if (q >= P.Q) return;  // whole waves leave; this kernel has no barrier
+  const int t = tile * kTileSpan + lane;
   const double* a = reinterpret_cast<const double*>(B.modal_t);
-  const double dt = P.dt_fine;
+  const double* S = P.synthesis + static_cast<long>(q) * P.C;
+  const long row = static_cast<long>(q) * tiles + tile;
+  const TileColumn c = tile_column(P, S, a, B.rate + static_cast<long>(q) * T, B.tile_x[row],
+                                   R->preionisation, t, lane);
+  if (lane == kWave - 1) B.tile_f[row] = c.j_incl;
+}
+
+// Pass E: Kerr, the fraction, the free and loss currents and the collars at every node
+// of the workgroup's tile, projected onto the modes. Each wave takes every
+// kNodeWaves-th node; the waves' projections are summed in wave order in LDS and the
+// tile's samples of the three streams written once.
+template <int C>
+__global__ void __launch_bounds__(kWave * kNodeWaves)
+    node_response(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
+  __shared__ double red[3 * C * kWave];
+  const int T = P.T, tiles = scan_tiles(T);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int tile = blockIdx.x;
+  const int t = tile * kTileSpan + lane;
+  // This lane's sample belongs to this tile unless the next tile's first lane holds it.
+  const bool owned = t < T && (lane < kTileSpan || t == T - 1);
+  const double* a = reinterpret_cast<const double*>(B.modal_t);
   const double density = R->density, f0 = R->preionisation;
-  const double g = P.t_window[t], gd = P.t_deriv[t];
-  double acc[3][kMaxModes];
+  const double g = t < T ? P.t_window[t] : 0.0, gd = t < T ? P.t_deriv[t] : 0.0;
+  double acc[3][C];
+#pragma unroll
   for (int s = 0; s < 3; ++s)
-    for (int m = 0; m < kMaxModes; ++m) acc[s][m] = 0.0;
-  for (int q = q0; q < q0 + kNodeBlock && q < P.Q; ++q) {
-    const double* S = P.synthesis + static_cast<long>(q) * P.C;
-    const double e = node_field(S, a, P.C, T, t);
-    double w = 0.0, f = f0, F = 0.0;
+#pragma unroll
+    for (int m = 0; m < C; ++m) acc[s][m] = 0.0;
+  for (int q = wave; q < P.Q; q += kNodeWaves) {
+    const double* S = P.synthesis + static_cast<long>(q) * C;
+    double e, current = 0.0;
     if (P.has_plasma) {
-      w = rate_of(fabs(e), pt, B.rate_c);
-      double tail_e = 0.0, tail_w = 0.0;
-      if (last && t + 1 < T) {
-        tail_e = node_field(S, a, P.C, T, t + 1);
-        tail_w = rate_of(fabs(tail_e), pt, B.rate_c);
-      }
-      double e_next, w_next;
-      next_of(e, w, lds, tail_e, tail_w, &e_next, &w_next);
-      const double inc = t + 1 < T ? (dt * (w_next + w)) / 2.0 : 0.0;
-      double excl;
-      const double incl = tile_scan(inc, lds, &excl);
-      const double off = B.tile_x[static_cast<long>(q) * tiles + tile];
-      const double x = off + excl;
-      const double x_next = off + incl;
-      f = fraction_of(f0, x);
-      const double f_next = fraction_of(f0, x_next);
-      const double gval = (f * P.plasma_cmr) * e;
-      const double g_next = (f_next * P.plasma_cmr) * e_next;
-      const double jinc = t + 1 < T ? (dt * (g_next + gval)) / 2.0 : 0.0;
-      double jexcl;
-      const double jincl = tile_scan(jinc, lds, &jexcl);
-      if (!kFinal) {
-        if (last) B.tile_f[static_cast<long>(q) * tiles + tile] = jincl;
-        continue;
-      }
-      F = B.tile_f[static_cast<long>(q) * tiles + tile] + jexcl;
+      const long row = static_cast<long>(q) * tiles + tile;
+      const TileColumn c = tile_column(P, S, a, B.rate + static_cast<long>(q) * T,
+                                       B.tile_x[row], f0, t, lane);
+      e = c.e;
+      const double F = B.tile_f[row] + c.j_excl;
+      const double squared = fabs(e) * fabs(e);
+      double loss = 0.0;
+      if (c.w > 0.0 && squared > 0.0) loss = (P.plasma_ip * c.w * (1.0 - c.f)) / squared;
+      current = density * (F + loss * e);
+    } else {
+      e = t < T ? node_field(S, a, C, T, t) : 0.0;
     }
-    if (!kFinal) continue;
     double pol = 0.0;
     if (P.has_kerr) {
       const double factor = P.kerr_coefficient * density;
       pol = (factor * (e * e)) * e;
     }
     pol = pol * g;
-    double current = 0.0;
-    if (P.has_plasma) {
-      const double squared = fabs(e) * fabs(e);
-      double loss = 0.0;
-      if (w > 0.0 && squared > 0.0) loss = (P.plasma_ip * w * (1.0 - f)) / squared;
-      current = density * (F + loss * e);
-    }
     const double gated = g * current, edge = gd * current;
-    for (int m = 0; m < P.C; ++m) {
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
       const double wmq = P.projection[static_cast<long>(m) * P.Q + q];
       acc[0][m] = acc[0][m] + wmq * pol;
       acc[1][m] = acc[1][m] + wmq * gated;
       acc[2][m] = acc[2][m] + wmq * edge;
     }
   }
-  if (kFinal) {
-    const long block = blockIdx.y;
-    for (int s = 0; s < 3; ++s)
-      for (int m = 0; m < P.C; ++m)
-        B.partial[((block * 3 + s) * P.C + m) * static_cast<long>(T) + t] = acc[s][m];
+  // Sum the waves' projections in wave order.
+  for (int w = 0; w < kNodeWaves; ++w) {
+    if (wave == w)
+#pragma unroll
+      for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+          double& cell = red[(s * C + m) * kWave + lane];
+          cell = w == 0 ? acc[s][m] : cell + acc[s][m];
+        }
+    __syncthreads();
   }
-}
-
-// Sum the node blocks' partial streams in block order into the packed stream rows.
-__global__ void stream_reduce(DeviceProblem P, FieldBuffers B, int n_blocks) {
-  const long n = 3L * P.C * P.T;
-  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  double v = 0.0;
-  for (int b = 0; b < n_blocks; ++b) v = v + B.partial[b * n + idx];
-  reinterpret_cast<double*>(B.streams)[idx] = v;
+  if (wave == 0 && owned) {
+    double* streams = reinterpret_cast<double*>(B.streams);
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+      for (int m = 0; m < C; ++m)
+        streams[(static_cast<long>(s) * C + m) * T + t] = red[(s * C + m) * kWave + lane];
+  }
 }
 
 // Unpack the three stream spectra, assemble the held current on the active bins and
@@ -413,50 +441,71 @@ __global__ void __launch_bounds__(kThreads)
   if (idx < CN) {
     const int b = static_cast<int>(idx % P.N);
     const double dz = r.dz;
+    const int width = r.width, n_read = r.n_read, n_orders = r.n_orders;
     cplx M[kMaxAdamsWidth], X[kMaxAdamsWidth];
-    for (int k = 0; k < r.width; ++k) M[k] = B.moments[k * CN + idx];
     X[0] = B.source[idx];
-    for (int j = 0; j < r.width - 1; ++j) X[1 + j] = B.store[r.slots[j] * CN + idx];
+#pragma unroll
+    for (int k = 0; k < kMaxAdamsWidth; ++k) {
+      M[k] = k < width ? B.moments[k * CN + idx] : make(0.0, 0.0);
+      if (k >= 1) X[k] = k < width ? B.store[r.slots[k - 1] * CN + idx] : make(0.0, 0.0);
+    }
     const cplx y = B.y[idx];
-    cplx adv = add(mul(B.flow[idx], y), apply_moments(M, r.corr_c, X, r.width, r.width, dz));
-    cplx comp_def[kMaxScored] = {};
+    cplx adv = add(mul(B.flow[idx], y), apply_moments(M, r.corr_c, X, width, width, dz));
+    cplx comp_def[kMaxScored];
+#pragma unroll
+    for (int i = 0; i < kMaxScored; ++i) comp_def[i] = make(0.0, 0.0);
     if (r.compensated) {
       const cplx x = weighted(r.x_w, P.phi, P.n_phi, CN, idx);
       const cplx reference = make(x.x / dz, x.y / dz);
       const cplx rate_end = weighted(r.rate_end_w, P.phi, P.n_phi, CN, idx);
       const cplx endpoint = mul(sub(rate_end, reference), B.lab[idx]);
       cplx carried[kMaxAdamsWidth];
-      for (int j = 0; j < r.n_read; ++j) carried[j] = B.carried[j * CN + idx];
-      for (int i = 0; i < r.n_orders; ++i) {
-        const int k = r.orders[i];
-        cplx corr = make(0.0, 0.0), pred = make(0.0, 0.0);
-        for (int j = 0; j < k; ++j) corr = add(corr, scale(r.comp_corr_w[i][1 + j], carried[j]));
-        corr = scale(dz, add(corr, scale(r.comp_corr_w[i][0], endpoint)));
-        for (int j = 0; j < k; ++j) pred = add(pred, scale(r.comp_pred_w[i][j], carried[j]));
-        pred = scale(dz, pred);
-        comp_def[i] = sub(corr, pred);
-        if (i == 0) adv = add(adv, corr);
+#pragma unroll
+      for (int j = 0; j < kMaxAdamsWidth; ++j)
+        carried[j] = j < n_read ? B.carried[j * CN + idx] : make(0.0, 0.0);
+#pragma unroll
+      for (int i = 0; i < kMaxScored; ++i) {
+        if (i < n_orders) {
+          const int k = r.orders[i];
+          cplx corr = make(0.0, 0.0), pred = make(0.0, 0.0);
+#pragma unroll
+          for (int j = 0; j < kMaxAdamsWidth; ++j)
+            if (j < k) corr = add(corr, scale(r.comp_corr_w[i][1 + j], carried[j]));
+          corr = scale(dz, add(corr, scale(r.comp_corr_w[i][0], endpoint)));
+#pragma unroll
+          for (int j = 0; j < kMaxAdamsWidth; ++j)
+            if (j < k) pred = add(pred, scale(r.comp_pred_w[i][j], carried[j]));
+          pred = scale(dz, pred);
+          comp_def[i] = sub(corr, pred);
+          if (i == 0) adv = add(adv, corr);
+        }
       }
     }
     B.advanced[idx] = adv;
     if (P.normmask[b]) {
       v[0] = norm2(y);
-      for (int i = 0; i < r.n_orders; ++i) {
-        cplx d = apply_moments(M, r.rows_c[i], X, r.width, r.width, dz);
-        if (r.compensated) d = add(d, comp_def[i]);
-        v[1 + i] = norm2(d);
+#pragma unroll
+      for (int i = 0; i < kMaxScored; ++i) {
+        if (i < n_orders) {
+          cplx d = apply_moments(M, r.rows_c[i], X, width, width, dz);
+          if (r.compensated) d = add(d, comp_def[i]);
+          v[1 + i] = norm2(d);
+        }
       }
     }
   }
   const int tid = threadIdx.x;
+#pragma unroll
   for (int s = 0; s < 1 + kMaxScored; ++s) red[s][tid] = v[s];
   __syncthreads();
   for (int w = kThreads / 2; w > 0; w >>= 1) {
     if (tid < w)
+#pragma unroll
       for (int s = 0; s < 1 + kMaxScored; ++s) red[s][tid] = red[s][tid] + red[s][tid + w];
     __syncthreads();
   }
   if (tid == 0)
+#pragma unroll
     for (int s = 0; s < 1 + kMaxScored; ++s) B.partials[s * gridDim.x + blockIdx.x] = red[s][0];
 }
 
@@ -508,38 +557,46 @@ void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuf
 void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,
                         FieldBuffers B, hipStream_t s, hipEvent_t* marks) {
   SP_CHECK(P.C <= kMaxModes, "the node passes project at most 8 modes");
-  SP_CHECK(P.T % kScanTile == 0, "the fine grid must hold whole scan tiles");
-  const int tiles = P.T / kScanTile;
-  const int node_blocks = (P.Q + kNodeBlock - 1) / kNodeBlock;
-  const dim3 grid(tiles, node_blocks);
+  SP_CHECK(P.T >= 2, "the fine grid needs two samples");
+  const int tiles = scan_tiles(P.T);
+  const dim3 block(kWave * kNodeWaves);
+  const dim3 node_grid(tiles, (P.Q + kNodeWaves - 1) / kNodeWaves);
   auto mark = [&](int i) {
     if (marks) SP_HIP(hipEventRecord(marks[i], s));
   };
   if (P.has_plasma) {
-    hipLaunchKernelGGL(plasma_exposure_tiles, grid, dim3(kScanTile), 0, s, P, pt, B);
+    hipLaunchKernelGGL(plasma_rate_tiles, node_grid, block, 0, s, P, pt, B);
     check();
     mark(0);
-    hipLaunchKernelGGL(plasma_tile_prefix, dim3(blocks_for(P.Q, 64)), dim3(64), 0, s, P.Q,
-                       tiles, B.tile_x);
+    hipLaunchKernelGGL(plasma_tile_prefix, dim3(P.Q), dim3(kWave), 0, s, tiles, B.tile_x);
     check();
     mark(1);
-    hipLaunchKernelGGL(node_response<false>, grid, dim3(kScanTile), 0, s, P, R, pt, B);
+    hipLaunchKernelGGL(plasma_current_tiles, node_grid, block, 0, s, P, R, B);
     check();
     mark(2);
-    hipLaunchKernelGGL(plasma_tile_prefix, dim3(blocks_for(P.Q, 64)), dim3(64), 0, s, P.Q,
-                       tiles, B.tile_f);
+    hipLaunchKernelGGL(plasma_tile_prefix, dim3(P.Q), dim3(kWave), 0, s, tiles, B.tile_f);
     check();
     mark(3);
   } else {
     for (int i = 0; i < 4; ++i) mark(i);
   }
-  hipLaunchKernelGGL(node_response<true>, grid, dim3(kScanTile), 0, s, P, R, pt, B);
+  switch (P.C) {
+#define SP_NODE_RESPONSE(c) \
+  case c:                   \
+    hipLaunchKernelGGL(node_response<c>, dim3(tiles), block, 0, s, P, R, B); \
+    break;
+    SP_NODE_RESPONSE(1)
+    SP_NODE_RESPONSE(2)
+    SP_NODE_RESPONSE(3)
+    SP_NODE_RESPONSE(4)
+    SP_NODE_RESPONSE(5)
+    SP_NODE_RESPONSE(6)
+    SP_NODE_RESPONSE(7)
+    SP_NODE_RESPONSE(8)
+#undef SP_NODE_RESPONSE
+  }
   check();
   mark(4);
-  const std::size_t n = 3UL * P.C * P.T;
-  hipLaunchKernelGGL(stream_reduce, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, P, B,
-                     node_blocks);
-  check();
 }
 
 void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,

@@ -4,10 +4,14 @@
 // One source evaluation is: scatter the lab-frame modal spectrum into the packed
 // real-transform input (1/T folded in) -> inverse FFT of T/2 packed points -> the node
 // passes (synthesis over the radial nodes, Kerr, the photoionisation rate and the two
-// cumulative plasma integrals as tile scans, the collars, and the projection of the
-// local polarisation and both held current streams back onto the modes) -> forward
-// FFT of the 3C packed streams -> finish (unpack, held-current assembly on the active
-// bins, prefactor and spectral collar).
+// cumulative plasma integrals as wave-tile scans, the collars, and the projection of
+// the local polarisation and both held current streams back onto the modes) ->
+// forward FFT of the 3C packed streams -> finish (unpack, held-current assembly on the
+// active bins, prefactor and spectral collar).
+//
+// The scans' tiles are one wavefront each: 64 samples holding 63 trapezoid increments,
+// the last sample shared with the next tile, so every increment's two ends sit in one
+// wave and no lane reaches outside it. Tile k starts at sample 63 k.
 #pragma once
 
 #include "device.hpp"
@@ -16,8 +20,12 @@ namespace sp {
 
 constexpr int kMaxAdamsWidth = 8;  // 1 + the highest scored order (MAX_ADAMS_ORDER 6)
 constexpr int kMaxScored = 3;      // the step's own order and two neighbours
-constexpr int kScanTile = 128;     // samples per plasma-scan tile (one per thread)
-constexpr int kNodeBlock = 64;     // radial nodes per workgroup of the node passes
+constexpr int kWave = 64;          // lanes of a wavefront
+constexpr int kTileSpan = kWave - 1;  // trapezoid increments per scan tile
+constexpr int kNodeWaves = 8;      // waves per workgroup of the node passes
+
+// Scan tiles of a fine grid of T samples: ceil((T - 1) / 63).
+__host__ __device__ inline int scan_tiles(int T) { return (T - 1 + kTileSpan - 1) / kTileSpan; }
 
 // What one Adams trial uploads (host scalars and the Lagrange tables of its stencil).
 struct AdamsRecord {
@@ -51,7 +59,7 @@ struct FieldBuffers {
   cplx* advanced;   // (C, N) corrected field
   cplx* modal_t;    // (C, T/2) packed real modal fields
   cplx* streams;    // (3, C, T/2) packed polarisation, gJ, g'J
-  double* partial;  // (node blocks, 3, C, T) projected stream partials
+  double* rate;     // (Q, T) ionisation rate at each node and sample
   double* tile_x;   // (Q, tiles) exposure tile sums, then offsets
   double* tile_f;   // (Q, tiles) free-current tile sums, then offsets
   double* partials; // (1 + kMaxScored, blocks) masked squared sums
@@ -71,9 +79,9 @@ void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuf
                           hipStream_t s);
 void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuffers B,
                           hipStream_t s);
-// `marks`, when given, holds five events recorded after the exposure tiles, its tile
-// prefix, the free-current tiles, its prefix and the node response (the last four
-// passes of a plain Kerr run record back to back).
+// `marks`, when given, holds five events recorded after the rate and exposure tiles,
+// their prefix, the free-current tiles, their prefix and the node response (a run
+// without plasma records the first four back to back).
 void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,
                         FieldBuffers B, hipStream_t s, hipEvent_t* marks = nullptr);
 void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,

@@ -31,7 +31,7 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   SP_CHECK(d.kind == 1 && d.method == 1,
            "the field workspace runs the real field grid under the exponential Adams march");
   const int C = d.n_modes, N = d.n_omega, T = d.n_to, Q = d.n_nodes;
-  SP_CHECK(T >= 2 * kScanTile && (T & (T - 1)) == 0, "n_to must be a power of two >= 256");
+  SP_CHECK(T >= 256 && (T & (T - 1)) == 0, "n_to must be a power of two >= 256");
   SP_CHECK(d.n_t >= 4 && T % d.n_t == 0, "n_to must be a multiple of n_t");
   SP_CHECK(N == d.n_t / 2 + 1, "a field grid has n_omega = n_t/2 + 1 bins");
   SP_CHECK(C >= 1 && C <= 8, "1..8 modes");
@@ -151,10 +151,10 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   B_.advanced = alloc<cplx>(CN);
   B_.modal_t = alloc<cplx>(static_cast<std::size_t>(C) * (T / 2));
   B_.streams = alloc<cplx>(3UL * C * (T / 2));
-  const int node_blocks = (Q + kNodeBlock - 1) / kNodeBlock;
-  B_.partial = alloc<double>(static_cast<std::size_t>(node_blocks) * 3 * C * T);
-  B_.tile_x = alloc<double>(static_cast<std::size_t>(Q) * (T / kScanTile));
-  B_.tile_f = alloc<double>(static_cast<std::size_t>(Q) * (T / kScanTile));
+  const std::size_t tiles = static_cast<std::size_t>(scan_tiles(T));
+  B_.rate = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
+  B_.tile_x = alloc<double>(static_cast<std::size_t>(Q) * tiles);
+  B_.tile_f = alloc<double>(static_cast<std::size_t>(Q) * tiles);
   n_blocks_ = static_cast<int>(correct_blocks(P_));
   B_.partials = alloc<double>((1 + kMaxScored) * static_cast<std::size_t>(n_blocks_));
   const auto roots = unit_roots(static_cast<std::size_t>(T));
@@ -196,10 +196,10 @@ void FieldWorkspace::release() {
 namespace {
 // The kernels (or transform plans) of one trial, in launch order.
 constexpr const char* kPhaseNames[FieldWorkspace::kPhases] = {
-    "adams_predict",         "field_scatter",  "inverse_fft",        "plasma_exposure_tiles",
-    "plasma_tile_prefix",    "plasma_current_tiles", "plasma_tile_prefix_current",
-    "node_response",         "stream_reduce",  "forward_fft",        "field_finish",
-    "adams_correct"};
+    "adams_predict",      "field_scatter",        "inverse_fft",
+    "plasma_rate_tiles",  "plasma_tile_prefix",   "plasma_current_tiles",
+    "plasma_tile_prefix_current", "node_response", "forward_fft",
+    "field_finish",       "adams_correct"};
 }  // namespace
 
 void FieldWorkspace::mark(int i) {
@@ -212,11 +212,10 @@ void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
   inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
   mark(3);
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_, marking_ ? marks_ + 4 : nullptr);
-  mark(9);
   fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
-  mark(10);
+  mark(9);
   launch_field_finish(P_, rec_dev_, B_, out, stream_);
-  mark(11);
+  mark(10);
 }
 
 void FieldWorkspace::enqueue_trial() {
@@ -225,7 +224,7 @@ void FieldWorkspace::enqueue_trial() {
   mark(1);
   enqueue_source(B_.lab, B_.source);
   launch_adams_correct(P_, rec_dev_, B_, stream_);
-  mark(12);
+  mark(11);
 }
 
 void FieldWorkspace::enqueue_start() {
