@@ -146,15 +146,34 @@ __device__ inline double fraction_of(double f0, double x) {
   return 1.0 - (1.0 - f0) * exp(-x);
 }
 
-// Inclusive scan across one wavefront in a fixed order (a shuffle ladder), so that the
-// result is the same on every run.
-__device__ inline double wave_scan(double v, int lane) {
-  for (int d = 1; d < kWave; d <<= 1) {
-    const double n = __shfl_up(v, d, kWave);
-    if (lane >= d) v = v + n;
-  }
+// Cross-lane moves of a double by DPP (two 32-bit halves, the same control), lanes with
+// no source or outside `kRowMask` reading zero. Controls (GFX9 DPP): row_shr:n is
+// 0x110 + n, row_bcast:15 is 0x142, row_bcast:31 is 0x143, wave_shl:1 is 0x130 (lane i
+// reads lane i + 1), wave_shr:1 is 0x138 (lane i reads lane i - 1).
+template <int kCtrl, int kRowMask, bool kBound>
+__device__ inline double dpp_move(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int rlo = __builtin_amdgcn_update_dpp(0, lo, kCtrl, kRowMask, 0xf, kBound);
+  const int rhi = __builtin_amdgcn_update_dpp(0, hi, kCtrl, kRowMask, 0xf, kBound);
+  return __hiloint2double(rhi, rlo);
+}
+
+// Inclusive scan across one wavefront in a fixed order: Hillis-Steele within each row of
+// 16 lanes, then row 0's total into row 1 and row 2's into row 3, then rows 0-1's total
+// into rows 2 and 3. The same on every run.
+__device__ inline double wave_scan(double v) {
+  v = v + dpp_move<0x111, 0xf, true>(v);
+  v = v + dpp_move<0x112, 0xf, true>(v);
+  v = v + dpp_move<0x114, 0xf, true>(v);
+  v = v + dpp_move<0x118, 0xf, true>(v);
+  v = v + dpp_move<0x142, 0xa, false>(v);
+  v = v + dpp_move<0x143, 0xc, false>(v);
   return v;
 }
+
+// The value of lane + 1 (zero in the last lane) and of lane - 1 (zero in the first).
+__device__ inline double from_next_lane(double v) { return dpp_move<0x130, 0xf, true>(v); }
+__device__ inline double from_previous_lane(double v) { return dpp_move<0x138, 0xf, true>(v); }
 
 __device__ inline double node_field(const double* S, const double* a, int C, int T, int t) {
   double e = 0.0;
@@ -167,7 +186,7 @@ __device__ inline double node_field(const double* S, const double* a, int C, int
 // free-current increments' scan. Lanes past the grid's end hold zeros.
 struct TileColumn {
   double e, w, f;      // field, rate and fraction at this lane's sample
-  double j_incl;       // inclusive free-current scan within the tile
+  double j_incl;       // inclusive free-current scan within the tile (lane 62: its total)
   double j_excl;       // exclusive free-current scan within the tile
 };
 
@@ -181,18 +200,15 @@ __device__ inline TileColumn tile_column(const DeviceProblem& P, const double* S
   c.e = inside ? node_field(S, a, P.C, T, t) : 0.0;
   c.w = inside ? w_row[t] : 0.0;
   const bool step = lane < kTileSpan && t + 1 < T;  // this lane's increment reaches t + 1
-  const double w_next = __shfl_down(c.w, 1, kWave);
+  const double w_next = from_next_lane(c.w);
   const double inc = step ? (dt * (w_next + c.w)) / 2.0 : 0.0;
-  const double incl = wave_scan(inc, lane);
-  double excl = __shfl_up(incl, 1, kWave);
-  if (lane == 0) excl = 0.0;
+  const double excl = from_previous_lane(wave_scan(inc));
   c.f = fraction_of(f0, x_off + excl);
   const double g = (c.f * P.plasma_cmr) * c.e;
-  const double g_next = __shfl_down(g, 1, kWave);
+  const double g_next = from_next_lane(g);
   const double jinc = step ? (dt * (g_next + g)) / 2.0 : 0.0;
-  c.j_incl = wave_scan(jinc, lane);
-  c.j_excl = __shfl_up(c.j_incl, 1, kWave);
-  if (lane == 0) c.j_excl = 0.0;
+  c.j_incl = wave_scan(jinc);
+  c.j_excl = from_previous_lane(c.j_incl);
   return c;
 }
 }  // namespace
@@ -285,10 +301,12 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   // The shared last sample is stored by the next tile's first lane.
   if (inside && (lane < kTileSpan || t == T - 1)) B.rate[static_cast<long>(q) * T + t] = w;
   const bool step = lane < kTileSpan && t + 1 < T;
-  const double w_next = __shfl_down(w, 1, kWave);
+  const double w_next = from_next_lane(w);
   const double inc = step ? (P.dt_fine * (w_next + w)) / 2.0 : 0.0;
-  const double incl = wave_scan(inc, lane);
-  if (lane == kWave - 1) B.tile_x[static_cast<long>(q) * tiles + tile] = incl;
+  const double incl = wave_scan(inc);
+  // The tile's total is lane 62's inclusive sum: the next tile's first sample then sits
+  // at exactly this tile's offset plus that sum.
+  if (lane == kTileSpan - 1) B.tile_x[static_cast<long>(q) * tiles + tile] = incl;
 }
 
 // Passes B and D: exclusive prefix of the tile sums of each node, one wavefront per
@@ -303,9 +321,7 @@ __global__ void __launch_bounds__(kWave) plasma_tile_prefix(int tiles, double* s
   const int k1 = k0 + run < tiles ? k0 + run : tiles;
   double total = 0.0;
   for (int k = k0; k < k1; ++k) total = total + row[k];
-  const double incl = wave_scan(total, lane);
-  double running = __shfl_up(incl, 1, kWave);
-  if (lane == 0) running = 0.0;
+  double running = from_previous_lane(wave_scan(total));
   for (int k = k0; k < k1; ++k) {
     const double here = row[k];
     row[k] = running;
@@ -326,15 +342,15 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   const long row = static_cast<long>(q) * tiles + tile;
   const TileColumn c = tile_column(P, S, a, B.rate + static_cast<long>(q) * T, B.tile_x[row],
                                    R->preionisation, t, lane);
-  if (lane == kWave - 1) B.tile_f[row] = c.j_incl;
+  if (lane == kTileSpan - 1) B.tile_f[row] = c.j_incl;
 }
 
 // Pass E: Kerr, the fraction, the free and loss currents and the collars at every node
 // of the workgroup's tile, projected onto the modes. Each wave takes every
-// kNodeWaves-th node; the waves' projections are summed in wave order in LDS and the
-// tile's samples of the three streams written once.
+// kResponseWaves-th node; the waves' projections are summed in wave order in LDS and
+// the tile's samples of the three streams written once.
 template <int C>
-__global__ void __launch_bounds__(kWave * kNodeWaves)
+__global__ void __launch_bounds__(kWave * kResponseWaves)
     node_response(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
   __shared__ double red[3 * C * kWave];
   const int T = P.T, tiles = scan_tiles(T);
@@ -351,7 +367,7 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   for (int s = 0; s < 3; ++s)
 #pragma unroll
     for (int m = 0; m < C; ++m) acc[s][m] = 0.0;
-  for (int q = wave; q < P.Q; q += kNodeWaves) {
+  for (int q = wave; q < P.Q; q += kResponseWaves) {
     const double* S = P.synthesis + static_cast<long>(q) * C;
     double e, current = 0.0;
     if (P.has_plasma) {
@@ -383,7 +399,7 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
     }
   }
   // Sum the waves' projections in wave order.
-  for (int w = 0; w < kNodeWaves; ++w) {
+  for (int w = 0; w < kResponseWaves; ++w) {
     if (wave == w)
 #pragma unroll
       for (int s = 0; s < 3; ++s)
@@ -509,13 +525,12 @@ __global__ void __launch_bounds__(kThreads)
     for (int s = 0; s < 1 + kMaxScored; ++s) B.partials[s * gridDim.x + blockIdx.x] = red[s][0];
 }
 
-__global__ void adams_accept(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
+__global__ void adams_accept(DeviceProblem P, int slot, int compensated, FieldBuffers B) {
   const long CN = static_cast<long>(P.C) * P.N;
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= CN) return;
-  const int slot = R->free_slot;
   B.store[slot * CN + idx] = B.source[idx];
-  if (R->compensated) B.states[slot * CN + idx] = B.lab[idx];
+  if (compensated) B.states[slot * CN + idx] = B.lab[idx];
   B.y[idx] = P.active[idx % P.N] ? B.advanced[idx] : make(0.0, 0.0);
 }
 
@@ -583,7 +598,8 @@ void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const Plas
   switch (P.C) {
 #define SP_NODE_RESPONSE(c) \
   case c:                   \
-    hipLaunchKernelGGL(node_response<c>, dim3(tiles), block, 0, s, P, R, B); \
+    hipLaunchKernelGGL(node_response<c>, dim3(tiles), dim3(kWave * kResponseWaves), 0, s, P, \
+                       R, B);                                                            \
     break;
     SP_NODE_RESPONSE(1)
     SP_NODE_RESPONSE(2)
@@ -611,9 +627,10 @@ void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuf
   check();
 }
 
-void launch_adams_accept(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+void launch_adams_accept(const DeviceProblem& P, int slot, bool compensated, FieldBuffers B,
                          hipStream_t s) {
-  hipLaunchKernelGGL(adams_accept, dim3(cn_blocks(P)), dim3(kThreads), 0, s, P, R, B);
+  hipLaunchKernelGGL(adams_accept, dim3(cn_blocks(P)), dim3(kThreads), 0, s, P, slot,
+                     compensated ? 1 : 0, B);
   check();
 }
 

@@ -22,7 +22,8 @@ constexpr int kMaxAdamsWidth = 8;  // 1 + the highest scored order (MAX_ADAMS_OR
 constexpr int kMaxScored = 3;      // the step's own order and two neighbours
 constexpr int kWave = 64;          // lanes of a wavefront
 constexpr int kTileSpan = kWave - 1;  // trapezoid increments per scan tile
-constexpr int kNodeWaves = 8;      // waves per workgroup of the node passes
+constexpr int kNodeWaves = 8;      // waves per workgroup of the rate and current passes
+constexpr int kResponseWaves = 16; // waves per workgroup of the node response
 
 // Scan tiles of a fine grid of T samples: ceil((T - 1) / 63).
 __host__ __device__ inline int scan_tiles(int T) { return (T - 1 + kTileSpan - 1) / kTileSpan; }
@@ -88,7 +89,9 @@ void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuff
                          cplx* out, hipStream_t s);
 void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                           hipStream_t s);
-void launch_adams_accept(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+// Commits the held trial into store slot `slot`; reads no record, so the host may go on
+// filling the next trial's record while it runs.
+void launch_adams_accept(const DeviceProblem& P, int slot, bool compensated, FieldBuffers B,
                          hipStream_t s);
 void launch_adams_start(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                         hipStream_t s);

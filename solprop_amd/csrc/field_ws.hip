@@ -384,11 +384,8 @@ std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Sten
 
 void FieldWorkspace::accept(double, int free, const adams::Stencil&,
                             const std::vector<adams::Entry>&) {
-  rec_host_->free_slot = free;
-  SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
-                        stream_));
-  launch_adams_accept(P_, rec_dev_, B_, stream_);
-  SP_HIP(hipStreamSynchronize(stream_));
+  // Enqueued without a wait: the next trial is ordered after it on the same stream.
+  launch_adams_accept(P_, free, compensated_, B_, stream_);
   slot_rates_[free] = held_rate_end_;
   accepted_ += 1;
 }
