@@ -60,9 +60,9 @@ def capillary_bytes(problem, order):
         "inverse_fft": 2 * ct,
         "plasma_rate_tiles": ct + table + qt + qk,
         "plasma_tile_prefix": 2 * qk,
-        "plasma_current_tiles": ct + table + qt + 2 * qk,
+        "plasma_current_tiles": ct + table + qt + 2 * qk + 2 * qt,
         "plasma_tile_prefix_current": 2 * qk,
-        "node_response": ct + 2 * table + qt + 2 * qk + 2 * 8 * T + 3 * ct,
+        "node_response": ct + 2 * table + 3 * qt + qk + 2 * 8 * T + 3 * ct,
         "forward_fft": 2 * 3 * ct,
         "field_finish": 3 * ct + cn * len(problem.prefactor_paths) + cn,
         "adams_correct": cn * (W + 1 + (W - 1) + 2) + cn,
@@ -105,6 +105,15 @@ def run_gpu(args):
         problem, config = workload_capillary()
     else:
         problem, config = workload_envelope()
+    if args.copy_probe_mib > 0:
+        # A plain device copy of known size, three times, for scaling memory counters.
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ws = Workspace(problem)
     session = ws.start(0.0, problem.field0)
 
@@ -220,6 +229,8 @@ def main():
     parser.add_argument("--warmup", type=int, default=30)
     parser.add_argument("--workload", choices=["capillary", "envelope"], default="capillary")
     parser.add_argument("--profile-steps", type=int, default=100)
+    parser.add_argument("--copy-probe-mib", type=int, default=0,
+                        help="run a plain device copy of this many MiB first (counter scaling)")
     parser.add_argument("--no-cpu", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1:

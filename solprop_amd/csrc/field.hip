@@ -32,6 +32,12 @@ __device__ inline cplx cdiv(cplx a, cplx b) {
 // from exp(x) beyond |x| = 3. Every loop over the entries is unrolled to the largest
 // order with the runtime order as a guard, so that `out` stays in registers.
 __device__ void phi_functions(cplx x, int order, cplx (&out)[kMaxAdamsWidth + 1]) {
+  if (x.x == 0.0 && x.y == 0.0) {
+    // The series at zero, exactly: phi_k(0) = 1/k! (what the Taylor path rounds to).
+#pragma unroll
+    for (int k = 0; k <= kMaxAdamsWidth; ++k) out[k] = make(kRecipFactorial[k], 0.0);
+    return;
+  }
   const double magnitude = hypot(x.x, x.y);
   if (magnitude > 3.0) {
     out[0] = cexp(x);
@@ -181,30 +187,45 @@ __device__ inline double node_field(const double* S, const double* a, int C, int
   return e;
 }
 
+// The field at a node from the modal fields of this lane's sample, Σ_m S[m] a_m, in the
+// same order as node_field.
+template <int C>
+__device__ inline double synthesise_at(const double* S, const double (&am)[C]) {
+  double e = 0.0;
+#pragma unroll
+  for (int m = 0; m < C; ++m) e = e + S[m] * am[m];
+  return e;
+}
+
+// The modal fields (C, T) real at sample t, zero past the grid's end.
+template <int C>
+__device__ inline void load_modal(const FieldBuffers& B, int T, int t, double (&am)[C]) {
+  const double* a = reinterpret_cast<const double*>(B.modal_t);
+#pragma unroll
+  for (int m = 0; m < C; ++m) am[m] = t < T ? a[static_cast<long>(m) * T + t] : 0.0;
+}
+
 // One node's exposure and free current over one scan tile (see field.hpp): the rate
 // read back, the exposure increments' scan plus the tile's offset, the fraction, the
 // free-current increments' scan. Lanes past the grid's end hold zeros.
 struct TileColumn {
-  double e, w, f;      // field, rate and fraction at this lane's sample
+  double w, f;         // rate and fraction at this lane's sample
   double j_incl;       // inclusive free-current scan within the tile (lane 62: its total)
   double j_excl;       // exclusive free-current scan within the tile
 };
 
-__device__ inline TileColumn tile_column(const DeviceProblem& P, const double* S,
-                                         const double* a, const double* w_row, double x_off,
-                                         double f0, int t, int lane) {
+__device__ inline TileColumn tile_column(const DeviceProblem& P, double e, const double* w_row,
+                                         double x_off, double f0, int t, int lane) {
   const int T = P.T;
   const double dt = P.dt_fine;
   TileColumn c;
-  const bool inside = t < T;
-  c.e = inside ? node_field(S, a, P.C, T, t) : 0.0;
-  c.w = inside ? w_row[t] : 0.0;
+  c.w = t < T ? w_row[t] : 0.0;
   const bool step = lane < kTileSpan && t + 1 < T;  // this lane's increment reaches t + 1
   const double w_next = from_next_lane(c.w);
   const double inc = step ? (dt * (w_next + c.w)) / 2.0 : 0.0;
   const double excl = from_previous_lane(wave_scan(inc));
   c.f = fraction_of(f0, x_off + excl);
-  const double g = (c.f * P.plasma_cmr) * c.e;
+  const double g = (c.f * P.plasma_cmr) * e;
   const double g_next = from_next_lane(g);
   const double jinc = step ? (dt * (g_next + g)) / 2.0 : 0.0;
   c.j_incl = wave_scan(jinc);
@@ -215,14 +236,32 @@ __device__ inline TileColumn tile_column(const DeviceProblem& P, const double* S
 
 // ------------------------------------------------------------- Adams predictor
 
-__global__ void adams_predict(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
+// The predictor. One wavefront per workgroup; every global operand is loaded before
+// any arithmetic, so that a thread waits on memory once rather than once per operand.
+__global__ void __launch_bounds__(kWave)
+    adams_predict(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const long CN = static_cast<long>(P.C) * P.N;
   if (idx >= CN) return;
   const AdamsRecord& r = *R;
   const double dz = r.dz;
   const int width = r.width, order = r.order, n_read = r.n_read;
-  const cplx x = weighted(r.x_w, P.phi, P.n_phi, CN, idx);
+  const bool compensated = r.compensated != 0;
+  // Loads.
+  cplx phi_t[kMaxTerms], F[kMaxAdamsWidth], U[kMaxAdamsWidth];
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t) phi_t[t] = t < P.n_phi ? P.phi[t * CN + idx] : make(0.0, 0.0);
+  const cplx y = B.y[idx];
+#pragma unroll
+  for (int j = 0; j < kMaxAdamsWidth; ++j) {
+    F[j] = j < order ? B.store[r.slots[j] * CN + idx] : make(0.0, 0.0);
+    U[j] = compensated && j < n_read ? B.states[r.slots[j] * CN + idx] : make(0.0, 0.0);
+  }
+  // x = Σ_t w_t Φ_t, in term order.
+  cplx x = make(0.0, 0.0);
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t)
+    if (t < P.n_phi) x = add(x, scale(r.x_w[t], phi_t[t]));
   cplx phi[kMaxAdamsWidth + 1];
   phi_functions(x, width, phi);
   cplx M[kMaxAdamsWidth];
@@ -234,24 +273,26 @@ __global__ void adams_predict(DeviceProblem P, const AdamsRecord* R, FieldBuffer
       B.moments[k * CN + idx] = M[k];
     }
   }
-  const cplx flow = cexp(x);
+  // exp(0) = 1 exactly: inactive bins (x = 0) skip the transcendental.
+  const bool still = x.x == 0.0 && x.y == 0.0;
+  const cplx flow = still ? make(1.0, 0.0) : cexp(x);
   B.flow[idx] = flow;
-  cplx F[kMaxAdamsWidth];
-#pragma unroll
-  for (int j = 0; j < kMaxAdamsWidth; ++j)
-    F[j] = j < order ? B.store[r.slots[j] * CN + idx] : make(0.0, 0.0);
-  cplx lab = add(mul(flow, B.y[idx]), apply_moments(M, r.pred_c, F, order, order, dz));
-  if (r.compensated) {
+  cplx lab = add(mul(flow, y), apply_moments(M, r.pred_c, F, order, order, dz));
+  if (compensated) {
     const cplx reference = make(x.x / dz, x.y / dz);
     cplx total = make(0.0, 0.0);
 #pragma unroll
     for (int j = 0; j < kMaxAdamsWidth; ++j) {
       if (j < n_read) {
-        const int slot = r.slots[j];
-        const cplx rate = weighted(r.slot_rate_w[j], P.phi, P.n_phi, CN, idx);
-        const cplx term = mul(sub(rate, reference), B.states[slot * CN + idx]);
+        cplx rate = make(0.0, 0.0);
+#pragma unroll
+        for (int t = 0; t < kMaxTerms; ++t)
+          if (t < P.n_phi) rate = add(rate, scale(r.slot_rate_w[j][t], phi_t[t]));
+        const cplx term = mul(sub(rate, reference), U[j]);
+        // A zero term (a bin with no field) carries zero whatever it is transported by.
+        const bool empty = term.x == 0.0 && term.y == 0.0;
         const cplx span = scale(1.0 - r.nodes[j], x);
-        const cplx carried = mul(cexp(span), term);
+        const cplx carried = empty ? make(0.0, 0.0) : mul(cexp(span), term);
         B.carried[j * CN + idx] = carried;
         if (j < order) total = add(total, scale(r.comp_pred_w[0][j], carried));
       }
@@ -285,28 +326,44 @@ __global__ void field_scatter(DeviceProblem P, const cplx* spectrum, cplx* modal
 }
 
 // Pass A: the rate at every node and sample, stored, and the exposure increments' sum
-// over each tile. One wave per (tile, node).
+// over each tile. A workgroup takes one tile and kNodeWaves * kNodesPerWave nodes; each
+// wave runs its nodes with the tile's modal fields in registers and the rate table in
+// LDS (read from global memory above kLdsTableIntervals intervals).
+template <int C>
 __global__ void __launch_bounds__(kWave * kNodeWaves)
     plasma_rate_tiles(DeviceProblem P, PlasmaTable pt, FieldBuffers B) {
+  __shared__ double table[4 * kLdsTableIntervals];
+  const bool in_lds = pt.n_intervals <= kLdsTableIntervals;
+  if (in_lds)
+    for (int i = threadIdx.x; i < 4 * pt.n_intervals; i += blockDim.x) table[i] = B.rate_c[i];
+  __syncthreads();
+  const double* coefficients = in_lds ? table : B.rate_c;
   const int T = P.T, tiles = scan_tiles(T);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const int q = blockIdx.y * kNodeWaves + wave, tile = blockIdx.x;
-  if (q >= P.Q) return;  // whole waves leave; this kernel has no barrier
+  const int tile = blockIdx.x;
   const int t = tile * kTileSpan + lane;
-  const double* a = reinterpret_cast<const double*>(B.modal_t);
-  const double* S = P.synthesis + static_cast<long>(q) * P.C;
   const bool inside = t < T;
-  const double e = inside ? node_field(S, a, P.C, T, t) : 0.0;
-  const double w = inside ? rate_of(fabs(e), pt, B.rate_c) : 0.0;
   // The shared last sample is stored by the next tile's first lane.
-  if (inside && (lane < kTileSpan || t == T - 1)) B.rate[static_cast<long>(q) * T + t] = w;
+  const bool stores = inside && (lane < kTileSpan || t == T - 1);
   const bool step = lane < kTileSpan && t + 1 < T;
-  const double w_next = from_next_lane(w);
-  const double inc = step ? (P.dt_fine * (w_next + w)) / 2.0 : 0.0;
-  const double incl = wave_scan(inc);
-  // The tile's total is lane 62's inclusive sum: the next tile's first sample then sits
-  // at exactly this tile's offset plus that sum.
-  if (lane == kTileSpan - 1) B.tile_x[static_cast<long>(q) * tiles + tile] = incl;
+  double am[C];
+  load_modal<C>(B, T, t, am);
+  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave;
+#pragma unroll
+  for (int i = 0; i < kNodesPerWave; ++i) {
+    const int q = q0 + i * kNodeWaves + wave;
+    if (q < P.Q) {  // uniform across the wave
+      const double e = synthesise_at<C>(P.synthesis + static_cast<long>(q) * C, am);
+      const double w = inside ? rate_of(fabs(e), pt, coefficients) : 0.0;
+      if (stores) B.rate[static_cast<long>(q) * T + t] = w;
+      const double w_next = from_next_lane(w);
+      const double inc = step ? (P.dt_fine * (w_next + w)) / 2.0 : 0.0;
+      const double incl = wave_scan(inc);
+      // The tile's total is lane 62's inclusive sum (lane 63 adds no increment): the next
+      // tile's first sample then sits at exactly this tile's offset plus that sum.
+      if (lane == kTileSpan - 1) B.tile_x[static_cast<long>(q) * tiles + tile] = incl;
+    }
+  }
 }
 
 // Passes B and D: exclusive prefix of the tile sums of each node, one wavefront per
@@ -329,20 +386,35 @@ __global__ void __launch_bounds__(kWave) plasma_tile_prefix(int tiles, double* s
   }
 }
 
-// Pass C: the free-current increments' sum over each tile. One wave per (tile, node).
+// Pass C: the free-current increments' sum over each tile, and each sample's fraction
+// and within-tile free current for the response pass. Same layout as pass A.
+template <int C>
 __global__ void __launch_bounds__(kWave * kNodeWaves)
     plasma_current_tiles(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
   const int T = P.T, tiles = scan_tiles(T);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const int q = blockIdx.y * kNodeWaves + wave, tile = blockIdx.x;
-  if (q >= P.Q) return;  // whole waves leave; this kernel has no barrier
+  const int tile = blockIdx.x;
   const int t = tile * kTileSpan + lane;
-  const double* a = reinterpret_cast<const double*>(B.modal_t);
-  const double* S = P.synthesis + static_cast<long>(q) * P.C;
-  const long row = static_cast<long>(q) * tiles + tile;
-  const TileColumn c = tile_column(P, S, a, B.rate + static_cast<long>(q) * T, B.tile_x[row],
-                                   R->preionisation, t, lane);
-  if (lane == kTileSpan - 1) B.tile_f[row] = c.j_incl;
+  const bool stores = t < T && (lane < kTileSpan || t == T - 1);
+  const double f0 = R->preionisation;
+  double am[C];
+  load_modal<C>(B, T, t, am);
+  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave;
+#pragma unroll
+  for (int i = 0; i < kNodesPerWave; ++i) {
+    const int q = q0 + i * kNodeWaves + wave;
+    if (q < P.Q) {  // uniform across the wave
+      const double e = synthesise_at<C>(P.synthesis + static_cast<long>(q) * C, am);
+      const long row = static_cast<long>(q) * tiles + tile;
+      const TileColumn c = tile_column(P, e, B.rate + static_cast<long>(q) * T, B.tile_x[row],
+                                       f0, t, lane);
+      if (lane == kTileSpan - 1) B.tile_f[row] = c.j_incl;
+      if (stores) {
+        B.frac[static_cast<long>(q) * T + t] = c.f;
+        B.free_local[static_cast<long>(q) * T + t] = c.j_excl;
+      }
+    }
+  }
 }
 
 // Pass E: Kerr, the fraction, the free and loss currents and the collars at every node
@@ -359,29 +431,27 @@ __global__ void __launch_bounds__(kWave * kResponseWaves)
   const int t = tile * kTileSpan + lane;
   // This lane's sample belongs to this tile unless the next tile's first lane holds it.
   const bool owned = t < T && (lane < kTileSpan || t == T - 1);
-  const double* a = reinterpret_cast<const double*>(B.modal_t);
-  const double density = R->density, f0 = R->preionisation;
+  const double density = R->density;
   const double g = t < T ? P.t_window[t] : 0.0, gd = t < T ? P.t_deriv[t] : 0.0;
+  double am[C];
+  load_modal<C>(B, T, t, am);
   double acc[3][C];
 #pragma unroll
   for (int s = 0; s < 3; ++s)
 #pragma unroll
     for (int m = 0; m < C; ++m) acc[s][m] = 0.0;
+#pragma unroll(C <= 4 ? 2 : 1)
   for (int q = wave; q < P.Q; q += kResponseWaves) {
-    const double* S = P.synthesis + static_cast<long>(q) * C;
-    double e, current = 0.0;
-    if (P.has_plasma) {
-      const long row = static_cast<long>(q) * tiles + tile;
-      const TileColumn c = tile_column(P, S, a, B.rate + static_cast<long>(q) * T,
-                                       B.tile_x[row], f0, t, lane);
-      e = c.e;
-      const double F = B.tile_f[row] + c.j_excl;
+    const double e = synthesise_at<C>(P.synthesis + static_cast<long>(q) * C, am);
+    double current = 0.0;
+    if (P.has_plasma && t < T) {
+      const long at = static_cast<long>(q) * T + t;
+      const double w = B.rate[at], f = B.frac[at];
+      const double F = B.tile_f[static_cast<long>(q) * tiles + tile] + B.free_local[at];
       const double squared = fabs(e) * fabs(e);
       double loss = 0.0;
-      if (c.w > 0.0 && squared > 0.0) loss = (P.plasma_ip * c.w * (1.0 - c.f)) / squared;
+      if (w > 0.0 && squared > 0.0) loss = (P.plasma_ip * w * (1.0 - f)) / squared;
       current = density * (F + loss * e);
-    } else {
-      e = t < T ? node_field(S, a, C, T, t) : 0.0;
     }
     double pol = 0.0;
     if (P.has_kerr) {
@@ -447,9 +517,11 @@ __global__ void field_finish(DeviceProblem P, const AdamsRecord* R, FieldBuffers
 
 // ------------------------------------------------------------- Adams corrector
 
-__global__ void __launch_bounds__(kThreads)
+// The corrector, the defects of the scored orders and their masked squared sums. One
+// wavefront per workgroup, operands loaded first; each wave's sums are reduced across
+// its lanes in a fixed order and written as that workgroup's partials.
+__global__ void __launch_bounds__(kWave)
     adams_correct(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
-  __shared__ double red[1 + kMaxScored][kThreads];
   const long CN = static_cast<long>(P.C) * P.N;
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const AdamsRecord& r = *R;
@@ -458,27 +530,38 @@ __global__ void __launch_bounds__(kThreads)
     const int b = static_cast<int>(idx % P.N);
     const double dz = r.dz;
     const int width = r.width, n_read = r.n_read, n_orders = r.n_orders;
-    cplx M[kMaxAdamsWidth], X[kMaxAdamsWidth];
+    const bool compensated = r.compensated != 0;
+    // Loads.
+    cplx M[kMaxAdamsWidth], X[kMaxAdamsWidth], carried[kMaxAdamsWidth], phi_t[kMaxTerms];
     X[0] = B.source[idx];
 #pragma unroll
     for (int k = 0; k < kMaxAdamsWidth; ++k) {
       M[k] = k < width ? B.moments[k * CN + idx] : make(0.0, 0.0);
       if (k >= 1) X[k] = k < width ? B.store[r.slots[k - 1] * CN + idx] : make(0.0, 0.0);
+      carried[k] = compensated && k < n_read ? B.carried[k * CN + idx] : make(0.0, 0.0);
     }
+#pragma unroll
+    for (int t = 0; t < kMaxTerms; ++t)
+      phi_t[t] = compensated && t < P.n_phi ? P.phi[t * CN + idx] : make(0.0, 0.0);
     const cplx y = B.y[idx];
-    cplx adv = add(mul(B.flow[idx], y), apply_moments(M, r.corr_c, X, width, width, dz));
+    const cplx flow = B.flow[idx];
+    const cplx lab = compensated ? B.lab[idx] : make(0.0, 0.0);
+    const bool scored = P.normmask[b] != 0;
+
+    cplx adv = add(mul(flow, y), apply_moments(M, r.corr_c, X, width, width, dz));
     cplx comp_def[kMaxScored];
 #pragma unroll
     for (int i = 0; i < kMaxScored; ++i) comp_def[i] = make(0.0, 0.0);
-    if (r.compensated) {
-      const cplx x = weighted(r.x_w, P.phi, P.n_phi, CN, idx);
-      const cplx reference = make(x.x / dz, x.y / dz);
-      const cplx rate_end = weighted(r.rate_end_w, P.phi, P.n_phi, CN, idx);
-      const cplx endpoint = mul(sub(rate_end, reference), B.lab[idx]);
-      cplx carried[kMaxAdamsWidth];
+    if (compensated) {
+      cplx x = make(0.0, 0.0), rate_end = make(0.0, 0.0);
 #pragma unroll
-      for (int j = 0; j < kMaxAdamsWidth; ++j)
-        carried[j] = j < n_read ? B.carried[j * CN + idx] : make(0.0, 0.0);
+      for (int t = 0; t < kMaxTerms; ++t)
+        if (t < P.n_phi) x = add(x, scale(r.x_w[t], phi_t[t]));
+#pragma unroll
+      for (int t = 0; t < kMaxTerms; ++t)
+        if (t < P.n_phi) rate_end = add(rate_end, scale(r.rate_end_w[t], phi_t[t]));
+      const cplx reference = make(x.x / dz, x.y / dz);
+      const cplx endpoint = mul(sub(rate_end, reference), lab);
 #pragma unroll
       for (int i = 0; i < kMaxScored; ++i) {
         if (i < n_orders) {
@@ -498,31 +581,24 @@ __global__ void __launch_bounds__(kThreads)
       }
     }
     B.advanced[idx] = adv;
-    if (P.normmask[b]) {
+    if (scored) {
       v[0] = norm2(y);
 #pragma unroll
       for (int i = 0; i < kMaxScored; ++i) {
         if (i < n_orders) {
           cplx d = apply_moments(M, r.rows_c[i], X, width, width, dz);
-          if (r.compensated) d = add(d, comp_def[i]);
+          if (compensated) d = add(d, comp_def[i]);
           v[1 + i] = norm2(d);
         }
       }
     }
   }
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x;
 #pragma unroll
-  for (int s = 0; s < 1 + kMaxScored; ++s) red[s][tid] = v[s];
-  __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if (tid < w)
-#pragma unroll
-      for (int s = 0; s < 1 + kMaxScored; ++s) red[s][tid] = red[s][tid] + red[s][tid + w];
-    __syncthreads();
+  for (int s = 0; s < 1 + kMaxScored; ++s) {
+    const double total = wave_scan(v[s]);
+    if (lane == kWave - 1) B.partials[s * gridDim.x + blockIdx.x] = total;
   }
-  if (tid == 0)
-#pragma unroll
-    for (int s = 0; s < 1 + kMaxScored; ++s) B.partials[s * gridDim.x + blockIdx.x] = red[s][0];
 }
 
 __global__ void adams_accept(DeviceProblem P, int slot, int compensated, FieldBuffers B) {
@@ -531,6 +607,17 @@ __global__ void adams_accept(DeviceProblem P, int slot, int compensated, FieldBu
   if (idx >= CN) return;
   B.store[slot * CN + idx] = B.source[idx];
   if (compensated) B.states[slot * CN + idx] = B.lab[idx];
+  B.y[idx] = P.active[idx % P.N] ? B.advanced[idx] : make(0.0, 0.0);
+}
+
+// The same commit, its slot read from the record: the head of the next trial's graph.
+__global__ void adams_commit(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
+  const long CN = static_cast<long>(P.C) * P.N;
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= CN) return;
+  const int slot = R->free_slot;
+  B.store[slot * CN + idx] = B.source[idx];
+  if (R->compensated) B.states[slot * CN + idx] = B.lab[idx];
   B.y[idx] = P.active[idx % P.N] ? B.advanced[idx] : make(0.0, 0.0);
 }
 
@@ -545,7 +632,7 @@ __global__ void adams_start(DeviceProblem P, const AdamsRecord* R, FieldBuffers 
 // -------------------------------------------------------------------- launchers
 
 unsigned correct_blocks(const DeviceProblem& P) {
-  return blocks_for(static_cast<std::size_t>(P.C) * P.N, kThreads);
+  return blocks_for(static_cast<std::size_t>(P.C) * P.N, kWave);
 }
 
 namespace {
@@ -557,7 +644,7 @@ unsigned cn_blocks(const DeviceProblem& P) {
 
 void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                           hipStream_t s) {
-  hipLaunchKernelGGL(adams_predict, dim3(cn_blocks(P)), dim3(kThreads), 0, s, P, R, B);
+  hipLaunchKernelGGL(adams_predict, dim3(correct_blocks(P)), dim3(kWave), 0, s, P, R, B);
   check();
 }
 
@@ -569,24 +656,26 @@ void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuf
   check();
 }
 
-void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,
-                        FieldBuffers B, hipStream_t s, hipEvent_t* marks) {
-  SP_CHECK(P.C <= kMaxModes, "the node passes project at most 8 modes");
-  SP_CHECK(P.T >= 2, "the fine grid needs two samples");
+namespace {
+template <int C>
+void launch_node_passes_for(const DeviceProblem& P, const AdamsRecord* R,
+                            const PlasmaTable& pt, FieldBuffers B, hipStream_t s,
+                            hipEvent_t* marks) {
   const int tiles = scan_tiles(P.T);
-  const dim3 block(kWave * kNodeWaves);
-  const dim3 node_grid(tiles, (P.Q + kNodeWaves - 1) / kNodeWaves);
+  const int per_block = kNodeWaves * kNodesPerWave;
+  const dim3 node_grid(tiles, (P.Q + per_block - 1) / per_block);
+  const dim3 node_block(kWave * kNodeWaves);
   auto mark = [&](int i) {
     if (marks) SP_HIP(hipEventRecord(marks[i], s));
   };
   if (P.has_plasma) {
-    hipLaunchKernelGGL(plasma_rate_tiles, node_grid, block, 0, s, P, pt, B);
+    hipLaunchKernelGGL(plasma_rate_tiles<C>, node_grid, node_block, 0, s, P, pt, B);
     check();
     mark(0);
     hipLaunchKernelGGL(plasma_tile_prefix, dim3(P.Q), dim3(kWave), 0, s, tiles, B.tile_x);
     check();
     mark(1);
-    hipLaunchKernelGGL(plasma_current_tiles, node_grid, block, 0, s, P, R, B);
+    hipLaunchKernelGGL(plasma_current_tiles<C>, node_grid, node_block, 0, s, P, R, B);
     check();
     mark(2);
     hipLaunchKernelGGL(plasma_tile_prefix, dim3(P.Q), dim3(kWave), 0, s, tiles, B.tile_f);
@@ -595,24 +684,27 @@ void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const Plas
   } else {
     for (int i = 0; i < 4; ++i) mark(i);
   }
-  switch (P.C) {
-#define SP_NODE_RESPONSE(c) \
-  case c:                   \
-    hipLaunchKernelGGL(node_response<c>, dim3(tiles), dim3(kWave * kResponseWaves), 0, s, P, \
-                       R, B);                                                            \
-    break;
-    SP_NODE_RESPONSE(1)
-    SP_NODE_RESPONSE(2)
-    SP_NODE_RESPONSE(3)
-    SP_NODE_RESPONSE(4)
-    SP_NODE_RESPONSE(5)
-    SP_NODE_RESPONSE(6)
-    SP_NODE_RESPONSE(7)
-    SP_NODE_RESPONSE(8)
-#undef SP_NODE_RESPONSE
-  }
+  hipLaunchKernelGGL(node_response<C>, dim3(tiles), dim3(kWave * kResponseWaves), 0, s, P, R,
+                     B);
   check();
   mark(4);
+}
+}  // namespace
+
+void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,
+                        FieldBuffers B, hipStream_t s, hipEvent_t* marks) {
+  SP_CHECK(P.C >= 1 && P.C <= kMaxModes, "the node passes project 1..8 modes");
+  SP_CHECK(P.T >= 2, "the fine grid needs two samples");
+  switch (P.C) {
+    case 1: launch_node_passes_for<1>(P, R, pt, B, s, marks); break;
+    case 2: launch_node_passes_for<2>(P, R, pt, B, s, marks); break;
+    case 3: launch_node_passes_for<3>(P, R, pt, B, s, marks); break;
+    case 4: launch_node_passes_for<4>(P, R, pt, B, s, marks); break;
+    case 5: launch_node_passes_for<5>(P, R, pt, B, s, marks); break;
+    case 6: launch_node_passes_for<6>(P, R, pt, B, s, marks); break;
+    case 7: launch_node_passes_for<7>(P, R, pt, B, s, marks); break;
+    default: launch_node_passes_for<8>(P, R, pt, B, s, marks); break;
+  }
 }
 
 void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
@@ -623,7 +715,7 @@ void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuff
 
 void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                           hipStream_t s) {
-  hipLaunchKernelGGL(adams_correct, dim3(correct_blocks(P)), dim3(kThreads), 0, s, P, R, B);
+  hipLaunchKernelGGL(adams_correct, dim3(correct_blocks(P)), dim3(kWave), 0, s, P, R, B);
   check();
 }
 
@@ -631,6 +723,12 @@ void launch_adams_accept(const DeviceProblem& P, int slot, bool compensated, Fie
                          hipStream_t s) {
   hipLaunchKernelGGL(adams_accept, dim3(cn_blocks(P)), dim3(kThreads), 0, s, P, slot,
                      compensated ? 1 : 0, B);
+  check();
+}
+
+void launch_adams_commit(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(adams_commit, dim3(cn_blocks(P)), dim3(kThreads), 0, s, P, R, B);
   check();
 }
 

@@ -23,6 +23,8 @@ constexpr int kMaxScored = 3;      // the step's own order and two neighbours
 constexpr int kWave = 64;          // lanes of a wavefront
 constexpr int kTileSpan = kWave - 1;  // trapezoid increments per scan tile
 constexpr int kNodeWaves = 8;      // waves per workgroup of the rate and current passes
+constexpr int kNodesPerWave = 4;   // nodes each wave of those passes runs
+constexpr int kLdsTableIntervals = 1024;  // rate-table intervals held in LDS
 constexpr int kResponseWaves = 16; // waves per workgroup of the node response
 
 // Scan tiles of a fine grid of T samples: ceil((T - 1) / 63).
@@ -61,6 +63,8 @@ struct FieldBuffers {
   cplx* modal_t;    // (C, T/2) packed real modal fields
   cplx* streams;    // (3, C, T/2) packed polarisation, gJ, g'J
   double* rate;     // (Q, T) ionisation rate at each node and sample
+  double* frac;     // (Q, T) ionised fraction
+  double* free_local;  // (Q, T) free current since the start of the sample's scan tile
   double* tile_x;   // (Q, tiles) exposure tile sums, then offsets
   double* tile_f;   // (Q, tiles) free-current tile sums, then offsets
   double* partials; // (1 + kMaxScored, blocks) masked squared sums
@@ -92,6 +96,9 @@ void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuf
 // Commits the held trial into store slot `slot`; reads no record, so the host may go on
 // filling the next trial's record while it runs.
 void launch_adams_accept(const DeviceProblem& P, int slot, bool compensated, FieldBuffers B,
+                         hipStream_t s);
+// The commit with its slot read from the record (`free_slot`).
+void launch_adams_commit(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                          hipStream_t s);
 void launch_adams_start(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                         hipStream_t s);

@@ -153,10 +153,16 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   B_.streams = alloc<cplx>(3UL * C * (T / 2));
   const std::size_t tiles = static_cast<std::size_t>(scan_tiles(T));
   B_.rate = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
+  B_.frac = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
+  B_.free_local = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
   B_.tile_x = alloc<double>(static_cast<std::size_t>(Q) * tiles);
   B_.tile_f = alloc<double>(static_cast<std::size_t>(Q) * tiles);
   n_blocks_ = static_cast<int>(correct_blocks(P_));
-  B_.partials = alloc<double>((1 + kMaxScored) * static_cast<std::size_t>(n_blocks_));
+  // The corrector's block sums land straight in mapped pinned host memory: the host
+  // reads them after the trial's one synchronisation, with no copy on the stream.
+  SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&partials_host_),
+                       sizeof(double) * (1 + kMaxScored) * n_blocks_, hipHostMallocMapped));
+  SP_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&B_.partials), partials_host_, 0));
   const auto roots = unit_roots(static_cast<std::size_t>(T));
   B_.roots_T = upload(roots.data(), roots.size());
   scratch_ = alloc<cplx>(3UL * C * (T / 2));
@@ -165,9 +171,8 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   rec_dev_ = alloc<AdamsRecord>(1);
   SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&rec_host_), sizeof(AdamsRecord)));
   std::memset(rec_host_, 0, sizeof(AdamsRecord));
-  SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&partials_host_),
-                       sizeof(double) * (1 + kMaxScored) * n_blocks_));
   for (hipEvent_t& e : marks_) SP_HIP(hipEventCreate(&e));
+  SP_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
   slot_rates_.assign(S, std::vector<double>(kMaxTerms, 0.0));
 }
 
@@ -183,6 +188,8 @@ void FieldWorkspace::release() {
     if (e) (void)hipEventDestroy(e);
     e = nullptr;
   }
+  if (done_) (void)hipEventDestroy(done_);
+  done_ = nullptr;
   if (stream_) (void)hipStreamDestroy(stream_);
   graphs_.clear();
   allocations_.clear();
@@ -218,7 +225,10 @@ void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
   mark(10);
 }
 
-void FieldWorkspace::enqueue_trial() {
+void FieldWorkspace::enqueue_trial(bool commit) {
+  // An accepted step's commit rides at the head of the next trial (its slot is in the
+  // record), so that the host launches one graph per attempt.
+  if (commit) launch_adams_commit(P_, rec_dev_, B_, stream_);
   mark(0);
   launch_adams_predict(P_, rec_dev_, B_, stream_);
   mark(1);
@@ -232,36 +242,54 @@ void FieldWorkspace::enqueue_start() {
   launch_adams_start(P_, rec_dev_, B_, stream_);
 }
 
-void FieldWorkspace::run(int key) {
+void FieldWorkspace::enqueue(int key) {
+  // The record's upload is the graph's first node: the host fills the pinned record,
+  // then launches one graph.
   SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
                         stream_));
+  if (key == kStart)
+    enqueue_start();
+  else
+    enqueue_trial(key == kCommitTrial);
+}
+
+void FieldWorkspace::run(int key) {
   // While profiling, the trial runs as direct launches with an event between kernels
   // (events recorded by graph nodes cannot be timed); the kernels are the same.
-  marking_ = profile_ && key == 1;
-  const int graph_key = key;
+  marking_ = profile_ && key != kStart;
   if (!use_graphs_ || marking_) {
-    if (key == 1)
-      enqueue_trial();
-    else
-      enqueue_start();
-  } else {
-    auto it = graphs_.find(graph_key);
-    if (it == graphs_.end()) {
-      hipGraph_t graph;
-      SP_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-      if (key == 1)
-        enqueue_trial();
-      else
-        enqueue_start();
-      SP_HIP(hipStreamEndCapture(stream_, &graph));
-      hipGraphExec_t exec;
-      SP_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-      SP_HIP(hipGraphDestroy(graph));
-      it = graphs_.emplace(graph_key, exec).first;
-    }
-    SP_HIP(hipGraphLaunch(it->second, stream_));
-    graph_launches_ += 1;
+    enqueue(key);
+    return;
   }
+  auto it = graphs_.find(key);
+  if (it == graphs_.end()) {
+    hipGraph_t graph;
+    SP_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    enqueue(key);
+    SP_HIP(hipStreamEndCapture(stream_, &graph));
+    hipGraphExec_t exec;
+    SP_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    SP_HIP(hipGraphDestroy(graph));
+    it = graphs_.emplace(key, exec).first;
+  }
+  SP_HIP(hipGraphLaunch(it->second, stream_));
+  graph_launches_ += 1;
+}
+
+void FieldWorkspace::wait() {
+  // Spin on the stream's last event: the host's next decision needs this trial's sums
+  // as soon as they land, and a blocking wait adds a wake-up.
+  SP_HIP(hipEventRecord(done_, stream_));
+  hipError_t status;
+  while ((status = hipEventQuery(done_)) == hipErrorNotReady) {
+  }
+  SP_HIP(status);
+}
+
+void FieldWorkspace::flush() {
+  if (pending_slot_ < 0) return;
+  launch_adams_accept(P_, pending_slot_, compensated_, B_, stream_);
+  pending_slot_ = -1;
 }
 
 std::vector<std::pair<const char*, double>> FieldWorkspace::phases() const {
@@ -284,9 +312,10 @@ void FieldWorkspace::fill_position(double zp) {
 // ------------------------------------------------------------------ the engine
 
 void FieldWorkspace::start(double z) {
+  flush();
   fill_position(z);
   for (int t = 0; t < P_.n_phi; ++t) slot_rates_[0][t] = rate_paths_[t](z);
-  run(0);
+  run(kStart);
   SP_HIP(hipStreamSynchronize(stream_));
   evaluations_ += 1;
 }
@@ -350,11 +379,13 @@ std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Sten
     lagrange_basis(points, k + 1, 1.0, true, nullptr, weights);
     for (int j = 0; j <= k; ++j) r.comp_corr_w[i][j] = weights[j];
   }
-  run(1);
-  SP_HIP(hipMemcpyAsync(partials_host_, B_.partials,
-                        sizeof(double) * (1 + kMaxScored) * n_blocks_, hipMemcpyDeviceToHost,
-                        stream_));
-  SP_HIP(hipStreamSynchronize(stream_));
+  r.free_slot = pending_slot_;
+  run(pending_slot_ >= 0 ? kCommitTrial : kTrial);
+  pending_slot_ = -1;
+  if (marking_)
+    SP_HIP(hipStreamSynchronize(stream_));
+  else
+    wait();
   if (marking_) {
     float ms = 0.0f;
     for (int i = 0; i < kPhases; ++i) {
@@ -384,8 +415,8 @@ std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Sten
 
 void FieldWorkspace::accept(double, int free, const adams::Stencil&,
                             const std::vector<adams::Entry>&) {
-  // Enqueued without a wait: the next trial is ordered after it on the same stream.
-  launch_adams_accept(P_, free, compensated_, B_, stream_);
+  // Deferred into the next trial's graph, or flushed before anything reads the state.
+  pending_slot_ = free;
   slot_rates_[free] = held_rate_end_;
   accepted_ += 1;
 }
@@ -395,6 +426,8 @@ void FieldWorkspace::accept(double, int free, const adams::Stencil&,
 void FieldWorkspace::load(double z, const double* field) {
   SP_CHECK(field != nullptr, "load needs a field");
   const std::size_t CN = static_cast<std::size_t>(P_.C) * P_.N;
+  pending_slot_ = -1;
+  SP_HIP(hipStreamSynchronize(stream_));
   SP_HIP(hipMemcpy(B_.y, field, CN * sizeof(cplx), hipMemcpyHostToDevice));
   adams::Engine& engine = *this;
   march_ = std::make_unique<adams::March>(policy_, engine, host_, z, dz_init_);
@@ -422,6 +455,7 @@ void FieldWorkspace::read_trial(double* out) {
 
 void FieldWorkspace::read_state(double* out) {
   if (!loaded_) throw Error(kState, "no state is loaded");
+  flush();
   SP_HIP(hipStreamSynchronize(stream_));
   SP_HIP(hipMemcpy(out, B_.y, sizeof(cplx) * P_.C * P_.N, hipMemcpyDeviceToHost));
 }
@@ -471,8 +505,10 @@ bool FieldWorkspace::advance(double target, long work_limit, std::vector<double>
     done = march_->march_to(target) && march::reached(march_->z, target);
   } catch (...) {
     host_.rows = nullptr;
+    flush();
     throw;
   }
+  flush();
   host_.rows = nullptr;
   z_ = march_->z;
   r->z = march_->z;
@@ -489,6 +525,7 @@ void FieldWorkspace::source(double z, const double* lab, double* out, double* st
   SP_CHECK(lab && out, "the source needs a field and an output");
   const std::size_t CN = static_cast<std::size_t>(P_.C) * P_.N;
   const std::size_t CT = static_cast<std::size_t>(P_.C) * P_.T;
+  flush();
   SP_HIP(hipStreamSynchronize(stream_));
   SP_HIP(hipMemcpy(B_.lab, lab, CN * sizeof(cplx), hipMemcpyHostToDevice));
   fill_position(z);
@@ -524,6 +561,7 @@ long FieldWorkspace::buffer(const char* name, double* out, long count) {
       {"states", B_.states, B_.states ? S * CN2 : 0},
       {"carried", B_.carried, B_.carried ? S * CN2 : 0},
   };
+  flush();
   for (const auto& row : table) {
     if (std::strcmp(row.name, name) != 0) continue;
     if (out && count > 0 && row.ptr) {

@@ -56,9 +56,13 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   T* upload(const T* host, std::size_t n);
   void fill_position(double z);
   void enqueue_source(const cplx* spectrum, cplx* out);
-  void enqueue_trial();
+  enum Graph { kTrial = 1, kStart = 2, kCommitTrial = 3 };
+  void enqueue_trial(bool commit);
   void enqueue_start();
+  void enqueue(int key);
   void run(int key);
+  void wait();
+  void flush();
   void mark(int i);
 
   DeviceProblem P_{};
@@ -96,6 +100,8 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   bool loaded_ = false;
   // Counters and timing.
   long trials_ = 0, evaluations_ = 0, accepted_ = 0, graph_launches_ = 0;
+  int pending_slot_ = -1;  // an accepted step's store slot, committed with the next trial
+  hipEvent_t done_ = nullptr;
   bool profile_ = false, marking_ = false;
   hipEvent_t marks_[kPhases + 1] = {};
   double phase_ms_[kPhases] = {};
