@@ -27,60 +27,56 @@ __device__ inline cplx cdiv(cplx a, cplx b) {
   return make((a.x * rat + a.y) * scl, (a.y * rat - a.x) * scl);
 }
 
-// phi_0..phi_order at one argument (src/solprop/backends/exponential.py:131-226):
+// phi_0..phi_kOrder at one argument (src/solprop/backends/exponential.py:131-226):
 // scaled Taylor series, downward recurrence and doubling near zero; upward recurrence
-// from exp(x) beyond |x| = 3. Every loop over the entries is unrolled to the largest
-// order with the runtime order as a guard, so that `out` stays in registers.
-__device__ void phi_functions(cplx x, int order, cplx (&out)[kMaxAdamsWidth + 1]) {
+// from exp(x) beyond |x| = 3. The order is a template argument, so that every loop has
+// a static bound and `out` stays in registers.
+template <int kOrder>
+__device__ void phi_functions(cplx x, cplx (&out)[kOrder + 1]) {
   if (x.x == 0.0 && x.y == 0.0) {
     // The series at zero, exactly: phi_k(0) = 1/k! (what the Taylor path rounds to).
 #pragma unroll
-    for (int k = 0; k <= kMaxAdamsWidth; ++k) out[k] = make(kRecipFactorial[k], 0.0);
+    for (int k = 0; k <= kOrder; ++k) out[k] = make(kRecipFactorial[k], 0.0);
     return;
   }
   const double magnitude = hypot(x.x, x.y);
   if (magnitude > 3.0) {
     out[0] = cexp(x);
 #pragma unroll
-    for (int k = 1; k <= kMaxAdamsWidth; ++k)
-      if (k <= order) out[k] = cdiv(sub(out[k - 1], make(kRecipFactorial[k - 1], 0.0)), x);
+    for (int k = 1; k <= kOrder; ++k)
+      out[k] = cdiv(sub(out[k - 1], make(kRecipFactorial[k - 1], 0.0)), x);
     return;
   }
   int squarings = 0;
   if (magnitude > 0.5) squarings = static_cast<int>(ceil(log2(fmax(magnitude, 0.5) / 0.5)));
   const double factor = exp2(static_cast<double>(squarings));
   const cplx y = make(x.x / factor, x.y / factor);
-  cplx term = make(kRecipFactorial[order], 0.0);
+  cplx term = make(kRecipFactorial[kOrder], 0.0);
   cplx total = term;
   for (int m = 1; m < 24; ++m) {
     const cplx ty = mul(term, y);
-    const double d = static_cast<double>(m + order);
+    const double d = static_cast<double>(m + kOrder);
     term = make(ty.x / d, ty.y / d);
     total = add(total, term);
   }
+  out[kOrder] = total;
 #pragma unroll
-  for (int k = 0; k <= kMaxAdamsWidth; ++k)
-    if (k == order) out[k] = total;
-#pragma unroll
-  for (int k = kMaxAdamsWidth - 1; k >= 1; --k)
-    if (k < order) out[k] = add(make(kRecipFactorial[k], 0.0), mul(y, out[k + 1]));
+  for (int k = kOrder - 1; k >= 1; --k)
+    out[k] = add(make(kRecipFactorial[k], 0.0), mul(y, out[k + 1]));
   out[0] = cexp(y);
   for (int level = 0; level < squarings; ++level) {
-    cplx doubled[kMaxAdamsWidth + 1];
+    cplx doubled[kOrder + 1];
     doubled[0] = mul(out[0], out[0]);
 #pragma unroll
-    for (int k = 1; k <= kMaxAdamsWidth; ++k) {
-      if (k <= order) {
-        cplx t = mul(out[0], out[k]);
+    for (int k = 1; k <= kOrder; ++k) {
+      cplx t = mul(out[0], out[k]);
 #pragma unroll
-        for (int j = 1; j <= k; ++j) t = add(t, scale(kRecipFactorial[k - j], out[j]));
-        const double p2 = exp2(static_cast<double>(k));
-        doubled[k] = make(t.x / p2, t.y / p2);
-      }
+      for (int j = 1; j <= k; ++j) t = add(t, scale(kRecipFactorial[k - j], out[j]));
+      const double p2 = exp2(static_cast<double>(k));
+      doubled[k] = make(t.x / p2, t.y / p2);
     }
 #pragma unroll
-    for (int k = 0; k <= kMaxAdamsWidth; ++k)
-      if (k <= order) out[k] = doubled[k];
+    for (int k = 0; k <= kOrder; ++k) out[k] = doubled[k];
   }
 }
 
@@ -91,26 +87,18 @@ __device__ inline cplx weighted(const double* w, const cplx* terms, int n, long 
   return total;
 }
 
-// dz Σ_k M_k (Σ_j c_jk X_j) over n_k columns and n_j rows (coefficients [j][k]), in
-// increasing k and j, zero coefficients skipped; unrolled to the largest width.
-__device__ inline cplx apply_moments(const cplx (&M)[kMaxAdamsWidth],
-                                     const double (*c)[kMaxAdamsWidth],
-                                     const cplx (&X)[kMaxAdamsWidth], int n_j, int n_k,
-                                     double dz) {
+// dz Σ_k M_k (Σ_j c_jk X_j) over a W x W table (coefficients [j][k], zero where the
+// stencil is narrower), in increasing k and j.
+template <int W>
+__device__ inline cplx apply_moments(const cplx (&M)[W], const double (*c)[kMaxAdamsWidth],
+                                     const cplx (&X)[W], double dz) {
   cplx total = make(0.0, 0.0);
 #pragma unroll
-  for (int k = 0; k < kMaxAdamsWidth; ++k) {
-    if (k < n_k) {
-      cplx combined = make(0.0, 0.0);
+  for (int k = 0; k < W; ++k) {
+    cplx combined = make(0.0, 0.0);
 #pragma unroll
-      for (int j = 0; j < kMaxAdamsWidth; ++j) {
-        if (j < n_j) {
-          const double w = c[j][k];
-          if (w != 0.0) combined = add(combined, scale(w, X[j]));
-        }
-      }
-      total = add(total, mul(M[k], combined));
-    }
+    for (int j = 0; j < W; ++j) combined = add(combined, scale(c[j][k], X[j]));
+    total = add(total, mul(M[k], combined));
   }
   return scale(dz, total);
 }
@@ -236,66 +224,62 @@ __device__ inline TileColumn tile_column(const DeviceProblem& P, double e, const
 
 // ------------------------------------------------------------- Adams predictor
 
-// The predictor. One wavefront per workgroup; every global operand is loaded before
-// any arithmetic, so that a thread waits on memory once rather than once per operand.
+// The predictor for a stencil of width W (moments M_0..M_{W-1}, W - 1 stored rows). One
+// wavefront per workgroup; every global operand is loaded before any arithmetic.
+// Coefficients past the stencil's order are zero in the record.
+template <int W>
 __global__ void __launch_bounds__(kWave)
-    adams_predict(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
+    adams_predict(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const long CN = static_cast<long>(P.C) * P.N;
   if (idx >= CN) return;
   const AdamsRecord& r = *R;
   const double dz = r.dz;
-  const int width = r.width, order = r.order, n_read = r.n_read;
   const bool compensated = r.compensated != 0;
   // Loads.
-  cplx phi_t[kMaxTerms], F[kMaxAdamsWidth], U[kMaxAdamsWidth];
+  cplx phi_t[kMaxTerms], F[W], U[W];
 #pragma unroll
   for (int t = 0; t < kMaxTerms; ++t) phi_t[t] = t < P.n_phi ? P.phi[t * CN + idx] : make(0.0, 0.0);
   const cplx y = B.y[idx];
 #pragma unroll
-  for (int j = 0; j < kMaxAdamsWidth; ++j) {
-    F[j] = j < order ? B.store[r.slots[j] * CN + idx] : make(0.0, 0.0);
-    U[j] = compensated && j < n_read ? B.states[r.slots[j] * CN + idx] : make(0.0, 0.0);
+  for (int j = 0; j < W; ++j) {
+    F[j] = j < W - 1 ? B.store[r.slots[j] * CN + idx] : make(0.0, 0.0);
+    U[j] = compensated && j < W - 1 ? B.states[r.slots[j] * CN + idx] : make(0.0, 0.0);
   }
   // x = Σ_t w_t Φ_t, in term order.
   cplx x = make(0.0, 0.0);
 #pragma unroll
   for (int t = 0; t < kMaxTerms; ++t)
     if (t < P.n_phi) x = add(x, scale(r.x_w[t], phi_t[t]));
-  cplx phi[kMaxAdamsWidth + 1];
-  phi_functions(x, width, phi);
-  cplx M[kMaxAdamsWidth];
+  cplx phi[W + 1];
+  phi_functions<W>(x, phi);
+  cplx M[W];
 #pragma unroll
-  for (int k = 0; k < kMaxAdamsWidth; ++k) {
-    M[k] = make(0.0, 0.0);
-    if (k < width) {
-      M[k] = scale(kFactorial[k], phi[k + 1]);
-      B.moments[k * CN + idx] = M[k];
-    }
+  for (int k = 0; k < W; ++k) {
+    M[k] = scale(kFactorial[k], phi[k + 1]);
+    B.moments[k * CN + idx] = M[k];
   }
   // exp(0) = 1 exactly: inactive bins (x = 0) skip the transcendental.
   const bool still = x.x == 0.0 && x.y == 0.0;
   const cplx flow = still ? make(1.0, 0.0) : cexp(x);
   B.flow[idx] = flow;
-  cplx lab = add(mul(flow, y), apply_moments(M, r.pred_c, F, order, order, dz));
+  cplx lab = add(mul(flow, y), apply_moments<W>(M, r.pred_c, F, dz));
   if (compensated) {
     const cplx reference = make(x.x / dz, x.y / dz);
     cplx total = make(0.0, 0.0);
 #pragma unroll
-    for (int j = 0; j < kMaxAdamsWidth; ++j) {
-      if (j < n_read) {
-        cplx rate = make(0.0, 0.0);
+    for (int j = 0; j < W - 1; ++j) {
+      cplx rate = make(0.0, 0.0);
 #pragma unroll
-        for (int t = 0; t < kMaxTerms; ++t)
-          if (t < P.n_phi) rate = add(rate, scale(r.slot_rate_w[j][t], phi_t[t]));
-        const cplx term = mul(sub(rate, reference), U[j]);
-        // A zero term (a bin with no field) carries zero whatever it is transported by.
-        const bool empty = term.x == 0.0 && term.y == 0.0;
-        const cplx span = scale(1.0 - r.nodes[j], x);
-        const cplx carried = empty ? make(0.0, 0.0) : mul(cexp(span), term);
-        B.carried[j * CN + idx] = carried;
-        if (j < order) total = add(total, scale(r.comp_pred_w[0][j], carried));
-      }
+      for (int t = 0; t < kMaxTerms; ++t)
+        if (t < P.n_phi) rate = add(rate, scale(r.slot_rate_w[j][t], phi_t[t]));
+      const cplx term = mul(sub(rate, reference), U[j]);
+      // A zero term (a bin with no field) carries zero whatever it is transported by.
+      const bool empty = term.x == 0.0 && term.y == 0.0;
+      const cplx span = scale(1.0 - r.nodes[j], x);
+      const cplx carried = empty ? make(0.0, 0.0) : mul(cexp(span), term);
+      B.carried[j * CN + idx] = carried;
+      total = add(total, scale(r.comp_pred_w[0][j], carried));
     }
     lab = add(lab, scale(dz, total));
   }
@@ -390,7 +374,7 @@ __global__ void __launch_bounds__(kWave) plasma_tile_prefix(int tiles, double* s
 // and within-tile free current for the response pass. Same layout as pass A.
 template <int C>
 __global__ void __launch_bounds__(kWave * kNodeWaves)
-    plasma_current_tiles(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
+    plasma_current_tiles(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
   const int T = P.T, tiles = scan_tiles(T);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const int tile = blockIdx.x;
@@ -423,7 +407,7 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
 // the tile's samples of the three streams written once.
 template <int C>
 __global__ void __launch_bounds__(kWave * kResponseWaves)
-    node_response(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
+    node_response(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
   __shared__ double red[3 * C * kWave];
   const int T = P.T, tiles = scan_tiles(T);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -492,7 +476,8 @@ __global__ void __launch_bounds__(kWave * kResponseWaves)
 
 // Unpack the three stream spectra, assemble the held current on the active bins and
 // normalise: k = (p W) (P~ + G~/(iω) - E~/(iω)^2).
-__global__ void field_finish(DeviceProblem P, const AdamsRecord* R, FieldBuffers B, cplx* out) {
+__global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B,
+                             cplx* out) {
   const long CN = static_cast<long>(P.C) * P.N;
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= CN) return;
@@ -517,11 +502,13 @@ __global__ void field_finish(DeviceProblem P, const AdamsRecord* R, FieldBuffers
 
 // ------------------------------------------------------------- Adams corrector
 
-// The corrector, the defects of the scored orders and their masked squared sums. One
-// wavefront per workgroup, operands loaded first; each wave's sums are reduced across
-// its lanes in a fixed order and written as that workgroup's partials.
+// The corrector, the defects of the scored orders and their masked squared sums, for a
+// stencil of width W. One wavefront per workgroup, operands loaded first; each wave's
+// sums are reduced across its lanes in a fixed order and written as that workgroup's
+// partials. Coefficients past each order are zero in the record.
+template <int W>
 __global__ void __launch_bounds__(kWave)
-    adams_correct(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
+    adams_correct(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
   const long CN = static_cast<long>(P.C) * P.N;
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const AdamsRecord& r = *R;
@@ -529,16 +516,16 @@ __global__ void __launch_bounds__(kWave)
   if (idx < CN) {
     const int b = static_cast<int>(idx % P.N);
     const double dz = r.dz;
-    const int width = r.width, n_read = r.n_read, n_orders = r.n_orders;
+    const int n_orders = r.n_orders;
     const bool compensated = r.compensated != 0;
     // Loads.
-    cplx M[kMaxAdamsWidth], X[kMaxAdamsWidth], carried[kMaxAdamsWidth], phi_t[kMaxTerms];
+    cplx M[W], X[W], carried[W], phi_t[kMaxTerms];
     X[0] = B.source[idx];
 #pragma unroll
-    for (int k = 0; k < kMaxAdamsWidth; ++k) {
-      M[k] = k < width ? B.moments[k * CN + idx] : make(0.0, 0.0);
-      if (k >= 1) X[k] = k < width ? B.store[r.slots[k - 1] * CN + idx] : make(0.0, 0.0);
-      carried[k] = compensated && k < n_read ? B.carried[k * CN + idx] : make(0.0, 0.0);
+    for (int k = 0; k < W; ++k) {
+      M[k] = B.moments[k * CN + idx];
+      if (k >= 1) X[k] = B.store[r.slots[k - 1] * CN + idx];
+      carried[k] = compensated && k < W - 1 ? B.carried[k * CN + idx] : make(0.0, 0.0);
     }
 #pragma unroll
     for (int t = 0; t < kMaxTerms; ++t)
@@ -548,7 +535,7 @@ __global__ void __launch_bounds__(kWave)
     const cplx lab = compensated ? B.lab[idx] : make(0.0, 0.0);
     const bool scored = P.normmask[b] != 0;
 
-    cplx adv = add(mul(flow, y), apply_moments(M, r.corr_c, X, width, width, dz));
+    cplx adv = add(mul(flow, y), apply_moments<W>(M, r.corr_c, X, dz));
     cplx comp_def[kMaxScored];
 #pragma unroll
     for (int i = 0; i < kMaxScored; ++i) comp_def[i] = make(0.0, 0.0);
@@ -565,15 +552,13 @@ __global__ void __launch_bounds__(kWave)
 #pragma unroll
       for (int i = 0; i < kMaxScored; ++i) {
         if (i < n_orders) {
-          const int k = r.orders[i];
           cplx corr = make(0.0, 0.0), pred = make(0.0, 0.0);
 #pragma unroll
-          for (int j = 0; j < kMaxAdamsWidth; ++j)
-            if (j < k) corr = add(corr, scale(r.comp_corr_w[i][1 + j], carried[j]));
+          for (int j = 0; j < W - 1; ++j)
+            corr = add(corr, scale(r.comp_corr_w[i][1 + j], carried[j]));
           corr = scale(dz, add(corr, scale(r.comp_corr_w[i][0], endpoint)));
 #pragma unroll
-          for (int j = 0; j < kMaxAdamsWidth; ++j)
-            if (j < k) pred = add(pred, scale(r.comp_pred_w[i][j], carried[j]));
+          for (int j = 0; j < W - 1; ++j) pred = add(pred, scale(r.comp_pred_w[i][j], carried[j]));
           pred = scale(dz, pred);
           comp_def[i] = sub(corr, pred);
           if (i == 0) adv = add(adv, corr);
@@ -586,7 +571,7 @@ __global__ void __launch_bounds__(kWave)
 #pragma unroll
       for (int i = 0; i < kMaxScored; ++i) {
         if (i < n_orders) {
-          cplx d = apply_moments(M, r.rows_c[i], X, width, width, dz);
+          cplx d = apply_moments<W>(M, r.rows_c[i], X, dz);
           if (compensated) d = add(d, comp_def[i]);
           v[1 + i] = norm2(d);
         }
@@ -611,7 +596,7 @@ __global__ void adams_accept(DeviceProblem P, int slot, int compensated, FieldBu
 }
 
 // The same commit, its slot read from the record: the head of the next trial's graph.
-__global__ void adams_commit(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
+__global__ void adams_commit(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
   const long CN = static_cast<long>(P.C) * P.N;
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= CN) return;
@@ -621,7 +606,7 @@ __global__ void adams_commit(DeviceProblem P, const AdamsRecord* R, FieldBuffers
   B.y[idx] = P.active[idx % P.N] ? B.advanced[idx] : make(0.0, 0.0);
 }
 
-__global__ void adams_start(DeviceProblem P, const AdamsRecord* R, FieldBuffers B) {
+__global__ void adams_start(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
   const long CN = static_cast<long>(P.C) * P.N;
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= CN) return;
@@ -643,8 +628,17 @@ unsigned cn_blocks(const DeviceProblem& P) {
 }  // namespace
 
 void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(adams_predict, dim3(correct_blocks(P)), dim3(kWave), 0, s, P, R, B);
+                          int width, hipStream_t s) {
+  SP_CHECK(width >= 2 && width <= kMaxAdamsWidth - 1, "stencil width 2..7");
+  const dim3 grid(correct_blocks(P)), block(kWave);
+  switch (width) {
+    case 2: hipLaunchKernelGGL(adams_predict<2>, grid, block, 0, s, P, R, B); break;
+    case 3: hipLaunchKernelGGL(adams_predict<3>, grid, block, 0, s, P, R, B); break;
+    case 4: hipLaunchKernelGGL(adams_predict<4>, grid, block, 0, s, P, R, B); break;
+    case 5: hipLaunchKernelGGL(adams_predict<5>, grid, block, 0, s, P, R, B); break;
+    case 6: hipLaunchKernelGGL(adams_predict<6>, grid, block, 0, s, P, R, B); break;
+    default: hipLaunchKernelGGL(adams_predict<7>, grid, block, 0, s, P, R, B); break;
+  }
   check();
 }
 
@@ -714,8 +708,17 @@ void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuff
 }
 
 void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(adams_correct, dim3(correct_blocks(P)), dim3(kWave), 0, s, P, R, B);
+                          int width, hipStream_t s) {
+  SP_CHECK(width >= 2 && width <= kMaxAdamsWidth - 1, "stencil width 2..7");
+  const dim3 grid(correct_blocks(P)), block(kWave);
+  switch (width) {
+    case 2: hipLaunchKernelGGL(adams_correct<2>, grid, block, 0, s, P, R, B); break;
+    case 3: hipLaunchKernelGGL(adams_correct<3>, grid, block, 0, s, P, R, B); break;
+    case 4: hipLaunchKernelGGL(adams_correct<4>, grid, block, 0, s, P, R, B); break;
+    case 5: hipLaunchKernelGGL(adams_correct<5>, grid, block, 0, s, P, R, B); break;
+    case 6: hipLaunchKernelGGL(adams_correct<6>, grid, block, 0, s, P, R, B); break;
+    default: hipLaunchKernelGGL(adams_correct<7>, grid, block, 0, s, P, R, B); break;
+  }
   check();
 }
 

@@ -80,8 +80,10 @@ struct PlasmaTable {
 
 unsigned correct_blocks(const DeviceProblem& P);
 
+// The predictor and corrector are compiled per stencil width (2..7); `width` must be
+// the record's.
 void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
-                          hipStream_t s);
+                          int width, hipStream_t s);
 void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuffers B,
                           hipStream_t s);
 // `marks`, when given, holds five events recorded after the rate and exposure tiles,
@@ -92,7 +94,7 @@ void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const Plas
 void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                          cplx* out, hipStream_t s);
 void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
-                          hipStream_t s);
+                          int width, hipStream_t s);
 // Commits the held trial into store slot `slot`; reads no record, so the host may go on
 // filling the next trial's record while it runs.
 void launch_adams_accept(const DeviceProblem& P, int slot, bool compensated, FieldBuffers B,

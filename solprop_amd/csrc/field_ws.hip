@@ -230,10 +230,10 @@ void FieldWorkspace::enqueue_trial(bool commit) {
   // record), so that the host launches one graph per attempt.
   if (commit) launch_adams_commit(P_, rec_dev_, B_, stream_);
   mark(0);
-  launch_adams_predict(P_, rec_dev_, B_, stream_);
+  launch_adams_predict(P_, rec_dev_, B_, width_, stream_);
   mark(1);
   enqueue_source(B_.lab, B_.source);
-  launch_adams_correct(P_, rec_dev_, B_, stream_);
+  launch_adams_correct(P_, rec_dev_, B_, width_, stream_);
   mark(11);
 }
 
@@ -247,16 +247,17 @@ void FieldWorkspace::enqueue(int key) {
   // then launches one graph.
   SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
                         stream_));
-  if (key == kStart)
+  const int kind = key & 15;
+  if (kind == kStart)
     enqueue_start();
   else
-    enqueue_trial(key == kCommitTrial);
+    enqueue_trial(kind == kCommitTrial);
 }
 
 void FieldWorkspace::run(int key) {
   // While profiling, the trial runs as direct launches with an event between kernels
   // (events recorded by graph nodes cannot be timed); the kernels are the same.
-  marking_ = profile_ && key != kStart;
+  marking_ = profile_ && (key & 15) != kStart;
   if (!use_graphs_ || marking_) {
     enqueue(key);
     return;
@@ -338,6 +339,12 @@ std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Sten
   }
   r.width = 1 + top;
   r.n_read = r.width - 1;
+  SP_CHECK(r.width >= 2 && r.width <= kMaxAdamsWidth - 1, "stencil width outside 2..7");
+  width_ = r.width;
+  // Unused slots point at row 0 and unused nodes are zero: the kernels read the whole
+  // width with zero coefficients past each order.
+  std::fill(r.slots, r.slots + kMaxAdamsWidth, 0);
+  std::fill(r.nodes, r.nodes + kMaxAdamsWidth, 0.0);
   SP_CHECK(r.width <= kMaxAdamsWidth, "the stencil is wider than the device record");
   for (int j = 0; j < r.n_read; ++j) {
     r.slots[j] = s.rows[j];
@@ -380,7 +387,7 @@ std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Sten
     for (int j = 0; j <= k; ++j) r.comp_corr_w[i][j] = weights[j];
   }
   r.free_slot = pending_slot_;
-  run(pending_slot_ >= 0 ? kCommitTrial : kTrial);
+  run((pending_slot_ >= 0 ? kCommitTrial : kTrial) + 16 * width_);
   pending_slot_ = -1;
   if (marking_)
     SP_HIP(hipStreamSynchronize(stream_));

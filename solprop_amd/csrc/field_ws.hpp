@@ -101,6 +101,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   // Counters and timing.
   long trials_ = 0, evaluations_ = 0, accepted_ = 0, graph_launches_ = 0;
   int pending_slot_ = -1;  // an accepted step's store slot, committed with the next trial
+  int width_ = 2;          // the current trial's stencil width
   hipEvent_t done_ = nullptr;
   bool profile_ = false, marking_ = false;
   hipEvent_t marks_[kPhases + 1] = {};
