@@ -267,6 +267,12 @@ int sp_adams_trace(const sp_adams_scenario* scenario, char* out, size_t outlen);
  * native/src/bindings/testing.cpp): one batched complex FFT of `n` points over
  * `batch` contiguous rows, unnormalised, sign -1 forward or +1 inverse, through the
  * library's own Stockham plan; host arrays in and out (complex, batch x n). */
+/* C (M x N) = A (M x K) . B (K x N), row-major FP64 host arrays, on the matrix cores: the
+ * radial route's Hankel-transform GEMM, exposed as the reference's `_testing` module
+ * exposes its kernels (native/src/bindings/testing.cpp:732). */
+int sp_testing_gemm(long M, long N, long K, const double* A, const double* B, double* C,
+                    char* msg, size_t msglen);
+
 int sp_testing_fft(const double* in, double* out, long n, long batch, int sign, char* msg,
                    size_t msglen);
 

@@ -152,6 +152,8 @@ SIGNATURES = {
     "sp_rk_trace": (_I, [C.POINTER(RKScenarioC), C.c_char_p, C.c_size_t]),
     "sp_adams_trace": (_I, [C.POINTER(AdamsScenarioC), C.c_char_p, C.c_size_t]),
     "sp_testing_fft": (_I, [_PD, _PD, C.c_long, C.c_long, _I, C.c_char_p, C.c_size_t]),
+    "sp_testing_gemm": (_I, [C.c_long, C.c_long, C.c_long, _PD, _PD, _PD, C.c_char_p,
+                             C.c_size_t]),
     "sp_version": (C.c_char_p, []),
 }
 

@@ -20,7 +20,7 @@ CSRC = HERE / "csrc"
 OUT = HERE / "libsolprop_mi355x.so"
 ARCH = "gfx950"
 
-HIP_SOURCES = ["fft.hip", "stage.hip", "field.hip", "adams.hip", "workspace.hip",
+HIP_SOURCES = ["fft.hip", "gemm.hip", "stage.hip", "field.hip", "adams.hip", "workspace.hip",
                "field_ws.hip"]
 HOST_SOURCES = ["capi.cpp"]
 STRICT_SOURCES = ["march.cpp", "trace.cpp", "scalars.cpp", "adams_march.cpp",

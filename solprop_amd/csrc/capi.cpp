@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "field_ws.hpp"
+#include "gemm.hpp"
 #include "trace.hpp"
 #include "workspace.hpp"
 
@@ -211,6 +212,32 @@ int sp_testing_fft(const double* in, double* out, long n, long batch, int sign, 
     SP_HIP(hipMemcpy(out, data, bytes, hipMemcpyDeviceToHost));
     SP_HIP(hipFree(data));
     SP_HIP(hipFree(scratch));
+    SP_HIP(hipStreamDestroy(s));
+  });
+}
+
+int sp_testing_gemm(long M, long N, long K, const double* A, const double* B, double* C,
+                    char* msg, size_t msglen) {
+  if (!A || !B || !C) return SP_VALUE;
+  return guarded(msg, msglen, [&] {
+    SP_CHECK(M > 0 && N > 0 && K > 0 && M * K < (1L << 28) && K * N < (1L << 28) &&
+                 M * N < (1L << 28),
+             "GEMM dimensions must be positive and below 2^28 elements per operand");
+    double *a = nullptr, *b = nullptr, *c = nullptr;
+    hipStream_t s = nullptr;
+    SP_HIP(hipStreamCreate(&s));
+    SP_HIP(hipMalloc(&a, sizeof(double) * M * K));
+    SP_HIP(hipMalloc(&b, sizeof(double) * K * N));
+    SP_HIP(hipMalloc(&c, sizeof(double) * M * N));
+    SP_HIP(hipMemcpy(a, A, sizeof(double) * M * K, hipMemcpyHostToDevice));
+    SP_HIP(hipMemcpy(b, B, sizeof(double) * K * N, hipMemcpyHostToDevice));
+    sp::launch_dgemm(static_cast<int>(M), static_cast<int>(N), static_cast<int>(K), a, K, b, N,
+                     c, N, s);
+    SP_HIP(hipStreamSynchronize(s));
+    SP_HIP(hipMemcpy(C, c, sizeof(double) * M * N, hipMemcpyDeviceToHost));
+    SP_HIP(hipFree(a));
+    SP_HIP(hipFree(b));
+    SP_HIP(hipFree(c));
     SP_HIP(hipStreamDestroy(s));
   });
 }

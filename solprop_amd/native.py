@@ -395,6 +395,19 @@ class Session:
         return self._ws.export(weight, optics)
 
 
+def testing_gemm(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """``a @ b`` in FP64 on the matrix cores (the radial route's Hankel GEMM)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    (m, k), (k2, n) = a.shape, b.shape
+    if k != k2:
+        raise ValueError("inner dimensions differ")
+    out = np.zeros((m, n))
+    msg = C.create_string_buffer(1024)
+    _check(_lib.lib().sp_testing_gemm(m, n, k, _ptr(a), _ptr(b), _ptr(out), msg, 1024), msg)
+    return out
+
+
 def testing_fft(x: np.ndarray, sign: int) -> np.ndarray:
     """The library's batched complex FFT over the last axis (unnormalised)."""
     arr = np.ascontiguousarray(np.asarray(x, dtype=np.complex128))

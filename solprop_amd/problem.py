@@ -703,3 +703,158 @@ def capillary_problem(
             "stepper": "exponential Adams, variable order <= 5",
         },
     )
+
+
+# ------------------------------------------------------------ free-space radial beam
+
+AR_SELLMEIER = ((20332.29e-8, 206.12e-6), (34458.31e-8, 8.066e-3))
+"""Argon's refractivity at 273.15 K and 1 bar, ``n - 1 = Σ B λ²/(λ² - C)`` with λ in µm
+(A. Börzsönyi et al., Appl. Opt. 47, 4856 (2008))."""
+AR_IONISATION_EV = 15.7596119
+AR_N2_PER_BAR = 1.0e-23  # m^2/W at 1 bar (argon's Kerr index; Wahlstrand et al.)
+
+
+def qdht(radius: float, n_r: int):
+    """The raw order-zero quasi-discrete Hankel transform (``hankel.py:29-190``).
+
+    M. Guizar-Sicairos and J. C. Gutiérrez-Vega, JOSA A 21, 53 (2004): with ``j_n`` the
+    zeros of J0, ``S = j_{N+1}`` and ``J_n = |J1(j_n)|``, the symmetric matrix
+    ``T_mn = 2 J0(j_m j_n / S) / (S J_m J_n)`` acts between the diagonal scalings
+    ``J``: the forward transform of samples at ``r_n = R j_n / S`` is
+    ``H_m = (R² / S) J_m Σ_n T_mn f_n / J_n`` (the quadrature of ``∫ r f J0(k r) dr`` at
+    ``k_m = j_m / R``), the inverse the same with ``S / R²``.
+    Returns ``(r, k_r, forward, inverse)``, the last two as (N, N) matrices acting on
+    column vectors.
+    """
+    from scipy.special import jn_zeros, jv
+
+    zeros = jn_zeros(0, n_r + 1)
+    edge = float(zeros[-1])
+    j1 = np.abs(jv(1, zeros[:-1]))
+    matrix = 2 * jv(0, np.outer(zeros[:-1], zeros[:-1]) / edge)
+    matrix /= edge * np.outer(j1, j1)
+    forward_scale = radius * radius / edge
+    conjugated = j1[:, None] * matrix / j1[None, :]
+    return (radius * zeros[:-1] / edge, zeros[:-1] / radius, forward_scale * conjugated,
+            conjugated / forward_scale)
+
+
+def radial_problem(
+    *,
+    n_t: int = 2**13,
+    window: float = 1.0e-12,
+    n_r: int = 512,
+    radius: float = 4.0e-3,
+    length: float = 0.5,
+    pressure: float = 1.0,
+    wavelength: float = 800e-9,
+    fwhm: float = 100e-15,
+    w0: float = 1.0e-3,
+    energy: float = 2.0e-3,
+    band=(200e-9, 4e-6),
+    temperature: float = 294.0,
+    rtol: float = 1e-6,
+    plasma: bool = True,
+    kerr: bool = True,
+    time_collar: float = 0.05,
+) -> Problem:
+    """Free-space radially symmetric beam in argon on a real field grid.
+
+    The state is the Hankel spectrum ``H(k_r, ω)`` at the QDHT's transverse frequencies
+    (one "mode" per k_r sample); the nonlinear response acts at the QDHT's radial samples,
+    so the synthesis is the inverse transform and the projection the forward one, both
+    dense N_r x N_r matrices. Per mode the linear operator is ``Φ = -i (k_z - ω/c) z`` with
+    ``k_z = sqrt(k² - k_r²)`` (evanescent components decay), the prefactor
+    ``-i ω² / (2 ε0 c² k_z)`` on propagating components. Kerr from argon's n2 and
+    tunnelling ionisation (ADK, argon's potential); argon, an atom, has no rotational
+    Raman response. Constant density, so one operator term.
+    """
+    n_to = n_t
+    dt = window / n_t
+    n = n_t // 2 + 1
+    omega = 2 * np.pi * np.arange(n) / (n_t * dt)
+    c = C_LIGHT
+    active, omega_window = _band(omega, band)
+    norm_mask = active.copy()
+    t = (np.arange(n_to) - n_to // 2) * dt
+    width = time_collar * window
+    edge = 0.5 * window - width
+    g = np.ones(n_to)
+    gp = np.zeros(n_to)
+    a = np.abs(t) > edge
+    s = (np.abs(t[a]) - edge) / width
+    g[a] = np.where(s < 1, 0.5 + 0.5 * np.cos(np.pi * s), 0.0)
+    gp[a] = np.where(s < 1, -0.5 * np.pi / width * np.sin(np.pi * s) * np.sign(t[a]), 0.0)
+    r, k_r, forward, inverse = qdht(radius, n_r)
+    # Argon's index at this pressure and temperature.
+    safe = np.where(omega > 0, omega, 1.0)
+    lam_um = 2 * np.pi * c / safe * 1e6
+    refr = np.zeros(n)
+    for b_coef, c_coef in AR_SELLMEIER:
+        refr += np.where(omega > 0, b_coef * lam_um**2 / (lam_um**2 - c_coef), 0.0)
+    refr *= pressure * 273.15 / temperature
+    k = omega * (1.0 + refr) / c
+    kz2 = k[None, :] ** 2 - k_r[:, None] ** 2
+    propagating = kz2 > 0
+    kz = np.where(propagating, np.sqrt(np.abs(kz2)), -1j * np.sqrt(np.abs(kz2)))
+    phi0 = np.where(active[None, :], -1j * (kz - omega[None, :] / c), 0.0)
+    phi_terms = np.ascontiguousarray(phi0.reshape(1, n_r, n).astype(np.complex128))
+    safe_kz = np.where(propagating, kz.real, 1.0)
+    pref = np.where(active[None, :] & propagating,
+                    -1j * omega[None, :] ** 2 / (2 * EPS0 * c**2 * safe_kz), 0.0)
+    prefactor = np.ascontiguousarray(pref.reshape(1, n_r, n).astype(np.complex128))
+    # Launch: a Gaussian beam with a Gaussian envelope, its energy from the real field.
+    omega0 = 2 * np.pi * c / wavelength
+    tau = fwhm / math.sqrt(2 * math.log(2))
+    e0 = math.sqrt(energy / (0.5 * EPS0 * c * (math.pi * w0**2 / 2) * tau * math.sqrt(math.pi / 2)))
+    field_rt = (e0 * np.exp(-(r[:, None] / w0) ** 2) * np.exp(-(t[None, :] / tau) ** 2)
+                * np.cos(omega0 * t[None, :]))
+    spectrum_r = np.fft.rfft(field_rt, axis=-1) * active[None, :]
+    field0 = np.ascontiguousarray(forward @ spectrum_r)
+    density = 1e5 * pressure / (KB * temperature)
+    kerr_group = None
+    if kerr:
+        chi3 = (4.0 / 3.0) * AR_N2_PER_BAR * pressure * EPS0 * c
+        kerr_group = KerrGroup(EPS0 * chi3 / density, "real")
+    plasma_group = None
+    if plasma:
+        plasma_group = rate_table(lambda f: adk_rate(f, AR_IONISATION_EV), 1e9, 2e12)
+    return Problem(
+        kind="field",
+        n_t=n_t,
+        n_to=n_to,
+        n_modes=n_r,
+        n_nodes=n_r,
+        crop_index=np.arange(n, dtype=np.int64),
+        active_mask=active,
+        norm_mask=norm_mask,
+        omega_window=omega_window,
+        omega=omega,
+        time_window=g,
+        time_window_derivative=gp,
+        dt_fine=dt,
+        z_end=length,
+        phi_terms=phi_terms,
+        phi_paths=(ScalarPath.position(),),
+        prefactor_terms=prefactor,
+        prefactor_paths=(ScalarPath.constant(1.0),),
+        synthesis=np.ascontiguousarray(inverse),
+        projection=np.ascontiguousarray(forward),
+        density=ScalarPath.constant(density),
+        preionisation=ScalarPath.constant(0.0),
+        kerr=kerr_group,
+        raman=None,
+        plasma=plasma_group,
+        field0=field0,
+        noise0=None,
+        noise_model="none",
+        rtol=rtol,
+        method="adams",
+        description={
+            "route": "free-space radial beam, QDHT field grid",
+            "n_t": n_t, "n_omega": n, "window_s": window, "n_r": n_r, "radius_m": radius,
+            "length_m": length, "pressure_bar": pressure, "gas": "Ar",
+            "wavelength_m": wavelength, "fwhm_s": fwhm, "w0_m": w0, "energy_J": energy,
+            "rtol": rtol, "stepper": "exponential Adams, variable order <= 5",
+        },
+    )
