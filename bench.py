@@ -9,11 +9,13 @@ and prints one JSON line. The inputs are resident in HBM before the timed region
 The default workload is the gas-filled capillary: 4 coupled HE_1m modes of a 125 um
 bore, 1.5 m of helium at 0.8 -> 0 bar, a real field of 2^14 samples over 400 fs (8193
 bins), a 7.5 fs 800 nm 275 uJ Gaussian, a 4-mode x 128-node radial quadrature, Kerr and
-tunnelling ionisation, the exponential Adams march at rtol 1e-6. ``--workload envelope``
-runs the single-mode GNLSE fibre case (2^13 bins, DOPRI5) instead.
+tunnelling ionisation, the exponential Adams march at rtol 1e-6. ``--workload radial``
+runs the free-space argon beam on a 512-point Hankel transform (its synthesis and
+projection as FP64 matrix-core GEMMs), ``--workload envelope`` the single-mode GNLSE
+fibre case (2^13 bins, DOPRI5).
 
 Beside the rate: the roofline figure of the trial's dominant kernel (its algorithmic
-bytes over its average duration), every kernel's duration, and the CPU oracle timed on
+bytes, or for a GEMM its flops, over its average duration), every kernel's duration, and the CPU oracle timed on
 a bounded sample of the same workload on this host's cores. The timed region replays
 the captured trial graphs; the kernel durations come from HIP events recorded between
 consecutive kernels on the workspace's stream while the same march continues for
@@ -31,9 +33,10 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
-def capillary_bytes(problem, order):
+def trial_bytes(problem, order):
     """Algorithmic HBM bytes of each kernel of one Adams trial at order `order`.
 
     Each array the kernel must read or write once, at its stored width: complex (C, N)
@@ -41,7 +44,9 @@ def capillary_bytes(problem, order):
     scan tiles' sums, the (Q, C) synthesis and (C, Q) projection tables; the rate-table
     lookups come from cache. W = order + 1 moments; the corrector reads W - 1 stored
     source rows and the predictor `order` of them; a varying operator adds the
-    transported states of the rows read.
+    transported states of the rows read. With more than 8 modes (the dense route) the
+    node passes read the node field the synthesis GEMM wrote, and the response writes
+    two node streams for the projection GEMM.
     """
     C, N, T, Q = problem.n_modes, problem.n_omega, problem.n_to, problem.n_nodes
     n_phi = len(problem.phi_paths)
@@ -53,20 +58,37 @@ def capillary_bytes(problem, order):
     qk = 8 * Q * tiles
     table = 8 * Q * C
     comp = n_phi > 1
+    dense = C > 8
+    plasma = problem.plasma is not None
+    n_streams = 3 if plasma else 1
+    field = qt if dense else ct + table  # what a node pass reads to know E at its nodes
     return {
         "adams_predict": cn * (n_phi + 1 + order + (order if comp else 0))
         + cn * (W + 2 + (order if comp else 0)),
         "field_scatter": cn + 8 * T + ct,
         "inverse_fft": 2 * ct,
-        "plasma_rate_tiles": ct + table + qt + qk,
+        "synthesis_gemm": table + ct + qt if dense else 0,
+        "plasma_rate_tiles": field + qt + qk,
         "plasma_tile_prefix": 2 * qk,
-        "plasma_current_tiles": ct + table + qt + 2 * qk + 2 * qt,
+        "plasma_current_tiles": field + qt + 2 * qk + 2 * qt,
         "plasma_tile_prefix_current": 2 * qk,
-        "node_response": ct + 2 * table + 3 * qt + qk + 2 * 8 * T + 3 * ct,
-        "forward_fft": 2 * 3 * ct,
-        "field_finish": 3 * ct + cn * len(problem.prefactor_paths) + cn,
+        "node_response": (qt + (3 * qt + qk if plasma else 0) + (2 if plasma else 1) * qt)
+        if dense else ct + 2 * table + 3 * qt + qk + 2 * 8 * T + 3 * ct,
+        "projection_gemm": table + (2 if plasma else 1) * qt + 2 * 8 * T + n_streams * ct
+        if dense else 0,
+        "forward_fft": 2 * n_streams * ct,
+        "field_finish": n_streams * ct + cn * len(problem.prefactor_paths) + cn,
         "adams_correct": cn * (W + 1 + (W - 1) + 2) + cn,
     }
+
+
+def trial_flops(problem):
+    """Multiply-add flops of the dense route's two GEMMs (2 per product term)."""
+    C, T, Q = problem.n_modes, problem.n_to, problem.n_nodes
+    if C <= 8:
+        return {}
+    width = 2 * T if problem.plasma is not None else T
+    return {"synthesis_gemm": 2.0 * Q * C * T, "projection_gemm": 2.0 * C * Q * width}
 
 
 def workload_capillary():
@@ -78,6 +100,21 @@ def workload_capillary():
                     "0.8->0 bar, real field 2^14 samples (8193 bins) over 400 fs, 7.5 fs "
                     "800 nm 275 uJ Gaussian, 4x128 radial quadrature, Kerr + tunnelling "
                     "plasma, exponential Adams (order <= 5) rtol 1e-6",
+        "n_t": problem.n_t, "n_modes": problem.n_modes, "n_nodes": problem.n_nodes,
+        "stepper": "exponential Adams PEC, variable order", "parallelism": "single GPU",
+    }
+    return problem, config
+
+
+def workload_radial():
+    from solprop_amd.problem import radial_problem
+
+    problem = radial_problem()
+    config = {
+        "workload": "free-space radial beam in argon, 1 bar, 0.5 m; 100 fs 800 nm Gaussian, "
+                    "w0 1 mm, 2 mJ; 512-point Hankel transform (dense 512x512) over 4097 "
+                    "frequency columns (2^13 samples over 1 ps); Kerr + tunnelling plasma; "
+                    "FP64 matrix-core GEMMs; exponential Adams (order <= 5) rtol 1e-6",
         "n_t": problem.n_t, "n_modes": problem.n_modes, "n_nodes": problem.n_nodes,
         "stepper": "exponential Adams PEC, variable order", "parallelism": "single GPU",
     }
@@ -101,10 +138,8 @@ def run_gpu(args):
 
     from solprop_amd.native import Workspace
 
-    if args.workload == "capillary":
-        problem, config = workload_capillary()
-    else:
-        problem, config = workload_envelope()
+    problem, config = {"capillary": workload_capillary, "radial": workload_radial,
+                       "envelope": workload_envelope}[args.workload]()
     if args.copy_probe_mib > 0:
         # A plain device copy of known size, three times, for scaling memory counters.
         n = args.copy_probe_mib * (1 << 20) // 8
@@ -150,19 +185,30 @@ def run_gpu(args):
     config["z_span_m"] = [z0, z1]
     value = steps / elapsed
     if phases:
-        algorithmic = capillary_bytes(problem, problem.adams_order)
-        kernels = {
-            name: {"avg_us": us, "bytes": algorithmic[name],
+        algorithmic = trial_bytes(problem, problem.adams_order)
+        flops = trial_flops(problem)
+        kernels = {}
+        for name, us in phases.items():
+            row = {"avg_us": us, "bytes": algorithmic[name],
                    "GB/s": algorithmic[name] / (us * 1e-6) / 1e9 if us > 0 else None}
-            for name, us in phases.items()
-        }
+            if name in flops:
+                row["flops"] = flops[name]
+                row["TFLOP/s"] = flops[name] / (us * 1e-6) / 1e12 if us > 0 else None
+            kernels[name] = row
         dominant = max(phases, key=phases.get)
-        achieved = kernels[dominant]["GB/s"]
+        if dominant in flops:
+            achieved = kernels[dominant]["TFLOP/s"]
+            bound, peak, unit = "mfma", FP64_MFMA_PEAK_TFLOPS, "TFLOP/s"
+            per_launch = {"flops_per_launch": flops[dominant]}
+        else:
+            achieved = kernels[dominant]["GB/s"]
+            bound, peak, unit = "hbm", HBM_PEAK_GBS, "GB/s"
+            per_launch = {"bytes_per_launch": algorithmic[dominant]}
         roofline = {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
-            "traffic": None, "kernel": dominant,
-            "bytes_per_launch": algorithmic[dominant], "avg_launch_us": phases[dominant],
+            "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": None if achieved is None else achieved / peak,
+            "traffic": None, "kernel": dominant, **per_launch,
+            "avg_launch_us": phases[dominant],
             "trial_us": trial_us, "kernels": kernels, "profiled_trials": trials,
             "timing": "HIP events between consecutive kernels on the workspace stream, "
                       "over the profiled trials that follow the timed region",
@@ -227,7 +273,8 @@ def main():
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=300)
     parser.add_argument("--warmup", type=int, default=30)
-    parser.add_argument("--workload", choices=["capillary", "envelope"], default="capillary")
+    parser.add_argument("--workload", choices=["capillary", "radial", "envelope"],
+                        default="capillary")
     parser.add_argument("--profile-steps", type=int, default=100)
     parser.add_argument("--copy-probe-mib", type=int, default=0,
                         help="run a plain device copy of this many MiB first (counter scaling)")

@@ -8,7 +8,6 @@ namespace sp {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMaxModes = 8;
 constexpr double kRecipFactorial[10] = {1.0,          1.0,          0.5,
                                         1.0 / 6.0,    1.0 / 24.0,   1.0 / 120.0,
                                         1.0 / 720.0,  1.0 / 5040.0, 1.0 / 40320.0,
@@ -169,16 +168,13 @@ __device__ inline double wave_scan(double v) {
 __device__ inline double from_next_lane(double v) { return dpp_move<0x130, 0xf, true>(v); }
 __device__ inline double from_previous_lane(double v) { return dpp_move<0x138, 0xf, true>(v); }
 
-__device__ inline double node_field(const double* S, const double* a, int C, int T, int t) {
-  double e = 0.0;
-  for (int m = 0; m < C; ++m) e = e + S[m] * a[static_cast<long>(m) * T + t];
-  return e;
-}
+// Registers a lane holds for its sample's modal fields (one unused for the dense route).
+constexpr int held(int C) { return C > 0 ? C : 1; }
 
-// The field at a node from the modal fields of this lane's sample, Σ_m S[m] a_m, in the
-// same order as node_field.
+// The field at a node from the modal fields of this lane's sample, Σ_m S[m] a_m, summed
+// in mode order.
 template <int C>
-__device__ inline double synthesise_at(const double* S, const double (&am)[C]) {
+__device__ inline double synthesise_at(const double* S, const double (&am)[held(C)]) {
   double e = 0.0;
 #pragma unroll
   for (int m = 0; m < C; ++m) e = e + S[m] * am[m];
@@ -187,10 +183,22 @@ __device__ inline double synthesise_at(const double* S, const double (&am)[C]) {
 
 // The modal fields (C, T) real at sample t, zero past the grid's end.
 template <int C>
-__device__ inline void load_modal(const FieldBuffers& B, int T, int t, double (&am)[C]) {
+__device__ inline void load_modal(const FieldBuffers& B, int T, int t, double (&am)[held(C)]) {
   const double* a = reinterpret_cast<const double*>(B.modal_t);
 #pragma unroll
   for (int m = 0; m < C; ++m) am[m] = t < T ? a[static_cast<long>(m) * T + t] : 0.0;
+}
+
+// The field at node q and this lane's sample t: synthesised from the lane's modal
+// fields, or on the dense route (C = 0) read from the node field. Zero past the end.
+template <int C>
+__device__ inline double field_at(const DeviceProblem& P, const FieldBuffers& B,
+                                  const double (&am)[held(C)], int q, int t) {
+  if constexpr (C > 0) {
+    return synthesise_at<C>(P.synthesis + static_cast<long>(q) * C, am);
+  } else {
+    return t < P.T ? B.node_field[static_cast<long>(q) * P.T + t] : 0.0;
+  }
 }
 
 // One node's exposure and free current over one scan tile (see field.hpp): the rate
@@ -330,14 +338,14 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   // The shared last sample is stored by the next tile's first lane.
   const bool stores = inside && (lane < kTileSpan || t == T - 1);
   const bool step = lane < kTileSpan && t + 1 < T;
-  double am[C];
+  double am[held(C)];
   load_modal<C>(B, T, t, am);
   const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave;
 #pragma unroll
   for (int i = 0; i < kNodesPerWave; ++i) {
     const int q = q0 + i * kNodeWaves + wave;
     if (q < P.Q) {  // uniform across the wave
-      const double e = synthesise_at<C>(P.synthesis + static_cast<long>(q) * C, am);
+      const double e = field_at<C>(P, B, am, q, t);
       const double w = inside ? rate_of(fabs(e), pt, coefficients) : 0.0;
       if (stores) B.rate[static_cast<long>(q) * T + t] = w;
       const double w_next = from_next_lane(w);
@@ -381,14 +389,14 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   const int t = tile * kTileSpan + lane;
   const bool stores = t < T && (lane < kTileSpan || t == T - 1);
   const double f0 = R->preionisation;
-  double am[C];
+  double am[held(C)];
   load_modal<C>(B, T, t, am);
   const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave;
 #pragma unroll
   for (int i = 0; i < kNodesPerWave; ++i) {
     const int q = q0 + i * kNodeWaves + wave;
     if (q < P.Q) {  // uniform across the wave
-      const double e = synthesise_at<C>(P.synthesis + static_cast<long>(q) * C, am);
+      const double e = field_at<C>(P, B, am, q, t);
       const long row = static_cast<long>(q) * tiles + tile;
       const TileColumn c = tile_column(P, e, B.rate + static_cast<long>(q) * T, B.tile_x[row],
                                        f0, t, lane);
@@ -474,6 +482,40 @@ __global__ void __launch_bounds__(kWave * kResponseWaves)
   }
 }
 
+// Pass E of the dense route, one thread per node and sample: the un-gated Kerr
+// polarisation and the total current J = ρ (F + I_p W (1 - f) E / |E|²), laid out
+// (Q, 2, T) for the projection GEMM, which applies g and g' (as (Q, T) polarisation
+// alone without plasma). A sample's fraction and within-tile current are those its
+// owning tile stored: tile t / 63, the last tile for the grid's last sample.
+__global__ void node_response_dense(DeviceProblem P, const AdamsRecord* __restrict__ R,
+                                    FieldBuffers B) {
+  const int T = P.T, tiles = scan_tiles(T);
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long>(P.Q) * T) return;
+  const int q = static_cast<int>(idx / T), t = static_cast<int>(idx % T);
+  const double density = R->density;
+  const double e = B.node_field[idx];
+  double pol = 0.0;
+  if (P.has_kerr) {
+    const double factor = P.kerr_coefficient * density;
+    pol = (factor * (e * e)) * e;
+  }
+  if (!P.has_plasma) {
+    B.node_streams[idx] = pol;
+    return;
+  }
+  const int tile = min(t / kTileSpan, tiles - 1);
+  const double w = B.rate[idx], f = B.frac[idx];
+  const double F = B.tile_f[static_cast<long>(q) * tiles + tile] + B.free_local[idx];
+  const double squared = fabs(e) * fabs(e);
+  double loss = 0.0;
+  if (w > 0.0 && squared > 0.0) loss = (P.plasma_ip * w * (1.0 - f)) / squared;
+  const double current = density * (F + loss * e);
+  const long row = 2L * q * T;
+  B.node_streams[row + t] = pol;
+  B.node_streams[row + T + t] = current;
+}
+
 // Unpack the three stream spectra, assemble the held current on the active bins and
 // normalise: k = (p W) (P~ + G~/(iω) - E~/(iω)^2).
 __global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B,
@@ -485,10 +527,14 @@ __global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R,
   const int half = P.T / 2;
   const int k = P.crop[b];
   const int ka = k % half, kb = (half - k) % half;
-  cplx X[3];
+  cplx X[3] = {};
+  const int n_streams = P.has_plasma ? 3 : 1;  // the currents are transformed with plasma
+#pragma unroll
   for (int s = 0; s < 3; ++s) {
-    const cplx* row = B.streams + (static_cast<long>(s) * P.C + c) * half;
-    X[s] = r2c_post(row[ka], row[kb], B.roots_T[k]);
+    if (s < n_streams) {
+      const cplx* row = B.streams + (static_cast<long>(s) * P.C + c) * half;
+      X[s] = r2c_post(row[ka], row[kb], B.roots_T[k]);
+    }
   }
   cplx spectrum = X[0];
   if (P.has_plasma && P.active[b]) {
@@ -678,8 +724,14 @@ void launch_node_passes_for(const DeviceProblem& P, const AdamsRecord* R,
   } else {
     for (int i = 0; i < 4; ++i) mark(i);
   }
-  hipLaunchKernelGGL(node_response<C>, dim3(tiles), dim3(kWave * kResponseWaves), 0, s, P, R,
-                     B);
+  if constexpr (C > 0) {
+    hipLaunchKernelGGL(node_response<C>, dim3(tiles), dim3(kWave * kResponseWaves), 0, s, P,
+                       R, B);
+  } else {
+    const std::size_t n = static_cast<std::size_t>(P.Q) * P.T;
+    hipLaunchKernelGGL(node_response_dense, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,
+                       s, P, R, B);
+  }
   check();
   mark(4);
 }
@@ -687,8 +739,13 @@ void launch_node_passes_for(const DeviceProblem& P, const AdamsRecord* R,
 
 void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,
                         FieldBuffers B, hipStream_t s, hipEvent_t* marks) {
-  SP_CHECK(P.C >= 1 && P.C <= kMaxModes, "the node passes project 1..8 modes");
   SP_CHECK(P.T >= 2, "the fine grid needs two samples");
+  if (B.node_field) {
+    SP_CHECK(B.node_streams, "the dense route needs its node streams");
+    launch_node_passes_for<0>(P, R, pt, B, s, marks);
+    return;
+  }
+  SP_CHECK(P.C >= 1 && P.C <= kMaxModes, "the node passes project 1..8 modes");
   switch (P.C) {
     case 1: launch_node_passes_for<1>(P, R, pt, B, s, marks); break;
     case 2: launch_node_passes_for<2>(P, R, pt, B, s, marks); break;

@@ -9,6 +9,11 @@
 // forward FFT of the 3C packed streams -> finish (unpack, held-current assembly on the
 // active bins, prefactor and spectral collar).
 //
+// With more than kMaxModes modes (the radial route's Hankel spectrum) the synthesis and
+// the projection are matrix-core GEMMs instead: the synthesis writes the node field
+// (Q, T), the node passes read it, the response writes each node's un-gated
+// polarisation and current, and the projection applies the time collars on the way out.
+//
 // The scans' tiles are one wavefront each: 64 samples holding 63 trapezoid increments,
 // the last sample shared with the next tile, so every increment's two ends sit in one
 // wave and no lane reaches outside it. Tile k starts at sample 63 k.
@@ -18,6 +23,7 @@
 
 namespace sp {
 
+constexpr int kMaxModes = 8;       // modes the node passes synthesise and project in registers
 constexpr int kMaxAdamsWidth = 8;  // 1 + the highest scored order (MAX_ADAMS_ORDER 6)
 constexpr int kMaxScored = 3;      // the step's own order and two neighbours
 constexpr int kWave = 64;          // lanes of a wavefront
@@ -67,6 +73,8 @@ struct FieldBuffers {
   double* free_local;  // (Q, T) free current since the start of the sample's scan tile
   double* tile_x;   // (Q, tiles) exposure tile sums, then offsets
   double* tile_f;   // (Q, tiles) free-current tile sums, then offsets
+  double* node_field;    // (Q, T) field at the nodes (dense route only)
+  double* node_streams;  // (Q, 2, T) un-gated polarisation and current (dense route only)
   double* partials; // (1 + kMaxScored, blocks) masked squared sums
   const cplx* roots_T;  // unit roots of T
   const double* rate_c; // (4, n_intervals) rate table
@@ -86,7 +94,8 @@ void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuf
                           int width, hipStream_t s);
 void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuffers B,
                           hipStream_t s);
-// `marks`, when given, holds five events recorded after the rate and exposure tiles,
+// The dense route (B.node_field set) expects the node field written; it leaves the
+// node streams for the projection GEMM. `marks`, when given, holds five events recorded after the rate and exposure tiles,
 // their prefix, the free-current tiles, their prefix and the node response (a run
 // without plasma records the first four back to back).
 void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,

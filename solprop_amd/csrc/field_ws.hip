@@ -1,5 +1,6 @@
 // The modal field-grid workspace under the exponential Adams march (see field_ws.hpp).
 #include "field_ws.hpp"
+#include "gemm.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -34,7 +35,7 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   SP_CHECK(T >= 256 && (T & (T - 1)) == 0, "n_to must be a power of two >= 256");
   SP_CHECK(d.n_t >= 4 && T % d.n_t == 0, "n_to must be a multiple of n_t");
   SP_CHECK(N == d.n_t / 2 + 1, "a field grid has n_omega = n_t/2 + 1 bins");
-  SP_CHECK(C >= 1 && C <= 8, "1..8 modes");
+  SP_CHECK(C >= 1, "at least one mode");
   SP_CHECK(Q >= 1, "at least one quadrature node");
   SP_CHECK(d.n_phi >= 1 && d.n_phi <= kMaxTerms, "1..4 phi terms");
   SP_CHECK(d.n_prefactor >= 1 && d.n_prefactor <= kMaxTerms, "1..4 prefactor terms");
@@ -67,6 +68,7 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   rtol_ = d.rtol;
   atol_ = d.atol;
   compensated_ = d.n_phi > 1;
+  dense_ = C > kMaxModes;
 
   policy_.order = d.adams_order;
   policy_.variable_order = d.adams_variable_order != 0;
@@ -151,12 +153,19 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   B_.advanced = alloc<cplx>(CN);
   B_.modal_t = alloc<cplx>(static_cast<std::size_t>(C) * (T / 2));
   B_.streams = alloc<cplx>(3UL * C * (T / 2));
+  // Streams a run never writes (the currents without plasma on the dense route) read as
+  // zeros.
+  SP_HIP(hipMemset(B_.streams, 0, 3UL * C * (T / 2) * sizeof(cplx)));
   const std::size_t tiles = static_cast<std::size_t>(scan_tiles(T));
   B_.rate = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
   B_.frac = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
   B_.free_local = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
   B_.tile_x = alloc<double>(static_cast<std::size_t>(Q) * tiles);
   B_.tile_f = alloc<double>(static_cast<std::size_t>(Q) * tiles);
+  if (dense_) {
+    B_.node_field = alloc<double>(static_cast<std::size_t>(Q) * T);
+    B_.node_streams = alloc<double>(static_cast<std::size_t>(Q) * (d.has_plasma ? 2 : 1) * T);
+  }
   n_blocks_ = static_cast<int>(correct_blocks(P_));
   // The corrector's block sums land straight in mapped pinned host memory: the host
   // reads them after the trial's one synchronisation, with no copy on the stream.
@@ -167,7 +176,9 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   B_.roots_T = upload(roots.data(), roots.size());
   scratch_ = alloc<cplx>(3UL * C * (T / 2));
   inv_plan_.init(T / 2, C);
-  fwd_plan_.init(T / 2, 3 * C);
+  // Without plasma only the polarisation stream is transformed (the finish reads the
+  // current streams on plasma runs alone).
+  fwd_plan_.init(T / 2, (d.has_plasma ? 3 : 1) * C);
   rec_dev_ = alloc<AdamsRecord>(1);
   SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&rec_host_), sizeof(AdamsRecord)));
   std::memset(rec_host_, 0, sizeof(AdamsRecord));
@@ -204,13 +215,30 @@ namespace {
 // The kernels (or transform plans) of one trial, in launch order.
 constexpr const char* kPhaseNames[FieldWorkspace::kPhases] = {
     "adams_predict",      "field_scatter",        "inverse_fft",
-    "plasma_rate_tiles",  "plasma_tile_prefix",   "plasma_current_tiles",
-    "plasma_tile_prefix_current", "node_response", "forward_fft",
-    "field_finish",       "adams_correct"};
+    "synthesis_gemm",     "plasma_rate_tiles",    "plasma_tile_prefix",
+    "plasma_current_tiles", "plasma_tile_prefix_current", "node_response",
+    "projection_gemm",    "forward_fft",          "field_finish",
+    "adams_correct"};
 }  // namespace
 
 void FieldWorkspace::mark(int i) {
   if (marking_) SP_HIP(hipEventRecord(marks_[i], stream_));
+}
+
+// The dense route's GEMMs: node field = S (Q x C) . modal fields (C x T), and the
+// streams = W (C x Q) . node streams with the time collars applied.
+void FieldWorkspace::enqueue_synthesis() {
+  if (!dense_) return;
+  launch_dgemm(P_.Q, P_.T, P_.C, P_.synthesis, P_.C,
+               reinterpret_cast<const double*>(B_.modal_t), P_.T, B_.node_field, P_.T,
+               stream_);
+}
+
+void FieldWorkspace::enqueue_projection() {
+  if (!dense_) return;
+  launch_collar_projection(P_.C, P_.T, P_.Q, P_.projection, B_.node_streams,
+                           P_.has_plasma != 0, P_.t_window, P_.t_deriv,
+                           reinterpret_cast<double*>(B_.streams), stream_);
 }
 
 void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
@@ -218,11 +246,15 @@ void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
   mark(2);
   inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
   mark(3);
-  launch_node_passes(P_, rec_dev_, pt_, B_, stream_, marking_ ? marks_ + 4 : nullptr);
-  fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
-  mark(9);
-  launch_field_finish(P_, rec_dev_, B_, out, stream_);
+  enqueue_synthesis();
+  mark(4);
+  launch_node_passes(P_, rec_dev_, pt_, B_, stream_, marking_ ? marks_ + 5 : nullptr);
+  enqueue_projection();
   mark(10);
+  fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
+  mark(11);
+  launch_field_finish(P_, rec_dev_, B_, out, stream_);
+  mark(12);
 }
 
 void FieldWorkspace::enqueue_trial(bool commit) {
@@ -234,7 +266,7 @@ void FieldWorkspace::enqueue_trial(bool commit) {
   mark(1);
   enqueue_source(B_.lab, B_.source);
   launch_adams_correct(P_, rec_dev_, B_, width_, stream_);
-  mark(11);
+  mark(13);
 }
 
 void FieldWorkspace::enqueue_start() {
@@ -541,7 +573,9 @@ void FieldWorkspace::source(double z, const double* lab, double* out, double* st
   marking_ = false;
   launch_field_scatter(P_, B_.lab, B_, stream_);
   inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
+  enqueue_synthesis();
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
+  enqueue_projection();
   SP_HIP(hipStreamSynchronize(stream_));
   if (modal) SP_HIP(hipMemcpy(modal, B_.modal_t, CT * sizeof(double), hipMemcpyDeviceToHost));
   if (streams)
@@ -567,6 +601,7 @@ long FieldWorkspace::buffer(const char* name, double* out, long count) {
       {"moments", B_.moments, S * CN2},
       {"states", B_.states, B_.states ? S * CN2 : 0},
       {"carried", B_.carried, B_.carried ? S * CN2 : 0},
+      {"node_field", B_.node_field, B_.node_field ? static_cast<long>(P_.Q) * P_.T : 0},
   };
   flush();
   for (const auto& row : table) {

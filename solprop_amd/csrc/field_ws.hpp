@@ -37,7 +37,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   long buffer(const char* name, double* out, long count) override;
   void set_profile(bool on) override { profile_ = on; }
   std::vector<std::pair<const char*, double>> phases() const override;
-  static constexpr int kPhases = 11;
+  static constexpr int kPhases = 13;
   double trial_us() const override { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
   long timed() const override { return timed_; }
 
@@ -56,6 +56,8 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   T* upload(const T* host, std::size_t n);
   void fill_position(double z);
   void enqueue_source(const cplx* spectrum, cplx* out);
+  void enqueue_synthesis();
+  void enqueue_projection();
   enum Graph { kTrial = 1, kStart = 2, kCommitTrial = 3 };
   void enqueue_trial(bool commit);
   void enqueue_start();
@@ -84,6 +86,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   ScalarPath density_, preion_;
   double rtol_ = 1e-6, atol_ = 0.0;
   bool compensated_ = false;
+  bool dense_ = false;  // synthesis and projection as GEMMs (more than kMaxModes modes)
   // The march, and the host it reports to: rows and the attempt limit of one call.
   struct CallHost final : adams::Host {
     std::vector<double>* rows = nullptr;

@@ -310,7 +310,8 @@ class Workspace:
         return (out, streams, modal) if intermediates else out
 
     def buffer(self, name: str) -> np.ndarray:
-        """A named device buffer of the field route, as complex rows of (C, N)."""
+        """A named device buffer of the field route: complex rows of (C, N), or for
+        ``node_field`` (the dense route's synthesised field) real (Q, T)."""
         size = C.c_long()
         msg = C.create_string_buffer(1024)
         _check(self._lib.sp_testing_buffer(self._handle, name.encode(), None, 0,
@@ -318,6 +319,8 @@ class Workspace:
         out = np.zeros(size.value)
         _check(self._lib.sp_testing_buffer(self._handle, name.encode(), _ptr(out), size.value,
                                            C.byref(size), msg, 1024), msg)
+        if name == "node_field":
+            return out.reshape(self.problem.n_nodes, self.problem.n_to)
         return out.view(np.complex128).reshape(-1, *self._shape)
 
     def phases(self) -> dict:

@@ -16,15 +16,12 @@ within 1e-8 relative L2.
 import numpy as np
 import pytest
 
-from oracle import adams, rk
+from field_stages import log_ulp_sensitivity, oracle_first_trial, oracle_source_stages, rel_l2
+from oracle import adams
 from solprop_amd.native import Settings, Workspace
 from solprop_amd.problem import capillary_problem
 
 pytestmark = pytest.mark.gpu
-
-
-def rel_l2(a, b):
-    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
 def small(**kw):
@@ -39,55 +36,6 @@ CASES = {
     "plasma_only": {"kerr": False},
     "constant_pressure": {"p1": 0.8},
 }
-
-
-def oracle_source_stages(problem, lab, z):
-    """The oracle's modal fields, its three projected time streams and its source."""
-    from oracle import passes as P
-
-    src = rk.StageSource(problem)
-    crop = problem.crop_index.astype(np.intp)
-    modal = P.to_time(lab, crop, problem.n_to, problem.kind)
-    field_t = P.synthesise(modal, problem.synthesis, 1.0, 1)
-    s = P.nonlinear_sources(
-        field_t, kerr=src.kerr, raman=None, raman_thg=False, plasma=src.plasma,
-        t_window_fine=problem.time_window, kind=problem.kind, density=problem.density(z),
-        preionisation=problem.preionisation(z), dt=problem.dt_fine)
-    streams = np.zeros((3, problem.n_modes, problem.n_to))
-    streams[0] = P.project(s.polarisation, problem.projection, 1.0, 1)[:, 0]
-    if s.current is not None:
-        streams[1] = P.project(problem.time_window * s.current, problem.projection, 1.0,
-                               1)[:, 0]
-        streams[2] = P.project(problem.time_window_derivative * s.current,
-                               problem.projection, 1.0, 1)[:, 0]
-    return modal[:, 0], streams, src.evaluate(lab, z, 1.0)
-
-
-def log_ulp_sensitivity(problem, lab, z, streams):
-    """How far each current stream moves when log|E| is off by one ulp in one direction.
-
-    The rate is exp of a cubic in log|E| with a log-slope of tens to hundreds, and the
-    free current late in the window is a small residual of an oscillating integral: a
-    systematic last-ulp difference between two libm ``log`` implementations (the
-    device's and the host's) moves the current streams by this much.
-    """
-    from oracle import passes as P
-
-    base = P.RateTable.log_evaluate
-    worst = np.zeros(3)
-    try:
-        for sign in (1.0, -1.0):
-            def shifted(self, lf, sign=sign):
-                return base(self, lf + sign * np.spacing(lf))
-
-            P.RateTable.log_evaluate = shifted
-            _, moved, _ = oracle_source_stages(problem, lab, z)
-            for i in range(3):
-                if np.any(streams[i]):
-                    worst[i] = max(worst[i], rel_l2(moved[i], streams[i]))
-    finally:
-        P.RateTable.log_evaluate = base
-    return worst
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
@@ -119,16 +67,6 @@ def test_source_stages_match_oracle(case):
     for i in (1, 2):
         if names[i] in errors:
             assert errors[names[i]] < max(1e-12, 4 * sensitivity[i]), (names[i], errors)
-
-
-def oracle_first_trial(problem, dz):
-    policy = adams.adams_policy_for(problem)
-    engine = adams.ExponentialEngine(problem, rk.StageSource(problem), policy.order + 2)
-    engine.state = problem.field0.reshape(problem.shape).copy()
-    engine.start(0.0)
-    st = adams.stencil(policy, 0.0, dz, [0.0], [0], 1, last_rejected=False)
-    norms = engine.trial(0.0, dz, st)
-    return engine.held[2], norms
 
 
 @pytest.mark.parametrize("case", sorted(CASES))

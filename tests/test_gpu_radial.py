@@ -1,0 +1,123 @@
+"""GPU parity of the radial route (the dense field route), through the C ABI.
+
+A small free-space argon beam: 1024 samples over 250 fs, a 32-point Hankel transform
+over a 4 mm aperture, a 30 fs 800 nm pulse of 2 mJ. On this route the synthesis and the
+projection are matrix-core GEMMs, and the projection applies the time collars after
+the sum over nodes instead of before it.
+
+Tolerances (class R, fixed before the run):
+- node field: the GEMM's error against the oracle's product is held to Higham's
+  bound, 4 K eps || |S| |a| ||_F with K the number of modes summed;
+- modal fields, polarisation, currents and source as on the capillary route (the
+  collars are applied once to the projected sum, an extra rounding of one ulp);
+- a first trial and a short march as on the capillary route.
+"""
+
+import numpy as np
+import pytest
+
+from field_stages import log_ulp_sensitivity, oracle_first_trial, oracle_source_stages, rel_l2
+from oracle import adams
+from solprop_amd.native import Workspace
+from solprop_amd.problem import radial_problem
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # A 1 mm beam: Kerr dominates, the plasma current is at the table's floor.
+    "wide_beam": {"w0": 1.0e-3},
+    # A 0.25 mm beam: a strong plasma current.
+    "narrow_beam": {"w0": 0.25e-3},
+    "kerr_only": {"w0": 0.25e-3, "plasma": False},
+}
+
+
+def small(**kw):
+    args = dict(n_t=1024, window=0.25e-12, n_r=32, length=0.02, fwhm=30e-15)
+    args.update(kw)
+    return radial_problem(**args)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_radial_source_stages_match_oracle(case):
+    from oracle import passes as P
+
+    print(f"radial source stages {case}", flush=True)
+    problem = small(**CASES[case])
+    lab = problem.field0.reshape(problem.shape)
+    z = 1e-3
+    modal, streams, source = oracle_source_stages(problem, lab, z)
+    field_t = P.synthesise(modal[:, None], problem.synthesis, 1.0, 1)[:, 0]
+    ws = Workspace(problem)
+    ws.load(0.0, problem.field0)
+    got, got_streams, got_modal = ws.source(z, lab, intermediates=True)
+    node_field = ws.buffer("node_field").reshape(field_t.shape)
+    ws.close()
+    bound = 4 * problem.n_modes * np.finfo(float).eps * np.linalg.norm(
+        np.abs(problem.synthesis) @ np.abs(modal))
+    node_error = float(np.linalg.norm(node_field - field_t))
+    errors = {"modal": rel_l2(got_modal, modal), "source": rel_l2(got, source.reshape(got.shape))}
+    names = ("polarisation", "gated current", "edge current")
+    for i, name in enumerate(names):
+        if np.any(streams[i]):
+            errors[name] = rel_l2(got_streams[i], streams[i])
+    sensitivity = log_ulp_sensitivity(problem, lab, z, streams)
+    print(errors, "node field", node_error, "bound", bound, "log-ulp sensitivity",
+          sensitivity.tolist(), flush=True)
+    assert node_error <= bound
+    assert errors["modal"] < 1e-14
+    assert errors["source"] < 1e-12
+    assert errors["polarisation"] < 1e-13
+    for i in (1, 2):
+        if names[i] in errors:
+            assert errors[names[i]] < max(1e-12, 4 * sensitivity[i]), (names[i], errors)
+    if not problem.plasma:
+        assert not np.any(got_streams[1]) and not np.any(got_streams[2])
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_radial_first_trial_matches_oracle(case):
+    print(f"radial probe trial {case}", flush=True)
+    problem = small(**CASES[case])
+    dz = 1e-4
+    advanced, norms = oracle_first_trial(problem, dz)
+    ws = Workspace(problem)
+    probe = ws.probe(0.0, problem.field0)
+    got = probe.trial(0.0, dz)
+    field = probe.field()
+    ws.close()
+    assert rel_l2(field, advanced.reshape(field.shape)) < 1e-11
+    assert abs(got - norms[0]) / norms[0] < 1e-8
+
+
+@pytest.mark.parametrize("case", ["wide_beam", "narrow_beam"])
+def test_radial_march_matches_oracle_decisions(case):
+    print(f"radial march {case}", flush=True)
+    problem = small(**CASES[case])
+    y0 = problem.field0.reshape(problem.shape)
+    expected, records = adams.march(problem, y0.copy(), max_attempts=30)
+    ws = Workspace(problem)
+    session = ws.start(0.0, problem.field0)
+    batch = session.advance(problem.z_end, work_limit=len(records))
+    rows = batch.attempts
+    state = session.state()
+    ws.close()
+    assert len(rows) == len(records)
+    assert [bool(r[3]) for r in rows] == [bool(r[3]) for r in records]
+    steps = np.array([r[1] for r in records])
+    assert np.allclose(rows[:, 1], steps, rtol=1e-7, atol=0)
+    assert rel_l2(state, expected.reshape(state.shape)) < 1e-8
+
+
+def test_radial_repeats_agree_to_the_bit():
+    print("radial determinism", flush=True)
+    problem = small(**CASES["narrow_beam"])
+    outs = []
+    for _ in range(2):
+        ws = Workspace(problem)
+        session = ws.start(0.0, problem.field0)
+        batch = session.advance(problem.z_end, work_limit=10)
+        outs.append((batch.attempts.copy(), session.state()))
+        ws.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])

@@ -36,11 +36,6 @@ def test_band_running_integral_is_refused():
     refused(dataclasses.replace(p, plasma=plasma), "band running-integral")
 
 
-def test_too_many_modes_are_refused():
-    p = capillary_problem(n_t=256, n_nodes=4, n_modes=9)
-    refused(p, "1..8 modes")
-
-
 def test_short_field_grid_is_refused():
     refused(capillary_problem(n_t=128, n_nodes=4), "power of two >= 256")
 
