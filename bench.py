@@ -64,7 +64,7 @@ def trial_bytes(problem, order):
     field = qt if dense else ct + table  # what a node pass reads to know E at its nodes
     return {
         "adams_predict": cn * (n_phi + 1 + order + (order if comp else 0))
-        + cn * (W + 2 + (order if comp else 0)),
+        + cn * (1 + (order if comp else 0)),
         "field_scatter": cn + 8 * T + ct,
         "inverse_fft": 2 * ct,
         "synthesis_gemm": table + ct + qt if dense else 0,
@@ -78,7 +78,9 @@ def trial_bytes(problem, order):
         if dense else 0,
         "forward_fft": 2 * n_streams * ct,
         "field_finish": n_streams * ct + cn * len(problem.prefactor_paths) + cn,
-        "adams_correct": cn * (W + 1 + (W - 1) + 2) + cn,
+        "adams_correct": cn * (n_phi + 1 + (W - 1) + 1 + (order + 1 if comp else 0)) + cn,
+        "adams_norms": 8 * (1 + 3) * ((C * N + (64 if C * N <= 4096 * 64 else 256) - 1)
+                                      // (64 if C * N <= 4096 * 64 else 256)),
     }
 
 

@@ -230,6 +230,27 @@ __device__ inline TileColumn tile_column(const DeviceProblem& P, double e, const
 }
 }  // namespace
 
+// The step's operator at one bin: x = Σ_t w_t Φ_t in term order, the moments
+// M_k = k! φ_{k+1}(x) and the flow exp(x) (exactly 1 where x = 0: inactive bins skip
+// the transcendental). The predictor and the corrector both form them from the Φ terms,
+// so neither array is stored.
+template <int W>
+__device__ inline cplx step_operator(const DeviceProblem& P, const AdamsRecord& r,
+                                     const cplx (&phi_t)[kMaxTerms], cplx (&M)[W],
+                                     cplx& flow) {
+  cplx x = make(0.0, 0.0);
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t)
+    if (t < P.n_phi) x = add(x, scale(r.x_w[t], phi_t[t]));
+  cplx phi[W + 1];
+  phi_functions<W>(x, phi);
+#pragma unroll
+  for (int k = 0; k < W; ++k) M[k] = scale(kFactorial[k], phi[k + 1]);
+  const bool still = x.x == 0.0 && x.y == 0.0;
+  flow = still ? make(1.0, 0.0) : cexp(x);
+  return x;
+}
+
 // ------------------------------------------------------------- Adams predictor
 
 // The predictor for a stencil of width W (moments M_0..M_{W-1}, W - 1 stored rows). One
@@ -254,23 +275,8 @@ __global__ void __launch_bounds__(kWave)
     F[j] = j < W - 1 ? B.store[r.slots[j] * CN + idx] : make(0.0, 0.0);
     U[j] = compensated && j < W - 1 ? B.states[r.slots[j] * CN + idx] : make(0.0, 0.0);
   }
-  // x = Σ_t w_t Φ_t, in term order.
-  cplx x = make(0.0, 0.0);
-#pragma unroll
-  for (int t = 0; t < kMaxTerms; ++t)
-    if (t < P.n_phi) x = add(x, scale(r.x_w[t], phi_t[t]));
-  cplx phi[W + 1];
-  phi_functions<W>(x, phi);
-  cplx M[W];
-#pragma unroll
-  for (int k = 0; k < W; ++k) {
-    M[k] = scale(kFactorial[k], phi[k + 1]);
-    B.moments[k * CN + idx] = M[k];
-  }
-  // exp(0) = 1 exactly: inactive bins (x = 0) skip the transcendental.
-  const bool still = x.x == 0.0 && x.y == 0.0;
-  const cplx flow = still ? make(1.0, 0.0) : cexp(x);
-  B.flow[idx] = flow;
+  cplx M[W], flow;
+  const cplx x = step_operator<W>(P, r, phi_t, M, flow);
   cplx lab = add(mul(flow, y), apply_moments<W>(M, r.pred_c, F, dz));
   if (compensated) {
     const cplx reference = make(x.x / dz, x.y / dz);
@@ -516,34 +522,53 @@ __global__ void node_response_dense(DeviceProblem P, const AdamsRecord* __restri
   B.node_streams[row + T + t] = current;
 }
 
-// Unpack the three stream spectra, assemble the held current on the active bins and
-// normalise: k = (p W) (P~ + G~/(iω) - E~/(iω)^2).
-__global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B,
-                             cplx* out) {
+// One coarse bin b of mode c from its two packed entries per stream (za at b, zb at its
+// mirror): unpack the stream spectra, assemble the held current on the active bins and
+// normalise, k = (p W) (P~ + G~/(iω) - E~/(iω)^2).
+__device__ inline void finish_bin(const DeviceProblem& P, const AdamsRecord& r,
+                                  const FieldBuffers& B, int c, int b, const cplx (&za)[3],
+                                  const cplx (&zb)[3], int n_streams, cplx* out) {
   const long CN = static_cast<long>(P.C) * P.N;
-  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= CN) return;
-  const int c = static_cast<int>(idx / P.N), b = static_cast<int>(idx % P.N);
-  const int half = P.T / 2;
-  const int k = P.crop[b];
-  const int ka = k % half, kb = (half - k) % half;
+  const long idx = static_cast<long>(c) * P.N + b;
   cplx X[3] = {};
-  const int n_streams = P.has_plasma ? 3 : 1;  // the currents are transformed with plasma
 #pragma unroll
-  for (int s = 0; s < 3; ++s) {
-    if (s < n_streams) {
-      const cplx* row = B.streams + (static_cast<long>(s) * P.C + c) * half;
-      X[s] = r2c_post(row[ka], row[kb], B.roots_T[k]);
-    }
-  }
+  for (int s = 0; s < 3; ++s)
+    if (s < n_streams) X[s] = r2c_post(za[s], zb[s], B.roots_T[b]);
   cplx spectrum = X[0];
   if (P.has_plasma && P.active[b]) {
     const cplx q = make(0.0, P.omega[b]);
     const cplx q2 = mul(q, q);
     spectrum = add(spectrum, sub(cdiv(X[1], q), cdiv(X[2], q2)));
   }
-  const cplx pref = weighted(R->pref_w, P.pref, P.n_pref, CN, idx);
+  const cplx pref = weighted(r.pref_w, P.pref, P.n_pref, CN, idx);
   out[idx] = mul(scale(P.omega_window[b], pref), spectrum);
+}
+
+// The finish over bin pairs: the packed real transform holds bins k and T/2 - k in the
+// same two entries (k = 0 pairs with the Nyquist bin T/2, both from entry 0), so one
+// thread reads them once and writes both bins that the coarse grid keeps.
+__global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B,
+                             cplx* out) {
+  const int half = P.T / 2;
+  const int pairs = half / 2 + 1;
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long>(P.C) * pairs) return;
+  const int c = static_cast<int>(idx / pairs), k = static_cast<int>(idx % pairs);
+  const int mirror = half - k;  // the partner bin, T/2 for k = 0
+  if (k >= P.N) return;         // neither bin is kept (mirror >= k)
+  const int ka = k, kb = mirror % half;
+  const int n_streams = P.has_plasma ? 3 : 1;  // the currents are transformed with plasma
+  cplx za[3] = {}, zb[3] = {};
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    if (s < n_streams) {
+      const cplx* row = B.streams + (static_cast<long>(s) * P.C + c) * half;
+      za[s] = row[ka];
+      zb[s] = row[kb];
+    }
+  }
+  finish_bin(P, *R, B, c, k, za, zb, n_streams, out);
+  if (mirror != k && mirror < P.N) finish_bin(P, *R, B, c, mirror, zb, za, n_streams, out);
 }
 
 // ------------------------------------------------------------- Adams corrector
@@ -553,7 +578,7 @@ __global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R,
 // sums are reduced across its lanes in a fixed order and written as that workgroup's
 // partials. Coefficients past each order are zero in the record.
 template <int W>
-__global__ void __launch_bounds__(kWave)
+__global__ void __launch_bounds__(kCorrectThreads)
     adams_correct(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
   const long CN = static_cast<long>(P.C) * P.N;
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -565,31 +590,28 @@ __global__ void __launch_bounds__(kWave)
     const int n_orders = r.n_orders;
     const bool compensated = r.compensated != 0;
     // Loads.
-    cplx M[W], X[W], carried[W], phi_t[kMaxTerms];
+    cplx X[W], carried[W], phi_t[kMaxTerms];
     X[0] = B.source[idx];
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-      M[k] = B.moments[k * CN + idx];
       if (k >= 1) X[k] = B.store[r.slots[k - 1] * CN + idx];
       carried[k] = compensated && k < W - 1 ? B.carried[k * CN + idx] : make(0.0, 0.0);
     }
 #pragma unroll
     for (int t = 0; t < kMaxTerms; ++t)
-      phi_t[t] = compensated && t < P.n_phi ? P.phi[t * CN + idx] : make(0.0, 0.0);
+      phi_t[t] = t < P.n_phi ? P.phi[t * CN + idx] : make(0.0, 0.0);
     const cplx y = B.y[idx];
-    const cplx flow = B.flow[idx];
     const cplx lab = compensated ? B.lab[idx] : make(0.0, 0.0);
     const bool scored = P.normmask[b] != 0;
+    cplx M[W], flow;
+    const cplx x = step_operator<W>(P, r, phi_t, M, flow);
 
     cplx adv = add(mul(flow, y), apply_moments<W>(M, r.corr_c, X, dz));
     cplx comp_def[kMaxScored];
 #pragma unroll
     for (int i = 0; i < kMaxScored; ++i) comp_def[i] = make(0.0, 0.0);
     if (compensated) {
-      cplx x = make(0.0, 0.0), rate_end = make(0.0, 0.0);
-#pragma unroll
-      for (int t = 0; t < kMaxTerms; ++t)
-        if (t < P.n_phi) x = add(x, scale(r.x_w[t], phi_t[t]));
+      cplx rate_end = make(0.0, 0.0);
 #pragma unroll
       for (int t = 0; t < kMaxTerms; ++t)
         if (t < P.n_phi) rate_end = add(rate_end, scale(r.rate_end_w[t], phi_t[t]));
@@ -624,12 +646,44 @@ __global__ void __launch_bounds__(kWave)
       }
     }
   }
-  const int lane = threadIdx.x;
+  // Each wave's sums across its lanes, then the workgroup's waves in order.
+  __shared__ double wave_sums[1 + kMaxScored][kCorrectThreads / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
   for (int s = 0; s < 1 + kMaxScored; ++s) {
     const double total = wave_scan(v[s]);
-    if (lane == kWave - 1) B.partials[s * gridDim.x + blockIdx.x] = total;
+    if (lane == kWave - 1) wave_sums[s][wave] = total;
   }
+  __syncthreads();
+  if (threadIdx.x < 1 + kMaxScored) {
+    const int s = threadIdx.x;
+    double total = wave_sums[s][0];
+    for (int w = 1; w < static_cast<int>(blockDim.x) / kWave; ++w) total = total + wave_sums[s][w];
+    B.partials[s * gridDim.x + blockIdx.x] = total;
+  }
+}
+
+// The corrector's block sums reduced to the 1 + kMaxScored totals in a fixed order, written
+// to the host's mapped memory. One workgroup: a group of kNormGroup threads per total,
+// thread j summing blocks j, j + kNormGroup, ... in order (coalesced across the group),
+// then a fixed pairwise tree in LDS.
+constexpr int kNormGroup = 256;
+constexpr int kNormThreads = kNormGroup * (1 + kMaxScored);
+__global__ void __launch_bounds__(kNormThreads)
+    adams_norms(const double* __restrict__ partials, int blocks, double* norms) {
+  __shared__ double tree[kNormThreads];
+  const int s = threadIdx.x / kNormGroup, j = threadIdx.x % kNormGroup;
+  const double* row = partials + static_cast<long>(s) * blocks;
+  double total = 0.0;
+#pragma unroll 8
+  for (int b = j; b < blocks; b += kNormGroup) total = total + row[b];
+  tree[threadIdx.x] = total;
+  __syncthreads();
+  for (int half = kNormGroup / 2; half > 0; half /= 2) {
+    if (j < half) tree[threadIdx.x] = tree[threadIdx.x] + tree[threadIdx.x + half];
+    __syncthreads();
+  }
+  if (j == 0) norms[s] = tree[threadIdx.x];
 }
 
 __global__ void adams_accept(DeviceProblem P, int slot, int compensated, FieldBuffers B) {
@@ -662,8 +716,15 @@ __global__ void adams_start(DeviceProblem P, const AdamsRecord* __restrict__ R, 
 
 // -------------------------------------------------------------------- launchers
 
+// One wave per workgroup while that leaves at most 4096 workgroups (small grids are
+// latency-bound: the most workgroups), else four, which keeps the block sums few.
+unsigned correct_threads(const DeviceProblem& P) {
+  const std::size_t n = static_cast<std::size_t>(P.C) * P.N;
+  return n <= 4096UL * kWave ? kWave : kCorrectThreads;
+}
+
 unsigned correct_blocks(const DeviceProblem& P) {
-  return blocks_for(static_cast<std::size_t>(P.C) * P.N, kWave);
+  return blocks_for(static_cast<std::size_t>(P.C) * P.N, correct_threads(P));
 }
 
 namespace {
@@ -760,14 +821,17 @@ void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const Plas
 
 void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                          cplx* out, hipStream_t s) {
-  hipLaunchKernelGGL(field_finish, dim3(cn_blocks(P)), dim3(kThreads), 0, s, P, R, B, out);
+  const std::size_t n = static_cast<std::size_t>(P.C) * (P.T / 4 + 1);
+  SP_CHECK(P.N <= P.T / 2 + 1, "the coarse grid lies within the fine one");
+  hipLaunchKernelGGL(field_finish, dim3(blocks_for(n, kWave)), dim3(kWave), 0, s, P, R, B,
+                     out);
   check();
 }
 
 void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                           int width, hipStream_t s) {
   SP_CHECK(width >= 2 && width <= kMaxAdamsWidth - 1, "stencil width 2..7");
-  const dim3 grid(correct_blocks(P)), block(kWave);
+  const dim3 grid(correct_blocks(P)), block(correct_threads(P));
   switch (width) {
     case 2: hipLaunchKernelGGL(adams_correct<2>, grid, block, 0, s, P, R, B); break;
     case 3: hipLaunchKernelGGL(adams_correct<3>, grid, block, 0, s, P, R, B); break;
@@ -776,6 +840,12 @@ void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuf
     case 6: hipLaunchKernelGGL(adams_correct<6>, grid, block, 0, s, P, R, B); break;
     default: hipLaunchKernelGGL(adams_correct<7>, grid, block, 0, s, P, R, B); break;
   }
+  check();
+}
+
+void launch_adams_norms(const DeviceProblem& P, FieldBuffers B, hipStream_t s) {
+  hipLaunchKernelGGL(adams_norms, dim3(1), dim3(kNormThreads), 0, s, B.partials,
+                     static_cast<int>(correct_blocks(P)), B.norms);
   check();
 }
 

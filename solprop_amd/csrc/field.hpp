@@ -32,6 +32,7 @@ constexpr int kNodeWaves = 8;      // waves per workgroup of the rate and curren
 constexpr int kNodesPerWave = 4;   // nodes each wave of those passes runs
 constexpr int kLdsTableIntervals = 1024;  // rate-table intervals held in LDS
 constexpr int kResponseWaves = 16; // waves per workgroup of the node response
+constexpr int kCorrectThreads = 256;  // the corrector's widest workgroup
 
 // Scan tiles of a fine grid of T samples: ceil((T - 1) / 63).
 __host__ __device__ inline int scan_tiles(int T) { return (T - 1 + kTileSpan - 1) / kTileSpan; }
@@ -60,8 +61,6 @@ struct FieldBuffers {
   cplx* y;          // (C, N) committed state
   cplx* store;      // (S, C, N) untransported source rows
   cplx* states;     // (S, C, N) the state each row was evaluated at
-  cplx* moments;    // (W, C, N) k! phi_{k+1}(x)
-  cplx* flow;       // (C, N) exp(x)
   cplx* carried;    // (W-1, C, N) transported compensation of the rows read
   cplx* lab;        // (C, N) predicted lab-frame field
   cplx* source;     // (C, N) source at the prediction
@@ -75,7 +74,8 @@ struct FieldBuffers {
   double* tile_f;   // (Q, tiles) free-current tile sums, then offsets
   double* node_field;    // (Q, T) field at the nodes (dense route only)
   double* node_streams;  // (Q, 2, T) un-gated polarisation and current (dense route only)
-  double* partials; // (1 + kMaxScored, blocks) masked squared sums
+  double* partials; // (1 + kMaxScored, blocks) masked squared sums of each block
+  double* norms;    // (1 + kMaxScored) their totals, in the host's mapped memory
   const cplx* roots_T;  // unit roots of T
   const double* rate_c; // (4, n_intervals) rate table
 };
@@ -86,6 +86,7 @@ struct PlasmaTable {
   int n_intervals;
 };
 
+unsigned correct_threads(const DeviceProblem& P);
 unsigned correct_blocks(const DeviceProblem& P);
 
 // The predictor and corrector are compiled per stencil width (2..7); `width` must be
@@ -102,8 +103,10 @@ void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const Plas
                         FieldBuffers B, hipStream_t s, hipEvent_t* marks = nullptr);
 void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                          cplx* out, hipStream_t s);
+// The corrector, writing its block sums to B.partials; then their reduction into B.norms.
 void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                           int width, hipStream_t s);
+void launch_adams_norms(const DeviceProblem& P, FieldBuffers B, hipStream_t s);
 // Commits the held trial into store slot `slot`; reads no record, so the host may go on
 // filling the next trial's record while it runs.
 void launch_adams_accept(const DeviceProblem& P, int slot, bool compensated, FieldBuffers B,

@@ -139,14 +139,12 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
     B_.rate_c = upload(d.rate_coefficients, 4 * static_cast<std::size_t>(d.rate_intervals));
   }
 
-  // Store slots, and the widest stencil's moments and carried rows.
+  // Store slots, and the widest stencil's carried rows.
   const int S = policy_.order + 2;
   const int W = S;
   B_.y = alloc<cplx>(CN);
   B_.store = alloc<cplx>(S * CN);
   B_.states = compensated_ ? alloc<cplx>(S * CN) : nullptr;
-  B_.moments = alloc<cplx>(W * CN);
-  B_.flow = alloc<cplx>(CN);
   B_.carried = compensated_ ? alloc<cplx>(W * CN) : nullptr;
   B_.lab = alloc<cplx>(CN);
   B_.source = alloc<cplx>(CN);
@@ -167,11 +165,12 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
     B_.node_streams = alloc<double>(static_cast<std::size_t>(Q) * (d.has_plasma ? 2 : 1) * T);
   }
   n_blocks_ = static_cast<int>(correct_blocks(P_));
-  // The corrector's block sums land straight in mapped pinned host memory: the host
-  // reads them after the trial's one synchronisation, with no copy on the stream.
-  SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&partials_host_),
-                       sizeof(double) * (1 + kMaxScored) * n_blocks_, hipHostMallocMapped));
-  SP_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&B_.partials), partials_host_, 0));
+  // The corrector's block sums stay on the device; their totals land in mapped pinned
+  // host memory, which the host reads after the trial's one synchronisation.
+  B_.partials = alloc<double>(static_cast<std::size_t>(1 + kMaxScored) * n_blocks_);
+  SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&norms_host_), sizeof(double) * (1 + kMaxScored),
+                       hipHostMallocMapped));
+  SP_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&B_.norms), norms_host_, 0));
   const auto roots = unit_roots(static_cast<std::size_t>(T));
   B_.roots_T = upload(roots.data(), roots.size());
   scratch_ = alloc<cplx>(3UL * C * (T / 2));
@@ -194,7 +193,7 @@ void FieldWorkspace::release() {
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   for (void* p : allocations_) (void)hipFree(p);
   if (rec_host_) (void)hipHostFree(rec_host_);
-  if (partials_host_) (void)hipHostFree(partials_host_);
+  if (norms_host_) (void)hipHostFree(norms_host_);
   for (hipEvent_t& e : marks_) {
     if (e) (void)hipEventDestroy(e);
     e = nullptr;
@@ -205,7 +204,7 @@ void FieldWorkspace::release() {
   graphs_.clear();
   allocations_.clear();
   rec_host_ = nullptr;
-  partials_host_ = nullptr;
+  norms_host_ = nullptr;
   stream_ = nullptr;
 }
 
@@ -218,7 +217,7 @@ constexpr const char* kPhaseNames[FieldWorkspace::kPhases] = {
     "synthesis_gemm",     "plasma_rate_tiles",    "plasma_tile_prefix",
     "plasma_current_tiles", "plasma_tile_prefix_current", "node_response",
     "projection_gemm",    "forward_fft",          "field_finish",
-    "adams_correct"};
+    "adams_correct",      "adams_norms"};
 }  // namespace
 
 void FieldWorkspace::mark(int i) {
@@ -267,6 +266,8 @@ void FieldWorkspace::enqueue_trial(bool commit) {
   enqueue_source(B_.lab, B_.source);
   launch_adams_correct(P_, rec_dev_, B_, width_, stream_);
   mark(13);
+  launch_adams_norms(P_, B_, stream_);
+  mark(14);
 }
 
 void FieldWorkspace::enqueue_start() {
@@ -436,17 +437,10 @@ std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Sten
     timed_ += 1;
   }
   held_rate_end_.assign(r.rate_end_w, r.rate_end_w + kMaxTerms);
-  const int nb = n_blocks_;
-  double s0 = 0.0;
-  for (int b = 0; b < nb; ++b) s0 = s0 + partials_host_[b];
-  const double magnitude = std::sqrt(s0);
+  const double magnitude = std::sqrt(norms_host_[0]);
   const double divisor = rtol_ * magnitude + atol_;
   std::vector<double> norms(r.n_orders);
-  for (int i = 0; i < r.n_orders; ++i) {
-    double si = 0.0;
-    for (int b = 0; b < nb; ++b) si = si + partials_host_[(1 + i) * nb + b];
-    norms[i] = std::sqrt(si) / divisor;
-  }
+  for (int i = 0; i < r.n_orders; ++i) norms[i] = std::sqrt(norms_host_[1 + i]) / divisor;
   trials_ += 1;
   evaluations_ += 1;
   return norms;
@@ -597,8 +591,7 @@ long FieldWorkspace::buffer(const char* name, double* out, long count) {
   } table[] = {
       {"y", B_.y, CN2},           {"lab", B_.lab, CN2},
       {"source", B_.source, CN2}, {"advanced", B_.advanced, CN2},
-      {"flow", B_.flow, CN2},     {"store", B_.store, S * CN2},
-      {"moments", B_.moments, S * CN2},
+      {"store", B_.store, S * CN2},
       {"states", B_.states, B_.states ? S * CN2 : 0},
       {"carried", B_.carried, B_.carried ? S * CN2 : 0},
       {"node_field", B_.node_field, B_.node_field ? static_cast<long>(P_.Q) * P_.T : 0},

@@ -37,7 +37,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   long buffer(const char* name, double* out, long count) override;
   void set_profile(bool on) override { profile_ = on; }
   std::vector<std::pair<const char*, double>> phases() const override;
-  static constexpr int kPhases = 13;
+  static constexpr int kPhases = 14;
   double trial_us() const override { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
   long timed() const override { return timed_; }
 
@@ -77,7 +77,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   cplx* scratch_ = nullptr;
   AdamsRecord* rec_dev_ = nullptr;
   AdamsRecord* rec_host_ = nullptr;
-  double* partials_host_ = nullptr;
+  double* norms_host_ = nullptr;
   int n_blocks_ = 0;
   std::map<int, hipGraphExec_t> graphs_;
   bool use_graphs_ = true;

@@ -21,32 +21,49 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 constexpr int kThreads = 256;
 constexpr int kPad = 2;  // LDS row padding against bank conflicts
 
+// An epilogue names, per output column n, where that column's values go (`column`), then
+// writes row m of it (`put`).
+
 // C[m][n] = v.
 struct Store {
   double* C;
   long ldc;
-  __device__ void operator()(int m, int n, double v) const { C[m * ldc + n] = v; }
+  struct Column {
+    double* p;
+  };
+  __device__ Column column(int n) const { return {C + n}; }
+  __device__ void put(const Column& c, int m, double v) const { c.p[m * ldc] = v; }
 };
 
-// The time collars applied to the projected node streams (see gemm.hpp).
+// The time collars applied to the projected node streams (see gemm.hpp): a polarisation
+// column scaled by g[t] into stream 0, a current column by g[t] into stream 1 and by
+// g'[t] into stream 2. The factors are loaded once per column.
 struct Collar {
   double* streams;
   const double* g;
   const double* gd;
   int C, T;
-  __device__ void operator()(int m, int n, double v) const {
-    if (n < T) {
-      streams[static_cast<long>(m) * T + n] = g[n] * v;
-    } else {
-      const int t = n - T;
-      streams[(static_cast<long>(C) + m) * T + t] = g[t] * v;
-      streams[(2L * C + m) * T + t] = gd[t] * v;
-    }
+  struct Column {
+    double* p0;
+    double* p1;
+    double f0, f1;
+  };
+  __device__ Column column(int n) const {
+    if (n < T) return {streams + n, nullptr, g[n], 0.0};
+    const int t = n - T;
+    return {streams + static_cast<long>(C) * T + t, streams + 2L * C * T + t, g[t], gd[t]};
+  }
+  __device__ void put(const Column& c, int m, double v) const {
+    const long at = static_cast<long>(m) * T;
+    c.p0[at] = c.f0 * v;
+    if (c.p1) c.p1[at] = c.f1 * v;
   }
 };
 
+// Two waves per SIMD: the register budget keeps the accumulators and the staged stage
+// within 256 registers a lane.
 template <int TM, int TN, class Out>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
     dgemm_kernel(int M, int N, int K, const double* __restrict__ A, long lda,
                  const double* __restrict__ B, long ldb, Out out) {
   constexpr int FM = TM / 32, FN = TN / 32;        // MFMA tiles per wave
@@ -121,15 +138,19 @@ __global__ void __launch_bounds__(kThreads)
     buf ^= 1;
   }
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+  for (int j = 0; j < FN; ++j) {
+    const int gn = n0 + wn + 16 * j + lane % 16;
+    if (gn < N) {
+      const typename Out::Column column = out.column(gn);
 #pragma unroll
-    for (int j = 0; j < FN; ++j)
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gm = m0 + wm + 16 * i + lane / 16 + 4 * r;
-        const int gn = n0 + wn + 16 * j + lane % 16;
-        if (gm < M && gn < N) out(gm, gn, acc[i][j][r]);
-      }
+        for (int r = 0; r < 4; ++r) {
+          const int gm = m0 + wm + 16 * i + lane / 16 + 4 * r;
+          if (gm < M) out.put(column, gm, acc[i][j][r]);
+        }
+    }
+  }
 }
 
 // Workgroups of a tile size over an M x N product.
