@@ -13,18 +13,35 @@ constexpr double kRecipFactorial[10] = {1.0,          1.0,          0.5,
                                         1.0 / 720.0,  1.0 / 5040.0, 1.0 / 40320.0,
                                         1.0 / 362880.0};
 constexpr double kFactorial[9] = {1, 1, 2, 6, 24, 120, 720, 5040, 40320};
+// 1/n, rounded to nearest, for the series' divisors n = m + order <= 30.
+constexpr double kRecipInteger[32] = {
+    0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,  1.0 / 6,  1.0 / 7,
+    1.0 / 8,    1.0 / 9,    1.0 / 10, 1.0 / 11, 1.0 / 12, 1.0 / 13, 1.0 / 14, 1.0 / 15,
+    1.0 / 16,   1.0 / 17,   1.0 / 18, 1.0 / 19, 1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23,
+    1.0 / 24,   1.0 / 25,   1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29, 1.0 / 30, 1.0 / 31};
 
-__device__ inline cplx cdiv(cplx a, cplx b) {
-  // Smith's algorithm, as NumPy divides complex numbers.
+// Division by a fixed complex b by Smith's algorithm, as NumPy divides complex numbers:
+// the ratio and scale depend on b alone, so a divisor used several times is prepared once.
+struct Divisor {
+  double rat, scl;
+  bool real_major;  // |b.x| >= |b.y|
+};
+
+__device__ inline Divisor divisor(cplx b) {
   if (fabs(b.x) >= fabs(b.y)) {
     const double rat = b.y / b.x;
-    const double scl = 1.0 / (b.x + b.y * rat);
-    return make((a.x + a.y * rat) * scl, (a.y - a.x * rat) * scl);
+    return {rat, 1.0 / (b.x + b.y * rat), true};
   }
   const double rat = b.x / b.y;
-  const double scl = 1.0 / (b.y + b.x * rat);
-  return make((a.x * rat + a.y) * scl, (a.y * rat - a.x) * scl);
+  return {rat, 1.0 / (b.y + b.x * rat), false};
 }
+
+__device__ inline cplx divide(cplx a, const Divisor& d) {
+  if (d.real_major) return make((a.x + a.y * d.rat) * d.scl, (a.y - a.x * d.rat) * d.scl);
+  return make((a.x * d.rat + a.y) * d.scl, (a.y * d.rat - a.x) * d.scl);
+}
+
+__device__ inline cplx cdiv(cplx a, cplx b) { return divide(a, divisor(b)); }
 
 // phi_0..phi_kOrder at one argument (src/solprop/backends/exponential.py:131-226):
 // scaled Taylor series, downward recurrence and doubling near zero; upward recurrence
@@ -41,21 +58,25 @@ __device__ void phi_functions(cplx x, cplx (&out)[kOrder + 1]) {
   const double magnitude = hypot(x.x, x.y);
   if (magnitude > 3.0) {
     out[0] = cexp(x);
+    const Divisor by_x = divisor(x);
 #pragma unroll
     for (int k = 1; k <= kOrder; ++k)
-      out[k] = cdiv(sub(out[k - 1], make(kRecipFactorial[k - 1], 0.0)), x);
+      out[k] = divide(sub(out[k - 1], make(kRecipFactorial[k - 1], 0.0)), by_x);
     return;
   }
   int squarings = 0;
   if (magnitude > 0.5) squarings = static_cast<int>(ceil(log2(fmax(magnitude, 0.5) / 0.5)));
-  const double factor = exp2(static_cast<double>(squarings));
-  const cplx y = make(x.x / factor, x.y / factor);
+  // Scaling by powers of two is exact: ldexp gives the quotient bit for bit.
+  const cplx y = make(ldexp(x.x, -squarings), ldexp(x.y, -squarings));
   cplx term = make(kRecipFactorial[kOrder], 0.0);
   cplx total = term;
+  // term_m = term_{m-1} y / (m + kOrder), the division as a product with the rounded
+  // reciprocal (within one ulp of the quotient per term; no division in the loop).
+#pragma unroll
   for (int m = 1; m < 24; ++m) {
     const cplx ty = mul(term, y);
-    const double d = static_cast<double>(m + kOrder);
-    term = make(ty.x / d, ty.y / d);
+    const double r = kRecipInteger[m + kOrder];
+    term = make(ty.x * r, ty.y * r);
     total = add(total, term);
   }
   out[kOrder] = total;
@@ -71,8 +92,7 @@ __device__ void phi_functions(cplx x, cplx (&out)[kOrder + 1]) {
       cplx t = mul(out[0], out[k]);
 #pragma unroll
       for (int j = 1; j <= k; ++j) t = add(t, scale(kRecipFactorial[k - j], out[j]));
-      const double p2 = exp2(static_cast<double>(k));
-      doubled[k] = make(t.x / p2, t.y / p2);
+      doubled[k] = make(ldexp(t.x, -k), ldexp(t.y, -k));
     }
 #pragma unroll
     for (int k = 0; k <= kOrder; ++k) out[k] = doubled[k];
@@ -415,22 +435,23 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   }
 }
 
-// Pass E: Kerr, the fraction, the free and loss currents and the collars at every node
-// of the workgroup's tile, projected onto the modes. Each wave takes every
-// kResponseWaves-th node; the waves' projections are summed in wave order in LDS and
-// the tile's samples of the three streams written once.
+// Pass E: Kerr, the fraction, the free and loss currents and the collars at every node,
+// projected onto the modes. A workgroup takes 64 consecutive samples, one a lane (its
+// own blocks, not the scan tiles: T/64 workgroups, T a power of two); each wave takes
+// every kResponseWaves-th node, and the waves' projections are summed in wave order in
+// LDS and written once. A sample's free current is its owning scan tile's offset plus
+// the within-tile current that tile stored: tile t / 63, the last tile for the grid's
+// last sample.
 template <int C>
 __global__ void __launch_bounds__(kWave * kResponseWaves)
     node_response(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
   __shared__ double red[3 * C * kWave];
   const int T = P.T, tiles = scan_tiles(T);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const int tile = blockIdx.x;
-  const int t = tile * kTileSpan + lane;
-  // This lane's sample belongs to this tile unless the next tile's first lane holds it.
-  const bool owned = t < T && (lane < kTileSpan || t == T - 1);
+  const int t = blockIdx.x * kWave + lane;  // < T: the grid is T/64 workgroups
+  const int tile = min(t / kTileSpan, tiles - 1);
   const double density = R->density;
-  const double g = t < T ? P.t_window[t] : 0.0, gd = t < T ? P.t_deriv[t] : 0.0;
+  const double g = P.t_window[t], gd = P.t_deriv[t];
   double am[C];
   load_modal<C>(B, T, t, am);
   double acc[3][C];
@@ -442,7 +463,7 @@ __global__ void __launch_bounds__(kWave * kResponseWaves)
   for (int q = wave; q < P.Q; q += kResponseWaves) {
     const double e = synthesise_at<C>(P.synthesis + static_cast<long>(q) * C, am);
     double current = 0.0;
-    if (P.has_plasma && t < T) {
+    if (P.has_plasma) {
       const long at = static_cast<long>(q) * T + t;
       const double w = B.rate[at], f = B.frac[at];
       const double F = B.tile_f[static_cast<long>(q) * tiles + tile] + B.free_local[at];
@@ -478,13 +499,10 @@ __global__ void __launch_bounds__(kWave * kResponseWaves)
         }
     __syncthreads();
   }
-  if (wave == 0 && owned) {
+  if (wave == 0) {
     double* streams = reinterpret_cast<double*>(B.streams);
 #pragma unroll
-    for (int s = 0; s < 3; ++s)
-#pragma unroll
-      for (int m = 0; m < C; ++m)
-        streams[(static_cast<long>(s) * C + m) * T + t] = red[(s * C + m) * kWave + lane];
+    for (int j = 0; j < 3 * C; ++j) streams[static_cast<long>(j) * T + t] = red[j * kWave + lane];
   }
 }
 
@@ -786,8 +804,9 @@ void launch_node_passes_for(const DeviceProblem& P, const AdamsRecord* R,
     for (int i = 0; i < 4; ++i) mark(i);
   }
   if constexpr (C > 0) {
-    hipLaunchKernelGGL(node_response<C>, dim3(tiles), dim3(kWave * kResponseWaves), 0, s, P,
-                       R, B);
+    SP_CHECK(P.T % kWave == 0, "the node response takes 64-sample blocks");
+    hipLaunchKernelGGL(node_response<C>, dim3(P.T / kWave), dim3(kWave * kResponseWaves), 0,
+                       s, P, R, B);
   } else {
     const std::size_t n = static_cast<std::size_t>(P.Q) * P.T;
     hipLaunchKernelGGL(node_response_dense, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,

@@ -135,3 +135,4 @@ def test_field_graphs_and_direct_launches_agree():
         ws.close()
     os.environ.pop("SOLPROP_MI355X_GRAPH", None)
     assert np.array_equal(results[0], results[1])
+
