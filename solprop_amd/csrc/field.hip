@@ -755,7 +755,7 @@ unsigned cn_blocks(const DeviceProblem& P) {
 void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                           int width, hipStream_t s) {
   SP_CHECK(width >= 2 && width <= kMaxAdamsWidth - 1, "stencil width 2..7");
-  const dim3 grid(correct_blocks(P)), block(kWave);
+  const dim3 grid(blocks_for(static_cast<std::size_t>(P.C) * P.N, kWave)), block(kWave);
   switch (width) {
     case 2: hipLaunchKernelGGL(adams_predict<2>, grid, block, 0, s, P, R, B); break;
     case 3: hipLaunchKernelGGL(adams_predict<3>, grid, block, 0, s, P, R, B); break;

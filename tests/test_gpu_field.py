@@ -136,3 +136,18 @@ def test_field_graphs_and_direct_launches_agree():
     os.environ.pop("SOLPROP_MI355X_GRAPH", None)
     assert np.array_equal(results[0], results[1])
 
+
+
+def test_first_trial_matches_oracle_at_full_size():
+    """The measured workload itself: 4 modes, 128 nodes, 2^14 samples over 400 fs."""
+    print("field probe trial at full size", flush=True)
+    problem = capillary_problem()
+    dz = 1e-4
+    advanced, norms = oracle_first_trial(problem, dz)
+    ws = Workspace(problem)
+    probe = ws.probe(0.0, problem.field0)
+    got = probe.trial(0.0, dz)
+    field = probe.field()
+    ws.close()
+    assert rel_l2(field, advanced.reshape(field.shape)) < 1e-11
+    assert abs(got - norms[0]) / norms[0] < 1e-8

@@ -121,3 +121,46 @@ def test_radial_repeats_agree_to_the_bit():
         ws.close()
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_radial_wide_grid_matches_oracle():
+    """C·N above 4096·64: the corrector runs four-wave workgroups and its block sums are
+    fewer than the bins; every bin of the predicted and corrected fields must still be
+    covered. A first trial to 1e-11 and a short march with the oracle's decisions."""
+    print("radial wide grid", flush=True)
+    problem = small(n_r=128, n_t=4096, w0=0.25e-3)
+    assert problem.n_modes * problem.n_omega > 4096 * 64
+    dz = 1e-4
+    advanced, norms = oracle_first_trial(problem, dz)
+    ws = Workspace(problem)
+    probe = ws.probe(0.0, problem.field0)
+    got = probe.trial(0.0, dz)
+    field = probe.field()
+    ws.close()
+    assert rel_l2(field, advanced.reshape(field.shape)) < 1e-11
+    assert abs(got - norms[0]) / norms[0] < 1e-8
+    y0 = problem.field0.reshape(problem.shape)
+    expected, records = adams.march(problem, y0.copy(), max_attempts=12)
+    ws = Workspace(problem)
+    session = ws.start(0.0, problem.field0)
+    rows = session.advance(problem.z_end, work_limit=len(records)).attempts
+    state = session.state()
+    ws.close()
+    assert [bool(r[3]) for r in rows] == [bool(r[3]) for r in records]
+    assert np.allclose(rows[:, 1], [r[1] for r in records], rtol=1e-7, atol=0)
+    assert rel_l2(state, expected.reshape(state.shape)) < 1e-8
+
+
+def test_radial_first_trial_matches_oracle_at_full_size():
+    """The measured workload itself: 512-point transform, 2^13 samples over 1 ps."""
+    print("radial probe trial at full size", flush=True)
+    problem = radial_problem()
+    dz = 1e-4
+    advanced, norms = oracle_first_trial(problem, dz)
+    ws = Workspace(problem)
+    probe = ws.probe(0.0, problem.field0)
+    got = probe.trial(0.0, dz)
+    field = probe.field()
+    ws.close()
+    assert rel_l2(field, advanced.reshape(field.shape)) < 1e-11
+    assert abs(got - norms[0]) / norms[0] < 1e-8
