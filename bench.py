@@ -29,6 +29,7 @@ import json
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -91,6 +92,20 @@ def trial_flops(problem):
         return {}
     width = 2 * T if problem.plasma is not None else T
     return {"synthesis_gemm": 2.0 * Q * C * T, "projection_gemm": 2.0 * C * Q * width}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter runs (FETCH_SIZE and
+    WRITE_SIZE, each scaled by what it reads for a plain copy of known size), or None when
+    no run covers this workload's dominant kernel."""
+    path = Path(__file__).resolve().parent / "profiles" / "traffic.json"
+    try:
+        row = json.loads(path.read_text()).get(workload)
+    except (OSError, ValueError):
+        return None
+    if not row or row.get("kernel") != kernel:
+        return None
+    return row["bytes"]
 
 
 def workload_capillary():
@@ -209,7 +224,8 @@ def run_gpu(args):
         roofline = {
             "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
             "frac": None if achieved is None else achieved / peak,
-            "traffic": None, "kernel": dominant, **per_launch,
+            "traffic": measured_traffic(args.workload, dominant), "kernel": dominant,
+            **per_launch,
             "avg_launch_us": phases[dominant],
             "trial_us": trial_us, "kernels": kernels, "profiled_trials": trials,
             "timing": "HIP events between consecutive kernels on the workspace stream, "
