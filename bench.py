@@ -63,18 +63,24 @@ def trial_bytes(problem, order):
     plasma = problem.plasma is not None
     n_streams = 3 if plasma else 1
     field = qt if dense else ct + table  # what a node pass reads to know E at its nodes
+    raman = problem.raman is not None
+    # The doubled-grid drive (Q rows of 2T reals): written, transformed both ways (read and
+    # written each time), multiplied by the kernel's spectrum in place.
+    qt2 = 16 * Q * T
     return {
         "adams_predict": cn * (n_phi + 1 + order + (order if comp else 0))
         + cn * (1 + (order if comp else 0)),
         "field_scatter": cn + 8 * T + ct,
         "inverse_fft": 2 * ct,
         "synthesis_gemm": table + ct + qt if dense else 0,
+        "raman": field + qt2 * 7 if raman else 0,
         "plasma_rate_tiles": field + qt + qk,
         "plasma_tile_prefix": 2 * qk,
         "plasma_current_tiles": field + qt + 2 * qk + 2 * qt,
         "plasma_tile_prefix_current": 2 * qk,
         "node_response": (qt + (3 * qt + qk if plasma else 0) + (2 if plasma else 1) * qt)
-        if dense else ct + 2 * table + 3 * qt + qk + 2 * 8 * T + 3 * ct,
+        + (qt if raman else 0)
+        if dense else ct + 2 * table + 3 * qt + qk + 2 * 8 * T + 3 * ct + (qt if raman else 0),
         "projection_gemm": table + (2 if plasma else 1) * qt + 2 * 8 * T + n_streams * ct
         if dense else 0,
         "forward_fft": 2 * n_streams * ct,

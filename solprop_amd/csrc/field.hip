@@ -477,6 +477,11 @@ __global__ void __launch_bounds__(kWave * kResponseWaves)
       const double factor = P.kerr_coefficient * density;
       pol = (factor * (e * e)) * e;
     }
+    if (P.n_raman > 0) {
+      const double* conv = reinterpret_cast<const double*>(B.raman_drive);
+      const double cv = conv[2L * q * T + t] * P.inv_2T;
+      pol = pol + ((P.raman_epsilon0 * density) * e) * cv;
+    }
     pol = pol * g;
     const double gated = g * current, edge = gd * current;
 #pragma unroll
@@ -524,6 +529,11 @@ __global__ void node_response_dense(DeviceProblem P, const AdamsRecord* __restri
     const double factor = P.kerr_coefficient * density;
     pol = (factor * (e * e)) * e;
   }
+  if (P.n_raman > 0) {
+    const double* conv = reinterpret_cast<const double*>(B.raman_drive);
+    const double cv = conv[2L * q * T + t] * P.inv_2T;
+    pol = pol + ((P.raman_epsilon0 * density) * e) * cv;
+  }
   if (!P.has_plasma) {
     B.node_streams[idx] = pol;
     return;
@@ -538,6 +548,50 @@ __global__ void node_response_dense(DeviceProblem P, const AdamsRecord* __restri
   const long row = 2L * q * T;
   B.node_streams[row + t] = pol;
   B.node_streams[row + T + t] = current;
+}
+
+// ------------------------------------------------------------- Raman response
+
+// The damped kernel h(t) = fraction Σ_r undamped_r(t) e^{-Γ_r t} on the doubled grid at
+// the record's rates, two reals a point: z[m] = (h(2m), h(2m+1)), m < T.
+__global__ void field_raman_kernel_fill(DeviceProblem P, const AdamsRecord* __restrict__ R,
+                                        cplx* z) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= P.T) return;
+  const long n2 = 2L * P.T;
+  double h[2];
+  for (int j = 0; j < 2; ++j) {
+    const long t = 2L * m + j;
+    const double delay = static_cast<double>(t) * P.dt_fine;
+    double total = 0.0;
+    for (int r = 0; r < P.n_raman; ++r)
+      total = total + P.raman_rows[r * n2 + t] * exp(-R->raman_rate[r] * delay);
+    h[j] = P.raman_fraction * total;
+  }
+  z[m] = make(h[0], h[1]);
+}
+
+// The drive E² of node q on the doubled grid, two reals a point: (E²(2m), E²(2m+1)) for
+// 2m < T, zeros on the padding half. The field is synthesised from the modal fields
+// (C modes), or on the dense route (C = 0) read from the node field.
+template <int C>
+__global__ void field_raman_drive(DeviceProblem P, FieldBuffers B) {
+  const int T = P.T;
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long>(P.Q) * T) return;
+  const int q = static_cast<int>(idx / T), m = static_cast<int>(idx % T);
+  if (2 * m >= T) {
+    B.raman_drive[idx] = make(0.0, 0.0);
+    return;
+  }
+  double e[2];
+  for (int j = 0; j < 2; ++j) {
+    const int t = 2 * m + j;
+    double am[held(C)];
+    load_modal<C>(B, T, t, am);
+    e[j] = field_at<C>(P, B, am, q, t);
+  }
+  B.raman_drive[idx] = make(e[0] * e[0], e[1] * e[1]);
 }
 
 // One coarse bin b of mode c from its two packed entries per stream (za at b, zb at its
@@ -836,6 +890,33 @@ void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const Plas
     case 7: launch_node_passes_for<7>(P, R, pt, B, s, marks); break;
     default: launch_node_passes_for<8>(P, R, pt, B, s, marks); break;
   }
+}
+
+void launch_field_raman_kernel_fill(const DeviceProblem& P, const AdamsRecord* R, cplx* z,
+                                    hipStream_t s) {
+  hipLaunchKernelGGL(field_raman_kernel_fill, dim3(blocks_for(P.T, kThreads)), dim3(kThreads),
+                     0, s, P, R, z);
+  check();
+}
+
+void launch_field_raman_drive(const DeviceProblem& P, FieldBuffers B, hipStream_t s) {
+  SP_CHECK(B.raman_drive, "the Raman drive buffer");
+  const dim3 grid(blocks_for(static_cast<std::size_t>(P.Q) * P.T, kThreads)), block(kThreads);
+  if (B.node_field) {
+    hipLaunchKernelGGL(field_raman_drive<0>, grid, block, 0, s, P, B);
+  } else {
+    switch (P.C) {
+      case 1: hipLaunchKernelGGL(field_raman_drive<1>, grid, block, 0, s, P, B); break;
+      case 2: hipLaunchKernelGGL(field_raman_drive<2>, grid, block, 0, s, P, B); break;
+      case 3: hipLaunchKernelGGL(field_raman_drive<3>, grid, block, 0, s, P, B); break;
+      case 4: hipLaunchKernelGGL(field_raman_drive<4>, grid, block, 0, s, P, B); break;
+      case 5: hipLaunchKernelGGL(field_raman_drive<5>, grid, block, 0, s, P, B); break;
+      case 6: hipLaunchKernelGGL(field_raman_drive<6>, grid, block, 0, s, P, B); break;
+      case 7: hipLaunchKernelGGL(field_raman_drive<7>, grid, block, 0, s, P, B); break;
+      default: hipLaunchKernelGGL(field_raman_drive<8>, grid, block, 0, s, P, B); break;
+    }
+  }
+  check();
 }
 
 void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,

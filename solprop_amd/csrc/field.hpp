@@ -55,6 +55,7 @@ struct AdamsRecord {
   int order, width, n_orders, n_read, compensated, free_slot, pad0, pad1;
   int orders[kMaxScored];
   int pad2;
+  double raman_rate[kMaxTerms];          // Raman decay rates at the source position
 };
 
 struct FieldBuffers {
@@ -72,6 +73,10 @@ struct FieldBuffers {
   double* free_local;  // (Q, T) free current since the start of the sample's scan tile
   double* tile_x;   // (Q, tiles) exposure tile sums, then offsets
   double* tile_f;   // (Q, tiles) free-current tile sums, then offsets
+  cplx* raman_drive;     // (Q, T) the drive on the doubled grid, two reals a point; then
+                         // its convolution with the kernel (times 2T)
+  cplx* raman_H;         // (T + 1) the kernel's spectrum on the doubled grid, dt folded in
+  const cplx* roots_2T;  // unit roots of 2T
   double* node_field;    // (Q, T) field at the nodes (dense route only)
   double* node_streams;  // (Q, 2, T) un-gated polarisation and current (dense route only)
   double* partials; // (1 + kMaxScored, blocks) masked squared sums of each block
@@ -88,6 +93,13 @@ struct PlasmaTable {
 
 unsigned correct_threads(const DeviceProblem& P);
 unsigned correct_blocks(const DeviceProblem& P);
+
+// The Raman response (real square drive): the damped kernel at the record's rates,
+// packed two reals a point over the doubled grid (`z`, T points) for its transform; and
+// the drive E² at every node, zero on the padding half, into B.raman_drive.
+void launch_field_raman_kernel_fill(const DeviceProblem& P, const AdamsRecord* R, cplx* z,
+                                    hipStream_t s);
+void launch_field_raman_drive(const DeviceProblem& P, FieldBuffers B, hipStream_t s);
 
 // The predictor and corrector are compiled per stencil width (2..7); `width` must be
 // the record's.

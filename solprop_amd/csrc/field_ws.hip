@@ -1,6 +1,7 @@
 // The modal field-grid workspace under the exponential Adams march (see field_ws.hpp).
 #include "field_ws.hpp"
 #include "gemm.hpp"
+#include "kernels.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -39,7 +40,11 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   SP_CHECK(Q >= 1, "at least one quadrature node");
   SP_CHECK(d.n_phi >= 1 && d.n_phi <= kMaxTerms, "1..4 phi terms");
   SP_CHECK(d.n_prefactor >= 1 && d.n_prefactor <= kMaxTerms, "1..4 prefactor terms");
-  SP_CHECK(d.n_raman == 0, "the field route runs Kerr and photoionisation (no Raman)");
+  SP_CHECK(d.n_raman >= 0 && d.n_raman <= kMaxTerms, "0..4 Raman rows");
+  SP_CHECK(d.n_raman == 0 || d.raman_thg != 0,
+           "the field route's Raman drive is the real square E^2; the analytic-signal "
+           "drive (a grid without the harmonic) is not built");
+  SP_CHECK(d.n_raman == 0 || d.raman_undamped, "the Raman rows are missing");
   SP_CHECK(!d.has_plasma || d.n_taps <= 1,
            "the band running-integral rule is not built; the trapezoid rule is");
   SP_CHECK(d.adams_order >= 1 && d.adams_order + 1 < kMaxAdamsWidth, "Adams order 1..6");
@@ -62,6 +67,11 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   for (int t = 0; t < d.n_prefactor; ++t) {
     pref_paths_.emplace_back();
     pref_paths_.back().assign(d.prefactor_paths[t]);
+  }
+  for (int r = 0; r < d.n_raman; ++r) {
+    raman_paths_.emplace_back();
+    raman_paths_.back().assign(d.raman_rates[r]);
+    raman_dynamic_ = raman_dynamic_ || !raman_paths_.back().constant();
   }
   density_.assign(d.density);
   preion_.assign(d.preionisation);
@@ -113,6 +123,11 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   P_.plasma_ip = d.ionisation_potential;
   P_.dt_fine = d.dt_fine;
   P_.inv_T = 1.0 / T;
+  P_.inv_2T = 1.0 / (2.0 * T);
+  P_.n_raman = d.n_raman;
+  P_.raman_fraction = d.raman_fraction;
+  P_.raman_epsilon0 = d.raman_epsilon0;
+  P_.raman_rows = d.n_raman ? upload(d.raman_undamped, d.n_raman * 2UL * T) : nullptr;
   std::vector<int> crop(N);
   for (int b = 0; b < N; ++b) crop[b] = b;
   P_.crop = upload(crop.data(), crop.size());
@@ -178,12 +193,32 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   // Without plasma only the polarisation stream is transformed (the finish reads the
   // current streams on plasma runs alone).
   fwd_plan_.init(T / 2, (d.has_plasma ? 3 : 1) * C);
+  if (d.n_raman) {
+    // The doubled-grid drive of every node, its transform scratch, the kernel's packed
+    // samples and spectrum, and the roots of 2T for the real-transform packing.
+    B_.raman_drive = alloc<cplx>(static_cast<std::size_t>(Q) * T);
+    raman_scratch_ = alloc<cplx>(static_cast<std::size_t>(Q) * T);
+    raman_kernel_z_ = alloc<cplx>(T);
+    B_.raman_H = alloc<cplx>(T + 1UL);
+    const auto roots2 = unit_roots(2 * static_cast<std::size_t>(T));
+    B_.roots_2T = upload(roots2.data(), roots2.size());
+    raman_plan_.init(T, Q);
+    raman_kernel_plan_.init(T, 1);
+  }
   rec_dev_ = alloc<AdamsRecord>(1);
   SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&rec_host_), sizeof(AdamsRecord)));
   std::memset(rec_host_, 0, sizeof(AdamsRecord));
   for (hipEvent_t& e : marks_) SP_HIP(hipEventCreate(&e));
   SP_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
   slot_rates_.assign(S, std::vector<double>(kMaxTerms, 0.0));
+  if (d.n_raman && !raman_dynamic_) {
+    // Constant rates: the kernel's spectrum once, here.
+    fill_position(0.0);
+    SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
+                          stream_));
+    enqueue_raman_kernel();
+    SP_HIP(hipStreamSynchronize(stream_));
+  }
 }
 
 FieldWorkspace::~FieldWorkspace() { release(); }
@@ -214,7 +249,7 @@ namespace {
 // The kernels (or transform plans) of one trial, in launch order.
 constexpr const char* kPhaseNames[FieldWorkspace::kPhases] = {
     "adams_predict",      "field_scatter",        "inverse_fft",
-    "synthesis_gemm",     "plasma_rate_tiles",    "plasma_tile_prefix",
+    "synthesis_gemm",     "raman",                "plasma_rate_tiles",    "plasma_tile_prefix",
     "plasma_current_tiles", "plasma_tile_prefix_current", "node_response",
     "projection_gemm",    "forward_fft",          "field_finish",
     "adams_correct",      "adams_norms"};
@@ -240,6 +275,24 @@ void FieldWorkspace::enqueue_projection() {
                            reinterpret_cast<double*>(B_.streams), stream_);
 }
 
+// The Raman kernel's spectrum at the record's rates: H = rfft(h) dt on the doubled grid.
+void FieldWorkspace::enqueue_raman_kernel() {
+  launch_field_raman_kernel_fill(P_, rec_dev_, raman_kernel_z_, stream_);
+  raman_kernel_plan_.exec(raman_kernel_z_, raman_scratch_, -1, stream_);
+  launch_raman_kernel_post(raman_kernel_z_, B_.raman_H, B_.roots_2T, P_.T, P_.dt_fine, stream_);
+}
+
+// The Raman convolution of every node's drive: E² on the doubled grid, transformed,
+// times the kernel's spectrum, transformed back (the node passes read it).
+void FieldWorkspace::enqueue_raman() {
+  if (!P_.n_raman) return;
+  if (raman_dynamic_) enqueue_raman_kernel();
+  launch_field_raman_drive(P_, B_, stream_);
+  raman_plan_.exec(B_.raman_drive, raman_scratch_, -1, stream_);
+  launch_raman_product(B_.raman_drive, B_.raman_H, B_.roots_2T, P_.T, P_.Q, stream_);
+  raman_plan_.exec(B_.raman_drive, raman_scratch_, +1, stream_);
+}
+
 void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
   launch_field_scatter(P_, spectrum, B_, stream_);
   mark(2);
@@ -247,13 +300,15 @@ void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
   mark(3);
   enqueue_synthesis();
   mark(4);
-  launch_node_passes(P_, rec_dev_, pt_, B_, stream_, marking_ ? marks_ + 5 : nullptr);
+  enqueue_raman();
+  mark(5);
+  launch_node_passes(P_, rec_dev_, pt_, B_, stream_, marking_ ? marks_ + 6 : nullptr);
   enqueue_projection();
-  mark(10);
-  fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
   mark(11);
-  launch_field_finish(P_, rec_dev_, B_, out, stream_);
+  fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
   mark(12);
+  launch_field_finish(P_, rec_dev_, B_, out, stream_);
+  mark(13);
 }
 
 void FieldWorkspace::enqueue_trial(bool commit) {
@@ -265,9 +320,9 @@ void FieldWorkspace::enqueue_trial(bool commit) {
   mark(1);
   enqueue_source(B_.lab, B_.source);
   launch_adams_correct(P_, rec_dev_, B_, width_, stream_);
-  mark(13);
-  launch_adams_norms(P_, B_, stream_);
   mark(14);
+  launch_adams_norms(P_, B_, stream_);
+  mark(15);
 }
 
 void FieldWorkspace::enqueue_start() {
@@ -340,6 +395,7 @@ void FieldWorkspace::fill_position(double zp) {
   for (int t = 0; t < P_.n_pref; ++t) r.pref_w[t] = pref_paths_[t](zp);
   r.density = density_(zp);
   r.preionisation = preion_(zp);
+  for (int i = 0; i < P_.n_raman; ++i) r.raman_rate[i] = raman_paths_[i](zp);
   r.compensated = compensated_ ? 1 : 0;
 }
 
@@ -568,6 +624,7 @@ void FieldWorkspace::source(double z, const double* lab, double* out, double* st
   launch_field_scatter(P_, B_.lab, B_, stream_);
   inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
   enqueue_synthesis();
+  enqueue_raman();
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
   enqueue_projection();
   SP_HIP(hipStreamSynchronize(stream_));
@@ -616,7 +673,8 @@ sp_counters FieldWorkspace::counters() const {
   c.accepted = accepted_;
   c.graph_launches = graph_launches_;
   c.device_bytes =
-      static_cast<long>(device_bytes_ + inv_plan_.bytes() + fwd_plan_.bytes());
+      static_cast<long>(device_bytes_ + inv_plan_.bytes() + fwd_plan_.bytes() +
+                        raman_plan_.bytes() + raman_kernel_plan_.bytes());
   return c;
 }
 

@@ -37,7 +37,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   long buffer(const char* name, double* out, long count) override;
   void set_profile(bool on) override { profile_ = on; }
   std::vector<std::pair<const char*, double>> phases() const override;
-  static constexpr int kPhases = 14;
+  static constexpr int kPhases = 15;
   double trial_us() const override { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
   long timed() const override { return timed_; }
 
@@ -58,6 +58,8 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   void enqueue_source(const cplx* spectrum, cplx* out);
   void enqueue_synthesis();
   void enqueue_projection();
+  void enqueue_raman_kernel();
+  void enqueue_raman();
   enum Graph { kTrial = 1, kStart = 2, kCommitTrial = 3 };
   void enqueue_trial(bool commit);
   void enqueue_start();
@@ -82,7 +84,12 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   std::map<int, hipGraphExec_t> graphs_;
   bool use_graphs_ = true;
   // Host scalar paths.
-  std::vector<ScalarPath> phi_paths_, rate_paths_, pref_paths_;
+  std::vector<ScalarPath> phi_paths_, rate_paths_, pref_paths_, raman_paths_;
+  bool raman_dynamic_ = false;  // Raman rates vary along z: the kernel is rebuilt per source
+  FftPlan raman_plan_;          // T points, Q rows: the doubled-grid drive
+  FftPlan raman_kernel_plan_;   // T points, one row: the kernel
+  cplx* raman_scratch_ = nullptr;
+  cplx* raman_kernel_z_ = nullptr;
   ScalarPath density_, preion_;
   double rtol_ = 1e-6, atol_ = 0.0;
   bool compensated_ = false;

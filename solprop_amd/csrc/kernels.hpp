@@ -16,7 +16,9 @@ void launch_rk_entry_envelope(const DeviceProblem& P, const TrialRecord* R, cons
                               const cplx* k, cplx* fine, cplx* einv, int stage, hipStream_t s);
 void launch_env_raman_drive(const DeviceProblem& P, const TrialRecord* R, const cplx* fine,
                             cplx* drive, int stage, hipStream_t s);
-void launch_raman_product(cplx* z, const cplx* H, const cplx* roots2T, int T, hipStream_t s);
+// Over `rows` consecutive rows of T packed points, one kernel spectrum for all.
+void launch_raman_product(cplx* z, const cplx* H, const cplx* roots2T, int T, int rows,
+                          hipStream_t s);
 void launch_raman_kernel_fill(const DeviceProblem& P, const TrialRecord* R, int stage, cplx* z,
                               hipStream_t s);
 void launch_raman_kernel_post(const cplx* z, cplx* H, const cplx* roots2T, int T, double dt,

@@ -93,11 +93,13 @@ __global__ void env_raman_drive(DeviceProblem P, const TrialRecord* R, const cpl
   drive[m] = make(0.5 * (e0.x * e0.x + e0.y * e0.y), 0.5 * (e1.x * e1.x + e1.y * e1.y));
 }
 
-// One pair (k, T-k) of the packed doubled-grid transform: unpack the real transform,
-// multiply by the kernel spectrum, and pack again for the inverse, in place.
+// One pair (k, T-k) of the packed doubled-grid transform of row blockIdx.y: unpack the
+// real transform, multiply by the kernel spectrum, and pack again for the inverse, in
+// place.
 __global__ void raman_product(cplx* z, const cplx* H, const cplx* roots2T, int T) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > T / 2) return;
+  z += static_cast<long>(blockIdx.y) * T;
   const int kk = (T - k) % T;
   const cplx za = z[k], zb = z[kk];
   const cplx xa = r2c_post(za, zb, roots2T[k]);
@@ -262,9 +264,10 @@ void launch_env_raman_drive(const DeviceProblem& P, const TrialRecord* R, const 
   check_launch();
 }
 
-void launch_raman_product(cplx* z, const cplx* H, const cplx* roots2T, int T, hipStream_t s) {
-  hipLaunchKernelGGL(raman_product, dim3(blocks_for(T / 2 + 1, kThreads)), dim3(kThreads), 0, s,
-                     z, H, roots2T, T);
+void launch_raman_product(cplx* z, const cplx* H, const cplx* roots2T, int T, int rows,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(raman_product, dim3(blocks_for(T / 2 + 1, kThreads), rows),
+                     dim3(kThreads), 0, s, z, H, roots2T, T);
   check_launch();
 }
 

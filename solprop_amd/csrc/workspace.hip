@@ -214,7 +214,7 @@ void Workspace::enqueue_stage(int stage) {
     if (raman_dynamic_) enqueue_raman_kernel(stage);
     launch_env_raman_drive(P_, rec_dev_, fine_, drive_, stage, stream_);
     raman_plan_.exec(drive_, scratch_, -1, stream_);
-    launch_raman_product(drive_, H_, roots2T_, P_.T, stream_);
+    launch_raman_product(drive_, H_, roots2T_, P_.T, 1, stream_);
     raman_plan_.exec(drive_, scratch_, +1, stream_);
   }
   launch_env_response(P_, rec_dev_, fine_, drive_, stage, stream_);

@@ -534,6 +534,32 @@ def rate_table(rate, field_min: float, field_max: float, n_intervals: int = 512)
     )
 
 
+def oscillator_raman(n_to: int, dt: float, per_molecule: float, tau1: float,
+                     rate: "ScalarPath") -> RamanGroup:
+    """A molecular Raman response on a real field grid: one damped-oscillator row.
+
+    ``h(t) = c (ω² + Γ²)/ω e^{-Γt} sin(ωt)`` with ``ω = 1/tau1`` and the decay ``Γ`` read
+    from the path ``rate``; the undamped row (without ``e^{-Γt}``) fills the first half of
+    the doubled grid, so ``∫h dt = c`` with ``c = per_molecule`` (the response per
+    molecule, in the units of the Kerr coefficient). The drive is the real square ``E²``
+    (the field grid keeps the harmonic): ``P = ε0 ρ E (h * E²)``
+    (``passes/responses.py:395-499``).
+    """
+    delays = np.arange(2 * n_to) * dt
+    frequency = 1.0 / tau1
+    decay = rate(0.0)
+    amplitude = (frequency**2 + decay**2) / frequency
+    row = np.where(delays > 0.0, amplitude * np.sin(frequency * delays), 0.0)
+    row[n_to:] = 0.0  # the padding half of the doubled grid stays zero
+    return RamanGroup(
+        undamped=np.ascontiguousarray(row.reshape(1, -1)),
+        fraction=per_molecule / EPS0,
+        epsilon0=EPS0,
+        rates=(rate,),
+        thg=True,
+    )
+
+
 def capillary_problem(
     *,
     n_t: int = 2**14,
@@ -554,6 +580,8 @@ def capillary_problem(
     kerr: bool = True,
     cladding_index: float = 1.45,
     time_collar: float = 0.06,
+    raman_share: float = 0.0,
+    raman_tau: tuple = (60e-15, 200e-15),
 ) -> Problem:
     """Gas-filled capillary on a real field grid: HE_1m modes, a pressure gradient.
 
@@ -656,9 +684,21 @@ def capillary_problem(
     field0 = np.zeros((n_modes, n), dtype=np.complex128)
     field0[0] = np.fft.rfft(a_t) * active
     kerr_group = None
+    chi3_per_rho = (4.0 / 3.0) * HE_N2_PER_BAR * EPS0 * c / (1e5 / (KB * 273.15))
     if kerr:
-        chi3_per_rho = (4.0 / 3.0) * HE_N2_PER_BAR * EPS0 * c / (1e5 / (KB * 273.15))
-        kerr_group = KerrGroup(EPS0 * chi3_per_rho, "real")
+        kerr_group = KerrGroup(EPS0 * chi3_per_rho * (1.0 - raman_share), "real")
+    raman_group = None
+    if raman_share > 0.0:
+        # A molecular response carrying `raman_share` of the third-order response, its
+        # dephasing rate growing with the local pressure (collisions).
+        tau1, tau2 = raman_tau
+
+        def decay(z):
+            return (0.5 + 0.5 * pressure(z) / max(p0, 1e-12)) / tau2
+
+        raman_group = oscillator_raman(
+            n_to, dt, EPS0 * chi3_per_rho * raman_share, tau1,
+            ScalarPath.chebyshev(decay, 0.0, length, degree=24, edges=edges))
     plasma_group = None
     if plasma:
         # Below 3 GV/m helium's rate is under 1e-150 /s: the table is exactly zero there.
@@ -687,7 +727,7 @@ def capillary_problem(
         density=rho_path,
         preionisation=ScalarPath.constant(0.0),
         kerr=kerr_group,
-        raman=None,
+        raman=raman_group,
         plasma=plasma_group,
         field0=np.ascontiguousarray(field0),
         noise0=None,
@@ -757,6 +797,8 @@ def radial_problem(
     plasma: bool = True,
     kerr: bool = True,
     time_collar: float = 0.05,
+    raman_share: float = 0.0,
+    raman_tau: tuple = (60e-15, 200e-15),
 ) -> Problem:
     """Free-space radially symmetric beam in argon on a real field grid.
 
@@ -813,9 +855,15 @@ def radial_problem(
     field0 = np.ascontiguousarray(forward @ spectrum_r)
     density = 1e5 * pressure / (KB * temperature)
     kerr_group = None
+    chi3 = (4.0 / 3.0) * AR_N2_PER_BAR * pressure * EPS0 * c
     if kerr:
-        chi3 = (4.0 / 3.0) * AR_N2_PER_BAR * pressure * EPS0 * c
-        kerr_group = KerrGroup(EPS0 * chi3 / density, "real")
+        kerr_group = KerrGroup(EPS0 * chi3 / density * (1.0 - raman_share), "real")
+    raman_group = None
+    if raman_share > 0.0:
+        # A molecular response carrying `raman_share` of the third-order response (argon
+        # itself, an atom, has none: this is for molecular gases and the tests).
+        raman_group = oscillator_raman(n_to, dt, EPS0 * chi3 / density * raman_share,
+                                       raman_tau[0], ScalarPath.constant(1.0 / raman_tau[1]))
     plasma_group = None
     if plasma:
         plasma_group = rate_table(lambda f: adk_rate(f, AR_IONISATION_EV), 1e9, 2e12)
@@ -843,7 +891,7 @@ def radial_problem(
         density=ScalarPath.constant(density),
         preionisation=ScalarPath.constant(0.0),
         kerr=kerr_group,
-        raman=None,
+        raman=raman_group,
         plasma=plasma_group,
         field0=field0,
         noise0=None,

@@ -18,8 +18,11 @@ def oracle_source_stages(problem, lab, z):
     crop = problem.crop_index.astype(np.intp)
     modal = P.to_time(lab, crop, problem.n_to, problem.kind)
     field_t = P.synthesise(modal, problem.synthesis, 1.0, 1)
+    raman = problem.raman
     s = P.nonlinear_sources(
-        field_t, kerr=src.kerr, raman=None, raman_thg=False, plasma=src.plasma,
+        field_t, kerr=src.kerr, raman=src.raman_kernel(z),
+        raman_thg=False if raman is None else raman.thg,
+        raman_epsilon0=P.EPS0 if raman is None else raman.epsilon0, plasma=src.plasma,
         t_window_fine=problem.time_window, kind=problem.kind, density=problem.density(z),
         preionisation=problem.preionisation(z), dt=problem.dt_fine)
     streams = np.zeros((3, problem.n_modes, problem.n_to))

@@ -35,6 +35,9 @@ CASES = {
     "kerr_only": {"plasma": False},
     "plasma_only": {"kerr": False},
     "constant_pressure": {"p1": 0.8},
+    # A molecular response taking half the third-order response, its dephasing rate
+    # following the pressure (the kernel is rebuilt at every source position).
+    "kerr_raman": {"raman_share": 0.5},
 }
 
 
@@ -85,7 +88,7 @@ def test_first_trial_matches_oracle(case):
 
 
 @pytest.mark.parametrize("case,proposal", [("kerr_plasma", None), ("constant_pressure", None),
-                                           ("kerr_plasma", 2e-3)])
+                                           ("kerr_plasma", 2e-3), ("kerr_raman", None)])
 def test_march_matches_oracle_decisions(case, proposal):
     print(f"field march {case} proposal={proposal}", flush=True)
     problem = small(**CASES[case])

@@ -29,6 +29,9 @@ CASES = {
     # A 0.25 mm beam: a strong plasma current.
     "narrow_beam": {"w0": 0.25e-3},
     "kerr_only": {"w0": 0.25e-3, "plasma": False},
+    # A molecular response taking half the third-order response (constant rate: the
+    # kernel is built once).
+    "raman": {"w0": 0.25e-3, "raman_share": 0.5},
 }
 
 
@@ -90,7 +93,7 @@ def test_radial_first_trial_matches_oracle(case):
     assert abs(got - norms[0]) / norms[0] < 1e-8
 
 
-@pytest.mark.parametrize("case", ["wide_beam", "narrow_beam"])
+@pytest.mark.parametrize("case", ["wide_beam", "narrow_beam", "raman"])
 def test_radial_march_matches_oracle_decisions(case):
     print(f"radial march {case}", flush=True)
     problem = small(**CASES[case])

@@ -49,3 +49,10 @@ def test_envelope_with_several_modes_is_refused():
         projection=np.ones((2, 1)))
     with pytest.raises((BackendRefusal, ValueError)):
         Workspace(two)
+
+
+def test_field_raman_without_the_harmonic_is_refused():
+    """The analytic-signal drive (|E_a|²/2, a grid without the harmonic) is not built."""
+    p = capillary_problem(n_t=256, n_nodes=4, raman_share=0.5)
+    refused(dataclasses.replace(p, raman=dataclasses.replace(p.raman, thg=False)),
+            "analytic-signal")
