@@ -83,7 +83,8 @@ def trial_bytes(problem, order):
         if dense else ct + 2 * table + 3 * qt + qk + 2 * 8 * T + 3 * ct + (qt if raman else 0),
         "projection_gemm": table + (2 if plasma else 1) * qt + 2 * 8 * T + n_streams * ct
         if dense else 0,
-        "forward_fft": 2 * n_streams * ct,
+        "absorber": 2 * ct + 8 * T if problem.absorber_scheme == "in_stage" else 0,
+        "forward_fft": 2 * (n_streams + (problem.absorber_scheme == "in_stage")) * ct,
         "field_finish": n_streams * ct + cn * len(problem.prefactor_paths) + cn,
         "adams_correct": cn * (n_phi + 1 + (W - 1) + 1 + (order + 1 if comp else 0)) + cn,
         "adams_norms": 8 * (1 + 3) * ((C * N + (64 if C * N <= 4096 * 64 else 256) - 1)
@@ -114,10 +115,10 @@ def measured_traffic(workload, kernel):
     return row["bytes"]
 
 
-def workload_capillary():
+def workload_capillary(raman_share=0.0):
     from solprop_amd.problem import capillary_problem
 
-    problem = capillary_problem()
+    problem = capillary_problem(raman_share=raman_share)
     config = {
         "workload": "gas-filled capillary UPPE, 4 HE_1m modes, 125 um bore, 1.5 m helium "
                     "0.8->0 bar, real field 2^14 samples (8193 bins) over 400 fs, 7.5 fs "
@@ -129,10 +130,10 @@ def workload_capillary():
     return problem, config
 
 
-def workload_radial():
+def workload_radial(raman_share=0.0):
     from solprop_amd.problem import radial_problem
 
-    problem = radial_problem()
+    problem = radial_problem(raman_share=raman_share)
     config = {
         "workload": "free-space radial beam in argon, 1 bar, 0.5 m; 100 fs 800 nm Gaussian, "
                     "w0 1 mm, 2 mJ; 512-point Hankel transform (dense 512x512) over 4097 "
@@ -161,8 +162,14 @@ def run_gpu(args):
 
     from solprop_amd.native import Workspace
 
-    problem, config = {"capillary": workload_capillary, "radial": workload_radial,
-                       "envelope": workload_envelope}[args.workload]()
+    if args.workload == "envelope":
+        problem, config = workload_envelope()
+    else:
+        builder = workload_capillary if args.workload == "capillary" else workload_radial
+        problem, config = builder(args.raman_share)
+        if args.raman_share > 0.0:
+            config["workload"] += (f"; molecular Raman response taking {args.raman_share:g} "
+                                   "of the third-order response")
     if args.copy_probe_mib > 0:
         # A plain device copy of known size, three times, for scaling memory counters.
         n = args.copy_probe_mib * (1 << 20) // 8
@@ -303,6 +310,9 @@ def main():
     parser.add_argument("--copy-probe-mib", type=int, default=0,
                         help="run a plain device copy of this many MiB first (counter scaling)")
     parser.add_argument("--no-cpu", action="store_true")
+    parser.add_argument("--raman-share", type=float, default=0.0,
+                        help="add a damped-oscillator Raman response taking this share of "
+                             "the third-order response (field workloads; off by default)")
     args = parser.parse_args()
     if args.gpus != 1:
         print("bench.py runs one GPU: one propagation problem runs on one device, and this "

@@ -120,6 +120,13 @@ typedef struct sp_problem_desc {
   int method;
   int adams_order, adams_variable_order, adams_equal_approach;
   double adams_safety, adams_max_growth, adams_min_factor;
+  /* Temporal absorber (Problem.Absorber: in_stage, split_step, fine, coarse), the collar
+   * as a power absorption rate alpha(t) in 1/m: 0 none; 1 in-stage (-alpha E / 2 on the
+   * fine time grid, transformed and added to every source on the active band,
+   * passes/stepping.py:286-345); 2 split-step (exp(-alpha h / 2) on the coarse time grid
+   * once per accepted step, passes/stepping.py:410-456). Field routes only. */
+  int absorber_scheme;
+  const double* absorber_rate;      /* (n_to) in-stage, (n_t) split-step */
 } sp_problem_desc;
 
 typedef struct sp_workspace sp_workspace;

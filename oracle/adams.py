@@ -741,6 +741,12 @@ class ExponentialEngine:
             self.states[free] = lab
             self.rates[free] = endpoint_rate
         self.state = P.apply_hard_mask(advanced, self.p.active_mask)
+        if self.p.absorber_scheme == "split_step":
+            # The workspace's finish: mask, the split collar, mask again
+            # (``numpy_ref/workspace.py:1020-1108``).
+            self.state = P.apply_hard_mask(
+                P.apply_temporal_rate(self.state, self.p.absorber_rate, dz, self.p.kind),
+                self.p.active_mask)
 
 
 class _PlainHost:

@@ -157,6 +157,22 @@ def error_norm(error, state, state_new, norm_mask, rtol, atol, weights=None):
     return float(np.linalg.norm(residual))
 
 
+def absorber_term(modal_t, temporal_rate_fine, crop, active_mask, kind: str):
+    """The in-stage absorber ``F[-alpha(t) E(t) / 2]``, cropped and masked to the active
+    band (``passes/stepping.py:286-345``; the term is ``boundaries.py:1159``
+    ``nonlinear_absorber_term``). The radial route's transverse collar matrix is not
+    restated (the problems here carry none)."""
+    term = -0.5 * temporal_rate_fine * modal_t
+    return to_frequency(term, crop, kind) * active_mask
+
+
+def apply_temporal_rate(state, rate, dz, kind: str):
+    """``y <- F[F^-1[y] exp(-alpha(t) h / 2)]`` on the coarse grid, the split-step
+    absorber (``passes/stepping.py:410-456``)."""
+    time = inverse(state, kind)
+    return forward(time * np.exp(-0.5 * rate * dz), kind)
+
+
 def apply_hard_mask(state, active_mask):
     """``y * active_mask`` once per accepted step (``passes/stepping.py:372-407``)."""
     return state * active_mask

@@ -124,8 +124,12 @@ class StageSource:
             spectrum = spectrum + P.held_plasma_spectrum(
                 gated, edge, p.omega, crop, p.active_mask
             )
+        absorber = None
+        if p.absorber_scheme == "in_stage":
+            # ADR-025's in-stage absorber, a term of its own (``source.py:391-401``).
+            absorber = P.absorber_term(modal_t, p.absorber_rate, crop, p.active_mask, kind)
         return P.stage_output(
-            spectrum, self.prefactor(z), p.omega_window, inverse_propagator
+            spectrum, self.prefactor(z), p.omega_window, inverse_propagator, absorber
         )
 
 

@@ -60,6 +60,12 @@ struct DeviceProblem {
   const double* synthesis;     // (Q, C)
   const double* projection;    // (C, Q)
   const double* raman_rows;    // (n_raman, 2T)
+  // The temporal absorber (field routes): the power rate alpha(t), 1/m, on the fine grid
+  // (in-stage scheme) or the coarse grid (split-step scheme); null without one. The
+  // in-stage term -alpha E / 2 rides the forward transform as stream `absorber_stream`
+  // (-1: none).
+  const double* absorber_rate;
+  int absorber_stream;
   // The Runge-Kutta tableau.
   int n_stages;
   const double* tab_a;         // (n_stages, n_stages)

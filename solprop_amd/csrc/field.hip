@@ -598,8 +598,8 @@ __global__ void field_raman_drive(DeviceProblem P, FieldBuffers B) {
 // mirror): unpack the stream spectra, assemble the held current on the active bins and
 // normalise, k = (p W) (P~ + G~/(iω) - E~/(iω)^2).
 __device__ inline void finish_bin(const DeviceProblem& P, const AdamsRecord& r,
-                                  const FieldBuffers& B, int c, int b, const cplx (&za)[3],
-                                  const cplx (&zb)[3], int n_streams, cplx* out) {
+                                  const FieldBuffers& B, int c, int b, const cplx (&za)[4],
+                                  const cplx (&zb)[4], int n_streams, cplx* out) {
   const long CN = static_cast<long>(P.C) * P.N;
   const long idx = static_cast<long>(c) * P.N + b;
   cplx X[3] = {};
@@ -613,7 +613,14 @@ __device__ inline void finish_bin(const DeviceProblem& P, const AdamsRecord& r,
     spectrum = add(spectrum, sub(cdiv(X[1], q), cdiv(X[2], q2)));
   }
   const cplx pref = weighted(r.pref_w, P.pref, P.n_pref, CN, idx);
-  out[idx] = mul(scale(P.omega_window[b], pref), spectrum);
+  cplx value = mul(scale(P.omega_window[b], pref), spectrum);
+  // The in-stage absorber's spectrum, on the active band, without the prefactor. Its
+  // stream follows the polarisation (and the two currents with plasma).
+  if (P.absorber_stream >= 0 && P.active[b]) {
+    const cplx a = P.has_plasma ? za[3] : za[1], ab = P.has_plasma ? zb[3] : zb[1];
+    value = add(value, r2c_post(a, ab, B.roots_T[b]));
+  }
+  out[idx] = value;
 }
 
 // The finish over bin pairs: the packed real transform holds bins k and T/2 - k in the
@@ -630,10 +637,10 @@ __global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R,
   if (k >= P.N) return;         // neither bin is kept (mirror >= k)
   const int ka = k, kb = mirror % half;
   const int n_streams = P.has_plasma ? 3 : 1;  // the currents are transformed with plasma
-  cplx za[3] = {}, zb[3] = {};
+  cplx za[4] = {}, zb[4] = {};
 #pragma unroll
-  for (int s = 0; s < 3; ++s) {
-    if (s < n_streams) {
+  for (int s = 0; s < 4; ++s) {
+    if (s < n_streams || s == P.absorber_stream) {
       const cplx* row = B.streams + (static_cast<long>(s) * P.C + c) * half;
       za[s] = row[ka];
       zb[s] = row[kb];
@@ -641,6 +648,44 @@ __global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R,
   }
   finish_bin(P, *R, B, c, k, za, zb, n_streams, out);
   if (mirror != k && mirror < P.N) finish_bin(P, *R, B, c, mirror, zb, za, n_streams, out);
+}
+
+// ------------------------------------------------------------ temporal absorber
+
+// The in-stage term -alpha(t) a_c(t) / 2 of every mode, as a stream of its own.
+__global__ void field_absorber_stream(DeviceProblem P, FieldBuffers B) {
+  const int T = P.T;
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long>(P.C) * T) return;
+  const int t = static_cast<int>(idx % T);
+  const double* modal = reinterpret_cast<const double*>(B.modal_t);
+  double* streams = reinterpret_cast<double*>(B.streams);
+  streams[static_cast<long>(P.absorber_stream) * P.C * T + idx] =
+      (-0.5 * P.absorber_rate[t]) * modal[idx];
+}
+
+// The split-step factor on the committed state's time samples: exp(-alpha(t) h / 2).
+__global__ void field_split_collar(DeviceProblem P, const AdamsRecord* __restrict__ R,
+                                   double dz, FieldBuffers B) {
+  const int T = P.T;
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long>(P.C) * T) return;
+  const int t = static_cast<int>(idx % T);
+  const double h = R ? R->accepted_dz : dz;
+  double* modal = reinterpret_cast<double*>(B.modal_t);
+  modal[idx] = modal[idx] * exp((-0.5 * P.absorber_rate[t]) * h);
+}
+
+// The collared state back on the coarse bins, the band imposed again.
+__global__ void field_split_unpack(DeviceProblem P, FieldBuffers B) {
+  const long CN = static_cast<long>(P.C) * P.N;
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= CN) return;
+  const int c = static_cast<int>(idx / P.N), b = static_cast<int>(idx % P.N);
+  const int half = P.T / 2;
+  const cplx* row = B.modal_t + static_cast<long>(c) * half;
+  const cplx x = r2c_post(row[b % half], row[(half - b) % half], B.roots_T[b]);
+  B.y[idx] = P.active[b] ? x : make(0.0, 0.0);
 }
 
 // ------------------------------------------------------------- Adams corrector
@@ -890,6 +935,34 @@ void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const Plas
     case 7: launch_node_passes_for<7>(P, R, pt, B, s, marks); break;
     default: launch_node_passes_for<8>(P, R, pt, B, s, marks); break;
   }
+}
+
+void launch_field_absorber_stream(const DeviceProblem& P, FieldBuffers B, hipStream_t s) {
+  SP_CHECK(P.absorber_rate && P.absorber_stream >= 1 && P.absorber_stream <= 3,
+           "the in-stage absorber's rate and stream");
+  const std::size_t n = static_cast<std::size_t>(P.C) * P.T;
+  hipLaunchKernelGGL(field_absorber_stream, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,
+                     s, P, B);
+  check();
+}
+
+void launch_field_split_pack(const DeviceProblem& P, FieldBuffers B, hipStream_t s) {
+  launch_field_scatter(P, B.y, B, s);
+}
+
+void launch_field_split_collar(const DeviceProblem& P, const AdamsRecord* R, double dz,
+                               FieldBuffers B, hipStream_t s) {
+  SP_CHECK(P.absorber_rate, "the split-step absorber's rate");
+  const std::size_t n = static_cast<std::size_t>(P.C) * P.T;
+  hipLaunchKernelGGL(field_split_collar, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                     P, R, dz, B);
+  check();
+}
+
+void launch_field_split_unpack(const DeviceProblem& P, FieldBuffers B, hipStream_t s) {
+  SP_CHECK(P.N == P.T / 2 + 1, "the split-step absorber needs the coarse grid to be the fine one");
+  hipLaunchKernelGGL(field_split_unpack, dim3(cn_blocks(P)), dim3(kThreads), 0, s, P, B);
+  check();
 }
 
 void launch_field_raman_kernel_fill(const DeviceProblem& P, const AdamsRecord* R, cplx* z,

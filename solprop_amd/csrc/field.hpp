@@ -56,6 +56,7 @@ struct AdamsRecord {
   int orders[kMaxScored];
   int pad2;
   double raman_rate[kMaxTerms];          // Raman decay rates at the source position
+  double accepted_dz;                    // the step the record's commit finishes
 };
 
 struct FieldBuffers {
@@ -100,6 +101,18 @@ unsigned correct_blocks(const DeviceProblem& P);
 void launch_field_raman_kernel_fill(const DeviceProblem& P, const AdamsRecord* R, cplx* z,
                                     hipStream_t s);
 void launch_field_raman_drive(const DeviceProblem& P, FieldBuffers B, hipStream_t s);
+
+// The in-stage absorber's stream, -alpha(t) a_c(t) / 2 from the modal fields, written as
+// stream P.absorber_stream of B.streams.
+void launch_field_absorber_stream(const DeviceProblem& P, FieldBuffers B, hipStream_t s);
+// The split-step absorber on the committed state, in three launches around the caller's
+// transforms of B.modal_t: pack y for the real inverse transform; after it, multiply by
+// exp(-alpha(t) h / 2) (h: R->accepted_dz, or `dz` when R is null); after the forward
+// transform, unpack into y on the active bins (zero elsewhere).
+void launch_field_split_pack(const DeviceProblem& P, FieldBuffers B, hipStream_t s);
+void launch_field_split_collar(const DeviceProblem& P, const AdamsRecord* R, double dz,
+                               FieldBuffers B, hipStream_t s);
+void launch_field_split_unpack(const DeviceProblem& P, FieldBuffers B, hipStream_t s);
 
 // The predictor and corrector are compiled per stencil width (2..7); `width` must be
 // the record's.

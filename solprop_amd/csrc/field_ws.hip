@@ -45,6 +45,11 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
            "the field route's Raman drive is the real square E^2; the analytic-signal "
            "drive (a grid without the harmonic) is not built");
   SP_CHECK(d.n_raman == 0 || d.raman_undamped, "the Raman rows are missing");
+  SP_CHECK(d.absorber_scheme >= 0 && d.absorber_scheme <= 2,
+           "absorber scheme 0 (none), 1 (in-stage) or 2 (split-step)");
+  SP_CHECK(d.absorber_scheme == 0 || d.absorber_rate, "the absorber's rate is missing");
+  SP_CHECK(d.absorber_scheme != 2 || d.n_t == d.n_to,
+           "the split-step absorber needs n_to == n_t (its collar acts on the coarse grid)");
   SP_CHECK(!d.has_plasma || d.n_taps <= 1,
            "the band running-integral rule is not built; the trapezoid rule is");
   SP_CHECK(d.adams_order >= 1 && d.adams_order + 1 < kMaxAdamsWidth, "Adams order 1..6");
@@ -128,6 +133,12 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   P_.raman_fraction = d.raman_fraction;
   P_.raman_epsilon0 = d.raman_epsilon0;
   P_.raman_rows = d.n_raman ? upload(d.raman_undamped, d.n_raman * 2UL * T) : nullptr;
+  in_stage_ = d.absorber_scheme == 1;
+  split_ = d.absorber_scheme == 2;
+  P_.absorber_rate = d.absorber_scheme ? upload(d.absorber_rate, static_cast<std::size_t>(T))
+                                       : nullptr;
+  // The in-stage absorber's stream follows the polarisation and, with plasma, the currents.
+  P_.absorber_stream = in_stage_ ? (d.has_plasma ? 3 : 1) : -1;
   std::vector<int> crop(N);
   for (int b = 0; b < N; ++b) crop[b] = b;
   P_.crop = upload(crop.data(), crop.size());
@@ -165,10 +176,11 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   B_.source = alloc<cplx>(CN);
   B_.advanced = alloc<cplx>(CN);
   B_.modal_t = alloc<cplx>(static_cast<std::size_t>(C) * (T / 2));
-  B_.streams = alloc<cplx>(3UL * C * (T / 2));
+  const std::size_t n_rows = (in_stage_ && d.has_plasma ? 4UL : 3UL) * C;
+  B_.streams = alloc<cplx>(n_rows * (T / 2));
   // Streams a run never writes (the currents without plasma on the dense route) read as
   // zeros.
-  SP_HIP(hipMemset(B_.streams, 0, 3UL * C * (T / 2) * sizeof(cplx)));
+  SP_HIP(hipMemset(B_.streams, 0, n_rows * (T / 2) * sizeof(cplx)));
   const std::size_t tiles = static_cast<std::size_t>(scan_tiles(T));
   B_.rate = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
   B_.frac = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
@@ -192,7 +204,7 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   inv_plan_.init(T / 2, C);
   // Without plasma only the polarisation stream is transformed (the finish reads the
   // current streams on plasma runs alone).
-  fwd_plan_.init(T / 2, (d.has_plasma ? 3 : 1) * C);
+  fwd_plan_.init(T / 2, ((d.has_plasma ? 3 : 1) + (in_stage_ ? 1 : 0)) * C);
   if (d.n_raman) {
     // The doubled-grid drive of every node, its transform scratch, the kernel's packed
     // samples and spectrum, and the roots of 2T for the real-transform packing.
@@ -251,7 +263,7 @@ constexpr const char* kPhaseNames[FieldWorkspace::kPhases] = {
     "adams_predict",      "field_scatter",        "inverse_fft",
     "synthesis_gemm",     "raman",                "plasma_rate_tiles",    "plasma_tile_prefix",
     "plasma_current_tiles", "plasma_tile_prefix_current", "node_response",
-    "projection_gemm",    "forward_fft",          "field_finish",
+    "projection_gemm",    "absorber",             "forward_fft",          "field_finish",
     "adams_correct",      "adams_norms"};
 }  // namespace
 
@@ -305,24 +317,29 @@ void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_, marking_ ? marks_ + 6 : nullptr);
   enqueue_projection();
   mark(11);
-  fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
+  if (in_stage_) launch_field_absorber_stream(P_, B_, stream_);
   mark(12);
-  launch_field_finish(P_, rec_dev_, B_, out, stream_);
+  fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
   mark(13);
+  launch_field_finish(P_, rec_dev_, B_, out, stream_);
+  mark(14);
 }
 
 void FieldWorkspace::enqueue_trial(bool commit) {
   // An accepted step's commit rides at the head of the next trial (its slot is in the
   // record), so that the host launches one graph per attempt.
-  if (commit) launch_adams_commit(P_, rec_dev_, B_, stream_);
+  if (commit) {
+    launch_adams_commit(P_, rec_dev_, B_, stream_);
+    if (split_) enqueue_split(rec_dev_, 0.0);
+  }
   mark(0);
   launch_adams_predict(P_, rec_dev_, B_, width_, stream_);
   mark(1);
   enqueue_source(B_.lab, B_.source);
   launch_adams_correct(P_, rec_dev_, B_, width_, stream_);
-  mark(14);
-  launch_adams_norms(P_, B_, stream_);
   mark(15);
+  launch_adams_norms(P_, B_, stream_);
+  mark(16);
 }
 
 void FieldWorkspace::enqueue_start() {
@@ -378,7 +395,20 @@ void FieldWorkspace::wait() {
 void FieldWorkspace::flush() {
   if (pending_slot_ < 0) return;
   launch_adams_accept(P_, pending_slot_, compensated_, B_, stream_);
+  if (split_) enqueue_split(nullptr, pending_dz_);
   pending_slot_ = -1;
+}
+
+// The split-step absorber on the committed state (the workspace's finish: the mask, the
+// collar over the accepted step, the mask again): y to the time samples through B.modal_t,
+// times exp(-alpha h / 2), and back. h is the record's accepted step when R is given
+// (inside a graph), else `dz`.
+void FieldWorkspace::enqueue_split(const AdamsRecord* R, double dz) {
+  launch_field_split_pack(P_, B_, stream_);
+  inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
+  launch_field_split_collar(P_, R, dz, B_, stream_);
+  inv_plan_.exec(B_.modal_t, scratch_, -1, stream_);
+  launch_field_split_unpack(P_, B_, stream_);
 }
 
 std::vector<std::pair<const char*, double>> FieldWorkspace::phases() const {
@@ -502,10 +532,12 @@ std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Sten
   return norms;
 }
 
-void FieldWorkspace::accept(double, int free, const adams::Stencil&,
+void FieldWorkspace::accept(double dz, int free, const adams::Stencil&,
                             const std::vector<adams::Entry>&) {
   // Deferred into the next trial's graph, or flushed before anything reads the state.
   pending_slot_ = free;
+  pending_dz_ = dz;
+  rec_host_->accepted_dz = dz;
   slot_rates_[free] = held_rate_end_;
   accepted_ += 1;
 }
@@ -627,6 +659,7 @@ void FieldWorkspace::source(double z, const double* lab, double* out, double* st
   enqueue_raman();
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
   enqueue_projection();
+  if (in_stage_) launch_field_absorber_stream(P_, B_, stream_);
   SP_HIP(hipStreamSynchronize(stream_));
   if (modal) SP_HIP(hipMemcpy(modal, B_.modal_t, CT * sizeof(double), hipMemcpyDeviceToHost));
   if (streams)

@@ -37,7 +37,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   long buffer(const char* name, double* out, long count) override;
   void set_profile(bool on) override { profile_ = on; }
   std::vector<std::pair<const char*, double>> phases() const override;
-  static constexpr int kPhases = 15;
+  static constexpr int kPhases = 16;
   double trial_us() const override { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
   long timed() const override { return timed_; }
 
@@ -59,6 +59,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   void enqueue_synthesis();
   void enqueue_projection();
   void enqueue_raman_kernel();
+  void enqueue_split(const AdamsRecord* R, double dz);
   void enqueue_raman();
   enum Graph { kTrial = 1, kStart = 2, kCommitTrial = 3 };
   void enqueue_trial(bool commit);
@@ -93,7 +94,10 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   ScalarPath density_, preion_;
   double rtol_ = 1e-6, atol_ = 0.0;
   bool compensated_ = false;
-  bool dense_ = false;  // synthesis and projection as GEMMs (more than kMaxModes modes)
+  bool dense_ = false;          // synthesis and projection as GEMMs (more than kMaxModes modes)
+  bool in_stage_ = false;       // the in-stage absorber: a stream of its own
+  bool split_ = false;          // the split-step absorber: applied at every commit
+  double pending_dz_ = 0.0;     // the accepted step a pending commit finishes
   // The march, and the host it reports to: rows and the attempt limit of one call.
   struct CallHost final : adams::Host {
     std::vector<double>* rows = nullptr;

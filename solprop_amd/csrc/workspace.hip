@@ -41,6 +41,8 @@ Workspace::Workspace(const sp_problem_desc& d) {
   SP_CHECK(d.n_phi >= 1 && d.n_phi <= kMaxTerms, "1..4 phi terms");
   SP_CHECK(d.n_prefactor >= 1 && d.n_prefactor <= kMaxTerms, "1..4 prefactor terms");
   SP_CHECK(d.n_raman >= 0 && d.n_raman <= kMaxTerms, "0..4 Raman rows");
+  SP_CHECK(d.absorber_scheme == 0,
+           "the envelope route runs no temporal absorber (the field routes do)");
   SP_CHECK(d.n_stages >= 1 && d.n_stages <= kMaxStages, "1..8 Runge-Kutta stages");
   SP_CHECK(!d.has_plasma, "the envelope route carries no field-grid plasma");
   SP_CHECK(d.crop_index && d.active_mask && d.norm_mask && d.omega_window && d.omega &&

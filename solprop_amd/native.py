@@ -202,6 +202,9 @@ def lower(problem: Problem, settings: Settings | None = None) -> tuple[_lib.Prob
     d.adams_variable_order = int(problem.adams_variable_order)
     d.adams_equal_approach = 1
     d.adams_safety, d.adams_max_growth, d.adams_min_factor = 0.8, 2.0, 0.2
+    d.absorber_scheme = {"none": 0, "in_stage": 1, "split_step": 2}[problem.absorber_scheme]
+    if problem.absorber_rate is not None:
+        d.absorber_rate = k.f64(problem.absorber_rate)
     return d, k
 
 

@@ -234,6 +234,10 @@ class Problem:
     method: str = "rk"  # "rk" (embedded Runge-Kutta) or "adams" (exponential Adams)
     adams_order: int = 5
     adams_variable_order: bool = True
+    # The temporal absorber (``Problem.Absorber``): "none", "in_stage" (the rate on the fine
+    # grid, n_to) or "split_step" (on the coarse grid, n_t), a power rate in 1/m.
+    absorber_scheme: str = "none"
+    absorber_rate: np.ndarray | None = None
     description: dict = field(default_factory=dict)
 
     @property
@@ -317,6 +321,16 @@ class Problem:
             raise ValueError("Raman rows must span the doubled grid, 2 n_to samples")
         if self.plasma is not None and self.kind != "field":
             raise ValueError("the field-grid plasma needs a real field grid")
+        if self.absorber_scheme not in ("none", "in_stage", "split_step"):
+            raise ValueError(f"unknown absorber scheme {self.absorber_scheme!r}")
+        if self.absorber_scheme != "none":
+            length = t if self.absorber_scheme == "in_stage" else self.n_t
+            rate = self.absorber_rate
+            if rate is None or rate.shape != (length,) or rate.dtype != np.float64:
+                raise ValueError(f"absorber_rate: expected ({length},) float64 for the "
+                                 f"{self.absorber_scheme} scheme")
+            if np.any(rate < 0.0) or not np.all(np.isfinite(rate)):
+                raise ValueError("absorber_rate must be finite and non-negative")
 
 
 # ----------------------------------------------------------------------- builders
@@ -347,6 +361,15 @@ def _band(omega: np.ndarray, band) -> tuple[np.ndarray, np.ndarray]:
     window[lo] = 0.5 - 0.5 * np.cos(np.pi * (omega[lo] - 0.5 * w_lo) / (0.5 * w_lo))
     hi = (omega > w_hi) & (omega < 1.1 * w_hi)
     window[hi] = 0.5 + 0.5 * np.cos(np.pi * (omega[hi] - w_hi) / (0.1 * w_hi))
+    return active, window
+
+
+def _field_band(omega: np.ndarray, band) -> tuple[np.ndarray, np.ndarray]:
+    """`_band` on a real field grid, whose zero-frequency and Nyquist bins stay inactive:
+    a real inverse transform keeps only their real parts (``stepping.py:419-423``)."""
+    active, window = _band(omega, band)
+    active[0] = active[-1] = False
+    window[0] = window[-1] = 0.0
     return active, window
 
 
@@ -534,6 +557,23 @@ def rate_table(rate, field_min: float, field_max: float, n_intervals: int = 512)
     )
 
 
+# The depth of the temporal absorber at the window's edge, as alpha * ell
+# (src/solprop/boundaries.py:101).
+TEMPORAL_ALPHA_LENGTH = 12.6
+
+
+def temporal_absorber_rate(profile: np.ndarray, reference_length: float) -> np.ndarray:
+    """The temporal collar as a power absorption rate (``boundaries.py:433-470``).
+
+    ``alpha(t) = TEMPORAL_ALPHA_LENGTH (1 - W(t)) / ell``: zero where the collar profile
+    ``W`` is one, ``TEMPORAL_ALPHA_LENGTH / ell`` where it reaches zero.
+    """
+    if not (np.isfinite(reference_length) and reference_length > 0.0):
+        raise ValueError("the reference length must be positive and finite")
+    w = np.asarray(profile, dtype=np.float64)
+    return np.ascontiguousarray(TEMPORAL_ALPHA_LENGTH * (1.0 - w) / reference_length + 0.0)
+
+
 def oscillator_raman(n_to: int, dt: float, per_molecule: float, tau1: float,
                      rate: "ScalarPath") -> RamanGroup:
     """A molecular Raman response on a real field grid: one damped-oscillator row.
@@ -582,6 +622,8 @@ def capillary_problem(
     time_collar: float = 0.06,
     raman_share: float = 0.0,
     raman_tau: tuple = (60e-15, 200e-15),
+    absorber: str = "none",
+    absorber_length: float | None = None,
 ) -> Problem:
     """Gas-filled capillary on a real field grid: HE_1m modes, a pressure gradient.
 
@@ -603,7 +645,7 @@ def capillary_problem(
     n = n_t // 2 + 1
     omega = 2 * np.pi * np.arange(n) / (n_t * dt)
     c = C_LIGHT
-    active, omega_window = _band(omega, band)
+    active, omega_window = _field_band(omega, band)
     norm_mask = active.copy()
     # Temporal collar and its derivative.
     t = (np.arange(n_to) - n_to // 2) * dt
@@ -734,6 +776,9 @@ def capillary_problem(
         noise_model="none",
         rtol=rtol,
         method="adams",
+        absorber_scheme=absorber,
+        absorber_rate=None if absorber == "none" else temporal_absorber_rate(
+            g, absorber_length if absorber_length else length / 20),
         description={
             "route": "capillary UPPE, modal field grid",
             "n_t": n_t, "n_omega": n, "window_s": window, "radius_m": radius,
@@ -799,6 +844,8 @@ def radial_problem(
     time_collar: float = 0.05,
     raman_share: float = 0.0,
     raman_tau: tuple = (60e-15, 200e-15),
+    absorber: str = "none",
+    absorber_length: float | None = None,
 ) -> Problem:
     """Free-space radially symmetric beam in argon on a real field grid.
 
@@ -816,7 +863,7 @@ def radial_problem(
     n = n_t // 2 + 1
     omega = 2 * np.pi * np.arange(n) / (n_t * dt)
     c = C_LIGHT
-    active, omega_window = _band(omega, band)
+    active, omega_window = _field_band(omega, band)
     norm_mask = active.copy()
     t = (np.arange(n_to) - n_to // 2) * dt
     width = time_collar * window
@@ -898,6 +945,9 @@ def radial_problem(
         noise_model="none",
         rtol=rtol,
         method="adams",
+        absorber_scheme=absorber,
+        absorber_rate=None if absorber == "none" else temporal_absorber_rate(
+            g, absorber_length if absorber_length else length / 20),
         description={
             "route": "free-space radial beam, QDHT field grid",
             "n_t": n_t, "n_omega": n, "window_s": window, "n_r": n_r, "radius_m": radius,

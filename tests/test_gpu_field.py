@@ -38,6 +38,9 @@ CASES = {
     # A molecular response taking half the third-order response, its dephasing rate
     # following the pressure (the kernel is rebuilt at every source position).
     "kerr_raman": {"raman_share": 0.5},
+    # The temporal collar as an absorbing rate, in every source or once per step.
+    "in_stage_absorber": {"absorber": "in_stage", "absorber_length": 2e-2},
+    "split_absorber": {"absorber": "split_step", "absorber_length": 2e-2},
 }
 
 
@@ -88,7 +91,9 @@ def test_first_trial_matches_oracle(case):
 
 
 @pytest.mark.parametrize("case,proposal", [("kerr_plasma", None), ("constant_pressure", None),
-                                           ("kerr_plasma", 2e-3), ("kerr_raman", None)])
+                                           ("kerr_plasma", 2e-3), ("kerr_raman", None),
+                                           ("in_stage_absorber", None),
+                                           ("split_absorber", None)])
 def test_march_matches_oracle_decisions(case, proposal):
     print(f"field march {case} proposal={proposal}", flush=True)
     problem = small(**CASES[case])
@@ -154,3 +159,19 @@ def test_first_trial_matches_oracle_at_full_size():
     ws.close()
     assert rel_l2(field, advanced.reshape(field.shape)) < 1e-11
     assert abs(got - norms[0]) / norms[0] < 1e-8
+
+
+@pytest.mark.parametrize("case", ["in_stage_absorber", "split_absorber"])
+def test_absorbers_change_the_march(case):
+    """Each absorber scheme acts: a short march with it ends measurably away from the
+    same march without it (both on the device)."""
+    print(f"field absorber acts {case}", flush=True)
+    ends = []
+    for kw in (CASES[case], {}):
+        problem = small(**kw)
+        ws = Workspace(problem)
+        session = ws.start(0.0, problem.field0)
+        session.advance(problem.z_end, work_limit=12)
+        ends.append(session.state())
+        ws.close()
+    assert rel_l2(ends[0], ends[1]) > 1e-10

@@ -249,3 +249,51 @@ def test_preionised_fraction_stays_below_one():
     fraction, _ = P.plasma_scan(f3, rate, plasma, 0.2, float(W["param_dt"]))
     assert np.all(np.diff(fraction[0]) >= -1e-15)
     assert fraction.min() >= 0.2 - 1e-15 and fraction.max() <= 1.0
+
+
+# ------------------------------------------------------------------- temporal absorber
+
+
+def test_absorber_term_docstring_example():
+    """``passes/stepping.py:335-340``: a unit field under rate 2 gives -8 at DC, 0 above."""
+    from oracle import passes as P
+
+    field = np.ones((1, 1, 8))
+    got = P.absorber_term(field, np.full(8, 2.0), np.arange(2, dtype=np.intp),
+                          np.ones(2, bool), kind="field")
+    assert np.array_equal(got.real, [[[-8.0, 0.0]]])
+    assert not np.any(got.imag)
+
+
+def test_split_absorber_is_an_exact_rate():
+    """``passes/stepping.py:410-456``: zero rate leaves the state; a constant rate is the
+    closed form y e^{-alpha h / 2} (a factor without the half fails it); a varying rate
+    telescopes, two steps equal to one of their sum."""
+    from oracle import passes as P
+
+    rng = np.random.default_rng(4)
+    n_t = 64
+    state = rng.normal(size=(2, 1, n_t // 2 + 1)) + 1j * rng.normal(size=(2, 1, n_t // 2 + 1))
+    state[..., 0] = state[..., 0].real  # a real signal's DC and Nyquist bins are real
+    state[..., -1] = state[..., -1].real
+    assert np.allclose(P.apply_temporal_rate(state, np.zeros(n_t), 0.3, "field"), state,
+                       rtol=0, atol=1e-14)
+    alpha, h = 3.0, 0.2
+    got = P.apply_temporal_rate(state, np.full(n_t, alpha), h, "field")
+    scale = np.abs(state).max()
+    assert np.allclose(got, state * np.exp(-alpha * h / 2), rtol=0, atol=1e-14 * scale)
+    assert not np.allclose(got, state * np.exp(-alpha * h), rtol=0, atol=1e-3 * scale)
+    varying = rng.uniform(0.0, 50.0, n_t)
+    two = P.apply_temporal_rate(P.apply_temporal_rate(state, varying, 0.1, "field"),
+                                varying, 0.25, "field")
+    one = P.apply_temporal_rate(state, varying, 0.35, "field")
+    assert np.allclose(two, one, rtol=0, atol=1e-13 * np.abs(state).max())
+
+
+def test_temporal_absorber_rate():
+    """``boundaries.py:433-470``: zero on the flat region, 12.6 / ell at the window's edge."""
+    from solprop_amd.problem import temporal_absorber_rate
+
+    rate = temporal_absorber_rate(np.array([1.0, 0.5, 0.0]), 0.1)
+    assert rate[0] == 0.0 and not np.signbit(rate[0])
+    assert np.isclose(rate[1], 63.0) and np.isclose(rate[2], 126.0)
