@@ -384,6 +384,147 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   }
 }
 
+// ---------------------------------------------- band running-integral rule
+
+// Samples a wave's window holds: its tile and a halo of K each side.
+constexpr int kBandSpan = kWave + 2 * kMaxBandHalf;
+
+// The band rule's filter at a lane of the wave's window (window[j] is sample
+// 63 tile - K + j): c_0 y_n + Σ_k c_k (y_{n-k} + y_{n+k}), the centre first and each
+// pair added before it is weighted (src/solprop/nonlinear/ionisation.py:611-653).
+__device__ inline double band_filter(const PlasmaTable& pt, const double* window, int lane) {
+  const int K = pt.n_taps - 1;
+  const double* y = window + K + lane;
+  double f = pt.taps[0] * y[0];
+  for (int k = 1; k <= K; ++k) f = f + pt.taps[k] * (y[-k] + y[k]);
+  return f;
+}
+
+// The rate at node q and sample t of the grid.
+template <int C>
+__device__ inline double rate_sample(const DeviceProblem& P, const PlasmaTable& pt,
+                                     const double* coefficients, const FieldBuffers& B, int q,
+                                     int t) {
+  double am[held(C)];
+  load_modal<C>(B, P.T, t, am);
+  return rate_of(fabs(field_at<C>(P, B, am, q, t)), pt, coefficients);
+}
+
+// The free-current integrand (f e²/m) E at node q and sample t, the fraction from the
+// floored exposure (the owning tile's offset plus the within-tile exposure).
+template <int C>
+__device__ inline double current_sample(const DeviceProblem& P, const FieldBuffers& B, int q,
+                                        int t, double f0, int tiles) {
+  const int owner = min(t / kTileSpan, tiles - 1);
+  const double x = B.tile_x[static_cast<long>(q) * tiles + owner] +
+                   B.x_local[static_cast<long>(q) * P.T + t];
+  const double f = fraction_of(f0, x > 0.0 ? x : 0.0);
+  double am[held(C)];
+  load_modal<C>(B, P.T, t, am);
+  return (f * P.plasma_cmr) * field_at<C>(P, B, am, q, t);
+}
+
+// Point reflection past the grid's ends: y(-m) = 2 y(0) - y(m), y(T-1+m) = 2 y(T-1) -
+// y(T-1-m) (ionisation.py:642-644).
+template <class Sample>
+__device__ inline double reflected(int T, int t, const Sample& sample) {
+  if (t < 0) return 2.0 * sample(0) - sample(-t);
+  if (t > T - 1) return 2.0 * sample(T - 1) - sample(2 * (T - 1) - t);
+  return sample(t);
+}
+
+// Pass A under the band rule: the rate over each wave's window, the raw rate stored for
+// the owned samples, the filtered rate's trapezoid increments scanned, the within-tile
+// exposure stored and the tile's sum. Every wave reaches both barriers of every node.
+template <int C>
+__global__ void __launch_bounds__(kWave * kNodeWaves)
+    plasma_rate_band(DeviceProblem P, PlasmaTable pt, FieldBuffers B) {
+  __shared__ double table[4 * kLdsTableIntervals];
+  __shared__ double windows[kNodeWaves][kBandSpan];
+  const bool in_lds = pt.n_intervals <= kLdsTableIntervals;
+  if (in_lds)
+    for (int i = threadIdx.x; i < 4 * pt.n_intervals; i += blockDim.x) table[i] = B.rate_c[i];
+  __syncthreads();
+  const double* coefficients = in_lds ? table : B.rate_c;
+  const int T = P.T, tiles = scan_tiles(T), K = pt.n_taps - 1;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int tile = blockIdx.x;
+  const int first = tile * kTileSpan - K;
+  const int t = tile * kTileSpan + lane;
+  const bool stores = t < T && (lane < kTileSpan || t == T - 1);
+  const bool step = lane < kTileSpan && t + 1 < T;
+  double* window = windows[wave];
+  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave;
+  for (int i = 0; i < kNodesPerWave; ++i) {
+    const int q = q0 + i * kNodeWaves + wave;
+    const bool active = q < P.Q;  // uniform across the wave
+    if (active) {
+      auto sample = [&](int u) { return rate_sample<C>(P, pt, coefficients, B, q, u); };
+      for (int j = lane; j < kWave + 2 * K; j += kWave) window[j] = reflected(T, first + j, sample);
+    }
+    __syncthreads();
+    if (active) {
+      const long row = static_cast<long>(q) * T;
+      if (stores) B.rate[row + t] = window[K + lane];
+      // Cross-lane moves with every lane of the wave active: a lane masked off reads as
+      // zero to its neighbours.
+      const double y = band_filter(pt, window, lane);
+      const double y_next = from_next_lane(y);
+      const double inc = step ? (P.dt_fine * (y_next + y)) / 2.0 : 0.0;
+      const double incl = wave_scan(inc);
+      const double excl = from_previous_lane(incl);
+      if (lane == kTileSpan - 1) B.tile_x[static_cast<long>(q) * tiles + tile] = incl;
+      if (stores) B.x_local[row + t] = excl;
+    }
+    __syncthreads();
+  }
+}
+
+// Pass C under the band rule: the free-current integrand over each wave's window (the
+// fraction from the stored exposure), filtered, its increments scanned; the fraction and
+// the within-tile free current stored for the owned samples, and the tile's sum.
+template <int C>
+__global__ void __launch_bounds__(kWave * kNodeWaves)
+    plasma_current_band(DeviceProblem P, PlasmaTable pt, const AdamsRecord* __restrict__ R,
+                        FieldBuffers B) {
+  __shared__ double windows[kNodeWaves][kBandSpan];
+  const int T = P.T, tiles = scan_tiles(T), K = pt.n_taps - 1;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int tile = blockIdx.x;
+  const int first = tile * kTileSpan - K;
+  const int t = tile * kTileSpan + lane;
+  const bool stores = t < T && (lane < kTileSpan || t == T - 1);
+  const bool step = lane < kTileSpan && t + 1 < T;
+  const double f0 = R->preionisation;
+  double* window = windows[wave];
+  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave;
+  for (int i = 0; i < kNodesPerWave; ++i) {
+    const int q = q0 + i * kNodeWaves + wave;
+    const bool active = q < P.Q;  // uniform across the wave
+    if (active) {
+      auto sample = [&](int u) { return current_sample<C>(P, B, q, u, f0, tiles); };
+      for (int j = lane; j < kWave + 2 * K; j += kWave) window[j] = reflected(T, first + j, sample);
+    }
+    __syncthreads();
+    if (active) {
+      const long row = static_cast<long>(q) * T;
+      // Cross-lane moves with every lane of the wave active (see the rate pass).
+      const double y = band_filter(pt, window, lane);
+      const double y_next = from_next_lane(y);
+      const double inc = step ? (P.dt_fine * (y_next + y)) / 2.0 : 0.0;
+      const double incl = wave_scan(inc);
+      const double excl = from_previous_lane(incl);
+      if (lane == kTileSpan - 1) B.tile_f[static_cast<long>(q) * tiles + tile] = incl;
+      if (stores) {
+        const double x = B.tile_x[static_cast<long>(q) * tiles + tile] + B.x_local[row + t];
+        B.frac[row + t] = fraction_of(f0, x > 0.0 ? x : 0.0);
+        B.free_local[row + t] = excl;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Passes B and D: exclusive prefix of the tile sums of each node, one wavefront per
 // node. Each lane sums a run of consecutive tiles in order, the wave scans the run
 // totals, and each lane then writes its run's offsets in order: a fixed order, so the
@@ -886,14 +1027,23 @@ void launch_node_passes_for(const DeviceProblem& P, const AdamsRecord* R,
   auto mark = [&](int i) {
     if (marks) SP_HIP(hipEventRecord(marks[i], s));
   };
+  const bool band = pt.n_taps > 1;
   if (P.has_plasma) {
-    hipLaunchKernelGGL(plasma_rate_tiles<C>, node_grid, node_block, 0, s, P, pt, B);
+    SP_CHECK(!band || (B.x_local && pt.n_taps <= kMaxBandHalf + 1 && 64 + 2 * (pt.n_taps - 1) <= kBandSpan),
+             "the band rule needs its exposure buffer and at most 8 taps each side");
+    if (band)
+      hipLaunchKernelGGL(plasma_rate_band<C>, node_grid, node_block, 0, s, P, pt, B);
+    else
+      hipLaunchKernelGGL(plasma_rate_tiles<C>, node_grid, node_block, 0, s, P, pt, B);
     check();
     mark(0);
     hipLaunchKernelGGL(plasma_tile_prefix, dim3(P.Q), dim3(kWave), 0, s, tiles, B.tile_x);
     check();
     mark(1);
-    hipLaunchKernelGGL(plasma_current_tiles<C>, node_grid, node_block, 0, s, P, R, B);
+    if (band)
+      hipLaunchKernelGGL(plasma_current_band<C>, node_grid, node_block, 0, s, P, pt, R, B);
+    else
+      hipLaunchKernelGGL(plasma_current_tiles<C>, node_grid, node_block, 0, s, P, R, B);
     check();
     mark(2);
     hipLaunchKernelGGL(plasma_tile_prefix, dim3(P.Q), dim3(kWave), 0, s, tiles, B.tile_f);

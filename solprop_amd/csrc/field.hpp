@@ -74,6 +74,7 @@ struct FieldBuffers {
   double* free_local;  // (Q, T) free current since the start of the sample's scan tile
   double* tile_x;   // (Q, tiles) exposure tile sums, then offsets
   double* tile_f;   // (Q, tiles) free-current tile sums, then offsets
+  double* x_local;  // (Q, T) exposure since the start of the sample's scan tile (band rule)
   cplx* raman_drive;     // (Q, T) the drive on the doubled grid, two reals a point; then
                          // its convolution with the kernel (times 2T)
   cplx* raman_H;         // (T + 1) the kernel's spectrum on the doubled grid, dt folded in
@@ -86,10 +87,17 @@ struct FieldBuffers {
   const double* rate_c; // (4, n_intervals) rate table
 };
 
+constexpr int kMaxBandHalf = 8;  // most taps each side of the band rule's filter
+
 struct PlasmaTable {
   double log_field_min, d_log_field, field_min, field_max, rate_max;
   double charge_mass_ratio, ionisation_potential;
   int n_intervals;
+  // The running integrals' rule: one tap (1.0) is the trapezoid; more are the band rule,
+  // c_0 .. c_K of a symmetric filter applied to each integrand before the trapezoid,
+  // the window's ends closed by point reflection and the exposure floored at zero.
+  int n_taps;
+  double taps[kMaxBandHalf + 1];
 };
 
 unsigned correct_threads(const DeviceProblem& P);

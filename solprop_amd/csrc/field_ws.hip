@@ -50,8 +50,8 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   SP_CHECK(d.absorber_scheme == 0 || d.absorber_rate, "the absorber's rate is missing");
   SP_CHECK(d.absorber_scheme != 2 || d.n_t == d.n_to,
            "the split-step absorber needs n_to == n_t (its collar acts on the coarse grid)");
-  SP_CHECK(!d.has_plasma || d.n_taps <= 1,
-           "the band running-integral rule is not built; the trapezoid rule is");
+  SP_CHECK(!d.has_plasma || (d.n_taps >= 1 && d.n_taps <= kMaxBandHalf + 1 && d.taps),
+           "the running-integral rule needs 1 (the trapezoid) to 9 (the band rule) taps");
   SP_CHECK(d.adams_order >= 1 && d.adams_order + 1 < kMaxAdamsWidth, "Adams order 1..6");
   SP_CHECK(d.noise_model != 2, "the modified noise model is not built on the field route");
   SP_CHECK(d.crop_index && d.active_mask && d.norm_mask && d.omega_window && d.omega &&
@@ -162,6 +162,8 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
     pt_.charge_mass_ratio = d.charge_mass_ratio;
     pt_.ionisation_potential = d.ionisation_potential;
     pt_.n_intervals = d.rate_intervals;
+    pt_.n_taps = d.n_taps;
+    for (int k = 0; k < d.n_taps; ++k) pt_.taps[k] = d.taps[k];
     B_.rate_c = upload(d.rate_coefficients, 4 * static_cast<std::size_t>(d.rate_intervals));
   }
 
@@ -187,6 +189,7 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   B_.free_local = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
   B_.tile_x = alloc<double>(static_cast<std::size_t>(Q) * tiles);
   B_.tile_f = alloc<double>(static_cast<std::size_t>(Q) * tiles);
+  if (d.has_plasma && d.n_taps > 1) B_.x_local = alloc<double>(static_cast<std::size_t>(Q) * T);
   if (dense_) {
     B_.node_field = alloc<double>(static_cast<std::size_t>(Q) * T);
     B_.node_streams = alloc<double>(static_cast<std::size_t>(Q) * (d.has_plasma ? 2 : 1) * T);
@@ -685,6 +688,12 @@ long FieldWorkspace::buffer(const char* name, double* out, long count) {
       {"states", B_.states, B_.states ? S * CN2 : 0},
       {"carried", B_.carried, B_.carried ? S * CN2 : 0},
       {"node_field", B_.node_field, B_.node_field ? static_cast<long>(P_.Q) * P_.T : 0},
+      {"rate", B_.rate, B_.rate ? static_cast<long>(P_.Q) * P_.T : 0},
+      {"frac", B_.frac, B_.frac ? static_cast<long>(P_.Q) * P_.T : 0},
+      {"free_local", B_.free_local, B_.free_local ? static_cast<long>(P_.Q) * P_.T : 0},
+      {"x_local", B_.x_local, B_.x_local ? static_cast<long>(P_.Q) * P_.T : 0},
+      {"tile_x", B_.tile_x, static_cast<long>(P_.Q) * scan_tiles(P_.T)},
+      {"tile_f", B_.tile_f, static_cast<long>(P_.Q) * scan_tiles(P_.T)},
   };
   flush();
   for (const auto& row : table) {

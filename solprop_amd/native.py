@@ -313,8 +313,10 @@ class Workspace:
         return (out, streams, modal) if intermediates else out
 
     def buffer(self, name: str) -> np.ndarray:
-        """A named device buffer of the field route: complex rows of (C, N), or for
-        ``node_field`` (the dense route's synthesised field) real (Q, T)."""
+        """A named device buffer of the field route: complex rows of (C, N); real (Q, T)
+        for the node arrays (``node_field``, the dense route's synthesised field;
+        ``rate``, ``frac``, ``free_local``, ``x_local``); real (Q, tiles) for the scan
+        tiles' offsets (``tile_x``, ``tile_f``)."""
         size = C.c_long()
         msg = C.create_string_buffer(1024)
         _check(self._lib.sp_testing_buffer(self._handle, name.encode(), None, 0,
@@ -322,8 +324,10 @@ class Workspace:
         out = np.zeros(size.value)
         _check(self._lib.sp_testing_buffer(self._handle, name.encode(), _ptr(out), size.value,
                                            C.byref(size), msg, 1024), msg)
-        if name == "node_field":
+        if name in ("node_field", "rate", "frac", "free_local", "x_local"):
             return out.reshape(self.problem.n_nodes, self.problem.n_to)
+        if name in ("tile_x", "tile_f"):
+            return out.reshape(self.problem.n_nodes, -1)
         return out.view(np.complex128).reshape(-1, *self._shape)
 
     def phases(self) -> dict:

@@ -535,6 +535,102 @@ def adk_rate(field, ip_ev: float, z: int = 1):
     return np.where(f > 0, w / au_time, 0.0)
 
 
+# The band running-integral rule's design (src/solprop/translate/plasma_integration.py:52-74).
+BAND_DESIGN_TOLERANCE = 1e-5
+BAND_MAX_HALF_WIDTH = 8
+BAND_ABOVE_BOUND = 1.05
+BAND_FIT_NODES = 4097
+BAND_CERTIFICATION_SAMPLES = 8193
+BAND_CONDITION_LIMIT = 1e10
+
+
+def trapezoid_multiplier(theta):
+    """``(theta/2) cot(theta/2)``, the trapezoid's multiplier relative to the exact
+    integral, with its series below 1e-4 (``plasma_integration.py:78-109``)."""
+    theta = np.asarray(theta, dtype=np.float64)
+    out = np.empty_like(theta)
+    small = theta < 1e-4
+    out[small] = 1.0 - theta[small] ** 2 / 12.0 - theta[small] ** 4 / 720.0
+    half = theta[~small] / 2.0
+    out[~small] = half / np.tan(half)
+    return out
+
+
+def band_composite(taps, theta):
+    """``H = C rho``: the filtered trapezoid's multiplier (``plasma_integration.py:278-292``)."""
+    theta = np.asarray(theta, dtype=np.float64)
+    response = np.full_like(theta, taps[0])
+    for k in range(1, len(taps)):
+        response = response + 2.0 * taps[k] * np.cos(k * theta)
+    return response * trapezoid_multiplier(theta)
+
+
+def _band_fit(half: int, theta_b: float):
+    """Least squares for ``C = 1/rho`` on ``[0, theta_b]`` under ``C(0) = 1``, columns
+    scaled to unit norm (``plasma_integration.py:255-276``)."""
+    if half == 0:
+        return np.ones(1), 1.0
+    theta = np.linspace(0.0, theta_b, BAND_FIT_NODES)
+    k = np.arange(1, half + 1)
+    design = 2.0 * (np.cos(np.outer(theta, k)) - 1.0)
+    scale = np.linalg.norm(design, axis=0)
+    scaled = design / scale
+    target = 1.0 / trapezoid_multiplier(theta) - 1.0
+    side, *_ = np.linalg.lstsq(scaled, target, rcond=None)
+    side = side / scale
+    return np.concatenate([[1.0 - 2.0 * side.sum()], side]), float(np.linalg.cond(scaled))
+
+
+def _certified_max(function, lower: float, upper: float) -> float:
+    """``max |function|`` on ``[lower, upper]``: dense samples, then a bounded search about
+    every sampled local maximum above half the largest (``plasma_integration.py:295-330``)."""
+    from scipy.optimize import minimize_scalar
+
+    theta = np.linspace(lower, upper, BAND_CERTIFICATION_SAMPLES)
+    values = np.abs(function(theta))
+    best = float(values.max())
+    inner = values[1:-1]
+    peaks = np.flatnonzero((inner >= values[:-2]) & (inner >= values[2:]) & (inner >= 0.5 * best))
+    for index in peaks + 1:
+        result = minimize_scalar(lambda x: -abs(float(function(np.array([x]))[0])),
+                                 bounds=(theta[index - 1], theta[index + 1]),
+                                 method="bounded", options={"xatol": 1e-12})
+        best = max(best, -float(result.fun))
+    return best
+
+
+def design_band_taps(dt: float, band_edge: float) -> np.ndarray:
+    """The band running-integral rule's taps for one sample spacing and band edge: the
+    smallest half-width K whose composite multiplier is within 1e-5 of one on the band
+    and at most 1.05 above it (``plasma_integration.py:112-253``)."""
+    theta_b = band_edge * dt
+    if not 0.0 < theta_b < math.pi:
+        raise ValueError(f"the band edge lies at or above Nyquist (theta_b = {theta_b:.4g})")
+    for half in range(BAND_MAX_HALF_WIDTH + 1):
+        taps, condition = _band_fit(half, theta_b)
+        if condition > BAND_CONDITION_LIMIT:
+            raise ValueError(f"the band-rule fit with K = {half} is ill-conditioned")
+        in_band = _certified_max(lambda t, c=taps: band_composite(c, t) - 1.0, 0.0, theta_b)
+        above = _certified_max(lambda t, c=taps: band_composite(c, t), theta_b, math.pi)
+        if in_band <= BAND_DESIGN_TOLERANCE and above <= BAND_ABOVE_BOUND:
+            return np.ascontiguousarray(taps)
+    raise ValueError(f"no band-rule design with K <= {BAND_MAX_HALF_WIDTH} at theta_b = {theta_b:.4g}")
+
+
+def with_integration(group: PlasmaGroup, integration: str, dt: float, omega, active):
+    """The plasma group under a running-integral rule: ``"band"`` (the reference's
+    default, ``nonlinear/responses.py:1091``) designs taps for this grid's spacing and the
+    active band's upper edge; ``"trapezoid"`` keeps the single identity tap."""
+    if integration == "trapezoid":
+        return group
+    if integration != "band":
+        raise ValueError(f"integration must be 'band' or 'trapezoid', got {integration!r}")
+    edge = float(np.max(np.asarray(omega)[np.asarray(active)]))
+    import dataclasses
+
+    return dataclasses.replace(group, taps=design_band_taps(dt, edge))
+
+
 def rate_table(rate, field_min: float, field_max: float, n_intervals: int = 512):
     """A cubic spline of ``log W`` in ``log |E|`` on uniform knots, as the reference's
     ``translate/ionisation.py:95`` RateTable holds it (coefficients highest first)."""
@@ -624,6 +720,7 @@ def capillary_problem(
     raman_tau: tuple = (60e-15, 200e-15),
     absorber: str = "none",
     absorber_length: float | None = None,
+    integration: str = "band",
 ) -> Problem:
     """Gas-filled capillary on a real field grid: HE_1m modes, a pressure gradient.
 
@@ -744,7 +841,9 @@ def capillary_problem(
     plasma_group = None
     if plasma:
         # Below 3 GV/m helium's rate is under 1e-150 /s: the table is exactly zero there.
-        plasma_group = rate_table(lambda f: adk_rate(f, HE_IONISATION_EV), 3e9, 2e12)
+        plasma_group = with_integration(
+            rate_table(lambda f: adk_rate(f, HE_IONISATION_EV), 3e9, 2e12), integration, dt,
+            omega, active)
     return Problem(
         kind="field",
         n_t=n_t,
@@ -846,6 +945,7 @@ def radial_problem(
     raman_tau: tuple = (60e-15, 200e-15),
     absorber: str = "none",
     absorber_length: float | None = None,
+    integration: str = "band",
 ) -> Problem:
     """Free-space radially symmetric beam in argon on a real field grid.
 
@@ -913,7 +1013,9 @@ def radial_problem(
                                        raman_tau[0], ScalarPath.constant(1.0 / raman_tau[1]))
     plasma_group = None
     if plasma:
-        plasma_group = rate_table(lambda f: adk_rate(f, AR_IONISATION_EV), 1e9, 2e12)
+        plasma_group = with_integration(
+            rate_table(lambda f: adk_rate(f, AR_IONISATION_EV), 1e9, 2e12), integration, dt,
+            omega, active)
     return Problem(
         kind="field",
         n_t=n_t,

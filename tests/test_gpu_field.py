@@ -16,7 +16,8 @@ within 1e-8 relative L2.
 import numpy as np
 import pytest
 
-from field_stages import log_ulp_sensitivity, oracle_first_trial, oracle_source_stages, rel_l2
+from field_stages import (exp_ulp_sensitivity, log_ulp_sensitivity, oracle_first_trial,
+                          oracle_source_stages, rel_l2, scan_sensitivity)
 from oracle import adams
 from solprop_amd.native import Settings, Workspace
 from solprop_amd.problem import capillary_problem
@@ -25,7 +26,9 @@ pytestmark = pytest.mark.gpu
 
 
 def small(**kw):
-    args = dict(n_t=2048, n_nodes=32, length=0.05)
+    # The trapezoid rule unless a case asks for the band rule (this coarse grid's full
+    # band lies beyond the band rule's design range; its cases take a narrower band).
+    args = dict(n_t=2048, n_nodes=32, length=0.05, integration="trapezoid")
     args.update(kw)
     return capillary_problem(**args)
 
@@ -41,6 +44,8 @@ CASES = {
     # The temporal collar as an absorbing rate, in every source or once per step.
     "in_stage_absorber": {"absorber": "in_stage", "absorber_length": 2e-2},
     "split_absorber": {"absorber": "split_step", "absorber_length": 2e-2},
+    # The band running-integral rule (the reference's default), 5 taps a side here.
+    "band_rule": {"integration": "band", "band": (300e-9, 4e-6)},
 }
 
 
@@ -48,8 +53,8 @@ CASES = {
 def test_source_stages_match_oracle(case):
     """Stage by stage at the launch field: modal fields and polarisation to 1e-14 and
     1e-13; the current streams within 1e-12 or four times their sensitivity to a one-ulp
-    bias of log|E| (:func:`log_ulp_sensitivity`), whichever is larger; the source to
-    1e-12."""
+    bias of log|E| or of the fraction's exponential (:func:`log_ulp_sensitivity`,
+    :func:`exp_ulp_sensitivity`), whichever is largest; the source to 1e-12."""
     print(f"field source stages {case}", flush=True)
     problem = small(**CASES[case])
     lab = problem.field0.reshape(problem.shape)
@@ -64,8 +69,9 @@ def test_source_stages_match_oracle(case):
     for i, name in enumerate(names):
         if np.any(streams[i]):
             errors[name] = rel_l2(got_streams[i], streams[i])
-    sensitivity = log_ulp_sensitivity(problem, lab, z, streams)
-    print(errors, "log-ulp sensitivity", sensitivity.tolist(), flush=True)
+    sensitivity = np.maximum(log_ulp_sensitivity(problem, lab, z, streams),
+                             exp_ulp_sensitivity(problem, lab, z, streams))
+    print(errors, "ulp sensitivity", sensitivity.tolist(), flush=True)
     assert errors["modal"] < 1e-14
     assert errors["source"] < 1e-12
     if "polarisation" in errors:
@@ -93,7 +99,7 @@ def test_first_trial_matches_oracle(case):
 @pytest.mark.parametrize("case,proposal", [("kerr_plasma", None), ("constant_pressure", None),
                                            ("kerr_plasma", 2e-3), ("kerr_raman", None),
                                            ("in_stage_absorber", None),
-                                           ("split_absorber", None)])
+                                           ("split_absorber", None), ("band_rule", None)])
 def test_march_matches_oracle_decisions(case, proposal):
     print(f"field march {case} proposal={proposal}", flush=True)
     problem = small(**CASES[case])
@@ -175,3 +181,43 @@ def test_absorbers_change_the_march(case):
         ends.append(session.state())
         ws.close()
     assert rel_l2(ends[0], ends[1]) > 1e-10
+
+
+@pytest.mark.parametrize("case", ["kerr_plasma", "band_rule"])
+def test_plasma_scan_stages_match_oracle(case):
+    """The node passes' stored arrays after one source evaluation, against the oracle's
+    scan of the same node field: the rate to 1e-13; the fraction and the free current
+    within 1e-12 and 1e-11, or four times their sensitivity to a one-ulp exponential in
+    the fraction (which cancels to a small number), whichever is larger."""
+    from oracle import passes as P
+    from oracle import rk
+
+    print(f"field plasma stages {case}", flush=True)
+    problem = small(**CASES[case])
+    lab = problem.field0.reshape(problem.shape)
+    z = 2e-4
+    src = rk.StageSource(problem)
+    modal = P.to_time(lab, problem.crop_index.astype(np.intp), problem.n_to, problem.kind)
+    field_t = P.synthesise(modal, problem.synthesis, 1.0, 1)
+    rate = P.ionisation_rate(field_t, src.plasma.rate)
+    rule = src.plasma.integral
+    exposure = rule.floor(rule.integrate(rate, problem.dt_fine))
+    fraction, free = P.plasma_scan(field_t, rate, src.plasma, problem.preionisation(z),
+                                   problem.dt_fine)
+    ws = Workspace(problem)
+    ws.load(0.0, problem.field0)
+    ws.source(z, lab)
+    got_rate, got_frac = ws.buffer("rate"), ws.buffer("frac")
+    free_local, tile_f = ws.buffer("free_local"), ws.buffer("tile_f")
+    ws.close()
+    tiles = tile_f.shape[1]
+    owner = np.minimum(np.arange(problem.n_to) // 63, tiles - 1)
+    got_free = tile_f[:, owner] + free_local
+    errors = {"rate": rel_l2(got_rate, rate), "fraction": rel_l2(got_frac, fraction),
+              "free current": rel_l2(got_free, free[:, 0])}
+    sensitivity = scan_sensitivity(field_t, rate, src.plasma, problem.preionisation(z),
+                                   problem.dt_fine, fraction, free)
+    print(errors, "exp-ulp sensitivity", sensitivity.tolist(), flush=True)
+    assert errors["rate"] < 1e-13
+    assert errors["fraction"] < max(1e-12, 4 * sensitivity[0])
+    assert errors["free current"] < max(1e-11, 4 * sensitivity[1])

@@ -16,7 +16,8 @@ Tolerances (class R, fixed before the run):
 import numpy as np
 import pytest
 
-from field_stages import log_ulp_sensitivity, oracle_first_trial, oracle_source_stages, rel_l2
+from field_stages import (exp_ulp_sensitivity, log_ulp_sensitivity, oracle_first_trial,
+                          oracle_source_stages, rel_l2, scan_sensitivity)
 from oracle import adams
 from solprop_amd.native import Workspace
 from solprop_amd.problem import radial_problem
@@ -32,11 +33,14 @@ CASES = {
     # A molecular response taking half the third-order response (constant rate: the
     # kernel is built once).
     "raman": {"w0": 0.25e-3, "raman_share": 0.5},
+    "band_rule": {"w0": 0.25e-3, "integration": "band", "band": (400e-9, 4e-6)},
 }
 
 
 def small(**kw):
-    args = dict(n_t=1024, window=0.25e-12, n_r=32, length=0.02, fwhm=30e-15)
+    # The trapezoid rule unless a case asks for the band rule (see the capillary tests).
+    args = dict(n_t=1024, window=0.25e-12, n_r=32, length=0.02, fwhm=30e-15,
+                integration="trapezoid")
     args.update(kw)
     return radial_problem(**args)
 
@@ -64,8 +68,9 @@ def test_radial_source_stages_match_oracle(case):
     for i, name in enumerate(names):
         if np.any(streams[i]):
             errors[name] = rel_l2(got_streams[i], streams[i])
-    sensitivity = log_ulp_sensitivity(problem, lab, z, streams)
-    print(errors, "node field", node_error, "bound", bound, "log-ulp sensitivity",
+    sensitivity = np.maximum(log_ulp_sensitivity(problem, lab, z, streams),
+                             exp_ulp_sensitivity(problem, lab, z, streams))
+    print(errors, "node field", node_error, "bound", bound, "ulp sensitivity",
           sensitivity.tolist(), flush=True)
     assert node_error <= bound
     assert errors["modal"] < 1e-14
@@ -93,7 +98,7 @@ def test_radial_first_trial_matches_oracle(case):
     assert abs(got - norms[0]) / norms[0] < 1e-8
 
 
-@pytest.mark.parametrize("case", ["wide_beam", "narrow_beam", "raman"])
+@pytest.mark.parametrize("case", ["wide_beam", "narrow_beam", "raman", "band_rule"])
 def test_radial_march_matches_oracle_decisions(case):
     print(f"radial march {case}", flush=True)
     problem = small(**CASES[case])
@@ -131,7 +136,7 @@ def test_radial_wide_grid_matches_oracle():
     fewer than the bins; every bin of the predicted and corrected fields must still be
     covered. A first trial to 1e-11 and a short march with the oracle's decisions."""
     print("radial wide grid", flush=True)
-    problem = small(n_r=128, n_t=4096, w0=0.25e-3)
+    problem = small(n_r=128, n_t=4096, w0=0.25e-3, integration="band", band=(400e-9, 4e-6))
     assert problem.n_modes * problem.n_omega > 4096 * 64
     dz = 1e-4
     advanced, norms = oracle_first_trial(problem, dz)
@@ -167,3 +172,41 @@ def test_radial_first_trial_matches_oracle_at_full_size():
     ws.close()
     assert rel_l2(field, advanced.reshape(field.shape)) < 1e-11
     assert abs(got - norms[0]) / norms[0] < 1e-8
+
+
+@pytest.mark.parametrize("case", ["narrow_beam", "band_rule"])
+def test_radial_plasma_scan_stages_match_oracle(case):
+    """The dense route's stored node arrays after one source evaluation, against the
+    oracle's scan of the device's own node field (so that the GEMM's rounding is not in
+    the comparison): rate, fraction and free current, to the capillary test's
+    tolerances."""
+    from oracle import passes as P
+    from oracle import rk
+
+    print(f"radial plasma stages {case}", flush=True)
+    problem = small(**CASES[case])
+    lab = problem.field0.reshape(problem.shape)
+    z = 1e-3
+    ws = Workspace(problem)
+    ws.load(0.0, problem.field0)
+    ws.source(z, lab)
+    node_field = ws.buffer("node_field")
+    got_rate, got_frac = ws.buffer("rate"), ws.buffer("frac")
+    free_local, tile_f = ws.buffer("free_local"), ws.buffer("tile_f")
+    ws.close()
+    src = rk.StageSource(problem)
+    field_t = node_field[:, None, :]
+    rate = P.ionisation_rate(field_t, src.plasma.rate)
+    fraction, free = P.plasma_scan(field_t, rate, src.plasma, problem.preionisation(z),
+                                   problem.dt_fine)
+    tiles = tile_f.shape[1]
+    owner = np.minimum(np.arange(problem.n_to) // 63, tiles - 1)
+    got_free = tile_f[:, owner] + free_local
+    errors = {"rate": rel_l2(got_rate, rate), "fraction": rel_l2(got_frac, fraction),
+              "free current": rel_l2(got_free, free[:, 0])}
+    sensitivity = scan_sensitivity(field_t, rate, src.plasma, problem.preionisation(z),
+                                   problem.dt_fine, fraction, free)
+    print(errors, "exp-ulp sensitivity", sensitivity.tolist(), flush=True)
+    assert errors["rate"] < 1e-13
+    assert errors["fraction"] < max(1e-12, 4 * sensitivity[0])
+    assert errors["free current"] < max(1e-11, 4 * sensitivity[1])

@@ -297,3 +297,67 @@ def test_temporal_absorber_rate():
     rate = temporal_absorber_rate(np.array([1.0, 0.5, 0.0]), 0.1)
     assert rate[0] == 0.0 and not np.signbit(rate[0])
     assert np.isclose(rate[1], 63.0) and np.isclose(rate[2], 126.0)
+
+
+# ------------------------------------------------------- band running-integral rule
+
+
+def test_band_rule_integrate_docstring_example():
+    """``nonlinear/ionisation.py:598-602``: taps (1.1, -0.05) on 0, 1, 2, 3 at dt 0.5."""
+    from oracle import passes as P
+
+    rule = P.RunningIntegral((1.1, -0.05))
+    assert rule.integrate(np.arange(4.0), 0.5).tolist() == [0.0, 0.25, 1.0, 2.25]
+
+
+def test_trapezoid_multiplier_docstring_example():
+    """``translate/plasma_integration.py:100-101``."""
+    from solprop_amd.problem import trapezoid_multiplier
+
+    assert f"{float(trapezoid_multiplier(np.array(np.pi / 3))):.6f}" == "0.906900"
+
+
+@pytest.mark.parametrize("theta_b", [0.5, 1.26, 1.5])
+def test_band_design_meets_its_certificate(theta_b):
+    """The designed taps' composite multiplier is within 1e-5 of one over the band (on a
+    sampling independent of the design's), one exactly at zero frequency, at most 1.05
+    above the band; the plain trapezoid (one identity tap) misses the first by far."""
+    from solprop_amd.problem import band_composite, design_band_taps
+
+    taps = design_band_taps(theta_b, 1.0)  # dt = theta_b, edge = 1 rad/s
+    band = np.linspace(0.0, theta_b, 30001)
+    assert np.abs(band_composite(taps, band) - 1.0).max() <= 1e-5
+    assert abs(band_composite(taps, np.array([0.0]))[0] - 1.0) < 1e-15
+    assert np.abs(band_composite(taps, np.linspace(theta_b, np.pi, 30001))).max() <= 1.05
+    assert np.abs(band_composite(np.ones(1), band) - 1.0).max() > 1e-3
+
+
+def test_band_design_refuses_a_band_too_close_to_nyquist():
+    """Below an oversampling of about 3.5 no design with K <= 8 certifies."""
+    from solprop_amd.problem import design_band_taps
+
+    with pytest.raises(ValueError, match="no band-rule design"):
+        design_band_taps(2.0, 1.0)
+    with pytest.raises(ValueError, match="Nyquist"):
+        design_band_taps(3.5, 1.0)
+
+
+def test_band_rule_integrates_a_band_limited_signal():
+    """Between interior samples a cosine inside the band integrates to the difference of
+    sin(ωt)/ω within the design tolerance (the trapezoid's error is ~θ²/12); the ends,
+    closed by reflection, only add a constant."""
+    from oracle import passes as P
+    from solprop_amd.problem import design_band_taps
+
+    dt, theta_b = 1.0, 1.2
+    rule = P.RunningIntegral(tuple(design_band_taps(dt, theta_b)))
+    t = np.arange(4096) * dt
+    omega = 1.0
+    got = rule.integrate(np.cos(omega * t), dt)
+    exact = np.sin(omega * t) / omega
+    inner = slice(50, -50)
+    got_span = got[inner] - got[50]
+    exact_span = exact[inner] - exact[50]
+    assert np.abs(got_span - exact_span).max() < 3e-5
+    plain = P.RunningIntegral((1.0,)).integrate(np.cos(omega * t), dt)
+    assert np.abs(plain[inner] - plain[50] - exact_span).max() > 1e-2

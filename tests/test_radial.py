@@ -64,7 +64,8 @@ def test_qdht_without_j1_scalings_fails_the_closed_form():
 
 
 def test_radial_record_uses_the_transform_pair():
-    problem = radial_problem(n_t=1024, window=0.25e-12, n_r=32, length=0.02, fwhm=30e-15)
+    problem = radial_problem(n_t=1024, window=0.25e-12, n_r=32, length=0.02, fwhm=30e-15,
+                             integration="trapezoid")
     problem.validate()
     r, k, forward, inverse = qdht(4.0e-3, 32)
     assert np.array_equal(problem.synthesis, inverse)

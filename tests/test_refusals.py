@@ -26,18 +26,20 @@ def test_envelope_under_adams_is_refused():
 
 
 def test_field_route_under_rk_is_refused():
-    refused(dataclasses.replace(capillary_problem(n_t=256, n_nodes=4), method="rk"),
+    refused(dataclasses.replace(capillary_problem(n_t=256, n_nodes=4, integration="trapezoid"),
+                                method="rk"),
             "exponential Adams")
 
 
-def test_band_running_integral_is_refused():
-    p = capillary_problem(n_t=256, n_nodes=4)
-    plasma = dataclasses.replace(p.plasma, taps=(0.5, 0.25))
-    refused(dataclasses.replace(p, plasma=plasma), "band running-integral")
+def test_band_rule_beyond_eight_taps_a_side_is_refused():
+    p = capillary_problem(n_t=256, n_nodes=4, integration="trapezoid")
+    plasma = dataclasses.replace(p.plasma, taps=np.full(10, 0.1))
+    refused(dataclasses.replace(p, plasma=plasma), "9 \\(the band rule\\) taps")
 
 
 def test_short_field_grid_is_refused():
-    refused(capillary_problem(n_t=128, n_nodes=4), "power of two >= 256")
+    refused(capillary_problem(n_t=128, n_nodes=4, integration="trapezoid"),
+            "power of two >= 256")
 
 
 def test_envelope_with_several_modes_is_refused():
@@ -53,6 +55,6 @@ def test_envelope_with_several_modes_is_refused():
 
 def test_field_raman_without_the_harmonic_is_refused():
     """The analytic-signal drive (|E_a|²/2, a grid without the harmonic) is not built."""
-    p = capillary_problem(n_t=256, n_nodes=4, raman_share=0.5)
+    p = capillary_problem(n_t=256, n_nodes=4, raman_share=0.5, integration="trapezoid")
     refused(dataclasses.replace(p, raman=dataclasses.replace(p.raman, thg=False)),
             "analytic-signal")
