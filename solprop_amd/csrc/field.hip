@@ -425,103 +425,162 @@ __device__ inline double current_sample(const DeviceProblem& P, const FieldBuffe
 }
 
 // Point reflection past the grid's ends: y(-m) = 2 y(0) - y(m), y(T-1+m) = 2 y(T-1) -
-// y(T-1-m) (ionisation.py:642-644).
+// y(T-1-m) (ionisation.py:642-644). Samples past T-1+K lie outside every owned
+// sample's stencil and read as zero.
 template <class Sample>
-__device__ inline double reflected(int T, int t, const Sample& sample) {
+__device__ inline double reflected(int T, int K, int t, const Sample& sample) {
+  if (t > T - 1 + K) return 0.0;
   if (t < 0) return 2.0 * sample(0) - sample(-t);
   if (t > T - 1) return 2.0 * sample(T - 1) - sample(2 * (T - 1) - t);
   return sample(t);
 }
 
-// Pass A under the band rule: the rate over each wave's window, the raw rate stored for
-// the owned samples, the filtered rate's trapezoid increments scanned, the within-tile
-// exposure stored and the tile's sum. Every wave reaches both barriers of every node.
+// A tile's halo of 2K samples: slot h < K is sample 63 tile - K + h, left of the tile's
+// 64 lanes; slot h >= K is sample 63 tile + 64 + (h - K), right of them.
+__device__ inline int halo_sample(int tile, int K, int h) {
+  return h < K ? tile * kTileSpan - K + h : tile * kTileSpan + kWave + (h - K);
+}
+
+// Where halo slot h sits in the window (window[j] is sample 63 tile - K + j).
+__device__ inline int halo_slot(int K, int h) { return h < K ? h : kWave + h; }
+
+// Orders a wave's LDS writes before its lanes read each other's values: the LDS serves
+// one wave's instructions in issue order, so only the compiler has to be held back.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The band passes run each wave's nodes as the trapezoid passes do (modal fields of the
+// lane's sample held in registers across them), with the filter's halo of every node of
+// the wave evaluated first in one pass: lane l takes slot l % 2K of the wave's node
+// l / 2K (2K <= 16, so four nodes fit one wave).
+static_assert(kNodesPerWave * 2 * kMaxBandHalf <= kWave, "one pass holds every halo");
+
+// Pass A under the band rule: the raw rate stored for the owned samples, the filtered
+// rate's trapezoid increments scanned, the within-tile exposure stored and the tile's
+// sum.
 template <int C>
 __global__ void __launch_bounds__(kWave * kNodeWaves)
     plasma_rate_band(DeviceProblem P, PlasmaTable pt, FieldBuffers B) {
   __shared__ double table[4 * kLdsTableIntervals];
   __shared__ double windows[kNodeWaves][kBandSpan];
+  __shared__ double halos[kNodeWaves][kNodesPerWave * 2 * kMaxBandHalf];
   const bool in_lds = pt.n_intervals <= kLdsTableIntervals;
   if (in_lds)
     for (int i = threadIdx.x; i < 4 * pt.n_intervals; i += blockDim.x) table[i] = B.rate_c[i];
   __syncthreads();
   const double* coefficients = in_lds ? table : B.rate_c;
-  const int T = P.T, tiles = scan_tiles(T), K = pt.n_taps - 1;
+  const int T = P.T, tiles = scan_tiles(T), K = pt.n_taps - 1, H = 2 * K;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const int tile = blockIdx.x;
-  const int first = tile * kTileSpan - K;
   const int t = tile * kTileSpan + lane;
   const bool stores = t < T && (lane < kTileSpan || t == T - 1);
   const bool step = lane < kTileSpan && t + 1 < T;
   double* window = windows[wave];
-  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave;
-  for (int i = 0; i < kNodesPerWave; ++i) {
-    const int q = q0 + i * kNodeWaves + wave;
-    const bool active = q < P.Q;  // uniform across the wave
-    if (active) {
+  double* halo = halos[wave];
+  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave + wave;
+  if (lane < H * kNodesPerWave) {
+    const int i = lane / H, h = lane - i * H;
+    const int q = q0 + i * kNodeWaves;
+    double v = 0.0;
+    if (q < P.Q) {
       auto sample = [&](int u) { return rate_sample<C>(P, pt, coefficients, B, q, u); };
-      for (int j = lane; j < kWave + 2 * K; j += kWave) window[j] = reflected(T, first + j, sample);
+      v = reflected(T, K, halo_sample(tile, K, h), sample);
     }
-    __syncthreads();
-    if (active) {
-      const long row = static_cast<long>(q) * T;
-      if (stores) B.rate[row + t] = window[K + lane];
-      // Cross-lane moves with every lane of the wave active: a lane masked off reads as
-      // zero to its neighbours.
-      const double y = band_filter(pt, window, lane);
-      const double y_next = from_next_lane(y);
-      const double inc = step ? (P.dt_fine * (y_next + y)) / 2.0 : 0.0;
-      const double incl = wave_scan(inc);
-      const double excl = from_previous_lane(incl);
-      if (lane == kTileSpan - 1) B.tile_x[static_cast<long>(q) * tiles + tile] = incl;
-      if (stores) B.x_local[row + t] = excl;
+    halo[lane] = v;
+  }
+  double am[held(C)];
+  load_modal<C>(B, T, t, am);
+  for (int i = 0; i < kNodesPerWave; ++i) {
+    const int q = q0 + i * kNodeWaves;
+    if (q >= P.Q) break;  // uniform across the wave, and so for every later node
+    double w;
+    if (t < T) {
+      w = rate_of(fabs(field_at<C>(P, B, am, q, t)), pt, coefficients);
+    } else {
+      auto sample = [&](int u) { return rate_sample<C>(P, pt, coefficients, B, q, u); };
+      w = reflected(T, K, t, sample);
     }
-    __syncthreads();
+    wave_sync();  // the previous node's window has been read
+    window[K + lane] = w;
+    if (lane < H) window[halo_slot(K, lane)] = halo[i * H + lane];
+    wave_sync();
+    const long row = static_cast<long>(q) * T;
+    if (stores) B.rate[row + t] = w;
+    // Cross-lane moves with every lane of the wave active: a lane masked off reads as
+    // zero to its neighbours.
+    const double y = band_filter(pt, window, lane);
+    const double y_next = from_next_lane(y);
+    const double inc = step ? (P.dt_fine * (y_next + y)) / 2.0 : 0.0;
+    const double incl = wave_scan(inc);
+    const double excl = from_previous_lane(incl);
+    if (lane == kTileSpan - 1) B.tile_x[static_cast<long>(q) * tiles + tile] = incl;
+    if (stores) B.x_local[row + t] = excl;
   }
 }
 
-// Pass C under the band rule: the free-current integrand over each wave's window (the
-// fraction from the stored exposure), filtered, its increments scanned; the fraction and
-// the within-tile free current stored for the owned samples, and the tile's sum.
+// Pass C under the band rule: the free-current integrand (the fraction from the stored
+// exposure) filtered and its increments scanned; the fraction and the within-tile free
+// current stored for the owned samples, and the tile's sum.
 template <int C>
 __global__ void __launch_bounds__(kWave * kNodeWaves)
     plasma_current_band(DeviceProblem P, PlasmaTable pt, const AdamsRecord* __restrict__ R,
                         FieldBuffers B) {
   __shared__ double windows[kNodeWaves][kBandSpan];
-  const int T = P.T, tiles = scan_tiles(T), K = pt.n_taps - 1;
+  __shared__ double halos[kNodeWaves][kNodesPerWave * 2 * kMaxBandHalf];
+  const int T = P.T, tiles = scan_tiles(T), K = pt.n_taps - 1, H = 2 * K;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const int tile = blockIdx.x;
-  const int first = tile * kTileSpan - K;
   const int t = tile * kTileSpan + lane;
+  const int owner = min(t / kTileSpan, tiles - 1);
   const bool stores = t < T && (lane < kTileSpan || t == T - 1);
   const bool step = lane < kTileSpan && t + 1 < T;
   const double f0 = R->preionisation;
   double* window = windows[wave];
-  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave;
-  for (int i = 0; i < kNodesPerWave; ++i) {
-    const int q = q0 + i * kNodeWaves + wave;
-    const bool active = q < P.Q;  // uniform across the wave
-    if (active) {
+  double* halo = halos[wave];
+  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave + wave;
+  if (lane < H * kNodesPerWave) {
+    const int i = lane / H, h = lane - i * H;
+    const int q = q0 + i * kNodeWaves;
+    double v = 0.0;
+    if (q < P.Q) {
       auto sample = [&](int u) { return current_sample<C>(P, B, q, u, f0, tiles); };
-      for (int j = lane; j < kWave + 2 * K; j += kWave) window[j] = reflected(T, first + j, sample);
+      v = reflected(T, K, halo_sample(tile, K, h), sample);
     }
-    __syncthreads();
-    if (active) {
-      const long row = static_cast<long>(q) * T;
-      // Cross-lane moves with every lane of the wave active (see the rate pass).
-      const double y = band_filter(pt, window, lane);
-      const double y_next = from_next_lane(y);
-      const double inc = step ? (P.dt_fine * (y_next + y)) / 2.0 : 0.0;
-      const double incl = wave_scan(inc);
-      const double excl = from_previous_lane(incl);
-      if (lane == kTileSpan - 1) B.tile_f[static_cast<long>(q) * tiles + tile] = incl;
-      if (stores) {
-        const double x = B.tile_x[static_cast<long>(q) * tiles + tile] + B.x_local[row + t];
-        B.frac[row + t] = fraction_of(f0, x > 0.0 ? x : 0.0);
-        B.free_local[row + t] = excl;
-      }
+    halo[lane] = v;
+  }
+  double am[held(C)];
+  load_modal<C>(B, T, t, am);
+  for (int i = 0; i < kNodesPerWave; ++i) {
+    const int q = q0 + i * kNodeWaves;
+    if (q >= P.Q) break;  // uniform across the wave, and so for every later node
+    const long row = static_cast<long>(q) * T;
+    double f = 0.0, y;
+    if (t < T) {
+      const double x = B.tile_x[static_cast<long>(q) * tiles + owner] + B.x_local[row + t];
+      f = fraction_of(f0, x > 0.0 ? x : 0.0);
+      y = (f * P.plasma_cmr) * field_at<C>(P, B, am, q, t);
+    } else {
+      auto sample = [&](int u) { return current_sample<C>(P, B, q, u, f0, tiles); };
+      y = reflected(T, K, t, sample);
     }
-    __syncthreads();
+    wave_sync();  // the previous node's window has been read
+    window[K + lane] = y;
+    if (lane < H) window[halo_slot(K, lane)] = halo[i * H + lane];
+    wave_sync();
+    // Cross-lane moves with every lane of the wave active (see the rate pass).
+    const double yf = band_filter(pt, window, lane);
+    const double y_next = from_next_lane(yf);
+    const double inc = step ? (P.dt_fine * (y_next + yf)) / 2.0 : 0.0;
+    const double incl = wave_scan(inc);
+    const double excl = from_previous_lane(incl);
+    if (lane == kTileSpan - 1) B.tile_f[static_cast<long>(q) * tiles + tile] = incl;
+    if (stores) {
+      B.frac[row + t] = f;
+      B.free_local[row + t] = excl;
+    }
   }
 }
 

@@ -414,10 +414,25 @@ void FieldWorkspace::enqueue_split(const AdamsRecord* R, double dz) {
   launch_field_split_unpack(P_, B_, stream_);
 }
 
+// The phases this problem's trial launches work in; the others are two events with
+// nothing between them and are not reported.
+bool FieldWorkspace::phase_used(int i) const {
+  switch (i) {
+    case 3:   // synthesis_gemm
+    case 10:  // projection_gemm
+      return dense_;
+    case 4: return P_.n_raman > 0;
+    case 5: case 6: case 7: case 8: return P_.has_plasma != 0;
+    case 11: return in_stage_;
+    default: return true;
+  }
+}
+
 std::vector<std::pair<const char*, double>> FieldWorkspace::phases() const {
   std::vector<std::pair<const char*, double>> out;
   for (int i = 0; i < kPhases; ++i)
-    out.emplace_back(kPhaseNames[i], timed_ ? phase_ms_[i] * 1e3 / timed_ : 0.0);
+    if (phase_used(i))
+      out.emplace_back(kPhaseNames[i], timed_ ? phase_ms_[i] * 1e3 / timed_ : 0.0);
   return out;
 }
 

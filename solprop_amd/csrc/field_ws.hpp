@@ -37,6 +37,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   long buffer(const char* name, double* out, long count) override;
   void set_profile(bool on) override { profile_ = on; }
   std::vector<std::pair<const char*, double>> phases() const override;
+  bool phase_used(int i) const;
   static constexpr int kPhases = 16;
   double trial_us() const override { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
   long timed() const override { return timed_; }
