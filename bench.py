@@ -61,6 +61,8 @@ def trial_bytes(problem, order):
     comp = n_phi > 1
     dense = C > 8
     plasma = problem.plasma is not None
+    # The band rule's rate pass also stores the within-tile exposure.
+    band = plasma and len(problem.plasma.taps) > 1
     n_streams = 3 if plasma else 1
     field = qt if dense else ct + table  # what a node pass reads to know E at its nodes
     raman = problem.raman is not None
@@ -74,7 +76,7 @@ def trial_bytes(problem, order):
         "inverse_fft": 2 * ct,
         "synthesis_gemm": table + ct + qt if dense else 0,
         "raman": field + qt2 * 7 if raman else 0,
-        "plasma_rate_tiles": field + qt + qk,
+        "plasma_rate_tiles": field + qt + qk + (qt if band else 0),
         "plasma_tile_prefix": 2 * qk,
         "plasma_current_tiles": field + qt + 2 * qk + 2 * qt,
         "plasma_tile_prefix_current": 2 * qk,

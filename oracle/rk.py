@@ -197,7 +197,9 @@ def march(problem, state, *, z0=0.0, proposal=None, targets=None, policy=None,
     """Adaptive march to each target (``marches/runge_kutta.py:133-210``).
 
     Returns ``(state, records)`` where records hold ``(z, dz, norm, accepted)`` per
-    attempt. The finish of an accepted step is the hard band mask.
+    attempt. The finish of an accepted step is the hard band mask and, under the
+    split-step absorber, the collar over the step and the mask again
+    (``src/solprop/backends/numpy_ref/workspace.py:1021-1106``).
     """
     source = StageSource(problem)
     if policy is None:
@@ -229,6 +231,11 @@ def march(problem, state, *, z0=0.0, proposal=None, targets=None, policy=None,
             attempts += 1
             if accepted:
                 state = P.apply_hard_mask(advanced, problem.active_mask)
+                if problem.absorber_scheme == "split_step":
+                    state = P.apply_hard_mask(
+                        P.apply_temporal_rate(state, problem.absorber_rate, stop.dz,
+                                              problem.kind),
+                        problem.active_mask)
                 st.z = end
                 st.norms.append(equivalent)
                 if policy.auto_init and len(st.norms) == 1:

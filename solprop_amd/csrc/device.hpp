@@ -26,7 +26,8 @@ static_assert(sizeof(StageRow) == kRowDoubles * sizeof(double), "stage row layou
 // What a trial uploads: the step and its stage rows, plus the whole step's row.
 struct TrialRecord {
   double dz;
-  double pad[kRowDoubles - 1];
+  double accepted_dz;  // the step the commit at this trial's head finishes (split-step)
+  double pad[kRowDoubles - 2];
   StageRow stage[kMaxStages];
   StageRow whole;
 };
@@ -60,10 +61,10 @@ struct DeviceProblem {
   const double* synthesis;     // (Q, C)
   const double* projection;    // (C, Q)
   const double* raman_rows;    // (n_raman, 2T)
-  // The temporal absorber (field routes): the power rate alpha(t), 1/m, on the fine grid
-  // (in-stage scheme) or the coarse grid (split-step scheme); null without one. The
-  // in-stage term -alpha E / 2 rides the forward transform as stream `absorber_stream`
-  // (-1: none).
+  // The temporal absorber: the power rate alpha(t), 1/m, on the fine grid (in-stage
+  // scheme) or the coarse grid (split-step scheme); null without one. The in-stage term
+  // -alpha E / 2 rides the forward transform as stream `absorber_stream` (-1: none); on
+  // the envelope route that stream is the fine row after the polarisation's.
   const double* absorber_rate;
   int absorber_stream;
   // The Runge-Kutta tableau.

@@ -32,5 +32,10 @@ void launch_rk_accept(const DeviceProblem& P, const TrialRecord* R, const cplx* 
                       hipStream_t s);
 void launch_commit_masked(const DeviceProblem& P, const cplx* adv, cplx* y, hipStream_t s);
 void launch_copy_points(const cplx* src, cplx* dst, long n, hipStream_t s);
+// The envelope route's split-step absorber, around the caller's inverse and forward
+// transforms of the coarse grid (the unpack is `launch_commit_masked` from the buffer).
+void launch_env_split_pack(const DeviceProblem& P, const cplx* y, cplx* buf, hipStream_t s);
+void launch_env_split_collar(const DeviceProblem& P, const TrialRecord* R, double dz, cplx* buf,
+                             hipStream_t s);
 
 }  // namespace sp

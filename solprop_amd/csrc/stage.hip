@@ -141,6 +141,11 @@ __global__ void env_response(DeviceProblem P, const TrialRecord* R, cplx* fine,
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P.T) return;
   const StageRow& row = R->stage[stage];
+  // The in-stage absorber's term from the modal field, before the polarisation takes
+  // its place (src/solprop/backends/numpy_ref/passes/stepping.py:286-345).
+  if (P.absorber_stream >= 0)
+    fine[static_cast<long>(P.absorber_stream) * P.T + t] =
+        scale(-0.5 * P.absorber_rate[t], fine[t]);
   const cplx e = scale(P.synthesis[0] * row.taper, fine[t]);
   cplx pol = make(0.0, 0.0);
   if (P.has_kerr) {
@@ -172,7 +177,11 @@ __global__ void rk_finish(DeviceProblem P, const TrialRecord* R, const cplx* fin
   const int b = static_cast<int>(idx % P.N);
   const cplx pref = phase_sum(R->stage[stage].pref_w, P.pref, P.n_pref, CN, idx);
   const cplx pt = fine[static_cast<long>(c) * P.NF + P.crop[b]];
-  const cplx normalised = mul(scale(P.omega_window[b], pref), pt);
+  cplx normalised = mul(scale(P.omega_window[b], pref), pt);
+  // The absorber's term joins unscaled, on the active band (stepping.py:88-145).
+  if (P.absorber_stream >= 0 && P.active[b])
+    normalised = add(normalised,
+                     fine[static_cast<long>(P.absorber_stream + c) * P.NF + P.crop[b]]);
   k[stage * CN + idx] = mul(einv[idx], normalised);
 }
 
@@ -235,6 +244,24 @@ __global__ void commit_masked(DeviceProblem P, const cplx* adv, cplx* y) {
   const long CN = static_cast<long>(P.C) * P.N;
   if (idx >= CN) return;
   y[idx] = P.active[idx % P.N] ? adv[idx] : make(0.0, 0.0);
+}
+
+// The split-step absorber on the committed envelope (the workspace's finish,
+// src/solprop/backends/numpy_ref/workspace.py:1021-1106): the masked state to time
+// samples (1/N folded in before the unscaled inverse transform), times exp(-alpha h / 2)
+// with h the accepted step (the record's, inside a graph; else `dz`), and, after the
+// forward transform, back onto the state with the band imposed again.
+__global__ void env_split_pack(DeviceProblem P, const cplx* y, cplx* buf) {
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long>(P.C) * P.N) return;
+  buf[idx] = scale(1.0 / P.N, y[idx]);
+}
+
+__global__ void env_split_collar(DeviceProblem P, const TrialRecord* R, double dz, cplx* buf) {
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long>(P.C) * P.N) return;
+  const double h = R ? R->accepted_dz : dz;
+  buf[idx] = scale(exp((-0.5 * P.absorber_rate[idx % P.N]) * h), buf[idx]);
 }
 
 __global__ void copy_points(const cplx* src, cplx* dst, long n) {
@@ -312,6 +339,22 @@ void launch_commit_masked(const DeviceProblem& P, const cplx* adv, cplx* y, hipS
   const std::size_t n = static_cast<std::size_t>(P.C) * P.N;
   hipLaunchKernelGGL(commit_masked, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, P, adv,
                      y);
+  check_launch();
+}
+
+void launch_env_split_pack(const DeviceProblem& P, const cplx* y, cplx* buf, hipStream_t s) {
+  const std::size_t n = static_cast<std::size_t>(P.C) * P.N;
+  hipLaunchKernelGGL(env_split_pack, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
+                     P, y, buf);
+  check_launch();
+}
+
+void launch_env_split_collar(const DeviceProblem& P, const TrialRecord* R, double dz, cplx* buf,
+                             hipStream_t s) {
+  SP_CHECK(P.absorber_rate, "the split-step absorber's rate");
+  const std::size_t n = static_cast<std::size_t>(P.C) * P.N;
+  hipLaunchKernelGGL(env_split_collar, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                     s, P, R, dz, buf);
   check_launch();
 }
 

@@ -41,8 +41,11 @@ Workspace::Workspace(const sp_problem_desc& d) {
   SP_CHECK(d.n_phi >= 1 && d.n_phi <= kMaxTerms, "1..4 phi terms");
   SP_CHECK(d.n_prefactor >= 1 && d.n_prefactor <= kMaxTerms, "1..4 prefactor terms");
   SP_CHECK(d.n_raman >= 0 && d.n_raman <= kMaxTerms, "0..4 Raman rows");
-  SP_CHECK(d.absorber_scheme == 0,
-           "the envelope route runs no temporal absorber (the field routes do)");
+  SP_CHECK(d.absorber_scheme >= 0 && d.absorber_scheme <= 2,
+           "absorber scheme 0 (none), 1 (in-stage) or 2 (split-step)");
+  SP_CHECK(d.absorber_scheme == 0 || d.absorber_rate, "the absorber's rate is missing");
+  SP_CHECK(d.absorber_scheme != 2 || (d.n_t >= 16 && (d.n_t & (d.n_t - 1)) == 0),
+           "the split-step absorber transforms the coarse grid: n_t a power of two >= 16");
   SP_CHECK(d.n_stages >= 1 && d.n_stages <= kMaxStages, "1..8 Runge-Kutta stages");
   SP_CHECK(!d.has_plasma, "the envelope route carries no field-grid plasma");
   SP_CHECK(d.crop_index && d.active_mask && d.norm_mask && d.omega_window && d.omega &&
@@ -69,7 +72,12 @@ Workspace::Workspace(const sp_problem_desc& d) {
   density_.assign(d.density);
   preion_.assign(d.preionisation);
   tab_c_.assign(d.tableau_c, d.tableau_c + d.n_stages);
-  fsal_ = d.first_same_as_last != 0;
+  in_stage_ = d.absorber_scheme == 1;
+  split_ = d.absorber_scheme == 2;
+  // The split collar changes the committed state after the trial: its last stage is no
+  // longer the new state's first, and every stage is evaluated
+  // (src/solprop/backends/numpy_ref/engines.py:99-178).
+  fsal_ = d.first_same_as_last != 0 && !split_;
 
   // Controller and landings (the run's policy, src/solprop/backends/records.py:227).
   auto& c = policy_.controller;
@@ -142,6 +150,10 @@ Workspace::Workspace(const sp_problem_desc& d) {
   P_.projection = device_copy(d.projection, static_cast<std::size_t>(d.n_nodes) * C);
   P_.raman_rows = d.n_raman ? device_copy(d.raman_undamped, d.n_raman * 2 * std::size_t(T))
                             : nullptr;
+  P_.absorber_rate =
+      d.absorber_scheme ? device_copy(d.absorber_rate, static_cast<std::size_t>(in_stage_ ? T : N))
+                        : nullptr;
+  P_.absorber_stream = in_stage_ ? C : -1;
   P_.tab_a = device_copy(d.tableau_a, static_cast<std::size_t>(d.n_stages) * d.n_stages);
   P_.tab_b = device_copy(d.tableau_b, d.n_stages);
   P_.tab_e = device_copy(d.tableau_e, d.n_stages);
@@ -151,9 +163,12 @@ Workspace::Workspace(const sp_problem_desc& d) {
   recycled_ = device_alloc<cplx>(CN);
   einv_ = device_alloc<cplx>(CN);
   k_ = device_alloc<cplx>(CN * d.n_stages);
-  fine_ = device_alloc<cplx>(static_cast<std::size_t>(C) * T);
-  scratch_ = device_alloc<cplx>(static_cast<std::size_t>(C > 1 ? C : 1) * T);
+  const std::size_t rows = in_stage_ ? 2 * C : C;
+  fine_ = device_alloc<cplx>(rows * T);
+  scratch_ = device_alloc<cplx>(rows * T);
   fine_plan_.init(T, C);
+  if (in_stage_) forward_plan_.init(T, rows);
+  if (split_) coarse_plan_.init(N, C);
   if (d.n_raman) {
     drive_ = device_alloc<cplx>(T);
     H_ = device_alloc<cplx>(T + 1);
@@ -220,13 +235,24 @@ void Workspace::enqueue_stage(int stage) {
     raman_plan_.exec(drive_, scratch_, +1, stream_);
   }
   launch_env_response(P_, rec_dev_, fine_, drive_, stage, stream_);
-  fine_plan_.exec(fine_, scratch_, -1, stream_);
+  (in_stage_ ? forward_plan_ : fine_plan_).exec(fine_, scratch_, -1, stream_);
   launch_rk_finish(P_, rec_dev_, fine_, einv_, k_, stage, stream_);
+}
+
+void Workspace::enqueue_commit(const TrialRecord* R, double dz) {
+  launch_commit_masked(P_, adv_, y_, stream_);
+  if (!split_) return;
+  // fine_ is free between trials: it holds the coarse grid's time samples here.
+  launch_env_split_pack(P_, y_, fine_, stream_);
+  coarse_plan_.exec(fine_, scratch_, +1, stream_);
+  launch_env_split_collar(P_, R, dz, fine_, stream_);
+  coarse_plan_.exec(fine_, scratch_, -1, stream_);
+  launch_commit_masked(P_, fine_, y_, stream_);
 }
 
 void Workspace::enqueue_trial(bool commit, Stage0 mode) {
   const long CN = static_cast<long>(P_.C) * P_.N;
-  if (commit) launch_commit_masked(P_, adv_, y_, stream_);
+  if (commit) enqueue_commit(rec_dev_, 0.0);
   if (mode == kCopyRecycled) launch_copy_points(recycled_, k_, CN, stream_);
   for (int i = (mode == kEvaluate ? 0 : 1); i < P_.n_stages; ++i) enqueue_stage(i);
   launch_rk_accept(P_, rec_dev_, y_, k_, adv_, recycled_, partials_, stream_);
@@ -259,6 +285,7 @@ double Workspace::trial(double z, double dz, bool full) {
   SP_CHECK(std::isfinite(dz) && dz > 0.0, "dz must be finite and positive");
   fill_record(z, dz);
   const bool commit = commit_pending_;
+  rec_host_->accepted_dz = commit ? pending_dz_ : 0.0;
   Stage0 mode = kEvaluate;
   if (!full && fsal_) {
     if (commit && k0_valid_)
@@ -297,16 +324,17 @@ double Workspace::trial(double z, double dz, bool full) {
   return norm;
 }
 
-void Workspace::accept(double) {
+void Workspace::accept(double dz) {
   if (!trial_held_) throw Error(kState, "accept without a held trial");
   commit_pending_ = true;
+  pending_dz_ = dz;
   trial_held_ = false;
   accepted_ += 1;
 }
 
 void Workspace::ensure_committed() {
   if (!commit_pending_) return;
-  launch_commit_masked(P_, adv_, y_, stream_);
+  enqueue_commit(nullptr, pending_dz_);
   SP_HIP(hipStreamSynchronize(stream_));
   commit_pending_ = false;
   // k[0] is still the committed state's source only if it was recycled into place.
@@ -420,7 +448,8 @@ sp_counters Workspace::counters() const {
   c.evaluations = evaluations_;
   c.accepted = accepted_;
   c.graph_launches = graph_launches_;
-  c.device_bytes = static_cast<long>(device_bytes_ + fine_plan_.bytes() + raman_plan_.bytes());
+  c.device_bytes = static_cast<long>(device_bytes_ + fine_plan_.bytes() + raman_plan_.bytes() +
+                                     forward_plan_.bytes() + coarse_plan_.bytes());
   return c;
 }
 

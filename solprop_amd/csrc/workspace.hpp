@@ -50,6 +50,9 @@ class Workspace final : public Propagator {
   void enqueue_stage(int stage);
   void enqueue_raman_kernel(int stage);
   void ensure_committed();
+  // The commit of the held trial: the band mask and, under the split-step absorber, the
+  // collar over the accepted step (R's, inside a graph; else `dz`) and the mask again.
+  void enqueue_commit(const TrialRecord* R, double dz);
   double finish_trial();
 
   DeviceProblem P_{};
@@ -57,6 +60,11 @@ class Workspace final : public Propagator {
   std::size_t device_bytes_ = 0;
   hipStream_t stream_ = nullptr;
   FftPlan fine_plan_, raman_plan_;
+  // The forward transform over the polarisation's rows and the in-stage absorber's, and
+  // the coarse grid's transform for the split-step absorber.
+  FftPlan forward_plan_, coarse_plan_;
+  bool in_stage_ = false, split_ = false;
+  double pending_dz_ = 0.0;
   // Host scalar paths.
   std::vector<ScalarPath> phi_paths_, pref_paths_, raman_paths_;
   ScalarPath density_, preion_;

@@ -393,6 +393,8 @@ def gnlse_problem(
     noise_model: str = "input",
     time_collar: float = 0.05,
     band=(700e-9, 5e-6),
+    absorber: str = "none",
+    absorber_length: float | None = None,
 ) -> Problem:
     """Single-mode envelope GNLSE: sech pulse at soliton order N, Kerr + Raman.
 
@@ -406,6 +408,10 @@ def gnlse_problem(
     ``(1 - f_R)|A|^2 A`` and Raman ``f_R A (h * |A|^2)`` with ``∫h = 1`` (the row holds
     ``2 h`` against the ``|A|^2/2`` drive; ``ε_0 = 1`` in these units).
     ``betas`` are β_2..β_6 in s^n/m (β_2 = -20 ps^2/km by default).
+    ``absorber`` ("none", "in_stage", "split_step") turns the time collar into a
+    temporal absorber of reference length ``absorber_length`` (default ``length / 20``):
+    its rate on the fine grid for the in-stage scheme, on the coarse grid (the collar
+    at ``n_t`` samples) for the split-step one.
     """
     n_to = ffac * n_t
     dt = window / n_t
@@ -486,6 +492,10 @@ def gnlse_problem(
         noise0=None if noise is None else np.ascontiguousarray(noise),
         noise_model=noise_model if noise is not None else "none",
         rtol=rtol,
+        absorber_scheme=absorber,
+        absorber_rate=None if absorber == "none" else temporal_absorber_rate(
+            t_window if absorber == "in_stage" else _collar(n_t, int(time_collar * n_t)),
+            absorber_length if absorber_length else length / 20),
         description={
             "route": "envelope GNLSE",
             "n_t": n_t,

@@ -55,10 +55,13 @@ def oracle_trial(problem, dz, z=0.0):
     return rk.rk_trial(problem, rk.StageSource(problem), y0, z, dz)
 
 
-@pytest.mark.parametrize("case", ["kerr_raman", "kerr_only", "modified_noise", "linear"])
+@pytest.mark.parametrize("case", ["kerr_raman", "kerr_only", "modified_noise", "linear",
+                                  "in_stage_absorber"])
 def test_probe_trial_matches_oracle(case):
     print(f"probe trial {case}", flush=True)
-    if case == "kerr_only":
+    if case == "in_stage_absorber":
+        problem = small(absorber="in_stage", absorber_length=1e-3)
+    elif case == "kerr_only":
         problem = small(raman_fraction=0.0)
     elif case == "modified_noise":
         problem = small(noise_model="modified")
@@ -113,6 +116,54 @@ def test_march_matches_oracle_decisions():
     steps = np.array([r[1] for r in records])
     assert np.allclose(rows[:, 1], steps, rtol=1e-9, atol=0)
     assert rel_l2(state, expected.reshape(state.shape)) < 1e-9
+
+
+@pytest.mark.parametrize("scheme", ["in_stage", "split_step"])
+def test_march_with_absorber_matches_oracle(scheme):
+    """Each absorber scheme on the envelope route: the oracle's decisions and end state
+    (as the plain march), and an end state measurably away from the march without it
+    (the noise floor reaches the collar, so the absorber has something to take).
+
+    The reference length is 1 mm (a rate of 12.6 / mm at the window's edge). At a
+    quarter of that the in-stage term is stiff for the pair (alpha h near 2.5): the
+    oracle's own march then moves its norms by 4e-5 and its end state by 6e-11 when the
+    launch field moves by 1e-12, and no march at rounding level can be held to 1e-9.
+    """
+    print(f"march with absorber {scheme}", flush=True)
+    problem = small(length=5e-3, absorber=scheme, absorber_length=1e-3)
+    y0 = problem.field0.reshape(problem.shape)
+    expected, records = rk.march(problem, y0)
+    plain, _ = rk.march(small(length=5e-3), y0)
+    ws = Workspace(problem)
+    session = ws.start(0.0, problem.field0)
+    rows = session.run(problem.z_end, work_limit=64)
+    state = session.state()
+    ws.close()
+    assert len(rows) == len(records)
+    steps = np.array([r[1] for r in records])
+    assert [bool(r[3]) for r in rows] == [bool(r[3]) for r in records]
+    assert np.allclose(rows[:, 1], steps, rtol=1e-9, atol=0)
+    assert rel_l2(state, expected.reshape(state.shape)) < 1e-9
+    assert rel_l2(expected, plain) > 1e-6
+
+
+def test_split_absorber_graphs_and_direct_launches_agree():
+    """The split collar reads the accepted step from the record inside a graph and from
+    the host when the state is read: both paths, graph and direct, agree to the bit."""
+    import os
+
+    print("split absorber graph vs direct", flush=True)
+    problem = small(length=1e-3, absorber="split_step")
+    results = []
+    for flag in ("1", "0"):
+        os.environ["SOLPROP_MI355X_GRAPH"] = flag
+        ws = Workspace(problem, Settings())
+        session = ws.start(0.0, problem.field0)
+        session.run(problem.z_end)
+        results.append(session.state())
+        ws.close()
+    os.environ.pop("SOLPROP_MI355X_GRAPH", None)
+    assert np.array_equal(results[0], results[1])
 
 
 def test_repeats_agree_to_the_bit():
