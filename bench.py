@@ -66,6 +66,8 @@ def trial_bytes(problem, order):
     n_streams = 3 if plasma else 1
     field = qt if dense else ct + table  # what a node pass reads to know E at its nodes
     raman = problem.raman is not None
+    in_stage = problem.absorber_scheme == "in_stage"
+    spatial = problem.spatial_absorber is not None
     # The doubled-grid drive (Q rows of 2T reals): written, transformed both ways (read and
     # written each time), multiplied by the kernel's spectrum in place.
     qt2 = 16 * Q * T
@@ -85,8 +87,11 @@ def trial_bytes(problem, order):
         if dense else ct + 2 * table + 3 * qt + qk + 2 * 8 * T + 3 * ct + (qt if raman else 0),
         "projection_gemm": table + (2 if plasma else 1) * qt + 2 * 8 * T + n_streams * ct
         if dense else 0,
-        "absorber": 2 * ct + 8 * T if problem.absorber_scheme == "in_stage" else 0,
-        "forward_fft": 2 * (n_streams + (problem.absorber_scheme == "in_stage")) * ct,
+        # The temporal term reads the modal fields and writes the stream (and reads it
+        # back after the collar's GEMM, which reads its matrix and the modal fields).
+        "absorber": (2 * ct + 8 * T + (ct if spatial else 0) if in_stage else 0)
+        + (8 * C * C + 2 * ct if spatial else 0),
+        "forward_fft": 2 * (n_streams + (in_stage or spatial)) * ct,
         "field_finish": n_streams * ct + cn * len(problem.prefactor_paths) + cn,
         "adams_correct": cn * (n_phi + 1 + (W - 1) + 1 + (order + 1 if comp else 0)) + cn,
         "adams_norms": 8 * (1 + 3) * ((C * N + (64 if C * N <= 4096 * 64 else 256) - 1)
@@ -95,12 +100,16 @@ def trial_bytes(problem, order):
 
 
 def trial_flops(problem):
-    """Multiply-add flops of the dense route's two GEMMs (2 per product term)."""
+    """Multiply-add flops of the dense route's GEMMs (2 per product term): the
+    synthesis, the projection and, with a transverse collar, its matrix product."""
     C, T, Q = problem.n_modes, problem.n_to, problem.n_nodes
     if C <= 8:
         return {}
     width = 2 * T if problem.plasma is not None else T
-    return {"synthesis_gemm": 2.0 * Q * C * T, "projection_gemm": 2.0 * C * Q * width}
+    flops = {"synthesis_gemm": 2.0 * Q * C * T, "projection_gemm": 2.0 * C * Q * width}
+    if problem.spatial_absorber is not None:
+        flops["absorber"] = 2.0 * C * C * T
+    return flops
 
 
 def measured_traffic(workload, kernel):
@@ -125,21 +134,25 @@ def workload_capillary(raman_share=0.0):
         "workload": "gas-filled capillary UPPE, 4 HE_1m modes, 125 um bore, 1.5 m helium "
                     "0.8->0 bar, real field 2^14 samples (8193 bins) over 400 fs, 7.5 fs "
                     "800 nm 275 uJ Gaussian, 4x128 radial quadrature, Kerr + tunnelling "
-                    "plasma, exponential Adams (order <= 5) rtol 1e-6",
+                    "plasma (band-limited running integral, "
+                    f"{len(problem.plasma.taps) - 1} taps a side), exponential Adams "
+                    "(order <= 5) rtol 1e-6",
         "n_t": problem.n_t, "n_modes": problem.n_modes, "n_nodes": problem.n_nodes,
         "stepper": "exponential Adams PEC, variable order", "parallelism": "single GPU",
     }
     return problem, config
 
 
-def workload_radial(raman_share=0.0):
+def workload_radial(raman_share=0.0, collar=None):
     from solprop_amd.problem import radial_problem
 
-    problem = radial_problem(raman_share=raman_share)
+    problem = radial_problem(raman_share=raman_share, collar=collar)
     config = {
         "workload": "free-space radial beam in argon, 1 bar, 0.5 m; 100 fs 800 nm Gaussian, "
                     "w0 1 mm, 2 mJ; 512-point Hankel transform (dense 512x512) over 4097 "
-                    "frequency columns (2^13 samples over 1 ps); Kerr + tunnelling plasma; "
+                    "frequency columns (2^13 samples over 1 ps); Kerr + tunnelling plasma "
+                    f"(band-limited running integral, {len(problem.plasma.taps) - 1} taps a "
+                    "side); "
                     "FP64 matrix-core GEMMs; exponential Adams (order <= 5) rtol 1e-6",
         "n_t": problem.n_t, "n_modes": problem.n_modes, "n_nodes": problem.n_nodes,
         "stepper": "exponential Adams PEC, variable order", "parallelism": "single GPU",
@@ -167,8 +180,17 @@ def run_gpu(args):
     if args.workload == "envelope":
         problem, config = workload_envelope()
     else:
-        builder = workload_capillary if args.workload == "capillary" else workload_radial
-        problem, config = builder(args.raman_share)
+        if args.workload == "capillary":
+            problem, config = workload_capillary(args.raman_share)
+        else:
+            collar = None
+            if args.collar:
+                width_mm, alpha0 = (float(v) for v in args.collar.split(","))
+                collar = (width_mm * 1e-3, alpha0)
+            problem, config = workload_radial(args.raman_share, collar)
+            if collar:
+                config["workload"] += (f"; transverse collar over the outer {width_mm:g} mm, "
+                                       f"peak {alpha0:g} / m")
         if args.raman_share > 0.0:
             config["workload"] += (f"; molecular Raman response taking {args.raman_share:g} "
                                    "of the third-order response")
@@ -312,6 +334,9 @@ def main():
     parser.add_argument("--copy-probe-mib", type=int, default=0,
                         help="run a plain device copy of this many MiB first (counter scaling)")
     parser.add_argument("--no-cpu", action="store_true")
+    parser.add_argument("--collar", default="",
+                        help="radial workload: add a transverse collar, 'WIDTH_MM,ALPHA0' "
+                             "(its matrix product runs as a third GEMM)")
     parser.add_argument("--raman-share", type=float, default=0.0,
                         help="add a damped-oscillator Raman response taking this share of "
                              "the third-order response (field workloads; off by default)")

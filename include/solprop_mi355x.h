@@ -124,9 +124,15 @@ typedef struct sp_problem_desc {
    * as a power absorption rate alpha(t) in 1/m: 0 none; 1 in-stage (-alpha E / 2 on the
    * fine time grid, transformed and added to every source on the active band,
    * passes/stepping.py:286-345); 2 split-step (exp(-alpha h / 2) on the coarse time grid
-   * once per accepted step, passes/stepping.py:410-456). Field routes only. */
+   * once per accepted step, passes/stepping.py:410-456). */
   int absorber_scheme;
   const double* absorber_rate;      /* (n_to) in-stage, (n_t) split-step */
+  /* The radial route's transverse collar (Problem.Spatial.absorber_matrix,
+   * native/src/solprop_native/_core.pyi:141): F diag(-alpha(r) / 2) B on the mode axis,
+   * (n_modes, n_modes) row-major, 1/m (src/solprop/backends/spatial.py:189-225); its
+   * product with the modal time field joins the in-stage absorber's term
+   * (passes/stepping.py:286-372). NULL without a collar; dense field route only. */
+  const double* spatial_absorber;
 } sp_problem_desc;
 
 typedef struct sp_workspace sp_workspace;

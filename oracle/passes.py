@@ -157,12 +157,17 @@ def error_norm(error, state, state_new, norm_mask, rtol, atol, weights=None):
     return float(np.linalg.norm(residual))
 
 
-def absorber_term(modal_t, temporal_rate_fine, crop, active_mask, kind: str):
-    """The in-stage absorber ``F[-alpha(t) E(t) / 2]``, cropped and masked to the active
-    band (``passes/stepping.py:286-345``; the term is ``boundaries.py:1159``
-    ``nonlinear_absorber_term``). The radial route's transverse collar matrix is not
-    restated (the problems here carry none)."""
-    term = -0.5 * temporal_rate_fine * modal_t
+def absorber_term(modal_t, temporal_rate_fine, crop, active_mask, kind: str, radial=None):
+    """The in-stage absorbers ``F[-alpha(t) E(t) / 2 + M E]``, cropped and masked to the
+    active band (``passes/stepping.py:286-372``; the temporal term is
+    ``boundaries.py:1159`` ``nonlinear_absorber_term``, ``M`` the radial collar's
+    matrix on the mode axis, ``spatial.py:189-225``). Either part may be absent."""
+    term = np.zeros_like(modal_t)
+    if temporal_rate_fine is not None:
+        term = -0.5 * temporal_rate_fine * modal_t
+    if radial is not None:
+        n_modes = modal_t.shape[0]
+        term = term + (radial @ modal_t.reshape(n_modes, -1)).reshape(modal_t.shape)
     return to_frequency(term, crop, kind) * active_mask
 
 

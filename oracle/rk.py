@@ -125,9 +125,12 @@ class StageSource:
                 gated, edge, p.omega, crop, p.active_mask
             )
         absorber = None
-        if p.absorber_scheme == "in_stage":
-            # ADR-025's in-stage absorber, a term of its own (``source.py:391-401``).
-            absorber = P.absorber_term(modal_t, p.absorber_rate, crop, p.active_mask, kind)
+        if p.absorber_scheme == "in_stage" or p.spatial_absorber is not None:
+            # ADR-025's in-stage absorber, a term of its own (``source.py:391-401``),
+            # with the radial collar's matrix product added to it (``source.py:133``).
+            absorber = P.absorber_term(
+                modal_t, p.absorber_rate if p.absorber_scheme == "in_stage" else None,
+                crop, p.active_mask, kind, radial=p.spatial_absorber)
         return P.stage_output(
             spectrum, self.prefactor(z), p.omega_window, inverse_propagator, absorber
         )

@@ -70,7 +70,7 @@ class ProblemDescC(C.Structure):
         ("method", _I), ("adams_order", _I), ("adams_variable_order", _I),
         ("adams_equal_approach", _I), ("adams_safety", _D), ("adams_max_growth", _D),
         ("adams_min_factor", _D),
-        ("absorber_scheme", _I), ("absorber_rate", _PD),
+        ("absorber_scheme", _I), ("absorber_rate", _PD), ("spatial_absorber", _PD),
     ]
 
 

@@ -852,16 +852,18 @@ __global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R,
 
 // ------------------------------------------------------------ temporal absorber
 
-// The in-stage term -alpha(t) a_c(t) / 2 of every mode, as a stream of its own.
-__global__ void field_absorber_stream(DeviceProblem P, FieldBuffers B) {
+// The in-stage term -alpha(t) a_c(t) / 2 of every mode, as a stream of its own, or added
+// to what the stream holds (the transverse collar's product).
+__global__ void field_absorber_stream(DeviceProblem P, FieldBuffers B, bool accumulate) {
   const int T = P.T;
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= static_cast<long>(P.C) * T) return;
   const int t = static_cast<int>(idx % T);
   const double* modal = reinterpret_cast<const double*>(B.modal_t);
-  double* streams = reinterpret_cast<double*>(B.streams);
-  streams[static_cast<long>(P.absorber_stream) * P.C * T + idx] =
-      (-0.5 * P.absorber_rate[t]) * modal[idx];
+  double* stream = reinterpret_cast<double*>(B.streams) +
+                   static_cast<long>(P.absorber_stream) * P.C * T;
+  const double term = (-0.5 * P.absorber_rate[t]) * modal[idx];
+  stream[idx] = accumulate ? term + stream[idx] : term;
 }
 
 // The split-step factor on the committed state's time samples: exp(-alpha(t) h / 2).
@@ -1146,12 +1148,13 @@ void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const Plas
   }
 }
 
-void launch_field_absorber_stream(const DeviceProblem& P, FieldBuffers B, hipStream_t s) {
+void launch_field_absorber_stream(const DeviceProblem& P, FieldBuffers B, bool accumulate,
+                                  hipStream_t s) {
   SP_CHECK(P.absorber_rate && P.absorber_stream >= 1 && P.absorber_stream <= 3,
            "the in-stage absorber's rate and stream");
   const std::size_t n = static_cast<std::size_t>(P.C) * P.T;
   hipLaunchKernelGGL(field_absorber_stream, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,
-                     s, P, B);
+                     s, P, B, accumulate);
   check();
 }
 

@@ -112,7 +112,8 @@ void launch_field_raman_drive(const DeviceProblem& P, FieldBuffers B, hipStream_
 
 // The in-stage absorber's stream, -alpha(t) a_c(t) / 2 from the modal fields, written as
 // stream P.absorber_stream of B.streams.
-void launch_field_absorber_stream(const DeviceProblem& P, FieldBuffers B, hipStream_t s);
+void launch_field_absorber_stream(const DeviceProblem& P, FieldBuffers B, bool accumulate,
+                                  hipStream_t s);
 // The split-step absorber on the committed state, in three launches around the caller's
 // transforms of B.modal_t: pack y for the real inverse transform; after it, multiply by
 // exp(-alpha(t) h / 2) (h: R->accepted_dz, or `dz` when R is null); after the forward

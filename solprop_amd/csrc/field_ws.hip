@@ -50,6 +50,8 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   SP_CHECK(d.absorber_scheme == 0 || d.absorber_rate, "the absorber's rate is missing");
   SP_CHECK(d.absorber_scheme != 2 || d.n_t == d.n_to,
            "the split-step absorber needs n_to == n_t (its collar acts on the coarse grid)");
+  SP_CHECK(!d.spatial_absorber || d.n_modes > kMaxModes,
+           "the transverse collar's matrix runs on the dense route (more than 8 modes)");
   SP_CHECK(!d.has_plasma || (d.n_taps >= 1 && d.n_taps <= kMaxBandHalf + 1 && d.taps),
            "the running-integral rule needs 1 (the trapezoid) to 9 (the band rule) taps");
   SP_CHECK(d.adams_order >= 1 && d.adams_order + 1 < kMaxAdamsWidth, "Adams order 1..6");
@@ -135,10 +137,14 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   P_.raman_rows = d.n_raman ? upload(d.raman_undamped, d.n_raman * 2UL * T) : nullptr;
   in_stage_ = d.absorber_scheme == 1;
   split_ = d.absorber_scheme == 2;
+  spatial_ = d.spatial_absorber != nullptr;
   P_.absorber_rate = d.absorber_scheme ? upload(d.absorber_rate, static_cast<std::size_t>(T))
                                        : nullptr;
-  // The in-stage absorber's stream follows the polarisation and, with plasma, the currents.
-  P_.absorber_stream = in_stage_ ? (d.has_plasma ? 3 : 1) : -1;
+  spatial_matrix_ =
+      spatial_ ? upload(d.spatial_absorber, static_cast<std::size_t>(C) * C) : nullptr;
+  // The in-stage absorbers' stream (the temporal term, the transverse collar's product,
+  // or their sum) follows the polarisation and, with plasma, the currents.
+  P_.absorber_stream = absorbing() ? (d.has_plasma ? 3 : 1) : -1;
   std::vector<int> crop(N);
   for (int b = 0; b < N; ++b) crop[b] = b;
   P_.crop = upload(crop.data(), crop.size());
@@ -178,7 +184,7 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   B_.source = alloc<cplx>(CN);
   B_.advanced = alloc<cplx>(CN);
   B_.modal_t = alloc<cplx>(static_cast<std::size_t>(C) * (T / 2));
-  const std::size_t n_rows = (in_stage_ && d.has_plasma ? 4UL : 3UL) * C;
+  const std::size_t n_rows = (absorbing() && d.has_plasma ? 4UL : 3UL) * C;
   B_.streams = alloc<cplx>(n_rows * (T / 2));
   // Streams a run never writes (the currents without plasma on the dense route) read as
   // zeros.
@@ -203,11 +209,13 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   SP_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&B_.norms), norms_host_, 0));
   const auto roots = unit_roots(static_cast<std::size_t>(T));
   B_.roots_T = upload(roots.data(), roots.size());
-  scratch_ = alloc<cplx>(3UL * C * (T / 2));
-  inv_plan_.init(T / 2, C);
   // Without plasma only the polarisation stream is transformed (the finish reads the
-  // current streams on plasma runs alone).
-  fwd_plan_.init(T / 2, ((d.has_plasma ? 3 : 1) + (in_stage_ ? 1 : 0)) * C);
+  // current streams on plasma runs alone). A two-pass transform needs scratch for every
+  // row it transforms.
+  const std::size_t forward_rows = ((d.has_plasma ? 3 : 1) + (absorbing() ? 1 : 0)) * C;
+  scratch_ = alloc<cplx>((forward_rows > 3UL * C ? forward_rows : 3UL * C) * (T / 2));
+  inv_plan_.init(T / 2, C);
+  fwd_plan_.init(T / 2, forward_rows);
   if (d.n_raman) {
     // The doubled-grid drive of every node, its transform scratch, the kernel's packed
     // samples and spectrum, and the roots of 2T for the real-transform packing.
@@ -290,6 +298,18 @@ void FieldWorkspace::enqueue_projection() {
                            reinterpret_cast<double*>(B_.streams), stream_);
 }
 
+// The in-stage absorbers' stream: the transverse collar's matrix times the modal time
+// fields (a C x C by C x T GEMM on the matrix cores), then the temporal term
+// -alpha(t) a_c(t) / 2 added to it (or written, without a collar).
+void FieldWorkspace::enqueue_absorbers() {
+  double* stream = reinterpret_cast<double*>(B_.streams) +
+                   static_cast<std::size_t>(P_.absorber_stream) * P_.C * P_.T;
+  if (spatial_)
+    launch_dgemm(P_.C, P_.T, P_.C, spatial_matrix_, P_.C,
+                 reinterpret_cast<const double*>(B_.modal_t), P_.T, stream, P_.T, stream_);
+  if (in_stage_) launch_field_absorber_stream(P_, B_, spatial_, stream_);
+}
+
 // The Raman kernel's spectrum at the record's rates: H = rfft(h) dt on the doubled grid.
 void FieldWorkspace::enqueue_raman_kernel() {
   launch_field_raman_kernel_fill(P_, rec_dev_, raman_kernel_z_, stream_);
@@ -320,7 +340,7 @@ void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_, marking_ ? marks_ + 6 : nullptr);
   enqueue_projection();
   mark(11);
-  if (in_stage_) launch_field_absorber_stream(P_, B_, stream_);
+  enqueue_absorbers();
   mark(12);
   fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
   mark(13);
@@ -423,7 +443,7 @@ bool FieldWorkspace::phase_used(int i) const {
       return dense_;
     case 4: return P_.n_raman > 0;
     case 5: case 6: case 7: case 8: return P_.has_plasma != 0;
-    case 11: return in_stage_;
+    case 11: return absorbing();
     default: return true;
   }
 }
@@ -677,7 +697,7 @@ void FieldWorkspace::source(double z, const double* lab, double* out, double* st
   enqueue_raman();
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
   enqueue_projection();
-  if (in_stage_) launch_field_absorber_stream(P_, B_, stream_);
+  enqueue_absorbers();
   SP_HIP(hipStreamSynchronize(stream_));
   if (modal) SP_HIP(hipMemcpy(modal, B_.modal_t, CT * sizeof(double), hipMemcpyDeviceToHost));
   if (streams)

@@ -98,6 +98,10 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   bool dense_ = false;          // synthesis and projection as GEMMs (more than kMaxModes modes)
   bool in_stage_ = false;       // the in-stage absorber: a stream of its own
   bool split_ = false;          // the split-step absorber: applied at every commit
+  bool spatial_ = false;        // the transverse collar's matrix (dense route)
+  double* spatial_matrix_ = nullptr;  // (C, C) row-major
+  bool absorbing() const { return in_stage_ || spatial_; }
+  void enqueue_absorbers();
   double pending_dz_ = 0.0;     // the accepted step a pending commit finishes
   // The march, and the host it reports to: rows and the attempt limit of one call.
   struct CallHost final : adams::Host {

@@ -205,6 +205,8 @@ def lower(problem: Problem, settings: Settings | None = None) -> tuple[_lib.Prob
     d.absorber_scheme = {"none": 0, "in_stage": 1, "split_step": 2}[problem.absorber_scheme]
     if problem.absorber_rate is not None:
         d.absorber_rate = k.f64(problem.absorber_rate)
+    if problem.spatial_absorber is not None:
+        d.spatial_absorber = k.f64(problem.spatial_absorber)
     return d, k
 
 

@@ -238,6 +238,9 @@ class Problem:
     # grid, n_to) or "split_step" (on the coarse grid, n_t), a power rate in 1/m.
     absorber_scheme: str = "none"
     absorber_rate: np.ndarray | None = None
+    # The radial route's transverse collar (``Problem.Spatial.absorber_matrix``):
+    # F diag(-alpha(r) / 2) B on the mode axis, (C, C) float64, 1/m; None without one.
+    spatial_absorber: np.ndarray | None = None
     description: dict = field(default_factory=dict)
 
     @property
@@ -331,6 +334,12 @@ class Problem:
                                  f"{self.absorber_scheme} scheme")
             if np.any(rate < 0.0) or not np.all(np.isfinite(rate)):
                 raise ValueError("absorber_rate must be finite and non-negative")
+        if self.spatial_absorber is not None:
+            m = self.spatial_absorber
+            if m.shape != (self.n_modes, self.n_modes) or m.dtype != np.float64:
+                raise ValueError("spatial_absorber: expected (n_modes, n_modes) float64")
+            if not np.all(np.isfinite(m)):
+                raise ValueError("spatial_absorber must be finite")
 
 
 # ----------------------------------------------------------------------- builders
@@ -933,6 +942,45 @@ def qdht(radius: float, n_r: int):
             conjugated / forward_scale)
 
 
+def planck_edge(x: np.ndarray, outer: float, inner: float) -> np.ndarray:
+    """One Planck-taper edge strictly between its breakpoints (``grid.py:153-166``):
+    ``1 / (1 + exp(w/(x - outer) + w/(x - inner)))`` with ``w = inner - outer``, the
+    exponent clipped to +-700 (past which the value is already 0 or 1)."""
+    width = inner - outer
+    exponent = width / (x - outer) + width / (x - inner)
+    return 1.0 / (1.0 + np.exp(np.clip(exponent, -700.0, 700.0)))
+
+
+def collar_profile(s: np.ndarray) -> np.ndarray:
+    """The spatial collar's profile ``1 - W`` on its normalised depth ``s``
+    (``spatial_absorber.py:110-142``): 0 at and before ``s = 0``, 1 at and beyond
+    ``s = 1``, the complement of a Planck edge between (every derivative continuous)."""
+    s = np.asarray(s, dtype=np.float64)
+    out = np.where(s >= 1.0, 1.0, 0.0)
+    inside = (s > 0.0) & (s < 1.0)
+    out[inside] = 1.0 - planck_edge(s[inside], 1.0, 0.0)
+    return out
+
+
+def radial_collar_rate(r: np.ndarray, radius: float, width: float, alpha0: float) -> np.ndarray:
+    """A transverse collar's power rate on the radial nodes: ``alpha0 P(s)`` with ``P``
+    the collar profile and ``s`` the depth into the outer ``width`` of the aperture
+    (the rate a ``SpatialAbsorberDesign`` places on the nodes,
+    ``spatial_absorber.py:550-660``; its calibration of ``alpha0`` and the width is the
+    caller's here)."""
+    if not (0.0 < width < radius and np.isfinite(alpha0) and alpha0 >= 0.0):
+        raise ValueError("the collar needs 0 < width < radius and a finite alpha0 >= 0")
+    return alpha0 * collar_profile((np.asarray(r) - (radius - width)) / width)
+
+
+def radial_absorber_matrix(forward: np.ndarray, inverse: np.ndarray, rate: np.ndarray,
+                           pad_scale: float = 1.0) -> np.ndarray:
+    """The collar composed on the mode axis, ``F diag(-alpha/2) B / pad_scale``
+    (``src/solprop/backends/spatial.py:189-225``): the rate is fixed in time and z, so
+    it commutes with the temporal transforms and is formed once."""
+    return np.ascontiguousarray(forward @ ((-0.5 * rate[:, None]) * (inverse / pad_scale)))
+
+
 def radial_problem(
     *,
     n_t: int = 2**13,
@@ -956,6 +1004,7 @@ def radial_problem(
     absorber: str = "none",
     absorber_length: float | None = None,
     integration: str = "band",
+    collar: tuple | None = None,
 ) -> Problem:
     """Free-space radially symmetric beam in argon on a real field grid.
 
@@ -966,7 +1015,9 @@ def radial_problem(
     ``k_z = sqrt(k² - k_r²)`` (evanescent components decay), the prefactor
     ``-i ω² / (2 ε0 c² k_z)`` on propagating components. Kerr from argon's n2 and
     tunnelling ionisation (ADK, argon's potential); argon, an atom, has no rotational
-    Raman response. Constant density, so one operator term.
+    Raman response. Constant density, so one operator term. ``collar = (width, alpha0)``
+    adds a transverse collar over the aperture's outer ``width`` (m) with peak power rate
+    ``alpha0`` (1/m), as the matrix ``F diag(-alpha(r)/2) B`` on the mode axis.
     """
     n_to = n_t
     dt = window / n_t
@@ -1060,6 +1111,8 @@ def radial_problem(
         absorber_scheme=absorber,
         absorber_rate=None if absorber == "none" else temporal_absorber_rate(
             g, absorber_length if absorber_length else length / 20),
+        spatial_absorber=None if collar is None else radial_absorber_matrix(
+            forward, inverse, radial_collar_rate(r, radius, collar[0], collar[1])),
         description={
             "route": "free-space radial beam, QDHT field grid",
             "n_t": n_t, "n_omega": n, "window_s": window, "n_r": n_r, "radius_m": radius,

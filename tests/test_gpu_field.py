@@ -152,10 +152,13 @@ def test_field_graphs_and_direct_launches_agree():
 
 
 
-def test_first_trial_matches_oracle_at_full_size():
-    """The measured workload itself: 4 modes, 128 nodes, 2^14 samples over 400 fs."""
-    print("field probe trial at full size", flush=True)
-    problem = capillary_problem()
+@pytest.mark.parametrize("absorber", ["none", "in_stage"])
+def test_first_trial_matches_oracle_at_full_size(absorber):
+    """The measured workload itself: 4 modes, 128 nodes, 2^14 samples over 400 fs; and
+    with the in-stage absorber, whose stream makes the forward transform four rows a
+    mode, each of 8192 points (two passes, through scratch sized for all of them)."""
+    print(f"field probe trial at full size, absorber {absorber}", flush=True)
+    problem = capillary_problem(absorber=absorber)
     dz = 1e-4
     advanced, norms = oracle_first_trial(problem, dz)
     ws = Workspace(problem)

@@ -34,6 +34,12 @@ CASES = {
     # kernel is built once).
     "raman": {"w0": 0.25e-3, "raman_share": 0.5},
     "band_rule": {"w0": 0.25e-3, "integration": "band", "band": (400e-9, 4e-6)},
+    # A transverse collar over the aperture's outer millimetre (peak 2000 / m) on the
+    # wide beam, which reaches it; alone, and with the in-stage temporal absorber added
+    # to the same stream.
+    "collar": {"w0": 1.0e-3, "collar": (1.0e-3, 2.0e3)},
+    "collar_in_stage": {"w0": 1.0e-3, "collar": (1.0e-3, 2.0e3), "absorber": "in_stage",
+                        "absorber_length": 2e-2},
 }
 
 
@@ -98,7 +104,8 @@ def test_radial_first_trial_matches_oracle(case):
     assert abs(got - norms[0]) / norms[0] < 1e-8
 
 
-@pytest.mark.parametrize("case", ["wide_beam", "narrow_beam", "raman", "band_rule"])
+@pytest.mark.parametrize("case", ["wide_beam", "narrow_beam", "raman", "band_rule", "collar",
+                                  "collar_in_stage"])
 def test_radial_march_matches_oracle_decisions(case):
     print(f"radial march {case}", flush=True)
     problem = small(**CASES[case])
@@ -115,6 +122,23 @@ def test_radial_march_matches_oracle_decisions(case):
     steps = np.array([r[1] for r in records])
     assert np.allclose(rows[:, 1], steps, rtol=1e-7, atol=0)
     assert rel_l2(state, expected.reshape(state.shape)) < 1e-8
+
+
+def test_collar_changes_the_march():
+    """The transverse collar acts: a short march with it ends measurably away from the
+    same march without it (the oracle's, which the device follows above)."""
+    print("radial collar acts", flush=True)
+    target = 2e-3
+    ends = []
+    for case in ("collar", "wide_beam"):
+        problem = small(**CASES[case])
+        ws = Workspace(problem)
+        session = ws.start(0.0, problem.field0)
+        batch = session.advance(target, work_limit=200)
+        assert batch.done
+        ends.append(session.state())
+        ws.close()
+    assert rel_l2(ends[0], ends[1]) > 1e-8
 
 
 def test_radial_repeats_agree_to_the_bit():

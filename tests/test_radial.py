@@ -79,3 +79,60 @@ def test_radial_record_uses_the_transform_pair():
     assert not np.any(problem.prefactor_terms[..., ~problem.active_mask])
     # Evanescent components decay: Re Φ <= 0 everywhere.
     assert np.all(problem.phi_terms.real <= 0)
+
+
+# ---------------------------------------------------------------- transverse collar
+
+
+def test_collar_profile_reference_values_and_symmetry():
+    """The reference's docstring values (``spatial_absorber.py:128-131``), the Planck
+    edge's point symmetry P(s) + P(1 - s) = 1 about the midpoint, and a profile rising
+    monotonically from 0 to 1 (a flipped exponent falls instead)."""
+    from solprop_amd.problem import collar_profile
+
+    assert collar_profile(np.array([-1.0, 0.0, 0.5, 1.0, 2.0])).tolist() == [
+        0.0, 0.0, 0.5, 1.0, 1.0]
+    s = np.linspace(0.001, 0.999, 999)
+    p = collar_profile(s)
+    assert np.abs(p + collar_profile(1.0 - s) - 1.0).max() < 1e-15
+    assert np.all(np.diff(p) >= 0.0)
+    middle = collar_profile(np.linspace(0.1, 0.9, 81))
+    assert np.all(np.diff(middle) > 0.0)
+    # Every derivative continuous: the profile leaves zero flatter than any power.
+    assert collar_profile(np.array([0.01]))[0] < 1e-40
+
+
+def test_collar_matrix_is_the_real_space_product():
+    """``F diag(-alpha/2) B`` applied to the Hankel spectrum of a smooth profile equals
+    the spectrum of ``-alpha(r) f(r) / 2`` (the rate acts pointwise at the nodes); a
+    constant rate gives ``-alpha/2`` times the identity to the transform's
+    near-orthogonality; the rate on the wrong axis, or without its half, fails."""
+    from solprop_amd.problem import radial_absorber_matrix, radial_collar_rate
+
+    r, k, forward, inverse = qdht(RADIUS, N_R)
+    rate = radial_collar_rate(r, RADIUS, 1.5e-3, 800.0)
+    # A 1.5 mm Gaussian: e^-7 at the aperture, so the collar has content to act on (the
+    # identity rests on B F = I, not on the profile vanishing at the edge).
+    f = np.exp(-((r / 1.5e-3) ** 2)) * (1.0 + r / RADIUS)
+    spectrum = forward @ f
+    m = radial_absorber_matrix(forward, inverse, rate)
+    exact = forward @ (-0.5 * rate * f)
+    assert np.abs(m @ spectrum - exact).max() < 1e-10 * np.abs(exact).max()
+    constant = radial_absorber_matrix(forward, inverse, np.full(N_R, 800.0))
+    assert np.abs(constant + 400.0 * np.eye(N_R)).max() < 400.0 * 1e-9
+    # Plausible mistakes: the rate applied on the spectral axis, or without the half.
+    spectral = (-0.5 * rate)[:, None] * np.eye(N_R)
+    assert np.abs(spectral @ spectrum - exact).max() > 1e-3 * np.abs(exact).max()
+    no_half = radial_absorber_matrix(forward, inverse, 2.0 * rate)
+    assert np.abs(no_half @ spectrum - exact).max() > 0.5 * np.abs(exact).max()
+
+
+def test_radial_record_carries_the_collar():
+    problem = radial_problem(n_t=1024, window=0.25e-12, n_r=32, length=0.02, fwhm=30e-15,
+                             integration="trapezoid", collar=(1e-3, 2e3))
+    problem.validate()
+    assert problem.spatial_absorber.shape == (32, 32)
+    # The rate is non-negative, so the matrix only takes energy: its symmetric part,
+    # in the transform's weighted inner product, is negative semidefinite; here its
+    # diagonal is non-positive.
+    assert np.all(np.diag(problem.spatial_absorber) <= 1e-12)
