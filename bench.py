@@ -78,9 +78,9 @@ def trial_bytes(problem, order):
         "inverse_fft": 2 * ct,
         "synthesis_gemm": table + ct + qt if dense else 0,
         "raman": field + qt2 * 7 if raman else 0,
-        "plasma_rate_tiles": field + qt + qk + (qt if band else 0),
+        "plasma_rate": field + qt + qk + (qt if band else 0),
         "plasma_tile_prefix": 2 * qk,
-        "plasma_current_tiles": field + qt + 2 * qk + 2 * qt,
+        "plasma_current": field + qt + 2 * qk + 2 * qt,
         "plasma_tile_prefix_current": 2 * qk,
         "node_response": (qt + (3 * qt + qk if plasma else 0) + (2 if plasma else 1) * qt)
         + (qt if raman else 0)
@@ -118,12 +118,10 @@ def measured_traffic(workload, kernel):
     no run covers this workload's dominant kernel."""
     path = Path(__file__).resolve().parent / "profiles" / "traffic.json"
     try:
-        row = json.loads(path.read_text()).get(workload)
-    except (OSError, ValueError):
+        row = json.loads(path.read_text()).get(workload, {}).get(kernel)
+    except (OSError, ValueError, AttributeError):
         return None
-    if not row or row.get("kernel") != kernel:
-        return None
-    return row["bytes"]
+    return None if not row else row["bytes"]
 
 
 def workload_capillary(raman_share=0.0):

@@ -272,8 +272,8 @@ namespace {
 // The kernels (or transform plans) of one trial, in launch order.
 constexpr const char* kPhaseNames[FieldWorkspace::kPhases] = {
     "adams_predict",      "field_scatter",        "inverse_fft",
-    "synthesis_gemm",     "raman",                "plasma_rate_tiles",    "plasma_tile_prefix",
-    "plasma_current_tiles", "plasma_tile_prefix_current", "node_response",
+    "synthesis_gemm",     "raman",                "plasma_rate",          "plasma_tile_prefix",
+    "plasma_current",       "plasma_tile_prefix_current", "node_response",
     "projection_gemm",    "absorber",             "forward_fft",          "field_finish",
     "adams_correct",      "adams_norms"};
 }  // namespace
