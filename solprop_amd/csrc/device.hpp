@@ -45,6 +45,9 @@ struct DeviceProblem {
   int has_kerr, kerr_real, has_plasma, noise_modified;
   double kerr_coefficient, kerr_self, kerr_conjugate;
   double raman_fraction, raman_epsilon0;
+  // The field routes' Raman drive: 0 the real square E^2 (the harmonic kept), 1 half the
+  // analytic signal's squared modulus, (E^2 + H[E]^2) / 2 (a grid without the harmonic).
+  int raman_analytic;
   double plasma_cmr, plasma_ip;  // e^2/m_e and the ionisation potential, J
   double dt_fine, inv_T, inv_2T;
   const int* fine_to_coarse;   // (NF) coarse bin of each fine bin, -1 in the gaps

@@ -324,6 +324,10 @@ __global__ void __launch_bounds__(kWave)
 
 // The lab-frame modal spectrum (1/T folded in) packed for the real inverse transform
 // of T points as a complex one of T/2.
+// With Hilbert set, the spectrum of the Hilbert transform instead: -i Y strictly
+// between DC and Nyquist, zero at both (fourier.py:243-300 keeps those two bins in the
+// analytic signal's real part only).
+template <bool Hilbert>
 __global__ void field_scatter(DeviceProblem P, const cplx* spectrum, cplx* modal_t,
                               const cplx* roots_T) {
   const int half = P.T / 2;
@@ -339,6 +343,10 @@ __global__ void field_scatter(DeviceProblem P, const cplx* spectrum, cplx* modal
   if (k == 0) {
     xa.y = 0.0;
     xb.y = 0.0;
+  }
+  if constexpr (Hilbert) {
+    xa = k == 0 ? make(0.0, 0.0) : rot(xa, -1);
+    xb = k == 0 ? make(0.0, 0.0) : rot(xb, -1);
   }
   modal_t[idx] = c2r_pre(xa, xb, roots_T[k]);
 }
@@ -789,9 +797,26 @@ __global__ void field_raman_drive(DeviceProblem P, FieldBuffers B) {
     const int t = 2 * m + j;
     double am[held(C)];
     load_modal<C>(B, T, t, am);
-    e[j] = field_at<C>(P, B, am, q, t);
+    const double field = field_at<C>(P, B, am, q, t);
+    if (!P.raman_analytic) {
+      e[j] = field * field;
+      continue;
+    }
+    // (E^2 + H[E]^2) / 2, H[E] synthesised from the modes' Hilbert transforms as E is
+    // from their fields (the transform is linear), or read on the dense route.
+    double hilbert;
+    if constexpr (C > 0) {
+      const double* h = reinterpret_cast<const double*>(B.modal_h);
+      double hm[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) hm[c] = h[static_cast<long>(c) * T + t];
+      hilbert = synthesise_at<C>(P.synthesis + static_cast<long>(q) * C, hm);
+    } else {
+      hilbert = B.node_hilbert[static_cast<long>(q) * T + t];
+    }
+    e[j] = 0.5 * (field * field + hilbert * hilbert);
   }
-  B.raman_drive[idx] = make(e[0] * e[0], e[1] * e[1]);
+  B.raman_drive[idx] = make(e[0], e[1]);
 }
 
 // One coarse bin b of mode c from its two packed entries per stream (za at b, zb at its
@@ -1071,8 +1096,17 @@ void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuf
 void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuffers B,
                           hipStream_t s) {
   const std::size_t n = static_cast<std::size_t>(P.C) * (P.T / 2);
-  hipLaunchKernelGGL(field_scatter, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, P,
-                     spectrum, B.modal_t, B.roots_T);
+  hipLaunchKernelGGL(field_scatter<false>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                     P, spectrum, B.modal_t, B.roots_T);
+  check();
+}
+
+void launch_field_scatter_hilbert(const DeviceProblem& P, const cplx* spectrum, FieldBuffers B,
+                                  hipStream_t s) {
+  SP_CHECK(B.modal_h, "the Hilbert transforms' buffer");
+  const std::size_t n = static_cast<std::size_t>(P.C) * (P.T / 2);
+  hipLaunchKernelGGL(field_scatter<true>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                     P, spectrum, B.modal_h, B.roots_T);
   check();
 }
 

@@ -79,6 +79,10 @@ struct FieldBuffers {
                          // its convolution with the kernel (times 2T)
   cplx* raman_H;         // (T + 1) the kernel's spectrum on the doubled grid, dt folded in
   const cplx* roots_2T;  // unit roots of 2T
+  cplx* modal_h;         // (C, T/2) packed Hilbert transforms of the modal fields
+                         // (the analytic Raman drive only)
+  double* node_hilbert;  // (Q, T) Hilbert transform of the node field (dense route,
+                         // analytic Raman drive)
   double* node_field;    // (Q, T) field at the nodes (dense route only)
   double* node_streams;  // (Q, 2, T) un-gated polarisation and current (dense route only)
   double* partials; // (1 + kMaxScored, blocks) masked squared sums of each block
@@ -129,6 +133,11 @@ void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuf
                           int width, hipStream_t s);
 void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuffers B,
                           hipStream_t s);
+// The modal spectra's Hilbert transforms packed the same way into B.modal_h: -i Y on the
+// bins strictly between DC and Nyquist, zero at both (the analytic signal's imaginary
+// part, src/solprop/fourier.py:243-300).
+void launch_field_scatter_hilbert(const DeviceProblem& P, const cplx* spectrum, FieldBuffers B,
+                                  hipStream_t s);
 // The dense route (B.node_field set) expects the node field written; it leaves the
 // node streams for the projection GEMM. `marks`, when given, holds five events recorded after the rate and exposure tiles,
 // their prefix, the free-current tiles, their prefix and the node response (a run

@@ -41,9 +41,6 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
   SP_CHECK(d.n_phi >= 1 && d.n_phi <= kMaxTerms, "1..4 phi terms");
   SP_CHECK(d.n_prefactor >= 1 && d.n_prefactor <= kMaxTerms, "1..4 prefactor terms");
   SP_CHECK(d.n_raman >= 0 && d.n_raman <= kMaxTerms, "0..4 Raman rows");
-  SP_CHECK(d.n_raman == 0 || d.raman_thg != 0,
-           "the field route's Raman drive is the real square E^2; the analytic-signal "
-           "drive (a grid without the harmonic) is not built");
   SP_CHECK(d.n_raman == 0 || d.raman_undamped, "the Raman rows are missing");
   SP_CHECK(d.absorber_scheme >= 0 && d.absorber_scheme <= 2,
            "absorber scheme 0 (none), 1 (in-stage) or 2 (split-step)");
@@ -135,6 +132,7 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   P_.raman_fraction = d.raman_fraction;
   P_.raman_epsilon0 = d.raman_epsilon0;
   P_.raman_rows = d.n_raman ? upload(d.raman_undamped, d.n_raman * 2UL * T) : nullptr;
+  P_.raman_analytic = d.n_raman && !d.raman_thg ? 1 : 0;
   in_stage_ = d.absorber_scheme == 1;
   split_ = d.absorber_scheme == 2;
   spatial_ = d.spatial_absorber != nullptr;
@@ -220,6 +218,10 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
     // The doubled-grid drive of every node, its transform scratch, the kernel's packed
     // samples and spectrum, and the roots of 2T for the real-transform packing.
     B_.raman_drive = alloc<cplx>(static_cast<std::size_t>(Q) * T);
+    if (P_.raman_analytic) {
+      B_.modal_h = alloc<cplx>(static_cast<std::size_t>(C) * (T / 2));
+      if (dense_) B_.node_hilbert = alloc<double>(static_cast<std::size_t>(Q) * T);
+    }
     raman_scratch_ = alloc<cplx>(static_cast<std::size_t>(Q) * T);
     raman_kernel_z_ = alloc<cplx>(T);
     B_.raman_H = alloc<cplx>(T + 1UL);
@@ -319,9 +321,19 @@ void FieldWorkspace::enqueue_raman_kernel() {
 
 // The Raman convolution of every node's drive: E² on the doubled grid, transformed,
 // times the kernel's spectrum, transformed back (the node passes read it).
-void FieldWorkspace::enqueue_raman() {
+void FieldWorkspace::enqueue_raman(const cplx* spectrum) {
   if (!P_.n_raman) return;
   if (raman_dynamic_) enqueue_raman_kernel();
+  if (P_.raman_analytic) {
+    // The modes' Hilbert transforms (a real inverse transform of -i Y), synthesised at
+    // the nodes by the drive pass, or by a GEMM on the dense route.
+    launch_field_scatter_hilbert(P_, spectrum, B_, stream_);
+    inv_plan_.exec(B_.modal_h, scratch_, +1, stream_);
+    if (dense_)
+      launch_dgemm(P_.Q, P_.T, P_.C, P_.synthesis, P_.C,
+                   reinterpret_cast<const double*>(B_.modal_h), P_.T, B_.node_hilbert, P_.T,
+                   stream_);
+  }
   launch_field_raman_drive(P_, B_, stream_);
   raman_plan_.exec(B_.raman_drive, raman_scratch_, -1, stream_);
   launch_raman_product(B_.raman_drive, B_.raman_H, B_.roots_2T, P_.T, P_.Q, stream_);
@@ -335,7 +347,7 @@ void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
   mark(3);
   enqueue_synthesis();
   mark(4);
-  enqueue_raman();
+  enqueue_raman(spectrum);
   mark(5);
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_, marking_ ? marks_ + 6 : nullptr);
   enqueue_projection();
@@ -694,7 +706,7 @@ void FieldWorkspace::source(double z, const double* lab, double* out, double* st
   launch_field_scatter(P_, B_.lab, B_, stream_);
   inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
   enqueue_synthesis();
-  enqueue_raman();
+  enqueue_raman(B_.lab);
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
   enqueue_projection();
   enqueue_absorbers();

@@ -61,7 +61,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   void enqueue_projection();
   void enqueue_raman_kernel();
   void enqueue_split(const AdamsRecord* R, double dz);
-  void enqueue_raman();
+  void enqueue_raman(const cplx* spectrum);
   enum Graph { kTrial = 1, kStart = 2, kCommitTrial = 3 };
   void enqueue_trial(bool commit);
   void enqueue_start();

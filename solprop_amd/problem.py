@@ -690,7 +690,7 @@ def temporal_absorber_rate(profile: np.ndarray, reference_length: float) -> np.n
 
 
 def oscillator_raman(n_to: int, dt: float, per_molecule: float, tau1: float,
-                     rate: "ScalarPath") -> RamanGroup:
+                     rate: "ScalarPath", drive: str = "real") -> RamanGroup:
     """A molecular Raman response on a real field grid: one damped-oscillator row.
 
     ``h(t) = c (ω² + Γ²)/ω e^{-Γt} sin(ωt)`` with ``ω = 1/tau1`` and the decay ``Γ`` read
@@ -698,8 +698,12 @@ def oscillator_raman(n_to: int, dt: float, per_molecule: float, tau1: float,
     the doubled grid, so ``∫h dt = c`` with ``c = per_molecule`` (the response per
     molecule, in the units of the Kerr coefficient). The drive is the real square ``E²``
     (the field grid keeps the harmonic): ``P = ε0 ρ E (h * E²)``
-    (``passes/responses.py:395-499``).
+    (``passes/responses.py:395-499``); ``drive="analytic"`` takes half the analytic
+    signal's squared modulus instead, ``|E_a|² / 2`` (a grid without the harmonic,
+    ``raman.py:929-990``).
     """
+    if drive not in ("real", "analytic"):
+        raise ValueError(f"unknown Raman drive {drive!r}")
     delays = np.arange(2 * n_to) * dt
     frequency = 1.0 / tau1
     decay = rate(0.0)
@@ -711,7 +715,7 @@ def oscillator_raman(n_to: int, dt: float, per_molecule: float, tau1: float,
         fraction=per_molecule / EPS0,
         epsilon0=EPS0,
         rates=(rate,),
-        thg=True,
+        thg=drive == "real",
     )
 
 
@@ -737,6 +741,7 @@ def capillary_problem(
     time_collar: float = 0.06,
     raman_share: float = 0.0,
     raman_tau: tuple = (60e-15, 200e-15),
+    raman_drive: str = "real",
     absorber: str = "none",
     absorber_length: float | None = None,
     integration: str = "band",
@@ -856,7 +861,7 @@ def capillary_problem(
 
         raman_group = oscillator_raman(
             n_to, dt, EPS0 * chi3_per_rho * raman_share, tau1,
-            ScalarPath.chebyshev(decay, 0.0, length, degree=24, edges=edges))
+            ScalarPath.chebyshev(decay, 0.0, length, degree=24, edges=edges), raman_drive)
     plasma_group = None
     if plasma:
         # Below 3 GV/m helium's rate is under 1e-150 /s: the table is exactly zero there.
@@ -1001,6 +1006,7 @@ def radial_problem(
     time_collar: float = 0.05,
     raman_share: float = 0.0,
     raman_tau: tuple = (60e-15, 200e-15),
+    raman_drive: str = "real",
     absorber: str = "none",
     absorber_length: float | None = None,
     integration: str = "band",
@@ -1071,7 +1077,8 @@ def radial_problem(
         # A molecular response carrying `raman_share` of the third-order response (argon
         # itself, an atom, has none: this is for molecular gases and the tests).
         raman_group = oscillator_raman(n_to, dt, EPS0 * chi3 / density * raman_share,
-                                       raman_tau[0], ScalarPath.constant(1.0 / raman_tau[1]))
+                                       raman_tau[0], ScalarPath.constant(1.0 / raman_tau[1]),
+                                       raman_drive)
     plasma_group = None
     if plasma:
         plasma_group = with_integration(

@@ -42,6 +42,8 @@ CASES = {
     # following the pressure (the kernel is rebuilt at every source position).
     "kerr_raman": {"raman_share": 0.5},
     # The temporal collar as an absorbing rate, in every source or once per step.
+    # The molecular response driven by half the analytic signal's squared modulus.
+    "kerr_raman_analytic": {"raman_share": 0.5, "raman_drive": "analytic"},
     "in_stage_absorber": {"absorber": "in_stage", "absorber_length": 2e-2},
     "split_absorber": {"absorber": "split_step", "absorber_length": 2e-2},
     # The band running-integral rule (the reference's default), 5 taps a side here.
@@ -98,6 +100,7 @@ def test_first_trial_matches_oracle(case):
 
 @pytest.mark.parametrize("case,proposal", [("kerr_plasma", None), ("constant_pressure", None),
                                            ("kerr_plasma", 2e-3), ("kerr_raman", None),
+                                           ("kerr_raman_analytic", None),
                                            ("in_stage_absorber", None),
                                            ("split_absorber", None), ("band_rule", None)])
 def test_march_matches_oracle_decisions(case, proposal):

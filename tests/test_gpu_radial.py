@@ -33,6 +33,7 @@ CASES = {
     # A molecular response taking half the third-order response (constant rate: the
     # kernel is built once).
     "raman": {"w0": 0.25e-3, "raman_share": 0.5},
+    "raman_analytic": {"w0": 0.25e-3, "raman_share": 0.5, "raman_drive": "analytic"},
     "band_rule": {"w0": 0.25e-3, "integration": "band", "band": (400e-9, 4e-6)},
     # A transverse collar over the aperture's outer millimetre (peak 2000 / m) on the
     # wide beam, which reaches it; alone, and with the in-stage temporal absorber added
@@ -104,8 +105,8 @@ def test_radial_first_trial_matches_oracle(case):
     assert abs(got - norms[0]) / norms[0] < 1e-8
 
 
-@pytest.mark.parametrize("case", ["wide_beam", "narrow_beam", "raman", "band_rule", "collar",
-                                  "collar_in_stage"])
+@pytest.mark.parametrize("case", ["wide_beam", "narrow_beam", "raman", "raman_analytic",
+                                  "band_rule", "collar", "collar_in_stage"])
 def test_radial_march_matches_oracle_decisions(case):
     print(f"radial march {case}", flush=True)
     problem = small(**CASES[case])

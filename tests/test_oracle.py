@@ -361,3 +361,25 @@ def test_band_rule_integrates_a_band_limited_signal():
     assert np.abs(got_span - exact_span).max() < 3e-5
     plain = P.RunningIntegral((1.0,)).integrate(np.cos(omega * t), dt)
     assert np.abs(plain[inner] - plain[50] - exact_span).max() > 1e-2
+
+
+def test_analytic_raman_drive_is_the_hilbert_pair():
+    """Half the analytic signal's squared modulus (``raman_drive`` without the harmonic,
+    ``nonlinear/raman.py:929-990``) equals (E^2 + H[E]^2)/2 with the Hilbert transform
+    formed independently, as the real inverse transform of -i Y strictly between DC and
+    Nyquist; the drive with the harmonic (E^2) does not."""
+    from oracle import passes as P
+
+    rng = np.random.default_rng(5)
+    n = 256
+    spectrum = rng.normal(size=n // 2 + 1) + 1j * rng.normal(size=n // 2 + 1)
+    spectrum[0] = spectrum[0].real
+    spectrum[-1] = spectrum[-1].real
+    field = np.fft.irfft(spectrum, n=n)
+    rotated = -1j * spectrum
+    rotated[0] = rotated[-1] = 0.0
+    hilbert = np.fft.irfft(rotated, n=n)
+    expected = 0.5 * (field**2 + hilbert**2)
+    got = P.raman_drive(field, "field", False)
+    assert np.abs(got - expected).max() < 1e-13 * expected.max()
+    assert np.abs(P.raman_drive(field, "field", True) - expected).max() > 0.1 * expected.max()

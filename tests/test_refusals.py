@@ -53,13 +53,6 @@ def test_envelope_with_several_modes_is_refused():
         Workspace(two)
 
 
-def test_field_raman_without_the_harmonic_is_refused():
-    """The analytic-signal drive (|E_a|²/2, a grid without the harmonic) is not built."""
-    p = capillary_problem(n_t=256, n_nodes=4, raman_share=0.5, integration="trapezoid")
-    refused(dataclasses.replace(p, raman=dataclasses.replace(p.raman, thg=False)),
-            "analytic-signal")
-
-
 def test_transverse_collar_off_the_dense_route_is_refused():
     p = capillary_problem(n_t=256, n_nodes=4, integration="trapezoid")
     refused(dataclasses.replace(p, spatial_absorber=np.zeros((p.n_modes, p.n_modes))),
