@@ -184,6 +184,20 @@ int sp_workspace_advance(sp_workspace* ws, double target, long work_limit, doubl
 int sp_workspace_export(sp_workspace* ws, const double* power_weight, int optics,
                         double* out, char* msg, size_t msglen);
 
+/* Plane statistics of the committed state (`GeneralState::statistics`,
+ * native/src/general/statistics.cpp:106-360; the definitions of
+ * src/solprop/stats/definitions.py:268-520). `flags` are StatisticsFlag's bits
+ * (native/include/solprop_native/problem/problem.hpp:44-60); built: omega0 (1 << 0),
+ * energy (1 << 1), energy_window (1 << 2), peak_power (1 << 3) and peak_power_total
+ * (1 << 5), on the modal power weight of one. `windows` holds `n_windows` (lo, hi)
+ * pairs in rad/s, gated lo <= omega < hi. `out` receives, in this order and for the
+ * flags set: omega0 (C; NaN for an empty column), energy (C), energy_window
+ * (n_windows x C), peak_power (C), peak_power_total (1); `count` the number written.
+ * Any other flag is refused with SP_VALUE. */
+int sp_workspace_statistics(sp_workspace* ws, unsigned long long flags, const double* windows,
+                            int n_windows, double* out, int capacity, int* count, char* msg,
+                            size_t msglen);
+
 /* Counters (`Workspace.counters()`): trials, evaluations, accepted steps, graph
  * replays, device bytes held. */
 typedef struct sp_counters {

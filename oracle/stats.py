@@ -1,0 +1,72 @@
+"""Restated plane statistics -- TEST INFRASTRUCTURE.
+
+The saved-plane statistics of ``src/solprop/stats/definitions.py`` on a stored spectrum
+of shape ``(n_modes, n_omega)`` (one polarisation), unit modal power weight:
+
+- :func:`energy` restates ``energy`` (``:363-397``) through ``fourier.energy_omega``
+  (``src/solprop/fourier.py:805-856``): ``(dt / n_t) sum_k w_k |X_k|^2`` with ``w`` one on
+  the envelope grid and ``[1, 4, ..., 4, 1]`` on the field grid.
+- :func:`omega0` restates ``omega0`` (``:268-361``): the centroid of ``|X|^2`` over every
+  bin, NaN for a column at most one eps of the largest column's total.
+- :func:`energy_window` restates ``energy_window`` (``:399-441``) and ``_window_masks``
+  (``:249-266``): hard half-open gates ``lo <= omega < hi``.
+- :func:`peak_power` and :func:`peak_power_total` restate ``peak_power`` (``:443-477``)
+  and ``peak_power_total`` (``:479-520``) over ``analytic_field`` (``:148-178``): the
+  coarse inverse transform, then on a field grid ``fourier.analytic_signal``.
+
+Pins (``tests/test_oracle.py``): Parseval's identity with the analytic signal's time
+sum, the window tiling summing to the total, the centroid of a single bin, a sampled
+envelope's peak, and an empty column's NaN.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import passes as P
+
+
+def _n_t(n_omega: int, kind: str) -> int:
+    return 2 * (n_omega - 1) if kind == "field" else n_omega
+
+
+def energy(field: np.ndarray, dt: float, kind: str) -> np.ndarray:
+    power = np.abs(field) ** 2
+    n_t = _n_t(field.shape[-1], kind)
+    if kind == "envelope":
+        return (dt / n_t) * np.sum(power, axis=-1)
+    interior = 4.0 * np.sum(power[..., 1:-1], axis=-1)
+    return (dt / n_t) * (power[..., 0] + interior + power[..., -1])
+
+
+def omega0(field: np.ndarray, omega: np.ndarray) -> np.ndarray:
+    density = np.abs(field) ** 2
+    total = np.sum(density, axis=-1)
+    weighted = np.sum(density * omega, axis=-1)
+    centroid = np.full(total.shape, np.nan)
+    floor = float(np.finfo(float).eps) * float(np.max(total, initial=0.0))
+    np.divide(weighted, total, out=centroid, where=total > floor)
+    return centroid
+
+
+def energy_window(field, omega, windows, dt, kind):
+    windows = np.asarray(windows, dtype=float).reshape(-1, 2)
+    masks = (omega[None, :] >= windows[:, :1]) & (omega[None, :] < windows[:, 1:])
+    return energy(field[None, ...] * masks[:, None, :], dt, kind)
+
+
+def analytic_field(field: np.ndarray, kind: str) -> np.ndarray:
+    samples = P.inverse(field, kind)
+    return P.analytic_signal(samples) if kind == "field" else samples
+
+
+def peak_power(field: np.ndarray, kind: str) -> np.ndarray:
+    return np.max(np.abs(analytic_field(field, kind)) ** 2, axis=-1)
+
+
+def peak_power_total(field: np.ndarray, kind: str) -> float:
+    return float(np.max(np.sum(np.abs(analytic_field(field, kind)) ** 2, axis=0)))
+
+
+__all__ = ["analytic_field", "energy", "energy_window", "omega0", "peak_power",
+           "peak_power_total"]

@@ -116,6 +116,18 @@ int sp_workspace_export(sp_workspace* ws, const double* power_weight, int optics
   return guarded(msg, msglen, [&] { ws->impl.export_state(power_weight, optics != 0, out); });
 }
 
+int sp_workspace_statistics(sp_workspace* ws, unsigned long long flags, const double* windows,
+                            int n_windows, double* out, int capacity, int* count, char* msg,
+                            size_t msglen) {
+  if (!ws || !count || (capacity > 0 && !out)) return SP_VALUE;
+  return guarded(msg, msglen, [&] {
+    const auto values = ws->impl.statistics(flags, windows, n_windows);
+    SP_CHECK(static_cast<long>(values.size()) <= capacity, "the statistics output is too small");
+    std::copy(values.begin(), values.end(), out);
+    *count = static_cast<int>(values.size());
+  });
+}
+
 int sp_workspace_counters(sp_workspace* ws, sp_counters* out) {
   if (!ws || !out) return SP_VALUE;
   *out = ws->impl.counters();

@@ -150,6 +150,7 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   P_.normmask = upload(d.norm_mask, N);
   P_.omega_window = upload(d.omega_window, N);
   P_.omega = upload(d.omega, N);
+  n_t_ = d.n_t;
   P_.t_window = upload(d.time_window, T);
   P_.t_deriv = upload(d.time_window_derivative, T);
   const std::size_t CN = static_cast<std::size_t>(C) * N;
@@ -755,6 +756,17 @@ long FieldWorkspace::buffer(const char* name, double* out, long count) {
   throw std::invalid_argument(std::string("no device buffer named ") + name);
 }
 
+std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double* windows,
+                                               int n_windows) {
+  if (!loaded_) throw Error(kState, "no state is loaded");
+  flush();
+  // The coarse grid: n_t samples spaced by the fine spacing times n_to / n_t.
+  if (!stats_)
+    stats_ = std::make_unique<PlaneStatistics>(P_.C, P_.N, n_t_, true,
+                                               P_.dt_fine * P_.T / n_t_, P_.omega);
+  return stats_->run(B_.y, flags, windows, n_windows, stream_);
+}
+
 sp_counters FieldWorkspace::counters() const {
   sp_counters c{};
   c.trials = trials_;
@@ -763,7 +775,8 @@ sp_counters FieldWorkspace::counters() const {
   c.graph_launches = graph_launches_;
   c.device_bytes =
       static_cast<long>(device_bytes_ + inv_plan_.bytes() + fwd_plan_.bytes() +
-                        raman_plan_.bytes() + raman_kernel_plan_.bytes());
+                        raman_plan_.bytes() + raman_kernel_plan_.bytes() +
+                        (stats_ ? stats_->bytes() : 0));
   return c;
 }
 

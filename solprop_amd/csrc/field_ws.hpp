@@ -14,6 +14,7 @@
 #include "field.hpp"
 #include "propagator.hpp"
 #include "scalars.hpp"
+#include "stats.hpp"
 
 namespace sp {
 
@@ -35,6 +36,8 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   void source(double z, const double* lab, double* out, double* streams,
               double* modal) override;
   long buffer(const char* name, double* out, long count) override;
+  std::vector<double> statistics(std::uint64_t flags, const double* windows,
+                                 int n_windows) override;
   void set_profile(bool on) override { profile_ = on; }
   std::vector<std::pair<const char*, double>> phases() const override;
   bool phase_used(int i) const;
@@ -99,6 +102,8 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   bool in_stage_ = false;       // the in-stage absorber: a stream of its own
   bool split_ = false;          // the split-step absorber: applied at every commit
   bool spatial_ = false;        // the transverse collar's matrix (dense route)
+  int n_t_ = 0;                 // coarse samples (the statistics' grid)
+  std::unique_ptr<PlaneStatistics> stats_;  // built at the first statistics call
   double* spatial_matrix_ = nullptr;  // (C, C) row-major
   bool absorbing() const { return in_stage_ || spatial_; }
   void enqueue_absorbers();

@@ -3,6 +3,7 @@
 // field route under the exponential Adams march (field_ws.hpp).
 #pragma once
 
+#include <cstdint>
 #include <stdexcept>
 #include <utility>
 #include <vector>
@@ -41,6 +42,10 @@ class Propagator {
     (void)name; (void)out; (void)count;
     throw std::invalid_argument("buffer readback is built on the field route only");
   }
+  // Plane statistics of the committed state (stats.hpp's flags, windows as (lo, hi)
+  // pairs in rad/s), packed in the reference's order.
+  virtual std::vector<double> statistics(std::uint64_t flags, const double* windows,
+                                         int n_windows) = 0;
   // The average device time of each kernel (or transform) of a trial, microseconds.
   virtual std::vector<std::pair<const char*, double>> phases() const { return {}; }
 };

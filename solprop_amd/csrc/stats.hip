@@ -1,0 +1,206 @@
+// Plane statistics (see stats.hpp): two reductions, both in a fixed order.
+#include "stats.hpp"
+
+#include <cmath>
+#include <limits>
+
+#include "kernels.hpp"
+
+namespace sp {
+
+namespace {
+
+constexpr int kStatThreads = 256;
+constexpr int kSums = 3 + kMaxStatWindows;  // weighted, plain and omega-weighted |X|^2
+
+// One workgroup per column: sum_k w_k |X_k|^2 (the energy's weights: dt / n_t, times 4
+// on a field grid's bins strictly between DC and Nyquist), sum_k |X_k|^2 and
+// sum_k omega_k |X_k|^2 (the centroid's, whose density prefactor cancels), and the
+// weighted sum inside each half-open window lo <= omega < hi. Each thread strides the
+// bins in order; the block then reduces in a fixed tree.
+__global__ void __launch_bounds__(kStatThreads)
+    stat_spectral_sums(const cplx* y, int N, int n_t, int field, double dt,
+                       const double* omega, const double* windows, int n_windows,
+                       double* sums) {
+  __shared__ double red[kStatThreads];
+  const int c = blockIdx.x;
+  const cplx* row = y + static_cast<long>(c) * N;
+  double acc[kSums];
+  for (int j = 0; j < kSums; ++j) acc[j] = 0.0;
+  const double unit = dt / n_t;
+  for (int k = threadIdx.x; k < N; k += kStatThreads) {
+    const double p = row[k].x * row[k].x + row[k].y * row[k].y;
+    const bool edge = k == 0 || (field && 2 * (N - 1) == n_t && k == N - 1);
+    const double w = (field && !edge) ? 4.0 * unit : unit;
+    const double wp = w * p;
+    acc[0] += wp;
+    acc[1] += p;
+    acc[2] += omega[k] * p;
+    // Unrolled over the largest window count: the accumulators stay in registers.
+#pragma unroll
+    for (int j = 0; j < kMaxStatWindows; ++j)
+      if (j < n_windows && omega[k] >= windows[2 * j] && omega[k] < windows[2 * j + 1])
+        acc[3 + j] += wp;
+  }
+#pragma unroll
+  for (int j = 0; j < kSums; ++j) {
+    if (j >= 3 + n_windows) break;  // uniform across the workgroup
+    red[threadIdx.x] = acc[j];
+    __syncthreads();
+    for (int width = kStatThreads / 2; width > 0; width >>= 1) {
+      if (threadIdx.x < width) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + width];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[static_cast<long>(c) * kSums + j] = red[0];
+    __syncthreads();
+  }
+}
+
+// The analytic signal's spectrum on the coarse grid, 1/n_t folded in for the unscaled
+// inverse transform: a field grid's one-sided bins doubled except DC and Nyquist, the
+// negative frequencies zero (src/solprop/fourier.py:243-300); an envelope's bins as
+// they are (src/solprop/stats/definitions.py:148-178).
+__global__ void stat_analytic_pack(const cplx* y, int C, int N, int n_t, int field,
+                                   cplx* out) {
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long>(C) * n_t) return;
+  const int c = static_cast<int>(idx / n_t), k = static_cast<int>(idx % n_t);
+  const double inv = 1.0 / n_t;
+  cplx v = make(0.0, 0.0);
+  if (!field) {
+    if (k < N) v = scale(inv, y[static_cast<long>(c) * N + k]);
+  } else if (k <= n_t / 2 && k < N) {
+    const double h = (k == 0 || k == n_t / 2) ? 1.0 : 2.0;
+    v = scale(h * inv, y[static_cast<long>(c) * N + k]);
+  }
+  out[idx] = v;
+}
+
+// Per block of samples, the largest |a_c(t)|^2 of each column and of sum_c |a_c(t)|^2;
+// blocks hold kStatThreads samples. A maximum does not depend on its order.
+__global__ void __launch_bounds__(kStatThreads)
+    stat_peak_blocks(const cplx* a, int C, int n_t, double* peaks) {
+  __shared__ double red[kStatThreads];
+  const int t = blockIdx.x * kStatThreads + threadIdx.x;
+  double total = 0.0;
+  for (int c = 0; c <= C; ++c) {
+    double v = 0.0;
+    if (c < C) {
+      if (t < n_t) {
+        const cplx z = a[static_cast<long>(c) * n_t + t];
+        v = z.x * z.x + z.y * z.y;
+        total = total + v;
+      }
+    } else {
+      v = total;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int width = kStatThreads / 2; width > 0; width >>= 1) {
+      if (threadIdx.x < width) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + width]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) peaks[static_cast<long>(blockIdx.x) * (C + 1) + c] = red[0];
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+PlaneStatistics::PlaneStatistics(int C, int N, int n_t, bool field, double dt,
+                                 const double* omega)
+    : C_(C), N_(N), n_t_(n_t), field_(field), dt_(dt), omega_(omega) {
+  SP_CHECK(C >= 1 && N >= 2 && n_t >= 16 && (n_t & (n_t - 1)) == 0,
+           "plane statistics need a power-of-two coarse grid of at least 16 samples");
+  SP_CHECK(field ? N <= n_t / 2 + 1 : N == n_t, "the stored bins do not fit the coarse grid");
+  const std::size_t ct = static_cast<std::size_t>(C) * n_t;
+  SP_HIP(hipMalloc(&time_, ct * sizeof(cplx)));
+  SP_HIP(hipMalloc(&scratch_, ct * sizeof(cplx)));
+  SP_HIP(hipMalloc(&windows_, 2 * kMaxStatWindows * sizeof(double)));
+  SP_HIP(hipMalloc(&sums_, static_cast<std::size_t>(C) * kSums * sizeof(double)));
+  peak_blocks_ = (n_t + kStatThreads - 1) / kStatThreads;
+  SP_HIP(hipMalloc(&peaks_, static_cast<std::size_t>(peak_blocks_) * (C + 1) * sizeof(double)));
+  SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_),
+                       sizeof(double) * (static_cast<std::size_t>(C) * kSums +
+                                         static_cast<std::size_t>(peak_blocks_) * (C + 1))));
+  bytes_ = 2 * ct * sizeof(cplx) + static_cast<std::size_t>(C) * kSums * sizeof(double) +
+           static_cast<std::size_t>(peak_blocks_) * (C + 1) * sizeof(double);
+  plan_.init(n_t, C);
+}
+
+PlaneStatistics::~PlaneStatistics() {
+  if (time_) (void)hipFree(time_);
+  if (scratch_) (void)hipFree(scratch_);
+  if (windows_) (void)hipFree(windows_);
+  if (sums_) (void)hipFree(sums_);
+  if (peaks_) (void)hipFree(peaks_);
+  if (host_) (void)hipHostFree(host_);
+}
+
+std::vector<double> PlaneStatistics::run(const cplx* y, std::uint64_t flags,
+                                         const double* windows, int n_windows, hipStream_t s) {
+  SP_CHECK((flags & ~kStatBuilt) == 0,
+           "statistics built: omega0, energy, energy_window, peak_power, peak_power_total");
+  SP_CHECK(n_windows >= 0 && n_windows <= kMaxStatWindows, "at most 16 statistics windows");
+  SP_CHECK(n_windows == 0 || windows, "the statistics windows are missing");
+  for (int j = 0; j < n_windows; ++j)
+    SP_CHECK(!(windows[2 * j] > windows[2 * j + 1]), "a statistics window's lo exceeds its hi");
+  const bool spectral = flags & (kStatOmega0 | kStatEnergy | kStatEnergyWindow);
+  const bool peaks = flags & (kStatPeakPower | kStatPeakPowerTotal);
+  const int used_windows = (flags & kStatEnergyWindow) ? n_windows : 0;
+  const std::size_t n_sums = static_cast<std::size_t>(C_) * kSums;
+  if (spectral) {
+    if (used_windows)
+      SP_HIP(hipMemcpyAsync(windows_, windows, 2 * used_windows * sizeof(double),
+                            hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(stat_spectral_sums, dim3(C_), dim3(kStatThreads), 0, s, y, N_, n_t_,
+                       field_ ? 1 : 0, dt_, omega_, windows_, used_windows, sums_);
+    SP_HIP(hipGetLastError());
+    SP_HIP(hipMemcpyAsync(host_, sums_, n_sums * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  if (peaks) {
+    const std::size_t ct = static_cast<std::size_t>(C_) * n_t_;
+    hipLaunchKernelGGL(stat_analytic_pack, dim3((ct + kStatThreads - 1) / kStatThreads),
+                       dim3(kStatThreads), 0, s, y, C_, N_, n_t_, field_ ? 1 : 0, time_);
+    SP_HIP(hipGetLastError());
+    plan_.exec(time_, scratch_, +1, s);
+    hipLaunchKernelGGL(stat_peak_blocks, dim3(peak_blocks_), dim3(kStatThreads), 0, s, time_,
+                       C_, n_t_, peaks_);
+    SP_HIP(hipGetLastError());
+    SP_HIP(hipMemcpyAsync(host_ + n_sums, peaks_,
+                          static_cast<std::size_t>(peak_blocks_) * (C_ + 1) * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  }
+  SP_HIP(hipStreamSynchronize(s));
+  std::vector<double> out;
+  const double* sums = host_;
+  if (flags & kStatOmega0) {
+    // An empty column (its density at most one eps of the largest column's) has no
+    // centroid (definitions.py:340-355).
+    double largest = 0.0;
+    for (int c = 0; c < C_; ++c) largest = std::fmax(largest, sums[c * kSums + 1]);
+    const double floor = std::numeric_limits<double>::epsilon() * largest;
+    for (int c = 0; c < C_; ++c) {
+      const double total = sums[c * kSums + 1];
+      out.push_back(total > floor ? sums[c * kSums + 2] / total
+                                  : std::numeric_limits<double>::quiet_NaN());
+    }
+  }
+  if (flags & kStatEnergy)
+    for (int c = 0; c < C_; ++c) out.push_back(sums[c * kSums]);
+  if (flags & kStatEnergyWindow)
+    for (int j = 0; j < n_windows; ++j)
+      for (int c = 0; c < C_; ++c) out.push_back(sums[c * kSums + 3 + j]);
+  const double* blocks = host_ + n_sums;
+  auto peak_of = [&](int c) {
+    double m = 0.0;
+    for (int b = 0; b < peak_blocks_; ++b) m = std::fmax(m, blocks[b * (C_ + 1) + c]);
+    return m;
+  };
+  if (flags & kStatPeakPower)
+    for (int c = 0; c < C_; ++c) out.push_back(peak_of(c));
+  if (flags & kStatPeakPowerTotal) out.push_back(peak_of(C_));
+  return out;
+}
+
+}  // namespace sp

@@ -442,6 +442,17 @@ bool Workspace::advance(double target, long work_limit, std::vector<double>& row
   return done;
 }
 
+std::vector<double> Workspace::statistics(std::uint64_t flags, const double* windows,
+                                          int n_windows) {
+  if (!loaded_) throw Error(kState, "no state is loaded");
+  ensure_committed();
+  // The coarse grid: N samples spaced by the fine spacing times T / N.
+  if (!stats_)
+    stats_ = std::make_unique<PlaneStatistics>(P_.C, P_.N, P_.N, false,
+                                               P_.dt_fine * P_.T / P_.N, P_.omega);
+  return stats_->run(y_, flags, windows, n_windows, stream_);
+}
+
 sp_counters Workspace::counters() const {
   sp_counters c{};
   c.trials = trials_;
@@ -449,7 +460,8 @@ sp_counters Workspace::counters() const {
   c.accepted = accepted_;
   c.graph_launches = graph_launches_;
   c.device_bytes = static_cast<long>(device_bytes_ + fine_plan_.bytes() + raman_plan_.bytes() +
-                                     forward_plan_.bytes() + coarse_plan_.bytes());
+                                     forward_plan_.bytes() + coarse_plan_.bytes() +
+                                     (stats_ ? stats_->bytes() : 0));
   return c;
 }
 

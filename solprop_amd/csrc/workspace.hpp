@@ -13,6 +13,7 @@
 #include "march.hpp"
 #include "propagator.hpp"
 #include "scalars.hpp"
+#include "stats.hpp"
 
 namespace sp {
 
@@ -34,6 +35,8 @@ class Workspace final : public Propagator {
   bool advance(double target, long work_limit, std::vector<double>& rows,
                sp_march_result* r) override;
 
+  std::vector<double> statistics(std::uint64_t flags, const double* windows,
+                                 int n_windows) override;
   void set_profile(bool on) override { profile_ = on; }
   double trial_us() const override { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
   long timed() const override { return timed_; }
@@ -64,6 +67,7 @@ class Workspace final : public Propagator {
   // the coarse grid's transform for the split-step absorber.
   FftPlan forward_plan_, coarse_plan_;
   bool in_stage_ = false, split_ = false;
+  std::unique_ptr<PlaneStatistics> stats_;  // built at the first statistics call
   double pending_dz_ = 0.0;
   // Host scalar paths.
   std::vector<ScalarPath> phi_paths_, pref_paths_, raman_paths_;

@@ -285,6 +285,39 @@ class Workspace:
         )
         return out
 
+    # StatisticsFlag's bits (native/include/solprop_native/problem/problem.hpp:44-60) for
+    # the statistics this library forms, in their packing order.
+    STATISTICS = {"omega0": 1 << 0, "energy": 1 << 1, "energy_window": 1 << 2,
+                  "peak_power": 1 << 3, "peak_power_total": 1 << 5}
+
+    def statistics(self, names, windows=None) -> dict:
+        """Plane statistics of the committed state (``stats/definitions.py``): a dict of
+        the requested names. ``windows`` is an (n, 2) array of (lo, hi) in rad/s for
+        ``energy_window``, whose value is (n, n_modes)."""
+        unknown = set(names) - set(self.STATISTICS)
+        if unknown:
+            raise BackendRefusal(f"statistics not built: {sorted(unknown)}")
+        flags = 0
+        for name in names:
+            flags |= self.STATISTICS[name]
+        w = np.zeros((0, 2)) if windows is None else np.ascontiguousarray(windows, np.float64)
+        n_modes = self.problem.n_modes
+        capacity = 3 * n_modes + len(w) * n_modes + 1
+        out = np.zeros(capacity)
+        count = C.c_int()
+        self._call("sp_workspace_statistics", flags, _ptr(w) if len(w) else None, len(w),
+                   _ptr(out), capacity, C.byref(count))
+        values, at, result = out[: count.value], 0, {}
+        sizes = {"omega0": n_modes, "energy": n_modes, "energy_window": len(w) * n_modes,
+                 "peak_power": n_modes, "peak_power_total": 1}
+        for name in sorted(self.STATISTICS, key=self.STATISTICS.get):
+            if name in names:
+                chunk = values[at: at + sizes[name]]
+                at += sizes[name]
+                result[name] = (chunk.reshape(len(w), n_modes) if name == "energy_window"
+                                else float(chunk[0]) if name == "peak_power_total" else chunk)
+        return result
+
     def counters(self) -> dict:
         c = _lib.CountersC()
         self._lib.sp_workspace_counters(self._handle, C.byref(c))

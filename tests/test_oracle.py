@@ -383,3 +383,76 @@ def test_analytic_raman_drive_is_the_hilbert_pair():
     got = P.raman_drive(field, "field", False)
     assert np.abs(got - expected).max() < 1e-13 * expected.max()
     assert np.abs(P.raman_drive(field, "field", True) - expected).max() > 0.1 * expected.max()
+
+
+# ------------------------------------------------------------------ plane statistics
+
+
+def _random_field_spectrum(rng, n_t, n_modes=3):
+    x = rng.normal(size=(n_modes, n_t)) * np.exp(-((np.arange(n_t) - n_t / 2) / (n_t / 8)) ** 2)
+    return np.fft.rfft(x, axis=-1)
+
+
+def test_energy_is_parseval_with_the_analytic_signal():
+    """``energy_omega``'s weights make it the analytic signal's time sum exactly
+    (``fourier.py:805-856``): dt sum_t |a(t)|^2, with ``a`` formed by numpy's FFT on the
+    full grid; on an envelope grid it is dt sum_t |A(t)|^2. Simpson-like weights
+    [1, 4, 2, 4, ..] (a plausible mistake) fail it."""
+    from oracle import stats as S
+
+    rng = np.random.default_rng(11)
+    n_t, dt = 256, 2e-15
+    X = _random_field_spectrum(rng, n_t)
+    x = np.fft.irfft(X, n=n_t, axis=-1)
+    h = np.zeros(n_t)
+    h[0] = h[n_t // 2] = 1.0
+    h[1:n_t // 2] = 2.0
+    a = np.fft.ifft(np.fft.fft(x, axis=-1) * h, axis=-1)
+    expected = dt * np.sum(np.abs(a) ** 2, axis=-1)
+    assert np.allclose(S.energy(X, dt, "field"), expected, rtol=1e-13, atol=0)
+    w = np.array([1.0] + [4.0 if k % 2 else 2.0 for k in range(1, n_t // 2)] + [1.0])
+    assert not np.allclose((dt / n_t) * np.sum(w * np.abs(X) ** 2, axis=-1), expected,
+                           rtol=1e-3, atol=0)
+    A = rng.normal(size=(2, 64)) + 1j * rng.normal(size=(2, 64))
+    assert np.allclose(S.energy(np.fft.fft(A, axis=-1), dt, "envelope"),
+                       dt * np.sum(np.abs(A) ** 2, axis=-1), rtol=1e-13, atol=0)
+
+
+def test_energy_windows_tile_the_total_and_the_centroid_of_one_bin():
+    """Windows that tile the axis partition the bins, so their energies sum to the total
+    (the reference's docstring identity, ``definitions.py:431-439``); a spectrum with
+    one occupied bin has that bin's frequency as its centroid; an empty column is NaN."""
+    from oracle import stats as S
+
+    rng = np.random.default_rng(12)
+    n_t, dt = 128, 1e-15
+    X = _random_field_spectrum(rng, n_t, 2)
+    omega = 2 * np.pi * np.arange(n_t // 2 + 1) / (n_t * dt)
+    cut = omega[20] + 0.5 * (omega[21] - omega[20])
+    tiles = np.array([[-np.inf, cut], [cut, np.inf]])
+    parts = S.energy_window(X, omega, tiles, dt, "field")
+    assert np.allclose(parts.sum(axis=0), S.energy(X, dt, "field"), rtol=1e-14, atol=0)
+    one = np.zeros((2, n_t // 2 + 1), complex)
+    one[0, 7] = 3.0 - 1.0j
+    centroid = S.omega0(one, omega)
+    assert centroid[0] == omega[7] and np.isnan(centroid[1])
+
+
+def test_peak_power_of_a_sampled_envelope_and_of_a_carrier():
+    """An envelope grid's peak power is the largest |A(t)|^2 sampled; a field grid's
+    carrier of amplitude E0 under a slow envelope has analytic power E0^2 at the peak
+    (|a| is the envelope, to the Hilbert transform's accuracy on a band-limited pulse);
+    the total's peak is the peak of the summed powers, not the sum of the peaks."""
+    from oracle import stats as S
+
+    n_t = 512
+    t = np.arange(n_t) - n_t / 2
+    env = np.exp(-(t / 40.0) ** 2)
+    A = np.stack([2.0 * env, np.roll(env, 100)])
+    assert np.allclose(S.peak_power(np.fft.fft(A, axis=-1), "envelope"), [4.0, 1.0],
+                       rtol=1e-13)
+    assert np.isclose(S.peak_power_total(np.fft.fft(A, axis=-1), "envelope"),
+                      np.max(np.sum(np.abs(A) ** 2, axis=0)), rtol=1e-13)
+    carrier = 3.0 * env * np.cos(2 * np.pi * 60 * np.arange(n_t) / n_t)
+    X = np.fft.rfft(carrier[None, :], axis=-1)
+    assert np.isclose(S.peak_power(X, "field")[0], 9.0, rtol=1e-10)
