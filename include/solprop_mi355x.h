@@ -198,6 +198,24 @@ int sp_workspace_statistics(sp_workspace* ws, unsigned long long flags, const do
                             int n_windows, double* out, int capacity, int* count, char* msg,
                             size_t msglen);
 
+/* The boundary monitor (`_bind_monitor`, src/solprop/monitor.py:440-606; the native
+ * workspace's monitor reading, native/src/general/statistics.cpp and
+ * statistics_monitor.inc). `weights` holds the monitor's four rows on the coarse time
+ * axis, (4, n_t): the low guard, the high guard, the collar and the collar's rate; NULL
+ * unbinds. `columns` is a spatial run's column quadrature (n_modes) or NULL. Binding
+ * empties the split collar's sums, as loading a state does. Under the split-step
+ * absorber every accepted step's finish then adds, from the masked state before the
+ * split, what it takes, sum P (1 - e^{-alpha h}), and keeps the energy and the guards'
+ * and collar's energies of P e^{-alpha h / 2}. */
+int sp_workspace_bind_monitor(sp_workspace* ws, const double* weights, const double* columns,
+                              char* msg, size_t msglen);
+/* The reading (`BoundaryMonitor.energies` and `split_sums`): out[0..4] the committed
+ * state's energy sum_t P(t) and its four weighted energies (weights . P), with
+ * P(t) = sum_columns w_c |x_c(t)|^2 on the coarse samples; out[5] the split's removal
+ * since the last reading (then emptied); out[6..9] the last accepted step's found
+ * energies (total, low guard, high guard, collar). */
+int sp_workspace_monitor_reading(sp_workspace* ws, double* out, char* msg, size_t msglen);
+
 /* Counters (`Workspace.counters()`): trials, evaluations, accepted steps, graph
  * replays, device bytes held. */
 typedef struct sp_counters {

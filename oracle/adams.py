@@ -643,6 +643,7 @@ class ExponentialEngine:
     """
 
     def __init__(self, problem, source, n_slots):
+        self.on_finish = None
         from oracle import rk
 
         self.p = problem
@@ -741,6 +742,8 @@ class ExponentialEngine:
             self.states[free] = lab
             self.rates[free] = endpoint_rate
         self.state = P.apply_hard_mask(advanced, self.p.active_mask)
+        if self.on_finish is not None:
+            self.on_finish(self.state, dz)
         if self.p.absorber_scheme == "split_step":
             # The workspace's finish: mask, the split collar, mask again
             # (``numpy_ref/workspace.py:1020-1108``).
@@ -782,16 +785,19 @@ def adams_policy_for(problem, **overrides) -> MarchPolicy:
 
 
 def march(problem, state, *, target=None, proposal=None, max_attempts=None, policy=None,
-          on_attempt=None):
+          on_attempt=None, on_finish=None):
     """Adaptive exponential Adams march of the problem (``HostAdamsMarch.march_to``).
 
     Returns ``(state, rows)`` with rows ``(end, dz, norm, accepted)`` per attempt.
+    ``on_finish(masked, dz)`` sees each accepted step's masked state before its split
+    collar (what a bound monitor's split sums read, ``numpy_ref/workspace.py:1021-1067``).
     """
     from oracle import rk
 
     policy = policy or adams_policy_for(problem)
     engine = ExponentialEngine(problem, rk.StageSource(problem), policy.order + 2)
     engine.state = state
+    engine.on_finish = on_finish
     host = _PlainHost()
     if on_attempt is not None or max_attempts is not None:
         inner = host.record

@@ -196,13 +196,14 @@ def rk_trial(problem, source: StageSource, state, z, dz):
 
 
 def march(problem, state, *, z0=0.0, proposal=None, targets=None, policy=None,
-          max_attempts=10**7, on_attempt=None):
+          max_attempts=10**7, on_attempt=None, on_finish=None):
     """Adaptive march to each target (``marches/runge_kutta.py:133-210``).
 
     Returns ``(state, records)`` where records hold ``(z, dz, norm, accepted)`` per
     attempt. The finish of an accepted step is the hard band mask and, under the
     split-step absorber, the collar over the step and the mask again
-    (``src/solprop/backends/numpy_ref/workspace.py:1021-1106``).
+    (``src/solprop/backends/numpy_ref/workspace.py:1021-1106``); ``on_finish(masked, dz)``
+    sees the masked state before the collar.
     """
     source = StageSource(problem)
     if policy is None:
@@ -234,6 +235,8 @@ def march(problem, state, *, z0=0.0, proposal=None, targets=None, policy=None,
             attempts += 1
             if accepted:
                 state = P.apply_hard_mask(advanced, problem.active_mask)
+                if on_finish is not None:
+                    on_finish(state, stop.dz)
                 if problem.absorber_scheme == "split_step":
                     state = P.apply_hard_mask(
                         P.apply_temporal_rate(state, problem.absorber_rate, stop.dz,

@@ -14,9 +14,13 @@ of shape ``(n_modes, n_omega)`` (one polarisation), unit modal power weight:
   and ``peak_power_total`` (``:479-520``) over ``analytic_field`` (``:148-178``): the
   coarse inverse transform, then on a field grid ``fourier.analytic_signal``.
 
+- :func:`monitor_energies` and :func:`split_sums` restate the boundary monitor's
+  reductions (``src/solprop/monitor.py:493-606``).
+
 Pins (``tests/test_oracle.py``): Parseval's identity with the analytic signal's time
 sum, the window tiling summing to the total, the centroid of a single bin, a sampled
-envelope's peak, and an empty column's NaN.
+envelope's peak, and an empty column's NaN; the monitor's total against the reference's
+docstring identity, and its removal against the split collar's own factor.
 """
 
 from __future__ import annotations
@@ -68,5 +72,32 @@ def peak_power_total(field: np.ndarray, kind: str) -> float:
     return float(np.max(np.sum(np.abs(analytic_field(field, kind)) ** 2, axis=0)))
 
 
-__all__ = ["analytic_field", "energy", "energy_window", "omega0", "peak_power",
-           "peak_power_total"]
+def monitor_power(field: np.ndarray, kind: str, columns=None) -> np.ndarray:
+    """``BoundaryMonitor.power`` (``src/solprop/monitor.py:493-516``): the coarse samples'
+    squared modulus, weighted by the columns' quadrature, summed over the columns."""
+    density = np.abs(P.inverse(field, kind)) ** 2
+    if columns is not None:
+        density = density * np.asarray(columns)[:, None]
+    return density.sum(axis=0)
+
+
+def monitor_energies(field, weights, kind, columns=None) -> np.ndarray:
+    """``BoundaryMonitor.energies`` (``monitor.py:518-549``): ``sum_t P`` and
+    ``weights @ P`` (four rows: low guard, high guard, collar, rate)."""
+    power = monitor_power(field, kind, columns)
+    return np.concatenate([[power.sum()], weights @ power])
+
+
+def split_sums(field, weights, step, kind, columns=None) -> np.ndarray:
+    """``BoundaryMonitor.split_sums`` (``monitor.py:551-606``): what the split takes,
+    ``P (1 - e^{-rate h})``, then the energy and the guards' and collar's energies of
+    ``P e^{-rate h / 2}``."""
+    power = monitor_power(field, kind, columns)
+    rate = weights[3]
+    taken = power @ -np.expm1(-rate * step)
+    half = power * np.exp(-0.5 * rate * step)
+    return np.concatenate([[taken, half.sum()], weights[:3] @ half])
+
+
+__all__ = ["analytic_field", "energy", "energy_window", "monitor_energies", "monitor_power",
+           "omega0", "peak_power", "peak_power_total", "split_sums"]

@@ -128,6 +128,17 @@ int sp_workspace_statistics(sp_workspace* ws, unsigned long long flags, const do
   });
 }
 
+int sp_workspace_bind_monitor(sp_workspace* ws, const double* weights, const double* columns,
+                              char* msg, size_t msglen) {
+  if (!ws) return SP_VALUE;
+  return guarded(msg, msglen, [&] { ws->impl.bind_monitor(weights, columns); });
+}
+
+int sp_workspace_monitor_reading(sp_workspace* ws, double* out, char* msg, size_t msglen) {
+  if (!ws || !out) return SP_VALUE;
+  return guarded(msg, msglen, [&] { ws->impl.monitor_reading(out); });
+}
+
 int sp_workspace_counters(sp_workspace* ws, sp_counters* out) {
   if (!ws || !out) return SP_VALUE;
   *out = ws->impl.counters();

@@ -442,6 +442,13 @@ void FieldWorkspace::flush() {
 void FieldWorkspace::enqueue_split(const AdamsRecord* R, double dz) {
   launch_field_split_pack(P_, B_, stream_);
   inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
+  // What the split takes and finds, from the masked state before it (a bound monitor;
+  // the split needs n_to == n_t, so these are the coarse samples).
+  if (monitor_.bound())
+    launch_monitor_sums(reinterpret_cast<const double*>(B_.modal_t), kSamplesReal, P_.C, P_.T,
+                        monitor_.columns(), monitor_.weights(), dz,
+                        R ? &R->accepted_dz : nullptr, kMonitorSplit, monitor_.partials(),
+                        monitor_.split(), stream_);
   launch_field_split_collar(P_, R, dz, B_, stream_);
   inv_plan_.exec(B_.modal_t, scratch_, -1, stream_);
   launch_field_split_unpack(P_, B_, stream_);
@@ -599,6 +606,7 @@ void FieldWorkspace::load(double z, const double* field) {
   SP_CHECK(field != nullptr, "load needs a field");
   const std::size_t CN = static_cast<std::size_t>(P_.C) * P_.N;
   pending_slot_ = -1;
+  monitor_.reset(stream_);
   SP_HIP(hipStreamSynchronize(stream_));
   SP_HIP(hipMemcpy(B_.y, field, CN * sizeof(cplx), hipMemcpyHostToDevice));
   adams::Engine& engine = *this;
@@ -754,6 +762,30 @@ long FieldWorkspace::buffer(const char* name, double* out, long count) {
     return row.doubles;
   }
   throw std::invalid_argument(std::string("no device buffer named ") + name);
+}
+
+void FieldWorkspace::bind_monitor(const double* weights, const double* columns) {
+  SP_HIP(hipStreamSynchronize(stream_));
+  // The captured trials hold the monitor's kernels or not: capture them again.
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  graphs_.clear();
+  if (!weights) {
+    monitor_.unbind();
+    return;
+  }
+  monitor_.bind(weights, columns, P_.C, n_t_, stream_);
+}
+
+void FieldWorkspace::monitor_reading(double* out) {
+  SP_CHECK(monitor_.bound(), "no boundary monitor is bound");
+  if (!loaded_) throw Error(kState, "no state is loaded");
+  flush();
+  if (!stats_)
+    stats_ = std::make_unique<PlaneStatistics>(P_.C, P_.N, n_t_, true,
+                                               P_.dt_fine * P_.T / n_t_, P_.omega);
+  stats_->reading(B_.y, monitor_.columns(), monitor_.weights(), monitor_.partials(),
+                  monitor_.reading(), stream_);
+  monitor_.drain(out, stream_);
 }
 
 std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double* windows,

@@ -46,6 +46,11 @@ class Propagator {
   // pairs in rad/s), packed in the reference's order.
   virtual std::vector<double> statistics(std::uint64_t flags, const double* windows,
                                          int n_windows) = 0;
+  // The boundary monitor (weights (4, n_t) on the coarse axis, or null to unbind; the
+  // columns' weights or null), and its reading of the committed state with the split
+  // collar's sums since the last reading (10 doubles).
+  virtual void bind_monitor(const double* weights, const double* columns) = 0;
+  virtual void monitor_reading(double* out) = 0;
   // The average device time of each kernel (or transform) of a trial, microseconds.
   virtual std::vector<std::pair<const char*, double>> phases() const { return {}; }
 };

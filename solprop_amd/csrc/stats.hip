@@ -105,7 +105,145 @@ __global__ void __launch_bounds__(kStatThreads)
   }
 }
 
+// The monitor's five sums, one partial set a block: each thread takes samples t in
+// order of its stride, P(t) summed over the columns in order, then a fixed tree.
+__global__ void __launch_bounds__(kMonitorThreads)
+    monitor_partials(const double* samples, int layout, int C, int n_t, const double* columns,
+                     const double* weights, double h, const double* h_at, int mode,
+                     double* partials) {
+  __shared__ double red[kMonitorThreads];
+  const double step = h_at ? *h_at : h;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int t = blockIdx.x * kMonitorThreads + threadIdx.x; t < n_t;
+       t += gridDim.x * kMonitorThreads) {
+    double p = 0.0;
+    for (int c = 0; c < C; ++c) {
+      double v;
+      if (layout == kSamplesReal) {
+        const double x = samples[static_cast<long>(c) * n_t + t];
+        v = x * x;
+      } else {
+        const double x = samples[2 * (static_cast<long>(c) * n_t + t)];
+        const double y = samples[2 * (static_cast<long>(c) * n_t + t) + 1];
+        v = layout == kSamplesComplex ? x * x + y * y : x * x;
+      }
+      p = p + (columns ? columns[c] * v : v);
+    }
+    const double rate = weights[3L * n_t + t];
+    if (mode == kMonitorReading) {
+      acc[0] += p;
+      acc[1] += weights[t] * p;
+      acc[2] += weights[static_cast<long>(n_t) + t] * p;
+      acc[3] += weights[2L * n_t + t] * p;
+      acc[4] += rate * p;
+    } else {
+      const double half = p * exp(-0.5 * rate * step);
+      acc[0] += p * -expm1(-rate * step);
+      acc[1] += half;
+      acc[2] += weights[t] * half;
+      acc[3] += weights[static_cast<long>(n_t) + t] * half;
+      acc[4] += weights[2L * n_t + t] * half;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    red[threadIdx.x] = acc[j];
+    __syncthreads();
+    for (int width = kMonitorThreads / 2; width > 0; width >>= 1) {
+      if (threadIdx.x < width) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + width];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[5L * blockIdx.x + j] = red[0];
+    __syncthreads();
+  }
+}
+
+// The blocks' partials summed in block order by one thread: the reading written, or
+// the split's removal added to the running total and the found energies stored.
+__global__ void monitor_finish(const double* partials, int blocks, int mode, double* acc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double total[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < blocks; ++b)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) total[j] = total[j] + partials[5L * b + j];
+  if (mode == kMonitorReading) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[j] = total[j];
+  } else {
+    acc[0] = acc[0] + total[0];
+#pragma unroll
+    for (int j = 1; j < 5; ++j) acc[j] = total[j];
+  }
+}
+
 }  // namespace
+
+void launch_monitor_sums(const double* samples, MonitorSamples layout, int C, int n_t,
+                         const double* columns, const double* weights, double h,
+                         const double* h_at, MonitorMode mode, double* partials, double* acc,
+                         hipStream_t s) {
+  SP_CHECK(samples && weights && partials && acc, "the monitor's buffers");
+  const int blocks = monitor_blocks(n_t);
+  hipLaunchKernelGGL(monitor_partials, dim3(blocks), dim3(kMonitorThreads), 0, s, samples,
+                     static_cast<int>(layout), C, n_t, columns, weights, h, h_at,
+                     static_cast<int>(mode), partials);
+  SP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(monitor_finish, dim3(1), dim3(64), 0, s, partials, blocks,
+                     static_cast<int>(mode), acc);
+  SP_HIP(hipGetLastError());
+}
+
+MonitorBuffers::~MonitorBuffers() {
+  if (weights_) (void)hipFree(weights_);
+  if (columns_) (void)hipFree(columns_);
+  if (partials_) (void)hipFree(partials_);
+  if (acc_) (void)hipFree(acc_);
+}
+
+void MonitorBuffers::bind(const double* weights, const double* columns, int C, int n_t,
+                          hipStream_t s) {
+  SP_CHECK(weights, "the monitor's weights are missing");
+  if (!weights_ || C != C_ || n_t != n_t_) {
+    if (weights_) (void)hipFree(weights_);
+    if (columns_) (void)hipFree(columns_);
+    if (partials_) (void)hipFree(partials_);
+    if (acc_) (void)hipFree(acc_);
+    C_ = C;
+    n_t_ = n_t;
+    SP_HIP(hipMalloc(&weights_, 4UL * n_t * sizeof(double)));
+    SP_HIP(hipMalloc(&columns_, static_cast<std::size_t>(C) * sizeof(double)));
+    SP_HIP(hipMalloc(&partials_, 5UL * monitor_blocks(n_t) * sizeof(double)));
+    SP_HIP(hipMalloc(&acc_, 10 * sizeof(double)));
+  }
+  SP_HIP(hipMemcpyAsync(weights_, weights, 4UL * n_t * sizeof(double), hipMemcpyHostToDevice, s));
+  has_columns_ = columns != nullptr;
+  if (columns)
+    SP_HIP(hipMemcpyAsync(columns_, columns, static_cast<std::size_t>(C) * sizeof(double),
+                          hipMemcpyHostToDevice, s));
+  SP_HIP(hipMemsetAsync(acc_, 0, 10 * sizeof(double), s));
+  SP_HIP(hipStreamSynchronize(s));
+  bound_ = true;
+}
+
+void MonitorBuffers::drain(double* out, hipStream_t s) {
+  SP_HIP(hipMemcpyAsync(out, acc_, 10 * sizeof(double), hipMemcpyDeviceToHost, s));
+  SP_HIP(hipMemsetAsync(acc_ + 5, 0, sizeof(double), s));
+  SP_HIP(hipStreamSynchronize(s));
+}
+
+void PlaneStatistics::reading(const cplx* y, const double* columns, const double* weights,
+                              double* partials, double* acc, hipStream_t s) {
+  // The coarse samples: the inverse transform (an envelope) or the real part of the
+  // analytic signal (a field, whose real part is the field itself).
+  const std::size_t ct = static_cast<std::size_t>(C_) * n_t_;
+  hipLaunchKernelGGL(stat_analytic_pack, dim3((ct + kStatThreads - 1) / kStatThreads),
+                     dim3(kStatThreads), 0, s, y, C_, N_, n_t_, field_ ? 1 : 0, time_);
+  SP_HIP(hipGetLastError());
+  plan_.exec(time_, scratch_, +1, s);
+  launch_monitor_sums(reinterpret_cast<const double*>(time_),
+                      field_ ? kSamplesRealPart : kSamplesComplex, C_, n_t_, columns, weights,
+                      0.0, nullptr, kMonitorReading, partials, acc, s);
+}
 
 PlaneStatistics::PlaneStatistics(int C, int N, int n_t, bool field, double dt,
                                  const double* omega)

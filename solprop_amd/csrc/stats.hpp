@@ -25,6 +25,55 @@ constexpr std::uint64_t kStatBuilt =
     kStatOmega0 | kStatEnergy | kStatEnergyWindow | kStatPeakPower | kStatPeakPowerTotal;
 constexpr int kMaxStatWindows = 16;
 
+// The boundary monitor's sums (src/solprop/monitor.py:493-606) over the time samples of
+// C columns: P(t) = sum_c w_c |s_c(t)|^2 (w_c one without column weights), then with the
+// monitor's four weight rows (low guard, high guard, collar, rate) either
+//   kMonitorReading: [sum P, low.P, high.P, collar.P, rate.P], or
+//   kMonitorSplit:   [sum P (1 - e^{-rate h}), sum H, low.H, high.H, collar.H] with
+//                    H = P e^{-rate h / 2} (what the split takes, and the power it finds
+//                    halfway through its factor),
+// and kMonitorSplit adds its first sum to acc[0] and stores the other four in
+// acc[1..4]; kMonitorReading writes the five to acc. Samples are real (stride one
+// double), complex taken whole, or complex taken by their real part.
+enum MonitorMode { kMonitorReading = 0, kMonitorSplit = 1 };
+enum MonitorSamples { kSamplesReal = 0, kSamplesComplex = 1, kSamplesRealPart = 2 };
+// The step h is read from the device at `h_at` when given (a record's accepted step,
+// inside a graph), else taken from `h`. `partials` holds 5 doubles a block.
+void launch_monitor_sums(const double* samples, MonitorSamples layout, int C, int n_t,
+                         const double* columns, const double* weights, double h,
+                         const double* h_at, MonitorMode mode, double* partials, double* acc,
+                         hipStream_t s);
+constexpr int kMonitorThreads = 256;
+inline int monitor_blocks(int n_t) { return (n_t + kMonitorThreads - 1) / kMonitorThreads; }
+
+// A bound monitor's device buffers: its weights (4, n_t), the columns' weights (C) or
+// none, block partials, the last reading (5) and the split's running removal and last
+// found energies (5). Binding uploads the weights and empties the split's sums.
+class MonitorBuffers {
+ public:
+  ~MonitorBuffers();
+  void bind(const double* weights, const double* columns, int C, int n_t, hipStream_t s);
+  void unbind() { bound_ = false; }
+  // Empty the reading and the split's sums (a state was loaded).
+  void reset(hipStream_t s) {
+    if (acc_) SP_HIP(hipMemsetAsync(acc_, 0, 10 * sizeof(double), s));
+  }
+  bool bound() const { return bound_; }
+  const double* weights() const { return weights_; }
+  const double* columns() const { return has_columns_ ? columns_ : nullptr; }
+  double* partials() const { return partials_; }
+  double* reading() const { return acc_; }
+  double* split() const { return acc_ + 5; }
+  // The last reading and the split's sums to the host (10 doubles), the split's
+  // running removal then emptied (the host's accumulator drains it).
+  void drain(double* out, hipStream_t s);
+
+ private:
+  bool bound_ = false, has_columns_ = false;
+  int C_ = 0, n_t_ = 0;
+  double *weights_ = nullptr, *columns_ = nullptr, *partials_ = nullptr, *acc_ = nullptr;
+};
+
 class PlaneStatistics {
  public:
   // C columns of N stored bins on a coarse grid of n_t samples spaced dt; field (real
@@ -39,6 +88,9 @@ class PlaneStatistics {
   std::vector<double> run(const cplx* y, std::uint64_t flags, const double* windows,
                           int n_windows, hipStream_t s);
   std::size_t bytes() const { return bytes_ + plan_.bytes(); }
+  // The monitor's five sums of the state y (kMonitorReading) into acc (device, 5).
+  void reading(const cplx* y, const double* columns, const double* weights, double* partials,
+               double* acc, hipStream_t s);
 
  private:
   int C_, N_, n_t_;

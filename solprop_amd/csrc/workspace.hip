@@ -245,6 +245,12 @@ void Workspace::enqueue_commit(const TrialRecord* R, double dz) {
   // fine_ is free between trials: it holds the coarse grid's time samples here.
   launch_env_split_pack(P_, y_, fine_, stream_);
   coarse_plan_.exec(fine_, scratch_, +1, stream_);
+  // What the split takes and finds, from the masked state before it (a bound monitor).
+  if (monitor_.bound())
+    launch_monitor_sums(reinterpret_cast<const double*>(fine_), kSamplesComplex, P_.C, P_.N,
+                        monitor_.columns(), monitor_.weights(), dz,
+                        R ? &R->accepted_dz : nullptr, kMonitorSplit, monitor_.partials(),
+                        monitor_.split(), stream_);
   launch_env_split_collar(P_, R, dz, fine_, stream_);
   coarse_plan_.exec(fine_, scratch_, -1, stream_);
   launch_commit_masked(P_, fine_, y_, stream_);
@@ -349,6 +355,7 @@ void Workspace::ensure_committed() {
 void Workspace::load(double z, const double* field) {
   SP_CHECK(field != nullptr, "load needs a field");
   const std::size_t CN = static_cast<std::size_t>(P_.C) * P_.N;
+  monitor_.reset(stream_);
   SP_HIP(hipMemcpyAsync(y_, field, CN * sizeof(cplx), hipMemcpyHostToDevice, stream_));
   if (P_.n_raman && !raman_dynamic_) {
     fill_record(z, 1.0);
@@ -440,6 +447,30 @@ bool Workspace::advance(double target, long work_limit, std::vector<double>& row
   r->accepted = host.accepted;
   r->evaluations = evaluations_;
   return done;
+}
+
+void Workspace::bind_monitor(const double* weights, const double* columns) {
+  SP_HIP(hipStreamSynchronize(stream_));
+  // The captured trials hold the monitor's kernels or not: capture them again.
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  graphs_.clear();
+  if (!weights) {
+    monitor_.unbind();
+    return;
+  }
+  monitor_.bind(weights, columns, P_.C, P_.N, stream_);
+}
+
+void Workspace::monitor_reading(double* out) {
+  SP_CHECK(monitor_.bound(), "no boundary monitor is bound");
+  if (!loaded_) throw Error(kState, "no state is loaded");
+  ensure_committed();
+  if (!stats_)
+    stats_ = std::make_unique<PlaneStatistics>(P_.C, P_.N, P_.N, false,
+                                               P_.dt_fine * P_.T / P_.N, P_.omega);
+  stats_->reading(y_, monitor_.columns(), monitor_.weights(), monitor_.partials(),
+                  monitor_.reading(), stream_);
+  monitor_.drain(out, stream_);
 }
 
 std::vector<double> Workspace::statistics(std::uint64_t flags, const double* windows,

@@ -37,6 +37,8 @@ class Workspace final : public Propagator {
 
   std::vector<double> statistics(std::uint64_t flags, const double* windows,
                                  int n_windows) override;
+  void bind_monitor(const double* weights, const double* columns) override;
+  void monitor_reading(double* out) override;
   void set_profile(bool on) override { profile_ = on; }
   double trial_us() const override { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
   long timed() const override { return timed_; }
@@ -68,6 +70,7 @@ class Workspace final : public Propagator {
   FftPlan forward_plan_, coarse_plan_;
   bool in_stage_ = false, split_ = false;
   std::unique_ptr<PlaneStatistics> stats_;  // built at the first statistics call
+  MonitorBuffers monitor_;
   double pending_dz_ = 0.0;
   // Host scalar paths.
   std::vector<ScalarPath> phi_paths_, pref_paths_, raman_paths_;

@@ -318,6 +318,25 @@ class Workspace:
                                 else float(chunk[0]) if name == "peak_power_total" else chunk)
         return result
 
+    def bind_monitor(self, weights: np.ndarray | None, columns: np.ndarray | None = None):
+        """Bind the boundary monitor's (4, n_t) weights (low guard, high guard, collar,
+        rate) on the coarse axis, or unbind with None (``_bind_monitor``)."""
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        if w is not None and w.shape != (4, self.problem.n_t):
+            raise ValueError(f"monitor weights: expected (4, {self.problem.n_t})")
+        col = None if columns is None else np.ascontiguousarray(columns, dtype=np.float64)
+        self._call("sp_workspace_bind_monitor", None if w is None else _ptr(w),
+                   None if col is None else _ptr(col))
+        self._monitor_keep = (w, col)
+
+    def monitor_reading(self) -> dict:
+        """The state's five energies, the split's removal since the last reading and
+        the last step's found energies (``BoundaryEnergies``)."""
+        out = np.zeros(10)
+        self._call("sp_workspace_monitor_reading", _ptr(out))
+        return {"total": out[0], "weighted": out[1:5].copy(), "taken": out[5],
+                "found": out[6:10].copy()}
+
     def counters(self) -> dict:
         c = _lib.CountersC()
         self._lib.sp_workspace_counters(self._handle, C.byref(c))

@@ -68,3 +68,84 @@ def test_statistics_match_oracle(route, marched):
         assert err <= 1e-12 * np.abs(expected[name]).max(), (name, err)
     # The windows tile the active band: their energies sum to the energy there.
     assert np.allclose(got["energy_window"].sum(axis=0), got["energy"], rtol=1e-12, atol=0)
+
+
+def monitor_weights(problem):
+    """A monitor's rows on the coarse axis: guards on the outer tenth each side, the
+    collar's complement, and the split collar's rate (or a ramp without one)."""
+    n_t = problem.n_t
+    t = np.arange(n_t) - n_t / 2
+    low = (t < -0.4 * n_t).astype(float)
+    high = (t > 0.4 * n_t).astype(float)
+    collar = np.clip((np.abs(t) - 0.3 * n_t) / (0.2 * n_t), 0.0, 1.0)
+    rate = (problem.absorber_rate if problem.absorber_scheme == "split_step"
+            else 100.0 * collar)
+    return np.ascontiguousarray(np.stack([low, high, collar, rate]))
+
+
+@pytest.mark.parametrize("route", sorted(PROBLEMS))
+def test_monitor_reading_matches_oracle(route):
+    """The five energies of the committed state, within 1e-12 of the total."""
+    print(f"monitor reading {route}", flush=True)
+    problem = PROBLEMS[route]()
+    weights = monitor_weights(problem)
+    ws = Workspace(problem)
+    session = ws.start(0.0, problem.field0)
+    session.advance(problem.z_end, work_limit=4)
+    ws.bind_monitor(weights)
+    state = session.state()
+    got = ws.monitor_reading()
+    ws.close()
+    field = state.reshape(problem.n_modes, problem.n_omega)
+    expected = S.monitor_energies(field, weights, problem.kind)
+    values = np.concatenate([[got["total"]], got["weighted"]])
+    assert np.abs(values - expected).max() <= 1e-12 * expected[0]
+    assert got["taken"] == 0.0
+
+
+SPLIT = {
+    "capillary": lambda: capillary_problem(n_t=2048, n_nodes=32, length=0.05,
+                                           integration="trapezoid", absorber="split_step",
+                                           absorber_length=2e-2),
+    "envelope": lambda: gnlse_problem(n_t=512, ffac=2, window=2e-12, fwhm=50e-15,
+                                      soliton_order=2.0, length=5e-3, absorber="split_step",
+                                      absorber_length=1e-3),
+}
+
+
+@pytest.mark.parametrize("route", sorted(SPLIT))
+def test_split_sums_follow_the_oracle_march(route):
+    """Under the split-step absorber with a bound monitor, each accepted step's finish
+    adds what its split takes and keeps what it finds: the reading after a short march
+    holds the sum over the oracle march's steps (from each masked state before its
+    split) and the last step's found energies, within the marches' own agreement
+    (1e-7); a second reading has nothing taken."""
+    from oracle import adams, rk
+
+    print(f"split sums {route}", flush=True)
+    problem = SPLIT[route]()
+    weights = monitor_weights(problem)
+    finishes = []
+    y0 = problem.field0.reshape(problem.shape)
+    if problem.method == "adams":
+        _, records = adams.march(problem, y0.copy(), max_attempts=12,
+                                 on_finish=lambda s, dz: finishes.append((s.copy(), dz)))
+    else:
+        _, records = rk.march(problem, y0.copy(), max_attempts=12,
+                              on_finish=lambda s, dz: finishes.append((s.copy(), dz)))
+    sums = [S.split_sums(s.reshape(problem.n_modes, problem.n_omega), weights, dz,
+                         problem.kind) for s, dz in finishes]
+    ws = Workspace(problem)
+    ws.load(0.0, problem.field0)
+    ws.bind_monitor(weights)
+    session = ws.start(0.0, problem.field0)
+    rows = session.advance(problem.z_end, work_limit=len(records)).attempts
+    first = ws.monitor_reading()
+    second = ws.monitor_reading()
+    ws.close()
+    assert len(rows) == len(records) and len(finishes) > 2
+    taken = sum(v[0] for v in sums)
+    assert taken > 0.0
+    assert abs(first["taken"] - taken) <= 1e-7 * taken
+    assert np.allclose(first["found"], sums[-1][1:], rtol=1e-7, atol=1e-7 * sums[-1][1])
+    assert second["taken"] == 0.0

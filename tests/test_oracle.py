@@ -456,3 +456,32 @@ def test_peak_power_of_a_sampled_envelope_and_of_a_carrier():
     carrier = 3.0 * env * np.cos(2 * np.pi * 60 * np.arange(n_t) / n_t)
     X = np.fft.rfft(carrier[None, :], axis=-1)
     assert np.isclose(S.peak_power(X, "field")[0], 9.0, rtol=1e-10)
+
+
+def test_monitor_total_and_split_removal():
+    """The monitor's total is sum |pulse|^2 for an envelope built as fft(pulse) (the
+    reference's docstring identity, ``monitor.py:536-546``); the split's removal equals
+    the energy the collar's own factor takes (``apply_temporal_rate``, an independent
+    path: the total minus the power of the collared state); zero rate takes nothing and
+    finds the whole energy."""
+    from oracle import passes as P
+    from oracle import stats as S
+
+    n = 256
+    t = np.arange(n) - n / 2
+    pulse = np.exp(-((t - 0.45 * n) / 6.0) ** 2) + 0.3 * np.exp(-(t / 20.0) ** 2)
+    field = np.fft.fft(pulse.astype(complex))[None, :]
+    edge = (np.abs(t) > 0.4 * n).astype(float)
+    rate = 50.0 * np.clip((np.abs(t) - 0.35 * n) / (0.15 * n), 0.0, 1.0)
+    weights = np.stack([(t < -0.4 * n) * 1.0, (t > 0.4 * n) * 1.0, edge, rate])
+    energies = S.monitor_energies(field, weights, "envelope")
+    assert np.isclose(energies[0], np.sum(np.abs(pulse) ** 2), rtol=1e-13)
+    h = 0.01
+    taken, found = S.split_sums(field, weights, h, "envelope")[:2]
+    collared = P.apply_temporal_rate(field, rate, h, "envelope")
+    after = S.monitor_power(collared, "envelope").sum()
+    assert np.isclose(taken, energies[0] - after, rtol=1e-12)
+    assert taken > 0.0
+    still = S.split_sums(field, np.stack([weights[0], weights[1], weights[2], 0 * rate]), h,
+                         "envelope")
+    assert still[0] == 0.0 and np.isclose(still[1], energies[0], rtol=1e-14)
