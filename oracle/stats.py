@@ -14,6 +14,7 @@ of shape ``(n_modes, n_omega)`` (one polarisation), unit modal power weight:
   and ``peak_power_total`` (``:479-520``) over ``analytic_field`` (``:148-178``): the
   coarse inverse transform, then on a field grid ``fourier.analytic_signal``.
 
+- :func:`ionisation` restates the plasma pair (``numpy_ref/statistics.py:502-546``).
 - :func:`monitor_energies` and :func:`split_sums` restate the boundary monitor's
   reductions (``src/solprop/monitor.py:493-606``).
 
@@ -99,5 +100,23 @@ def split_sums(field, weights, step, kind, columns=None) -> np.ndarray:
     return np.concatenate([[taken, half.sum()], weights[:3] @ half])
 
 
+def ionisation(problem, state, z):
+    """The electron density and the peak ionisation rate at a plane
+    (``src/solprop/backends/numpy_ref/statistics.py:502-546``): the plasma scan of the
+    node fields, the node of largest fraction at the window's end (the first), density
+    times that fraction, and that node's largest rate."""
+    from oracle import rk
+
+    src = rk.StageSource(problem)
+    crop = problem.crop_index.astype(np.intp)
+    modal = P.to_time(state.reshape(problem.shape), crop, problem.n_to, problem.kind)
+    nodes = P.synthesise(modal, problem.synthesis, 1.0, 1)
+    rate = P.ionisation_rate(nodes, src.plasma.rate)
+    fraction, _ = P.plasma_scan(nodes, rate, src.plasma, problem.preionisation(z),
+                                problem.dt_fine)
+    densest = int(np.argmax(fraction[:, -1]))
+    return float(problem.density(z)) * fraction[densest, -1], float(np.max(rate[densest]))
+
+
 __all__ = ["analytic_field", "energy", "energy_window", "monitor_energies", "monitor_power",
-           "omega0", "peak_power", "peak_power_total", "split_sums"]
+           "ionisation", "omega0", "peak_power", "peak_power_total", "split_sums"]

@@ -796,7 +796,28 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
   if (!stats_)
     stats_ = std::make_unique<PlaneStatistics>(P_.C, P_.N, n_t_, true,
                                                P_.dt_fine * P_.T / n_t_, P_.omega);
-  return stats_->run(B_.y, flags, windows, n_windows, stream_);
+  std::vector<double> out = stats_->run(B_.y, flags & ~kStatPlasma, windows, n_windows, stream_);
+  if (flags & kStatPlasma) {
+    SP_CHECK(P_.has_plasma, "the electron density and the peak ionisation rate need a "
+                            "compiled plasma response, and this problem declares none");
+    // The plasma column at the committed state and its position: the node passes on its
+    // modal fields, then the densest node's end fraction and largest rate.
+    fill_position(z_);
+    SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
+                          stream_));
+    launch_field_scatter(P_, B_.y, B_, stream_);
+    inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
+    enqueue_synthesis();
+    launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
+    launch_plasma_densest(B_.frac, B_.rate, P_.Q, P_.T, B_.partials, stream_);
+    double densest[2];
+    SP_HIP(hipMemcpyAsync(densest, B_.partials, sizeof(densest), hipMemcpyDeviceToHost,
+                          stream_));
+    SP_HIP(hipStreamSynchronize(stream_));
+    if (flags & kStatElectronDensity) out.push_back(density_(z_) * densest[0]);
+    if (flags & kStatPeakIonisationRate) out.push_back(densest[1]);
+  }
+  return out;
 }
 
 sp_counters FieldWorkspace::counters() const {

@@ -176,7 +176,50 @@ __global__ void monitor_finish(const double* partials, int blocks, int mode, dou
   }
 }
 
+// One workgroup: the densest node by a serial scan in node order (the first maximum),
+// then the largest rate of its row by a fixed tree.
+__global__ void __launch_bounds__(kStatThreads)
+    plasma_densest(const double* frac, const double* rate, int Q, int T, double* out) {
+  __shared__ double red[kStatThreads];
+  __shared__ int densest;
+  __shared__ double largest;
+  if (threadIdx.x == 0) {
+    int best = 0;
+    double value = frac[static_cast<long>(T) - 1];
+    for (int q = 1; q < Q; ++q) {
+      const double f = frac[static_cast<long>(q) * T + T - 1];
+      if (f > value) {
+        value = f;
+        best = q;
+      }
+    }
+    densest = best;
+    largest = value;
+  }
+  __syncthreads();
+  double peak = 0.0;
+  for (int t = threadIdx.x; t < T; t += kStatThreads)
+    peak = fmax(peak, rate[static_cast<long>(densest) * T + t]);
+  red[threadIdx.x] = peak;
+  __syncthreads();
+  for (int width = kStatThreads / 2; width > 0; width >>= 1) {
+    if (threadIdx.x < width) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + width]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = largest;
+    out[1] = red[0];
+  }
+}
+
 }  // namespace
+
+void launch_plasma_densest(const double* frac, const double* rate, int Q, int T, double* out,
+                           hipStream_t s) {
+  SP_CHECK(frac && rate && out && Q >= 1 && T >= 1, "the plasma column's arrays");
+  hipLaunchKernelGGL(plasma_densest, dim3(1), dim3(kStatThreads), 0, s, frac, rate, Q, T, out);
+  SP_HIP(hipGetLastError());
+}
 
 void launch_monitor_sums(const double* samples, MonitorSamples layout, int C, int n_t,
                          const double* columns, const double* weights, double h,

@@ -13,14 +13,23 @@
 
 namespace sp {
 
-// The flags of native/include/solprop_native/problem/problem.hpp:44-60 this build forms.
+// The flags of native/include/solprop_native/problem/problem.hpp:44-60 this build forms
+// (the plasma pair on the field routes, by the workspace).
 enum : std::uint64_t {
   kStatOmega0 = 1ULL << 0U,
   kStatEnergy = 1ULL << 1U,
   kStatEnergyWindow = 1ULL << 2U,
   kStatPeakPower = 1ULL << 3U,
   kStatPeakPowerTotal = 1ULL << 5U,
+  // The field routes' plasma column at the committed state (read by the workspace).
+  kStatElectronDensity = 1ULL << 7U,
+  kStatPeakIonisationRate = 1ULL << 8U,
 };
+constexpr std::uint64_t kStatPlasma = kStatElectronDensity | kStatPeakIonisationRate;
+// The node of largest ionised fraction at the window's end (the first such node) and
+// its largest rate: out[0] that fraction, out[1] the rate (numpy_ref/statistics.py:502-546).
+void launch_plasma_densest(const double* frac, const double* rate, int Q, int T, double* out,
+                           hipStream_t s);
 constexpr std::uint64_t kStatBuilt =
     kStatOmega0 | kStatEnergy | kStatEnergyWindow | kStatPeakPower | kStatPeakPowerTotal;
 constexpr int kMaxStatWindows = 16;

@@ -288,7 +288,8 @@ class Workspace:
     # StatisticsFlag's bits (native/include/solprop_native/problem/problem.hpp:44-60) for
     # the statistics this library forms, in their packing order.
     STATISTICS = {"omega0": 1 << 0, "energy": 1 << 1, "energy_window": 1 << 2,
-                  "peak_power": 1 << 3, "peak_power_total": 1 << 5}
+                  "peak_power": 1 << 3, "peak_power_total": 1 << 5,
+                  "electron_density": 1 << 7, "peak_ionisation_rate": 1 << 8}
 
     def statistics(self, names, windows=None) -> dict:
         """Plane statistics of the committed state (``stats/definitions.py``): a dict of
@@ -302,20 +303,22 @@ class Workspace:
             flags |= self.STATISTICS[name]
         w = np.zeros((0, 2)) if windows is None else np.ascontiguousarray(windows, np.float64)
         n_modes = self.problem.n_modes
-        capacity = 3 * n_modes + len(w) * n_modes + 1
+        capacity = 3 * n_modes + len(w) * n_modes + 3
         out = np.zeros(capacity)
         count = C.c_int()
         self._call("sp_workspace_statistics", flags, _ptr(w) if len(w) else None, len(w),
                    _ptr(out), capacity, C.byref(count))
         values, at, result = out[: count.value], 0, {}
         sizes = {"omega0": n_modes, "energy": n_modes, "energy_window": len(w) * n_modes,
-                 "peak_power": n_modes, "peak_power_total": 1}
+                 "peak_power": n_modes, "peak_power_total": 1, "electron_density": 1,
+                 "peak_ionisation_rate": 1}
+        scalars = ("peak_power_total", "electron_density", "peak_ionisation_rate")
         for name in sorted(self.STATISTICS, key=self.STATISTICS.get):
             if name in names:
                 chunk = values[at: at + sizes[name]]
                 at += sizes[name]
                 result[name] = (chunk.reshape(len(w), n_modes) if name == "energy_window"
-                                else float(chunk[0]) if name == "peak_power_total" else chunk)
+                                else float(chunk[0]) if name in scalars else chunk)
         return result
 
     def bind_monitor(self, weights: np.ndarray | None, columns: np.ndarray | None = None):

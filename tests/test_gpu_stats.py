@@ -149,3 +149,28 @@ def test_split_sums_follow_the_oracle_march(route):
     assert abs(first["taken"] - taken) <= 1e-7 * taken
     assert np.allclose(first["found"], sums[-1][1:], rtol=1e-7, atol=1e-7 * sums[-1][1])
     assert second["taken"] == 0.0
+
+
+@pytest.mark.parametrize("route", ["capillary", "radial"])
+def test_plasma_statistics_match_oracle(route):
+    """The electron density and the peak ionisation rate at the committed state after a
+    short march, against the oracle's scan of the same state: the rate to 1e-12, the
+    density (a fraction 1 - (1 - f0) e^{-X} that cancels at small exposure) to 1e-9."""
+    print(f"plasma statistics {route}", flush=True)
+    # The trapezoid rule on one route and the band rule on the other.
+    if route == "capillary":
+        problem = capillary_problem(n_t=2048, n_nodes=32, length=0.05, integration="trapezoid")
+    else:
+        problem = radial_problem(n_t=1024, window=0.25e-12, n_r=32, length=0.02,
+                                 fwhm=30e-15, w0=0.25e-3, integration="band",
+                                 band=(400e-9, 4e-6))
+    ws = Workspace(problem)
+    session = ws.start(0.0, problem.field0)
+    session.advance(problem.z_end, work_limit=4)
+    state, z = session.state(), session.z
+    got = ws.statistics(["electron_density", "peak_ionisation_rate"])
+    ws.close()
+    density, rate = S.ionisation(problem, state, z)
+    assert density > 0.0 and rate > 0.0
+    assert abs(got["electron_density"] - density) <= 1e-9 * density
+    assert abs(got["peak_ionisation_rate"] - rate) <= 1e-12 * rate
