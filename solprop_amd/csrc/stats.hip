@@ -44,15 +44,16 @@ __global__ void __launch_bounds__(kStatThreads)
   }
 #pragma unroll
   for (int j = 0; j < kSums; ++j) {
-    if (j >= 3 + n_windows) break;  // uniform across the workgroup
-    red[threadIdx.x] = acc[j];
-    __syncthreads();
-    for (int width = kStatThreads / 2; width > 0; width >>= 1) {
-      if (threadIdx.x < width) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + width];
+    if (j < 3 + n_windows) {  // uniform across the workgroup
+      red[threadIdx.x] = acc[j];
+      __syncthreads();
+      for (int width = kStatThreads / 2; width > 0; width >>= 1) {
+        if (threadIdx.x < width) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + width];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) sums[static_cast<long>(c) * kSums + j] = red[0];
       __syncthreads();
     }
-    if (threadIdx.x == 0) sums[static_cast<long>(c) * kSums + j] = red[0];
-    __syncthreads();
   }
 }
 
