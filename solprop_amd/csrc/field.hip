@@ -404,6 +404,7 @@ __device__ inline double band_filter(const PlasmaTable& pt, const double* window
   const int K = pt.n_taps - 1;
   const double* y = window + K + lane;
   double f = pt.taps[0] * y[0];
+#pragma unroll 2
   for (int k = 1; k <= K; ++k) f = f + pt.taps[k] * (y[-k] + y[k]);
   return f;
 }
@@ -461,20 +462,30 @@ __device__ inline void wave_sync() {
 }
 
 // The band passes run each wave's nodes as the trapezoid passes do (modal fields of the
-// lane's sample held in registers across them), with the filter's halo of every node of
-// the wave evaluated first in one pass: lane l takes slot l % 2K of the wave's node
-// l / 2K (2K <= 16, so four nodes fit one wave).
-static_assert(kNodesPerWave * 2 * kMaxBandHalf <= kWave, "one pass holds every halo");
+// lane's sample held in registers across them), with the samples a node needs beyond
+// its own lanes evaluated first for every node of the wave: the filter's halo of 2K
+// samples and, in the last tile, the K reflections past the grid's end that the tile's
+// own lanes hold (slot 2K + r is sample T + r). Keeping those evaluations out of the
+// node loop keeps its registers few.
+constexpr int kBandSlots = 3 * kMaxBandHalf;
+
+// Slot h of a node's extra samples in a tile with 2K halo slots.
+__device__ inline int extra_sample(int tile, int K, int T, int h) {
+  return h < 2 * K ? halo_sample(tile, K, h) : T + (h - 2 * K);
+}
+
+// The band passes ask for six (rate: its LDS table allows three workgroups a CU) and
+// eight (current) waves a SIMD: at most 80 and 64 VGPRs, with no scratch.
 
 // Pass A under the band rule: the raw rate stored for the owned samples, the filtered
 // rate's trapezoid increments scanned, the within-tile exposure stored and the tile's
 // sum.
 template <int C>
-__global__ void __launch_bounds__(kWave * kNodeWaves)
+__global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_waves_per_eu(6)))
     plasma_rate_band(DeviceProblem P, PlasmaTable pt, FieldBuffers B) {
   __shared__ double table[4 * kLdsTableIntervals];
   __shared__ double windows[kNodeWaves][kBandSpan];
-  __shared__ double halos[kNodeWaves][kNodesPerWave * 2 * kMaxBandHalf];
+  __shared__ double halos[kNodeWaves][kNodesPerWave * kBandSlots];
   const bool in_lds = pt.n_intervals <= kLdsTableIntervals;
   if (in_lds)
     for (int i = threadIdx.x; i < 4 * pt.n_intervals; i += blockDim.x) table[i] = B.rate_c[i];
@@ -489,31 +500,31 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   double* window = windows[wave];
   double* halo = halos[wave];
   const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave + wave;
-  if (lane < H * kNodesPerWave) {
-    const int i = lane / H, h = lane - i * H;
+  const int S = H + (tile == tiles - 1 ? K : 0);  // extra samples a node needs
+  for (int idx = lane; idx < kNodesPerWave * S; idx += kWave) {
+    const int i = idx / S, h = idx - i * S;
     const int q = q0 + i * kNodeWaves;
     double v = 0.0;
     if (q < P.Q) {
       auto sample = [&](int u) { return rate_sample<C>(P, pt, coefficients, B, q, u); };
-      v = reflected(T, K, halo_sample(tile, K, h), sample);
+      v = reflected(T, K, extra_sample(tile, K, T, h), sample);
     }
-    halo[lane] = v;
+    halo[i * kBandSlots + h] = v;
   }
+  wave_sync();
   double am[held(C)];
   load_modal<C>(B, T, t, am);
   for (int i = 0; i < kNodesPerWave; ++i) {
     const int q = q0 + i * kNodeWaves;
     if (q >= P.Q) break;  // uniform across the wave, and so for every later node
-    double w;
-    if (t < T) {
+    double w = 0.0;
+    if (t < T)
       w = rate_of(fabs(field_at<C>(P, B, am, q, t)), pt, coefficients);
-    } else {
-      auto sample = [&](int u) { return rate_sample<C>(P, pt, coefficients, B, q, u); };
-      w = reflected(T, K, t, sample);
-    }
+    else if (t - T < K)  // the last tile's lanes past the end: their reflections
+      w = halo[i * kBandSlots + H + (t - T)];
     wave_sync();  // the previous node's window has been read
     window[K + lane] = w;
-    if (lane < H) window[halo_slot(K, lane)] = halo[i * H + lane];
+    if (lane < H) window[halo_slot(K, lane)] = halo[i * kBandSlots + lane];
     wave_sync();
     const long row = static_cast<long>(q) * T;
     if (stores) B.rate[row + t] = w;
@@ -533,11 +544,11 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
 // exposure) filtered and its increments scanned; the fraction and the within-tile free
 // current stored for the owned samples, and the tile's sum.
 template <int C>
-__global__ void __launch_bounds__(kWave * kNodeWaves)
+__global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_waves_per_eu(8)))
     plasma_current_band(DeviceProblem P, PlasmaTable pt, const AdamsRecord* __restrict__ R,
                         FieldBuffers B) {
   __shared__ double windows[kNodeWaves][kBandSpan];
-  __shared__ double halos[kNodeWaves][kNodesPerWave * 2 * kMaxBandHalf];
+  __shared__ double halos[kNodeWaves][kNodesPerWave * kBandSlots];
   const int T = P.T, tiles = scan_tiles(T), K = pt.n_taps - 1, H = 2 * K;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const int tile = blockIdx.x;
@@ -549,34 +560,35 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   double* window = windows[wave];
   double* halo = halos[wave];
   const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave + wave;
-  if (lane < H * kNodesPerWave) {
-    const int i = lane / H, h = lane - i * H;
+  const int S = H + (tile == tiles - 1 ? K : 0);  // extra samples a node needs
+  for (int idx = lane; idx < kNodesPerWave * S; idx += kWave) {
+    const int i = idx / S, h = idx - i * S;
     const int q = q0 + i * kNodeWaves;
     double v = 0.0;
     if (q < P.Q) {
       auto sample = [&](int u) { return current_sample<C>(P, B, q, u, f0, tiles); };
-      v = reflected(T, K, halo_sample(tile, K, h), sample);
+      v = reflected(T, K, extra_sample(tile, K, T, h), sample);
     }
-    halo[lane] = v;
+    halo[i * kBandSlots + h] = v;
   }
+  wave_sync();
   double am[held(C)];
   load_modal<C>(B, T, t, am);
   for (int i = 0; i < kNodesPerWave; ++i) {
     const int q = q0 + i * kNodeWaves;
     if (q >= P.Q) break;  // uniform across the wave, and so for every later node
     const long row = static_cast<long>(q) * T;
-    double f = 0.0, y;
+    double f = 0.0, y = 0.0;
     if (t < T) {
       const double x = B.tile_x[static_cast<long>(q) * tiles + owner] + B.x_local[row + t];
       f = fraction_of(f0, x > 0.0 ? x : 0.0);
       y = (f * P.plasma_cmr) * field_at<C>(P, B, am, q, t);
-    } else {
-      auto sample = [&](int u) { return current_sample<C>(P, B, q, u, f0, tiles); };
-      y = reflected(T, K, t, sample);
+    } else if (t - T < K) {  // the last tile's lanes past the end: their reflections
+      y = halo[i * kBandSlots + H + (t - T)];
     }
     wave_sync();  // the previous node's window has been read
     window[K + lane] = y;
-    if (lane < H) window[halo_slot(K, lane)] = halo[i * H + lane];
+    if (lane < H) window[halo_slot(K, lane)] = halo[i * kBandSlots + lane];
     wave_sync();
     // Cross-lane moves with every lane of the wave active (see the rate pass).
     const double yf = band_filter(pt, window, lane);
