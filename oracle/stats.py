@@ -69,6 +69,13 @@ def peak_power(field: np.ndarray, kind: str) -> np.ndarray:
     return np.max(np.abs(analytic_field(field, kind)) ** 2, axis=-1)
 
 
+def power_trace(field: np.ndarray, kind: str) -> np.ndarray:
+    """The total power ``sum_modes |a_m(t)|^2``, the trace ``fwhm_t`` measures
+    (``definitions.py:865-923``; the native backend's packed trace,
+    ``native/src/general/statistics.cpp:80-93``)."""
+    return np.sum(np.abs(analytic_field(field, kind)) ** 2, axis=0)
+
+
 def peak_power_total(field: np.ndarray, kind: str) -> float:
     return float(np.max(np.sum(np.abs(analytic_field(field, kind)) ** 2, axis=0)))
 
@@ -119,4 +126,4 @@ def ionisation(problem, state, z):
 
 
 __all__ = ["analytic_field", "energy", "energy_window", "monitor_energies", "monitor_power",
-           "ionisation", "omega0", "peak_power", "peak_power_total", "split_sums"]
+           "ionisation", "omega0", "peak_power", "peak_power_total", "power_trace", "split_sums"]

@@ -796,7 +796,9 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
   if (!stats_)
     stats_ = std::make_unique<PlaneStatistics>(P_.C, P_.N, n_t_, true,
                                                P_.dt_fine * P_.T / n_t_, P_.omega);
-  std::vector<double> out = stats_->run(B_.y, flags & ~kStatPlasma, windows, n_windows, stream_);
+  std::vector<double> trace;
+  std::vector<double> out =
+      stats_->run(B_.y, flags & ~kStatPlasma, windows, n_windows, stream_, &trace);
   if (flags & kStatPlasma) {
     SP_CHECK(P_.has_plasma, "the electron density and the peak ionisation rate need a "
                             "compiled plasma response, and this problem declares none");
@@ -817,6 +819,7 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
     if (flags & kStatElectronDensity) out.push_back(density_(z_) * densest[0]);
     if (flags & kStatPeakIonisationRate) out.push_back(densest[1]);
   }
+  out.insert(out.end(), trace.begin(), trace.end());
   return out;
 }
 

@@ -106,6 +106,18 @@ __global__ void __launch_bounds__(kStatThreads)
   }
 }
 
+// The total power trace: sum_c |a_c(t)|^2 in column order at each sample.
+__global__ void stat_total_power(const cplx* a, int C, int n_t, double* trace) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_t) return;
+  double sum = 0.0;
+  for (int c = 0; c < C; ++c) {
+    const cplx z = a[static_cast<long>(c) * n_t + t];
+    sum = sum + (z.x * z.x + z.y * z.y);
+  }
+  trace[t] = sum;
+}
+
 // The monitor's five sums, one partial set a block: each thread takes samples t in
 // order of its stride, P(t) summed over the columns in order, then a fixed tree.
 __global__ void __launch_bounds__(kMonitorThreads)
@@ -302,10 +314,12 @@ PlaneStatistics::PlaneStatistics(int C, int N, int n_t, bool field, double dt,
   SP_HIP(hipMalloc(&sums_, static_cast<std::size_t>(C) * kSums * sizeof(double)));
   peak_blocks_ = (n_t + kStatThreads - 1) / kStatThreads;
   SP_HIP(hipMalloc(&peaks_, static_cast<std::size_t>(peak_blocks_) * (C + 1) * sizeof(double)));
+  SP_HIP(hipMalloc(&trace_, static_cast<std::size_t>(n_t) * sizeof(double)));
   SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_),
                        sizeof(double) * (static_cast<std::size_t>(C) * kSums +
                                          static_cast<std::size_t>(peak_blocks_) * (C + 1))));
   bytes_ = 2 * ct * sizeof(cplx) + static_cast<std::size_t>(C) * kSums * sizeof(double) +
+           static_cast<std::size_t>(n_t) * sizeof(double) +
            static_cast<std::size_t>(peak_blocks_) * (C + 1) * sizeof(double);
   plan_.init(n_t, C);
 }
@@ -316,11 +330,13 @@ PlaneStatistics::~PlaneStatistics() {
   if (windows_) (void)hipFree(windows_);
   if (sums_) (void)hipFree(sums_);
   if (peaks_) (void)hipFree(peaks_);
+  if (trace_) (void)hipFree(trace_);
   if (host_) (void)hipHostFree(host_);
 }
 
 std::vector<double> PlaneStatistics::run(const cplx* y, std::uint64_t flags,
-                                         const double* windows, int n_windows, hipStream_t s) {
+                                         const double* windows, int n_windows, hipStream_t s,
+                                         std::vector<double>* trace) {
   SP_CHECK((flags & ~kStatBuilt) == 0,
            "statistics built: omega0, energy, energy_window, peak_power, peak_power_total");
   SP_CHECK(n_windows >= 0 && n_windows <= kMaxStatWindows, "at most 16 statistics windows");
@@ -328,7 +344,8 @@ std::vector<double> PlaneStatistics::run(const cplx* y, std::uint64_t flags,
   for (int j = 0; j < n_windows; ++j)
     SP_CHECK(!(windows[2 * j] > windows[2 * j + 1]), "a statistics window's lo exceeds its hi");
   const bool spectral = flags & (kStatOmega0 | kStatEnergy | kStatEnergyWindow);
-  const bool peaks = flags & (kStatPeakPower | kStatPeakPowerTotal);
+  const bool peaks = flags & (kStatPeakPower | kStatPeakPowerTotal | kStatFwhm);
+  SP_CHECK(!(flags & kStatFwhm) || trace, "the power trace needs somewhere to go");
   const int used_windows = (flags & kStatEnergyWindow) ? n_windows : 0;
   const std::size_t n_sums = static_cast<std::size_t>(C_) * kSums;
   if (spectral) {
@@ -352,6 +369,14 @@ std::vector<double> PlaneStatistics::run(const cplx* y, std::uint64_t flags,
     SP_HIP(hipMemcpyAsync(host_ + n_sums, peaks_,
                           static_cast<std::size_t>(peak_blocks_) * (C_ + 1) * sizeof(double),
                           hipMemcpyDeviceToHost, s));
+    if (flags & kStatFwhm) {
+      hipLaunchKernelGGL(stat_total_power, dim3((n_t_ + kStatThreads - 1) / kStatThreads),
+                         dim3(kStatThreads), 0, s, time_, C_, n_t_, trace_);
+      SP_HIP(hipGetLastError());
+      trace->assign(n_t_, 0.0);
+      SP_HIP(hipMemcpyAsync(trace->data(), trace_, static_cast<std::size_t>(n_t_) * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    }
   }
   SP_HIP(hipStreamSynchronize(s));
   std::vector<double> out;

@@ -21,6 +21,9 @@ enum : std::uint64_t {
   kStatEnergyWindow = 1ULL << 2U,
   kStatPeakPower = 1ULL << 3U,
   kStatPeakPowerTotal = 1ULL << 5U,
+  // The total power trace sum_modes |a_m(t)|^2 (n_t values; the host measures the width,
+  // native/src/general/statistics.cpp:80-93,291-294), packed after the plasma pair.
+  kStatFwhm = 1ULL << 6U,
   // The field routes' plasma column at the committed state (read by the workspace).
   kStatElectronDensity = 1ULL << 7U,
   kStatPeakIonisationRate = 1ULL << 8U,
@@ -30,8 +33,8 @@ constexpr std::uint64_t kStatPlasma = kStatElectronDensity | kStatPeakIonisation
 // its largest rate: out[0] that fraction, out[1] the rate (numpy_ref/statistics.py:502-546).
 void launch_plasma_densest(const double* frac, const double* rate, int Q, int T, double* out,
                            hipStream_t s);
-constexpr std::uint64_t kStatBuilt =
-    kStatOmega0 | kStatEnergy | kStatEnergyWindow | kStatPeakPower | kStatPeakPowerTotal;
+constexpr std::uint64_t kStatBuilt = kStatOmega0 | kStatEnergy | kStatEnergyWindow |
+                                     kStatPeakPower | kStatPeakPowerTotal | kStatFwhm;
 constexpr int kMaxStatWindows = 16;
 
 // The boundary monitor's sums (src/solprop/monitor.py:493-606) over the time samples of
@@ -93,9 +96,11 @@ class PlaneStatistics {
   PlaneStatistics& operator=(const PlaneStatistics&) = delete;
   // The requested statistics of the state y (C, N) in the reference's packing order:
   // omega0 (C), energy (C), energy_window (n_windows x C, window-major), peak_power (C),
-  // peak_power_total (1). `windows` holds n_windows (lo, hi) pairs in rad/s.
+  // peak_power_total (1). `windows` holds n_windows (lo, hi) pairs in rad/s. With
+  // kStatFwhm the total power trace (n_t) goes to `trace`, which the caller packs after
+  // the statistics that precede it.
   std::vector<double> run(const cplx* y, std::uint64_t flags, const double* windows,
-                          int n_windows, hipStream_t s);
+                          int n_windows, hipStream_t s, std::vector<double>* trace = nullptr);
   std::size_t bytes() const { return bytes_ + plan_.bytes(); }
   // The monitor's five sums of the state y (kMonitorReading) into acc (device, 5).
   void reading(const cplx* y, const double* columns, const double* weights, double* partials,
@@ -108,7 +113,7 @@ class PlaneStatistics {
   const double* omega_;
   FftPlan plan_;
   cplx *time_ = nullptr, *scratch_ = nullptr;
-  double *windows_ = nullptr, *sums_ = nullptr, *peaks_ = nullptr;
+  double *windows_ = nullptr, *sums_ = nullptr, *peaks_ = nullptr, *trace_ = nullptr;
   double* host_ = nullptr;
   int peak_blocks_ = 0;
   std::size_t bytes_ = 0;

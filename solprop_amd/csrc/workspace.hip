@@ -481,7 +481,10 @@ std::vector<double> Workspace::statistics(std::uint64_t flags, const double* win
   if (!stats_)
     stats_ = std::make_unique<PlaneStatistics>(P_.C, P_.N, P_.N, false,
                                                P_.dt_fine * P_.T / P_.N, P_.omega);
-  return stats_->run(y_, flags, windows, n_windows, stream_);
+  std::vector<double> trace;
+  std::vector<double> out = stats_->run(y_, flags, windows, n_windows, stream_, &trace);
+  out.insert(out.end(), trace.begin(), trace.end());
+  return out;
 }
 
 sp_counters Workspace::counters() const {

@@ -289,7 +289,8 @@ class Workspace:
     # the statistics this library forms, in their packing order.
     STATISTICS = {"omega0": 1 << 0, "energy": 1 << 1, "energy_window": 1 << 2,
                   "peak_power": 1 << 3, "peak_power_total": 1 << 5,
-                  "electron_density": 1 << 7, "peak_ionisation_rate": 1 << 8}
+                  "fwhm": 1 << 6, "electron_density": 1 << 7,
+                  "peak_ionisation_rate": 1 << 8}
 
     def statistics(self, names, windows=None) -> dict:
         """Plane statistics of the committed state (``stats/definitions.py``): a dict of
@@ -303,7 +304,7 @@ class Workspace:
             flags |= self.STATISTICS[name]
         w = np.zeros((0, 2)) if windows is None else np.ascontiguousarray(windows, np.float64)
         n_modes = self.problem.n_modes
-        capacity = 3 * n_modes + len(w) * n_modes + 3
+        capacity = 3 * n_modes + len(w) * n_modes + 3 + self.problem.n_t
         out = np.zeros(capacity)
         count = C.c_int()
         self._call("sp_workspace_statistics", flags, _ptr(w) if len(w) else None, len(w),
@@ -311,9 +312,13 @@ class Workspace:
         values, at, result = out[: count.value], 0, {}
         sizes = {"omega0": n_modes, "energy": n_modes, "energy_window": len(w) * n_modes,
                  "peak_power": n_modes, "peak_power_total": 1, "electron_density": 1,
-                 "peak_ionisation_rate": 1}
+                 "peak_ionisation_rate": 1, "fwhm": self.problem.n_t}
+        # The packing order: the flags' order, except that the power trace (the fwhm
+        # flag's) follows the plasma pair (native/src/general/statistics.cpp:160-177).
+        order = ["omega0", "energy", "energy_window", "peak_power", "peak_power_total",
+                 "electron_density", "peak_ionisation_rate", "fwhm"]
         scalars = ("peak_power_total", "electron_density", "peak_ionisation_rate")
-        for name in sorted(self.STATISTICS, key=self.STATISTICS.get):
+        for name in order:
             if name in names:
                 chunk = values[at: at + sizes[name]]
                 at += sizes[name]
