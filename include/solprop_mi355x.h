@@ -188,14 +188,15 @@ int sp_workspace_export(sp_workspace* ws, const double* power_weight, int optics
  * native/src/general/statistics.cpp:106-360; the definitions of
  * src/solprop/stats/definitions.py:268-520). `flags` are StatisticsFlag's bits
  * (native/include/solprop_native/problem/problem.hpp:44-60); built: omega0 (1 << 0),
- * energy (1 << 1), energy_window (1 << 2), peak_power (1 << 3), peak_power_total
- * (1 << 5), fwhm (1 << 6), on the modal power weight of one, and on the field routes electron_density
+ * energy (1 << 1), energy_window (1 << 2), peak_power (1 << 3), peak_power_window
+ * (1 << 4), peak_power_total (1 << 5), fwhm (1 << 6), on the modal power weight of one, and on the field routes electron_density
  * (1 << 7) and peak_ionisation_rate (1 << 8): density(z) times the end-of-window
  * fraction of the node of largest fraction, and that node's largest rate
  * (numpy_ref/statistics.py:502-546), from the plasma column at the committed state. `windows` holds `n_windows` (lo, hi)
  * pairs in rad/s, gated lo <= omega < hi. `out` receives, in this order and for the
  * flags set: omega0 (C; NaN for an empty column), energy (C), energy_window
- * (n_windows x C), peak_power (C), peak_power_total (1), electron_density (1),
+ * (n_windows x C), peak_power (C), peak_power_window (n_windows x C),
+ * peak_power_total (1), electron_density (1),
  * peak_ionisation_rate (1), and for fwhm (1 << 6) the total power trace
  * sum_modes |a_m(t)|^2 (n_t; the host measures the width, as from the native backend's
  * trace); `count` the number written.

@@ -76,6 +76,14 @@ def power_trace(field: np.ndarray, kind: str) -> np.ndarray:
     return np.sum(np.abs(analytic_field(field, kind)) ** 2, axis=0)
 
 
+def peak_power_window(field, omega, windows, kind):
+    """``peak_power_window`` (``definitions.py:725-760``): the spectrum gated by each
+    hard half-open window, then each mode's peak power."""
+    windows = np.asarray(windows, dtype=float).reshape(-1, 2)
+    masks = (omega[None, :] >= windows[:, :1]) & (omega[None, :] < windows[:, 1:])
+    return np.stack([peak_power(field * m, kind) for m in masks])
+
+
 def peak_power_total(field: np.ndarray, kind: str) -> float:
     return float(np.max(np.sum(np.abs(analytic_field(field, kind)) ** 2, axis=0)))
 
@@ -126,4 +134,4 @@ def ionisation(problem, state, z):
 
 
 __all__ = ["analytic_field", "energy", "energy_window", "monitor_energies", "monitor_power",
-           "ionisation", "omega0", "peak_power", "peak_power_total", "power_trace", "split_sums"]
+           "ionisation", "omega0", "peak_power", "peak_power_total", "peak_power_window", "power_trace", "split_sums"]

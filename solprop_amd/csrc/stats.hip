@@ -61,16 +61,19 @@ __global__ void __launch_bounds__(kStatThreads)
 // inverse transform: a field grid's one-sided bins doubled except DC and Nyquist, the
 // negative frequencies zero (src/solprop/fourier.py:243-300); an envelope's bins as
 // they are (src/solprop/stats/definitions.py:148-178).
+// With `gate` (lo, hi) given, the spectrum is first gated to lo <= omega < hi, as
+// peak_power_window gates it (definitions.py:725-760).
 __global__ void stat_analytic_pack(const cplx* y, int C, int N, int n_t, int field,
-                                   cplx* out) {
+                                   const double* omega, const double* gate, cplx* out) {
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= static_cast<long>(C) * n_t) return;
   const int c = static_cast<int>(idx / n_t), k = static_cast<int>(idx % n_t);
   const double inv = 1.0 / n_t;
   cplx v = make(0.0, 0.0);
+  const bool open = k < N && (!gate || (omega[k] >= gate[0] && omega[k] < gate[1]));
   if (!field) {
-    if (k < N) v = scale(inv, y[static_cast<long>(c) * N + k]);
-  } else if (k <= n_t / 2 && k < N) {
+    if (open) v = scale(inv, y[static_cast<long>(c) * N + k]);
+  } else if (k <= n_t / 2 && open) {
     const double h = (k == 0 || k == n_t / 2) ? 1.0 : 2.0;
     v = scale(h * inv, y[static_cast<long>(c) * N + k]);
   }
@@ -293,7 +296,8 @@ void PlaneStatistics::reading(const cplx* y, const double* columns, const double
   // analytic signal (a field, whose real part is the field itself).
   const std::size_t ct = static_cast<std::size_t>(C_) * n_t_;
   hipLaunchKernelGGL(stat_analytic_pack, dim3((ct + kStatThreads - 1) / kStatThreads),
-                     dim3(kStatThreads), 0, s, y, C_, N_, n_t_, field_ ? 1 : 0, time_);
+                     dim3(kStatThreads), 0, s, y, C_, N_, n_t_, field_ ? 1 : 0, omega_,
+                     nullptr, time_);
   SP_HIP(hipGetLastError());
   plan_.exec(time_, scratch_, +1, s);
   launch_monitor_sums(reinterpret_cast<const double*>(time_),
@@ -360,7 +364,8 @@ std::vector<double> PlaneStatistics::run(const cplx* y, std::uint64_t flags,
   if (peaks) {
     const std::size_t ct = static_cast<std::size_t>(C_) * n_t_;
     hipLaunchKernelGGL(stat_analytic_pack, dim3((ct + kStatThreads - 1) / kStatThreads),
-                       dim3(kStatThreads), 0, s, y, C_, N_, n_t_, field_ ? 1 : 0, time_);
+                       dim3(kStatThreads), 0, s, y, C_, N_, n_t_, field_ ? 1 : 0, omega_,
+                     nullptr, time_);
     SP_HIP(hipGetLastError());
     plan_.exec(time_, scratch_, +1, s);
     hipLaunchKernelGGL(stat_peak_blocks, dim3(peak_blocks_), dim3(kStatThreads), 0, s, time_,
@@ -376,6 +381,34 @@ std::vector<double> PlaneStatistics::run(const cplx* y, std::uint64_t flags,
       trace->assign(n_t_, 0.0);
       SP_HIP(hipMemcpyAsync(trace->data(), trace_, static_cast<std::size_t>(n_t_) * sizeof(double),
                             hipMemcpyDeviceToHost, s));
+    }
+  }
+  // Each window's gated spectrum through its own transform: the peak of every column.
+  std::vector<double> window_peaks;
+  if ((flags & kStatPeakPowerWindow) && n_windows > 0) {
+    if (!(flags & kStatEnergyWindow))
+      SP_HIP(hipMemcpyAsync(windows_, windows, 2 * n_windows * sizeof(double),
+                            hipMemcpyHostToDevice, s));
+    const std::size_t ct = static_cast<std::size_t>(C_) * n_t_;
+    const std::size_t per = static_cast<std::size_t>(peak_blocks_) * (C_ + 1);
+    std::vector<double> blocks(per);
+    for (int j = 0; j < n_windows; ++j) {
+      hipLaunchKernelGGL(stat_analytic_pack, dim3((ct + kStatThreads - 1) / kStatThreads),
+                         dim3(kStatThreads), 0, s, y, C_, N_, n_t_, field_ ? 1 : 0, omega_,
+                         windows_ + 2 * j, time_);
+      SP_HIP(hipGetLastError());
+      plan_.exec(time_, scratch_, +1, s);
+      hipLaunchKernelGGL(stat_peak_blocks, dim3(peak_blocks_), dim3(kStatThreads), 0, s, time_,
+                         C_, n_t_, peaks_);
+      SP_HIP(hipGetLastError());
+      SP_HIP(hipMemcpyAsync(blocks.data(), peaks_, per * sizeof(double), hipMemcpyDeviceToHost,
+                            s));
+      SP_HIP(hipStreamSynchronize(s));
+      for (int c = 0; c < C_; ++c) {
+        double m = 0.0;
+        for (int b = 0; b < peak_blocks_; ++b) m = std::fmax(m, blocks[b * (C_ + 1) + c]);
+        window_peaks.push_back(m);
+      }
     }
   }
   SP_HIP(hipStreamSynchronize(s));
@@ -406,6 +439,7 @@ std::vector<double> PlaneStatistics::run(const cplx* y, std::uint64_t flags,
   };
   if (flags & kStatPeakPower)
     for (int c = 0; c < C_; ++c) out.push_back(peak_of(c));
+  out.insert(out.end(), window_peaks.begin(), window_peaks.end());
   if (flags & kStatPeakPowerTotal) out.push_back(peak_of(C_));
   return out;
 }

@@ -20,6 +20,7 @@ enum : std::uint64_t {
   kStatEnergy = 1ULL << 1U,
   kStatEnergyWindow = 1ULL << 2U,
   kStatPeakPower = 1ULL << 3U,
+  kStatPeakPowerWindow = 1ULL << 4U,
   kStatPeakPowerTotal = 1ULL << 5U,
   // The total power trace sum_modes |a_m(t)|^2 (n_t values; the host measures the width,
   // native/src/general/statistics.cpp:80-93,291-294), packed after the plasma pair.
@@ -34,7 +35,8 @@ constexpr std::uint64_t kStatPlasma = kStatElectronDensity | kStatPeakIonisation
 void launch_plasma_densest(const double* frac, const double* rate, int Q, int T, double* out,
                            hipStream_t s);
 constexpr std::uint64_t kStatBuilt = kStatOmega0 | kStatEnergy | kStatEnergyWindow |
-                                     kStatPeakPower | kStatPeakPowerTotal | kStatFwhm;
+                                     kStatPeakPower | kStatPeakPowerWindow |
+                                     kStatPeakPowerTotal | kStatFwhm;
 constexpr int kMaxStatWindows = 16;
 
 // The boundary monitor's sums (src/solprop/monitor.py:493-606) over the time samples of
@@ -96,7 +98,7 @@ class PlaneStatistics {
   PlaneStatistics& operator=(const PlaneStatistics&) = delete;
   // The requested statistics of the state y (C, N) in the reference's packing order:
   // omega0 (C), energy (C), energy_window (n_windows x C, window-major), peak_power (C),
-  // peak_power_total (1). `windows` holds n_windows (lo, hi) pairs in rad/s. With
+  // peak_power_window (n_windows x C), peak_power_total (1). `windows` holds n_windows (lo, hi) pairs in rad/s. With
   // kStatFwhm the total power trace (n_t) goes to `trace`, which the caller packs after
   // the statistics that precede it.
   std::vector<double> run(const cplx* y, std::uint64_t flags, const double* windows,

@@ -288,7 +288,8 @@ class Workspace:
     # StatisticsFlag's bits (native/include/solprop_native/problem/problem.hpp:44-60) for
     # the statistics this library forms, in their packing order.
     STATISTICS = {"omega0": 1 << 0, "energy": 1 << 1, "energy_window": 1 << 2,
-                  "peak_power": 1 << 3, "peak_power_total": 1 << 5,
+                  "peak_power": 1 << 3, "peak_power_window": 1 << 4,
+                  "peak_power_total": 1 << 5,
                   "fwhm": 1 << 6, "electron_density": 1 << 7,
                   "peak_ionisation_rate": 1 << 8}
 
@@ -304,25 +305,27 @@ class Workspace:
             flags |= self.STATISTICS[name]
         w = np.zeros((0, 2)) if windows is None else np.ascontiguousarray(windows, np.float64)
         n_modes = self.problem.n_modes
-        capacity = 3 * n_modes + len(w) * n_modes + 3 + self.problem.n_t
+        capacity = 3 * n_modes + 2 * len(w) * n_modes + 3 + self.problem.n_t
         out = np.zeros(capacity)
         count = C.c_int()
         self._call("sp_workspace_statistics", flags, _ptr(w) if len(w) else None, len(w),
                    _ptr(out), capacity, C.byref(count))
         values, at, result = out[: count.value], 0, {}
         sizes = {"omega0": n_modes, "energy": n_modes, "energy_window": len(w) * n_modes,
-                 "peak_power": n_modes, "peak_power_total": 1, "electron_density": 1,
+                 "peak_power": n_modes, "peak_power_window": len(w) * n_modes,
+                 "peak_power_total": 1, "electron_density": 1,
                  "peak_ionisation_rate": 1, "fwhm": self.problem.n_t}
         # The packing order: the flags' order, except that the power trace (the fwhm
         # flag's) follows the plasma pair (native/src/general/statistics.cpp:160-177).
-        order = ["omega0", "energy", "energy_window", "peak_power", "peak_power_total",
-                 "electron_density", "peak_ionisation_rate", "fwhm"]
+        order = ["omega0", "energy", "energy_window", "peak_power", "peak_power_window",
+                 "peak_power_total", "electron_density", "peak_ionisation_rate", "fwhm"]
         scalars = ("peak_power_total", "electron_density", "peak_ionisation_rate")
         for name in order:
             if name in names:
                 chunk = values[at: at + sizes[name]]
                 at += sizes[name]
-                result[name] = (chunk.reshape(len(w), n_modes) if name == "energy_window"
+                windowed = name in ("energy_window", "peak_power_window")
+                result[name] = (chunk.reshape(len(w), n_modes) if windowed
                                 else float(chunk[0]) if name in scalars else chunk)
         return result
 

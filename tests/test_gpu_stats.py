@@ -16,7 +16,8 @@ from solprop_amd.problem import capillary_problem, gnlse_problem, radial_problem
 
 pytestmark = pytest.mark.gpu
 
-NAMES = ["omega0", "energy", "energy_window", "peak_power", "peak_power_total", "fwhm"]
+NAMES = ["omega0", "energy", "energy_window", "peak_power", "peak_power_window",
+         "peak_power_total", "fwhm"]
 
 PROBLEMS = {
     "capillary": lambda: capillary_problem(n_t=2048, n_nodes=32, length=0.05,
@@ -37,6 +38,7 @@ def oracle_statistics(problem, state, windows):
         "energy": S.energy(field, dt, kind),
         "energy_window": S.energy_window(field, problem.omega, windows, dt, kind),
         "peak_power": S.peak_power(field, kind),
+        "peak_power_window": S.peak_power_window(field, problem.omega, windows, kind),
         "peak_power_total": S.peak_power_total(field, kind),
         "fwhm": S.power_trace(field, kind),
     }
@@ -65,7 +67,7 @@ def test_statistics_match_oracle(route, marched):
     for name in ("omega0", "energy", "energy_window"):
         err = np.abs(np.asarray(got[name]) - expected[name]).max()
         assert err <= 1e-13 * np.abs(expected[name]).max(), (name, err)
-    for name in ("peak_power", "peak_power_total", "fwhm"):
+    for name in ("peak_power", "peak_power_window", "peak_power_total", "fwhm"):
         err = np.abs(np.asarray(got[name]) - expected[name]).max()
         assert err <= 1e-12 * np.abs(expected[name]).max(), (name, err)
     # The windows tile the active band: their energies sum to the energy there.
