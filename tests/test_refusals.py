@@ -57,3 +57,14 @@ def test_transverse_collar_off_the_dense_route_is_refused():
     p = capillary_problem(n_t=256, n_nodes=4, integration="trapezoid")
     refused(dataclasses.replace(p, spatial_absorber=np.zeros((p.n_modes, p.n_modes))),
             "dense route")
+
+
+def test_unbuilt_statistics_are_refused_by_name():
+    """A statistic this library does not form is refused before the library is asked
+    (the quadrature estimate, populations); the refusal names it."""
+    from solprop_amd.native import Workspace as W
+
+    ws = W.__new__(W)  # no device: the check precedes every library call
+    ws.problem = capillary_problem(n_t=256, n_nodes=4, integration="trapezoid")
+    with pytest.raises(BackendRefusal, match="populations"):
+        W.statistics(ws, ["energy", "populations"])
