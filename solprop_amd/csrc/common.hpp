@@ -28,6 +28,15 @@ __host__ __device__ inline cplx rot(cplx a, int sign) {
   return sign < 0 ? make(a.y, -a.x) : make(-a.y, a.x);
 }
 
+// A real inverse transform of 2M points as a complex one of M: the pair of half-spectrum
+// bins (k, M - k) packed as Z = (Ya + conj Yb) + i conj(W) (Ya - conj Yb), W = W_2M^k
+// (twice E + iO of the even and odd samples' transforms).
+__host__ __device__ inline cplx c2r_pre(cplx ya, cplx yb, cplx w) {
+  const cplx s = add(ya, conj(yb));
+  const cplx d = sub(ya, conj(yb));
+  return add(s, rot(mul(conj(w), d), +1));
+}
+
 // exp(z) in real form, as the reference requires (no complex exp): exp(re)(cos, sin).
 __device__ inline cplx cexp(cplx z) {
   double s, c;

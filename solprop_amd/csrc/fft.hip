@@ -12,7 +12,31 @@ struct PassArgs {
   long in_sj, in_ss, out_sk, out_ss, batch_in, batch_out;
   int global_twiddle;
   int mask_n;  // N - 1 for the global twiddle index
+  // When `spec` is set, the first pass loads its input from a half spectrum instead:
+  // row r's point k is c2r_pre(X[k], X[M - k], W_2M^k) of X = spec row r (spec_n bins,
+  // zero beyond) scaled by spec_scale, DC and Nyquist taken real (the packing of a real
+  // inverse transform of 2M points as a complex one of M).
+  const cplx* spec;
+  const cplx* spec_roots;  // W_2M^k, k < M
+  int spec_n;
+  double spec_scale;
+  long spec_m;             // M, the points of a row
 };
+
+// The packed point at global index g (row g / M, point g % M) from the half spectrum.
+__device__ inline cplx packed_point(const PassArgs& a, long g) {
+  const long M = a.spec_m;
+  const long r = g / M, k = g - r * M;
+  const cplx* row = a.spec + r * a.spec_n;
+  const long kb = M - k;
+  cplx xa = k < a.spec_n ? scale(a.spec_scale, row[k]) : make(0.0, 0.0);
+  cplx xb = kb < a.spec_n ? scale(a.spec_scale, row[kb]) : make(0.0, 0.0);
+  if (k == 0) {  // DC and Nyquist are real in a real transform (as irfft discards)
+    xa.y = 0.0;
+    xb.y = 0.0;
+  }
+  return c2r_pre(xa, xb, a.spec_roots[k]);
+}
 
 template <int LOG2L, int G>
 struct Shape {
@@ -52,7 +76,9 @@ __global__ void __launch_bounds__((Shape<LOG2L, G>::NT))
       j = e % L;
       s = e / L;
     }
-    buf[s * STRIDE + j] = src[j * a.in_sj + (s0 + s) * a.in_ss];
+    const long at = j * a.in_sj + (s0 + s) * a.in_ss;
+    buf[s * STRIDE + j] =
+        a.spec ? packed_point(a, static_cast<long>(blockIdx.y) * a.batch_in + at) : src[at];
   }
   __syncthreads();
 
@@ -316,13 +342,37 @@ void FftPlan::launch(const FftPass& p, const cplx* in, cplx* out, int sign,
   KernelFn fn = kernel_for(p.log2L, log2g);
   const int g = 1 << log2g;
   PassArgs a{p.in_sj, p.in_ss, p.out_sk, p.out_ss, p.batch_in, p.batch_out,
-             p.global_twiddle ? 1 : 0, static_cast<int>((1L << p.log2N) - 1)};
+             p.global_twiddle ? 1 : 0, static_cast<int>((1L << p.log2N) - 1),
+             nullptr, nullptr, 0, 0.0, static_cast<long>(n_)};
+  if (spec_) {
+    a.spec = spec_->spectrum;
+    a.spec_roots = spec_->roots;
+    a.spec_n = spec_->n_bins;
+    a.spec_scale = spec_->scale;
+  }
   const unsigned groups = static_cast<unsigned>(p.n_seq / g);
   const unsigned rows = p.batch_in == 0 && p.batch_out == 0 ? 1u : static_cast<unsigned>(batch_);
   const cplx* tw_l = (&p == &passes_[0]) ? roots_l_[0] : roots_l_[1];
   hipLaunchKernelGGL(fn, dim3(groups, rows), dim3(threads_for(p.log2L, log2g)), 0, stream,
                      in, out, tw_l, roots_n_, a, sign);
   SP_HIP(hipGetLastError());
+}
+
+void FftPlan::exec_packed(const PackedSpectrum& spectrum, cplx* data, cplx* scratch, int sign,
+                          hipStream_t stream) const {
+  SP_CHECK(batch_ >= 1 && spectrum.spectrum && spectrum.roots &&
+               spectrum.n_bins >= 1 && spectrum.n_bins <= static_cast<long>(n_) + 1,
+           "a packed spectrum of at most M + 1 bins a row");
+  // Only the first pass loads: a row of M points is the whole input of each row.
+  spec_ = &spectrum;
+  if (passes_.size() == 1) {
+    launch(passes_[0], data, data, sign, stream);
+  } else {
+    launch(passes_[0], data, scratch, sign, stream);
+    spec_ = nullptr;
+    launch(passes_[1], scratch, data, sign, stream);
+  }
+  spec_ = nullptr;
 }
 
 void FftPlan::exec(cplx* data, cplx* scratch, int sign, hipStream_t stream) const {

@@ -30,6 +30,15 @@ struct FftPass {
   int log2N = 0;            // length of the global twiddle table
 };
 
+// A half spectrum for FftPlan::exec_packed: `n_bins` bins a row (rows contiguous),
+// scaled by `scale`, with the unit roots W_2M^k of the real transform of 2M points.
+struct PackedSpectrum {
+  const cplx* spectrum;
+  const cplx* roots;
+  long n_bins;
+  double scale;
+};
+
 class FftPlan {
  public:
   FftPlan() = default;
@@ -41,6 +50,10 @@ class FftPlan {
   void init(std::size_t n, std::size_t batch);
   // In place on data (rows of n); `scratch` holds n*batch points for the four-step.
   void exec(cplx* data, cplx* scratch, int sign, hipStream_t stream) const;
+  // The same transform with its input packed from a half spectrum on the way in (the
+  // first pass reads the spectrum; `data` need not hold anything), the result in `data`.
+  void exec_packed(const PackedSpectrum& spectrum, cplx* data, cplx* scratch, int sign,
+                   hipStream_t stream) const;
   std::size_t n() const { return n_; }
   std::size_t batch() const { return batch_; }
   bool two_pass() const { return passes_.size() == 2; }
@@ -52,6 +65,7 @@ class FftPlan {
   void launch(const FftPass& p, const cplx* in, cplx* out, int sign, hipStream_t s) const;
   std::size_t n_ = 0, batch_ = 0, bytes_ = 0;
   std::vector<FftPass> passes_;
+  mutable const PackedSpectrum* spec_ = nullptr;  // set for the first pass of exec_packed
   cplx* roots_n_ = nullptr;   // length n
   cplx* roots_l_[2] = {nullptr, nullptr};  // per-pass local tables, length L
 };

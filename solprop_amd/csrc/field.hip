@@ -128,11 +128,6 @@ __device__ inline cplx r2c_post(cplx za, cplx zb, cplx w) {
   return add(e, mul(w, o));
 }
 
-__device__ inline cplx c2r_pre(cplx ya, cplx yb, cplx w) {
-  const cplx s = add(ya, conj(yb));
-  const cplx d = sub(ya, conj(yb));
-  return add(s, rot(mul(conj(w), d), +1));
-}
 
 // The rate table (src/solprop/translate/ionisation.py:242-313): exactly zero below
 // field_min, held above field_max, exp(cubic in log|E|) between. Evaluated without

@@ -342,9 +342,10 @@ void FieldWorkspace::enqueue_raman(const cplx* spectrum) {
 }
 
 void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
-  launch_field_scatter(P_, spectrum, B_, stream_);
   mark(2);
-  inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
+  // The half spectrum is packed for the real inverse transform as its first pass loads it.
+  inv_plan_.exec_packed(PackedSpectrum{spectrum, B_.roots_T, P_.N, P_.inv_T}, B_.modal_t,
+                        scratch_, +1, stream_);
   mark(3);
   enqueue_synthesis();
   mark(4);
@@ -461,6 +462,7 @@ bool FieldWorkspace::phase_used(int i) const {
     case 3:   // synthesis_gemm
     case 10:  // projection_gemm
       return dense_;
+    case 1: return false;  // the packing rides the inverse transform's first pass
     case 4: return P_.n_raman > 0;
     case 5: case 6: case 7: case 8: return P_.has_plasma != 0;
     case 11: return absorbing();
@@ -712,8 +714,8 @@ void FieldWorkspace::source(double z, const double* lab, double* out, double* st
   SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
                         stream_));
   marking_ = false;
-  launch_field_scatter(P_, B_.lab, B_, stream_);
-  inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
+  inv_plan_.exec_packed(PackedSpectrum{B_.lab, B_.roots_T, P_.N, P_.inv_T}, B_.modal_t,
+                        scratch_, +1, stream_);
   enqueue_synthesis();
   enqueue_raman(B_.lab);
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_);

@@ -49,13 +49,6 @@ __device__ inline cplx r2c_post(cplx za, cplx zb, cplx w) {
   return add(e, mul(w, o));
 }
 
-__device__ inline cplx c2r_pre(cplx ya, cplx yb, cplx w) {
-  // Z = (Ya + conj Yb) + i conj(W) (Ya - conj Yb): twice E + iO of the packed pair.
-  const cplx s = add(ya, conj(yb));
-  const cplx d = sub(ya, conj(yb));
-  return add(s, rot(mul(conj(w), d), +1));
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------- stage entry
