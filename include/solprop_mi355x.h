@@ -176,6 +176,18 @@ typedef struct sp_march_result {
 int sp_workspace_advance(sp_workspace* ws, double target, long work_limit, double* rows,
                          long max_rows, sp_march_result* out, char* msg, size_t msglen);
 
+/* The batched variant for scans of independent instances (the process pools of
+ * src/solprop/scan.py): `n` workspaces advanced concurrently to their own targets, one
+ * host thread a workspace, each on its workspace's own stream, so that the instances'
+ * launches and host decisions overlap on the one GPU. Each instance's march is the
+ * one sp_workspace_advance would make (the same decisions and results, to the bit).
+ * `rows` holds n blocks of max_rows x 4 (end, step, norm, accepted), `out` n results,
+ * `status` n codes; the call returns SP_OK when every instance did, else the first
+ * failing instance's code, with its message in `msg`. */
+int sp_batch_advance(sp_workspace* const* ws, int n, const double* targets, long work_limit,
+                     double* rows, long max_rows, sp_march_result* out, int* status, char* msg,
+                     size_t msglen);
+
 /* `Session.export(ticket, recipe, out)` (native/src/solprop_native/session.py:960;
  * src/solprop/backends/export.py:52-207): the committed state converted by the recipe,
  * to the bit: first the product with sqrt(w) when `power_weight` (complex-state shape

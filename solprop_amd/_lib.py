@@ -150,6 +150,8 @@ SIGNATURES = {
     "sp_testing_buffer": (_I, [C.c_void_p, C.c_char_p, _PD, C.c_long, C.POINTER(C.c_long),
                                C.c_char_p, C.c_size_t]),
     "sp_workspace_phases": (_I, [C.c_void_p, _PD, _I, C.POINTER(_I), C.c_char_p, C.c_size_t]),
+    "sp_batch_advance": (_I, [C.POINTER(C.c_void_p), _I, _PD, C.c_long, _PD, C.c_long,
+                              C.c_void_p, C.POINTER(_I), C.c_char_p, C.c_size_t]),
     "sp_workspace_bind_monitor": (_I, [C.c_void_p, _PD, _PD, C.c_char_p, C.c_size_t]),
     "sp_workspace_monitor_reading": (_I, [C.c_void_p, _PD, C.c_char_p, C.c_size_t]),
     "sp_workspace_statistics": (_I, [C.c_void_p, C.c_ulonglong, _PD, _I, _PD, _I, C.POINTER(_I),

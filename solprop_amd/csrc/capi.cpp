@@ -4,6 +4,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "field_ws.hpp"
@@ -108,6 +109,31 @@ int sp_workspace_advance(sp_workspace* ws, double target, long work_limit, doubl
   if (kept > 0) std::memcpy(rows, held.data(), sizeof(double) * 4 * kept);
   out->n_rows = static_cast<int>(kept);
   return status;
+}
+
+int sp_batch_advance(sp_workspace* const* ws, int n, const double* targets, long work_limit,
+                     double* rows, long max_rows, sp_march_result* out, int* status, char* msg,
+                     size_t msglen) {
+  if (!ws || n < 1 || !targets || !out || !status) return SP_VALUE;
+  for (int i = 0; i < n; ++i)
+    if (!ws[i]) return SP_VALUE;
+  std::vector<std::vector<char>> messages(n, std::vector<char>(msglen > 0 ? msglen : 1, 0));
+  std::vector<std::thread> threads;
+  threads.reserve(n);
+  for (int i = 0; i < n; ++i)
+    threads.emplace_back([&, i] {
+      status[i] = sp_workspace_advance(ws[i], targets[i], work_limit,
+                                       rows ? rows + 4 * max_rows * i : nullptr,
+                                       rows ? max_rows : 0, out + i, messages[i].data(),
+                                       messages[i].size());
+    });
+  for (auto& t : threads) t.join();
+  for (int i = 0; i < n; ++i)
+    if (status[i] != SP_OK) {
+      write_message(msg, msglen, messages[i].data());
+      return status[i];
+    }
+  return SP_OK;
 }
 
 int sp_workspace_export(sp_workspace* ws, const double* power_weight, int optics,

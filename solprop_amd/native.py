@@ -406,6 +406,29 @@ class Workspace:
         return dict(zip(labels, us[: count.value].tolist(), strict=True))
 
 
+def advance_batch(sessions, targets, work_limit: int = 256) -> list:
+    """Advance independent sessions concurrently (``sp_batch_advance``): one host thread
+    a workspace, each on its own stream. Each session's march is the one
+    :meth:`Session.advance` would make. Returns each session's :class:`Batch`."""
+    n = len(sessions)
+    targets = np.ascontiguousarray(np.broadcast_to(np.asarray(targets, float), (n,)))
+    handles = (C.c_void_p * n)(*[s._ws._handle for s in sessions])
+    rows = np.zeros((n, work_limit, 4))
+    results = (_lib.MarchResultC * n)()
+    status = (C.c_int * n)()
+    msg = C.create_string_buffer(1024)
+    lib = sessions[0]._ws._lib
+    _check(lib.sp_batch_advance(handles, n, _ptr(targets), int(work_limit), _ptr(rows),
+                                int(work_limit), C.cast(results, C.c_void_p), status, msg,
+                                1024), msg)
+    out = []
+    for i, session in enumerate(sessions):
+        r = results[i]
+        session.z = r.z
+        out.append(Batch(r.z, bool(r.done), rows[i, : r.n_rows].copy(), r.evaluations))
+    return out
+
+
 class Probe:
     """One trial from a loaded state, committing nothing (``_probe.py:33-70``)."""
 

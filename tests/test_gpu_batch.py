@@ -1,0 +1,48 @@
+"""GPU: the batched advance of independent instances (a scan's points) through the C
+ABI. Each instance's march under ``sp_batch_advance`` must be the one its own
+``sp_workspace_advance`` makes: the same attempts and the same end state, to the bit,
+whatever the other instances do on the GPU meanwhile (each has its own stream and
+graphs; the decisions are the host's, per instance).
+"""
+
+import numpy as np
+import pytest
+
+from solprop_amd.native import Workspace, advance_batch
+from solprop_amd.problem import capillary_problem, gnlse_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def instances():
+    """Points of a scan over energy and pressure on the capillary route, and two
+    envelope points: different shapes and routes in one batch."""
+    out = [capillary_problem(n_t=2048, n_nodes=32, length=0.05, integration="trapezoid",
+                             energy=e, p0=p)
+           for e, p in [(275e-6, 0.8), (200e-6, 0.8), (275e-6, 0.5), (350e-6, 0.6)]]
+    out += [gnlse_problem(n_t=512, ffac=2, window=2e-12, fwhm=50e-15, soliton_order=s,
+                          length=5e-3) for s in (2.0, 3.0)]
+    return out
+
+
+def test_batch_matches_instances_run_alone():
+    print("batched advance of 6 instances", flush=True)
+    problems = instances()
+    limit = 40
+    alone = []
+    for p in problems:
+        ws = Workspace(p)
+        session = ws.start(0.0, p.field0)
+        batch = session.advance(p.z_end, work_limit=limit)
+        alone.append((batch.attempts.copy(), session.state()))
+        ws.close()
+    spaces = [Workspace(p) for p in problems]
+    sessions = [ws.start(0.0, p.field0) for ws, p in zip(spaces, problems)]
+    batches = advance_batch(sessions, [p.z_end for p in problems], work_limit=limit)
+    together = [(b.attempts.copy(), s.state()) for b, s in zip(batches, sessions)]
+    for ws in spaces:
+        ws.close()
+    for (rows_a, state_a), (rows_b, state_b) in zip(alone, together):
+        assert len(rows_a) > 0
+        assert np.array_equal(rows_a, rows_b)
+        assert np.array_equal(state_a, state_b)
