@@ -46,3 +46,40 @@ def test_batch_matches_instances_run_alone():
         assert len(rows_a) > 0
         assert np.array_equal(rows_a, rows_b)
         assert np.array_equal(state_a, state_b)
+
+
+@pytest.mark.parametrize("route", ["capillary", "envelope"])
+def test_zero_field_is_refused_as_the_oracle_refuses_it(route):
+    """An empty launch field (no light at all): the scaled norm is zero over a zero scale,
+    every attempt is rejected until the step would fall below the floor, and the march
+    stops with a convergence error; the device stops the same way, after the same
+    number of attempts."""
+    from oracle import adams, rk
+    from oracle.controller import ConvergenceError as OracleConvergence
+    from solprop_amd.native import ConvergenceError
+
+    print(f"zero field {route}", flush=True)
+    if route == "capillary":
+        problem = capillary_problem(n_t=2048, n_nodes=32, length=0.05, integration="trapezoid")
+    else:
+        problem = gnlse_problem(n_t=512, ffac=2, window=2e-12, fwhm=50e-15, length=5e-3,
+                                noise_seed=None)
+    zero = np.zeros_like(problem.field0)
+    y0 = zero.reshape(problem.shape)
+    seen = []
+    march = adams.march if problem.method == "adams" else rk.march
+    with pytest.raises(OracleConvergence):
+        march(problem, y0.copy(), max_attempts=12, on_attempt=lambda *a: seen.append(a))
+    ws = Workspace(problem)
+    session = ws.start(0.0, zero)
+    with pytest.raises(ConvergenceError):
+        session.advance(problem.z_end, work_limit=12)
+    counters = ws.counters()
+    ws.close()
+    # Zero over a zero scale: the norms are not numbers, every attempt is rejected and
+    # the step shrinks to the floor, on both sides after the same attempts.
+    # The Adams march's last attempt, taken at the floor's fixed step and rejected, raises
+    # inside its decision before its row is recorded (the Runge-Kutta march records it
+    # first); the workspace took that trial.
+    last = 1 if problem.method == "adams" else 0
+    assert len(seen) >= 1 and counters["trials"] == len(seen) + last
