@@ -48,6 +48,7 @@ def test_batch_matches_instances_run_alone():
         assert np.array_equal(state_a, state_b)
 
 
+@pytest.mark.filterwarnings("ignore:invalid value encountered in divide:RuntimeWarning")
 @pytest.mark.parametrize("route", ["capillary", "envelope"])
 def test_zero_field_is_refused_as_the_oracle_refuses_it(route):
     """An empty launch field (no light at all): the scaled norm is zero over a zero scale,
