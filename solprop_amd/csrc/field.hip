@@ -603,12 +603,34 @@ __global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_wave
 // node. Each lane sums a run of consecutive tiles in order, the wave scans the run
 // totals, and each lane then writes its run's offsets in order: a fixed order, so the
 // offsets are the same on every run.
+// Runs of up to kPrefixRun tiles a lane (grids of up to 512 tiles, 32257 samples) are
+// loaded into registers at once, so that their loads overlap; longer runs take the loop.
+// The additions are the same, in the same order, either way.
+constexpr int kPrefixRun = 8;
+
 __global__ void __launch_bounds__(kWave) plasma_tile_prefix(int tiles, double* sums) {
   const int lane = threadIdx.x;
   double* row = sums + static_cast<long>(blockIdx.x) * tiles;
   const int run = (tiles + kWave - 1) / kWave;
   const int k0 = lane * run;
   const int k1 = k0 + run < tiles ? k0 + run : tiles;
+  if (run <= kPrefixRun) {  // uniform across the wave
+    double v[kPrefixRun];
+#pragma unroll
+    for (int r = 0; r < kPrefixRun; ++r) v[r] = k0 + r < k1 ? row[k0 + r] : 0.0;
+    double total = 0.0;
+#pragma unroll
+    for (int r = 0; r < kPrefixRun; ++r)
+      if (k0 + r < k1) total = total + v[r];
+    double running = from_previous_lane(wave_scan(total));
+#pragma unroll
+    for (int r = 0; r < kPrefixRun; ++r)
+      if (k0 + r < k1) {
+        row[k0 + r] = running;
+        running = running + v[r];
+      }
+    return;
+  }
   double total = 0.0;
   for (int k = k0; k < k1; ++k) total = total + row[k];
   double running = from_previous_lane(wave_scan(total));
