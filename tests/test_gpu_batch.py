@@ -9,24 +9,27 @@ import numpy as np
 import pytest
 
 from solprop_amd.native import Workspace, advance_batch
-from solprop_amd.problem import capillary_problem, gnlse_problem
+from solprop_amd.problem import capillary_problem, gnlse_problem, radial_problem
 
 pytestmark = pytest.mark.gpu
 
 
 def instances():
-    """Points of a scan over energy and pressure on the capillary route, and two
-    envelope points: different shapes and routes in one batch."""
+    """Points of a scan over energy and pressure on the capillary route, two envelope
+    points and a radial one: different shapes and routes in one batch."""
     out = [capillary_problem(n_t=2048, n_nodes=32, length=0.05, integration="trapezoid",
                              energy=e, p0=p)
            for e, p in [(275e-6, 0.8), (200e-6, 0.8), (275e-6, 0.5), (350e-6, 0.6)]]
     out += [gnlse_problem(n_t=512, ffac=2, window=2e-12, fwhm=50e-15, soliton_order=s,
                           length=5e-3) for s in (2.0, 3.0)]
+    # The dense route: its GEMMs run on its own stream beside the others' kernels.
+    out.append(radial_problem(n_t=1024, window=0.25e-12, n_r=32, length=0.02, fwhm=30e-15,
+                              w0=0.25e-3, integration="trapezoid"))
     return out
 
 
 def test_batch_matches_instances_run_alone():
-    print("batched advance of 6 instances", flush=True)
+    print("batched advance of 7 instances", flush=True)
     problems = instances()
     limit = 40
     alone = []
