@@ -657,6 +657,12 @@ class ExponentialEngine:
         self.compensated = len(problem.phi_paths) > 1
         self.held = None
 
+    @staticmethod
+    def residual_rate(rate, reference):
+        """The operator's rate less the step's constant reference, ``D(z_j) - x/h``: the
+        linear part left to the quadrature (``numpy_ref/multistep.py:170-204,1072-1085``)."""
+        return rate - reference
+
     def lab_source(self, lab, z):
         """The source at a lab-frame field, unit frame factor (``workspace.py:819``)."""
         return self.source.evaluate(lab, z, 1.0)
@@ -690,7 +696,7 @@ class ExponentialEngine:
             used = list(nodes[: len(rows_read)])
             carried = np.stack([
                 np.exp((1.0 - node) * exponent)
-                * ((self.rates[slot] - reference) * self.states[slot])
+                * (self.residual_rate(self.rates[slot], reference) * self.states[slot])
                 for slot, node in zip(rows_read, used, strict=True)
             ])
 
@@ -710,7 +716,7 @@ class ExponentialEngine:
         endpoint_rate = None
         if self.compensated:
             endpoint_rate = p.phi_rate(z + dz)
-            endpoint = (endpoint_rate - exponent / dz) * lab
+            endpoint = self.residual_rate(endpoint_rate, exponent / dz) * lab
 
             def corr(k):
                 return csum([1.0, *used[:k]], endpoint, k)
@@ -785,17 +791,19 @@ def adams_policy_for(problem, **overrides) -> MarchPolicy:
 
 
 def march(problem, state, *, target=None, proposal=None, max_attempts=None, policy=None,
-          on_attempt=None, on_finish=None):
+          on_attempt=None, on_finish=None, engine=ExponentialEngine, marches=None):
     """Adaptive exponential Adams march of the problem (``HostAdamsMarch.march_to``).
 
     Returns ``(state, rows)`` with rows ``(end, dz, norm, accepted)`` per attempt.
     ``on_finish(masked, dz)`` sees each accepted step's masked state before its split
     collar (what a bound monitor's split sums read, ``numpy_ref/workspace.py:1021-1067``).
+    ``marches``, when given, receives the :class:`AdamsMarch` (its ledger holds the
+    orders of the accepted steps).
     """
     from oracle import rk
 
     policy = policy or adams_policy_for(problem)
-    engine = ExponentialEngine(problem, rk.StageSource(problem), policy.order + 2)
+    engine = engine(problem, rk.StageSource(problem), policy.order + 2)
     engine.state = state
     engine.on_finish = on_finish
     host = _PlainHost()
@@ -811,6 +819,8 @@ def march(problem, state, *, target=None, proposal=None, max_attempts=None, poli
 
         host.record = record
     am = AdamsMarch(engine, policy, host, proposal=proposal or 1e-4 * problem.z_end)
+    if marches is not None:
+        marches.append(am)
     try:
         am.march_to(problem.z_end if target is None else target)
     except StopIteration:

@@ -1,8 +1,10 @@
 """ctypes binding of the C ABI declared in ``include/solprop_mi355x.h``.
 
 The library is loaded from the package directory (built in-tree by
-``solprop_amd.build``). There is no fallback: a missing or stale library raises
+``solprop_amd.build``). There is no fallback: a missing library raises
 :class:`LibraryMissing`, so no caller can run the hot path anywhere but on the device.
+The library is not checked against its sources here; ``solprop_amd.build`` compiles
+every listed source each time it runs and refuses a list naming a missing file.
 """
 
 from __future__ import annotations

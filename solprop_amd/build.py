@@ -20,8 +20,8 @@ CSRC = HERE / "csrc"
 OUT = HERE / "libsolprop_mi355x.so"
 ARCH = "gfx950"
 
-HIP_SOURCES = ["fft.hip", "gemm.hip", "stage.hip", "field.hip", "adams.hip", "workspace.hip",
-               "field_ws.hip", "stats.hip"]
+HIP_SOURCES = ["fft.hip", "gemm.hip", "stage.hip", "field.hip", "workspace.hip", "field_ws.hip",
+               "stats.hip"]
 HOST_SOURCES = ["capi.cpp"]
 STRICT_SOURCES = ["march.cpp", "trace.cpp", "scalars.cpp", "adams_march.cpp",
                   "adams_trace.cpp"]
@@ -59,10 +59,12 @@ def build(verbose: bool = False) -> Path:
     hipcc = _hipcc()
     jobs = []
     objects = []
+    missing = [name for name in (*HIP_SOURCES, *HOST_SOURCES, *STRICT_SOURCES)
+               if not (CSRC / name).exists()]
+    if missing:
+        raise FileNotFoundError("listed library sources are missing: " + ", ".join(missing))
     for name in HIP_SOURCES:
         src = CSRC / name
-        if not src.exists():
-            continue
         obj = work / (name + ".o")
         jobs.append([hipcc, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
         objects.append(obj)

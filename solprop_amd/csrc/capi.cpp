@@ -1,5 +1,6 @@
 // The extern "C" boundary (include/solprop_mi355x.h): status codes and messages in
 // place of exceptions, host arrays in and out.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -57,6 +58,27 @@ int guarded(char* msg, std::size_t len, F&& body) {
     return SP_BACKEND;
   }
 }
+
+// A testing entry's stream and device buffers, released on every exit path.
+struct DeviceScope {
+  hipStream_t stream = nullptr;
+  std::vector<void*> buffers;
+  DeviceScope() { SP_HIP(hipStreamCreate(&stream)); }
+  ~DeviceScope() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : buffers) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  DeviceScope(const DeviceScope&) = delete;
+  DeviceScope& operator=(const DeviceScope&) = delete;
+  template <typename T>
+  T* alloc(std::size_t bytes) {
+    void* p = nullptr;
+    SP_HIP(hipMalloc(&p, bytes));
+    buffers.push_back(p);
+    return static_cast<T*>(p);
+  }
+};
 
 }  // namespace
 
@@ -117,6 +139,16 @@ int sp_batch_advance(sp_workspace* const* ws, int n, const double* targets, long
   if (!ws || n < 1 || !targets || !out || !status) return SP_VALUE;
   for (int i = 0; i < n; ++i)
     if (!ws[i]) return SP_VALUE;
+  {
+    // One workspace twice would share its stream, graphs and pinned record between two
+    // threads: refused.
+    std::vector<const sp_workspace*> seen(ws, ws + n);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end()) {
+      write_message(msg, msglen, "a workspace appears more than once in the batch");
+      return SP_VALUE;
+    }
+  }
   std::vector<std::vector<char>> messages(n, std::vector<char>(msglen > 0 ? msglen : 1, 0));
   std::vector<std::thread> threads;
   threads.reserve(n);
@@ -250,18 +282,13 @@ int sp_testing_fft(const double* in, double* out, long n, long batch, int sign, 
     sp::FftPlan plan;
     plan.init(static_cast<std::size_t>(n), static_cast<std::size_t>(batch));
     const std::size_t bytes = sizeof(sp::cplx) * static_cast<std::size_t>(n * batch);
-    sp::cplx *data = nullptr, *scratch = nullptr;
-    hipStream_t s = nullptr;
-    SP_HIP(hipStreamCreate(&s));
-    SP_HIP(hipMalloc(&data, bytes));
-    SP_HIP(hipMalloc(&scratch, bytes));
+    DeviceScope scope;
+    sp::cplx* data = scope.alloc<sp::cplx>(bytes);
+    sp::cplx* scratch = scope.alloc<sp::cplx>(bytes);
     SP_HIP(hipMemcpy(data, in, bytes, hipMemcpyHostToDevice));
-    plan.exec(data, scratch, sign, s);
-    SP_HIP(hipStreamSynchronize(s));
+    plan.exec(data, scratch, sign, scope.stream);
+    SP_HIP(hipStreamSynchronize(scope.stream));
     SP_HIP(hipMemcpy(out, data, bytes, hipMemcpyDeviceToHost));
-    SP_HIP(hipFree(data));
-    SP_HIP(hipFree(scratch));
-    SP_HIP(hipStreamDestroy(s));
   });
 }
 
@@ -272,22 +299,16 @@ int sp_testing_gemm(long M, long N, long K, const double* A, const double* B, do
     SP_CHECK(M > 0 && N > 0 && K > 0 && M * K < (1L << 28) && K * N < (1L << 28) &&
                  M * N < (1L << 28),
              "GEMM dimensions must be positive and below 2^28 elements per operand");
-    double *a = nullptr, *b = nullptr, *c = nullptr;
-    hipStream_t s = nullptr;
-    SP_HIP(hipStreamCreate(&s));
-    SP_HIP(hipMalloc(&a, sizeof(double) * M * K));
-    SP_HIP(hipMalloc(&b, sizeof(double) * K * N));
-    SP_HIP(hipMalloc(&c, sizeof(double) * M * N));
+    DeviceScope scope;
+    double* a = scope.alloc<double>(sizeof(double) * M * K);
+    double* b = scope.alloc<double>(sizeof(double) * K * N);
+    double* c = scope.alloc<double>(sizeof(double) * M * N);
     SP_HIP(hipMemcpy(a, A, sizeof(double) * M * K, hipMemcpyHostToDevice));
     SP_HIP(hipMemcpy(b, B, sizeof(double) * K * N, hipMemcpyHostToDevice));
     sp::launch_dgemm(static_cast<int>(M), static_cast<int>(N), static_cast<int>(K), a, K, b, N,
-                     c, N, s);
-    SP_HIP(hipStreamSynchronize(s));
+                     c, N, scope.stream);
+    SP_HIP(hipStreamSynchronize(scope.stream));
     SP_HIP(hipMemcpy(C, c, sizeof(double) * M * N, hipMemcpyDeviceToHost));
-    SP_HIP(hipFree(a));
-    SP_HIP(hipFree(b));
-    SP_HIP(hipFree(c));
-    SP_HIP(hipStreamDestroy(s));
   });
 }
 

@@ -2,6 +2,7 @@
 #include "field_ws.hpp"
 #include "gemm.hpp"
 #include "kernels.hpp"
+#include "sync.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -406,13 +407,7 @@ void FieldWorkspace::run(int key) {
   }
   auto it = graphs_.find(key);
   if (it == graphs_.end()) {
-    hipGraph_t graph;
-    SP_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    enqueue(key);
-    SP_HIP(hipStreamEndCapture(stream_, &graph));
-    hipGraphExec_t exec;
-    SP_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    SP_HIP(hipGraphDestroy(graph));
+    const hipGraphExec_t exec = capture_graph(stream_, [&] { enqueue(key); });
     it = graphs_.emplace(key, exec).first;
   }
   SP_HIP(hipGraphLaunch(it->second, stream_));
@@ -420,13 +415,11 @@ void FieldWorkspace::run(int key) {
 }
 
 void FieldWorkspace::wait() {
-  // Spin on the stream's last event: the host's next decision needs this trial's sums
-  // as soon as they land, and a blocking wait adds a wake-up.
+  // Poll the stream's last event: the host's next decision needs this trial's sums as
+  // soon as they land (a blocking wait adds a wake-up); past a short spin the poll
+  // yields and backs off, and a device that never finishes is reported.
   SP_HIP(hipEventRecord(done_, stream_));
-  hipError_t status;
-  while ((status = hipEventQuery(done_)) == hipErrorNotReady) {
-  }
-  SP_HIP(status);
+  wait_event(done_);
 }
 
 void FieldWorkspace::flush() {

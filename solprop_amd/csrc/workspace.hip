@@ -1,6 +1,7 @@
 // The device workspace (see workspace.hpp). Host code: uploads, the per-trial scalar
 // record, the stage sequence, graph capture and the march's engine.
 #include "workspace.hpp"
+#include "sync.hpp"
 
 #include <cmath>
 #include <cstdlib>
@@ -306,13 +307,8 @@ double Workspace::trial(double z, double dz, bool full) {
   if (use_graphs_) {
     auto it = graphs_.find(key);
     if (it == graphs_.end()) {
-      hipGraph_t graph;
-      SP_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-      enqueue_trial(commit, mode);
-      SP_HIP(hipStreamEndCapture(stream_, &graph));
-      hipGraphExec_t exec;
-      SP_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-      SP_HIP(hipGraphDestroy(graph));
+      const hipGraphExec_t exec =
+          capture_graph(stream_, [&] { enqueue_trial(commit, mode); });
       it = graphs_.emplace(key, exec).first;
     }
     SP_HIP(hipGraphLaunch(it->second, stream_));

@@ -227,3 +227,32 @@ def test_plasma_scan_stages_match_oracle(case):
     assert errors["rate"] < 1e-13
     assert errors["fraction"] < max(1e-12, 4 * sensitivity[0])
     assert errors["free current"] < max(1e-11, 4 * sensitivity[1])
+
+
+def test_march_matches_oracle_at_full_size():
+    """The measured workload marched: 4 modes, 128 nodes, 2^14 samples, the band rule and
+    the pressure gradient's two-term operator, so every trial after the first carries the
+    transported compensation. 40 attempts take the order from 1 to 5 (the oracle's
+    ledger is checked to reach it); the device makes the same decisions, its steps
+    within 1e-7 and its end state within 1e-8."""
+    print("field march at full size, 40 attempts", flush=True)
+    problem = capillary_problem()
+    y0 = problem.field0.reshape(problem.shape)
+    marches = []
+    expected, records = adams.march(problem, y0.copy(), max_attempts=40, marches=marches)
+    assert max(marches[0].ledger.orders) == 5
+    ws = Workspace(problem)
+    session = ws.start(0.0, problem.field0)
+    batch = session.advance(problem.z_end, work_limit=len(records))
+    rows = batch.attempts
+    state = session.state()
+    ws.close()
+    assert len(rows) == len(records)
+    assert [bool(r[3]) for r in rows] == [bool(r[3]) for r in records]
+    steps = np.array([r[1] for r in records])
+    assert np.allclose(rows[:, 1], steps, rtol=1e-7, atol=0)
+    norms = np.array([r[2] for r in records])
+    assert np.allclose(rows[:, 2], norms, rtol=1e-6, atol=0)
+    error = rel_l2(state, expected.reshape(state.shape))
+    print("end-state relative L2", error, flush=True)
+    assert error < 1e-8
