@@ -176,10 +176,11 @@ typedef struct sp_march_result {
 int sp_workspace_advance(sp_workspace* ws, double target, long work_limit, double* rows,
                          long max_rows, sp_march_result* out, char* msg, size_t msglen);
 
-/* The batched variant for scans of independent instances (the process pools of
- * src/solprop/scan.py): `n` workspaces advanced concurrently to their own targets, one
- * host thread a workspace, each on its workspace's own stream, so that the instances'
- * launches and host decisions overlap on the one GPU. Each instance's march is the
+/* The batched variant for independent workspaces of any route (the process pools of
+ * src/solprop/driver/scan.py:30): `n` workspaces advanced concurrently to their own
+ * targets by a fixed pool of host threads, each on its workspace's own stream, so that
+ * the instances' launches and host decisions overlap on the one GPU (a workspace listed
+ * twice is refused). A scan of field-grid points runs faster as one sp_scan. Each instance's march is the
  * one sp_workspace_advance would make (the same decisions and results, to the bit).
  * `rows` holds n blocks of max_rows x 4 (end, step, norm, accepted), `out` n results,
  * `status` n codes; the call returns SP_OK when every instance did, else the first
@@ -187,6 +188,49 @@ int sp_workspace_advance(sp_workspace* ws, double target, long work_limit, doubl
 int sp_batch_advance(sp_workspace* const* ws, int n, const double* targets, long work_limit,
                      double* rows, long max_rows, sp_march_result* out, int* status, char* msg,
                      size_t msglen);
+
+/* ------------------------------------------------------------ batched scans
+ *
+ * A parameter scan's points as one batched march (the reference runs them on process
+ * pools: src/solprop/driver/scan.py:30, docs/dev/architecture.md:774-786). Every instance
+ * is a field-grid problem under the exponential Adams march with its own grid (n_to a
+ * power of two), modes (1..6), nodes, operator, plasma rule and launch field, and its own
+ * adaptive step control; none may carry a Raman response, an absorber or the dense
+ * route. Each round of the scan runs every active instance's next trial (or start) as
+ * one captured HIP graph of grouped launches on one stream; the host concludes each
+ * instance's attempt and plans its next. Each instance's march is the one
+ * sp_workspace_advance makes for its problem alone: the same decisions, rows and end
+ * state, to the bit. */
+typedef struct sp_scan sp_scan;
+
+/* Create the scan's `n` instances (the `Workspace(problem)` of each point,
+ * _core.pyi:595-607) and capture its round. SP_VALUE names the first instance the scan
+ * refuses. */
+int sp_scan_create(const sp_problem_desc* descs, int n, sp_scan** out, char* msg,
+                   size_t msglen);
+void sp_scan_destroy(sp_scan* scan);
+/* Load instance `i`'s state at z (complex (C_i, N_i)), as sp_workspace_load. */
+int sp_scan_load(sp_scan* scan, int i, double z, const double* field, char* msg,
+                 size_t msglen);
+/* Advance every instance towards its own target, at most `work_limit` attempts each (the
+ * reference's per-point `Session.advance(work_limit, ...)`, _core.pyi:468-499). `rows`
+ * holds n blocks of max_rows x 4 (end, step, norm, accepted), `out` n results, `status`
+ * n codes (an instance's convergence failure stops that instance only); returns SP_OK
+ * when every instance did, else the first failing instance's code with its message. */
+int sp_scan_advance(sp_scan* scan, const double* targets, long work_limit, double* rows,
+                    long max_rows, sp_march_result* out, int* status, char* msg,
+                    size_t msglen);
+/* Instance `i`'s committed state, complex (C_i, N_i). */
+int sp_scan_state(sp_scan* scan, int i, double* out, char* msg, size_t msglen);
+/* The scan's rounds so far, the host's wait for their graphs and its planning time,
+ * milliseconds, and the device bytes its instances hold. */
+typedef struct sp_scan_timing {
+  long rounds;
+  double device_ms;
+  double host_ms;
+  long device_bytes;
+} sp_scan_timing;
+int sp_scan_timing_of(sp_scan* scan, sp_scan_timing* out);
 
 /* `Session.export(ticket, recipe, out)` (native/src/solprop_native/session.py:960;
  * src/solprop/backends/export.py:52-207): the committed state converted by the recipe,

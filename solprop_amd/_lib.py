@@ -132,6 +132,12 @@ class AdamsScenarioC(C.Structure):
     ]
 
 
+class ScanTimingC(C.Structure):
+    _fields_ = [
+        ("rounds", C.c_long), ("device_ms", _D), ("host_ms", _D), ("device_bytes", C.c_long),
+    ]
+
+
 # The exported entry points and their signatures (include/solprop_mi355x.h).
 SIGNATURES = {
     "sp_workspace_create": (_I, [C.POINTER(ProblemDescC), C.POINTER(C.c_void_p), C.c_char_p,
@@ -154,6 +160,14 @@ SIGNATURES = {
     "sp_workspace_phases": (_I, [C.c_void_p, _PD, _I, C.POINTER(_I), C.c_char_p, C.c_size_t]),
     "sp_batch_advance": (_I, [C.POINTER(C.c_void_p), _I, _PD, C.c_long, _PD, C.c_long,
                               C.c_void_p, C.POINTER(_I), C.c_char_p, C.c_size_t]),
+    "sp_scan_create": (_I, [C.POINTER(ProblemDescC), _I, C.POINTER(C.c_void_p), C.c_char_p,
+                            C.c_size_t]),
+    "sp_scan_destroy": (None, [C.c_void_p]),
+    "sp_scan_load": (_I, [C.c_void_p, _I, _D, _PD, C.c_char_p, C.c_size_t]),
+    "sp_scan_advance": (_I, [C.c_void_p, _PD, C.c_long, _PD, C.c_long, C.c_void_p,
+                             C.POINTER(_I), C.c_char_p, C.c_size_t]),
+    "sp_scan_state": (_I, [C.c_void_p, _I, _PD, C.c_char_p, C.c_size_t]),
+    "sp_scan_timing_of": (_I, [C.c_void_p, C.POINTER(ScanTimingC)]),
     "sp_workspace_bind_monitor": (_I, [C.c_void_p, _PD, _PD, C.c_char_p, C.c_size_t]),
     "sp_workspace_monitor_reading": (_I, [C.c_void_p, _PD, C.c_char_p, C.c_size_t]),
     "sp_workspace_statistics": (_I, [C.c_void_p, C.c_ulonglong, _PD, _I, _PD, _I, C.POINTER(_I),

@@ -101,6 +101,11 @@ bool March::attempt(double dz, bool fixed, double end, bool kink, double* out_no
   const Stencil s = stencil(p_, z, dz, history, order, last_rejected);
   ledger.predictor += 1;
   const std::vector<double> norms = engine_.trial(z, dz, s);
+  return decide(s, norms, dz, fixed, end, kink, out_norm);
+}
+
+bool March::decide(const Stencil& s, const std::vector<double>& norms, double dz, bool fixed,
+                   double end, bool kink, double* out_norm) {
   const double norm = norms[0];
   *out_norm = norm;
   if (!(norm <= 1.0) && !(fixed && std::isfinite(norm))) {
@@ -171,14 +176,61 @@ void March::reject(double dz, double norm, int order_) {
 }
 
 bool March::march_to(double target) {
+  stopped_ = false;
+  for (;;) {
+    Request r;
+    switch (plan(target, &r)) {
+      case Request::kDone: return true;
+      case Request::kHalted: return false;
+      case Request::kStart:
+        engine_.start(z);
+        started();
+        break;
+      case Request::kTrial:
+        conclude(r, engine_.trial(r.z, r.dz, r.stencil));
+        break;
+    }
+  }
+}
+
+void March::started() {
+  history.assign(1, Entry{z, 0, 0});
+  order = p_.variable_order ? 1 : p_.order;
+}
+
+bool March::conclude(const Request& r, const std::vector<double>& norms) {
+  double norm = 0.0;
+  const bool accepted = decide(r.stencil, norms, r.dz, false, r.end, r.kink, &norm);
+  host_.record(r.end, r.dz, norm, accepted);
+  if (accepted) {
+    host_.stepped();
+    if (host_.natural_stop(r.target)) stopped_ = true;
+  }
+  return accepted;
+}
+
+Request::Kind March::plan(double target, Request* out) {
   const std::vector<double> planes = p_.save_planes, kinks = p_.hard_kinks;
   std::vector<double> edges = p_.breakpoints;
   if (!p_.interval_cap && edges.size() > 2) edges = {edges.front(), edges.back()};
   const march::Targets targets{&planes, &kinks, &edges, p_.dz_max, target, p_.stretch};
-  while (!march::reached(z, target)) {
-    if (host_.halt()) return false;
+  Request& r = *out;
+  r = Request{};
+  if (stopped_) {
+    stopped_ = false;
+    return r.kind = Request::kDone;
+  }
+  {
+    if (march::reached(z, target)) return r.kind = Request::kDone;
+    if (host_.halt()) return r.kind = Request::kHalted;
     afford();
-    if (history.empty()) start();
+    if (history.empty()) {
+      // The start's evaluation (the reference's start): the caller runs it, then
+      // `started` opens the history; the next plan goes on to the trial.
+      ledger.startup += 1;
+      r.z = z;
+      return r.kind = Request::kStart;
+    }
     // The collar's re-entry cap (adams_core.collar_reentry).
     const bool quiet = host_.collar_quiet();
     double prop = host_.cap(proposal, z);
@@ -211,15 +263,15 @@ bool March::march_to(double target) {
           if (std::fabs(k - end) <= tolerance) on_kink = true;
       kink = stop.target == march::kHardKink || on_kink;
     }
-    double norm = 0.0;
-    const bool accepted = attempt(dz, false, end, kink, &norm);
-    host_.record(end, dz, norm, accepted);
-    if (accepted) {
-      host_.stepped();
-      if (host_.natural_stop(target)) return true;
-    }
+    r.stencil = stencil(p_, z, dz, history, order, last_rejected);
+    ledger.predictor += 1;
+    r.z = z;
+    r.dz = dz;
+    r.end = end;
+    r.kink = kink;
+    r.target = target;
+    return r.kind = Request::kTrial;
   }
-  return true;
 }
 
 bool March::replay_to(const std::vector<double>& steps, const std::vector<double>& edges,

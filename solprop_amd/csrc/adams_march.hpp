@@ -85,11 +85,31 @@ Stencil stencil(const Policy& p, double z, double dz, const std::vector<Entry>& 
 double factor(const Policy& p, double norm, int order, double growth);
 double order_weight(const Policy& p, double norm, int order, double dz, bool quiet);
 
+// One action a march asks of its engine: evaluate the source at the start of a history,
+// or take a trial step.
+struct Request {
+  enum Kind { kDone, kHalted, kStart, kTrial } kind = kDone;
+  double z = 0.0, dz = 0.0, end = 0.0;
+  double target = 0.0;  // the position the march is headed for
+  bool kink = false;
+  Stencil stencil;
+};
+
 class March {
  public:
   March(const Policy& p, Engine& e, Host& h, double z0, double proposal);
   // Step adaptively to `target`; false when the host halted between attempts.
   bool march_to(double target);
+  // The same march driven from outside, one action at a time, for engines that run many
+  // marches' trials together: `plan` says what the march needs next towards `target`
+  // (kDone: reached, or the host stopped it naturally; kHalted: the host's attempt limit;
+  // kStart: evaluate the source at z, then call `started`; kTrial: run the request's
+  // trial, then hand its norms to `conclude`). march_to is this loop with the engine's
+  // start and trial in between, so both make the same decisions.
+  Request::Kind plan(double target, Request* r);
+  void started();
+  // The attempt's decision from its norms (the step's own order first); true if accepted.
+  bool conclude(const Request& r, const std::vector<double>& norms);
   // Replay a fixed schedule towards `target`.
   bool replay_to(const std::vector<double>& steps, const std::vector<double>& edges,
                  double target);
@@ -108,6 +128,9 @@ class March {
   void afford();
   void start();
   bool attempt(double dz, bool fixed, double end, bool kink, double* norm);
+  bool decide(const Stencil& s, const std::vector<double>& norms, double dz, bool fixed,
+              double end, bool kink, double* norm);
+  bool stopped_ = false;  // the host stopped the march naturally after an accepted step
   void reject(double dz, double norm, int order_);
   const Policy& p_;
   Engine& engine_;

@@ -5,10 +5,12 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <vector>
 
 #include "field_ws.hpp"
+#include "scan.hpp"
 #include "gemm.hpp"
 #include "trace.hpp"
 #include "workspace.hpp"
@@ -29,6 +31,8 @@ struct sp_workspace {
 };
 
 namespace {
+
+constexpr int kBatchThreads = 16;
 
 void write_message(char* msg, std::size_t len, const char* text) {
   if (!msg || len == 0) return;
@@ -150,21 +154,85 @@ int sp_batch_advance(sp_workspace* const* ws, int n, const double* targets, long
     }
   }
   std::vector<std::vector<char>> messages(n, std::vector<char>(msglen > 0 ? msglen : 1, 0));
-  std::vector<std::thread> threads;
-  threads.reserve(n);
-  for (int i = 0; i < n; ++i)
-    threads.emplace_back([&, i] {
-      status[i] = sp_workspace_advance(ws[i], targets[i], work_limit,
-                                       rows ? rows + 4 * max_rows * i : nullptr,
-                                       rows ? max_rows : 0, out + i, messages[i].data(),
-                                       messages[i].size());
-    });
-  for (auto& t : threads) t.join();
+  // A fixed pool, not a thread per instance: at most kBatchThreads instances advance at
+  // once, each waiting on its own stream with a backed-off poll.
+  static sp::WorkerPool pool(kBatchThreads);
+  static std::mutex one_batch;  // the pool runs one loop at a time
+  std::lock_guard<std::mutex> lock(one_batch);
+  pool.run(n, [&](int i) {
+    status[i] = sp_workspace_advance(ws[i], targets[i], work_limit,
+                                     rows ? rows + 4 * max_rows * i : nullptr,
+                                     rows ? max_rows : 0, out + i, messages[i].data(),
+                                     messages[i].size());
+  });
   for (int i = 0; i < n; ++i)
     if (status[i] != SP_OK) {
       write_message(msg, msglen, messages[i].data());
       return status[i];
     }
+  return SP_OK;
+}
+
+struct sp_scan {
+  sp::Scan impl;
+  sp_scan(const sp_problem_desc* d, int n) : impl(d, n) {}
+};
+
+int sp_scan_create(const sp_problem_desc* descs, int n, sp_scan** out, char* msg,
+                   size_t msglen) {
+  if (!descs || !out || n < 1) {
+    write_message(msg, msglen, "a scan needs its problems and an output handle");
+    return SP_VALUE;
+  }
+  *out = nullptr;
+  return guarded(msg, msglen, [&] { *out = new sp_scan(descs, n); });
+}
+
+void sp_scan_destroy(sp_scan* scan) { delete scan; }
+
+int sp_scan_load(sp_scan* scan, int i, double z, const double* field, char* msg,
+                 size_t msglen) {
+  if (!scan) return SP_VALUE;
+  return guarded(msg, msglen, [&] { scan->impl.load(i, z, field); });
+}
+
+int sp_scan_advance(sp_scan* scan, const double* targets, long work_limit, double* rows,
+                    long max_rows, sp_march_result* out, int* status, char* msg,
+                    size_t msglen) {
+  if (!scan || !targets || !out || !status) return SP_VALUE;
+  const int n = scan->impl.size();
+  std::vector<std::vector<double>> held;
+  std::vector<std::string> messages;
+  const int code = guarded(msg, msglen, [&] {
+    scan->impl.advance(targets, work_limit, held, out, status, messages);
+  });
+  if (code != SP_OK) return code;
+  for (int i = 0; i < n; ++i) {
+    const long count = static_cast<long>(held[i].size() / 4);
+    const long kept = rows ? std::min(count, max_rows) : 0;
+    if (kept > 0) std::memcpy(rows + 4 * max_rows * i, held[i].data(), sizeof(double) * 4 * kept);
+    out[i].n_rows = static_cast<int>(kept);
+  }
+  for (int i = 0; i < n; ++i)
+    if (status[i] != SP_OK) {
+      write_message(msg, msglen, ("scan instance " + std::to_string(i) + ": " + messages[i]).c_str());
+      return status[i];
+    }
+  return SP_OK;
+}
+
+int sp_scan_state(sp_scan* scan, int i, double* out, char* msg, size_t msglen) {
+  if (!scan || !out) return SP_VALUE;
+  return guarded(msg, msglen, [&] { scan->impl.state(i, out); });
+}
+
+int sp_scan_timing_of(sp_scan* scan, sp_scan_timing* out) {
+  if (!scan || !out) return SP_VALUE;
+  const sp::ScanTiming& t = scan->impl.timing();
+  out->rounds = t.rounds;
+  out->device_ms = t.device_ms;
+  out->host_ms = t.host_ms;
+  out->device_bytes = scan->impl.device_bytes();
   return SP_OK;
 }
 

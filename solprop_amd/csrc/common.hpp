@@ -81,6 +81,21 @@ enum Status : int {
     if (!(cond)) throw ::sp::Error(::sp::kValue, (msg));     \
   } while (0)
 
+// The owner of workgroup `block` in a grouped launch whose owners hold consecutive ranges:
+// owner i holds first[i] .. first[i + 1] - 1 (first[0] = 0, first[n] = the grid). A binary
+// search, the same in every lane of the workgroup.
+__device__ inline int owner_of(const int* __restrict__ first, int n, int block) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= block)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 inline unsigned blocks_for(std::size_t n, unsigned threads) {
   return static_cast<unsigned>((n + threads - 1) / threads);
 }

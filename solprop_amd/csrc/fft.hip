@@ -8,20 +8,7 @@ namespace {
 
 constexpr int kMaxElems = 4096;  // complex doubles a workgroup holds in LDS (64 KiB)
 
-struct PassArgs {
-  long in_sj, in_ss, out_sk, out_ss, batch_in, batch_out;
-  int global_twiddle;
-  int mask_n;  // N - 1 for the global twiddle index
-  // When `spec` is set, the first pass loads its input from a half spectrum instead:
-  // row r's point k is c2r_pre(X[k], X[M - k], W_2M^k) of X = spec row r (spec_n bins,
-  // zero beyond) scaled by spec_scale, DC and Nyquist taken real (the packing of a real
-  // inverse transform of 2M points as a complex one of M).
-  const cplx* spec;
-  const cplx* spec_roots;  // W_2M^k, k < M
-  int spec_n;
-  double spec_scale;
-  long spec_m;             // M, the points of a row
-};
+using PassArgs = FftPassArgs;
 
 // The packed point at global index g (row g / M, point g % M) from the half spectrum.
 __device__ inline cplx packed_point(const PassArgs& a, long g) {
@@ -53,18 +40,18 @@ __device__ inline cplx twiddle(const cplx* table, int index, int sign) {
   return sign > 0 ? make(w.x, -w.y) : w;
 }
 
+// One workgroup's pass: sequence group `group` of batch row `row`, staged in `buf`
+// (G * STRIDE points).
 template <int LOG2L, int G>
-__global__ void __launch_bounds__((Shape<LOG2L, G>::NT))
-    fft_pass_kernel(const cplx* in, cplx* out,
-                    const cplx* __restrict__ tw_l, const cplx* __restrict__ tw_n,
-                    PassArgs a, int sign) {
+__device__ __forceinline__ void fft_pass_body(const cplx* in, cplx* out, const cplx* __restrict__ tw_l,
+                                     const cplx* __restrict__ tw_n, const PassArgs& a, int sign,
+                                     int group, int row, cplx* buf) {
   using S = Shape<LOG2L, G>;
   constexpr int L = S::L, E = S::E, NT = S::NT, STRIDE = S::STRIDE;
-  __shared__ cplx buf[G * STRIDE];
   const int tid = threadIdx.x;
-  const long s0 = static_cast<long>(blockIdx.x) * G;
-  const cplx* src = in + static_cast<long>(blockIdx.y) * a.batch_in;
-  cplx* dst = out + static_cast<long>(blockIdx.y) * a.batch_out;
+  const long s0 = static_cast<long>(group) * G;
+  const cplx* src = in + static_cast<long>(row) * a.batch_in;
+  cplx* dst = out + static_cast<long>(row) * a.batch_out;
 
   // Load G sequences, the contiguous axis fastest across the threads.
   for (int e = tid; e < E; e += NT) {
@@ -78,7 +65,7 @@ __global__ void __launch_bounds__((Shape<LOG2L, G>::NT))
     }
     const long at = j * a.in_sj + (s0 + s) * a.in_ss;
     buf[s * STRIDE + j] =
-        a.spec ? packed_point(a, static_cast<long>(blockIdx.y) * a.batch_in + at) : src[at];
+        a.spec ? packed_point(a, static_cast<long>(row) * a.batch_in + at) : src[at];
   }
   __syncthreads();
 
@@ -168,6 +155,70 @@ __global__ void __launch_bounds__((Shape<LOG2L, G>::NT))
       value = mul(value, twiddle(tw_n, static_cast<int>(idx), sign));
     }
     dst[k * a.out_sk + (s0 + s) * a.out_ss] = value;
+  }
+}
+
+template <int LOG2L, int G>
+__global__ void __launch_bounds__((Shape<LOG2L, G>::NT))
+    fft_pass_kernel(const cplx* in, cplx* out,
+                    const cplx* __restrict__ tw_l, const cplx* __restrict__ tw_n,
+                    PassArgs a, int sign) {
+  using S = Shape<LOG2L, G>;
+  __shared__ cplx buf[G * S::STRIDE];
+  fft_pass_body<LOG2L, G>(in, out, tw_l, tw_n, a, sign, blockIdx.x, blockIdx.y, buf);
+}
+
+// The same pass over the jobs of a grouped launch: each workgroup finds its job by the
+// jobs' first workgroups (a binary search, uniform across the workgroup) and runs that
+// job's sequence group and row.
+template <int LOG2L, int G>
+__global__ void __launch_bounds__((Shape<LOG2L, G>::NT))
+    fft_pass_grouped(const FftJob* __restrict__ jobs, const int* __restrict__ first, int n_jobs,
+                     int sign) {
+  using S = Shape<LOG2L, G>;
+  __shared__ cplx buf[G * S::STRIDE];
+  const int block = blockIdx.x;
+  const int i = owner_of(first, n_jobs, block);
+  const FftJob& job = jobs[i];
+  if (job.gate && *job.gate == 0) return;
+  const int local = block - first[i];
+  fft_pass_body<LOG2L, G>(job.in, job.out, job.tw_l, job.tw_n, job.a, sign, local % job.groups,
+                          local / job.groups, buf);
+}
+
+using GroupedFn = void (*)(const FftJob*, const int*, int, int);
+
+template <int LOG2L, int G>
+constexpr GroupedFn pick_grouped() {
+  if constexpr ((1 << LOG2L) * G <= kMaxElems)
+    return fft_pass_grouped<LOG2L, G>;
+  else
+    return nullptr;
+}
+
+template <int LOG2L>
+GroupedFn grouped_for_g(int log2g) {
+  switch (log2g) {
+    case 0: return pick_grouped<LOG2L, 1>();
+    case 1: return pick_grouped<LOG2L, 2>();
+    case 2: return pick_grouped<LOG2L, 4>();
+    case 3: return pick_grouped<LOG2L, 8>();
+    case 4: return pick_grouped<LOG2L, 16>();
+    default: return nullptr;
+  }
+}
+
+GroupedFn grouped_for(int log2l, int log2g) {
+  switch (log2l) {
+    case 5: return grouped_for_g<5>(log2g);
+    case 6: return grouped_for_g<6>(log2g);
+    case 7: return grouped_for_g<7>(log2g);
+    case 8: return grouped_for_g<8>(log2g);
+    case 9: return grouped_for_g<9>(log2g);
+    case 10: return grouped_for_g<10>(log2g);
+    case 11: return grouped_for_g<11>(log2g);
+    case 12: return grouped_for_g<12>(log2g);
+    default: return nullptr;
   }
 }
 
@@ -355,6 +406,44 @@ void FftPlan::launch(const FftPass& p, const cplx* in, cplx* out, int sign,
   const cplx* tw_l = (&p == &passes_[0]) ? roots_l_[0] : roots_l_[1];
   hipLaunchKernelGGL(fn, dim3(groups, rows), dim3(threads_for(p.log2L, log2g)), 0, stream,
                      in, out, tw_l, roots_n_, a, sign);
+  SP_HIP(hipGetLastError());
+}
+
+std::vector<FftJob> FftPlan::jobs(cplx* data, cplx* scratch, const PackedSpectrum* packed,
+                                  const int* gate) const {
+  std::vector<FftJob> out;
+  for (std::size_t i = 0; i < passes_.size(); ++i) {
+    const FftPass& p = passes_[i];
+    const bool last = i + 1 == passes_.size();
+    FftJob j{};
+    j.in = i == 0 ? data : scratch;
+    j.out = last ? data : scratch;
+    j.tw_l = i == 0 ? roots_l_[0] : roots_l_[1];
+    j.tw_n = roots_n_;
+    j.a = PassArgs{p.in_sj, p.in_ss, p.out_sk, p.out_ss, p.batch_in, p.batch_out,
+                   p.global_twiddle ? 1 : 0, static_cast<int>((1L << p.log2N) - 1),
+                   nullptr, nullptr, 0, 0.0, static_cast<long>(n_)};
+    if (i == 0 && packed) {
+      j.a.spec = packed->spectrum;
+      j.a.spec_roots = packed->roots;
+      j.a.spec_n = static_cast<int>(packed->n_bins);
+      j.a.spec_scale = packed->scale;
+    }
+    j.groups = p.n_seq >> p.log2G;
+    j.rows = p.batch_in == 0 && p.batch_out == 0 ? 1 : static_cast<int>(batch_);
+    j.gate = gate;
+    out.push_back(j);
+  }
+  return out;
+}
+
+void launch_fft_grouped(int log2L, int log2G, const FftJob* jobs, const int* first, int n_jobs,
+                        int blocks, int sign, hipStream_t stream) {
+  GroupedFn fn = grouped_for(log2L, log2G);
+  SP_CHECK(fn != nullptr, "no grouped FFT pass kernel for this shape");
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(threads_for(log2L, log2G)), 0, stream, jobs, first,
+                     n_jobs, sign);
   SP_HIP(hipGetLastError());
 }
 

@@ -30,6 +30,37 @@ struct FftPass {
   int log2N = 0;            // length of the global twiddle table
 };
 
+// The strides and scalings of one pass's launch (what the pass kernel reads). When
+// `spec` is set, the first pass loads its input from a half spectrum instead: row r's
+// point k is c2r_pre(X[k], X[M - k], W_2M^k) of X = spec row r (spec_n bins, zero beyond)
+// scaled by spec_scale, DC and Nyquist taken real (the packing of a real inverse
+// transform of 2M points as a complex one of M).
+struct FftPassArgs {
+  long in_sj, in_ss, out_sk, out_ss, batch_in, batch_out;
+  int global_twiddle;
+  int mask_n;  // N - 1 for the global twiddle index
+  const cplx* spec;
+  const cplx* spec_roots;  // W_2M^k, k < M
+  int spec_n;
+  double spec_scale;
+  long spec_m;             // M, the points of a row
+};
+
+// One pass of one transform in a grouped launch, which runs the same pass shape for
+// many independent transforms (one per instance of a batched march): the pass's
+// operands and strides, its workgroups per batch row and its rows. The job runs when
+// `*gate` is non-zero (a null gate always runs).
+struct FftJob {
+  const cplx* in;
+  cplx* out;
+  const cplx* tw_l;
+  const cplx* tw_n;
+  FftPassArgs a;
+  int groups;
+  int rows;
+  const int* gate;
+};
+
 // A half spectrum for FftPlan::exec_packed: `n_bins` bins a row (rows contiguous),
 // scaled by `scale`, with the unit roots W_2M^k of the real transform of 2M points.
 struct PackedSpectrum {
@@ -54,6 +85,14 @@ class FftPlan {
   // first pass reads the spectrum; `data` need not hold anything), the result in `data`.
   void exec_packed(const PackedSpectrum& spectrum, cplx* data, cplx* scratch, int sign,
                    hipStream_t stream) const;
+  // The passes exec (packed == null) or exec_packed would launch on these buffers, in
+  // order, as grouped-launch jobs, and each pass's shape (log2 of its length and of the
+  // sequences a workgroup owns).
+  std::vector<FftJob> jobs(cplx* data, cplx* scratch, const PackedSpectrum* packed,
+                           const int* gate) const;
+  int passes() const { return static_cast<int>(passes_.size()); }
+  int pass_log2L(int i) const { return passes_[static_cast<std::size_t>(i)].log2L; }
+  int pass_log2G(int i) const { return passes_[static_cast<std::size_t>(i)].log2G; }
   std::size_t n() const { return n_; }
   std::size_t batch() const { return batch_; }
   bool two_pass() const { return passes_.size() == 2; }
@@ -69,6 +108,11 @@ class FftPlan {
   cplx* roots_n_ = nullptr;   // length n
   cplx* roots_l_[2] = {nullptr, nullptr};  // per-pass local tables, length L
 };
+
+// One grouped launch of a pass shape over `n_jobs` jobs (device array `jobs`): job i
+// owns workgroups first[i] .. first[i+1] - 1 (device array, first[n_jobs] = `blocks`).
+void launch_fft_grouped(int log2L, int log2G, const FftJob* jobs, const int* first, int n_jobs,
+                        int blocks, int sign, hipStream_t stream);
 
 // Host-computed unit roots exp(-2 pi i t / n), t < n, correctly rounded through
 // extended precision and the octant symmetries.

@@ -272,9 +272,8 @@ __device__ inline cplx step_operator(const DeviceProblem& P, const AdamsRecord& 
 // wavefront per workgroup; every global operand is loaded before any arithmetic.
 // Coefficients past the stencil's order are zero in the record.
 template <int W>
-__global__ void __launch_bounds__(kWave)
-    adams_predict(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
-  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void predict_body(const DeviceProblem& P, const AdamsRecord* __restrict__ R,
+                                    const FieldBuffers& B, long idx) {
   const long CN = static_cast<long>(P.C) * P.N;
   if (idx >= CN) return;
   const AdamsRecord& r = *R;
@@ -315,6 +314,12 @@ __global__ void __launch_bounds__(kWave)
   B.lab[idx] = lab;
 }
 
+template <int W>
+__global__ void __launch_bounds__(kWave)
+    adams_predict(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
+  predict_body<W>(P, R, B, static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
 // ---------------------------------------------------- the source's transforms
 
 // The lab-frame modal spectrum (1/T folded in) packed for the real inverse transform
@@ -351,9 +356,9 @@ __global__ void field_scatter(DeviceProblem P, const cplx* spectrum, cplx* modal
 // wave runs its nodes with the tile's modal fields in registers and the rate table in
 // LDS (read from global memory above kLdsTableIntervals intervals).
 template <int C>
-__global__ void __launch_bounds__(kWave * kNodeWaves)
-    plasma_rate_tiles(DeviceProblem P, PlasmaTable pt, FieldBuffers B) {
-  __shared__ double table[4 * kLdsTableIntervals];
+__device__ __forceinline__ void rate_tiles_body(const DeviceProblem& P, const PlasmaTable& pt,
+                                       const FieldBuffers& B, int tile, int group,
+                                       double* table) {
   const bool in_lds = pt.n_intervals <= kLdsTableIntervals;
   if (in_lds)
     for (int i = threadIdx.x; i < 4 * pt.n_intervals; i += blockDim.x) table[i] = B.rate_c[i];
@@ -361,7 +366,6 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   const double* coefficients = in_lds ? table : B.rate_c;
   const int T = P.T, tiles = scan_tiles(T);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const int tile = blockIdx.x;
   const int t = tile * kTileSpan + lane;
   const bool inside = t < T;
   // The shared last sample is stored by the next tile's first lane.
@@ -369,7 +373,7 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   const bool step = lane < kTileSpan && t + 1 < T;
   double am[held(C)];
   load_modal<C>(B, T, t, am);
-  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave;
+  const int q0 = group * kNodeWaves * kNodesPerWave;
 #pragma unroll
   for (int i = 0; i < kNodesPerWave; ++i) {
     const int q = q0 + i * kNodeWaves + wave;
@@ -385,6 +389,13 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
       if (lane == kTileSpan - 1) B.tile_x[static_cast<long>(q) * tiles + tile] = incl;
     }
   }
+}
+
+template <int C>
+__global__ void __launch_bounds__(kWave * kNodeWaves)
+    plasma_rate_tiles(DeviceProblem P, PlasmaTable pt, FieldBuffers B) {
+  __shared__ double table[4 * kLdsTableIntervals];
+  rate_tiles_body<C>(P, pt, B, blockIdx.x, blockIdx.y, table);
 }
 
 // ---------------------------------------------- band running-integral rule
@@ -475,12 +486,18 @@ __device__ inline int extra_sample(int tile, int K, int T, int h) {
 // Pass A under the band rule: the raw rate stored for the owned samples, the filtered
 // rate's trapezoid increments scanned, the within-tile exposure stored and the tile's
 // sum.
+// LDS a band pass's workgroup holds: each wave's window and its nodes' extra samples.
+struct BandLds {
+  double windows[kNodeWaves][kBandSpan];
+  double halos[kNodeWaves][kNodesPerWave * kBandSlots];
+};
+
 template <int C>
-__global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_waves_per_eu(6)))
-    plasma_rate_band(DeviceProblem P, PlasmaTable pt, FieldBuffers B) {
-  __shared__ double table[4 * kLdsTableIntervals];
-  __shared__ double windows[kNodeWaves][kBandSpan];
-  __shared__ double halos[kNodeWaves][kNodesPerWave * kBandSlots];
+__device__ __forceinline__ void rate_band_body(const DeviceProblem& P, const PlasmaTable& pt,
+                                      const FieldBuffers& B, int tile, int group, double* table,
+                                      BandLds& lds) {
+  auto& windows = lds.windows;
+  auto& halos = lds.halos;
   const bool in_lds = pt.n_intervals <= kLdsTableIntervals;
   if (in_lds)
     for (int i = threadIdx.x; i < 4 * pt.n_intervals; i += blockDim.x) table[i] = B.rate_c[i];
@@ -488,13 +505,12 @@ __global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_wave
   const double* coefficients = in_lds ? table : B.rate_c;
   const int T = P.T, tiles = scan_tiles(T), K = pt.n_taps - 1, H = 2 * K;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const int tile = blockIdx.x;
   const int t = tile * kTileSpan + lane;
   const bool stores = t < T && (lane < kTileSpan || t == T - 1);
   const bool step = lane < kTileSpan && t + 1 < T;
   double* window = windows[wave];
   double* halo = halos[wave];
-  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave + wave;
+  const int q0 = group * kNodeWaves * kNodesPerWave + wave;
   const int S = H + (tile == tiles - 1 ? K : 0);  // extra samples a node needs
   for (int idx = lane; idx < kNodesPerWave * S; idx += kWave) {
     const int i = idx / S, h = idx - i * S;
@@ -535,18 +551,26 @@ __global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_wave
   }
 }
 
+template <int C>
+__global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_waves_per_eu(6)))
+    plasma_rate_band(DeviceProblem P, PlasmaTable pt, FieldBuffers B) {
+  __shared__ double table[4 * kLdsTableIntervals];
+  __shared__ BandLds lds;
+  rate_band_body<C>(P, pt, B, blockIdx.x, blockIdx.y, table, lds);
+}
+
 // Pass C under the band rule: the free-current integrand (the fraction from the stored
 // exposure) filtered and its increments scanned; the fraction and the within-tile free
 // current stored for the owned samples, and the tile's sum.
 template <int C>
-__global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_waves_per_eu(8)))
-    plasma_current_band(DeviceProblem P, PlasmaTable pt, const AdamsRecord* __restrict__ R,
-                        FieldBuffers B) {
-  __shared__ double windows[kNodeWaves][kBandSpan];
-  __shared__ double halos[kNodeWaves][kNodesPerWave * kBandSlots];
+__device__ __forceinline__ void current_band_body(const DeviceProblem& P, const PlasmaTable& pt,
+                                         const AdamsRecord* __restrict__ R,
+                                         const FieldBuffers& B, int tile, int group,
+                                         BandLds& lds) {
+  auto& windows = lds.windows;
+  auto& halos = lds.halos;
   const int T = P.T, tiles = scan_tiles(T), K = pt.n_taps - 1, H = 2 * K;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const int tile = blockIdx.x;
   const int t = tile * kTileSpan + lane;
   const int owner = min(t / kTileSpan, tiles - 1);
   const bool stores = t < T && (lane < kTileSpan || t == T - 1);
@@ -554,7 +578,7 @@ __global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_wave
   const double f0 = R->preionisation;
   double* window = windows[wave];
   double* halo = halos[wave];
-  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave + wave;
+  const int q0 = group * kNodeWaves * kNodesPerWave + wave;
   const int S = H + (tile == tiles - 1 ? K : 0);  // extra samples a node needs
   for (int idx = lane; idx < kNodesPerWave * S; idx += kWave) {
     const int i = idx / S, h = idx - i * S;
@@ -599,6 +623,14 @@ __global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_wave
   }
 }
 
+template <int C>
+__global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_waves_per_eu(8)))
+    plasma_current_band(DeviceProblem P, PlasmaTable pt, const AdamsRecord* __restrict__ R,
+                        FieldBuffers B) {
+  __shared__ BandLds lds;
+  current_band_body<C>(P, pt, R, B, blockIdx.x, blockIdx.y, lds);
+}
+
 // Passes B and D: exclusive prefix of the tile sums of each node, one wavefront per
 // node. Each lane sums a run of consecutive tiles in order, the wave scans the run
 // totals, and each lane then writes its run's offsets in order: a fixed order, so the
@@ -608,9 +640,9 @@ __global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_wave
 // The additions are the same, in the same order, either way.
 constexpr int kPrefixRun = 8;
 
-__global__ void __launch_bounds__(kWave) plasma_tile_prefix(int tiles, double* sums) {
+__device__ __forceinline__ void tile_prefix_body(int tiles, double* sums, int node) {
   const int lane = threadIdx.x;
-  double* row = sums + static_cast<long>(blockIdx.x) * tiles;
+  double* row = sums + static_cast<long>(node) * tiles;
   const int run = (tiles + kWave - 1) / kWave;
   const int k0 = lane * run;
   const int k1 = k0 + run < tiles ? k0 + run : tiles;
@@ -641,20 +673,24 @@ __global__ void __launch_bounds__(kWave) plasma_tile_prefix(int tiles, double* s
   }
 }
 
+__global__ void __launch_bounds__(kWave) plasma_tile_prefix(int tiles, double* sums) {
+  tile_prefix_body(tiles, sums, blockIdx.x);
+}
+
 // Pass C: the free-current increments' sum over each tile, and each sample's fraction
 // and within-tile free current for the response pass. Same layout as pass A.
 template <int C>
-__global__ void __launch_bounds__(kWave * kNodeWaves)
-    plasma_current_tiles(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
+__device__ __forceinline__ void current_tiles_body(const DeviceProblem& P,
+                                          const AdamsRecord* __restrict__ R,
+                                          const FieldBuffers& B, int tile, int group) {
   const int T = P.T, tiles = scan_tiles(T);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const int tile = blockIdx.x;
   const int t = tile * kTileSpan + lane;
   const bool stores = t < T && (lane < kTileSpan || t == T - 1);
   const double f0 = R->preionisation;
   double am[held(C)];
   load_modal<C>(B, T, t, am);
-  const int q0 = blockIdx.y * kNodeWaves * kNodesPerWave;
+  const int q0 = group * kNodeWaves * kNodesPerWave;
 #pragma unroll
   for (int i = 0; i < kNodesPerWave; ++i) {
     const int q = q0 + i * kNodeWaves + wave;
@@ -672,6 +708,12 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
   }
 }
 
+template <int C>
+__global__ void __launch_bounds__(kWave * kNodeWaves)
+    plasma_current_tiles(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
+  current_tiles_body<C>(P, R, B, blockIdx.x, blockIdx.y);
+}
+
 // Pass E: Kerr, the fraction, the free and loss currents and the collars at every node,
 // projected onto the modes. A workgroup takes 64 consecutive samples, one a lane (its
 // own blocks, not the scan tiles: T/64 workgroups, T a power of two); each wave takes
@@ -680,12 +722,12 @@ __global__ void __launch_bounds__(kWave * kNodeWaves)
 // the within-tile current that tile stored: tile t / 63, the last tile for the grid's
 // last sample.
 template <int C>
-__global__ void __launch_bounds__(kWave * kResponseWaves)
-    node_response(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
-  __shared__ double red[3 * C * kWave];
+__device__ __forceinline__ void node_response_body(const DeviceProblem& P,
+                                          const AdamsRecord* __restrict__ R,
+                                          const FieldBuffers& B, int block, double* red) {
   const int T = P.T, tiles = scan_tiles(T);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const int t = blockIdx.x * kWave + lane;  // < T: the grid is T/64 workgroups
+  const int t = block * kWave + lane;  // < T: the grid is T/64 workgroups
   const int tile = min(t / kTileSpan, tiles - 1);
   const double density = R->density;
   const double g = P.t_window[t], gd = P.t_deriv[t];
@@ -746,6 +788,13 @@ __global__ void __launch_bounds__(kWave * kResponseWaves)
 #pragma unroll
     for (int j = 0; j < 3 * C; ++j) streams[static_cast<long>(j) * T + t] = red[j * kWave + lane];
   }
+}
+
+template <int C>
+__global__ void __launch_bounds__(kWave * kResponseWaves)
+    node_response(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
+  __shared__ double red[3 * C * kWave];
+  node_response_body<C>(P, R, B, blockIdx.x, red);
 }
 
 // Pass E of the dense route, one thread per node and sample: the un-gated Kerr
@@ -880,11 +929,10 @@ __device__ inline void finish_bin(const DeviceProblem& P, const AdamsRecord& r,
 // The finish over bin pairs: the packed real transform holds bins k and T/2 - k in the
 // same two entries (k = 0 pairs with the Nyquist bin T/2, both from entry 0), so one
 // thread reads them once and writes both bins that the coarse grid keeps.
-__global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B,
-                             cplx* out) {
+__device__ __forceinline__ void finish_body(const DeviceProblem& P, const AdamsRecord* __restrict__ R,
+                                   const FieldBuffers& B, cplx* out, long idx) {
   const int half = P.T / 2;
   const int pairs = half / 2 + 1;
-  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= static_cast<long>(P.C) * pairs) return;
   const int c = static_cast<int>(idx / pairs), k = static_cast<int>(idx % pairs);
   const int mirror = half - k;  // the partner bin, T/2 for k = 0
@@ -902,6 +950,11 @@ __global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R,
   }
   finish_bin(P, *R, B, c, k, za, zb, n_streams, out);
   if (mirror != k && mirror < P.N) finish_bin(P, *R, B, c, mirror, zb, za, n_streams, out);
+}
+
+__global__ void field_finish(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B,
+                             cplx* out) {
+  finish_body(P, R, B, out, static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
 // ------------------------------------------------------------ temporal absorber
@@ -950,11 +1003,15 @@ __global__ void field_split_unpack(DeviceProblem P, FieldBuffers B) {
 // stencil of width W. One wavefront per workgroup, operands loaded first; each wave's
 // sums are reduced across its lanes in a fixed order and written as that workgroup's
 // partials. Coefficients past each order are zero in the record.
+// The corrector's partial sums of one workgroup, in its waves' LDS cells.
+using WaveSums = double[1 + kMaxScored][kCorrectThreads / kWave];
+
 template <int W>
-__global__ void __launch_bounds__(kCorrectThreads)
-    adams_correct(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
+__device__ __forceinline__ void correct_body(const DeviceProblem& P, const AdamsRecord* __restrict__ R,
+                                    const FieldBuffers& B, int block, int blocks,
+                                    WaveSums& wave_sums) {
   const long CN = static_cast<long>(P.C) * P.N;
-  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long idx = static_cast<long>(block) * blockDim.x + threadIdx.x;
   const AdamsRecord& r = *R;
   double v[1 + kMaxScored] = {0.0, 0.0, 0.0, 0.0};
   if (idx < CN) {
@@ -1020,7 +1077,6 @@ __global__ void __launch_bounds__(kCorrectThreads)
     }
   }
   // Each wave's sums across its lanes, then the workgroup's waves in order.
-  __shared__ double wave_sums[1 + kMaxScored][kCorrectThreads / kWave];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
   for (int s = 0; s < 1 + kMaxScored; ++s) {
@@ -1032,8 +1088,15 @@ __global__ void __launch_bounds__(kCorrectThreads)
     const int s = threadIdx.x;
     double total = wave_sums[s][0];
     for (int w = 1; w < static_cast<int>(blockDim.x) / kWave; ++w) total = total + wave_sums[s][w];
-    B.partials[s * gridDim.x + blockIdx.x] = total;
+    B.partials[s * blocks + block] = total;
   }
+}
+
+template <int W>
+__global__ void __launch_bounds__(kCorrectThreads)
+    adams_correct(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
+  __shared__ WaveSums wave_sums;
+  correct_body<W>(P, R, B, blockIdx.x, gridDim.x, wave_sums);
 }
 
 // The corrector's block sums reduced to the 1 + kMaxScored totals in a fixed order, written
@@ -1042,9 +1105,8 @@ __global__ void __launch_bounds__(kCorrectThreads)
 // then a fixed pairwise tree in LDS.
 constexpr int kNormGroup = 256;
 constexpr int kNormThreads = kNormGroup * (1 + kMaxScored);
-__global__ void __launch_bounds__(kNormThreads)
-    adams_norms(const double* __restrict__ partials, int blocks, double* norms) {
-  __shared__ double tree[kNormThreads];
+__device__ __forceinline__ void norms_body(const double* __restrict__ partials, int blocks,
+                                  double* norms, double* tree) {
   const int s = threadIdx.x / kNormGroup, j = threadIdx.x % kNormGroup;
   const double* row = partials + static_cast<long>(s) * blocks;
   double total = 0.0;
@@ -1059,6 +1121,12 @@ __global__ void __launch_bounds__(kNormThreads)
   if (j == 0) norms[s] = tree[threadIdx.x];
 }
 
+__global__ void __launch_bounds__(kNormThreads)
+    adams_norms(const double* __restrict__ partials, int blocks, double* norms) {
+  __shared__ double tree[kNormThreads];
+  norms_body(partials, blocks, norms, tree);
+}
+
 __global__ void adams_accept(DeviceProblem P, int slot, int compensated, FieldBuffers B) {
   const long CN = static_cast<long>(P.C) * P.N;
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1069,9 +1137,9 @@ __global__ void adams_accept(DeviceProblem P, int slot, int compensated, FieldBu
 }
 
 // The same commit, its slot read from the record: the head of the next trial's graph.
-__global__ void adams_commit(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
+__device__ __forceinline__ void commit_body(const DeviceProblem& P, const AdamsRecord* __restrict__ R,
+                                   const FieldBuffers& B, long idx) {
   const long CN = static_cast<long>(P.C) * P.N;
-  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= CN) return;
   const int slot = R->free_slot;
   B.store[slot * CN + idx] = B.source[idx];
@@ -1079,12 +1147,203 @@ __global__ void adams_commit(DeviceProblem P, const AdamsRecord* __restrict__ R,
   B.y[idx] = P.active[idx % P.N] ? B.advanced[idx] : make(0.0, 0.0);
 }
 
-__global__ void adams_start(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
+__global__ void adams_commit(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
+  commit_body(P, R, B, static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+__device__ __forceinline__ void start_body(const DeviceProblem& P, const AdamsRecord* __restrict__ R,
+                                  const FieldBuffers& B, long idx) {
   const long CN = static_cast<long>(P.C) * P.N;
-  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= CN) return;
   B.store[idx] = B.source[idx];
   if (R->compensated) B.states[idx] = B.y[idx];
+}
+
+__global__ void adams_start(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
+  start_body(P, R, B, static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+// ------------------------------------------------------------ batched marches
+
+// The grouped kernels (field.hpp): each workgroup finds its instance by binary search
+// over the launch's first workgroups, reads the instance's record and runs the plain
+// kernel's body for its local workgroup, with the instance's mode count and stencil
+// width as a switch (uniform across the workgroup).
+
+namespace {
+
+struct Owned {
+  const BatchInstance* inst;
+  const AdamsRecord* R;
+  int local;
+};
+
+// The grouped kernels take the instance table and the first workgroups as read-only,
+// unaliased arguments, so that the uniform reads of an instance's problem and record
+// are scalar loads, as they are from the plain kernels' arguments.
+#define SP_BATCH_ARGS                                                                \
+  const BatchInstance* __restrict__ insts, const AdamsRecord* __restrict__ records, \
+      const int* __restrict__ first, int n_inst
+
+__device__ inline Owned owned(const BatchInstance* __restrict__ insts,
+                              const AdamsRecord* __restrict__ records,
+                              const int* __restrict__ first, int n_inst) {
+  const int i = owner_of(first, n_inst, blockIdx.x);
+  return {insts + i, records + i, static_cast<int>(blockIdx.x) - first[i]};
+}
+
+template <template <int> class Body, typename... Args>
+__device__ __forceinline__ void with_width(int width, Args&&... args) {
+  switch (width) {
+    case 2: Body<2>::run(args...); break;
+    case 3: Body<3>::run(args...); break;
+    case 4: Body<4>::run(args...); break;
+    case 5: Body<5>::run(args...); break;
+    case 6: Body<6>::run(args...); break;
+    default: Body<7>::run(args...); break;
+  }
+}
+
+template <template <int> class Body, typename... Args>
+__device__ __forceinline__ void with_modes(int modes, Args&&... args) {
+  switch (modes) {
+    case 1: Body<1>::run(args...); break;
+    case 2: Body<2>::run(args...); break;
+    case 3: Body<3>::run(args...); break;
+    case 4: Body<4>::run(args...); break;
+    case 5: Body<5>::run(args...); break;
+    default: Body<6>::run(args...); break;
+  }
+}
+
+template <int W>
+struct PredictAt {
+  __device__ __forceinline__ static void run(const BatchInstance& in, const AdamsRecord* __restrict__ R,
+                             long idx) {
+    predict_body<W>(in.P, R, in.B, idx);
+  }
+};
+
+template <int W>
+struct CorrectAt {
+  __device__ __forceinline__ static void run(const BatchInstance& in, const AdamsRecord* __restrict__ R,
+                             int block, WaveSums& sums) {
+    correct_body<W>(in.P, R, in.B, block, in.correct_blocks, sums);
+  }
+};
+
+template <int C>
+struct RateAt {
+  __device__ __forceinline__ static void run(const BatchInstance& in, int tile, int group, double* table,
+                             BandLds& lds) {
+    if (in.pt.n_taps > 1)
+      rate_band_body<C>(in.P, in.pt, in.B, tile, group, table, lds);
+    else
+      rate_tiles_body<C>(in.P, in.pt, in.B, tile, group, table);
+  }
+};
+
+template <int C>
+struct CurrentAt {
+  __device__ __forceinline__ static void run(const BatchInstance& in, const AdamsRecord* __restrict__ R,
+                             int tile, int group, BandLds& lds) {
+    if (in.pt.n_taps > 1)
+      current_band_body<C>(in.P, in.pt, R, in.B, tile, group, lds);
+    else
+      current_tiles_body<C>(in.P, R, in.B, tile, group);
+  }
+};
+
+template <int C>
+struct ResponseAt {
+  __device__ __forceinline__ static void run(const BatchInstance& in, const AdamsRecord* __restrict__ R,
+                             int block, double* red) {
+    node_response_body<C>(in.P, R, in.B, block, red);
+  }
+};
+
+}  // namespace
+
+__global__ void __launch_bounds__(kThreads) batch_commit(SP_BATCH_ARGS) {
+  const Owned o = owned(insts, records, first, n_inst);
+  if (o.R->free_slot < 0) return;
+  commit_body(o.inst->P, o.R, o.inst->B, static_cast<long>(o.local) * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(kWave) batch_predict(SP_BATCH_ARGS) {
+  const Owned o = owned(insts, records, first, n_inst);
+  const BatchInstance& in = *o.inst;
+  const int action = o.R->action;
+  const long idx = static_cast<long>(o.local) * blockDim.x + threadIdx.x;
+  if (action == kBatchStart) {
+    // The start's source is evaluated at the state itself.
+    if (idx < static_cast<long>(in.P.C) * in.P.N) in.B.lab[idx] = in.B.y[idx];
+    return;
+  }
+  if (action != kBatchTrial) return;
+  with_width<PredictAt>(o.R->width, in, o.R, idx);
+}
+
+__global__ void __launch_bounds__(kWave * kNodeWaves) batch_rate(SP_BATCH_ARGS) {
+  __shared__ double table[4 * kLdsTableIntervals];
+  __shared__ BandLds lds;
+  const Owned o = owned(insts, records, first, n_inst);
+  const BatchInstance& in = *o.inst;
+  if (o.R->action == kBatchIdle) return;
+  const int tiles = scan_tiles(in.P.T);
+  with_modes<RateAt>(in.P.C, in, o.local % tiles, o.local / tiles, table, lds);
+}
+
+__global__ void __launch_bounds__(kWave) batch_prefix(SP_BATCH_ARGS, int current) {
+  const Owned o = owned(insts, records, first, n_inst);
+  const BatchInstance& in = *o.inst;
+  if (o.R->action == kBatchIdle) return;
+  tile_prefix_body(scan_tiles(in.P.T), current ? in.B.tile_f : in.B.tile_x, o.local);
+}
+
+__global__ void __launch_bounds__(kWave * kNodeWaves) batch_current(SP_BATCH_ARGS) {
+  __shared__ BandLds lds;
+  const Owned o = owned(insts, records, first, n_inst);
+  const BatchInstance& in = *o.inst;
+  if (o.R->action == kBatchIdle) return;
+  const int tiles = scan_tiles(in.P.T);
+  with_modes<CurrentAt>(in.P.C, in, o.R, o.local % tiles, o.local / tiles, lds);
+}
+
+__global__ void __launch_bounds__(kWave * kResponseWaves) batch_response(SP_BATCH_ARGS) {
+  __shared__ double red[3 * kMaxBatchModes * kWave];
+  const Owned o = owned(insts, records, first, n_inst);
+  const BatchInstance& in = *o.inst;
+  if (o.R->action == kBatchIdle) return;
+  with_modes<ResponseAt>(in.P.C, in, o.R, o.local, red);
+}
+
+__global__ void __launch_bounds__(kWave) batch_finish(SP_BATCH_ARGS) {
+  const Owned o = owned(insts, records, first, n_inst);
+  const BatchInstance& in = *o.inst;
+  if (o.R->action == kBatchIdle) return;
+  finish_body(in.P, o.R, in.B, in.B.source, static_cast<long>(o.local) * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(kWave) batch_correct(SP_BATCH_ARGS) {
+  __shared__ WaveSums sums;
+  const Owned o = owned(insts, records, first, n_inst);
+  const BatchInstance& in = *o.inst;
+  const int action = o.R->action;
+  if (action == kBatchStart) {
+    start_body(in.P, o.R, in.B, static_cast<long>(o.local) * blockDim.x + threadIdx.x);
+    return;
+  }
+  if (action != kBatchTrial) return;
+  with_width<CorrectAt>(o.R->width, in, o.R, o.local, sums);
+}
+
+__global__ void __launch_bounds__(kNormThreads) batch_norms(SP_BATCH_ARGS) {
+  __shared__ double tree[kNormThreads];
+  const Owned o = owned(insts, records, first, n_inst);
+  const BatchInstance& in = *o.inst;
+  if (o.R->action != kBatchTrial) return;
+  norms_body(in.B.partials, in.correct_blocks, in.B.norms, tree);
 }
 
 // -------------------------------------------------------------------- launchers
@@ -1315,5 +1574,40 @@ void launch_adams_start(const DeviceProblem& P, const AdamsRecord* R, FieldBuffe
   hipLaunchKernelGGL(adams_start, dim3(cn_blocks(P)), dim3(kThreads), 0, s, P, R, B);
   check();
 }
+
+}  // namespace sp
+
+namespace sp {
+
+namespace {
+template <typename K>
+void grouped(K kernel, const BatchGrid& g, unsigned threads, hipStream_t s) {
+  if (g.blocks <= 0) return;
+  hipLaunchKernelGGL(kernel, dim3(g.blocks), dim3(threads), 0, s, g.inst, g.records, g.first,
+                     g.n);
+  SP_HIP(hipGetLastError());
+}
+}  // namespace
+
+void launch_batch_commit(const BatchGrid& g, hipStream_t s) { grouped(batch_commit, g, kThreads, s); }
+void launch_batch_predict(const BatchGrid& g, hipStream_t s) { grouped(batch_predict, g, kWave, s); }
+void launch_batch_rate(const BatchGrid& g, hipStream_t s) {
+  grouped(batch_rate, g, kWave * kNodeWaves, s);
+}
+void launch_batch_prefix(const BatchGrid& g, bool current, hipStream_t s) {
+  if (g.blocks <= 0) return;
+  hipLaunchKernelGGL(batch_prefix, dim3(g.blocks), dim3(kWave), 0, s, g.inst, g.records,
+                     g.first, g.n, current ? 1 : 0);
+  SP_HIP(hipGetLastError());
+}
+void launch_batch_current(const BatchGrid& g, hipStream_t s) {
+  grouped(batch_current, g, kWave * kNodeWaves, s);
+}
+void launch_batch_response(const BatchGrid& g, hipStream_t s) {
+  grouped(batch_response, g, kWave * kResponseWaves, s);
+}
+void launch_batch_finish(const BatchGrid& g, hipStream_t s) { grouped(batch_finish, g, kWave, s); }
+void launch_batch_correct(const BatchGrid& g, hipStream_t s) { grouped(batch_correct, g, kWave, s); }
+void launch_batch_norms(const BatchGrid& g, hipStream_t s) { grouped(batch_norms, g, kNormThreads, s); }
 
 }  // namespace sp

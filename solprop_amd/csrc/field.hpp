@@ -52,7 +52,9 @@ struct AdamsRecord {
   double comp_pred_w[kMaxScored][kMaxAdamsWidth];      // weights of nodes[:k]
   double comp_corr_w[kMaxScored][kMaxAdamsWidth + 1];  // weights of [1, nodes[:k]]
   int slots[kMaxAdamsWidth];             // store rows read, newest first
-  int order, width, n_orders, n_read, compensated, free_slot, pad0, pad1;
+  // `action` is read by the grouped kernels of a batched march only: 0 the instance is
+  // idle this round, 1 it starts (the source at its state), 2 it takes a trial.
+  int order, width, n_orders, n_read, compensated, free_slot, action, pad1;
   int orders[kMaxScored];
   int pad2;
   double raman_rate[kMaxTerms];          // Raman decay rates at the source position
@@ -103,6 +105,49 @@ struct PlasmaTable {
   int n_taps;
   double taps[kMaxBandHalf + 1];
 };
+
+// ------------------------------------------------------------ batched marches
+
+constexpr int kMaxBatchModes = 6;  // modes an instance of a batched march may have
+
+enum BatchAction : int { kBatchIdle = 0, kBatchStart = 1, kBatchTrial = 2 };
+
+// One instance of a batched march as the grouped kernels read it (device memory): its
+// problem, buffers and plasma table, and its corrector's workgroups.
+struct BatchInstance {
+  DeviceProblem P;
+  FieldBuffers B;
+  PlasmaTable pt;
+  int correct_blocks;
+};
+
+// A grouped launch: `n` instances (device arrays `inst` and their records `records`),
+// instance i owning workgroups first[i] .. first[i+1] - 1 of a grid of `blocks` (device
+// array `first`).
+struct BatchGrid {
+  const BatchInstance* inst;
+  const AdamsRecord* records;
+  const int* first;
+  int n;
+  int blocks;
+};
+
+// The grouped kernels of one batched trial, in launch order: each runs its pass for every
+// instance whose record asks for one (a start evaluates the source at the state and
+// stores it; a trial predicts, evaluates and corrects), idle instances' workgroups
+// exiting at once. The commit applies a pending accepted step (free_slot >= 0) whatever
+// the action. Grids are sized on the host: commit ceil(C N / 256), predict and correct
+// ceil(C N / 64), the two node passes tiles x ceil(Q / 32) (plasma instances only), the
+// prefixes Q, the response T / 64, the finish ceil(C (T/4 + 1) / 64), the norms 1.
+void launch_batch_commit(const BatchGrid& g, hipStream_t s);
+void launch_batch_predict(const BatchGrid& g, hipStream_t s);
+void launch_batch_rate(const BatchGrid& g, hipStream_t s);
+void launch_batch_prefix(const BatchGrid& g, bool current, hipStream_t s);
+void launch_batch_current(const BatchGrid& g, hipStream_t s);
+void launch_batch_response(const BatchGrid& g, hipStream_t s);
+void launch_batch_finish(const BatchGrid& g, hipStream_t s);
+void launch_batch_correct(const BatchGrid& g, hipStream_t s);
+void launch_batch_norms(const BatchGrid& g, hipStream_t s);
 
 unsigned correct_threads(const DeviceProblem& P);
 unsigned correct_blocks(const DeviceProblem& P);

@@ -29,7 +29,11 @@ T* FieldWorkspace::upload(const T* host, std::size_t n) {
   return p;
 }
 
-FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
+FieldWorkspace::FieldWorkspace(const sp_problem_desc& d, hipStream_t shared) {
+  if (shared) {
+    stream_ = shared;
+    own_stream_ = false;
+  }
   // Refusals first, before anything is allocated.
   SP_CHECK(d.kind == 1 && d.method == 1,
            "the field workspace runs the real field grid under the exponential Adams march");
@@ -110,7 +114,7 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d) {
 
 void FieldWorkspace::allocate(const sp_problem_desc& d) {
   const int C = d.n_modes, N = d.n_omega, T = d.n_to, Q = d.n_nodes;
-  SP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  if (own_stream_) SP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 
   P_.kind = 1;
   P_.C = C;
@@ -233,7 +237,8 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
     raman_kernel_plan_.init(T, 1);
   }
   rec_dev_ = alloc<AdamsRecord>(1);
-  SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&rec_host_), sizeof(AdamsRecord)));
+  SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&own_rec_host_), sizeof(AdamsRecord)));
+  rec_host_ = own_rec_host_;
   std::memset(rec_host_, 0, sizeof(AdamsRecord));
   for (hipEvent_t& e : marks_) SP_HIP(hipEventCreate(&e));
   SP_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
@@ -254,7 +259,7 @@ void FieldWorkspace::release() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   for (void* p : allocations_) (void)hipFree(p);
-  if (rec_host_) (void)hipHostFree(rec_host_);
+  if (own_rec_host_) (void)hipHostFree(own_rec_host_);
   if (norms_host_) (void)hipHostFree(norms_host_);
   for (hipEvent_t& e : marks_) {
     if (e) (void)hipEventDestroy(e);
@@ -262,10 +267,11 @@ void FieldWorkspace::release() {
   }
   if (done_) (void)hipEventDestroy(done_);
   done_ = nullptr;
-  if (stream_) (void)hipStreamDestroy(stream_);
+  if (stream_ && own_stream_) (void)hipStreamDestroy(stream_);
   graphs_.clear();
   allocations_.clear();
   rec_host_ = nullptr;
+  own_rec_host_ = nullptr;
   norms_host_ = nullptr;
   stream_ = nullptr;
 }
@@ -493,7 +499,7 @@ void FieldWorkspace::start(double z) {
   evaluations_ += 1;
 }
 
-std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Stencil& s) {
+void FieldWorkspace::fill_trial(double z, double dz, const adams::Stencil& s) {
   AdamsRecord& r = *rec_host_;
   const double end = z + dz;
   r.dz = dz;
@@ -559,6 +565,22 @@ std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Sten
     for (int j = 0; j <= k; ++j) r.comp_corr_w[i][j] = weights[j];
   }
   r.free_slot = pending_slot_;
+  held_rate_end_.assign(r.rate_end_w, r.rate_end_w + kMaxTerms);
+}
+
+std::vector<double> FieldWorkspace::trial_norms() {
+  const AdamsRecord& r = *rec_host_;
+  const double magnitude = std::sqrt(norms_host_[0]);
+  const double divisor = rtol_ * magnitude + atol_;
+  std::vector<double> norms(r.n_orders);
+  for (int i = 0; i < r.n_orders; ++i) norms[i] = std::sqrt(norms_host_[1 + i]) / divisor;
+  trials_ += 1;
+  evaluations_ += 1;
+  return norms;
+}
+
+std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Stencil& s) {
+  fill_trial(z, dz, s);
   run((pending_slot_ >= 0 ? kCommitTrial : kTrial) + 16 * width_);
   pending_slot_ = -1;
   if (marking_)
@@ -575,14 +597,7 @@ std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Sten
     time_total_ms_ += ms;
     timed_ += 1;
   }
-  held_rate_end_.assign(r.rate_end_w, r.rate_end_w + kMaxTerms);
-  const double magnitude = std::sqrt(norms_host_[0]);
-  const double divisor = rtol_ * magnitude + atol_;
-  std::vector<double> norms(r.n_orders);
-  for (int i = 0; i < r.n_orders; ++i) norms[i] = std::sqrt(norms_host_[1 + i]) / divisor;
-  trials_ += 1;
-  evaluations_ += 1;
-  return norms;
+  return trial_norms();
 }
 
 void FieldWorkspace::accept(double dz, int free, const adams::Stencil&,
@@ -816,6 +831,140 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
   }
   out.insert(out.end(), trace.begin(), trace.end());
   return out;
+}
+
+// ------------------------------------------------------------ batched marching
+
+std::string FieldWorkspace::batch_refusal() const {
+  if (dense_) return "a batched march runs the modal route (at most 6 modes), not the dense one";
+  if (P_.C > kMaxBatchModes) return "a batched march instance has at most 6 modes";
+  if (P_.n_raman) return "a batched march instance has no Raman response";
+  if (absorbing() || split_) return "a batched march instance has no absorber";
+  if (correct_threads(P_) != kWave) return "the instance's corrector needs one-wave workgroups";
+  return {};
+}
+
+void FieldWorkspace::batch_bind(AdamsRecord* host, AdamsRecord* device) {
+  *host = *rec_host_;
+  rec_host_ = host;
+  rec_dev_ = device;
+  // The workspace's own graphs read its own record: they are not used again.
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  graphs_.clear();
+  use_graphs_ = false;
+}
+
+BatchInstance FieldWorkspace::batch_instance() const {
+  BatchInstance in{};
+  in.P = P_;
+  in.B = B_;
+  in.pt = pt_;
+  in.correct_blocks = n_blocks_;
+  return in;
+}
+
+std::vector<FftJob> FieldWorkspace::batch_inverse_jobs(const int* gate) const {
+  const PackedSpectrum packed{B_.lab, B_.roots_T, P_.N, P_.inv_T};
+  // The plan copies the spectrum's fields into the jobs; `packed` may go.
+  return inv_plan_.jobs(B_.modal_t, scratch_, &packed, gate);
+}
+
+std::vector<FftJob> FieldWorkspace::batch_forward_jobs(const int* gate) const {
+  return fwd_plan_.jobs(B_.streams, scratch_, nullptr, gate);
+}
+
+void FieldWorkspace::batch_begin(double target, long work_limit, std::vector<double>* rows) {
+  if (!loaded_) throw Error(kState, "load a state before advancing");
+  host_.rows = rows;
+  host_.limit = work_limit;
+  host_.attempts = 0;
+  host_.accepted = 0;
+  batch_target_ = target;
+  batch_finished_ = false;
+  batch_done_ = false;
+  batch_status_ = kOk;
+  batch_message_.clear();
+}
+
+int FieldWorkspace::batch_plan() {
+  AdamsRecord& r = *rec_host_;
+  // A pending commit rides at the head of the round whatever the action.
+  r.free_slot = pending_slot_;
+  r.action = kBatchIdle;
+  if (batch_finished_) return kBatchIdle;
+  try {
+    switch (march_->plan(batch_target_, &request_)) {
+      case adams::Request::kDone:
+        batch_finished_ = true;
+        batch_done_ = march::reached(march_->z, batch_target_);
+        return kBatchIdle;
+      case adams::Request::kHalted:
+        batch_finished_ = true;
+        return kBatchIdle;
+      case adams::Request::kStart:
+        fill_position(request_.z);
+        for (int t = 0; t < P_.n_phi; ++t) slot_rates_[0][t] = rate_paths_[t](request_.z);
+        r.action = kBatchStart;
+        return kBatchStart;
+      case adams::Request::kTrial:
+        fill_trial(request_.z, request_.dz, request_.stencil);
+        r.action = kBatchTrial;
+        return kBatchTrial;
+    }
+  } catch (const Error& e) {
+    batch_status_ = e.code;
+    batch_message_ = e.what();
+  } catch (const march::ConvergenceError& e) {
+    batch_status_ = kConvergence;
+    batch_message_ = e.what();
+  } catch (const std::exception& e) {
+    batch_status_ = kBackend;
+    batch_message_ = e.what();
+  }
+  batch_finished_ = true;
+  r.action = kBatchIdle;
+  return kBatchIdle;
+}
+
+void FieldWorkspace::batch_conclude(int action) {
+  // The round carried the pending commit (if any) at its head.
+  pending_slot_ = -1;
+  if (action == kBatchStart) {
+    evaluations_ += 1;
+    march_->started();
+    return;
+  }
+  if (action != kBatchTrial) return;
+  try {
+    march_->conclude(request_, trial_norms());
+  } catch (const Error& e) {
+    batch_status_ = e.code;
+    batch_message_ = e.what();
+    batch_finished_ = true;
+  } catch (const march::ConvergenceError& e) {
+    batch_status_ = kConvergence;
+    batch_message_ = e.what();
+    batch_finished_ = true;
+  } catch (const std::exception& e) {
+    batch_status_ = kBackend;
+    batch_message_ = e.what();
+    batch_finished_ = true;
+  }
+}
+
+int FieldWorkspace::batch_end(sp_march_result* out, std::string* message) {
+  // An accepted step the last round concluded is committed here, on the batch's stream.
+  flush();
+  host_.rows = nullptr;
+  z_ = march_->z;
+  out->z = march_->z;
+  out->proposal = march_->proposal;
+  out->done = batch_done_ ? 1 : 0;
+  out->attempts = host_.attempts;
+  out->accepted = host_.accepted;
+  out->evaluations = evaluations_;
+  if (message) *message = batch_message_;
+  return batch_status_;
 }
 
 sp_counters FieldWorkspace::counters() const {

@@ -5,6 +5,7 @@
 #pragma once
 
 #include <map>
+#include <string>
 #include <memory>
 #include <vector>
 
@@ -20,7 +21,9 @@ namespace sp {
 
 class FieldWorkspace final : public Propagator, private adams::Engine {
  public:
-  explicit FieldWorkspace(const sp_problem_desc& desc);
+  // `shared`, when given, is the stream the workspace runs on (a batched march's: the
+  // workspace neither creates nor destroys a stream of its own).
+  explicit FieldWorkspace(const sp_problem_desc& desc, hipStream_t shared = nullptr);
   ~FieldWorkspace() override;
   FieldWorkspace(const FieldWorkspace&) = delete;
   FieldWorkspace& operator=(const FieldWorkspace&) = delete;
@@ -43,6 +46,31 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   void set_profile(bool on) override { profile_ = on; }
   std::vector<std::pair<const char*, double>> phases() const override;
   bool phase_used(int i) const;
+
+  // ---- batched marching (scan.hpp): the scan runs many workspaces' trials in one set of
+  // grouped launches; each workspace plans its own march's next action and concludes it.
+  // Why this problem cannot join a batched march (empty when it can).
+  std::string batch_refusal() const;
+  // Use a batch's record slot (pinned host and device) instead of the workspace's own.
+  void batch_bind(AdamsRecord* host, AdamsRecord* device);
+  // The grouped kernels' view of this instance, and its transforms as grouped-launch jobs
+  // gated by its record's action.
+  BatchInstance batch_instance() const;
+  std::vector<FftJob> batch_inverse_jobs(const int* gate) const;
+  std::vector<FftJob> batch_forward_jobs(const int* gate) const;
+  const FftPlan& inverse_plan() const { return inv_plan_; }
+  const FftPlan& forward_plan() const { return fwd_plan_; }
+  const DeviceProblem& device_problem() const { return P_; }
+  // One advance call's course: begin, then per round plan (fills the record; returns the
+  // action) and, after the round's launches, conclude; finished() once the march stops.
+  void batch_begin(double target, long work_limit, std::vector<double>* rows);
+  int batch_plan();
+  void batch_conclude(int action);
+  bool batch_finished() const { return batch_finished_; }
+  bool batch_pending() const { return pending_slot_ >= 0; }
+  // The call's outcome: the status (SP_OK or the march's failure) and its message, and
+  // the result record; `batch_end` flushes a commit no round carried.
+  int batch_end(sp_march_result* r, std::string* message);
   static constexpr int kPhases = 16;
   double trial_us() const override { return timed_ ? time_total_ms_ * 1e3 / timed_ : 0.0; }
   long timed() const override { return timed_; }
@@ -75,6 +103,8 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   void wait();
   void flush();
   void mark(int i);
+  void fill_trial(double z, double dz, const adams::Stencil& s);
+  std::vector<double> trial_norms();
 
   DeviceProblem P_{};
   PlasmaTable pt_{};
@@ -82,10 +112,12 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   std::vector<void*> allocations_;
   std::size_t device_bytes_ = 0;
   hipStream_t stream_ = nullptr;
+  bool own_stream_ = true;
   FftPlan inv_plan_, fwd_plan_;
   cplx* scratch_ = nullptr;
   AdamsRecord* rec_dev_ = nullptr;
   AdamsRecord* rec_host_ = nullptr;
+  AdamsRecord* own_rec_host_ = nullptr;  // the workspace's pinned record (freed at release)
   double* norms_host_ = nullptr;
   int n_blocks_ = 0;
   std::map<int, hipGraphExec_t> graphs_;
@@ -125,6 +157,12 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   std::vector<double> held_rate_end_;
   double z_ = 0.0, dz_init_ = 0.0;
   bool loaded_ = false;
+  // A batched advance call's state.
+  adams::Request request_;
+  double batch_target_ = 0.0;
+  bool batch_finished_ = true, batch_done_ = false;
+  int batch_status_ = 0;
+  std::string batch_message_;
   // Counters and timing.
   long trials_ = 0, evaluations_ = 0, accepted_ = 0, graph_launches_ = 0;
   int pending_slot_ = -1;  // an accepted step's store slot, committed with the next trial

@@ -407,8 +407,8 @@ class Workspace:
 
 
 def advance_batch(sessions, targets, work_limit: int = 256) -> list:
-    """Advance independent sessions concurrently (``sp_batch_advance``): one host thread
-    a workspace, each on its own stream. Each session's march is the one
+    """Advance independent sessions concurrently (``sp_batch_advance``): a fixed pool of
+    host threads, each workspace on its own stream. Each session's march is the one
     :meth:`Session.advance` would make. Returns each session's :class:`Batch`."""
     n = len(sessions)
     targets = np.ascontiguousarray(np.broadcast_to(np.asarray(targets, float), (n,)))
@@ -427,6 +427,91 @@ def advance_batch(sessions, targets, work_limit: int = 256) -> list:
         session.z = r.z
         out.append(Batch(r.z, bool(r.done), rows[i, : r.n_rows].copy(), r.evaluations))
     return out
+
+
+class Scan:
+    """A parameter scan's points as one batched march (``sp_scan_*``).
+
+    The reference runs a scan's points on a process pool, one ``Session`` each
+    (``src/solprop/driver/scan.py:30``); here all points advance together, every round
+    of trials one captured graph of grouped launches on the GPU, each point keeping its
+    own adaptive march. ``advance`` returns one :class:`Batch` per point, each the one
+    that point's own :meth:`Session.advance` would return.
+    """
+
+    def __init__(self, problems, settings: Settings | None = None):
+        self._lib = _lib.lib()
+        self._keeps = []
+        descs = (_lib.ProblemDescC * len(problems))()
+        for i, p in enumerate(problems):
+            p.validate()
+            desc, keep = lower(p, settings)
+            descs[i] = desc
+            self._keeps.append(keep)
+        self.problems = list(problems)
+        handle = C.c_void_p()
+        msg = C.create_string_buffer(1024)
+        _check(self._lib.sp_scan_create(descs, len(problems), C.byref(handle), msg, 1024), msg)
+        self._handle = handle
+        self.z = np.zeros(len(problems))
+
+    def __len__(self) -> int:
+        return len(self.problems)
+
+    def close(self) -> None:
+        if self._handle:
+            self._lib.sp_scan_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, i: int, z: float, field: np.ndarray) -> None:
+        field = np.ascontiguousarray(field, dtype=np.complex128)
+        msg = C.create_string_buffer(1024)
+        _check(self._lib.sp_scan_load(self._handle, int(i), float(z), _ptr(field), msg, 1024),
+               msg)
+        self.z[i] = z
+
+    def start(self, z: float = 0.0) -> None:
+        """Load every point's own launch field at z."""
+        for i, p in enumerate(self.problems):
+            self.load(i, z, p.field0)
+
+    def advance(self, targets, work_limit: int = 256) -> list:
+        n = len(self.problems)
+        targets = np.ascontiguousarray(np.broadcast_to(np.asarray(targets, float), (n,)))
+        rows = np.zeros((n, work_limit, 4))
+        results = (_lib.MarchResultC * n)()
+        status = (C.c_int * n)()
+        msg = C.create_string_buffer(1024)
+        code = self._lib.sp_scan_advance(self._handle, _ptr(targets), int(work_limit), _ptr(rows),
+                                         int(work_limit), C.cast(results, C.c_void_p), status,
+                                         msg, 1024)
+        out = []
+        for i in range(n):
+            r = results[i]
+            self.z[i] = r.z
+            out.append(Batch(r.z, bool(r.done), rows[i, : r.n_rows].copy(), r.evaluations))
+        self.status = [int(v) for v in status]
+        _check(code, msg)
+        return out
+
+    def state(self, i: int) -> np.ndarray:
+        p = self.problems[i]
+        out = np.zeros((p.n_modes, p.n_omega), dtype=np.complex128)
+        msg = C.create_string_buffer(1024)
+        _check(self._lib.sp_scan_state(self._handle, int(i), _ptr(out), msg, 1024), msg)
+        return out
+
+    def timing(self) -> dict:
+        t = _lib.ScanTimingC()
+        _check(self._lib.sp_scan_timing_of(self._handle, C.byref(t)), b"")
+        return {"rounds": t.rounds, "device_ms": t.device_ms, "host_ms": t.host_ms,
+                "device_bytes": t.device_bytes}
 
 
 class Probe:

@@ -68,3 +68,21 @@ def test_unbuilt_statistics_are_refused_by_name():
     ws.problem = capillary_problem(n_t=256, n_nodes=4, integration="trapezoid")
     with pytest.raises(BackendRefusal, match="populations"):
         W.statistics(ws, ["energy", "populations"])
+
+
+def test_scan_refuses_a_point_it_cannot_batch():
+    """A scan refuses an instance with a Raman response (or more than 6 modes, or an
+    absorber) before allocating anything, naming the instance."""
+    from solprop_amd.native import Scan
+
+    ok = capillary_problem(n_t=256, n_nodes=4, integration="trapezoid")
+    raman = capillary_problem(n_t=256, n_nodes=4, integration="trapezoid", raman_share=0.5)
+    with pytest.raises(BackendRefusal, match="scan instance 1: .*Raman"):
+        Scan([ok, raman])
+    wide = capillary_problem(n_t=256, n_nodes=4, n_modes=7, integration="trapezoid")
+    with pytest.raises(BackendRefusal, match="scan instance 0: .*1 to 6 modes"):
+        Scan([wide])
+    absorbing = capillary_problem(n_t=256, n_nodes=4, integration="trapezoid",
+                                  absorber="in_stage")
+    with pytest.raises(BackendRefusal, match="absorber"):
+        Scan([ok, absorbing])
