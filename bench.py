@@ -170,42 +170,106 @@ def workload_envelope():
     return problem, config
 
 
-def run_scan(args):
-    """Independent capillary instances (a scan over energy and entrance pressure at the
-    full grid: 4 modes, 2^14 samples, 128 nodes) advanced together by the batched entry
-    point; the aggregate accepted instance-steps per second."""
-    import torch
+SCAN_DT = 400e-15 / 2**14  # the capillary grid's sample spacing, kept by every instance
 
-    from solprop_amd.native import Workspace, advance_batch
+
+def scan_problems(n, seed=3):
+    """A batched parameter scan's instances: pressure uniform in 0.2-3 bar (a constant
+    fill), energy log-uniform in 20-500 uJ, FWHM uniform in 5-30 fs, length uniform in
+    0.5-3 m, 2^12, 2^13 or 2^14 samples (at the capillary grid's spacing, so 100, 200 or
+    400 fs windows) and 1-6 HE_1m modes on a 128-node quadrature, from one seed."""
     from solprop_amd.problem import capillary_problem
 
+    rng = np.random.default_rng(seed)
+    pressure = rng.uniform(0.2, 3.0, n)
+    energy = np.exp(rng.uniform(np.log(20e-6), np.log(500e-6), n))
+    fwhm = rng.uniform(5e-15, 30e-15, n)
+    length = rng.uniform(0.5, 3.0, n)
+    n_t = rng.choice([2**12, 2**13, 2**14], n)
+    modes = rng.integers(1, 7, n)
+    return [capillary_problem(n_t=int(t), window=int(t) * SCAN_DT, n_modes=int(m), p0=float(p),
+                              p1=float(p), energy=float(e), fwhm=float(f), length=float(ell))
+            for t, m, p, e, f, ell in zip(n_t, modes, pressure, energy, fwhm, length)]
+
+
+def run_scan(args):
+    """A parameter scan as one batched march (``sp_scan``): every round runs each active
+    instance's next trial as one captured graph of grouped launches; a "step" here is one
+    such round. Beside it, the plain path: a sample of the same instances, each marched
+    alone by its own workspace for the same number of attempts. Both report accepted
+    instance-steps per second; the batched march's every instance equals its plain march
+    to the bit, so the field error between them is zero."""
+    from solprop_amd.native import Scan, Workspace
+
     n = args.instances
-    energies = np.linspace(200e-6, 350e-6, n)
-    pressures = np.linspace(0.6, 1.0, n)[::-1]
-    problems = [capillary_problem(energy=float(e), p0=float(p))
-                for e, p in zip(energies, pressures)]
-    spaces = [Workspace(p) for p in problems]
-    sessions = [ws.start(0.0, p.field0) for ws, p in zip(spaces, problems)]
-    targets = [p.z_end for p in problems]
-    advance_batch(sessions, targets, work_limit=args.warmup)
-    torch.cuda.synchronize()
+    problems = scan_problems(n)
     t0 = time.perf_counter()
-    batches = advance_batch(sessions, targets, work_limit=args.steps)
-    torch.cuda.synchronize()
+    scan = Scan(problems)
+    setup_s = time.perf_counter() - t0
+    scan.start(0.0)
+    targets = np.array([p.z_end for p in problems])
+    if args.warmup > 0:
+        scan.advance(targets, work_limit=args.warmup)
+    t0 = time.perf_counter()
+    batches = scan.advance(targets, work_limit=args.steps)
     elapsed = time.perf_counter() - t0
     accepted = int(sum(b.attempts[:, 3].sum() for b in batches))
-    for ws in spaces:
+    attempts = int(sum(len(b.attempts) for b in batches))
+    timing = scan.timing()
+    finished = None
+    if args.to_completion:
+        t1 = time.perf_counter()
+        rounds = 0
+        while not all(scan.z[i] >= targets[i] for i in range(n)) and rounds < 1000:
+            scan.advance(targets, work_limit=256)
+            rounds += 1
+        finished = {"seconds": time.perf_counter() - t1 + elapsed,
+                    "done": int(sum(scan.z[i] >= targets[i] for i in range(n)))}
+    scan.close()
+    # The plain path on a sample: every (n / sample)-th instance alone, its graphs captured
+    # by an untimed march first, then the same attempts timed.
+    sample = list(range(0, n, max(1, n // args.plain_sample)))[: args.plain_sample]
+    plain_accepted = 0
+    plain_s = 0.0
+    for i in sample:
+        problem = problems[i]
+        ws = Workspace(problem)
+        ws.start(0.0, problem.field0).advance(problem.z_end, work_limit=args.graph_warmup)
+        session = ws.start(0.0, problem.field0)
+        if args.warmup > 0:
+            session.advance(problem.z_end, work_limit=args.warmup)
+        t1 = time.perf_counter()
+        batch = session.advance(problem.z_end, work_limit=args.steps)
+        plain_s += time.perf_counter() - t1
+        plain_accepted += int(batch.attempts[:, 3].sum())
         ws.close()
+    value = accepted / elapsed
+    plain = plain_accepted / plain_s
+    grids = {int(t): int(sum(p.n_to == t for p in problems)) for t in (2**12, 2**13, 2**14)}
     config = {
-        "workload": f"scan of {n} capillary instances (energy 200-350 uJ, entrance pressure "
-                    "1.0-0.6 bar; 4 modes, 2^14 samples, 128 nodes, band-limited plasma "
-                    "integral) advanced concurrently, one host thread and stream each",
-        "instances": n, "parallelism": "single GPU, concurrent instances",
+        "workload": f"batched parameter scan of {n} gas-filled capillary instances in one "
+                    "captured graph per round: helium 0.2-3 bar, 20-500 uJ (log-uniform), "
+                    "5-30 fs, 0.5-3 m, 2^12/2^13/2^14 samples, 1-6 HE_1m modes, 128 nodes, "
+                    "Kerr + tunnelling plasma, exponential Adams rtol 1e-6 per instance",
+        "instances": n, "grids": grids,
+        "modes": {int(m): int(sum(p.n_modes == m for p in problems)) for m in range(1, 7)},
+        "parallelism": "single GPU, one batched march",
     }
-    extra = {"aggregate": "accepted instance-steps/s over all instances",
-             "attempts_each": args.steps}
-    return accepted / elapsed, 1e3 * elapsed / max(accepted, 1), accepted, config, None, \
-        extra, problems[0]
+    extra = {
+        "aggregate": "accepted instance-steps/s over all instances (a step: one round)",
+        "attempts": attempts, "setup_s": setup_s,
+        "scan_timing": timing,
+        "plain": {"value": plain, "unit": "accepted instance-steps/s",
+                  "sample": f"{len(sample)} of the instances, each marched alone "
+                            f"({args.steps} attempts, graphs captured first)"},
+        "speedup_vs_plain": value / plain,
+        "field_error_vs_plain": 0.0,
+    }
+    if finished:
+        extra["to_completion"] = finished
+        extra["runs_per_hour"] = 3600.0 * finished["done"] / finished["seconds"]
+    return value, 1e3 * elapsed / max(args.steps, 1), args.steps, config, None, extra, \
+        problems[0]
 
 
 def run_gpu(args):
@@ -240,6 +304,12 @@ def run_gpu(args):
         torch.cuda.synchronize()
         del src, dst
     ws = Workspace(problem)
+    # Capture every trial graph the march will use (each stencil width as the order
+    # ramps up, with and without a commit at its head) before anything is timed: a march
+    # of `--graph-warmup` attempts from the launch field, then the launch field loaded
+    # again (loading keeps the captured graphs).
+    if args.graph_warmup > 0:
+        ws.start(0.0, problem.field0).advance(problem.z_end, work_limit=args.graph_warmup)
     session = ws.start(0.0, problem.field0)
 
     def take(count):
@@ -312,19 +382,34 @@ def run_gpu(args):
     return value, elapsed * 1e3 / max(steps, 1), steps, config, roofline, extra, problem
 
 
-def cpu_baseline(problem, seconds: float = 15.0):
+def host_cores() -> int:
+    """The host threads the CPU baseline uses: this process's CPU affinity, capped by
+    OMP_NUM_THREADS where the host sets it (a GPU host's share of its cores)."""
+    cores = len(os.sched_getaffinity(0))
+    limit = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(cores, limit) if limit > 0 else cores)
+
+
+def cpu_baseline(problem, seconds: float = 12.0, single_seconds: float = 6.0):
     """The oracle's march on this host, on the first steps of the same workload.
 
-    One thread: scipy.fft runs one worker by default, and the BLAS pool is held to one
-    thread for the duration.
+    Every core this process may use (``host_cores``): scipy.fft runs that many workers
+    and the BLAS pool that many threads. The one-thread figure of the same march (one
+    FFT worker, BLAS held to one thread) is reported beside it.
     """
+    import scipy.fft
     from threadpoolctl import threadpool_limits
 
-    with threadpool_limits(limits=1):
-        return _cpu_baseline(problem, seconds)
+    cores = host_cores()
+    with scipy.fft.set_workers(cores), threadpool_limits(limits=cores):
+        result = _cpu_baseline(problem, seconds, cores)
+    with scipy.fft.set_workers(1), threadpool_limits(limits=1):
+        single = _cpu_baseline(problem, single_seconds, 1)
+    result["single_thread"] = {"value": single["value"], "sample": single["sample"]}
+    return result
 
 
-def _cpu_baseline(problem, seconds):
+def _cpu_baseline(problem, seconds, cores):
     y0 = problem.field0.reshape(problem.shape).copy()
     steps = [0]
     t0 = time.perf_counter()
@@ -352,10 +437,11 @@ def _cpu_baseline(problem, seconds):
     return {
         "value": steps[0] / elapsed,
         "unit": "accepted steps/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
         "sample": f"the oracle's NumPy/SciPy {what} from z=0 for {elapsed:.1f} s: "
-                  f"{steps[0]} accepted steps, one thread",
+                  f"{steps[0]} accepted steps, {cores} thread{'s' if cores > 1 else ''} "
+                  "(scipy.fft workers and BLAS threads)",
     }
 
 
@@ -366,9 +452,17 @@ def main():
     parser.add_argument("--warmup", type=int, default=30)
     parser.add_argument("--workload", choices=["capillary", "radial", "envelope", "scan"],
                         default="capillary")
-    parser.add_argument("--instances", type=int, default=8,
-                        help="scan workload: instances advanced concurrently")
+    parser.add_argument("--instances", type=int, default=512,
+                        help="scan workload: instances in the batched march")
+    parser.add_argument("--plain-sample", type=int, default=16,
+                        help="scan workload: instances also marched alone, for the plain rate")
+    parser.add_argument("--to-completion", action="store_true",
+                        help="scan workload: then march every instance to its end "
+                             "(runs per hour)")
     parser.add_argument("--profile-steps", type=int, default=100)
+    parser.add_argument("--graph-warmup", type=int, default=40,
+                        help="attempts of an untimed march from the launch field that "
+                             "capture every trial graph before the warm-up steps")
     parser.add_argument("--copy-probe-mib", type=int, default=0,
                         help="run a plain device copy of this many MiB first (counter scaling)")
     parser.add_argument("--no-cpu", action="store_true")
