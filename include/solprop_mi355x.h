@@ -384,6 +384,21 @@ int sp_testing_gemm(long M, long N, long K, const double* A, const double* B, do
 int sp_testing_fft(const double* in, double* out, long n, long batch, int sign, char* msg,
                    size_t msglen);
 
+/* A batched complex FFT that allocates nothing, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It has no counterpart in the reference: it is sp_testing_fft on device
+ * buffers. `in` and `out` are device pointers to `batch` contiguous rows of `n` complex
+ * points (they may be equal: in place); `stream` is a hipStream_t (NULL: the default
+ * stream). The work is enqueued on `stream` and not waited for. The twiddle tables and
+ * the intermediate of transforms above 4096 points live in one static device array per
+ * GPU (about 17 MiB, held until the process ends); n <= 65536 and n * batch <= 2^20 are
+ * accepted, larger sizes are refused with SP_VALUE. Calls that overlap on one GPU race on
+ * that array. A call at a length other than the last call's on that GPU rebuilds the
+ * tables first (a copy on `stream`, then a wait for it). Results equal sp_testing_fft's
+ * to the bit. */
+int sp_fft_noalloc(const double* in, double* out, long n, long batch, int sign, void* stream,
+                   char* msg, size_t msglen);
+
 /* Library identification: the gfx target and build string. */
 const char* sp_version(void);
 

@@ -177,6 +177,8 @@ SIGNATURES = {
     "sp_testing_fft": (_I, [_PD, _PD, C.c_long, C.c_long, _I, C.c_char_p, C.c_size_t]),
     "sp_testing_gemm": (_I, [C.c_long, C.c_long, C.c_long, _PD, _PD, _PD, C.c_char_p,
                              C.c_size_t]),
+    "sp_fft_noalloc": (_I, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, _I, C.c_void_p,
+                            C.c_char_p, C.c_size_t]),
     "sp_version": (C.c_char_p, []),
 }
 

@@ -323,12 +323,13 @@ std::vector<cplx> unit_roots(std::size_t n) {
 }
 
 FftPlan::~FftPlan() {
+  if (!owns_tables_) return;
   if (roots_n_) (void)hipFree(roots_n_);
   for (auto* p : roots_l_)
     if (p) (void)hipFree(p);
 }
 
-void FftPlan::init(std::size_t n, std::size_t batch) {
+void FftPlan::layout(std::size_t n, std::size_t batch) {
   SP_CHECK(n >= 4 && (n & (n - 1)) == 0, "FFT length must be a power of two >= 4");
   SP_CHECK(n <= (std::size_t{1} << 24), "FFT length above 2^24 is not supported");
   SP_CHECK(batch >= 1, "FFT batch must be positive");
@@ -336,13 +337,6 @@ void FftPlan::init(std::size_t n, std::size_t batch) {
   batch_ = batch;
   const int m = ilog2(n);
   passes_.clear();
-  auto upload = [&](std::size_t len, cplx** dst) {
-    const auto host = unit_roots(len);
-    SP_HIP(hipMalloc(dst, len * sizeof(cplx)));
-    SP_HIP(hipMemcpy(*dst, host.data(), len * sizeof(cplx), hipMemcpyHostToDevice));
-    bytes_ += len * sizeof(cplx);
-  };
-  upload(n, &roots_n_);
   if (n <= static_cast<std::size_t>(kMaxElems)) {
     FftPass p;
     p.log2L = m;
@@ -354,7 +348,6 @@ void FftPlan::init(std::size_t n, std::size_t batch) {
     p.batch_in = p.batch_out = 0;
     p.n_seq = static_cast<int>(batch);
     passes_.push_back(p);
-    upload(n, &roots_l_[0]);
   } else {
     const int m1 = m / 2, m2 = m - m1;
     const long n1 = 1L << m1, n2 = 1L << m2;
@@ -380,11 +373,55 @@ void FftPlan::init(std::size_t n, std::size_t batch) {
     p2.n_seq = static_cast<int>(n1);
     passes_.push_back(p1);
     passes_.push_back(p2);
-    upload(static_cast<std::size_t>(n1), &roots_l_[0]);
-    upload(static_cast<std::size_t>(n2), &roots_l_[1]);
   }
   for (const auto& p : passes_)
     SP_CHECK(kernel_for(p.log2L, p.log2G) != nullptr, "no FFT pass kernel for this shape");
+}
+
+std::size_t FftPlan::table_points(std::size_t n) {
+  if (n <= static_cast<std::size_t>(kMaxElems)) return 2 * n;
+  const int m = ilog2(n);
+  const int m1 = m / 2, m2 = m - m1;
+  return n + (std::size_t{1} << m1) + (std::size_t{1} << m2);
+}
+
+void FftPlan::init(std::size_t n, std::size_t batch) {
+  layout(n, batch);
+  owns_tables_ = true;
+  auto upload = [&](std::size_t len, cplx** dst) {
+    const auto host = unit_roots(len);
+    SP_HIP(hipMalloc(dst, len * sizeof(cplx)));
+    SP_HIP(hipMemcpy(*dst, host.data(), len * sizeof(cplx), hipMemcpyHostToDevice));
+    bytes_ += len * sizeof(cplx);
+  };
+  upload(n, &roots_n_);
+  upload(static_cast<std::size_t>(1) << passes_[0].log2L, &roots_l_[0]);
+  if (passes_.size() == 2)
+    upload(static_cast<std::size_t>(1) << passes_[1].log2L, &roots_l_[1]);
+}
+
+void FftPlan::init_external(std::size_t n, std::size_t batch, cplx* tables) {
+  SP_CHECK(tables != nullptr, "external FFT tables need device memory");
+  layout(n, batch);
+  owns_tables_ = false;
+  roots_n_ = tables;
+  roots_l_[0] = tables + n;
+  roots_l_[1] = passes_.size() == 2 ? roots_l_[0] + (std::size_t{1} << passes_[0].log2L)
+                                    : nullptr;
+}
+
+void FftPlan::upload_tables(hipStream_t stream) const {
+  std::vector<cplx> host = unit_roots(n_);
+  const auto local = [&](int pass) {
+    const auto roots = unit_roots(std::size_t{1} << passes_[pass].log2L);
+    host.insert(host.end(), roots.begin(), roots.end());
+  };
+  local(0);
+  if (passes_.size() == 2) local(1);
+  SP_HIP(hipMemcpyAsync(roots_n_, host.data(), host.size() * sizeof(cplx),
+                        hipMemcpyHostToDevice, stream));
+  // The host rows must outlive the copy.
+  SP_HIP(hipStreamSynchronize(stream));
 }
 
 void FftPlan::launch(const FftPass& p, const cplx* in, cplx* out, int sign,

@@ -79,6 +79,12 @@ class FftPlan {
 
   // A complex transform of n points over `batch` contiguous rows of n.
   void init(std::size_t n, std::size_t batch);
+  // The same plan with its twiddle tables in device memory the caller owns, at least
+  // table_points(n) points at `tables`: nothing is allocated or freed. The tables hold
+  // nothing until upload_tables has run (on `stream`, which it then synchronises).
+  void init_external(std::size_t n, std::size_t batch, cplx* tables);
+  void upload_tables(hipStream_t stream) const;
+  static std::size_t table_points(std::size_t n);
   // In place on data (rows of n); `scratch` holds n*batch points for the four-step.
   void exec(cplx* data, cplx* scratch, int sign, hipStream_t stream) const;
   // The same transform with its input packed from a half spectrum on the way in (the
@@ -101,8 +107,10 @@ class FftPlan {
   std::size_t bytes() const { return bytes_; }
 
  private:
+  void layout(std::size_t n, std::size_t batch);
   void launch(const FftPass& p, const cplx* in, cplx* out, int sign, hipStream_t s) const;
   std::size_t n_ = 0, batch_ = 0, bytes_ = 0;
+  bool owns_tables_ = true;
   std::vector<FftPass> passes_;
   mutable const PackedSpectrum* spec_ = nullptr;  // set for the first pass of exec_packed
   cplx* roots_n_ = nullptr;   // length n
