@@ -234,8 +234,8 @@ def run_scan(args):
     for i in sample:
         problem = problems[i]
         ws = Workspace(problem)
-        ws.start(0.0, problem.field0).advance(problem.z_end, work_limit=args.graph_warmup)
-        session = ws.start(0.0, problem.field0)
+        ws.march(0.0, problem.field0).advance(problem.z_end, work_limit=args.graph_warmup)
+        session = ws.march(0.0, problem.field0)
         if args.warmup > 0:
             session.advance(problem.z_end, work_limit=args.warmup)
         t1 = time.perf_counter()
@@ -309,8 +309,8 @@ def run_gpu(args):
     # of `--graph-warmup` attempts from the launch field, then the launch field loaded
     # again (loading keeps the captured graphs).
     if args.graph_warmup > 0:
-        ws.start(0.0, problem.field0).advance(problem.z_end, work_limit=args.graph_warmup)
-    session = ws.start(0.0, problem.field0)
+        ws.march(0.0, problem.field0).advance(problem.z_end, work_limit=args.graph_warmup)
+    session = ws.march(0.0, problem.field0)
 
     def take(count):
         accepted = attempts = 0

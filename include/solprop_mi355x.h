@@ -141,7 +141,8 @@ typedef struct sp_workspace sp_workspace;
  * validates, allocates every buffer and plans every transform once. */
 int sp_workspace_create(const sp_problem_desc* desc, sp_workspace** out, char* msg,
                         size_t msglen);
-/* `Workspace.close()`; NULL is accepted. */
+/* `Workspace.close()`; NULL is accepted. A workspace an open session leases is released
+ * when that session closes. */
 void sp_workspace_destroy(sp_workspace* ws);
 
 /* Load a committed state at z (the reference's upload at start, trial.cpp:356-390).
@@ -231,6 +232,110 @@ typedef struct sp_scan_timing {
   long device_bytes;
 } sp_scan_timing;
 int sp_scan_timing_of(sp_scan* scan, sp_scan_timing* out);
+
+/* ------------------------------------------------------------------ sessions
+ *
+ * `Workspace.start(run) -> Session` and the session's event protocol (_core.pyi:468-558;
+ * native/include/solprop_native/session/session.hpp; native/src/session/session.cpp:695;
+ * docs/dev/kernel-contract.md "What a native call takes / returns"). A session leases its
+ * workspace: while it is open the workspace's own march, load and trial entries refuse
+ * with SP_STATE. Each advance runs bounded march calls until the caller has work and
+ * returns one batch: a sequence number, one event and the chunk of attempt rows made
+ * since the previous batch, 6 doubles a row (end, step, norm -- infinity when not
+ * finite --, accepted, order, wall seconds since the previous event). A batch is pending
+ * until acknowledged; advancing across it is SP_STATE. An observation or terminal batch
+ * holds a ticket: (sequence, generation) exports the committed state it names while the
+ * batch is pending. */
+typedef struct sp_session sp_session;
+
+/* The run's plan (the reference's session::Plan): the save planes, increasing, the last
+ * one the run's end; whether the launch state is observed first; every `step_planes`-th
+ * accepted step observed (0: none); the plane statistics (sp_workspace_statistics's
+ * flags and windows) taken at every observation (0: none); the chunk buffer's attempts
+ * (a full buffer yields a progress event) and one march call's bound, 256 attempts and
+ * 0.1 s in the reference (0 or less: those defaults). */
+typedef struct sp_session_plan {
+  const double* planes;
+  int n_planes;
+  int launch;
+  long step_planes;
+  unsigned long long statistics;
+  const double* windows;
+  int n_windows;
+  long attempt_capacity;
+  long call_attempts;
+  double call_seconds;
+} sp_session_plan;
+
+/* SessionEvent (_core.pyi:517-558): kind 0 observation, 1 progress, 2 snapshot,
+ * 3 terminal, 4 warning; observation 0 launch, 1 plane, 2 step, 3 natural; outcome
+ * 0 none, 1 completed, 2 stopped, 3 failed; for a failure the march's reason (0 reached,
+ * 1 host step, 3 bound, 5 floor, 6 native error, 11 stopped), its stage (1 startup,
+ * 2 trial, 3 accept), category (1 convergence, 2 backend, 3 value), message and, at the
+ * floor, the proposal below it. */
+typedef struct sp_session_event {
+  int kind;
+  int observation;
+  double z;
+  long plane_index;
+  int has_stats;
+  int ticket;
+  unsigned long long generation;
+  int outcome;
+  int reason;
+  int category;
+  int stage;
+  double floor_proposal;
+  char message[256];
+} sp_session_event;
+
+/* SessionBatch (_core.pyi:501-515): its sequence, event, the number of attempt rows
+ * written to the caller's buffer and of plane statistics written to its stats buffer. */
+typedef struct sp_session_batch {
+  unsigned long long sequence;
+  sp_session_event event;
+  long n_attempts;
+  int n_stats;
+} sp_session_batch;
+
+/* SessionStatus (_core.pyi:560-587): state 0 ready, 1 pending, 2 finished, 3 closed. */
+typedef struct sp_session_status {
+  double z;
+  long plane_index;
+  unsigned long long attempts, rejected;
+  int state, outcome;
+  unsigned long long generation, march_calls, batches, exports, export_bytes;
+} sp_session_status;
+
+/* `Workspace.start(run)`: load `field` (complex (C, N)) at z and open a session. */
+int sp_session_start(sp_workspace* ws, double z, const double* field,
+                     const sp_session_plan* plan, sp_session** out, char* msg, size_t msglen);
+/* `Session.advance(work_limit, deadline) -> SessionBatch`: at most `work_limit` attempts
+ * (at least one) and `deadline` seconds, both read between attempts. `attempts` holds
+ * `capacity` rows of 6 (at least the plan's attempt_capacity), `stats` `stats_capacity`
+ * doubles. */
+int sp_session_advance(sp_session* s, long work_limit, double deadline, sp_session_batch* out,
+                       double* attempts, long capacity, double* stats, int stats_capacity,
+                       char* msg, size_t msglen);
+/* `Session.acknowledge(sequence)`: release the pending batch. */
+int sp_session_acknowledge(sp_session* s, unsigned long long sequence, char* msg,
+                           size_t msglen);
+/* `Session.request_stop()`: a stop at the next attempt boundary; any thread, no lock. */
+void sp_session_request_stop(sp_session* s);
+/* `Session.status()`: any thread, no lock. */
+int sp_session_status_of(sp_session* s, sp_session_status* out);
+/* `Session.export(ticket, recipe, out)` (native/src/solprop_native/session.py:960): the
+ * pending batch's state by the recipe of sp_workspace_export. */
+int sp_session_export(sp_session* s, unsigned long long sequence, unsigned long long generation,
+                      const double* power_weight, int optics, double* out, char* msg,
+                      size_t msglen);
+/* `Session.fail_outside()`: a failed terminal batch at the committed state (with the
+ * attempts the chunk still holds) when none is pending; SP_STATE when a batch is pending
+ * or the session has ended. */
+int sp_session_fail_outside(sp_session* s, const char* reason, sp_session_batch* out,
+                            double* attempts, long capacity, char* msg, size_t msglen);
+/* `Session.close()`: ends the lease; NULL is accepted. */
+void sp_session_close(sp_session* s);
 
 /* `Session.export(ticket, recipe, out)` (native/src/solprop_native/session.py:960;
  * src/solprop/backends/export.py:52-207): the committed state converted by the recipe,

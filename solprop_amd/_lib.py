@@ -138,6 +138,40 @@ class ScanTimingC(C.Structure):
     ]
 
 
+class SessionPlanC(C.Structure):
+    _fields_ = [
+        ("planes", _PD), ("n_planes", _I), ("launch", _I), ("step_planes", C.c_long),
+        ("statistics", C.c_ulonglong), ("windows", _PD), ("n_windows", _I),
+        ("attempt_capacity", C.c_long), ("call_attempts", C.c_long), ("call_seconds", _D),
+    ]
+
+
+class SessionEventC(C.Structure):
+    _fields_ = [
+        ("kind", _I), ("observation", _I), ("z", _D), ("plane_index", C.c_long),
+        ("has_stats", _I), ("ticket", _I), ("generation", C.c_ulonglong), ("outcome", _I),
+        ("reason", _I), ("category", _I), ("stage", _I), ("floor_proposal", _D),
+        ("message", C.c_char * 256),
+    ]
+
+
+class SessionBatchC(C.Structure):
+    _fields_ = [
+        ("sequence", C.c_ulonglong), ("event", SessionEventC), ("n_attempts", C.c_long),
+        ("n_stats", _I),
+    ]
+
+
+class SessionStatusC(C.Structure):
+    _fields_ = [
+        ("z", _D), ("plane_index", C.c_long), ("attempts", C.c_ulonglong),
+        ("rejected", C.c_ulonglong), ("state", _I), ("outcome", _I),
+        ("generation", C.c_ulonglong), ("march_calls", C.c_ulonglong),
+        ("batches", C.c_ulonglong), ("exports", C.c_ulonglong),
+        ("export_bytes", C.c_ulonglong),
+    ]
+
+
 # The exported entry points and their signatures (include/solprop_mi355x.h).
 SIGNATURES = {
     "sp_workspace_create": (_I, [C.POINTER(ProblemDescC), C.POINTER(C.c_void_p), C.c_char_p,
@@ -179,6 +213,18 @@ SIGNATURES = {
                              C.c_size_t]),
     "sp_fft_noalloc": (_I, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, _I, C.c_void_p,
                             C.c_char_p, C.c_size_t]),
+    "sp_session_start": (_I, [C.c_void_p, _D, _PD, C.POINTER(SessionPlanC),
+                              C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]),
+    "sp_session_advance": (_I, [C.c_void_p, C.c_long, _D, C.POINTER(SessionBatchC), _PD,
+                                C.c_long, _PD, _I, C.c_char_p, C.c_size_t]),
+    "sp_session_acknowledge": (_I, [C.c_void_p, C.c_ulonglong, C.c_char_p, C.c_size_t]),
+    "sp_session_request_stop": (None, [C.c_void_p]),
+    "sp_session_status_of": (_I, [C.c_void_p, C.POINTER(SessionStatusC)]),
+    "sp_session_export": (_I, [C.c_void_p, C.c_ulonglong, C.c_ulonglong, _PD, _I, _PD,
+                               C.c_char_p, C.c_size_t]),
+    "sp_session_fail_outside": (_I, [C.c_void_p, C.c_char_p, C.POINTER(SessionBatchC), _PD,
+                                     C.c_long, C.c_char_p, C.c_size_t]),
+    "sp_session_close": (None, [C.c_void_p]),
     "sp_version": (C.c_char_p, []),
 }
 

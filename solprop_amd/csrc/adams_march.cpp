@@ -201,6 +201,7 @@ void March::started() {
 bool March::conclude(const Request& r, const std::vector<double>& norms) {
   double norm = 0.0;
   const bool accepted = decide(r.stencil, norms, r.dz, false, r.end, r.kink, &norm);
+  host_.attempt_order(r.stencil.order);
   host_.record(r.end, r.dz, norm, accepted);
   if (accepted) {
     host_.stepped();
@@ -298,6 +299,7 @@ bool March::replay_to(const std::vector<double>& steps, const std::vector<double
       throw Diverged(buf);
     }
     cursor += 1;
+    host_.attempt_order(order);
     host_.record(end, dz, norm, true);
     host_.stepped();
     if (host_.natural_stop(target)) return true;

@@ -75,6 +75,8 @@ struct Host {
   virtual void record(double end, double dz, double norm, bool accepted) {
     (void)end; (void)dz; (void)norm; (void)accepted;
   }
+  // The predictor order of the attempt about to be recorded.
+  virtual void attempt_order(int order) { (void)order; }
   virtual void stepped() {}
   virtual bool natural_stop(double target) { (void)target; return false; }
   virtual bool halt() { return false; }

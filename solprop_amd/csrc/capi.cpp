@@ -1,6 +1,7 @@
 // The extern "C" boundary (include/solprop_mi355x.h): status codes and messages in
 // place of exceptions, host arrays in and out.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -11,6 +12,7 @@
 
 #include "field_ws.hpp"
 #include "scan.hpp"
+#include "session.hpp"
 #include "gemm.hpp"
 #include "trace.hpp"
 #include "workspace.hpp"
@@ -27,7 +29,19 @@ std::unique_ptr<sp::Propagator> make_propagator(const sp_problem_desc& d) {
 struct sp_workspace {
   std::unique_ptr<sp::Propagator> owner;
   sp::Propagator& impl;
-  explicit sp_workspace(const sp_problem_desc& d) : owner(make_propagator(d)), impl(*owner) {}
+  int n_modes, n_omega;
+  // An open session leases the workspace: its own march, load and trial refuse meanwhile.
+  // Destroying a leased workspace defers its release until the session closes.
+  std::mutex lease_mu;
+  std::atomic<bool> leased{false};
+  bool doomed = false;
+  explicit sp_workspace(const sp_problem_desc& d)
+      : owner(make_propagator(d)), impl(*owner), n_modes(d.n_modes), n_omega(d.n_omega) {}
+};
+
+struct sp_session {
+  sp_workspace* ws;
+  std::unique_ptr<sp::session::Session> impl;
 };
 
 namespace {
@@ -51,6 +65,9 @@ int guarded(char* msg, std::size_t len, F&& body) {
   } catch (const sp::march::ConvergenceError& e) {
     write_message(msg, len, e.what());
     return SP_CONVERGENCE;
+  } catch (const sp::session::LifecycleError& e) {
+    write_message(msg, len, e.what());
+    return SP_STATE;
   } catch (const std::invalid_argument& e) {
     write_message(msg, len, e.what());
     return SP_VALUE;
@@ -61,6 +78,11 @@ int guarded(char* msg, std::size_t len, F&& body) {
     write_message(msg, len, e.what());
     return SP_BACKEND;
   }
+}
+
+void refuse_leased(const sp_workspace* ws) {
+  if (ws->leased.load())
+    throw sp::Error(sp::kState, "the workspace is leased by an open session");
 }
 
 // A testing entry's stream and device buffers, released on every exit path.
@@ -98,18 +120,34 @@ int sp_workspace_create(const sp_problem_desc* desc, sp_workspace** out, char* m
   return guarded(msg, msglen, [&] { *out = new sp_workspace(*desc); });
 }
 
-void sp_workspace_destroy(sp_workspace* ws) { delete ws; }
+void sp_workspace_destroy(sp_workspace* ws) {
+  if (!ws) return;
+  {
+    std::lock_guard<std::mutex> lock(ws->lease_mu);
+    if (ws->leased.load()) {
+      ws->doomed = true;  // released when its session closes
+      return;
+    }
+  }
+  delete ws;
+}
 
 int sp_workspace_load(sp_workspace* ws, double z, const double* field, char* msg,
                       size_t msglen) {
   if (!ws) return SP_VALUE;
-  return guarded(msg, msglen, [&] { ws->impl.load(z, field); });
+  return guarded(msg, msglen, [&] {
+    refuse_leased(ws);
+    ws->impl.load(z, field);
+  });
 }
 
 int sp_workspace_trial(sp_workspace* ws, double z, double dz, double* norm, char* msg,
                        size_t msglen) {
   if (!ws || !norm) return SP_VALUE;
-  return guarded(msg, msglen, [&] { *norm = ws->impl.probe_trial(z, dz); });
+  return guarded(msg, msglen, [&] {
+    refuse_leased(ws);
+    *norm = ws->impl.probe_trial(z, dz);
+  });
 }
 
 int sp_workspace_trial_field(sp_workspace* ws, double* out, char* msg, size_t msglen) {
@@ -128,6 +166,7 @@ int sp_workspace_advance(sp_workspace* ws, double target, long work_limit, doubl
   std::memset(out, 0, sizeof(*out));
   std::vector<double> held;
   const int status = guarded(msg, msglen, [&] {
+    refuse_leased(ws);
     ws->impl.advance(target, work_limit > 0 ? work_limit : 1, held, out);
   });
   const long n = static_cast<long>(held.size() / 4);
@@ -378,6 +417,149 @@ int sp_testing_gemm(long M, long N, long K, const double* A, const double* B, do
     SP_HIP(hipStreamSynchronize(scope.stream));
     SP_HIP(hipMemcpy(C, c, sizeof(double) * M * N, hipMemcpyDeviceToHost));
   });
+}
+
+// ------------------------------------------------------------------ sessions
+
+namespace {
+void fill_batch(const sp::session::Batch& b, sp_session_batch* out, double* attempts,
+                long capacity, double* stats, int stats_capacity) {
+  std::memset(out, 0, sizeof(*out));
+  out->sequence = b.sequence;
+  const sp::session::Event& e = b.event;
+  out->event.kind = static_cast<int>(e.kind);
+  out->event.observation = static_cast<int>(e.observation);
+  out->event.z = e.z;
+  out->event.plane_index = e.plane_index;
+  out->event.has_stats = e.has_stats ? 1 : 0;
+  out->event.ticket = e.ticket ? 1 : 0;
+  out->event.generation = e.generation;
+  out->event.outcome = static_cast<int>(e.outcome);
+  out->event.reason = static_cast<int>(e.reason);
+  out->event.category = static_cast<int>(e.category);
+  out->event.stage = static_cast<int>(e.stage);
+  out->event.floor_proposal = e.floor_proposal;
+  std::snprintf(out->event.message, sizeof(out->event.message), "%s", e.message.c_str());
+  const long rows = static_cast<long>(b.attempts.size() / sp::kWideColumns);
+  SP_CHECK(rows <= capacity, "the attempts buffer is smaller than the batch's chunk");
+  if (rows > 0) std::memcpy(attempts, b.attempts.data(), sizeof(double) * b.attempts.size());
+  out->n_attempts = rows;
+  const int n_stats = static_cast<int>(b.stats.size());
+  SP_CHECK(n_stats <= stats_capacity, "the statistics buffer is smaller than the batch's");
+  if (n_stats > 0) std::memcpy(stats, b.stats.data(), sizeof(double) * b.stats.size());
+  out->n_stats = n_stats;
+}
+}  // namespace
+
+int sp_session_start(sp_workspace* ws, double z, const double* field,
+                     const sp_session_plan* plan, sp_session** out, char* msg, size_t msglen) {
+  if (!ws || !plan || !out) {
+    write_message(msg, msglen, "start needs a workspace, a plan and an output handle");
+    return SP_VALUE;
+  }
+  *out = nullptr;
+  return guarded(msg, msglen, [&] {
+    SP_CHECK(plan->n_planes >= 1 && plan->planes, "a session needs its planes");
+    SP_CHECK(plan->n_windows >= 0 && (plan->n_windows == 0 || plan->windows),
+             "windows are (lo, hi) pairs");
+    {
+      std::lock_guard<std::mutex> lock(ws->lease_mu);
+      if (ws->doomed) throw sp::Error(sp::kState, "the workspace is closed");
+      bool expected = false;
+      if (!ws->leased.compare_exchange_strong(expected, true))
+        throw sp::Error(sp::kState, "the workspace is leased by an open session");
+    }
+    try {
+      sp::session::Plan p;
+      p.planes.assign(plan->planes, plan->planes + plan->n_planes);
+      p.launch = plan->launch != 0;
+      p.step_planes = plan->step_planes;
+      p.statistics = plan->statistics;
+      if (plan->n_windows)
+        p.windows.assign(plan->windows, plan->windows + 2 * plan->n_windows);
+      if (plan->attempt_capacity > 0) p.attempt_capacity = plan->attempt_capacity;
+      if (plan->call_attempts > 0) p.call_attempts = plan->call_attempts;
+      if (plan->call_seconds > 0.0) p.call_seconds = plan->call_seconds;
+      auto session = std::make_unique<sp::session::Session>(ws->impl, ws->n_modes,
+                                                            ws->n_omega, p, z, field);
+      *out = new sp_session{ws, std::move(session)};
+    } catch (...) {
+      ws->leased.store(false);
+      throw;
+    }
+  });
+}
+
+int sp_session_advance(sp_session* s, long work_limit, double deadline, sp_session_batch* out,
+                       double* attempts, long capacity, double* stats, int stats_capacity,
+                       char* msg, size_t msglen) {
+  if (!s || !out || (capacity > 0 && !attempts) || (stats_capacity > 0 && !stats))
+    return SP_VALUE;
+  return guarded(msg, msglen, [&] {
+    const sp::session::Batch b = s->impl->advance(work_limit, deadline);
+    fill_batch(b, out, attempts, capacity, stats, stats_capacity);
+  });
+}
+
+int sp_session_acknowledge(sp_session* s, unsigned long long sequence, char* msg,
+                           size_t msglen) {
+  if (!s) return SP_VALUE;
+  return guarded(msg, msglen, [&] { s->impl->acknowledge(sequence); });
+}
+
+void sp_session_request_stop(sp_session* s) {
+  if (s) s->impl->request_stop();
+}
+
+int sp_session_status_of(sp_session* s, sp_session_status* out) {
+  if (!s || !out) return SP_VALUE;
+  const sp::session::Status st = s->impl->status();
+  out->z = st.z;
+  out->plane_index = st.plane_index;
+  out->attempts = st.attempts;
+  out->rejected = st.rejected;
+  out->state = static_cast<int>(st.state);
+  out->outcome = static_cast<int>(st.outcome);
+  out->generation = st.generation;
+  out->march_calls = st.march_calls;
+  out->batches = st.batches;
+  out->exports = st.exports;
+  out->export_bytes = st.export_bytes;
+  return SP_OK;
+}
+
+int sp_session_export(sp_session* s, unsigned long long sequence, unsigned long long generation,
+                      const double* power_weight, int optics, double* out, char* msg,
+                      size_t msglen) {
+  if (!s || !out) return SP_VALUE;
+  return guarded(msg, msglen, [&] {
+    s->impl->export_field(sequence, generation, power_weight, optics != 0, out);
+  });
+}
+
+int sp_session_fail_outside(sp_session* s, const char* reason, sp_session_batch* out,
+                            double* attempts, long capacity, char* msg, size_t msglen) {
+  if (!s || !out || (capacity > 0 && !attempts)) return SP_VALUE;
+  return guarded(msg, msglen, [&] {
+    sp::session::Batch b;
+    if (!s->impl->fail_outside(reason ? reason : "failed outside the session", &b))
+      throw sp::session::LifecycleError("a batch is pending or the session has ended");
+    fill_batch(b, out, attempts, capacity, nullptr, 0);
+  });
+}
+
+void sp_session_close(sp_session* s) {
+  if (!s) return;
+  s->impl->close();
+  sp_workspace* ws = s->ws;
+  delete s;
+  bool release = false;
+  {
+    std::lock_guard<std::mutex> lock(ws->lease_mu);
+    ws->leased.store(false);
+    release = ws->doomed;
+  }
+  if (release) delete ws;
 }
 
 const char* sp_version(void) { return "solprop_mi355x 0.1 gfx950 fp64"; }

@@ -672,40 +672,32 @@ void FieldWorkspace::export_state(const double* weight, bool optics, double* out
   }
 }
 
-void FieldWorkspace::CallHost::record(double end, double dz, double norm, bool acc) {
-  if (rows) {
-    rows->push_back(end);
-    rows->push_back(dz);
-    rows->push_back(norm);
-    rows->push_back(acc ? 1.0 : 0.0);
-  }
-  attempts += 1;
-  accepted += acc ? 1 : 0;
-}
-
-bool FieldWorkspace::advance(double target, long work_limit, std::vector<double>& rows,
-                             sp_march_result* r) {
+bool FieldWorkspace::advance_bounded(const Bounds& b, int columns, std::vector<double>& rows,
+                                     sp_march_result* r) {
   if (!loaded_) throw Error(kState, "load a state before advancing");
-  host_.rows = &rows;
-  host_.limit = work_limit;
-  host_.attempts = 0;
-  host_.accepted = 0;
+  host_.log.begin(&rows, columns, b);
   bool done = false;
   try {
-    done = march_->march_to(target) && march::reached(march_->z, target);
+    done = march_->march_to(b.target) && march::reached(march_->z, b.target);
   } catch (...) {
-    host_.rows = nullptr;
+    host_.log.rows = nullptr;
+    z_ = march_->z;
+    r->z = march_->z;
+    r->proposal = march_->proposal;
+    r->attempts = host_.log.attempts;
+    r->accepted = host_.log.accepted;
+    r->evaluations = evaluations_;
     flush();
     throw;
   }
   flush();
-  host_.rows = nullptr;
+  host_.log.rows = nullptr;
   z_ = march_->z;
   r->z = march_->z;
   r->proposal = march_->proposal;
   r->done = done ? 1 : 0;
-  r->attempts = host_.attempts;
-  r->accepted = host_.accepted;
+  r->attempts = host_.log.attempts;
+  r->accepted = host_.log.accepted;
   r->evaluations = evaluations_;
   return done;
 }
@@ -875,10 +867,10 @@ std::vector<FftJob> FieldWorkspace::batch_forward_jobs(const int* gate) const {
 
 void FieldWorkspace::batch_begin(double target, long work_limit, std::vector<double>* rows) {
   if (!loaded_) throw Error(kState, "load a state before advancing");
-  host_.rows = rows;
-  host_.limit = work_limit;
-  host_.attempts = 0;
-  host_.accepted = 0;
+  Bounds b;
+  b.target = target;
+  b.attempts = work_limit;
+  host_.log.begin(rows, 4, b);
   batch_target_ = target;
   batch_finished_ = false;
   batch_done_ = false;
@@ -955,13 +947,13 @@ void FieldWorkspace::batch_conclude(int action) {
 int FieldWorkspace::batch_end(sp_march_result* out, std::string* message) {
   // An accepted step the last round concluded is committed here, on the batch's stream.
   flush();
-  host_.rows = nullptr;
+  host_.log.rows = nullptr;
   z_ = march_->z;
   out->z = march_->z;
   out->proposal = march_->proposal;
   out->done = batch_done_ ? 1 : 0;
-  out->attempts = host_.attempts;
-  out->accepted = host_.accepted;
+  out->attempts = host_.log.attempts;
+  out->accepted = host_.log.accepted;
   out->evaluations = evaluations_;
   if (message) *message = batch_message_;
   return batch_status_;

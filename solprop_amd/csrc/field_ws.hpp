@@ -33,8 +33,8 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   void read_trial(double* out) override;
   void read_state(double* out) override;
   void export_state(const double* weight, bool optics, double* out) override;
-  bool advance(double target, long work_limit, std::vector<double>& rows,
-               sp_march_result* r) override;
+  bool advance_bounded(const Bounds& b, int columns, std::vector<double>& rows,
+                       sp_march_result* r) override;
   sp_counters counters() const override;
   void source(double z, const double* lab, double* out, double* streams,
               double* modal) override;
@@ -145,10 +145,12 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   double pending_dz_ = 0.0;     // the accepted step a pending commit finishes
   // The march, and the host it reports to: rows and the attempt limit of one call.
   struct CallHost final : adams::Host {
-    std::vector<double>* rows = nullptr;
-    long limit = -1, attempts = 0, accepted = 0;
-    void record(double end, double dz, double norm, bool acc) override;
-    bool halt() override { return limit >= 0 && attempts >= limit; }
+    AttemptLog log;
+    void record(double end, double dz, double norm, bool acc) override {
+      log.record(end, dz, norm, acc);
+    }
+    void attempt_order(int order) override { log.order = order; }
+    bool halt() override { return log.halt(); }
   };
   adams::Policy policy_;
   CallHost host_;

@@ -3,6 +3,9 @@
 // field route under the exponential Adams march (field_ws.hpp).
 #pragma once
 
+#include <atomic>
+#include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <utility>
@@ -14,6 +17,68 @@ extern "C" {
 
 namespace sp {
 
+// The bounds of one march call (the reference's per-call bounds, docs/dev/kernel-contract.md
+// "What a native call takes"): the target, the most attempts, the accepted steps after
+// which the caller has work, the call's wall time and a stop flag, each checked between
+// attempts.
+struct Bounds {
+  double target = 0.0;
+  long attempts = 1;
+  long accepted = 0;      // 0: no such bound
+  double seconds = -1.0;  // < 0: none
+  const std::atomic<bool>* stop = nullptr;
+};
+
+// Attempt rows: 4 columns (end, step, norm, accepted) or kWideColumns with the
+// attempt's order and its wall time since the call began, s (the norm then written
+// as infinity when it is not finite).
+constexpr int kWideColumns = 6;
+
+// What a march's host records of a bounded call, and when it halts it.
+struct AttemptLog {
+  std::vector<double>* rows = nullptr;
+  int columns = 4;
+  long limit = -1, accept_limit = 0, attempts = 0, accepted = 0;
+  double seconds = -1.0;
+  const std::atomic<bool>* stop = nullptr;
+  std::chrono::steady_clock::time_point began;
+  int order = 0;
+
+  void begin(std::vector<double>* r, int cols, const Bounds& b) {
+    rows = r;
+    columns = cols;
+    limit = b.attempts;
+    accept_limit = b.accepted;
+    seconds = b.seconds;
+    stop = b.stop;
+    attempts = accepted = 0;
+    began = std::chrono::steady_clock::now();
+  }
+  double elapsed() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - began).count();
+  }
+  void record(double end, double dz, double norm, bool acc) {
+    if (rows) {
+      rows->push_back(end);
+      rows->push_back(dz);
+      rows->push_back(columns == kWideColumns && !std::isfinite(norm) ? INFINITY : norm);
+      rows->push_back(acc ? 1.0 : 0.0);
+      if (columns == kWideColumns) {
+        rows->push_back(static_cast<double>(order));
+        rows->push_back(elapsed());
+      }
+    }
+    attempts += 1;
+    accepted += acc ? 1 : 0;
+  }
+  bool halt() const {
+    if (limit >= 0 && attempts >= limit) return true;
+    if (accept_limit > 0 && accepted >= accept_limit) return true;
+    if (attempts > 0 && seconds >= 0.0 && elapsed() >= seconds) return true;
+    return stop && stop->load(std::memory_order_acquire);
+  }
+};
+
 class Propagator {
  public:
   virtual ~Propagator() = default;
@@ -23,8 +88,19 @@ class Propagator {
   virtual void read_trial(double* out) = 0;
   virtual void read_state(double* out) = 0;
   virtual void export_state(const double* weight, bool optics, double* out) = 0;
-  virtual bool advance(double target, long work_limit, std::vector<double>& rows,
-                       sp_march_result* r) = 0;
+  // A bounded march call: rows of `columns` (4 or kWideColumns) doubles per attempt;
+  // true when the target was reached.
+  virtual bool advance_bounded(const Bounds& b, int columns, std::vector<double>& rows,
+                               sp_march_result* r) = 0;
+  bool advance(double target, long work_limit, std::vector<double>& rows,
+               sp_march_result* r) {
+    Bounds b;
+    b.target = target;
+    b.attempts = work_limit;
+    return advance_bounded(b, 4, rows, r);
+  }
+  // The order of the method's attempts when it is fixed (a Runge-Kutta pair), else 0.
+  virtual int fixed_order() const { return 0; }
   virtual sp_counters counters() const = 0;
   virtual void set_profile(bool on) = 0;
   virtual double trial_us() const = 0;

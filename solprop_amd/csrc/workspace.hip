@@ -414,33 +414,41 @@ struct WorkspaceEngine final : march::Engine {
 };
 
 struct SessionHost final : march::Host {
-  std::vector<double>& rows;
-  long limit, attempts = 0, accepted = 0;
-  SessionHost(std::vector<double>& r, long l) : rows(r), limit(l) {}
+  AttemptLog& log;
+  explicit SessionHost(AttemptLog& l) : log(l) {}
   void record(double end, double dz, double norm, bool acc) override {
-    rows.push_back(end);
-    rows.push_back(dz);
-    rows.push_back(norm);
-    rows.push_back(acc ? 1.0 : 0.0);
-    attempts += 1;
-    accepted += acc ? 1 : 0;
+    log.record(end, dz, norm, acc);
   }
-  bool halt() override { return attempts >= limit; }
+  bool halt() override { return log.halt(); }
 };
 }  // namespace
 
-bool Workspace::advance(double target, long work_limit, std::vector<double>& rows,
-                        sp_march_result* r) {
+bool Workspace::advance_bounded(const Bounds& b, int columns, std::vector<double>& rows,
+                                sp_march_result* r) {
   if (!loaded_) throw Error(kState, "load a state before advancing");
   WorkspaceEngine engine(*this);
-  SessionHost host(rows, work_limit);
-  const bool done = march::march_to(policy_, engine, host, state_, target) &&
-                    march::reached(state_.z, target);
+  AttemptLog log;
+  log.begin(&rows, columns, b);
+  // A Runge-Kutta pair's attempts are all of its order: the estimator's plus one.
+  log.order = policy_.controller.estimator_order + 1;
+  SessionHost host(log);
+  bool done = false;
+  try {
+    done = march::march_to(policy_, engine, host, state_, b.target) &&
+           march::reached(state_.z, b.target);
+  } catch (...) {
+    r->z = state_.z;
+    r->proposal = state_.proposal;
+    r->attempts = log.attempts;
+    r->accepted = log.accepted;
+    r->evaluations = evaluations_;
+    throw;
+  }
   r->z = state_.z;
   r->proposal = state_.proposal;
   r->done = done ? 1 : 0;
-  r->attempts = host.attempts;
-  r->accepted = host.accepted;
+  r->attempts = log.attempts;
+  r->accepted = log.accepted;
   r->evaluations = evaluations_;
   return done;
 }

@@ -32,8 +32,8 @@ class Workspace final : public Propagator {
   void read_state(double* out) override;
   void read_trial(double* out) override;
   void export_state(const double* weight, bool optics, double* out) override;
-  bool advance(double target, long work_limit, std::vector<double>& rows,
-               sp_march_result* r) override;
+  bool advance_bounded(const Bounds& b, int columns, std::vector<double>& rows,
+                       sp_march_result* r) override;
 
   std::vector<double> statistics(std::uint64_t flags, const double* windows,
                                  int n_windows) override;

@@ -4,8 +4,10 @@ Mirrors ``native/src/solprop_native`` -- ``_lowering._lower`` (``_lowering.py:33
 fills the problem record, :class:`Workspace` is ``_core.Workspace``
 (``_core.pyi:595-607``), :meth:`Workspace.probe` returns a :class:`Probe`
 (``_core.pyi:36-49``, ``_probe.py:33-70``) and :meth:`Workspace.start` a
-:class:`Session` whose :meth:`Session.advance` marches natively until its target or
-its work bound (``_core.pyi:468-499``). Errors map onto the reference's classes:
+:class:`Session` whose :meth:`Session.advance` runs until the caller has work and
+returns one :class:`SessionBatch` with one :class:`SessionEvent`
+(``_core.pyi:468-558``); :meth:`Workspace.march` marches to a target directly. Errors
+map onto the reference's classes:
 a refusal before allocation is :class:`BackendRefusal`, the step floor is
 :class:`ConvergenceError`, everything else :class:`BackendError`.
 """
@@ -14,7 +16,9 @@ from __future__ import annotations
 
 import ctypes as C
 import math
-from dataclasses import dataclass
+import weakref
+from collections.abc import Mapping
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -223,12 +227,18 @@ class Workspace:
         _check(self._lib.sp_workspace_create(C.byref(desc), C.byref(handle), msg, 1024), msg)
         self._handle = handle
         self._shape = (problem.n_modes, problem.n_omega)
+        self._session = None  # a weak reference to the open session
 
     @property
     def closed(self) -> bool:
         return self._handle is None
 
     def close(self) -> None:
+        """Close the open session first, if any, then release the workspace."""
+        session = self._session() if self._session is not None else None
+        if session is not None:
+            session.close()
+        self._session = None
         if self._handle is not None:
             self._lib.sp_workspace_destroy(self._handle)
             self._handle = None
@@ -253,9 +263,19 @@ class Workspace:
         self.load(z, field)
         return Probe(self, z)
 
-    def start(self, z: float, field: np.ndarray) -> "Session":
+    def march(self, z: float, field: np.ndarray) -> "March":
+        """Load a state and march it to targets directly (``sp_workspace_advance``)."""
         self.load(z, field)
-        return Session(self, z)
+        return March(self, z)
+
+    def start(self, run: Mapping) -> "Session":
+        """``_core.Workspace.start(run) -> Session`` (``_core.pyi:595-607``): a session
+        of events over this workspace from ``run``'s launch (see :class:`Session`)."""
+        if self._handle is None:
+            raise LifecycleError("the workspace is closed")
+        session = Session(self, run)
+        self._session = weakref.ref(session)
+        return session
 
     def trial(self, z: float, dz: float) -> float:
         norm = C.c_double()
@@ -514,6 +534,217 @@ class Scan:
                 "device_bytes": t.device_bytes}
 
 
+SessionLifecycleError = LifecycleError
+
+# SessionEvent's kinds, observations and outcomes (native/include/solprop_native/session/
+# session.hpp: EventKind, Observation, Outcome).
+EVENT_KINDS = ("observation", "progress", "snapshot", "terminal", "warning")
+OBSERVATIONS = ("launch", "plane", "step", "natural")
+OUTCOMES = ("none", "completed", "stopped", "failed")
+STATES = ("ready", "pending", "finished", "closed")
+
+
+@dataclass
+class SessionEvent:
+    """``_core.SessionEvent`` (``_core.pyi:517-558``)."""
+
+    kind: int
+    observation: int
+    z: float
+    plane_index: int
+    has_stats: bool
+    ticket: bool
+    generation: int
+    outcome: int
+    reason: int
+    category: int
+    stage: int
+    message: str
+    floor_proposal: float
+
+    @property
+    def kind_name(self) -> str:
+        return EVENT_KINDS[self.kind]
+
+    @property
+    def observation_name(self) -> str:
+        return OBSERVATIONS[self.observation]
+
+    @property
+    def outcome_name(self) -> str:
+        return OUTCOMES[self.outcome]
+
+
+@dataclass
+class SessionBatch:
+    """``_core.SessionBatch`` (``_core.pyi:501-515``): one event and the chunk of attempt
+    rows (end, step, norm, accepted, order, wall) made since the previous batch."""
+
+    sequence: int
+    event: SessionEvent
+    attempts: np.ndarray
+    plane_stats: np.ndarray
+    row_z: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    row_step: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int64))
+    row_values: np.ndarray = field(default_factory=lambda: np.zeros(0))
+
+    @property
+    def ticket(self) -> tuple[int, int]:
+        return (self.sequence, self.event.generation)
+
+
+@dataclass
+class SessionStatus:
+    """``_core.SessionStatus`` (``_core.pyi:560-587``)."""
+
+    z: float
+    plane_index: int
+    attempts: int
+    rejected: int
+    state: int
+    outcome: int
+    generation: int
+    march_calls: int
+    rows_taken: int
+    batches: int
+    exports: int
+    export_bytes: int
+    buffer_bytes: int
+
+
+class Session:
+    """``_core.Session`` (``_core.pyi:468-499``): one trajectory's march as events.
+
+    ``run`` holds the launch (``z``, ``field``), the plan (``planes``: save planes,
+    increasing, the last the run's end; ``launch``: observe the launch first;
+    ``step_planes``: observe every k-th accepted step; ``statistics``: plane statistics
+    names taken at each observation, with ``windows``) and the buffers' and calls' bounds
+    (``attempt_capacity``, ``call_attempts``, ``call_seconds``). The session leases its
+    workspace until :meth:`close`.
+    """
+
+    def __init__(self, workspace: Workspace, run: Mapping):
+        self._ws = workspace
+        self._lib = workspace._lib
+        problem = workspace.problem
+        z = float(run.get("z", 0.0))
+        launch = run.get("field", problem.field0)
+        launch = np.ascontiguousarray(np.asarray(launch, dtype=np.complex128)
+                                      .reshape(workspace._shape))
+        planes = np.ascontiguousarray(run.get("planes", [problem.z_end]), dtype=np.float64)
+        names = list(run.get("statistics", ()))
+        unknown = set(names) - set(Workspace.STATISTICS)
+        if unknown:
+            raise BackendRefusal(f"statistics not built: {sorted(unknown)}")
+        flags = 0
+        for name in names:
+            flags |= Workspace.STATISTICS[name]
+        windows = run.get("windows")
+        windows = (np.zeros((0, 2)) if windows is None
+                   else np.ascontiguousarray(windows, dtype=np.float64).reshape(-1, 2))
+        self._capacity = int(run.get("attempt_capacity", 4096))
+        plan = _lib.SessionPlanC()
+        plan.planes = _ptr(planes)
+        plan.n_planes = len(planes)
+        plan.launch = int(bool(run.get("launch", True)))
+        plan.step_planes = int(run.get("step_planes", 0))
+        plan.statistics = flags
+        plan.windows = _ptr(windows)
+        plan.n_windows = len(windows)
+        plan.attempt_capacity = self._capacity
+        plan.call_attempts = int(run.get("call_attempts", 256))
+        plan.call_seconds = float(run.get("call_seconds", 0.1))
+        self._keep = (planes, windows, launch)
+        self._names = names
+        self._n_windows = len(windows)
+        handle = C.c_void_p()
+        msg = C.create_string_buffer(1024)
+        _check(self._lib.sp_session_start(workspace._handle, z, _ptr(launch), C.byref(plan),
+                                          C.byref(handle), msg, 1024), msg)
+        self._handle = handle
+        self._rows = np.zeros((self._capacity, 6))
+        # As many doubles as Workspace.statistics packs at most.
+        n_modes = problem.n_modes
+        self._stats = np.zeros(3 * n_modes + 2 * len(windows) * n_modes + 3 + problem.n_t)
+
+    def _call(self, name, *args):
+        if self._handle is None:
+            raise LifecycleError("the session is closed")
+        msg = C.create_string_buffer(1024)
+        _check(getattr(self._lib, name)(self._handle, *args, msg, 1024), msg)
+
+    def _batch(self, b) -> SessionBatch:
+        e = b.event
+        event = SessionEvent(e.kind, e.observation, e.z, e.plane_index, bool(e.has_stats),
+                             bool(e.ticket), int(e.generation), e.outcome, e.reason,
+                             e.category, e.stage, e.message.decode(errors="replace"),
+                             e.floor_proposal)
+        return SessionBatch(int(b.sequence), event, self._rows[: b.n_attempts].copy(),
+                            self._stats[: b.n_stats].copy())
+
+    def advance(self, work_limit: int = 256, deadline: float = 0.1) -> SessionBatch:
+        b = _lib.SessionBatchC()
+        self._call("sp_session_advance", int(work_limit), float(deadline), C.byref(b),
+                   _ptr(self._rows), self._capacity, _ptr(self._stats), len(self._stats))
+        return self._batch(b)
+
+    def acknowledge(self, sequence: int) -> None:
+        self._call("sp_session_acknowledge", int(sequence))
+
+    def request_stop(self) -> None:
+        if self._handle is not None:
+            self._lib.sp_session_request_stop(self._handle)
+
+    def status(self) -> SessionStatus:
+        if self._handle is None:
+            raise LifecycleError("the session is closed")
+        s = _lib.SessionStatusC()
+        _check(self._lib.sp_session_status_of(self._handle, C.byref(s)), b"")
+        return SessionStatus(s.z, s.plane_index, s.attempts, s.rejected, s.state, s.outcome,
+                             s.generation, s.march_calls, 0, s.batches, s.exports,
+                             s.export_bytes, self._rows.nbytes)
+
+    def export(self, ticket: tuple[int, int], recipe: Mapping, out: np.ndarray) -> None:
+        """The ticket's state into ``out`` (complex (C, N), C-contiguous, writable) by
+        ``recipe``: ``power_weight`` (or None) then ``optics`` (``apply_recipe``)."""
+        if out.dtype != np.complex128 or out.shape != self._ws._shape or \
+                not out.flags.c_contiguous or not out.flags.writeable:
+            raise BackendRefusal("export needs a writable C-contiguous complex128 "
+                                 f"array of shape {self._ws._shape}")
+        weight = recipe.get("power_weight")
+        if weight is not None:
+            weight = np.ascontiguousarray(np.broadcast_to(weight, self._ws._shape),
+                                          dtype=np.float64)
+        sequence, generation = ticket
+        self._call("sp_session_export", int(sequence), int(generation),
+                   None if weight is None else _ptr(weight), int(bool(recipe.get("optics", True))),
+                   out.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def fail_outside(self) -> SessionBatch | None:
+        if self._handle is None:
+            return None
+        b = _lib.SessionBatchC()
+        msg = C.create_string_buffer(1024)
+        status = self._lib.sp_session_fail_outside(self._handle, b"failed outside the session",
+                                                   C.byref(b), _ptr(self._rows), self._capacity,
+                                                   msg, 1024)
+        if status == _lib.SP_STATE:
+            return None
+        _check(status, msg)
+        return self._batch(b)
+
+    def close(self) -> None:
+        if self._handle is not None:
+            self._lib.sp_session_close(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Probe:
     """One trial from a loaded state, committing nothing (``_probe.py:33-70``)."""
 
@@ -540,8 +771,9 @@ class Batch:
     evaluations: int
 
 
-class Session:
-    """A native march over the workspace (``_core.pyi`` Session)."""
+class March:
+    """A march over the workspace to a target z, attempt rows returned per call (the
+    native march call under the reference's session, without its events)."""
 
     def __init__(self, workspace: Workspace, z: float):
         self._ws = workspace
@@ -729,6 +961,7 @@ def adams_trace(scenario: dict) -> dict:
 
 __all__ = [
     "BackendError", "BackendRefusal", "Batch", "ConvergenceError", "LifecycleError",
-    "Probe", "Session", "Settings", "Workspace", "adams_trace", "lower", "rk_trace",
+    "March", "Probe", "Session", "SessionBatch", "SessionEvent", "SessionLifecycleError",
+    "SessionStatus", "Settings", "Workspace", "adams_trace", "lower", "rk_trace",
 ]
 _ = math

@@ -35,12 +35,12 @@ def test_batch_matches_instances_run_alone():
     alone = []
     for p in problems:
         ws = Workspace(p)
-        session = ws.start(0.0, p.field0)
+        session = ws.march(0.0, p.field0)
         batch = session.advance(p.z_end, work_limit=limit)
         alone.append((batch.attempts.copy(), session.state()))
         ws.close()
     spaces = [Workspace(p) for p in problems]
-    sessions = [ws.start(0.0, p.field0) for ws, p in zip(spaces, problems)]
+    sessions = [ws.march(0.0, p.field0) for ws, p in zip(spaces, problems)]
     batches = advance_batch(sessions, [p.z_end for p in problems], work_limit=limit)
     together = [(b.attempts.copy(), s.state()) for b, s in zip(batches, sessions)]
     for ws in spaces:
@@ -75,7 +75,7 @@ def test_zero_field_is_refused_as_the_oracle_refuses_it(route):
     with pytest.raises(OracleConvergence):
         march(problem, y0.copy(), max_attempts=12, on_attempt=lambda *a: seen.append(a))
     ws = Workspace(problem)
-    session = ws.start(0.0, zero)
+    session = ws.march(0.0, zero)
     with pytest.raises(ConvergenceError):
         session.advance(problem.z_end, work_limit=12)
     counters = ws.counters()

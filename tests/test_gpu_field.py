@@ -111,7 +111,7 @@ def test_march_matches_oracle_decisions(case, proposal):
     if proposal is not None:
         assert not all(r[3] for r in records)  # the large first step is rejected
     ws = Workspace(problem, Settings(dz_init=proposal))
-    session = ws.start(0.0, problem.field0)
+    session = ws.march(0.0, problem.field0)
     batch = session.advance(problem.z_end, work_limit=len(records))
     rows = batch.attempts
     state = session.state()
@@ -129,7 +129,7 @@ def test_field_repeats_agree_to_the_bit():
     outs = []
     for _ in range(2):
         ws = Workspace(problem)
-        session = ws.start(0.0, problem.field0)
+        session = ws.march(0.0, problem.field0)
         batch = session.advance(problem.z_end, work_limit=12)
         outs.append((batch.attempts.copy(), session.state()))
         ws.close()
@@ -146,7 +146,7 @@ def test_field_graphs_and_direct_launches_agree():
     for flag in ("1", "0"):
         os.environ["SOLPROP_MI355X_GRAPH"] = flag
         ws = Workspace(problem)
-        session = ws.start(0.0, problem.field0)
+        session = ws.march(0.0, problem.field0)
         session.advance(problem.z_end, work_limit=12)
         results.append(session.state())
         ws.close()
@@ -182,7 +182,7 @@ def test_absorbers_change_the_march(case):
     for kw in (CASES[case], {}):
         problem = small(**kw)
         ws = Workspace(problem)
-        session = ws.start(0.0, problem.field0)
+        session = ws.march(0.0, problem.field0)
         session.advance(problem.z_end, work_limit=12)
         ends.append(session.state())
         ws.close()
@@ -242,7 +242,7 @@ def test_march_matches_oracle_at_full_size():
     expected, records = adams.march(problem, y0.copy(), max_attempts=40, marches=marches)
     assert max(marches[0].ledger.orders) == 5
     ws = Workspace(problem)
-    session = ws.start(0.0, problem.field0)
+    session = ws.march(0.0, problem.field0)
     batch = session.advance(problem.z_end, work_limit=len(records))
     rows = batch.attempts
     state = session.state()

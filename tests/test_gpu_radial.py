@@ -113,7 +113,7 @@ def test_radial_march_matches_oracle_decisions(case):
     y0 = problem.field0.reshape(problem.shape)
     expected, records = adams.march(problem, y0.copy(), max_attempts=30)
     ws = Workspace(problem)
-    session = ws.start(0.0, problem.field0)
+    session = ws.march(0.0, problem.field0)
     batch = session.advance(problem.z_end, work_limit=len(records))
     rows = batch.attempts
     state = session.state()
@@ -134,7 +134,7 @@ def test_collar_changes_the_march():
     for case in ("collar", "wide_beam"):
         problem = small(**CASES[case])
         ws = Workspace(problem)
-        session = ws.start(0.0, problem.field0)
+        session = ws.march(0.0, problem.field0)
         batch = session.advance(target, work_limit=200)
         assert batch.done
         ends.append(session.state())
@@ -148,7 +148,7 @@ def test_radial_repeats_agree_to_the_bit():
     outs = []
     for _ in range(2):
         ws = Workspace(problem)
-        session = ws.start(0.0, problem.field0)
+        session = ws.march(0.0, problem.field0)
         batch = session.advance(problem.z_end, work_limit=10)
         outs.append((batch.attempts.copy(), session.state()))
         ws.close()
@@ -175,7 +175,7 @@ def test_radial_wide_grid_matches_oracle():
     y0 = problem.field0.reshape(problem.shape)
     expected, records = adams.march(problem, y0.copy(), max_attempts=12)
     ws = Workspace(problem)
-    session = ws.start(0.0, problem.field0)
+    session = ws.march(0.0, problem.field0)
     rows = session.advance(problem.z_end, work_limit=len(records)).attempts
     state = session.state()
     ws.close()

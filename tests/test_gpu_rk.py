@@ -107,7 +107,7 @@ def test_march_matches_oracle_decisions():
     y0 = problem.field0.reshape(problem.shape)
     expected, records = rk.march(problem, y0)
     ws = Workspace(problem)
-    session = ws.start(0.0, problem.field0)
+    session = ws.march(0.0, problem.field0)
     rows = session.run(problem.z_end, work_limit=64)
     state = session.state()
     ws.close()
@@ -135,7 +135,7 @@ def test_march_with_absorber_matches_oracle(scheme):
     expected, records = rk.march(problem, y0)
     plain, _ = rk.march(small(length=5e-3), y0)
     ws = Workspace(problem)
-    session = ws.start(0.0, problem.field0)
+    session = ws.march(0.0, problem.field0)
     rows = session.run(problem.z_end, work_limit=64)
     state = session.state()
     ws.close()
@@ -158,7 +158,7 @@ def test_split_absorber_graphs_and_direct_launches_agree():
     for flag in ("1", "0"):
         os.environ["SOLPROP_MI355X_GRAPH"] = flag
         ws = Workspace(problem, Settings())
-        session = ws.start(0.0, problem.field0)
+        session = ws.march(0.0, problem.field0)
         session.run(problem.z_end)
         results.append(session.state())
         ws.close()
@@ -172,7 +172,7 @@ def test_repeats_agree_to_the_bit():
     outs = []
     for _ in range(2):
         ws = Workspace(problem)
-        session = ws.start(0.0, problem.field0)
+        session = ws.march(0.0, problem.field0)
         rows = session.run(problem.z_end)
         outs.append((rows.copy(), session.state()))
         ws.close()
@@ -203,7 +203,7 @@ def test_graphs_and_direct_launches_agree():
     for flag in ("1", "0"):
         os.environ["SOLPROP_MI355X_GRAPH"] = flag
         ws = Workspace(problem, Settings())
-        session = ws.start(0.0, problem.field0)
+        session = ws.march(0.0, problem.field0)
         session.run(problem.z_end)
         results.append(session.state())
         ws.close()

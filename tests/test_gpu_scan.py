@@ -38,7 +38,7 @@ def points():
 
 def alone(problem, limit):
     ws = Workspace(problem)
-    session = ws.start(0.0, problem.field0)
+    session = ws.march(0.0, problem.field0)
     batch = session.advance(problem.z_end, work_limit=limit)
     out = (batch.attempts.copy(), session.state())
     ws.close()

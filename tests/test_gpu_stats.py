@@ -53,7 +53,7 @@ def test_statistics_match_oracle(route, marched):
     middle = 0.5 * (w.min() + w.max())
     windows = np.array([[w.min(), middle], [middle, np.inf]])
     ws = Workspace(problem)
-    session = ws.start(0.0, problem.field0)
+    session = ws.march(0.0, problem.field0)
     if marched:
         session.advance(problem.z_end, work_limit=6)
     state = session.state()
@@ -94,7 +94,7 @@ def test_monitor_reading_matches_oracle(route):
     problem = PROBLEMS[route]()
     weights = monitor_weights(problem)
     ws = Workspace(problem)
-    session = ws.start(0.0, problem.field0)
+    session = ws.march(0.0, problem.field0)
     session.advance(problem.z_end, work_limit=4)
     ws.bind_monitor(weights)
     state = session.state()
@@ -142,7 +142,7 @@ def test_split_sums_follow_the_oracle_march(route):
     ws = Workspace(problem)
     ws.load(0.0, problem.field0)
     ws.bind_monitor(weights)
-    session = ws.start(0.0, problem.field0)
+    session = ws.march(0.0, problem.field0)
     rows = session.advance(problem.z_end, work_limit=len(records)).attempts
     first = ws.monitor_reading()
     second = ws.monitor_reading()
@@ -169,7 +169,7 @@ def test_plasma_statistics_match_oracle(route):
                                  fwhm=30e-15, w0=0.25e-3, integration="band",
                                  band=(400e-9, 4e-6))
     ws = Workspace(problem)
-    session = ws.start(0.0, problem.field0)
+    session = ws.march(0.0, problem.field0)
     session.advance(problem.z_end, work_limit=4)
     state, z = session.state(), session.z
     got = ws.statistics(["electron_density", "peak_ionisation_rate"])
