@@ -504,6 +504,46 @@ int sp_testing_fft(const double* in, double* out, long n, long batch, int sign, 
 int sp_fft_noalloc(const double* in, double* out, long n, long batch, int sign, void* stream,
                    char* msg, size_t msglen);
 
+/* ---------------------------------------------------------- learned surrogate
+ *
+ * A 1-D Fourier neural operator that screens a parameter scan before the solver runs:
+ * it maps a point's input spectrum (n bins) and its scan parameters to the exit spectrum
+ * (the project's own component; the reference has no counterpart). Trained and run on
+ * the device: every dense product (the 1x1 convolutions, the truncated DFT onto the
+ * first `modes` bins and its inverse, the head) on the matrix cores in BF16 with FP32
+ * accumulation; the per-mode complex mixing, loss, reductions and Adam in FP32. The
+ * parameters are one float32 vector in the order of oracle/fno.py's layout; the input
+ * channels are the spectrum, the parameters and the grid coordinate s/n. */
+typedef struct sp_fno sp_fno;
+typedef struct sp_fno_shape {
+  int n_params, width, modes, layers, hidden, n;
+} sp_fno_shape;
+int sp_fno_create(const sp_fno_shape* shape, int max_batch, sp_fno** out, char* msg,
+                  size_t msglen);
+void sp_fno_destroy(sp_fno* f);
+long sp_fno_parameter_count(const sp_fno* f);
+int sp_fno_set_parameters(sp_fno* f, const float* flat, char* msg, size_t msglen);
+int sp_fno_get_parameters(sp_fno* f, float* flat, char* msg, size_t msglen);
+/* out (batch, n) from spectra (batch, n) and params (batch, n_params), host arrays;
+ * batch at most max_batch. */
+int sp_fno_forward(sp_fno* f, const float* spectra, const float* params, int batch, float* out,
+                   char* msg, size_t msglen);
+/* The mean squared error against target (batch, n) and its gradient (flat, host). */
+int sp_fno_gradient(sp_fno* f, const float* spectra, const float* params, const float* target,
+                    int batch, double* loss, float* grad, char* msg, size_t msglen);
+/* A device-resident training set of n rows; then `steps` Adam steps (beta 0.9, 0.999,
+ * eps 1e-8) over minibatches of its rows, `indices` (steps x batch) drawn by the caller,
+ * each step's loss into `losses` and the device time into `device_ms`. */
+int sp_fno_load_dataset(sp_fno* f, const float* spectra, const float* params,
+                        const float* targets, long n, char* msg, size_t msglen);
+int sp_fno_train(sp_fno* f, const int* indices, long steps, int batch, float lr, float* losses,
+                 double* device_ms, char* msg, size_t msglen);
+int sp_fno_reset_optimizer(sp_fno* f);
+/* C (M x N) = A (M x K) . B (K x N) on the BF16 matrix cores, host float32 arrays
+ * rounded to BF16 (a lane-layout check of the surrogate's product). */
+int sp_testing_gemm_bf16(long M, long N, long K, const float* A, const float* B, float* C,
+                         char* msg, size_t msglen);
+
 /* Library identification: the gfx target and build string. */
 const char* sp_version(void);
 

@@ -172,6 +172,13 @@ class SessionStatusC(C.Structure):
     ]
 
 
+class FnoShapeC(C.Structure):
+    _fields_ = [("n_params", _I), ("width", _I), ("modes", _I), ("layers", _I), ("hidden", _I),
+                ("n", _I)]
+
+
+_PF = C.POINTER(C.c_float)
+
 # The exported entry points and their signatures (include/solprop_mi355x.h).
 SIGNATURES = {
     "sp_workspace_create": (_I, [C.POINTER(ProblemDescC), C.POINTER(C.c_void_p), C.c_char_p,
@@ -225,6 +232,22 @@ SIGNATURES = {
     "sp_session_fail_outside": (_I, [C.c_void_p, C.c_char_p, C.POINTER(SessionBatchC), _PD,
                                      C.c_long, C.c_char_p, C.c_size_t]),
     "sp_session_close": (None, [C.c_void_p]),
+    "sp_fno_create": (_I, [C.POINTER(FnoShapeC), _I, C.POINTER(C.c_void_p), C.c_char_p,
+                           C.c_size_t]),
+    "sp_fno_destroy": (None, [C.c_void_p]),
+    "sp_fno_parameter_count": (C.c_long, [C.c_void_p]),
+    "sp_fno_set_parameters": (_I, [C.c_void_p, _PF, C.c_char_p, C.c_size_t]),
+    "sp_fno_get_parameters": (_I, [C.c_void_p, _PF, C.c_char_p, C.c_size_t]),
+    "sp_fno_forward": (_I, [C.c_void_p, _PF, _PF, _I, _PF, C.c_char_p, C.c_size_t]),
+    "sp_fno_gradient": (_I, [C.c_void_p, _PF, _PF, _PF, _I, _PD, _PF, C.c_char_p,
+                             C.c_size_t]),
+    "sp_fno_load_dataset": (_I, [C.c_void_p, _PF, _PF, _PF, C.c_long, C.c_char_p,
+                                 C.c_size_t]),
+    "sp_fno_train": (_I, [C.c_void_p, C.POINTER(_I), C.c_long, _I, C.c_float, _PF, _PD,
+                          C.c_char_p, C.c_size_t]),
+    "sp_fno_reset_optimizer": (_I, [C.c_void_p]),
+    "sp_testing_gemm_bf16": (_I, [C.c_long, C.c_long, C.c_long, _PF, _PF, _PF, C.c_char_p,
+                                  C.c_size_t]),
     "sp_version": (C.c_char_p, []),
 }
 

@@ -13,6 +13,7 @@
 #include "field_ws.hpp"
 #include "scan.hpp"
 #include "session.hpp"
+#include "fno.hpp"
 #include "gemm.hpp"
 #include "trace.hpp"
 #include "workspace.hpp"
@@ -37,6 +38,10 @@ struct sp_workspace {
   bool doomed = false;
   explicit sp_workspace(const sp_problem_desc& d)
       : owner(make_propagator(d)), impl(*owner), n_modes(d.n_modes), n_omega(d.n_omega) {}
+};
+
+struct sp_fno {
+  std::unique_ptr<sp::fno::Network> impl;
 };
 
 struct sp_session {
@@ -560,6 +565,83 @@ void sp_session_close(sp_session* s) {
     release = ws->doomed;
   }
   if (release) delete ws;
+}
+
+// ---------------------------------------------------------- learned surrogate
+
+int sp_fno_create(const sp_fno_shape* shape, int max_batch, sp_fno** out, char* msg,
+                  size_t msglen) {
+  if (!shape || !out) return SP_VALUE;
+  *out = nullptr;
+  return guarded(msg, msglen, [&] {
+    sp::fno::Shape s;
+    s.n_params = shape->n_params;
+    s.width = shape->width;
+    s.modes = shape->modes;
+    s.layers = shape->layers;
+    s.hidden = shape->hidden;
+    s.n = shape->n;
+    *out = new sp_fno{std::make_unique<sp::fno::Network>(s, max_batch)};
+  });
+}
+
+void sp_fno_destroy(sp_fno* f) { delete f; }
+
+long sp_fno_parameter_count(const sp_fno* f) { return f ? f->impl->parameter_count() : 0; }
+
+int sp_fno_set_parameters(sp_fno* f, const float* flat, char* msg, size_t msglen) {
+  if (!f || !flat) return SP_VALUE;
+  return guarded(msg, msglen, [&] { f->impl->set_parameters(flat); });
+}
+
+int sp_fno_get_parameters(sp_fno* f, float* flat, char* msg, size_t msglen) {
+  if (!f || !flat) return SP_VALUE;
+  return guarded(msg, msglen, [&] { f->impl->get_parameters(flat); });
+}
+
+int sp_fno_forward(sp_fno* f, const float* spectra, const float* params, int batch, float* out,
+                   char* msg, size_t msglen) {
+  if (!f) return SP_VALUE;
+  return guarded(msg, msglen, [&] { f->impl->forward(spectra, params, batch, out); });
+}
+
+int sp_fno_gradient(sp_fno* f, const float* spectra, const float* params, const float* target,
+                    int batch, double* loss, float* grad, char* msg, size_t msglen) {
+  if (!f || !loss) return SP_VALUE;
+  return guarded(msg, msglen,
+                 [&] { *loss = f->impl->gradient(spectra, params, target, batch, grad); });
+}
+
+int sp_fno_load_dataset(sp_fno* f, const float* spectra, const float* params,
+                        const float* targets, long n, char* msg, size_t msglen) {
+  if (!f) return SP_VALUE;
+  return guarded(msg, msglen, [&] { f->impl->load_dataset(spectra, params, targets, n); });
+}
+
+int sp_fno_train(sp_fno* f, const int* indices, long steps, int batch, float lr, float* losses,
+                 double* device_ms, char* msg, size_t msglen) {
+  if (!f) return SP_VALUE;
+  return guarded(msg, msglen, [&] {
+    f->impl->train(indices, steps, batch, lr, losses);
+    if (device_ms) *device_ms = f->impl->train_ms();
+  });
+}
+
+int sp_fno_reset_optimizer(sp_fno* f) {
+  if (!f) return SP_VALUE;
+  char msg[8];
+  return guarded(msg, sizeof msg, [&] { f->impl->reset_optimizer(); });
+}
+
+int sp_testing_gemm_bf16(long M, long N, long K, const float* A, const float* B, float* C,
+                         char* msg, size_t msglen) {
+  if (!A || !B || !C) return SP_VALUE;
+  return guarded(msg, msglen, [&] {
+    SP_CHECK(M > 0 && N > 0 && K > 0 && M < (1L << 20) && N < (1L << 20) && K < (1L << 20),
+             "GEMM dimensions must be positive");
+    sp::fno::testing_gemm_bf16(static_cast<int>(M), static_cast<int>(N), static_cast<int>(K), A,
+                               B, C);
+  });
 }
 
 const char* sp_version(void) { return "solprop_mi355x 0.1 gfx950 fp64"; }
