@@ -60,6 +60,11 @@ struct GemmArgs {
   bf16* act16;
   long ld16, s16b;
   float alpha;
+  // Split K (reduced-batch or single-batch products only): workgroup z sums its share of
+  // the (batch, K tile) iterations into partial[z] (M x N, raw); a second kernel adds the
+  // shares in split order and applies the epilogue.
+  int splits;
+  float* partial;
 };
 
 constexpr int kBM = 64, kBN = 64, kBK = 32, kPad = 8;
@@ -74,12 +79,22 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
   v4f acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-  const int b_first = g.reduce_batch ? 0 : blockIdx.z;
-  const int b_last = g.reduce_batch ? g.batch : blockIdx.z + 1;
-  for (int b = b_first; b < b_last; ++b) {
+  // The (batch, K tile) iterations this workgroup sums, in order.
+  const int k_tiles = (g.K + kBK - 1) / kBK;
+  const int total = (g.reduce_batch ? g.batch : 1) * k_tiles;
+  int it0 = 0, it1 = total, b_base = g.reduce_batch ? 0 : blockIdx.z;
+  if (g.splits > 1) {
+    const int per = (total + g.splits - 1) / g.splits;
+    it0 = blockIdx.z * per;
+    it1 = min(total, it0 + per);
+    b_base = 0;
+  }
+  for (int it = it0; it < it1; ++it) {
+    const int b = b_base + it / k_tiles;
+    const int k0 = (it % k_tiles) * kBK;
     const bf16* A = g.A + b * g.sAb;
     const bf16* B = g.B + b * g.sBb;
-    for (int k0 = 0; k0 < g.K; k0 += kBK) {
+    {
       // A tile: 64 x 32, eight elements a thread along the unit-stride axis.
       if (g.sAm == 1 && g.sAk != 1) {
         const int k = tid >> 3, mc = (tid & 7) * 8;
@@ -132,6 +147,20 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
     }
   }
   // Epilogue: lane holds rows 4 (lane / 16) + r of column lane % 16 of each 16 x 16 tile.
+  if (g.splits > 1) {
+    float* part = g.partial + static_cast<long>(blockIdx.z) * g.M * g.N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+          const int n = n0 + wn + 16 * j + (lane & 15);
+          if (m < g.M && n < g.N) part[static_cast<long>(m) * g.N + n] = acc[i][j][r];
+        }
+    return;
+  }
   const int bc = g.reduce_batch ? 0 : blockIdx.z;
   float* C = g.C + bc * g.sCb;
 #pragma unroll
@@ -154,11 +183,49 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
       }
 }
 
-void launch_gemm(const GemmArgs& g, hipStream_t s) {
+// The split shares summed in split order, then the epilogue (one thread an element).
+__global__ void splitk_reduce(GemmArgs g) {
+  const long e = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<long>(g.M) * g.N) return;
+  const int m = static_cast<int>(e / g.N), n = static_cast<int>(e % g.N);
+  float acc = 0.f;
+  for (int z = 0; z < g.splits; ++z) acc += g.partial[static_cast<long>(z) * g.M * g.N + e];
+  float v = g.alpha * acc;
+  float* at = g.C + static_cast<long>(m) * g.ldc + n;
+  if (g.accumulate) v += *at;
+  if (g.bias) v += g.bias[m];
+  *at = v;
+  if (g.act16) g.act16[static_cast<long>(m) * g.ld16 + n] = to_bf16(g.act == 1 ? gelu(v) : v);
+}
+
+// Launch one product; a reduced-batch or single-batch product with too few output tiles
+// to fill the chip splits its K iterations over up to `max_splits` workgroups a tile
+// (`scratch` holds max_splits x M x N floats at most `scratch_floats`).
+void launch_gemm(GemmArgs g, hipStream_t s, float* scratch = nullptr, long scratch_floats = 0) {
   SP_CHECK(g.M > 0 && g.N > 0 && g.K > 0 && g.batch > 0, "GEMM dimensions must be positive");
-  const dim3 grid((g.N + kBN - 1) / kBN, (g.M + kBM - 1) / kBM, g.reduce_batch ? 1 : g.batch);
+  const int gx = (g.N + kBN - 1) / kBN, gy = (g.M + kBM - 1) / kBM;
+  g.splits = 1;
+  if (scratch && (g.reduce_batch || g.batch == 1)) {
+    const long tiles = static_cast<long>(gx) * gy;
+    const long iters = static_cast<long>(g.reduce_batch ? g.batch : 1) * ((g.K + kBK - 1) / kBK);
+    long want = (512 + tiles - 1) / tiles;
+    want = std::min(want, iters / 4);
+    want = std::min(want, scratch_floats / (static_cast<long>(g.M) * g.N));
+    want = std::min<long>(want, 1024);
+    if (want > 1) {
+      g.splits = static_cast<int>(want);
+      g.partial = scratch;
+    }
+  }
+  const dim3 grid(gx, gy, g.splits > 1 ? g.splits : (g.reduce_batch ? 1 : g.batch));
   hipLaunchKernelGGL(gemm_bf16, grid, dim3(256), 0, s, g);
   SP_HIP(hipGetLastError());
+  if (g.splits > 1) {
+    const long count = static_cast<long>(g.M) * g.N;
+    hipLaunchKernelGGL(splitk_reduce, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256),
+                       0, s, g);
+    SP_HIP(hipGetLastError());
+  }
 }
 
 GemmArgs gemm(const bf16* A, long sAm, long sAk, long sAb, const bf16* B, long sBk, long sBn,
@@ -535,6 +602,7 @@ Network::Network(const Shape& s, int max_batch) : s_(s), off_(offsets(s)), max_b
     dh1_ = alloc<float>(B * H * n);
     dh116_ = alloc<bf16>(B * H * n);
     partials_ = alloc<float>(blocks(B * n, 256) + 1);  // the partial sums, then the loss
+    split_ = alloc<float>(kSplitFloats);
     SP_HIP(hipStreamSynchronize(stream_));
   } catch (...) {
     for (void* p : owned_) (void)hipFree(p);
@@ -591,12 +659,12 @@ void Network::run_forward(int B) {
   for (int l = 0; l < s_.layers; ++l) {
     // vhat = v F over all (b, channel) rows: (B W, n) x (n, 2M).
     launch_gemm(gemm(v16_[l], n, 1, 0, f16_, 2 * M, 1, 0, vhat_[l], 2 * M, 0, B * W, 2 * M, n, 1),
-                stream_);
+                stream_, split_, kSplitFloats);
     hipLaunchKernelGGL(mix_forward, dim3(M, B), dim3(W), 0, stream_, vhat_[l],
                        param_ + off_.r_re[l], param_ + off_.r_im[l], yhat16_, W, M);
     SP_HIP(hipGetLastError());
     // z = yhat G: (B W, 2M) x (2M, n).
-    launch_gemm(gemm(yhat16_, 2 * M, 1, 0, g16_, n, 1, 0, z_[l], n, 0, B * W, n, 2 * M, 1), stream_);
+    launch_gemm(gemm(yhat16_, 2 * M, 1, 0, g16_, n, 1, 0, z_[l], n, 0, B * W, n, 2 * M, 1), stream_, split_, kSplitFloats);
     // z += W v + b per sample; the activation (gelu, or none on the last layer) in BF16.
     GemmArgs g = gemm(param16_ + off_.w[l], W, 1, 0, v16_[l], n, 1, static_cast<long>(W) * n,
                       z_[l], n, static_cast<long>(W) * n, W, n, W, B);
@@ -606,7 +674,7 @@ void Network::run_forward(int B) {
     g.act16 = v16_[l + 1];
     g.ld16 = n;
     g.s16b = static_cast<long>(W) * n;
-    launch_gemm(g, stream_);
+    launch_gemm(g, stream_, split_, kSplitFloats);
   }
   // Head: h1 = Q1 v_L + q1 (gelu in BF16), out = q2 . h + q2b.
   GemmArgs g = gemm(param16_ + off_.q1_w, W, 1, 0, v16_[s_.layers], n, 1, static_cast<long>(W) * n,
@@ -616,7 +684,7 @@ void Network::run_forward(int B) {
   g.act16 = h16_;
   g.ld16 = n;
   g.s16b = static_cast<long>(H) * n;
-  launch_gemm(g, stream_);
+  launch_gemm(g, stream_, split_, kSplitFloats);
   hipLaunchKernelGGL(head_out, dim3(blocks(n, 256), B), dim3(256), 0, stream_, h16_,
                      param_ + off_.q2_w, param_ + off_.q2_b, out_, H, n);
   SP_HIP(hipGetLastError());
@@ -636,13 +704,13 @@ void Network::run_backward(int B) {
   GemmArgs g = gemm(dh116_, n, 1, static_cast<long>(H) * n, v16_[s_.layers], 1, n, Wn,
                     grad_ + off_.q1_w, W, 0, H, W, n, B);
   g.reduce_batch = 1;
-  launch_gemm(g, stream_);
+  launch_gemm(g, stream_, split_, kSplitFloats);
   hipLaunchKernelGGL(channel_sum, dim3(H), dim3(256), 0, stream_, dh1_, grad_ + off_.q1_b, H, n, B);
   SP_HIP(hipGetLastError());
   // dv_L = Q1^T dh1: (W, H) x (H, n) per sample.
   launch_gemm(gemm(param16_ + off_.q1_w, 1, W, 0, dh116_, n, 1, static_cast<long>(H) * n, dv_, n,
                    Wn, W, n, H, B),
-              stream_);
+              stream_, split_, kSplitFloats);
   for (int l = s_.layers - 1; l >= 0; --l) {
     const long count = static_cast<long>(B) * Wn;
     hipLaunchKernelGGL(act_backward, dim3(blocks(count, 256)), dim3(256), 0, stream_, dv_, z_[l],
@@ -651,11 +719,11 @@ void Network::run_backward(int B) {
     // dW = sum_b dz_b v_b^T, db = sum dz.
     GemmArgs gw = gemm(dz16_, n, 1, Wn, v16_[l], 1, n, Wn, grad_ + off_.w[l], W, 0, W, W, n, B);
     gw.reduce_batch = 1;
-    launch_gemm(gw, stream_);
+    launch_gemm(gw, stream_, split_, kSplitFloats);
     hipLaunchKernelGGL(channel_sum, dim3(W), dim3(256), 0, stream_, dz_, grad_ + off_.b[l], W, n, B);
     SP_HIP(hipGetLastError());
     // dyhat = dz G^T: (B W, n) x (n, 2M).
-    launch_gemm(gemm(dz16_, n, 1, 0, g16_, 1, n, 0, dyhat_, 2 * M, 0, B * W, 2 * M, n, 1), stream_);
+    launch_gemm(gemm(dz16_, n, 1, 0, g16_, 1, n, 0, dyhat_, 2 * M, 0, B * W, 2 * M, n, 1), stream_, split_, kSplitFloats);
     hipLaunchKernelGGL(mix_backward_weight, dim3(M, W), dim3(W), 0, stream_, vhat_[l], dyhat_,
                        grad_ + off_.r_re[l], grad_ + off_.r_im[l], W, M, B);
     SP_HIP(hipGetLastError());
@@ -664,10 +732,10 @@ void Network::run_backward(int B) {
     SP_HIP(hipGetLastError());
     // dv = W^T dz (per sample), then += dvhat F^T: (B W, 2M) x (2M, n).
     launch_gemm(gemm(param16_ + off_.w[l], 1, W, 0, dz16_, n, 1, Wn, dv_, n, Wn, W, n, W, B),
-                stream_);
+                stream_, split_, kSplitFloats);
     GemmArgs gf = gemm(dvhat16_, 2 * M, 1, 0, f16_, 1, 2 * M, 0, dv_, n, 0, B * W, n, 2 * M, 1);
     gf.accumulate = 1;
-    launch_gemm(gf, stream_);
+    launch_gemm(gf, stream_, split_, kSplitFloats);
   }
   hipLaunchKernelGGL(lift_backward, dim3(W), dim3(256), 0, stream_, dv_, spec_, prm_,
                      grad_ + off_.lift_w, grad_ + off_.lift_b, W, n, s_.n_params, B);

@@ -97,6 +97,9 @@ class Network {
   float *dout_ = nullptr, *dv_ = nullptr, *dz_ = nullptr, *dyhat_ = nullptr, *dh1_ = nullptr;
   bf16 *dz16_ = nullptr, *dvhat16_ = nullptr, *dh116_ = nullptr, *dout16_ = nullptr;
   float* partials_ = nullptr;    // loss partial sums
+  // Split-K shares of the long-K products (weight gradients, the truncated transforms).
+  static constexpr long kSplitFloats = 16L << 20;
+  float* split_ = nullptr;
   float* losses_ = nullptr;      // per training step
   long losses_cap_ = 0;
   int* indices_ = nullptr;
