@@ -216,6 +216,7 @@ def run_scan(args):
     accepted = int(sum(b.attempts[:, 3].sum() for b in batches))
     attempts = int(sum(len(b.attempts) for b in batches))
     timing = scan.timing()
+    schedule = scan.schedule()
     finished = None
     if args.to_completion:
         t1 = time.perf_counter()
@@ -260,6 +261,7 @@ def run_scan(args):
         "aggregate": "accepted instance-steps/s over all instances (a step: one round)",
         "attempts": attempts, "setup_s": setup_s,
         "scan_timing": timing,
+        "autotuned_schedule": schedule,
         "plain": {"value": plain, "unit": "accepted instance-steps/s",
                   "sample": f"{len(sample)} of the instances, each marched alone "
                             f"({args.steps} attempts, graphs captured first)"},

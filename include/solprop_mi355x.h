@@ -232,6 +232,13 @@ typedef struct sp_scan_timing {
   long device_bytes;
 } sp_scan_timing;
 int sp_scan_timing_of(sp_scan* scan, sp_scan_timing* out);
+/* The scan's autotuned schedule as JSON (the project's own tuner; the reference has
+ * none): for every grouped transform pass its direction, points, jobs, the plan's and the
+ * chosen sequences a workgroup, and each candidate's measured microseconds. The choice is
+ * a launch shape only: results are the same to the bit whichever is chosen. Timed at
+ * sp_scan_create, once per pass shape and process; SOLPROP_MI355X_AUTOTUNE=0 keeps the
+ * plans' own choice. SP_VALUE when `len` is too small. */
+int sp_scan_schedule(sp_scan* scan, char* out, size_t len);
 
 /* ------------------------------------------------------------------ sessions
  *

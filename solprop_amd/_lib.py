@@ -209,6 +209,7 @@ SIGNATURES = {
                              C.POINTER(_I), C.c_char_p, C.c_size_t]),
     "sp_scan_state": (_I, [C.c_void_p, _I, _PD, C.c_char_p, C.c_size_t]),
     "sp_scan_timing_of": (_I, [C.c_void_p, C.POINTER(ScanTimingC)]),
+    "sp_scan_schedule": (_I, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "sp_workspace_bind_monitor": (_I, [C.c_void_p, _PD, _PD, C.c_char_p, C.c_size_t]),
     "sp_workspace_monitor_reading": (_I, [C.c_void_p, _PD, C.c_char_p, C.c_size_t]),
     "sp_workspace_statistics": (_I, [C.c_void_p, C.c_ulonglong, _PD, _I, _PD, _I, C.POINTER(_I),

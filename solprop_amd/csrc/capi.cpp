@@ -280,6 +280,14 @@ int sp_scan_timing_of(sp_scan* scan, sp_scan_timing* out) {
   return SP_OK;
 }
 
+int sp_scan_schedule(sp_scan* scan, char* out, size_t len) {
+  if (!scan || !out) return SP_VALUE;
+  const std::string& s = scan->impl.schedule();
+  if (s.size() + 1 > len) return SP_VALUE;
+  std::memcpy(out, s.c_str(), s.size() + 1);
+  return SP_OK;
+}
+
 int sp_workspace_export(sp_workspace* ws, const double* power_weight, int optics,
                         double* out, char* msg, size_t msglen) {
   if (!ws || !out) return SP_VALUE;

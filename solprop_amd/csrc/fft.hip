@@ -484,6 +484,8 @@ void launch_fft_grouped(int log2L, int log2G, const FftJob* jobs, const int* fir
   SP_HIP(hipGetLastError());
 }
 
+bool fft_grouped_available(int log2L, int log2G) { return grouped_for(log2L, log2G) != nullptr; }
+
 void FftPlan::exec_packed(const PackedSpectrum& spectrum, cplx* data, cplx* scratch, int sign,
                           hipStream_t stream) const {
   SP_CHECK(batch_ >= 1 && spectrum.spectrum && spectrum.roots &&

@@ -121,6 +121,8 @@ class FftPlan {
 // owns workgroups first[i] .. first[i+1] - 1 (device array, first[n_jobs] = `blocks`).
 void launch_fft_grouped(int log2L, int log2G, const FftJob* jobs, const int* first, int n_jobs,
                         int blocks, int sign, hipStream_t stream);
+// Whether a grouped pass kernel exists for this shape.
+bool fft_grouped_available(int log2L, int log2G);
 
 // Host-computed unit roots exp(-2 pi i t / n), t < n, correctly rounded through
 // extended precision and the octant symmetries.

@@ -3,8 +3,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <utility>
 
 #include "sync.hpp"
@@ -219,6 +221,114 @@ void Scan::plan_launches() {
   }
   inverse_ = group_passes(inv, inv_plans);
   forward_ = group_passes(fwd, fwd_plans);
+  const char* env = std::getenv("SOLPROP_MI355X_AUTOTUNE");
+  if (!(env && env[0] == '0')) {
+    schedule_.clear();
+    tune(inverse_, +1, "inverse");
+    tune(forward_, -1, "forward");
+    schedule_ = "[" + schedule_ + "]";
+  }
+}
+
+// Re-plan a group's jobs at `log2G` sequences a workgroup.
+void Scan::set_schedule(FftGroup& g, int log2G) {
+  std::vector<FftJob> jobs = g.host;
+  std::vector<int> first;
+  int total = 0;
+  for (std::size_t i = 0; i < jobs.size(); ++i) {
+    jobs[i].groups = g.sequences[i] >> log2G;
+    first.push_back(total);
+    total += jobs[i].groups * jobs[i].rows;
+  }
+  first.push_back(total);
+  SP_HIP(hipMemcpy(g.jobs, jobs.data(), jobs.size() * sizeof(FftJob), hipMemcpyHostToDevice));
+  SP_HIP(hipMemcpy(g.first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+  g.log2G = log2G;
+  g.blocks = total;
+}
+
+namespace {
+std::mutex g_tuned_mu;
+std::map<std::string, int> g_tuned;  // pass shape -> sequences a workgroup (log2)
+}  // namespace
+
+void Scan::tune(std::vector<FftGroup>& groups, int sign, const char* direction) {
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  SP_HIP(hipEventCreate(&t0));
+  SP_HIP(hipEventCreate(&t1));
+  try {
+    for (std::size_t gi = 0; gi < groups.size(); ++gi) {
+      FftGroup& g = groups[gi];
+      // The key: direction, pass shape and every job's (sequences, rows).
+      std::vector<std::pair<int, int>> shape;
+      for (std::size_t i = 0; i < g.host.size(); ++i)
+        shape.emplace_back(g.sequences[i], g.host[i].rows);
+      std::sort(shape.begin(), shape.end());
+      std::string key = std::string(direction) + ":" + std::to_string(g.log2L);
+      for (const auto& sr : shape) key += ":" + std::to_string(sr.first) + "x" + std::to_string(sr.second);
+      const int planned = g.log2G;
+      std::string timings;
+      int best = planned;
+      bool cached = false;
+      {
+        std::lock_guard<std::mutex> lock(g_tuned_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+          best = it->second;
+          cached = true;
+        }
+      }
+      if (!cached) {
+        // The timed jobs run ungated (the rounds' gates are closed before any load).
+        std::vector<FftJob> open = g.host;
+        for (FftJob& j : open) j.gate = nullptr;
+        const std::vector<FftJob> saved = g.host;
+        g.host = open;
+        float best_ms = 0.0f;
+        for (int cand = 0; cand <= 4; ++cand) {
+          if ((g.log2L + cand) > 12) break;
+          bool fits = true;
+          for (int sq : g.sequences) fits = fits && sq % (1 << cand) == 0;
+          if (!fits || !fft_grouped_available(g.log2L, cand)) continue;
+          set_schedule(g, cand);
+          for (int r = 0; r < 2; ++r)
+            launch_fft_grouped(g.log2L, g.log2G, g.jobs, g.first, g.n, g.blocks, sign, stream_);
+          SP_HIP(hipEventRecord(t0, stream_));
+          constexpr int kReps = 5;
+          for (int r = 0; r < kReps; ++r)
+            launch_fft_grouped(g.log2L, g.log2G, g.jobs, g.first, g.n, g.blocks, sign, stream_);
+          SP_HIP(hipEventRecord(t1, stream_));
+          SP_HIP(hipEventSynchronize(t1));
+          float ms = 0.0f;
+          SP_HIP(hipEventElapsedTime(&ms, t0, t1));
+          ms /= kReps;
+          if (!timings.empty()) timings += ", ";
+          timings += "\"" + std::to_string(1 << cand) + "\": " + std::to_string(1e3 * ms);
+          if (best_ms == 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best = cand;
+          }
+        }
+        g.host = saved;
+        std::lock_guard<std::mutex> lock(g_tuned_mu);
+        g_tuned[key] = best;
+      }
+      set_schedule(g, best);
+      if (!schedule_.empty()) schedule_ += ", ";
+      schedule_ += "{\"direction\": \"" + std::string(direction) + "\", \"points\": " +
+                   std::to_string(1 << g.log2L) + ", \"jobs\": " + std::to_string(g.n) +
+                   ", \"planned\": " + std::to_string(1 << planned) + ", \"chosen\": " +
+                   std::to_string(1 << best) + ", \"cached\": " + (cached ? "true" : "false") +
+                   ", \"us\": {" + timings + "}}";
+    }
+  } catch (...) {
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    throw;
+  }
+  SP_HIP(hipStreamSynchronize(stream_));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
 }
 
 // The transforms' passes as grouped launches: every instance's first pass, one launch per
@@ -239,6 +349,8 @@ std::vector<Scan::FftGroup> Scan::group_passes(
       g.log2L = kv.first.first;
       g.log2G = kv.first.second;
       g.n = static_cast<int>(kv.second.size());
+      g.host = kv.second;
+      for (const FftJob& j : kv.second) g.sequences.push_back(j.groups << g.log2G);
       std::vector<int> first;
       int total = 0;
       for (const FftJob& j : kv.second) {

@@ -72,6 +72,9 @@ class Scan {
   void state(int i, double* out);
   FieldWorkspace& instance(int i) { return *inst_[static_cast<std::size_t>(i)]; }
   const ScanTiming& timing() const { return timing_; }
+  // The tuned schedule as JSON: per grouped pass, its shape, the choice and each
+  // candidate's measured microseconds.
+  const std::string& schedule() const { return schedule_; }
   long device_bytes() const;
 
  private:
@@ -84,9 +87,17 @@ class Scan {
     FftJob* jobs = nullptr;
     int* first = nullptr;
     int n = 0, blocks = 0;
+    std::vector<FftJob> host;    // the jobs as planned (at log2G)
+    std::vector<int> sequences;  // each job's sequences a row
   };
   void release();
   void plan_launches();
+  // The autotuner: for each grouped transform pass, the sequences a workgroup owns (a
+  // launch shape only: each sequence's arithmetic is the same whichever is chosen) timed
+  // over every candidate the pass kernels provide, the fastest kept; chosen once per pass
+  // shape and process (`SOLPROP_MI355X_AUTOTUNE=0`: the plans' own choice).
+  void tune(std::vector<FftGroup>& groups, int sign, const char* direction);
+  void set_schedule(FftGroup& g, int log2G);
   std::vector<FftGroup> group_passes(const std::vector<std::vector<FftJob>>& per_instance,
                                      const std::vector<const FftPlan*>& plans);
   void enqueue_round();
@@ -105,6 +116,7 @@ class Scan {
   hipEvent_t done_ = nullptr;
   std::unique_ptr<WorkerPool> pool_;
   ScanTiming timing_;
+  std::string schedule_ = "[]";
 };
 
 }  // namespace sp

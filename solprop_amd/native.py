@@ -527,6 +527,14 @@ class Scan:
         _check(self._lib.sp_scan_state(self._handle, int(i), _ptr(out), msg, 1024), msg)
         return out
 
+    def schedule(self) -> list:
+        """The autotuned launch schedule of the grouped transform passes."""
+        import json
+
+        buf = C.create_string_buffer(1 << 16)
+        _check(self._lib.sp_scan_schedule(self._handle, buf, len(buf)), b"schedule too long")
+        return json.loads(buf.value.decode())
+
     def timing(self) -> dict:
         t = _lib.ScanTimingC()
         _check(self._lib.sp_scan_timing_of(self._handle, C.byref(t)), b"")

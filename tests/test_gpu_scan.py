@@ -121,3 +121,34 @@ def test_scan_runs_instances_to_their_own_ends():
         assert abs(b.z - p.z_end) < 1e-15
         ref = alone(p, 400)
         assert np.array_equal(ref[1], end)
+
+
+def test_autotuned_schedule_changes_no_result(monkeypatch):
+    """The autotuner picks each grouped transform pass's sequences a workgroup by timing;
+    a launch shape only, so a scan tuned and one held to the plans' own shapes make the
+    same attempts and end states to the bit. The schedule names every pass it timed."""
+    print("autotuned scan vs untuned", flush=True)
+    problems = [capillary_problem(n_t=16384, n_modes=m, n_nodes=16, energy=e, length=0.01,
+                                  integration="trapezoid")
+                for m, e in [(1, 200e-6), (2, 275e-6), (3, 150e-6)]]
+    problems += [capillary_problem(n_t=8192, n_modes=2, n_nodes=16, energy=250e-6,
+                                   length=0.01, integration="trapezoid")]
+    results = []
+    for tuned in (True, False):
+        monkeypatch.setenv("SOLPROP_MI355X_AUTOTUNE", "1" if tuned else "0")
+        scan = Scan(problems)
+        schedule = scan.schedule()
+        scan.start(0.0)
+        batches = scan.advance([p.z_end for p in problems], work_limit=12)
+        results.append(([b.attempts.copy() for b in batches],
+                         [scan.state(i) for i in range(len(problems))], schedule))
+        scan.close()
+    (rows_a, states_a, schedule), (rows_b, states_b, untuned) = results
+    print("schedule", schedule, flush=True)
+    assert untuned == []
+    assert schedule and all(len(entry["us"]) >= 1 for entry in schedule if not entry["cached"])
+    assert any(entry["points"] in (64, 128) for entry in schedule)  # the four-step passes
+    for a, b in zip(rows_a, rows_b):
+        assert len(a) > 0 and np.array_equal(a, b)
+    for a, b in zip(states_a, states_b):
+        assert np.array_equal(a, b)
