@@ -428,6 +428,17 @@ def run_gpu(args):
         if args.raman_share > 0.0:
             config["workload"] += (f"; molecular Raman response taking {args.raman_share:g} "
                                    "of the third-order response")
+        if args.low_storage_steps > 0:
+            # The low-storage RK4IP replay of a uniform schedule instead of the adaptive
+            # march: four source evaluations a step, every step accepted.
+            import dataclasses
+
+            edges = np.linspace(0.0, problem.z_end, args.low_storage_steps + 1)
+            problem = dataclasses.replace(problem, method="low_storage", schedule=edges)
+            config["stepper"] = (f"low-storage RK4IP replay, {args.low_storage_steps} uniform "
+                                 "steps")
+            config["workload"] += (f"; low-storage RK4IP replay of {args.low_storage_steps} "
+                                   "uniform steps instead of the adaptive march")
     if args.copy_probe_mib > 0:
         # A plain device copy of known size, three times, for scaling memory counters.
         n = args.copy_probe_mib * (1 << 20) // 8
@@ -608,6 +619,9 @@ def main():
     parser.add_argument("--graph-warmup", type=int, default=40,
                         help="attempts of an untimed march from the launch field that "
                              "capture every trial graph before the warm-up steps")
+    parser.add_argument("--low-storage-steps", type=int, default=0,
+                        help="field workloads: replay this many uniform steps with the "
+                             "low-storage RK4IP march instead of the adaptive one")
     parser.add_argument("--copy-probe-mib", type=int, default=0,
                         help="run a plain device copy of this many MiB first (counter scaling)")
     parser.add_argument("--no-cpu", action="store_true")

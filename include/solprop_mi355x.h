@@ -116,7 +116,8 @@ typedef struct sp_problem_desc {
   double dz_init;                   /* the first proposal, m */
   /* Method: 0 the embedded Runge-Kutta march (envelope route), 1 the exponential Adams
    * march (field route), with the settings of Multistep
-   * (src/solprop/stepper/multistep.py:136-214). */
+   * (src/solprop/stepper/multistep.py:136-214), 2 the low-storage RK4IP replay of a fixed
+   * schedule (field route; `schedule_edges` below). */
   int method;
   int adams_order, adams_variable_order, adams_equal_approach;
   double adams_safety, adams_max_growth, adams_min_factor;
@@ -133,6 +134,12 @@ typedef struct sp_problem_desc {
    * product with the modal time field joins the in-stage absorber's term
    * (passes/stepping.py:286-372). NULL without a collar; dense field route only. */
   const double* spatial_absorber;
+  /* Method 2, the low-storage RK4IP replay (Problem.Stepper's low-storage march;
+   * native/include/solprop_native/march/low_storage.hpp, src/solprop/backends/numpy_ref/
+   * engines.py:236-520): the fixed schedule's edges, `n_schedule` steps (n_schedule + 1
+   * increasing positions, m), every step accepted. Field route only; 0 / NULL otherwise. */
+  int n_schedule;
+  const double* schedule_edges;
 } sp_problem_desc;
 
 typedef struct sp_workspace sp_workspace;
@@ -424,7 +431,8 @@ int sp_testing_source(sp_workspace* ws, double z, const double* lab, double* out
                       double* streams, double* modal, char* msg, size_t msglen);
 
 /* A named device buffer of the field route ("y", "lab", "source", "advanced", "flow",
- * "store", "moments", "states", "carried") copied into `out` (at most `count`
+ * "store", "moments", "states", "carried"; "stiffness", the low-storage replay's last
+ * step's indicator h rho, numpy_ref/engines.py:268-276) copied into `out` (at most `count`
  * doubles); `*size` receives the buffer's length in doubles. The reference's
  * `Engines.field` / `Engines.stage` readbacks (native/src/bindings/testing.cpp:601-602). */
 int sp_testing_buffer(sp_workspace* ws, const char* name, double* out, long count, long* size,

@@ -73,6 +73,7 @@ class ProblemDescC(C.Structure):
         ("adams_equal_approach", _I), ("adams_safety", _D), ("adams_max_growth", _D),
         ("adams_min_factor", _D),
         ("absorber_scheme", _I), ("absorber_rate", _PD), ("spatial_absorber", _PD),
+        ("n_schedule", _I), ("schedule_edges", _PD),
     ]
 
 

@@ -1346,6 +1346,94 @@ __global__ void __launch_bounds__(kNormThreads) batch_norms(SP_BATCH_ARGS) {
   norms_body(in.B.partials, in.correct_blocks, in.B.norms, tree);
 }
 
+// ------------------------------------------------- the low-storage RK4IP passes
+//
+// The four register passes of one RK4IP step (J. Hult, J. Lightwave Technol. 25, 3770
+// (2007), in the three-register form of the reference's low-storage engines:
+// src/solprop/backends/numpy_ref/engines.py:432-480), each after its source evaluation
+// S = B.source: A = B.y (the state), Bacc the accumulator, G the previous source (for the
+// stiffness indicator), C = B.lab the stage field the next source reads. R[0].x_w and
+// R[1].x_w are the weights of Phi(z_m) - Phi(z) and Phi(z + h) - Phi(z_m); the flows
+// P = exp(.) and Q = exp(.) are formed per element from the operator's terms. The passes
+// with a norm leave one masked sum per workgroup: pass 2 |S2 - g|^2, pass 3 |S3 - S2|^2,
+// pass 4 |A|^2 (the state's, after the hard mask).
+__device__ inline cplx ls_flow(const DeviceProblem& P, const double* w, long idx, long CN) {
+  cplx x = make(0.0, 0.0);
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t)
+    if (t < P.n_phi) x = add(x, scale(w[t], P.phi[t * CN + idx]));
+  return cexp(x);
+}
+
+template <int Pass>
+__global__ void __launch_bounds__(kLsThreads)
+    ls_pass(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B, cplx* acc,
+            cplx* prev, double* partials) {
+  __shared__ double tree[kLsThreads];
+  const long CN = static_cast<long>(P.C) * P.N;
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const double h = R[0].dz;
+  double sq = 0.0;
+  if (idx < CN) {
+    const cplx s = B.source[idx];
+    const bool scored = P.normmask[idx % P.N] != 0;
+    if constexpr (Pass == 1) {
+      const cplx flow = ls_flow(P, R[0].x_w, idx, CN);
+      const cplx a = mul(flow, B.y[idx]);
+      const cplx g = mul(flow, s);
+      B.y[idx] = a;
+      prev[idx] = g;
+      acc[idx] = add(a, scale(h / 6.0, g));
+      B.lab[idx] = add(a, scale(0.5 * h, g));
+    } else if constexpr (Pass == 2) {
+      const cplx d = sub(s, prev[idx]);
+      if (scored) sq = norm2(d);
+      acc[idx] = add(acc[idx], scale(h / 3.0, s));
+      B.lab[idx] = add(B.y[idx], scale(0.5 * h, s));
+      prev[idx] = s;
+    } else if constexpr (Pass == 3) {
+      const cplx d = sub(s, prev[idx]);
+      if (scored) sq = norm2(d);
+      acc[idx] = add(acc[idx], scale(h / 3.0, s));
+      const cplx flow = ls_flow(P, R[1].x_w, idx, CN);
+      B.lab[idx] = mul(flow, add(B.y[idx], scale(h, s)));
+    } else {
+      const cplx flow = ls_flow(P, R[1].x_w, idx, CN);
+      cplx a = add(mul(flow, acc[idx]), scale(h / 6.0, s));
+      if (!P.active[idx % P.N]) a = make(0.0, 0.0);
+      B.y[idx] = a;
+      if (scored) sq = norm2(a);
+    }
+  }
+  if constexpr (Pass > 1) {
+    tree[threadIdx.x] = sq;
+    __syncthreads();
+    for (int half = kLsThreads / 2; half > 0; half /= 2) {
+      if (threadIdx.x < half) tree[threadIdx.x] = tree[threadIdx.x] + tree[threadIdx.x + half];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[static_cast<long>(Pass - 2) * gridDim.x + blockIdx.x] = tree[0];
+  }
+}
+
+// The three sums from the passes' workgroup sums, in a fixed order, into `norms`.
+__global__ void __launch_bounds__(kLsThreads)
+    ls_norms(const double* __restrict__ partials, int blocks, double* norms) {
+  __shared__ double tree[kLsThreads];
+  for (int k = 0; k < 3; ++k) {
+    double total = 0.0;
+    for (int b = threadIdx.x; b < blocks; b += kLsThreads) total = total + partials[k * blocks + b];
+    tree[threadIdx.x] = total;
+    __syncthreads();
+    for (int half = kLsThreads / 2; half > 0; half /= 2) {
+      if (threadIdx.x < half) tree[threadIdx.x] = tree[threadIdx.x] + tree[threadIdx.x + half];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) norms[k] = tree[0];
+    __syncthreads();
+  }
+}
+
 // -------------------------------------------------------------------- launchers
 
 // One wave per workgroup while that leaves at most 4096 workgroups (small grids are
@@ -1609,5 +1697,28 @@ void launch_batch_response(const BatchGrid& g, hipStream_t s) {
 void launch_batch_finish(const BatchGrid& g, hipStream_t s) { grouped(batch_finish, g, kWave, s); }
 void launch_batch_correct(const BatchGrid& g, hipStream_t s) { grouped(batch_correct, g, kWave, s); }
 void launch_batch_norms(const BatchGrid& g, hipStream_t s) { grouped(batch_norms, g, kNormThreads, s); }
+
+unsigned ls_blocks(const DeviceProblem& P) {
+  return blocks_for(static_cast<std::size_t>(P.C) * P.N, kLsThreads);
+}
+
+void launch_ls_pass(int pass, const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                    cplx* acc, cplx* prev, double* partials, hipStream_t s) {
+  const dim3 grid(ls_blocks(P)), block(kLsThreads);
+  switch (pass) {
+    case 1: hipLaunchKernelGGL(ls_pass<1>, grid, block, 0, s, P, R, B, acc, prev, partials); break;
+    case 2: hipLaunchKernelGGL(ls_pass<2>, grid, block, 0, s, P, R, B, acc, prev, partials); break;
+    case 3: hipLaunchKernelGGL(ls_pass<3>, grid, block, 0, s, P, R, B, acc, prev, partials); break;
+    default: hipLaunchKernelGGL(ls_pass<4>, grid, block, 0, s, P, R, B, acc, prev, partials); break;
+  }
+  SP_HIP(hipGetLastError());
+}
+
+void launch_ls_norms(const DeviceProblem& P, const double* partials, double* norms,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(ls_norms, dim3(1), dim3(kLsThreads), 0, s, partials,
+                     static_cast<int>(ls_blocks(P)), norms);
+  SP_HIP(hipGetLastError());
+}
 
 }  // namespace sp

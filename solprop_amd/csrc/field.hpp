@@ -33,6 +33,7 @@ constexpr int kNodesPerWave = 4;   // nodes each wave of those passes runs
 constexpr int kLdsTableIntervals = 1024;  // rate-table intervals held in LDS
 constexpr int kResponseWaves = 16; // waves per workgroup of the node response
 constexpr int kCorrectThreads = 256;  // the corrector's widest workgroup
+constexpr int kLsThreads = 256;    // workgroup of the low-storage register passes
 
 // Scan tiles of a fine grid of T samples: ceil((T - 1) / 63).
 __host__ __device__ inline int scan_tiles(int T) { return (T - 1 + kTileSpan - 1) / kTileSpan; }
@@ -195,6 +196,13 @@ void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuff
 void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                           int width, hipStream_t s);
 void launch_adams_norms(const DeviceProblem& P, FieldBuffers B, hipStream_t s);
+// The low-storage RK4IP step's register passes (1..4, each after its source evaluation)
+// and the three masked sums they leave (|S2 - g|^2, |S3 - S2|^2, |A|^2) into `norms`.
+unsigned ls_blocks(const DeviceProblem& P);
+void launch_ls_pass(int pass, const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                    cplx* acc, cplx* prev, double* partials, hipStream_t s);
+void launch_ls_norms(const DeviceProblem& P, const double* partials, double* norms,
+                     hipStream_t s);
 // Commits the held trial into store slot `slot`; reads no record, so the host may go on
 // filling the next trial's record while it runs.
 void launch_adams_accept(const DeviceProblem& P, int slot, bool compensated, FieldBuffers B,

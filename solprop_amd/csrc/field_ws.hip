@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 
 namespace sp {
@@ -35,8 +36,17 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d, hipStream_t shared) {
     own_stream_ = false;
   }
   // Refusals first, before anything is allocated.
-  SP_CHECK(d.kind == 1 && d.method == 1,
-           "the field workspace runs the real field grid under the exponential Adams march");
+  SP_CHECK(d.kind == 1 && (d.method == 1 || d.method == 2),
+           "the field workspace runs the real field grid under the exponential Adams march "
+           "or the low-storage RK4IP replay");
+  if (d.method == 2) {
+    SP_CHECK(d.n_schedule >= 1 && d.schedule_edges, "the low-storage replay needs a schedule");
+    for (int i = 0; i < d.n_schedule; ++i)
+      SP_CHECK(d.schedule_edges[i + 1] > d.schedule_edges[i],
+               "the schedule's edges must increase");
+    SP_CHECK(d.absorber_scheme == 0 && !d.spatial_absorber,
+             "the low-storage replay is built without absorbers");
+  }
   const int C = d.n_modes, N = d.n_omega, T = d.n_to, Q = d.n_nodes;
   SP_CHECK(T >= 256 && (T & (T - 1)) == 0, "n_to must be a power of two >= 256");
   SP_CHECK(d.n_t >= 4 && T % d.n_t == 0, "n_to must be a multiple of n_t");
@@ -102,6 +112,11 @@ FieldWorkspace::FieldWorkspace(const sp_problem_desc& d, hipStream_t shared) {
   policy_.breakpoints = {0.0, d.z_end};
   dz_init_ = d.dz_init > 0.0 ? d.dz_init : 1e-4 * d.z_end;
 
+  low_storage_ = d.method == 2;
+  if (low_storage_) {
+    edges_.assign(d.schedule_edges, d.schedule_edges + d.n_schedule + 1);
+    for (int i = 0; i < d.n_schedule; ++i) steps_.push_back(edges_[i + 1] - edges_[i]);
+  }
   const char* env = std::getenv("SOLPROP_MI355X_GRAPH");
   use_graphs_ = !(env && env[0] == '0');
   try {
@@ -240,6 +255,16 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&own_rec_host_), sizeof(AdamsRecord)));
   rec_host_ = own_rec_host_;
   std::memset(rec_host_, 0, sizeof(AdamsRecord));
+  if (low_storage_) {
+    const std::size_t CN = static_cast<std::size_t>(C) * N;
+    ls_acc_ = alloc<cplx>(CN);
+    ls_prev_ = alloc<cplx>(CN);
+    ls_save_ = alloc<cplx>(CN);
+    ls_rec_dev_ = alloc<AdamsRecord>(3);
+    SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&ls_rec_host_), 3 * sizeof(AdamsRecord)));
+    std::memset(ls_rec_host_, 0, 3 * sizeof(AdamsRecord));
+    ls_partials_ = alloc<double>(3UL * ls_blocks(P_));
+  }
   for (hipEvent_t& e : marks_) SP_HIP(hipEventCreate(&e));
   SP_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
   slot_rates_.assign(S, std::vector<double>(kMaxTerms, 0.0));
@@ -248,7 +273,7 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
     fill_position(0.0);
     SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
                           stream_));
-    enqueue_raman_kernel();
+    enqueue_raman_kernel(rec_dev_);
     SP_HIP(hipStreamSynchronize(stream_));
   }
 }
@@ -258,6 +283,10 @@ FieldWorkspace::~FieldWorkspace() { release(); }
 void FieldWorkspace::release() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  if (ls_graph_) (void)hipGraphExecDestroy(ls_graph_);
+  ls_graph_ = nullptr;
+  if (ls_rec_host_) (void)hipHostFree(ls_rec_host_);
+  ls_rec_host_ = nullptr;
   for (void* p : allocations_) (void)hipFree(p);
   if (own_rec_host_) (void)hipHostFree(own_rec_host_);
   if (norms_host_) (void)hipHostFree(norms_host_);
@@ -321,17 +350,17 @@ void FieldWorkspace::enqueue_absorbers() {
 }
 
 // The Raman kernel's spectrum at the record's rates: H = rfft(h) dt on the doubled grid.
-void FieldWorkspace::enqueue_raman_kernel() {
-  launch_field_raman_kernel_fill(P_, rec_dev_, raman_kernel_z_, stream_);
+void FieldWorkspace::enqueue_raman_kernel(const AdamsRecord* R) {
+  launch_field_raman_kernel_fill(P_, R, raman_kernel_z_, stream_);
   raman_kernel_plan_.exec(raman_kernel_z_, raman_scratch_, -1, stream_);
   launch_raman_kernel_post(raman_kernel_z_, B_.raman_H, B_.roots_2T, P_.T, P_.dt_fine, stream_);
 }
 
 // The Raman convolution of every node's drive: E² on the doubled grid, transformed,
 // times the kernel's spectrum, transformed back (the node passes read it).
-void FieldWorkspace::enqueue_raman(const cplx* spectrum) {
+void FieldWorkspace::enqueue_raman(const cplx* spectrum, const AdamsRecord* R) {
   if (!P_.n_raman) return;
-  if (raman_dynamic_) enqueue_raman_kernel();
+  if (raman_dynamic_) enqueue_raman_kernel(R);
   if (P_.raman_analytic) {
     // The modes' Hilbert transforms (a real inverse transform of -i Y), synthesised at
     // the nodes by the drive pass, or by a GEMM on the dense route.
@@ -348,7 +377,8 @@ void FieldWorkspace::enqueue_raman(const cplx* spectrum) {
   raman_plan_.exec(B_.raman_drive, raman_scratch_, +1, stream_);
 }
 
-void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
+void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out, const AdamsRecord* R) {
+  const AdamsRecord* rec = R ? R : rec_dev_;
   mark(2);
   // The half spectrum is packed for the real inverse transform as its first pass loads it.
   inv_plan_.exec_packed(PackedSpectrum{spectrum, B_.roots_T, P_.N, P_.inv_T}, B_.modal_t,
@@ -356,16 +386,16 @@ void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out) {
   mark(3);
   enqueue_synthesis();
   mark(4);
-  enqueue_raman(spectrum);
+  enqueue_raman(spectrum, rec);
   mark(5);
-  launch_node_passes(P_, rec_dev_, pt_, B_, stream_, marking_ ? marks_ + 6 : nullptr);
+  launch_node_passes(P_, rec, pt_, B_, stream_, marking_ ? marks_ + 6 : nullptr);
   enqueue_projection();
   mark(11);
   enqueue_absorbers();
   mark(12);
   fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
   mark(13);
-  launch_field_finish(P_, rec_dev_, B_, out, stream_);
+  launch_field_finish(P_, rec, B_, out, stream_);
   mark(14);
 }
 
@@ -479,8 +509,8 @@ std::vector<std::pair<const char*, double>> FieldWorkspace::phases() const {
 
 // -------------------------------------------------------------- host scalars
 
-void FieldWorkspace::fill_position(double zp) {
-  AdamsRecord& r = *rec_host_;
+void FieldWorkspace::fill_position(double zp, AdamsRecord* into) {
+  AdamsRecord& r = into ? *into : *rec_host_;
   for (int t = 0; t < P_.n_pref; ++t) r.pref_w[t] = pref_paths_[t](zp);
   r.density = density_(zp);
   r.preionisation = preion_(zp);
@@ -610,6 +640,112 @@ void FieldWorkspace::accept(double dz, int free, const adams::Stencil&,
   accepted_ += 1;
 }
 
+// ------------------------------------------------ the low-storage RK4IP replay
+
+void FieldWorkspace::enqueue_ls_step() {
+  SP_HIP(hipMemcpyAsync(ls_rec_dev_, ls_rec_host_, 3 * sizeof(AdamsRecord),
+                        hipMemcpyHostToDevice, stream_));
+  // Pass 1 reads the source at the committed state, z; passes 2 and 3 the stage field at
+  // the midpoint; pass 4 at the end.
+  enqueue_source(B_.y, B_.source, ls_rec_dev_);
+  launch_ls_pass(1, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_);
+  enqueue_source(B_.lab, B_.source, ls_rec_dev_ + 1);
+  launch_ls_pass(2, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_);
+  enqueue_source(B_.lab, B_.source, ls_rec_dev_ + 1);
+  launch_ls_pass(3, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_);
+  enqueue_source(B_.lab, B_.source, ls_rec_dev_ + 2);
+  launch_ls_pass(4, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_);
+  launch_ls_norms(P_, ls_partials_, B_.norms, stream_);
+}
+
+double FieldWorkspace::ls_trial(double z, double dz) {
+  const double mid = z + 0.5 * dz, end = z + dz;
+  // The three positions' scalars (each evaluation reads its own position), and the
+  // flows' weights: Phi(mid) - Phi(z) in record 0, Phi(end) - Phi(mid) in record 1.
+  const double at[3] = {z, mid, end};
+  for (int i = 0; i < 3; ++i) fill_position(at[i], ls_rec_host_ + i);
+  ls_rec_host_[0].dz = dz;
+  for (int t = 0; t < P_.n_phi; ++t) {
+    ls_rec_host_[0].x_w[t] = path_increment(phi_paths_[t], z, mid);
+    ls_rec_host_[1].x_w[t] = path_increment(phi_paths_[t], mid, end);
+  }
+  marking_ = false;
+  if (use_graphs_) {
+    if (!ls_graph_) ls_graph_ = capture_graph(stream_, [&] { enqueue_ls_step(); });
+    SP_HIP(hipGraphLaunch(ls_graph_, stream_));
+    graph_launches_ += 1;
+  } else {
+    enqueue_ls_step();
+  }
+  wait();
+  trials_ += 1;
+  evaluations_ += 4;
+  const double lower = std::sqrt(norms_host_[0]), upper = std::sqrt(norms_host_[1]);
+  stiffness_ = lower > 0.0 ? 2.0 * upper / lower : std::nan("");
+  const double squared = norms_host_[2];
+  // The history's norm is "not estimated"; a non-finite state is the replay's divergence.
+  return std::isfinite(squared) ? 0.0 : squared;
+}
+
+void FieldWorkspace::ls_accept(double dz) {
+  // The hard mask was applied by the step's last pass; nothing else finishes a step here.
+  (void)dz;
+  accepted_ += 1;
+}
+
+namespace {
+struct LsEngine final : march::Engine {
+  std::function<double(double, double)> step;
+  std::function<void(double)> commit;
+  double trial(double z, double dz) override { return step(z, dz); }
+  void accept(double dz) override { commit(dz); }
+};
+
+struct LsHost final : march::Host {
+  AttemptLog& log;
+  explicit LsHost(AttemptLog& l) : log(l) {}
+  void record(double end, double dz, double norm, bool acc) override {
+    log.record(end, dz, norm, acc);
+  }
+  bool halt() override { return log.halt(); }
+};
+}  // namespace
+
+bool FieldWorkspace::ls_advance(const Bounds& b, int columns, std::vector<double>& rows,
+                                sp_march_result* r) {
+  LsEngine engine;
+  engine.step = [this](double z, double dz) { return ls_trial(z, dz); };
+  engine.commit = [this](double dz) { ls_accept(dz); };
+  AttemptLog log;
+  log.begin(&rows, columns, b);
+  log.order = 4;  // RK4IP
+  LsHost host(log);
+  // The replay stops at the target or at the schedule's end, whichever comes first.
+  const double target = std::min(b.target, edges_.back());
+  bool done = false;
+  try {
+    done = march::replay_to(engine, host, ls_state_, steps_, edges_, target) &&
+           march::reached(ls_state_.z, target);
+  } catch (...) {
+    z_ = ls_state_.z;
+    r->z = ls_state_.z;
+    r->attempts = log.attempts;
+    r->accepted = log.accepted;
+    r->evaluations = evaluations_;
+    throw;
+  }
+  z_ = ls_state_.z;
+  r->z = ls_state_.z;
+  r->proposal = ls_state_.cursor < static_cast<long>(steps_.size())
+                    ? steps_[static_cast<std::size_t>(ls_state_.cursor)]
+                    : 0.0;
+  r->done = done ? 1 : 0;
+  r->attempts = log.attempts;
+  r->accepted = log.accepted;
+  r->evaluations = evaluations_;
+  return done;
+}
+
 // -------------------------------------------------------------- the interface
 
 void FieldWorkspace::load(double z, const double* field) {
@@ -621,12 +757,33 @@ void FieldWorkspace::load(double z, const double* field) {
   SP_HIP(hipMemcpy(B_.y, field, CN * sizeof(cplx), hipMemcpyHostToDevice));
   adams::Engine& engine = *this;
   march_ = std::make_unique<adams::March>(policy_, engine, host_, z, dz_init_);
+  if (low_storage_) {
+    // A replay resumes at one of its schedule's edges.
+    long cursor = -1;
+    for (std::size_t i = 0; i < edges_.size(); ++i)
+      if (march::reached(z, edges_[i]) && march::reached(edges_[i], z))
+        cursor = static_cast<long>(i);
+    SP_CHECK(cursor >= 0, "a replay loads its state at one of its schedule's edges");
+    ls_state_ = march::State{};
+    ls_state_.z = edges_[static_cast<std::size_t>(cursor)];
+    ls_state_.cursor = cursor;
+  }
   z_ = z;
   loaded_ = true;
 }
 
 double FieldWorkspace::probe_trial(double z, double dz) {
   if (!loaded_) throw Error(kState, "load a state before taking a trial");
+  if (low_storage_) {
+    // One step from the committed state, which is kept: the step's end into `advanced`.
+    const std::size_t bytes = sizeof(cplx) * P_.C * P_.N;
+    SP_HIP(hipMemcpyAsync(ls_save_, B_.y, bytes, hipMemcpyDeviceToDevice, stream_));
+    const double norm = ls_trial(z, dz);
+    SP_HIP(hipMemcpyAsync(B_.advanced, B_.y, bytes, hipMemcpyDeviceToDevice, stream_));
+    SP_HIP(hipMemcpyAsync(B_.y, ls_save_, bytes, hipMemcpyDeviceToDevice, stream_));
+    SP_HIP(hipStreamSynchronize(stream_));
+    return norm;
+  }
   if (march_->history.empty()) {
     march_->z = z;
     start(z);
@@ -675,6 +832,7 @@ void FieldWorkspace::export_state(const double* weight, bool optics, double* out
 bool FieldWorkspace::advance_bounded(const Bounds& b, int columns, std::vector<double>& rows,
                                      sp_march_result* r) {
   if (!loaded_) throw Error(kState, "load a state before advancing");
+  if (low_storage_) return ls_advance(b, columns, rows, r);
   host_.log.begin(&rows, columns, b);
   bool done = false;
   try {
@@ -717,7 +875,7 @@ void FieldWorkspace::source(double z, const double* lab, double* out, double* st
   inv_plan_.exec_packed(PackedSpectrum{B_.lab, B_.roots_T, P_.N, P_.inv_T}, B_.modal_t,
                         scratch_, +1, stream_);
   enqueue_synthesis();
-  enqueue_raman(B_.lab);
+  enqueue_raman(B_.lab, rec_dev_);
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
   enqueue_projection();
   enqueue_absorbers();
@@ -733,6 +891,11 @@ void FieldWorkspace::source(double z, const double* lab, double* out, double* st
 
 long FieldWorkspace::buffer(const char* name, double* out, long count) {
   SP_CHECK(name != nullptr, "a buffer name is required");
+  if (std::strcmp(name, "stiffness") == 0) {
+    // The low-storage replay's last step's h rho (a host value).
+    if (out && count > 0) out[0] = stiffness_;
+    return 1;
+  }
   const long CN2 = 2L * P_.C * P_.N;
   const long S = policy_.order + 2;
   const struct {
@@ -829,6 +992,7 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
 
 std::string FieldWorkspace::batch_refusal() const {
   if (dense_) return "a batched march runs the modal route (at most 6 modes), not the dense one";
+  if (low_storage_) return "a batched march instance runs the exponential Adams march";
   if (P_.C > kMaxBatchModes) return "a batched march instance has at most 6 modes";
   if (P_.n_raman) return "a batched march instance has no Raman response";
   if (absorbing() || split_) return "a batched march instance has no absorber";

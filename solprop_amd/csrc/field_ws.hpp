@@ -88,13 +88,31 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   T* alloc(std::size_t n);
   template <typename T>
   T* upload(const T* host, std::size_t n);
-  void fill_position(double z);
-  void enqueue_source(const cplx* spectrum, cplx* out);
+  void fill_position(double z, AdamsRecord* into = nullptr);
+  // The source at a lab-frame spectrum into `out`, its position's scalars read from `R`
+  // (the workspace's record when null).
+  void enqueue_source(const cplx* spectrum, cplx* out, const AdamsRecord* R = nullptr);
   void enqueue_synthesis();
   void enqueue_projection();
-  void enqueue_raman_kernel();
+  void enqueue_raman_kernel(const AdamsRecord* R);
   void enqueue_split(const AdamsRecord* R, double dz);
-  void enqueue_raman(const cplx* spectrum);
+  void enqueue_raman(const cplx* spectrum, const AdamsRecord* R);
+  // ---- the low-storage RK4IP replay (method 2): a fixed schedule, every step accepted.
+  // One step is one captured graph: the three positions' records up, then the source at
+  // z, m, m and z + h, each followed by its register pass, and the passes' masked sums.
+  void enqueue_ls_step();
+  double ls_trial(double z, double dz);  // 0.0, or the state's non-finite squared norm
+  void ls_accept(double dz);
+  bool ls_advance(const Bounds& b, int columns, std::vector<double>& rows, sp_march_result* r);
+  bool low_storage_ = false;
+  std::vector<double> edges_, steps_;
+  cplx *ls_acc_ = nullptr, *ls_prev_ = nullptr, *ls_save_ = nullptr;
+  AdamsRecord* ls_rec_dev_ = nullptr;   // (3): z, the midpoint, z + h
+  AdamsRecord* ls_rec_host_ = nullptr;  // (3) pinned
+  double* ls_partials_ = nullptr;       // (3, blocks)
+  hipGraphExec_t ls_graph_ = nullptr;
+  march::State ls_state_;
+  double stiffness_ = 0.0;              // h rho of the last step (NaN when undefined)
   enum Graph { kTrial = 1, kStart = 2, kCommitTrial = 3 };
   void enqueue_trial(bool commit);
   void enqueue_start();

@@ -201,7 +201,11 @@ def lower(problem: Problem, settings: Settings | None = None) -> tuple[_lib.Prob
     d.dz_max = problem.z_end if s.dz_max is None else s.dz_max
     d.stretch = s.stretch
     d.dz_init = 1e-4 * problem.z_end if s.dz_init is None else s.dz_init
-    d.method = 1 if problem.method == "adams" else 0
+    d.method = {"rk": 0, "adams": 1, "low_storage": 2}[problem.method]
+    if problem.schedule is not None:
+        edges = np.ascontiguousarray(problem.schedule, dtype=np.float64)
+        d.n_schedule = len(edges) - 1
+        d.schedule_edges = k.f64(edges)
     d.adams_order = problem.adams_order
     d.adams_variable_order = int(problem.adams_variable_order)
     d.adams_equal_approach = 1
@@ -413,6 +417,8 @@ class Workspace:
             return out.reshape(self.problem.n_nodes, self.problem.n_to)
         if name in ("tile_x", "tile_f"):
             return out.reshape(self.problem.n_nodes, -1)
+        if name == "stiffness":
+            return out
         return out.view(np.complex128).reshape(-1, *self._shape)
 
     def phases(self) -> dict:

@@ -231,7 +231,10 @@ class Problem:
     noise_model: str = "none"  # "none", "input" or "modified"
     rtol: float = 1e-6
     atol: float = 0.0
-    method: str = "rk"  # "rk" (embedded Runge-Kutta) or "adams" (exponential Adams)
+    # "rk" (embedded Runge-Kutta), "adams" (exponential Adams) or "low_storage" (the
+    # RK4IP replay of the fixed schedule `schedule`, its edges in m; field route)
+    method: str = "rk"
+    schedule: np.ndarray | None = None
     adams_order: int = 5
     adams_variable_order: bool = True
     # The temporal absorber (``Problem.Absorber``): "none", "in_stage" (the rate on the fine
