@@ -406,6 +406,181 @@ def run_surrogate(args):
         extra, None
 
 
+def run_fno(args):
+    """The surrogate network's training step and inference alone (the full shape: 4
+    layers, width 64, 64 modes, 2048 bins, batch 64) on synthetic rows: a step is one
+    Adam step over a batch; beside it the forward rate in samples per second. The FLOP
+    count is the network's dense products (the BF16 matrix cores) per sample."""
+    from oracle import fno as fno_oracle  # initial parameters only (a seeded draw)
+    from solprop_amd import surrogate as sg
+
+    shape = sg.Shape()
+    batch = 64
+    rng = np.random.default_rng(args.seed)
+    rows = 4096
+    spectra = rng.uniform(0, 1, (rows, shape.n)).astype(np.float32)
+    params = rng.uniform(0, 1, (rows, shape.n_params)).astype(np.float32)
+    targets = rng.uniform(-0.1, 0.1, (rows, shape.n)).astype(np.float32)
+    net = sg.Network(shape, max_batch=batch)
+    o = fno_oracle.Shape(in_ch=shape.in_ch, width=shape.width, modes=shape.modes,
+                         layers=shape.layers, hidden=shape.hidden, n=shape.n)
+    net.set_parameters(fno_oracle.init(o, seed=args.seed))
+    net.load_dataset(spectra, params, targets)
+    per_epoch = rows // batch
+    warm = max(1, args.warmup)
+    net.train(epochs=max(1, (warm + per_epoch - 1) // per_epoch), batch=batch, seed=1)
+    epochs = max(1, (args.steps + per_epoch - 1) // per_epoch)
+    losses, ms = net.train(epochs=epochs, batch=batch, seed=2)
+    steps = len(losses)
+    t0 = time.perf_counter()
+    net.forward(spectra, params)
+    infer_s = time.perf_counter() - t0
+    net.close()
+    W, M, n, H, L = shape.width, shape.modes, shape.n, shape.hidden, shape.layers
+    # Dense products a sample, forward: per layer the truncated transform and its inverse
+    # (W x n x 2M each) and the 1x1 convolution (W x W x n); the lift's head (H x W x n).
+    forward = 2.0 * (L * (2 * W * n * 2 * M + W * W * n) + H * W * n)
+    # Training: the forward, then per product its two gradients.
+    train = 3.0 * forward
+    value = steps * batch / (ms * 1e-3)
+    achieved = train * batch * steps / (ms * 1e-3) / 1e12
+    config = {"workload": "surrogate network training: 1-D Fourier neural operator, 4 layers, "
+                          "width 64, 64 modes, 2048 bins, batch 64, Adam, BF16 matrix-core "
+                          "products with FP32 accumulation, synthetic rows",
+              "batch": batch, "parallelism": "single GPU"}
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
+                "frac": achieved / 2500.0, "traffic": None, "kernel": "training step",
+                "flops_per_sample": train,
+                "timing": "device time of the training steps (events around the loop)"}
+    extra = {"metric": "surrogate training samples/s", "unit": "samples/s",
+             "inference_samples_per_s": rows / infer_s,
+             "inference_tflops": forward * rows / infer_s / 1e12, "train_steps": steps,
+             "dtype_note": "bf16 products, fp32 accumulation and optimizer"}
+    return value, ms / steps, steps, config, roofline, extra, None
+
+
+# Algorithmic bytes a launch moves, per point of the volume: a transform pass reads and
+# writes each complex point once (32); the t rows' forward pass also reads the prefactor
+# (48); the register passes read and write the low-storage registers (pass 1: y, source,
+# operator in, y, g, B, C out; 2: source, g, B, y in, B, C, g out; 3: source, g, B,
+# operator, y in, B, C out; 4: operator, B, source in, y out).
+CARTESIAN_PHASE_BYTES = {
+    "record upload": 0.0, "t inverse": 32.0, "x pass": 32.0, "y response": 32.0,
+    "t forward": 48.0, "ls pass 1": 112.0, "ls pass 2": 112.0, "ls pass 3": 112.0,
+    "ls pass 4": 64.0, "norms": 0.0,
+}
+CARTESIAN_LAUNCHES = {"t inverse": 4, "x pass": 8, "y response": 4, "t forward": 4,
+                      "ls pass 1": 1, "ls pass 2": 1, "ls pass 3": 1, "ls pass 4": 1}
+
+
+def run_cartesian(args):
+    """The free-space Kerr beam on a Cartesian (ny, nx, nt) grid, fixed RK4IP steps of
+    0.5 um (the reference's ``kerr_beam_*`` cases, benchmarks/registry.py:348-379, on
+    power-of-two axes): a step is one low-storage step of the whole volume, four source
+    evaluations of two rank-3 transforms each. The rate is point-steps per second (the
+    volume's samples times the steps)."""
+    import torch
+
+    from solprop_amd.native import Workspace
+    from solprop_amd.problem import cartesian_problem
+
+    nx, ny, nt = (int(v) for v in args.cartesian_grid.split(","))
+    total = 1 + args.warmup + args.steps + args.profile_steps
+    t_build = time.perf_counter()
+    problem = cartesian_problem(nx=nx, ny=ny, n_t=nt, spacing=1.05e-6, steps=total,
+                                length=0.5e-6 * total)
+    build_s = time.perf_counter() - t_build
+    volume = nx * ny * nt
+    ws = Workspace(problem)
+    march = ws.march(0.0, problem.field0)
+    edges = problem.schedule
+
+    def take(count, start):
+        batch = march.advance(float(edges[start + count]), work_limit=count)
+        assert len(batch.attempts) == count
+        return start + count
+
+    at = take(1 + args.warmup, 0)  # the first step captures the step's graph
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    at = take(args.steps, at)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ws.profile(True)
+    if args.profile_steps > 0:
+        at = take(args.profile_steps, at)
+    torch.cuda.synchronize()
+    trial_us, trials = ws.timing()
+    phases = ws.phases()
+    device_bytes = ws.counters()["device_bytes"]
+    ws.close()
+    value = volume * args.steps / elapsed
+    kernels = {}
+    for name, us in phases.items():
+        per = CARTESIAN_PHASE_BYTES.get(name, 0.0) * volume
+        kernels[name] = {"avg_us": us, "bytes": per,
+                         "GB/s": per / (us * 1e-6) / 1e9 if us > 0 and per > 0 else None}
+    # The dominant kernel by its share of the step (launches a step times duration).
+    per_step = CARTESIAN_LAUNCHES
+    shares = {k: phases[k] * n for k, n in per_step.items() if k in phases}
+    roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None, "traffic": None, "kernel": None}
+    if shares:
+        dominant = max(shares, key=shares.get)
+        achieved = kernels[dominant]["GB/s"]
+        roofline = {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("cartesian", dominant),
+            "kernel": dominant, "bytes_per_launch": kernels[dominant]["bytes"],
+            "avg_launch_us": phases[dominant], "trial_us": trial_us, "kernels": kernels,
+            "profiled_trials": trials,
+            "step_bytes": sum(CARTESIAN_PHASE_BYTES[k] * n for k, n in per_step.items())
+            * volume,
+            "timing": "HIP events between consecutive launches on the workspace stream, "
+                      "over the profiled steps that follow the timed region",
+        }
+    config = {"workload": f"free-space Kerr beam in fused silica on a Cartesian "
+                          f"{ny} x {nx} x {nt} envelope grid (y, x, t), 260 nm 30 fs 0.1 uJ "
+                          "Gaussian of 40 um waist, 1.05 um transverse spacing, 250 fs window, "
+                          "fixed low-storage RK4IP steps of 0.5 um",
+              "grid": [ny, nx, nt], "points": volume, "parallelism": "single GPU"}
+    extra = {"metric": "Cartesian point-steps/s", "unit": "point-steps/s",
+             "device_bytes": device_bytes, "host_build_s": build_s,
+             "reference_published": {
+                 "ms_per_step": 208.0, "grid": [768, 768, 600], "device": "AMD Instinct MI325X",
+                 "point_steps_per_s": 768 * 768 * 600 / 0.208,
+                 "source": "docs/dev/benchmarks.md:533-535 (a low-storage step at 768 x 768 "
+                           "x 600, after the 2026-11-05 changes)"}}
+    return value, elapsed * 1e3 / args.steps, args.steps, config, roofline, extra, problem
+
+
+def cartesian_cpu_baseline(seconds: float = 12.0):
+    """The oracle's NumPy replay of the same route on a 128 x 128 x 256 sample of the
+    workload (one thread), steps until about `seconds` have passed."""
+    from oracle import cartesian as cart
+    from oracle import low_storage as ls
+    from solprop_amd.problem import cartesian_problem
+
+    problem = cartesian_problem(nx=128, ny=128, n_t=256, spacing=1.05e-6 * 4, steps=64,
+                                length=0.5e-6 * 64)
+    state = problem.field0.copy()
+    norm_mask = problem.norm_mask.astype(bool)
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < seconds and steps < 64:
+        z = problem.schedule[steps]
+        dz = problem.schedule[steps + 1] - z
+        state, _, _ = ls.step(state, z, dz, lambda u, _z: cart.source(problem, u),
+                              lambda a, b: cart.flows(problem, a, b), norm_mask)
+        steps += 1
+    elapsed = time.perf_counter() - t0
+    points = 128 * 128 * 256
+    return {"value": points * steps / elapsed, "unit": "point-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"the oracle's NumPy replay (numpy.fft, one thread) of {steps} steps on "
+                      f"a 128 x 128 x 256 grid of the same beam, {elapsed:.1f} s"}
+
+
 def run_gpu(args):
     import torch
 
@@ -596,7 +771,8 @@ def main():
     parser.add_argument("--steps", type=int, default=300)
     parser.add_argument("--warmup", type=int, default=30)
     parser.add_argument("--workload",
-                        choices=["capillary", "radial", "envelope", "scan", "surrogate"],
+                        choices=["capillary", "radial", "envelope", "scan", "surrogate", "fno",
+                                 "cartesian"],
                         default="capillary")
     parser.add_argument("--train-runs", type=int, default=2048,
                         help="surrogate workload: solver runs generating training spectra")
@@ -615,6 +791,8 @@ def main():
     parser.add_argument("--to-completion", action="store_true",
                         help="scan workload: then march every instance to its end "
                              "(runs per hour)")
+    parser.add_argument("--cartesian-grid", default="512,512,1024",
+                        help="cartesian workload: nx,ny,nt (powers of two, at most 4096)")
     parser.add_argument("--profile-steps", type=int, default=100)
     parser.add_argument("--graph-warmup", type=int, default=40,
                         help="attempts of an untimed march from the launch field that "
@@ -636,10 +814,15 @@ def main():
         print("bench.py runs one GPU: one propagation problem runs on one device, and this "
               "build measures the single-GPU path only (--gpus 1).", file=sys.stderr)
         sys.exit(2)
-    runner = {"scan": run_scan, "surrogate": run_surrogate}.get(args.workload, run_gpu)
+    runner = {"scan": run_scan, "surrogate": run_surrogate, "fno": run_fno,
+              "cartesian": run_cartesian}.get(args.workload, run_gpu)
     value, ms, steps, config, roofline, extra, problem = runner(args)
-    baseline = (None if args.no_cpu or args.workload in ("scan", "surrogate")
-                else cpu_baseline(problem))
+    if args.no_cpu or args.workload in ("scan", "surrogate", "fno"):
+        baseline = None
+    elif args.workload == "cartesian":
+        baseline = cartesian_cpu_baseline()
+    else:
+        baseline = cpu_baseline(problem)
     line = {
         "metric": extra.pop("metric", "propagation steps/s per GPU"),
         "value": value,

@@ -54,7 +54,8 @@ typedef struct sp_scalar_path {
  * T = n_to, Q = n_nodes. */
 typedef struct sp_problem_desc {
   /* Dims (_core.pyi Problem.Dims) */
-  int kind; /* 0 envelope (complex time samples), 1 field (real) */
+  int kind; /* 0 envelope (complex time samples), 1 field (real), 2 the free-space
+             * Cartesian envelope on an (nx, ny, nt) grid (grid_nx, grid_ny below) */
   int n_t, n_to, n_modes, n_nodes, n_omega;
   /* Grid (Problem.Grid) */
   const int64_t* crop_index;        /* (N) */
@@ -137,9 +138,17 @@ typedef struct sp_problem_desc {
   /* Method 2, the low-storage RK4IP replay (Problem.Stepper's low-storage march;
    * native/include/solprop_native/march/low_storage.hpp, src/solprop/backends/numpy_ref/
    * engines.py:236-520): the fixed schedule's edges, `n_schedule` steps (n_schedule + 1
-   * increasing positions, m), every step accepted. Field route only; 0 / NULL otherwise. */
+   * increasing positions, m), every step accepted. Field and Cartesian routes; 0 / NULL
+   * otherwise. */
   int n_schedule;
   const double* schedule_edges;
+  /* Kind 2, the free-space Cartesian route (the reference's `prop_free` geometry,
+   * docs/dev/benchmarks.md:529-537): the transverse grid, nx rows of ny points, each
+   * point a column of n_t = n_to = n_omega time samples, so n_modes = nx ny. The state
+   * is the unnormalised 3-D spectrum Y(kx, ky, w), w fastest; one operator table
+   * (n_phi = 1, W(z) = z), one prefactor table, the analytic Kerr response, method 2.
+   * 0 otherwise. */
+  int grid_nx, grid_ny;
 } sp_problem_desc;
 
 typedef struct sp_workspace sp_workspace;
@@ -379,6 +388,30 @@ int sp_workspace_export(sp_workspace* ws, const double* power_weight, int optics
 int sp_workspace_statistics(sp_workspace* ws, unsigned long long flags, const double* windows,
                             int n_windows, double* out, int capacity, int* count, char* msg,
                             size_t msglen);
+
+/* The statistics with every input the native workspace's `statistics` takes for them
+ * (GeneralWorkspace::statistics, native/src/general/statistics.cpp:106-360): the flags
+ * and windows above, and for peak_intensity (1 << 9) the on-axis synthesis,
+ * `on_axis_rows` (1 or 2) rows of n_modes, with the intensity factor: the largest over
+ * the coarse samples of sum_rows |sum_m on_axis[row, m] a_m(t)|^2, times the factor
+ * (statistics.cpp:517-603; a_m the modes' analytic signals of the peak powers). On the
+ * field routes populations (1 << 11) too: every quadrature node's end-of-window ionised
+ * fraction and largest ionisation rate, (n_nodes, 2), one channel
+ * (statistics.cpp:636-650). Packed in the native order: ..., electron_density,
+ * peak_ionisation_rate, peak_intensity (1), the fwhm trace (n_t), populations
+ * (2 n_nodes). quadrature_error (1 << 10) and the plasma integration diagnostics
+ * (1 << 12) are refused with SP_VALUE. */
+typedef struct sp_statistics_request {
+  unsigned long long flags;
+  const double* windows;
+  int n_windows;
+  const double* on_axis;
+  int on_axis_rows;
+  double intensity_factor;
+} sp_statistics_request;
+int sp_workspace_statistics_request(sp_workspace* ws, const sp_statistics_request* request,
+                                    double* out, int capacity, int* count, char* msg,
+                                    size_t msglen);
 
 /* The boundary monitor (`_bind_monitor`, src/solprop/monitor.py:440-606; the native
  * workspace's monitor reading, native/src/general/statistics.cpp and

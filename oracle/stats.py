@@ -14,14 +14,19 @@ of shape ``(n_modes, n_omega)`` (one polarisation), unit modal power weight:
   and ``peak_power_total`` (``:479-520``) over ``analytic_field`` (``:148-178``): the
   coarse inverse transform, then on a field grid ``fourier.analytic_signal``.
 
-- :func:`ionisation` restates the plasma pair (``numpy_ref/statistics.py:502-546``).
+- :func:`ionisation` restates the plasma pair (``numpy_ref/statistics.py:502-546``);
+  :func:`populations` every node's pair of the same scan (``native/src/general/
+  statistics.cpp:636-650``); :func:`peak_intensity` the on-axis synthesis's peak
+  (``statistics.cpp:517-603``).
 - :func:`monitor_energies` and :func:`split_sums` restate the boundary monitor's
   reductions (``src/solprop/monitor.py:493-606``).
 
 Pins (``tests/test_oracle.py``): Parseval's identity with the analytic signal's time
 sum, the window tiling summing to the total, the centroid of a single bin, a sampled
 envelope's peak, and an empty column's NaN; the monitor's total against the reference's
-docstring identity, and its removal against the split collar's own factor.
+docstring identity, and its removal against the split collar's own factor; the peak
+intensity of a one-hot row against the peak power of that mode and of a row of ones
+against the coherent sum's peak; the populations' densest row against the plasma pair.
 """
 
 from __future__ import annotations
@@ -115,11 +120,19 @@ def split_sums(field, weights, step, kind, columns=None) -> np.ndarray:
     return np.concatenate([[taken, half.sum()], weights[:3] @ half])
 
 
-def ionisation(problem, state, z):
-    """The electron density and the peak ionisation rate at a plane
-    (``src/solprop/backends/numpy_ref/statistics.py:502-546``): the plasma scan of the
-    node fields, the node of largest fraction at the window's end (the first), density
-    times that fraction, and that node's largest rate."""
+def peak_intensity(field: np.ndarray, kind: str, on_axis, factor: float) -> float:
+    """``peak_intensity`` (the native backend's ``statistics_intensity``,
+    ``native/src/general/statistics.cpp:517-603``): the on-axis synthesis rows applied to
+    the modes' analytic signals, the largest over the samples of the rows' summed squared
+    moduli, times the intensity factor."""
+    rows = np.atleast_2d(np.asarray(on_axis, dtype=np.float64))
+    values = rows @ analytic_field(field, kind)
+    return float(factor * np.max(np.sum(np.abs(values) ** 2, axis=0)))
+
+
+def plasma_column(problem, state, z):
+    """The plasma scan of the node fields at a plane: the ionised fraction and the rate,
+    each (n_nodes, n_to)."""
     from oracle import rk
 
     src = rk.StageSource(problem)
@@ -129,9 +142,26 @@ def ionisation(problem, state, z):
     rate = P.ionisation_rate(nodes, src.plasma.rate)
     fraction, _ = P.plasma_scan(nodes, rate, src.plasma, problem.preionisation(z),
                                 problem.dt_fine)
+    return fraction, rate
+
+
+def populations(problem, state, z) -> np.ndarray:
+    """``populations`` (``statistics.cpp:636-650``, one channel): each node's ionised
+    fraction at the window's end and its largest rate, (n_nodes, 2)."""
+    fraction, rate = plasma_column(problem, state, z)
+    return np.stack([fraction[:, -1], rate.max(axis=1)], axis=1)
+
+
+def ionisation(problem, state, z):
+    """The electron density and the peak ionisation rate at a plane
+    (``src/solprop/backends/numpy_ref/statistics.py:502-546``): the plasma scan of the
+    node fields, the node of largest fraction at the window's end (the first), density
+    times that fraction, and that node's largest rate."""
+    fraction, rate = plasma_column(problem, state, z)
     densest = int(np.argmax(fraction[:, -1]))
     return float(problem.density(z)) * fraction[densest, -1], float(np.max(rate[densest]))
 
 
 __all__ = ["analytic_field", "energy", "energy_window", "monitor_energies", "monitor_power",
-           "ionisation", "omega0", "peak_power", "peak_power_total", "peak_power_window", "power_trace", "split_sums"]
+           "ionisation", "omega0", "peak_intensity", "peak_power", "peak_power_total",
+           "peak_power_window", "plasma_column", "populations", "power_trace", "split_sums"]

@@ -74,6 +74,7 @@ class ProblemDescC(C.Structure):
         ("adams_min_factor", _D),
         ("absorber_scheme", _I), ("absorber_rate", _PD), ("spatial_absorber", _PD),
         ("n_schedule", _I), ("schedule_edges", _PD),
+        ("grid_nx", _I), ("grid_ny", _I),
     ]
 
 
@@ -137,6 +138,11 @@ class ScanTimingC(C.Structure):
     _fields_ = [
         ("rounds", C.c_long), ("device_ms", _D), ("host_ms", _D), ("device_bytes", C.c_long),
     ]
+
+
+class StatisticsRequestC(C.Structure):
+    _fields_ = [("flags", C.c_ulonglong), ("windows", _PD), ("n_windows", _I),
+                ("on_axis", _PD), ("on_axis_rows", _I), ("intensity_factor", _D)]
 
 
 class SessionPlanC(C.Structure):
@@ -215,6 +221,8 @@ SIGNATURES = {
     "sp_workspace_monitor_reading": (_I, [C.c_void_p, _PD, C.c_char_p, C.c_size_t]),
     "sp_workspace_statistics": (_I, [C.c_void_p, C.c_ulonglong, _PD, _I, _PD, _I, C.POINTER(_I),
                                      C.c_char_p, C.c_size_t]),
+    "sp_workspace_statistics_request": (_I, [C.c_void_p, C.POINTER(StatisticsRequestC), _PD, _I,
+                                             C.POINTER(_I), C.c_char_p, C.c_size_t]),
     "sp_rk_trace": (_I, [C.POINTER(RKScenarioC), C.c_char_p, C.c_size_t]),
     "sp_adams_trace": (_I, [C.POINTER(AdamsScenarioC), C.c_char_p, C.c_size_t]),
     "sp_testing_fft": (_I, [_PD, _PD, C.c_long, C.c_long, _I, C.c_char_p, C.c_size_t]),

@@ -22,7 +22,7 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["fft.hip", "gemm.hip", "stage.hip", "field.hip", "workspace.hip", "field_ws.hip",
                "stats.hip", "scan.hip", "fft_noalloc.hip",
-               "fno.hip"]
+               "fno.hip", "cartesian.hip"]
 HOST_SOURCES = ["capi.cpp", "session.cpp"]
 STRICT_SOURCES = ["march.cpp", "trace.cpp", "scalars.cpp", "adams_march.cpp",
                   "adams_trace.cpp"]

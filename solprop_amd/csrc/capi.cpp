@@ -10,6 +10,7 @@
 #include <thread>
 #include <vector>
 
+#include "cartesian.hpp"
 #include "field_ws.hpp"
 #include "scan.hpp"
 #include "session.hpp"
@@ -20,9 +21,10 @@
 
 namespace {
 // The envelope route runs the embedded Runge-Kutta march; the modal field route the
-// exponential Adams march.
+// exponential Adams march (or the low-storage replay); the Cartesian route the replay.
 std::unique_ptr<sp::Propagator> make_propagator(const sp_problem_desc& d) {
   if (d.kind == 1) return std::make_unique<sp::FieldWorkspace>(d);
+  if (d.kind == 2) return std::make_unique<sp::CartesianWorkspace>(d);
   return std::make_unique<sp::Workspace>(d);
 }
 }  // namespace
@@ -299,7 +301,24 @@ int sp_workspace_statistics(sp_workspace* ws, unsigned long long flags, const do
                             size_t msglen) {
   if (!ws || !count || (capacity > 0 && !out)) return SP_VALUE;
   return guarded(msg, msglen, [&] {
-    const auto values = ws->impl.statistics(flags, windows, n_windows);
+    const auto values = ws->impl.statistics(flags, windows, n_windows, sp::StatExtras{});
+    SP_CHECK(static_cast<long>(values.size()) <= capacity, "the statistics output is too small");
+    std::copy(values.begin(), values.end(), out);
+    *count = static_cast<int>(values.size());
+  });
+}
+
+int sp_workspace_statistics_request(sp_workspace* ws, const sp_statistics_request* request,
+                                    double* out, int capacity, int* count, char* msg,
+                                    size_t msglen) {
+  if (!ws || !request || !count || (capacity > 0 && !out)) return SP_VALUE;
+  return guarded(msg, msglen, [&] {
+    sp::StatExtras extras;
+    extras.on_axis = request->on_axis;
+    extras.rows = request->on_axis_rows;
+    extras.intensity_factor = request->intensity_factor;
+    const auto values =
+        ws->impl.statistics(request->flags, request->windows, request->n_windows, extras);
     SP_CHECK(static_cast<long>(values.size()) <= capacity, "the statistics output is too small");
     std::copy(values.begin(), values.end(), out);
     *count = static_cast<int>(values.size());

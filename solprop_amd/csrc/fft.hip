@@ -1,12 +1,12 @@
 // Stockham radix-4/2 FFT passes staged in LDS, FP64, for gfx950.
 #include "fft.hpp"
+#include "fft_lds.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace sp {
 namespace {
-
-constexpr int kMaxElems = 4096;  // complex doubles a workgroup holds in LDS (64 KiB)
 
 using PassArgs = FftPassArgs;
 
@@ -23,21 +23,6 @@ __device__ inline cplx packed_point(const PassArgs& a, long g) {
     xb.y = 0.0;
   }
   return c2r_pre(xa, xb, a.spec_roots[k]);
-}
-
-template <int LOG2L, int G>
-struct Shape {
-  static constexpr int L = 1 << LOG2L;
-  static constexpr int E = L * G;
-  static constexpr int NT = (E / 4 >= 256) ? 256 : (E / 4 >= 1 ? E / 4 : 1);
-  static constexpr int STRIDE = L + (G > 1 ? 1 : 0);  // pad rows against bank conflicts
-  static constexpr int PER4 = (E / 4 + NT - 1) / NT;
-  static constexpr int PER2 = (E / 2 + NT - 1) / NT;
-};
-
-__device__ inline cplx twiddle(const cplx* table, int index, int sign) {
-  const cplx w = table[index];
-  return sign > 0 ? make(w.x, -w.y) : w;
 }
 
 // One workgroup's pass: sequence group `group` of batch row `row`, staged in `buf`
@@ -69,75 +54,7 @@ __device__ __forceinline__ void fft_pass_body(const cplx* in, cplx* out, const c
   }
   __syncthreads();
 
-  int p = 1;
-  // Radix-4 Stockham stages: inputs at i + m L/4, outputs at 4(i-k) + k + m p.
-  for (; 4 * p <= L; p *= 4) {
-    cplx v[S::PER4][4];
-    const int step = L / (4 * p);
-#pragma unroll
-    for (int r = 0; r < S::PER4; ++r) {
-      const int b = tid + r * NT;
-      if (b < E / 4) {
-        const int s = b / (L / 4), i = b % (L / 4);
-        const int k = i & (p - 1);
-        const cplx* x = buf + s * STRIDE;
-        cplx u0 = x[i], u1 = x[i + L / 4], u2 = x[i + L / 2], u3 = x[i + 3 * L / 4];
-        if (p > 1) {
-          u1 = mul(u1, twiddle(tw_l, k * step, sign));
-          u2 = mul(u2, twiddle(tw_l, 2 * k * step, sign));
-          u3 = mul(u3, twiddle(tw_l, 3 * k * step, sign));
-        }
-        const cplx a0 = add(u0, u2), a1 = sub(u0, u2), a2 = add(u1, u3);
-        const cplx a3 = rot(sub(u1, u3), sign);
-        v[r][0] = add(a0, a2);
-        v[r][1] = add(a1, a3);
-        v[r][2] = sub(a0, a2);
-        v[r][3] = sub(a1, a3);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < S::PER4; ++r) {
-      const int b = tid + r * NT;
-      if (b < E / 4) {
-        const int s = b / (L / 4), i = b % (L / 4);
-        const int k = i & (p - 1);
-        cplx* y = buf + s * STRIDE + ((i - k) << 2) + k;
-        y[0] = v[r][0];
-        y[p] = v[r][1];
-        y[2 * p] = v[r][2];
-        y[3 * p] = v[r][3];
-      }
-    }
-    __syncthreads();
-  }
-  if (p < L) {  // one radix-2 stage with p = L/2: output j = i
-    cplx v[S::PER2][2];
-#pragma unroll
-    for (int r = 0; r < S::PER2; ++r) {
-      const int b = tid + r * NT;
-      if (b < E / 2) {
-        const int s = b / (L / 2), i = b % (L / 2);
-        const cplx* x = buf + s * STRIDE;
-        const cplx u0 = x[i];
-        const cplx u1 = mul(x[i + L / 2], twiddle(tw_l, i, sign));
-        v[r][0] = add(u0, u1);
-        v[r][1] = sub(u0, u1);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < S::PER2; ++r) {
-      const int b = tid + r * NT;
-      if (b < E / 2) {
-        const int s = b / (L / 2), i = b % (L / 2);
-        cplx* y = buf + s * STRIDE;
-        y[i] = v[r][0];
-        y[i + L / 2] = v[r][1];
-      }
-    }
-    __syncthreads();
-  }
+  lds_transform<LOG2L, G>(buf, tw_l, sign);
 
   // Store, with the four-step twiddle W_N^{s k} where asked.
   for (int e = tid; e < E; e += NT) {
@@ -376,6 +293,46 @@ void FftPlan::layout(std::size_t n, std::size_t batch) {
   }
   for (const auto& p : passes_)
     SP_CHECK(kernel_for(p.log2L, p.log2G) != nullptr, "no FFT pass kernel for this shape");
+}
+
+void FftPlan::init_axis(std::size_t n, std::size_t inner, std::size_t outer,
+                        std::size_t max_points) {
+  SP_CHECK(n >= 4 && (n & (n - 1)) == 0 && n <= static_cast<std::size_t>(kMaxElems),
+           "an axis transform has a power-of-two length from 4 to 4096");
+  SP_CHECK(inner >= 1 && outer >= 1 && inner <= (std::size_t{1} << 30),
+           "an axis transform's strides must be positive");
+  n_ = n;
+  batch_ = outer;
+  owns_tables_ = true;
+  passes_.clear();
+  const int m = ilog2(n);
+  FftPass p;
+  p.log2L = m;
+  // Sequences a workgroup owns: as many neighbours as LDS holds (at most 16, 256 bytes
+  // of each row), so that every load and store moves contiguous runs of G points.
+  int g = 0;
+  const std::size_t cap = std::min(max_points, static_cast<std::size_t>(kMaxElems));
+  while (g < 4 && (std::size_t{1} << (m + g + 1)) <= cap && (std::size_t{1} << (g + 1)) <= inner &&
+         inner % (std::size_t{1} << (g + 1)) == 0)
+    ++g;
+  p.log2G = g;
+  p.in_sj = static_cast<long>(inner);
+  p.in_ss = 1;
+  p.out_sk = static_cast<long>(inner);
+  p.out_ss = 1;
+  // Rows are the outer blocks; a single block needs no row stride.
+  p.batch_in = p.batch_out = outer > 1 ? static_cast<long>(n * inner) : 0;
+  p.n_seq = static_cast<int>(inner);
+  passes_.push_back(p);
+  SP_CHECK(kernel_for(p.log2L, p.log2G) != nullptr, "no FFT pass kernel for this shape");
+  auto upload = [&](std::size_t len, cplx** dst) {
+    const auto host = unit_roots(len);
+    SP_HIP(hipMalloc(dst, len * sizeof(cplx)));
+    SP_HIP(hipMemcpy(*dst, host.data(), len * sizeof(cplx), hipMemcpyHostToDevice));
+    bytes_ += len * sizeof(cplx);
+  };
+  upload(n, &roots_n_);
+  upload(n, &roots_l_[0]);
 }
 
 std::size_t FftPlan::table_points(std::size_t n) {

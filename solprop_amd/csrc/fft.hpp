@@ -83,6 +83,12 @@ class FftPlan {
   // table_points(n) points at `tables`: nothing is allocated or freed. The tables hold
   // nothing until upload_tables has run (on `stream`, which it then synchronises).
   void init_external(std::size_t n, std::size_t batch, cplx* tables);
+  // A transform along one axis of a row-major array (outer, n, inner): `outer` blocks of
+  // n points each `inner` apart, every one of the `inner` sequences of a block
+  // transformed (n <= 4096, one LDS pass; in place through exec). A workgroup owns up to
+  // 16 neighbouring sequences, at most `max_points` points in all.
+  void init_axis(std::size_t n, std::size_t inner, std::size_t outer,
+                 std::size_t max_points = 4096);
   void upload_tables(hipStream_t stream) const;
   static std::size_t table_points(std::size_t n);
   // In place on data (rows of n); `scratch` holds n*batch points for the four-step.

@@ -954,7 +954,7 @@ void FieldWorkspace::monitor_reading(double* out) {
 }
 
 std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double* windows,
-                                               int n_windows) {
+                                               int n_windows, const StatExtras& extras) {
   if (!loaded_) throw Error(kState, "no state is loaded");
   flush();
   // The coarse grid: n_t samples spaced by the fine spacing times n_to / n_t.
@@ -962,11 +962,14 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
     stats_ = std::make_unique<PlaneStatistics>(P_.C, P_.N, n_t_, true,
                                                P_.dt_fine * P_.T / n_t_, P_.omega);
   std::vector<double> trace;
+  double intensity = 0.0;
   std::vector<double> out =
-      stats_->run(B_.y, flags & ~kStatPlasma, windows, n_windows, stream_, &trace);
-  if (flags & kStatPlasma) {
-    SP_CHECK(P_.has_plasma, "the electron density and the peak ionisation rate need a "
-                            "compiled plasma response, and this problem declares none");
+      stats_->run(B_.y, flags & ~(kStatPlasma | kStatPopulations), windows, n_windows, stream_,
+                  &trace, &extras, &intensity);
+  std::vector<double> populations;
+  if (flags & (kStatPlasma | kStatPopulations)) {
+    SP_CHECK(P_.has_plasma, "the plasma statistics need a compiled plasma response, and this "
+                            "problem declares none");
     // The plasma column at the committed state and its position: the node passes on its
     // modal fields, then the densest node's end fraction and largest rate.
     fill_position(z_);
@@ -980,11 +983,22 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
     double densest[2];
     SP_HIP(hipMemcpyAsync(densest, B_.partials, sizeof(densest), hipMemcpyDeviceToHost,
                           stream_));
+    if (flags & kStatPopulations) {
+      // Each node's end-of-window fraction and largest rate.
+      if (!populations_) populations_ = alloc<double>(2 * static_cast<std::size_t>(P_.Q));
+      launch_plasma_populations(B_.frac, B_.rate, P_.Q, P_.T, populations_, stream_);
+      populations.assign(2 * static_cast<std::size_t>(P_.Q), 0.0);
+      SP_HIP(hipMemcpyAsync(populations.data(), populations_,
+                            populations.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            stream_));
+    }
     SP_HIP(hipStreamSynchronize(stream_));
     if (flags & kStatElectronDensity) out.push_back(density_(z_) * densest[0]);
     if (flags & kStatPeakIonisationRate) out.push_back(densest[1]);
   }
+  if (flags & kStatPeakIntensity) out.push_back(intensity);
   out.insert(out.end(), trace.begin(), trace.end());
+  out.insert(out.end(), populations.begin(), populations.end());
   return out;
 }
 

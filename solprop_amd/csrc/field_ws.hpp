@@ -40,7 +40,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
               double* modal) override;
   long buffer(const char* name, double* out, long count) override;
   std::vector<double> statistics(std::uint64_t flags, const double* windows,
-                                 int n_windows) override;
+                                 int n_windows, const StatExtras& extras) override;
   void bind_monitor(const double* weights, const double* columns) override;
   void monitor_reading(double* out) override;
   void set_profile(bool on) override { profile_ = on; }
@@ -156,6 +156,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   bool spatial_ = false;        // the transverse collar's matrix (dense route)
   int n_t_ = 0;                 // coarse samples (the statistics' grid)
   std::unique_ptr<PlaneStatistics> stats_;  // built at the first statistics call
+  double* populations_ = nullptr;            // (2 Q), at the first populations call
   MonitorBuffers monitor_;
   double* spatial_matrix_ = nullptr;  // (C, C) row-major
   bool absorbing() const { return in_stage_ || spatial_; }

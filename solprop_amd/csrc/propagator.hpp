@@ -79,6 +79,15 @@ struct AttemptLog {
   }
 };
 
+// What the peak intensity reads beside the state: the on-axis synthesis, `rows` (1 or 2)
+// rows of the C columns, and the intensity factor (the reference's `on_axis` and
+// `intensity_factor`, native/src/general/statistics.cpp:517-603).
+struct StatExtras {
+  const double* on_axis = nullptr;
+  int rows = 0;
+  double intensity_factor = 1.0;
+};
+
 class Propagator {
  public:
   virtual ~Propagator() = default;
@@ -119,9 +128,9 @@ class Propagator {
     throw std::invalid_argument("buffer readback is built on the field route only");
   }
   // Plane statistics of the committed state (stats.hpp's flags, windows as (lo, hi)
-  // pairs in rad/s), packed in the reference's order.
+  // pairs in rad/s, the peak intensity's inputs), packed in the reference's order.
   virtual std::vector<double> statistics(std::uint64_t flags, const double* windows,
-                                         int n_windows) = 0;
+                                         int n_windows, const StatExtras& extras) = 0;
   // The boundary monitor (weights (4, n_t) on the coarse axis, or null to unbind; the
   // columns' weights or null), and its reading of the committed state with the split
   // collar's sums since the last reading (10 doubles).

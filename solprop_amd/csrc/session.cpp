@@ -66,7 +66,7 @@ void Session::observe(Batch& b, Observation kind, long plane) {
   if (plan_.statistics) {
     b.stats = ws_.statistics(plan_.statistics,
                              plan_.windows.empty() ? nullptr : plan_.windows.data(),
-                             static_cast<int>(plan_.windows.size() / 2));
+                             static_cast<int>(plan_.windows.size() / 2), StatExtras{});
     b.event.has_stats = true;
   }
 }

@@ -109,6 +109,35 @@ __global__ void __launch_bounds__(kStatThreads)
   }
 }
 
+// Per block of samples, the largest on-axis power sum_rows |sum_c M[row, c] a_c(t)|^2,
+// the columns summed in order (native/src/general/statistics.cpp:517-603).
+__global__ void __launch_bounds__(kStatThreads)
+    stat_intensity_blocks(const cplx* a, int C, int n_t, const double* M, int rows,
+                          double* peaks) {
+  __shared__ double red[kStatThreads];
+  const int t = blockIdx.x * kStatThreads + threadIdx.x;
+  double power = 0.0;
+  if (t < n_t) {
+    for (int row = 0; row < rows; ++row) {
+      double re = 0.0, im = 0.0;
+      for (int c = 0; c < C; ++c) {
+        const double m = M[row * C + c];
+        const cplx z = a[static_cast<long>(c) * n_t + t];
+        re = re + m * z.x;
+        im = im + m * z.y;
+      }
+      power = power + (re * re + im * im);
+    }
+  }
+  red[threadIdx.x] = power;
+  __syncthreads();
+  for (int width = kStatThreads / 2; width > 0; width >>= 1) {
+    if (threadIdx.x < width) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + width]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) peaks[blockIdx.x] = red[0];
+}
+
 // The total power trace: sum_c |a_c(t)|^2 in column order at each sample.
 __global__ void stat_total_power(const cplx* a, int C, int n_t, double* trace) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -228,7 +257,33 @@ __global__ void __launch_bounds__(kStatThreads)
   }
 }
 
+// One workgroup a node: its end-of-window fraction and its largest rate by a fixed tree.
+__global__ void __launch_bounds__(kStatThreads)
+    plasma_populations(const double* frac, const double* rate, int T, double* out) {
+  __shared__ double red[kStatThreads];
+  const long q = blockIdx.x;
+  double peak = 0.0;
+  for (int t = threadIdx.x; t < T; t += kStatThreads) peak = fmax(peak, rate[q * T + t]);
+  red[threadIdx.x] = peak;
+  __syncthreads();
+  for (int width = kStatThreads / 2; width > 0; width >>= 1) {
+    if (threadIdx.x < width) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + width]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * q] = frac[q * T + T - 1];
+    out[2 * q + 1] = red[0];
+  }
+}
+
 }  // namespace
+
+void launch_plasma_populations(const double* frac, const double* rate, int Q, int T,
+                               double* out, hipStream_t s) {
+  SP_CHECK(frac && rate && out && Q >= 1 && T >= 1, "the plasma column's arrays");
+  hipLaunchKernelGGL(plasma_populations, dim3(Q), dim3(kStatThreads), 0, s, frac, rate, T, out);
+  SP_HIP(hipGetLastError());
+}
 
 void launch_plasma_densest(const double* frac, const double* rate, int Q, int T, double* out,
                            hipStream_t s) {
@@ -319,6 +374,8 @@ PlaneStatistics::PlaneStatistics(int C, int N, int n_t, bool field, double dt,
   peak_blocks_ = (n_t + kStatThreads - 1) / kStatThreads;
   SP_HIP(hipMalloc(&peaks_, static_cast<std::size_t>(peak_blocks_) * (C + 1) * sizeof(double)));
   SP_HIP(hipMalloc(&trace_, static_cast<std::size_t>(n_t) * sizeof(double)));
+  SP_HIP(hipMalloc(&on_axis_, 2 * static_cast<std::size_t>(C) * sizeof(double)));
+  SP_HIP(hipMalloc(&intensity_, static_cast<std::size_t>(peak_blocks_) * sizeof(double)));
   SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_),
                        sizeof(double) * (static_cast<std::size_t>(C) * kSums +
                                          static_cast<std::size_t>(peak_blocks_) * (C + 1))));
@@ -335,20 +392,30 @@ PlaneStatistics::~PlaneStatistics() {
   if (sums_) (void)hipFree(sums_);
   if (peaks_) (void)hipFree(peaks_);
   if (trace_) (void)hipFree(trace_);
+  if (on_axis_) (void)hipFree(on_axis_);
+  if (intensity_) (void)hipFree(intensity_);
   if (host_) (void)hipHostFree(host_);
 }
 
 std::vector<double> PlaneStatistics::run(const cplx* y, std::uint64_t flags,
                                          const double* windows, int n_windows, hipStream_t s,
-                                         std::vector<double>* trace) {
+                                         std::vector<double>* trace, const StatExtras* extras,
+                                         double* intensity) {
   SP_CHECK((flags & ~kStatBuilt) == 0,
-           "statistics built: omega0, energy, energy_window, peak_power, peak_power_total");
+           "statistics built: omega0, energy, energy_window, peak_power, peak_power_window, "
+           "peak_power_total, fwhm, peak_intensity");
+  const bool wants_intensity = flags & kStatPeakIntensity;
+  if (wants_intensity) {
+    SP_CHECK(extras && intensity && extras->on_axis && (extras->rows == 1 || extras->rows == 2),
+             "on-axis synthesis must have one or two rows and n_modes columns");
+    SP_CHECK(std::isfinite(extras->intensity_factor), "the intensity factor must be finite");
+  }
   SP_CHECK(n_windows >= 0 && n_windows <= kMaxStatWindows, "at most 16 statistics windows");
   SP_CHECK(n_windows == 0 || windows, "the statistics windows are missing");
   for (int j = 0; j < n_windows; ++j)
     SP_CHECK(!(windows[2 * j] > windows[2 * j + 1]), "a statistics window's lo exceeds its hi");
   const bool spectral = flags & (kStatOmega0 | kStatEnergy | kStatEnergyWindow);
-  const bool peaks = flags & (kStatPeakPower | kStatPeakPowerTotal | kStatFwhm);
+  const bool peaks = flags & (kStatPeakPower | kStatPeakPowerTotal | kStatFwhm | kStatPeakIntensity);
   SP_CHECK(!(flags & kStatFwhm) || trace, "the power trace needs somewhere to go");
   const int used_windows = (flags & kStatEnergyWindow) ? n_windows : 0;
   const std::size_t n_sums = static_cast<std::size_t>(C_) * kSums;
@@ -374,6 +441,14 @@ std::vector<double> PlaneStatistics::run(const cplx* y, std::uint64_t flags,
     SP_HIP(hipMemcpyAsync(host_ + n_sums, peaks_,
                           static_cast<std::size_t>(peak_blocks_) * (C_ + 1) * sizeof(double),
                           hipMemcpyDeviceToHost, s));
+    if (wants_intensity) {
+      SP_HIP(hipMemcpyAsync(on_axis_, extras->on_axis,
+                            static_cast<std::size_t>(extras->rows) * C_ * sizeof(double),
+                            hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(stat_intensity_blocks, dim3(peak_blocks_), dim3(kStatThreads), 0, s,
+                         time_, C_, n_t_, on_axis_, extras->rows, intensity_);
+      SP_HIP(hipGetLastError());
+    }
     if (flags & kStatFwhm) {
       hipLaunchKernelGGL(stat_total_power, dim3((n_t_ + kStatThreads - 1) / kStatThreads),
                          dim3(kStatThreads), 0, s, time_, C_, n_t_, trace_);
@@ -411,7 +486,18 @@ std::vector<double> PlaneStatistics::run(const cplx* y, std::uint64_t flags,
       }
     }
   }
+  std::vector<double> intensity_blocks;
+  if (wants_intensity) {
+    intensity_blocks.assign(peak_blocks_, 0.0);
+    SP_HIP(hipMemcpyAsync(intensity_blocks.data(), intensity_,
+                          intensity_blocks.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
   SP_HIP(hipStreamSynchronize(s));
+  if (wants_intensity) {
+    double peak = 0.0;
+    for (double v : intensity_blocks) peak = std::fmax(peak, v);
+    *intensity = extras->intensity_factor * peak;
+  }
   std::vector<double> out;
   const double* sums = host_;
   if (flags & kStatOmega0) {

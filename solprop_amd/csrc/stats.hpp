@@ -10,6 +10,7 @@
 
 #include "device.hpp"
 #include "fft.hpp"
+#include "propagator.hpp"
 
 namespace sp {
 
@@ -28,15 +29,25 @@ enum : std::uint64_t {
   // The field routes' plasma column at the committed state (read by the workspace).
   kStatElectronDensity = 1ULL << 7U,
   kStatPeakIonisationRate = 1ULL << 8U,
+  // The largest on-axis intensity: the on-axis synthesis applied to the modes' analytic
+  // signals, max over t of sum over its rows of the squared modulus, times the intensity
+  // factor (native/src/general/statistics.cpp:517-603); packed after the plasma pair.
+  kStatPeakIntensity = 1ULL << 9U,
+  // Each node's end-of-window ionised fraction and largest rate, (Q, 2), one channel
+  // (statistics.cpp:636-650); packed last, after the power trace.
+  kStatPopulations = 1ULL << 11U,
 };
 constexpr std::uint64_t kStatPlasma = kStatElectronDensity | kStatPeakIonisationRate;
 // The node of largest ionised fraction at the window's end (the first such node) and
 // its largest rate: out[0] that fraction, out[1] the rate (numpy_ref/statistics.py:502-546).
 void launch_plasma_densest(const double* frac, const double* rate, int Q, int T, double* out,
                            hipStream_t s);
+// Every node's end-of-window fraction and largest rate into out (2 Q).
+void launch_plasma_populations(const double* frac, const double* rate, int Q, int T,
+                               double* out, hipStream_t s);
 constexpr std::uint64_t kStatBuilt = kStatOmega0 | kStatEnergy | kStatEnergyWindow |
                                      kStatPeakPower | kStatPeakPowerWindow |
-                                     kStatPeakPowerTotal | kStatFwhm;
+                                     kStatPeakPowerTotal | kStatFwhm | kStatPeakIntensity;
 constexpr int kMaxStatWindows = 16;
 
 // The boundary monitor's sums (src/solprop/monitor.py:493-606) over the time samples of
@@ -101,8 +112,11 @@ class PlaneStatistics {
   // peak_power_window (n_windows x C), peak_power_total (1). `windows` holds n_windows (lo, hi) pairs in rad/s. With
   // kStatFwhm the total power trace (n_t) goes to `trace`, which the caller packs after
   // the statistics that precede it.
+  // With kStatPeakIntensity the largest on-axis intensity goes to `intensity`, from
+  // `extras`' on-axis rows and factor.
   std::vector<double> run(const cplx* y, std::uint64_t flags, const double* windows,
-                          int n_windows, hipStream_t s, std::vector<double>* trace = nullptr);
+                          int n_windows, hipStream_t s, std::vector<double>* trace = nullptr,
+                          const StatExtras* extras = nullptr, double* intensity = nullptr);
   std::size_t bytes() const { return bytes_ + plan_.bytes(); }
   // The monitor's five sums of the state y (kMonitorReading) into acc (device, 5).
   void reading(const cplx* y, const double* columns, const double* weights, double* partials,
@@ -116,6 +130,8 @@ class PlaneStatistics {
   FftPlan plan_;
   cplx *time_ = nullptr, *scratch_ = nullptr;
   double *windows_ = nullptr, *sums_ = nullptr, *peaks_ = nullptr, *trace_ = nullptr;
+  double* on_axis_ = nullptr;  // (2, C) the peak intensity's synthesis rows
+  double* intensity_ = nullptr;  // (peak blocks) its block maxima
   double* host_ = nullptr;
   int peak_blocks_ = 0;
   std::size_t bytes_ = 0;

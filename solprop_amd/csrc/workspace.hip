@@ -478,15 +478,21 @@ void Workspace::monitor_reading(double* out) {
 }
 
 std::vector<double> Workspace::statistics(std::uint64_t flags, const double* windows,
-                                          int n_windows) {
+                                          int n_windows, const StatExtras& extras) {
   if (!loaded_) throw Error(kState, "no state is loaded");
   ensure_committed();
   // The coarse grid: N samples spaced by the fine spacing times T / N.
   if (!stats_)
     stats_ = std::make_unique<PlaneStatistics>(P_.C, P_.N, P_.N, false,
                                                P_.dt_fine * P_.T / P_.N, P_.omega);
+  SP_CHECK(!(flags & (kStatPlasma | kStatPopulations)),
+           "the plasma statistics read the field-grid plasma column, and the envelope route "
+           "carries none");
   std::vector<double> trace;
-  std::vector<double> out = stats_->run(y_, flags, windows, n_windows, stream_, &trace);
+  double intensity = 0.0;
+  std::vector<double> out =
+      stats_->run(y_, flags, windows, n_windows, stream_, &trace, &extras, &intensity);
+  if (flags & kStatPeakIntensity) out.push_back(intensity);
   out.insert(out.end(), trace.begin(), trace.end());
   return out;
 }

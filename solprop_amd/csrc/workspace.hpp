@@ -36,7 +36,7 @@ class Workspace final : public Propagator {
                        sp_march_result* r) override;
 
   std::vector<double> statistics(std::uint64_t flags, const double* windows,
-                                 int n_windows) override;
+                                 int n_windows, const StatExtras& extras) override;
   void bind_monitor(const double* weights, const double* columns) override;
   void monitor_reading(double* out) override;
   void set_profile(bool on) override { profile_ = on; }

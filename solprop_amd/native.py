@@ -126,7 +126,9 @@ def lower(problem: Problem, settings: Settings | None = None) -> tuple[_lib.Prob
     s = settings or Settings()
     k = _Keep()
     d = _lib.ProblemDescC()
-    d.kind = 0 if problem.kind == "envelope" else 1
+    d.kind = {"envelope": 0, "field": 1, "cartesian": 2}[problem.kind]
+    if problem.transverse is not None:
+        d.grid_nx, d.grid_ny = (int(v) for v in problem.transverse)
     d.n_t, d.n_to = problem.n_t, problem.n_to
     d.n_modes, d.n_nodes, d.n_omega = problem.n_modes, problem.n_nodes, problem.n_omega
     d.crop_index = k.i64(problem.crop_index)
@@ -315,12 +317,16 @@ class Workspace:
                   "peak_power": 1 << 3, "peak_power_window": 1 << 4,
                   "peak_power_total": 1 << 5,
                   "fwhm": 1 << 6, "electron_density": 1 << 7,
-                  "peak_ionisation_rate": 1 << 8}
+                  "peak_ionisation_rate": 1 << 8, "peak_intensity": 1 << 9,
+                  "populations": 1 << 11}
 
-    def statistics(self, names, windows=None) -> dict:
+    def statistics(self, names, windows=None, on_axis=None, intensity_factor=1.0) -> dict:
         """Plane statistics of the committed state (``stats/definitions.py``): a dict of
         the requested names. ``windows`` is an (n, 2) array of (lo, hi) in rad/s for
-        ``energy_window``, whose value is (n, n_modes)."""
+        ``energy_window``, whose value is (n, n_modes). ``peak_intensity`` reads the
+        on-axis synthesis ``on_axis`` ((rows, n_modes), one or two rows) and the
+        intensity factor; ``populations`` is (n_nodes, 2): each node's end-of-window
+        ionised fraction and largest rate."""
         unknown = set(names) - set(self.STATISTICS)
         if unknown:
             raise BackendRefusal(f"statistics not built: {sorted(unknown)}")
@@ -328,29 +334,41 @@ class Workspace:
         for name in names:
             flags |= self.STATISTICS[name]
         w = np.zeros((0, 2)) if windows is None else np.ascontiguousarray(windows, np.float64)
-        n_modes = self.problem.n_modes
-        capacity = 3 * n_modes + 2 * len(w) * n_modes + 3 + self.problem.n_t
+        n_modes, n_nodes = self.problem.n_modes, self.problem.n_nodes
+        capacity = 3 * n_modes + 2 * len(w) * n_modes + 4 + self.problem.n_t + 2 * n_nodes
         out = np.zeros(capacity)
         count = C.c_int()
-        self._call("sp_workspace_statistics", flags, _ptr(w) if len(w) else None, len(w),
-                   _ptr(out), capacity, C.byref(count))
+        axis = None
+        if on_axis is not None:
+            axis = np.ascontiguousarray(np.atleast_2d(on_axis), np.float64)
+        request = _lib.StatisticsRequestC(
+            flags, _ptr(w) if len(w) else None, len(w), None if axis is None else _ptr(axis),
+            0 if axis is None else axis.shape[0], float(intensity_factor))
+        self._call("sp_workspace_statistics_request", C.byref(request), _ptr(out), capacity,
+                   C.byref(count))
         values, at, result = out[: count.value], 0, {}
         sizes = {"omega0": n_modes, "energy": n_modes, "energy_window": len(w) * n_modes,
                  "peak_power": n_modes, "peak_power_window": len(w) * n_modes,
                  "peak_power_total": 1, "electron_density": 1,
-                 "peak_ionisation_rate": 1, "fwhm": self.problem.n_t}
-        # The packing order: the flags' order, except that the power trace (the fwhm
-        # flag's) follows the plasma pair (native/src/general/statistics.cpp:160-177).
+                 "peak_ionisation_rate": 1, "peak_intensity": 1, "fwhm": self.problem.n_t,
+                 "populations": 2 * n_nodes}
+        # The packing order (native/src/general/statistics.cpp:160-182): the flags'
+        # order, except that the power trace (the fwhm flag's) follows the peak
+        # intensity, and the populations come last.
         order = ["omega0", "energy", "energy_window", "peak_power", "peak_power_window",
-                 "peak_power_total", "electron_density", "peak_ionisation_rate", "fwhm"]
-        scalars = ("peak_power_total", "electron_density", "peak_ionisation_rate")
+                 "peak_power_total", "electron_density", "peak_ionisation_rate",
+                 "peak_intensity", "fwhm", "populations"]
+        scalars = ("peak_power_total", "electron_density", "peak_ionisation_rate",
+                   "peak_intensity")
         for name in order:
             if name in names:
                 chunk = values[at: at + sizes[name]]
                 at += sizes[name]
                 windowed = name in ("energy_window", "peak_power_window")
                 result[name] = (chunk.reshape(len(w), n_modes) if windowed
-                                else float(chunk[0]) if name in scalars else chunk)
+                                else float(chunk[0]) if name in scalars
+                                else chunk.reshape(n_nodes, 2) if name == "populations"
+                                else chunk)
         return result
 
     def bind_monitor(self, weights: np.ndarray | None, columns: np.ndarray | None = None):
@@ -650,6 +668,9 @@ class Session:
         unknown = set(names) - set(Workspace.STATISTICS)
         if unknown:
             raise BackendRefusal(f"statistics not built: {sorted(unknown)}")
+        if "peak_intensity" in names:
+            raise BackendRefusal("a session's plan carries no on-axis synthesis: take "
+                                 "peak_intensity with Workspace.statistics")
         flags = 0
         for name in names:
             flags |= Workspace.STATISTICS[name]
@@ -679,7 +700,8 @@ class Session:
         self._rows = np.zeros((self._capacity, 6))
         # As many doubles as Workspace.statistics packs at most.
         n_modes = problem.n_modes
-        self._stats = np.zeros(3 * n_modes + 2 * len(windows) * n_modes + 3 + problem.n_t)
+        self._stats = np.zeros(3 * n_modes + 2 * len(windows) * n_modes + 4 + problem.n_t
+                               + 2 * problem.n_nodes)
 
     def _call(self, name, *args):
         if self._handle is None:

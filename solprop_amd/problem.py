@@ -200,7 +200,9 @@ class Problem:
     ``T = n_to``.
     """
 
-    kind: str  # "envelope" (complex time samples) or "field" (real)
+    # "envelope" (complex time samples), "field" (real) or "cartesian" (the free-space
+    # envelope on an (ny, nx, nt) grid: columns iy nx + ix, `transverse` = (nx, ny))
+    kind: str
     n_t: int
     n_to: int
     n_modes: int
@@ -244,6 +246,8 @@ class Problem:
     # The radial route's transverse collar (``Problem.Spatial.absorber_matrix``):
     # F diag(-alpha(r) / 2) B on the mode axis, (C, C) float64, 1/m; None without one.
     spatial_absorber: np.ndarray | None = None
+    # The Cartesian route's transverse grid (nx, ny); None on the other routes.
+    transverse: tuple | None = None
     description: dict = field(default_factory=dict)
 
     @property
@@ -294,8 +298,14 @@ class Problem:
     def validate(self) -> None:
         """Refuse shapes and dtypes the library cannot run (before any allocation)."""
         n, t, c = self.n_omega, self.n_to, self.n_modes
+        if self.kind == "cartesian":
+            self._validate_cartesian()
+            return
+        if self.transverse is not None:
+            raise ValueError("a transverse grid belongs to the Cartesian route")
         if self.kind not in ("field", "envelope"):
-            raise ValueError(f"kind must be 'field' or 'envelope', got {self.kind!r}")
+            raise ValueError(f"kind must be 'field', 'envelope' or 'cartesian', got "
+                             f"{self.kind!r}")
         if t & (t - 1) or t < 16:
             raise ValueError(f"n_to must be a power of two of at least 16, got {t}")
         if self.n_t > t or t % self.n_t:
@@ -343,6 +353,53 @@ class Problem:
                 raise ValueError("spatial_absorber: expected (n_modes, n_modes) float64")
             if not np.all(np.isfinite(m)):
                 raise ValueError("spatial_absorber must be finite")
+
+    def _validate_cartesian(self) -> None:
+        """The free-space route's refusals: a power-of-two (ny, nx, nt) grid of at most
+        4096 points an axis, one constant operator table with W(z) = z, one prefactor
+        table, the analytic Kerr response alone, the low-storage replay."""
+        if self.transverse is None or len(self.transverse) != 2:
+            raise ValueError("the Cartesian route needs its transverse grid (nx, ny)")
+        nx, ny = (int(v) for v in self.transverse)
+        t = self.n_to
+        for name, v in (("nx", nx), ("ny", ny), ("n_t", t)):
+            if v < 4 or v > 4096 or v & (v - 1):
+                raise ValueError(f"the Cartesian grid's {name} must be a power of two from "
+                                 f"4 to 4096, got {v}")
+        if self.n_t != t or self.n_omega != t:
+            raise ValueError("a Cartesian grid keeps every bin: n_t = n_to = n_omega")
+        if not np.array_equal(self.crop_index, np.arange(t)):
+            raise ValueError("a Cartesian grid's crop is the identity")
+        if self.n_modes != nx * ny:
+            raise ValueError(f"a Cartesian grid has nx ny = {nx * ny} columns, got "
+                             f"n_modes = {self.n_modes}")
+        c, n = self.n_modes, t
+        for name, arr, shape, dtype in (
+            ("active_mask", self.active_mask, (n,), np.bool_),
+            ("norm_mask", self.norm_mask, (n,), np.bool_),
+            ("phi_terms", self.phi_terms, (1, c, n), np.complex128),
+            ("prefactor_terms", self.prefactor_terms, (1, c, n), np.complex128),
+        ):
+            if arr.shape != shape or arr.dtype != dtype:
+                raise ValueError(f"{name}: expected {shape} {np.dtype(dtype)}, got "
+                                 f"{arr.shape} {arr.dtype}")
+        if not self.active_mask.all():
+            raise ValueError("a Cartesian grid's bins are all active (the tables carry the "
+                             "band)")
+        if len(self.phi_paths) != 1 or self.phi_paths[0].kind != 1:
+            raise ValueError("the Cartesian operator is one constant table, W(z) = z")
+        if len(self.prefactor_paths) != 1 or self.prefactor_paths[0].kind != 0:
+            raise ValueError("the Cartesian prefactor is one constant table")
+        if self.density.kind != 0:
+            raise ValueError("the Cartesian medium is homogeneous (a constant density)")
+        if self.kerr is not None and self.kerr.cubic_form != "analytic":
+            raise ValueError("the Cartesian envelope takes the analytic Kerr form")
+        if self.raman is not None or self.plasma is not None:
+            raise ValueError("the Cartesian route is built with the Kerr response alone")
+        if self.absorber_scheme != "none" or self.spatial_absorber is not None:
+            raise ValueError("the Cartesian route has no absorbers")
+        if self.method != "low_storage" or self.schedule is None:
+            raise ValueError("the Cartesian route runs the low-storage replay of a schedule")
 
 
 # ----------------------------------------------------------------------- builders
@@ -1129,5 +1186,165 @@ def radial_problem(
             "length_m": length, "pressure_bar": pressure, "gas": "Ar",
             "wavelength_m": wavelength, "fwhm_s": fwhm, "w0_m": w0, "energy_J": energy,
             "rtol": rtol, "stepper": "exponential Adams, variable order <= 5",
+        },
+    )
+
+
+# -------------------------------------------------------------- free-space Cartesian
+
+# Fused silica (I. H. Malitson, JOSA 55, 1205 (1965)): n^2 - 1 = sum B lambda^2 /
+# (lambda^2 - C^2), lambda and C in um (the material of the reference's Kerr-beam cases,
+# benchmarks/builders.py `kerr`).
+SILICA_SELLMEIER = ((0.6961663, 0.0684043), (0.4079426, 0.1162414), (0.8974794, 9.896161))
+
+
+def silica_wavenumber(omega: np.ndarray) -> np.ndarray:
+    """k(omega) = n(omega) omega / c in fused silica, 1/m (positive omega only)."""
+    lam_um = 2e6 * np.pi * C_LIGHT / np.asarray(omega, dtype=np.float64)
+    n2 = np.ones_like(lam_um)
+    for b, c in SILICA_SELLMEIER:
+        n2 = n2 + b * lam_um**2 / (lam_um**2 - c**2)
+    return np.sqrt(n2) * np.asarray(omega) / C_LIGHT
+
+
+def transverse_axis(n: int, half_width: float) -> tuple[np.ndarray, np.ndarray]:
+    """A periodic axis of n samples over [-a, a): positions (centred) and wavenumbers
+    (unshifted FFT order), rad/m."""
+    dx = 2.0 * half_width / n
+    x = (np.arange(n) - n // 2) * dx
+    return x, 2.0 * np.pi * np.fft.fftfreq(n, dx)
+
+
+def cartesian_tables(k: np.ndarray, kperp_sq: np.ndarray, omega: np.ndarray,
+                     active: np.ndarray, beta0: float, beta1: float, omega0: float,
+                     rows: int = 4096) -> tuple[np.ndarray, np.ndarray]:
+    """The free-space operator and prefactor tables, (C, N) each, in the internal
+    convention (bin w at omega0 + dw, fields as exp(+i omega t)):
+
+        D = -i (k_z - beta0 - beta1 dw),   p = -i omega^2 / (2 eps0 c^2 k_z)
+
+    with k_z = sqrt(k^2 - kperp^2) on the propagating bins; an evanescent bin decays at
+    |k_z| and has no source; inactive bins have neither (the spatial split of the
+    reference's separable operator, src/solprop/translate/free.py:161-249, formed here
+    whole, per element). Built a block of `rows` columns at a time."""
+    c_count, n = kperp_sq.size, k.size
+    phi = np.zeros((c_count, n), np.complex128)
+    pref = np.zeros((c_count, n), np.complex128)
+    k2 = np.where(active, k, 0.0) ** 2
+    dw = omega - omega0
+    for r0 in range(0, c_count, rows):
+        arg = k2[None, :] - kperp_sq[r0:r0 + rows, None]
+        prop = (arg > 0.0) & active[None, :]
+        kz = np.sqrt(np.abs(arg))
+        phi[r0:r0 + rows] = np.where(prop, -1j * (kz - beta0 - beta1 * dw[None, :]),
+                                     np.where(active[None, :], -kz, 0.0))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            pref[r0:r0 + rows] = np.where(
+                prop, -1j * omega[None, :] ** 2 / (2.0 * EPS0 * C_LIGHT**2 * kz), 0.0)
+    return phi, pref
+
+
+def cartesian_problem(
+    *,
+    nx: int = 64,
+    ny: int = 64,
+    n_t: int = 256,
+    spacing: float = 1.05e-6,
+    window: float = 250e-15,
+    wavelength: float = 260e-9,
+    band=(143e-9, 600e-9),
+    fwhm: float = 30e-15,
+    energy: float = 1e-7,
+    waist: float = 40e-6,
+    chi3: float = 2.2e-22,
+    length: float = 2e-6,
+    steps: int = 4,
+    kerr: bool = True,
+) -> Problem:
+    """A Kerr beam in fused silica on a Cartesian (ny, nx, n_t) envelope grid, marched
+    by `steps` fixed RK4IP steps over `length`.
+
+    The reference's `kerr_beam_*` cases (benchmarks/registry.py:348-379,
+    benchmarks/builders.py `kerr`): a 260 nm, 30 fs, 0.1 uJ Gaussian of 40 um waist
+    in Malitson's silica with chi3 = 2.2e-22 m^2/V^2, a 250 fs window, the band 143-600
+    nm, four 0.5 um steps over 2 um, the transverse spacing of the 384-point case (1.05
+    um). Power-of-two axes here. Kerr: P = eps0 chi3 (3/4) |E|^2 E (E in V/m), so the
+    coefficient is eps0 chi3 with the self factor 3/4. The launch is
+    E = E0 exp(-r^2 / w0^2) exp(-2 ln 2 t^2 / tau^2), E0 from the energy at the carrier's
+    index (U = n0 eps0 c E0^2 / 2 * pi w0^2 / 2 * tau sqrt(pi / (4 ln 2)))."""
+    c_count = nx * ny
+    dt = window / n_t
+    omega0 = 2.0 * np.pi * C_LIGHT / wavelength
+    dw = 2.0 * np.pi * np.fft.fftfreq(n_t, dt)
+    omega = omega0 + dw
+    lo, hi = 2.0 * np.pi * C_LIGHT / band[1], 2.0 * np.pi * C_LIGHT / band[0]
+    active = (omega > lo) & (omega < hi)
+    k = np.zeros(n_t)
+    k[active] = silica_wavenumber(omega[active])
+    beta0 = float(silica_wavenumber(np.array([omega0]))[0])
+    h = 1e-4 * omega0
+    beta1 = float((silica_wavenumber(np.array([omega0 + h]))[0]
+                   - silica_wavenumber(np.array([omega0 - h]))[0]) / (2.0 * h))
+    x, kx = transverse_axis(nx, 0.5 * nx * spacing)
+    y, ky = transverse_axis(ny, 0.5 * ny * spacing)
+    kperp_sq = (ky[:, None] ** 2 + kx[None, :] ** 2).reshape(c_count)
+    phi, pref = cartesian_tables(k, kperp_sq, omega, active, beta0, beta1, omega0)
+    n0 = beta0 * C_LIGHT / omega0
+    tau = fwhm
+    area = 0.5 * np.pi * waist**2
+    duration = tau * math.sqrt(math.pi / (4.0 * math.log(2.0)))
+    e0 = math.sqrt(2.0 * energy / (n0 * EPS0 * C_LIGHT * area * duration))
+    t = (np.arange(n_t) - n_t // 2) * dt
+    envelope_t = np.exp(-2.0 * math.log(2.0) * t**2 / tau**2)
+    profile_x = np.exp(-(x**2) / waist**2)
+    profile_y = np.exp(-(y**2) / waist**2)
+    # fftn of the separable launch: the product of the axes' transforms.
+    spec_t = np.fft.fft(e0 * envelope_t) * active
+    spec_x = np.fft.fft(profile_x)
+    spec_y = np.fft.fft(profile_y)
+    field0 = ((spec_y[:, None] * spec_x[None, :]).reshape(c_count, 1)
+              * spec_t[None, :]).astype(np.complex128)
+    edges = np.linspace(0.0, length, steps + 1)
+    return Problem(
+        kind="cartesian",
+        n_t=n_t,
+        n_to=n_t,
+        n_modes=c_count,
+        n_nodes=0,
+        crop_index=np.arange(n_t, dtype=np.int64),
+        active_mask=np.ones(n_t, dtype=bool),
+        norm_mask=active.copy(),
+        omega_window=active.astype(np.float64),
+        omega=omega,
+        time_window=np.ones(n_t),
+        time_window_derivative=np.zeros(n_t),
+        dt_fine=dt,
+        z_end=length,
+        phi_terms=phi.reshape(1, c_count, n_t),
+        phi_paths=(ScalarPath.position(),),
+        prefactor_terms=pref.reshape(1, c_count, n_t),
+        prefactor_paths=(ScalarPath.constant(1.0),),
+        synthesis=np.zeros((0, c_count)),
+        projection=np.zeros((c_count, 0)),
+        density=ScalarPath.constant(1.0),
+        preionisation=ScalarPath.constant(0.0),
+        kerr=KerrGroup(EPS0 * chi3, "analytic", self_factor=0.75) if kerr else None,
+        field0=field0,
+        method="low_storage",
+        schedule=edges,
+        transverse=(nx, ny),
+        description={
+            "route": "free-space Cartesian",
+            "grid": (ny, nx, n_t),
+            "spacing_m": spacing,
+            "window_s": window,
+            "wavelength_m": wavelength,
+            "band_m": tuple(band),
+            "fwhm_s": fwhm,
+            "energy_J": energy,
+            "waist_m": waist,
+            "chi3_m2_per_V2": chi3,
+            "length_m": length,
+            "steps": steps,
         },
     )

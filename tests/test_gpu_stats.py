@@ -155,6 +155,52 @@ def test_split_sums_follow_the_oracle_march(route):
     assert second["taken"] == 0.0
 
 
+@pytest.mark.parametrize("route", sorted(PROBLEMS))
+def test_peak_intensity_matches_oracle(route):
+    """The on-axis synthesis's peak (one row, and two rows) after a short march, with
+    every other statistic: within 1e-12 relative, and packed between the plasma pair and
+    the power trace."""
+    print(f"peak intensity {route}", flush=True)
+    problem = PROBLEMS[route]()
+    ws = Workspace(problem)
+    session = ws.march(0.0, problem.field0)
+    session.advance(problem.z_end, work_limit=4)
+    state = session.state()
+    c = problem.n_modes
+    rng = np.random.default_rng(5)
+    field = state.reshape(c, problem.n_omega)
+    for rows in (rng.uniform(-1, 1, (1, c)), rng.uniform(-1, 1, (2, c))):
+        got = ws.statistics(["peak_power_total", "peak_intensity", "fwhm"], on_axis=rows,
+                            intensity_factor=1.7)
+        expected = S.peak_intensity(field, problem.kind, rows, 1.7)
+        assert abs(got["peak_intensity"] - expected) <= 1e-12 * expected
+        assert np.allclose(got["fwhm"], S.power_trace(field, problem.kind), rtol=0,
+                           atol=1e-12 * np.max(got["fwhm"]))
+    with pytest.raises(Exception, match="on-axis"):
+        ws.statistics(["peak_intensity"])
+    ws.close()
+
+
+def test_populations_match_oracle():
+    """Every node's end fraction and largest rate after a short march, with the plasma
+    pair: the rates to 1e-12, the fractions to 1e-9 (as the electron density)."""
+    print("populations capillary", flush=True)
+    problem = capillary_problem(n_t=2048, n_nodes=32, length=0.05, integration="trapezoid")
+    ws = Workspace(problem)
+    session = ws.march(0.0, problem.field0)
+    session.advance(problem.z_end, work_limit=4)
+    state, z = session.state(), session.z
+    got = ws.statistics(["electron_density", "peak_ionisation_rate", "fwhm", "populations"])
+    ws.close()
+    expected = S.populations(problem, state, z)
+    pops = got["populations"]
+    assert pops.shape == (32, 2)
+    assert np.allclose(pops[:, 1], expected[:, 1], rtol=1e-12, atol=0)
+    assert np.allclose(pops[:, 0], expected[:, 0], rtol=1e-9, atol=1e-300)
+    densest = int(np.argmax(pops[:, 0]))
+    assert got["peak_ionisation_rate"] == pops[densest, 1]
+
+
 @pytest.mark.parametrize("route", ["capillary", "radial"])
 def test_plasma_statistics_match_oracle(route):
     """The electron density and the peak ionisation rate at the committed state after a

@@ -458,6 +458,43 @@ def test_peak_power_of_a_sampled_envelope_and_of_a_carrier():
     assert np.isclose(S.peak_power(X, "field")[0], 9.0, rtol=1e-10)
 
 
+def test_peak_intensity_rows():
+    """A one-hot on-axis row picks that mode's peak power; a row of ones the peak of the
+    modes' coherent sum (two pulses in phase at one instant add in amplitude); two rows
+    add their powers before the maximum; the factor scales."""
+    from oracle import stats as S
+
+    n_t = 256
+    t = np.arange(n_t) - n_t / 2
+    env = np.exp(-(t / 20.0) ** 2)
+    A = np.stack([2.0 * env, env, np.roll(env, 64)])
+    X = np.fft.fft(A, axis=-1)
+    peaks = S.peak_power(X, "envelope")
+    for m in range(3):
+        row = np.eye(3)[m]
+        assert np.isclose(S.peak_intensity(X, "envelope", row, 1.0), peaks[m], rtol=1e-13)
+    assert np.isclose(S.peak_intensity(X, "envelope", [1.0, 1.0, 0.0], 1.0), 9.0, rtol=1e-13)
+    two = S.peak_intensity(X, "envelope", [[1.0, 0.0, 0.0], [0.0, 0.0, 1.0]], 2.5)
+    assert np.isclose(two, 2.5 * np.max(4.0 * env**2 + np.roll(env, 64) ** 2), rtol=1e-13)
+
+
+def test_populations_hold_the_plasma_pair():
+    """The populations' densest node is the plasma pair's: its end fraction times the
+    density is the electron density, its largest rate the peak ionisation rate."""
+    from oracle import stats as S
+    from solprop_amd.problem import capillary_problem
+
+    problem = capillary_problem(n_t=512, n_nodes=8, length=0.05, integration="trapezoid")
+    state = problem.field0 * 3.0
+    pops = S.populations(problem, state, 0.0)
+    density, rate = S.ionisation(problem, state, 0.0)
+    assert pops.shape == (8, 2) and np.all(pops >= 0.0)
+    densest = int(np.argmax(pops[:, 0]))
+    assert np.isclose(problem.density(0.0) * pops[densest, 0], density, rtol=1e-15)
+    assert pops[densest, 1] == rate
+    assert np.all(pops[:, 0] <= 1.0)
+
+
 def test_monitor_total_and_split_removal():
     """The monitor's total is sum |pulse|^2 for an envelope built as fft(pulse) (the
     reference's docstring identity, ``monitor.py:536-546``); the split's removal equals
