@@ -57,12 +57,14 @@ def trial_bytes(problem, order):
     qt = 8 * Q * T
     tiles = (T - 1 + 62) // 63
     qk = 8 * Q * tiles
+    groups = (Q + 31) // 32  # node groups of the fused passes
     table = 8 * Q * C
     comp = n_phi > 1
     dense = C > 8
     plasma = problem.plasma is not None
-    # The band rule's rate pass also stores the within-tile exposure.
-    band = plasma and len(problem.plasma.taps) > 1
+    # The band rule's rate pass also stores the within-tile exposure, and so does the fused
+    # route's under either rule (with one zero-tile flag a node and tile).
+    band = plasma and (len(problem.plasma.taps) > 1 or not dense)
     n_streams = 3 if plasma else 1
     field = qt if dense else ct + table  # what a node pass reads to know E at its nodes
     raman = problem.raman is not None
@@ -78,10 +80,17 @@ def trial_bytes(problem, order):
         "inverse_fft": 2 * ct,
         "synthesis_gemm": table + ct + qt if dense else 0,
         "raman": field + qt2 * 7 if raman else 0,
-        "plasma_rate": field + qt + qk + (qt if band else 0),
+        "plasma_rate": field + qt + qk + (qt if band else 0) + (Q * tiles if plasma and not dense else 0),
         "plasma_tile_prefix": 2 * qk,
         "plasma_current": field + qt + 2 * qk + 2 * qt,
         "plasma_tile_prefix_current": 2 * qk,
+        # The fused-node route (at most 8 modes) with plasma: the free-current pass fused
+        # with the response reads the exposure and rate (and the offsets and zero-tile
+        # flags) and writes each node group's 2C projected rows and C projected tile
+        # totals; the combination reads those and writes the 3C streams.
+        "plasma_current_response": field + 2 * qt + qk + Q * tiles + groups * (2 * ct + 8 * C * tiles)
+        + (qt if raman else 0),
+        "plasma_combine": groups * (2 * ct + 8 * C * tiles) + 2 * 8 * T + 3 * ct,
         "node_response": (qt + (3 * qt + qk if plasma else 0) + (2 if plasma else 1) * qt)
         + (qt if raman else 0)
         if dense else ct + 2 * table + 3 * qt + qk + 2 * 8 * T + 3 * ct + (qt if raman else 0),

@@ -439,6 +439,23 @@ __device__ inline double current_sample(const DeviceProblem& P, const FieldBuffe
   return (f * P.plasma_cmr) * field_at<C>(P, B, am, q, t);
 }
 
+// A zero in device memory that nothing writes: loads a flag redirects read it.
+__device__ double kZeroSample = 0.0;
+
+// The same where the rate pass ran sparse: a flagged tile's within-tile exposure is zero.
+template <int C>
+__device__ inline double current_sample_sparse(const DeviceProblem& P, const FieldBuffers& B,
+                                               int q, int t, double f0, int tiles) {
+  const int owner = min(t / kTileSpan, tiles - 1);
+  const long at = static_cast<long>(q) * tiles + owner;
+  const double* local = B.zero_tiles[at] ? &kZeroSample : B.x_local + static_cast<long>(q) * P.T + t;
+  const double x = B.tile_x[at] + *local;
+  const double f = fraction_of(f0, x > 0.0 ? x : 0.0);
+  double am[held(C)];
+  load_modal<C>(B, P.T, t, am);
+  return (f * P.plasma_cmr) * field_at<C>(P, B, am, q, t);
+}
+
 // Point reflection past the grid's ends: y(-m) = 2 y(0) - y(m), y(T-1+m) = 2 y(T-1) -
 // y(T-1-m) (ionisation.py:642-644). Samples past T-1+K lie outside every owned
 // sample's stencil and read as zero.
@@ -467,6 +484,17 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Whether a node's whole filter window is zero: every lane's own value and the node's 2K
+// halo values (wave-uniform). With a positive centre tap the filter of such a window is
+// +0.0 at every lane, and so are the trapezoid increments and their scans, so a pass
+// may store those zeros without forming them (the same bits): away from the pulse the
+// rate vanishes, and before it the free-current integrand.
+__device__ inline bool window_is_zero(const PlasmaTable& pt, double own,
+                                      const double* halo_row, int lane) {
+  const double h = lane < 2 * (pt.n_taps - 1) ? halo_row[lane] : 0.0;
+  return pt.taps[0] > 0.0 && __ballot(own != 0.0 || h != 0.0) == 0;
+}
+
 // The band passes run each wave's nodes as the trapezoid passes do (modal fields of the
 // lane's sample held in registers across them), with the samples a node needs beyond
 // its own lanes evaluated first for every node of the wave: the filter's halo of 2K
@@ -480,8 +508,9 @@ __device__ inline int extra_sample(int tile, int K, int T, int h) {
   return h < 2 * K ? halo_sample(tile, K, h) : T + (h - 2 * K);
 }
 
-// The band passes ask for six (rate: its LDS table allows three workgroups a CU) and
-// eight (current) waves a SIMD: at most 80 and 64 VGPRs, with no scratch.
+// Up to four modes the band passes ask for six (rate) and eight (current) waves a SIMD:
+// at most 80 and 64 VGPRs, with no scratch; more modes hold more registers and ask for
+// fewer waves, again with no scratch.
 
 // Pass A under the band rule: the raw rate stored for the owned samples, the filtered
 // rate's trapezoid increments scanned, the within-tile exposure stored and the tile's
@@ -495,14 +524,13 @@ struct BandLds {
 template <int C>
 __device__ __forceinline__ void rate_band_body(const DeviceProblem& P, const PlasmaTable& pt,
                                       const FieldBuffers& B, int tile, int group, double* table,
-                                      BandLds& lds) {
+                                      BandLds& lds, bool sparse) {
   auto& windows = lds.windows;
   auto& halos = lds.halos;
-  const bool in_lds = pt.n_intervals <= kLdsTableIntervals;
-  if (in_lds)
-    for (int i = threadIdx.x; i < 4 * pt.n_intervals; i += blockDim.x) table[i] = B.rate_c[i];
-  __syncthreads();
-  const double* coefficients = in_lds ? table : B.rate_c;
+  (void)table;
+  // The rate table is read through the vector caches: most workgroups lie where the field
+  // is below the table's threshold and never read it, so no workgroup waits to stage it.
+  const double* coefficients = B.rate_c;
   const int T = P.T, tiles = scan_tiles(T), K = pt.n_taps - 1, H = 2 * K;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const int t = tile * kTileSpan + lane;
@@ -512,6 +540,10 @@ __device__ __forceinline__ void rate_band_body(const DeviceProblem& P, const Pla
   double* halo = halos[wave];
   const int q0 = group * kNodeWaves * kNodesPerWave + wave;
   const int S = H + (tile == tiles - 1 ? K : 0);  // extra samples a node needs
+  // The lane's own modal fields first, so that their loads overlap the halo's (up to four
+  // modes: more held across the halo would spill).
+  double am[held(C)];
+  if constexpr (C <= 4) load_modal<C>(B, T, t, am);
   for (int idx = lane; idx < kNodesPerWave * S; idx += kWave) {
     const int i = idx / S, h = idx - i * S;
     const int q = q0 + i * kNodeWaves;
@@ -523,8 +555,7 @@ __device__ __forceinline__ void rate_band_body(const DeviceProblem& P, const Pla
     halo[i * kBandSlots + h] = v;
   }
   wave_sync();
-  double am[held(C)];
-  load_modal<C>(B, T, t, am);
+  if constexpr (C > 4) load_modal<C>(B, T, t, am);
   for (int i = 0; i < kNodesPerWave; ++i) {
     const int q = q0 + i * kNodeWaves;
     if (q >= P.Q) break;  // uniform across the wave, and so for every later node
@@ -533,12 +564,19 @@ __device__ __forceinline__ void rate_band_body(const DeviceProblem& P, const Pla
       w = rate_of(fabs(field_at<C>(P, B, am, q, t)), pt, coefficients);
     else if (t - T < K)  // the last tile's lanes past the end: their reflections
       w = halo[i * kBandSlots + H + (t - T)];
+    const long row = static_cast<long>(q) * T;
+    const bool zero = window_is_zero(pt, w, halo + i * kBandSlots, lane);
+    if (B.zero_tiles && lane == 0) B.zero_tiles[static_cast<long>(q) * tiles + tile] = zero;
+    if (stores && !(zero && sparse)) B.rate[row + t] = w;
+    if (zero) {
+      if (lane == kTileSpan - 1) B.tile_x[static_cast<long>(q) * tiles + tile] = 0.0;
+      if (stores && !sparse) B.x_local[row + t] = 0.0;
+      continue;
+    }
     wave_sync();  // the previous node's window has been read
     window[K + lane] = w;
     if (lane < H) window[halo_slot(K, lane)] = halo[i * kBandSlots + lane];
     wave_sync();
-    const long row = static_cast<long>(q) * T;
-    if (stores) B.rate[row + t] = w;
     // Cross-lane moves with every lane of the wave active: a lane masked off reads as
     // zero to its neighbours.
     const double y = band_filter(pt, window, lane);
@@ -552,11 +590,11 @@ __device__ __forceinline__ void rate_band_body(const DeviceProblem& P, const Pla
 }
 
 template <int C>
-__global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_waves_per_eu(6)))
-    plasma_rate_band(DeviceProblem P, PlasmaTable pt, FieldBuffers B) {
-  __shared__ double table[4 * kLdsTableIntervals];
+__global__ void __launch_bounds__(kWave * kNodeWaves)
+    __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4)))
+    plasma_rate_band(DeviceProblem P, PlasmaTable pt, FieldBuffers B, int sparse) {
   __shared__ BandLds lds;
-  rate_band_body<C>(P, pt, B, blockIdx.x, blockIdx.y, table, lds);
+  rate_band_body<C>(P, pt, B, blockIdx.x, blockIdx.y, nullptr, lds, sparse != 0);
 }
 
 // Pass C under the band rule: the free-current integrand (the fraction from the stored
@@ -624,11 +662,249 @@ __device__ __forceinline__ void current_band_body(const DeviceProblem& P, const 
 }
 
 template <int C>
-__global__ void __launch_bounds__(kWave * kNodeWaves) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ void __launch_bounds__(kWave * kNodeWaves)
+    __attribute__((amdgpu_waves_per_eu(C <= 4 ? 8 : (C < 8 ? 5 : 4))))
     plasma_current_band(DeviceProblem P, PlasmaTable pt, const AdamsRecord* __restrict__ R,
                         FieldBuffers B) {
   __shared__ BandLds lds;
   current_band_body<C>(P, pt, R, B, blockIdx.x, blockIdx.y, lds);
+}
+
+// ------------------------------ the free-current pass fused with the node response
+//
+// On the fused-node route the free current F(q, t) = F_off(q, tile) + F_local(q, t)
+// enters the streams only through the projection Σ_q W[m, q] F(q, t), which is linear:
+// the fused pass projects each node's tile-local current (with the loss current and the
+// polarisation, which need no offset) over the group's nodes as it scans, and projects
+// the nodes' tile totals; the combination adds Σ_{j < tile} of the projected totals
+// once every tile's total is known. No per-node array is written after the exposure.
+
+// Pass C fused with pass E: the free-current integrand filtered and scanned as in the
+// band pass (either rule: the trapezoid is one tap), then at the lane's sample the
+// polarisation (Kerr and Raman) and the current F_local + I_p W (1 - f) E / |E|², each
+// projected onto the modes and summed over the wave's nodes in order, then over the
+// waves in order (`red`, 2C rows of kWave for each wave). The group's rows of B.group_streams receive
+// the sums at the tile's owned samples, and B.tile_p its nodes' free-current tile
+// totals projected onto the modes (nodes in order; `totals` holds kNodesPerGroup).
+template <int C>
+__device__ __forceinline__ void current_response_body(const DeviceProblem& P,
+                                                      const PlasmaTable& pt,
+                                                      const AdamsRecord* __restrict__ R,
+                                                      const FieldBuffers& B, int tile,
+                                                      int group, BandLds& lds, double* red,
+                                                      double* totals) {
+  auto& windows = lds.windows;
+  auto& halos = lds.halos;
+  const int T = P.T, tiles = scan_tiles(T), K = pt.n_taps - 1, H = 2 * K;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int t = tile * kTileSpan + lane;
+  const int owner = min(t / kTileSpan, tiles - 1);
+  const bool stores = t < T && (lane < kTileSpan || t == T - 1);
+  const bool step = lane < kTileSpan && t + 1 < T;
+  const double f0 = R->preionisation;
+  const double density = R->density;
+  const double kerr = P.has_kerr ? P.kerr_coefficient * density : 0.0;
+  double* window = windows[wave];
+  double* halo = halos[wave];
+  const int base = group * kNodesPerGroup;
+  const int q0 = base + wave;
+  const int S = H + (tile == tiles - 1 ? K : 0);  // extra samples a node needs
+  // Every node's exposure and rate at the lane's sample are loaded before the halo is
+  // evaluated, so that their latency overlaps it (the node loop's LDS fences would
+  // otherwise hold each node's loads until the previous node is done).
+  // A tile the rate pass flagged holds no stores: its rate and within-tile exposure are
+  // zero, and the exposure there is the tile's offset (x + 0.0 is x).
+  // No load sits under a branch (a branch would make each wait for the one before): the
+  // flags and offsets are loaded from clamped addresses, all together; then the stored
+  // tiles' entries, where a flagged tile reads a constant zero instead.
+  const int tc = min(t, T - 1);
+  unsigned char zero[kNodesPerWave];
+  double xs[kNodesPerWave], ws[kNodesPerWave];
+#pragma unroll
+  for (int i = 0; i < kNodesPerWave; ++i) {
+    const long tq = static_cast<long>(min(q0 + i * kNodeWaves, P.Q - 1)) * tiles + owner;
+    zero[i] = B.zero_tiles[tq];
+    xs[i] = B.tile_x[tq];
+  }
+#pragma unroll
+  for (int i = 0; i < kNodesPerWave; ++i) {
+    const int q = q0 + i * kNodeWaves;
+    const bool here = q < P.Q && t < T;
+    const long at = static_cast<long>(min(q, P.Q - 1)) * T + tc;
+    const double* x_at = zero[i] ? &kZeroSample : B.x_local + at;
+    const double* w_at = zero[i] ? &kZeroSample : B.rate + at;
+    const double local = *x_at, w = *w_at;
+    xs[i] = here ? xs[i] + local : 0.0;  // a flagged tile's offset plus 0.0
+    ws[i] = here ? w : 0.0;
+  }
+  for (int idx = lane; idx < kNodesPerWave * S; idx += kWave) {
+    const int i = idx / S, h = idx - i * S;
+    const int q = q0 + i * kNodeWaves;
+    double v = 0.0;
+    if (q < P.Q) {
+      auto sample = [&](int u) { return current_sample_sparse<C>(P, B, q, u, f0, tiles); };
+      v = reflected(T, K, extra_sample(tile, K, T, h), sample);
+    }
+    halo[i * kBandSlots + h] = v;
+  }
+  wave_sync();
+  double am[held(C)];
+  load_modal<C>(B, T, t, am);
+  double acc_p[C], acc_c[C];
+#pragma unroll
+  for (int m = 0; m < C; ++m) acc_p[m] = acc_c[m] = 0.0;
+#pragma unroll
+  for (int i = 0; i < kNodesPerWave; ++i) {
+    const int q = q0 + i * kNodeWaves;
+    if (q >= P.Q) break;  // uniform across the wave, and so for every later node
+    double f = 0.0, e = 0.0, y = 0.0;
+    if (t < T) {
+      const double x = xs[i];
+      f = fraction_of(f0, x > 0.0 ? x : 0.0);
+      e = field_at<C>(P, B, am, q, t);
+      y = (f * P.plasma_cmr) * e;
+    } else if (t - T < K) {  // the last tile's lanes past the end: their reflections
+      y = halo[i * kBandSlots + H + (t - T)];
+    }
+    // Before the pulse the integrand vanishes: the scan is zero, and a zero of either sign
+    // leaves the projections as they are.
+    double local = 0.0, incl = 0.0;
+    if (!window_is_zero(pt, y, halo + i * kBandSlots, lane)) {
+      wave_sync();  // the previous node's window has been read
+      window[K + lane] = y;
+      if (lane < H) window[halo_slot(K, lane)] = halo[i * kBandSlots + lane];
+      wave_sync();
+      // Cross-lane moves with every lane of the wave active (see the rate pass).
+      const double yf = band_filter(pt, window, lane);
+      const double y_next = from_next_lane(yf);
+      const double inc = step ? (P.dt_fine * (y_next + yf)) / 2.0 : 0.0;
+      incl = wave_scan(inc);
+      local = from_previous_lane(incl);
+    }
+    if (lane == kTileSpan - 1) totals[i * kNodeWaves + wave] = incl;
+    if (t < T) {
+      const double w = ws[i];
+      const double squared = fabs(e) * fabs(e);
+      double loss = 0.0;
+      if (w > 0.0 && squared > 0.0) loss = (P.plasma_ip * w * (1.0 - f)) / squared;
+      const double current = local + loss * e;
+      double pol = (kerr * (e * e)) * e;
+      if (P.n_raman > 0) {
+        const double* conv = reinterpret_cast<const double*>(B.raman_drive);
+        pol = pol + ((P.raman_epsilon0 * density) * e) * (conv[2L * q * T + t] * P.inv_2T);
+      }
+#pragma unroll
+      for (int m = 0; m < C; ++m) {
+        const double wmq = P.projection[static_cast<long>(m) * P.Q + q];
+        acc_p[m] = acc_p[m] + wmq * pol;
+        acc_c[m] = acc_c[m] + wmq * current;
+      }
+    }
+  }
+  // Each wave's projections to its own rows of `red`, then the waves' sums in wave order.
+  const int waves = static_cast<int>(blockDim.x) / kWave;
+#pragma unroll
+  for (int m = 0; m < C; ++m) {
+    red[(wave * 2 * C + m) * kWave + lane] = acc_p[m];
+    red[(wave * 2 * C + C + m) * kWave + lane] = acc_c[m];
+  }
+  __syncthreads();
+  if (stores)
+    for (int j = wave; j < 2 * C; j += waves) {
+      double sum = red[j * kWave + lane];
+      for (int w = 1; w < waves; ++w) sum = sum + red[(w * 2 * C + j) * kWave + lane];
+      B.group_streams[(static_cast<long>(group) * 2 * C + j) * T + t] = sum;
+    }
+  if (static_cast<int>(threadIdx.x) < C) {
+    const int m = threadIdx.x;
+    double s = 0.0;
+    for (int n = 0; n < kNodesPerGroup && base + n < P.Q; ++n)
+      s = s + P.projection[static_cast<long>(m) * P.Q + base + n] * totals[n];
+    B.tile_p[(static_cast<long>(group) * C + m) * tiles + tile] = s;
+  }
+}
+
+// Five waves a SIMD up to four modes (at most 96 VGPRs): the 32 waves of each of T/63
+// tiles then take two rounds of the machine at 2^14 samples rather than three.
+template <int C>
+__global__ void __launch_bounds__(kWave * kNodeWaves)
+    __attribute__((amdgpu_waves_per_eu(C <= 4 ? 5 : 4)))
+    plasma_current_response(DeviceProblem P, PlasmaTable pt, const AdamsRecord* __restrict__ R,
+                            FieldBuffers B) {
+  __shared__ BandLds lds;
+  __shared__ double red[kNodeWaves * 2 * C * kWave];
+  __shared__ double totals[kNodesPerGroup];
+  current_response_body<C>(P, pt, R, B, blockIdx.x, blockIdx.y, lds, red, totals);
+}
+
+// The sum of a double over the wave's 64 lanes in a fixed order, in every lane.
+__device__ inline double wave_total(double v) {
+  const double s = wave_scan(v);
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(s), kWave - 1),
+                          __builtin_amdgcn_readlane(__double2loint(s), kWave - 1));
+}
+
+// The combination over one block of 64 samples: the projected free-current offset of
+// each mode at the block's (at most two) tiles, Σ_{j < k} Σ_g tile_p[g, m, j] (lane l
+// sums the tiles j ≡ l mod 64 in order, each tile's groups in order, then the wave's
+// total in a fixed order, so every block that needs tile k forms the same value); then at
+// each sample the groups' projections summed in order, the offset added to the current,
+// and the streams written: g P, g ρ J, g' ρ J. Any number of waves; `offsets` holds 2C.
+template <int C>
+__device__ __forceinline__ void combine_body(const DeviceProblem& P,
+                                             const AdamsRecord* __restrict__ R,
+                                             const FieldBuffers& B, int block,
+                                             double* offsets) {
+  const int T = P.T, tiles = scan_tiles(T), G = node_groups(P.Q);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int waves = static_cast<int>(blockDim.x) / kWave;
+  const int t0 = block * kWave;
+  const int k0 = min(t0 / kTileSpan, tiles - 1);
+  // The second tile's sums are the first's with tile k0 added last by its lane, which is
+  // where the direct sum over j < k0 + 1 adds it: the same value either way.
+  const int k1 = min(k0 + 1, tiles - 1);
+  for (int m = wave; m < C; m += waves) {
+    double s = 0.0;
+    for (int j = lane; j < k0; j += kWave) {
+      double tj = 0.0;
+      for (int g = 0; g < G; ++g) tj = tj + B.tile_p[(static_cast<long>(g) * C + m) * tiles + j];
+      s = s + tj;
+    }
+    double s1 = s;
+    if (k1 > k0 && lane == k0 % kWave) {
+      double tj = 0.0;
+      for (int g = 0; g < G; ++g) tj = tj + B.tile_p[(static_cast<long>(g) * C + m) * tiles + k0];
+      s1 = s + tj;
+    }
+    const double o0 = wave_total(s), o1 = wave_total(s1);
+    if (lane == 0) {
+      offsets[m] = o0;
+      offsets[C + m] = o1;
+    }
+  }
+  __syncthreads();
+  const int t = t0 + lane;  // < T: the grid is T/64 blocks
+  const int kk = min(t / kTileSpan, tiles - 1) - k0;
+  const double g = P.t_window[t], gd = P.t_deriv[t], density = R->density;
+  double* streams = reinterpret_cast<double*>(B.streams);
+  for (int m = wave; m < C; m += waves) {
+    double sp = 0.0, sc = 0.0;
+    for (int gi = 0; gi < G; ++gi) {
+      sp = sp + B.group_streams[(static_cast<long>(gi) * 2 * C + m) * T + t];
+      sc = sc + B.group_streams[(static_cast<long>(gi) * 2 * C + C + m) * T + t];
+    }
+    const double current = density * (sc + offsets[kk * C + m]);
+    streams[static_cast<long>(m) * T + t] = g * sp;
+    streams[static_cast<long>(C + m) * T + t] = g * current;
+    streams[static_cast<long>(2 * C + m) * T + t] = gd * current;
+  }
+}
+
+template <int C>
+__global__ void __launch_bounds__(kWave * 4)
+    plasma_response_combine(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
+  __shared__ double offsets[2 * C];
+  combine_body<C>(P, R, B, blockIdx.x, offsets);
 }
 
 // Passes B and D: exclusive prefix of the tile sums of each node, one wavefront per
@@ -1236,8 +1512,11 @@ template <int C>
 struct RateAt {
   __device__ __forceinline__ static void run(const BatchInstance& in, int tile, int group, double* table,
                              BandLds& lds) {
-    if (in.pt.n_taps > 1)
-      rate_band_body<C>(in.P, in.pt, in.B, tile, group, table, lds);
+    // The fused passes read the within-tile exposure, which the band pass stores (under
+    // either rule: one tap is the trapezoid).
+    const bool fused = fused_node_passes(in.P, in.B);
+    if (in.pt.n_taps > 1 || fused)
+      rate_band_body<C>(in.P, in.pt, in.B, tile, group, table, lds, fused);
     else
       rate_tiles_body<C>(in.P, in.pt, in.B, tile, group, table);
   }
@@ -1246,8 +1525,10 @@ struct RateAt {
 template <int C>
 struct CurrentAt {
   __device__ __forceinline__ static void run(const BatchInstance& in, const AdamsRecord* __restrict__ R,
-                             int tile, int group, BandLds& lds) {
-    if (in.pt.n_taps > 1)
+                             int tile, int group, BandLds& lds, double* red, double* totals) {
+    if (fused_node_passes(in.P, in.B))
+      current_response_body<C>(in.P, in.pt, R, in.B, tile, group, lds, red, totals);
+    else if (in.pt.n_taps > 1)
       current_band_body<C>(in.P, in.pt, R, in.B, tile, group, lds);
     else
       current_tiles_body<C>(in.P, R, in.B, tile, group);
@@ -1258,7 +1539,10 @@ template <int C>
 struct ResponseAt {
   __device__ __forceinline__ static void run(const BatchInstance& in, const AdamsRecord* __restrict__ R,
                              int block, double* red) {
-    node_response_body<C>(in.P, R, in.B, block, red);
+    if (fused_node_passes(in.P, in.B))
+      combine_body<C>(in.P, R, in.B, block, red);
+    else
+      node_response_body<C>(in.P, R, in.B, block, red);
   }
 };
 
@@ -1303,11 +1587,13 @@ __global__ void __launch_bounds__(kWave) batch_prefix(SP_BATCH_ARGS, int current
 
 __global__ void __launch_bounds__(kWave * kNodeWaves) batch_current(SP_BATCH_ARGS) {
   __shared__ BandLds lds;
+  __shared__ double red[kNodeWaves * 2 * kMaxBatchModes * kWave];
+  __shared__ double totals[kNodesPerGroup];
   const Owned o = owned(insts, records, first, n_inst);
   const BatchInstance& in = *o.inst;
   if (o.R->action == kBatchIdle) return;
   const int tiles = scan_tiles(in.P.T);
-  with_modes<CurrentAt>(in.P.C, in, o.R, o.local % tiles, o.local / tiles, lds);
+  with_modes<CurrentAt>(in.P.C, in, o.R, o.local % tiles, o.local / tiles, lds, red, totals);
 }
 
 __global__ void __launch_bounds__(kWave * kResponseWaves) batch_response(SP_BATCH_ARGS) {
@@ -1503,7 +1789,7 @@ void launch_node_passes_for(const DeviceProblem& P, const AdamsRecord* R,
     SP_CHECK(!band || (B.x_local && pt.n_taps <= kMaxBandHalf + 1 && 64 + 2 * (pt.n_taps - 1) <= kBandSpan),
              "the band rule needs its exposure buffer and at most 8 taps each side");
     if (band)
-      hipLaunchKernelGGL(plasma_rate_band<C>, node_grid, node_block, 0, s, P, pt, B);
+      hipLaunchKernelGGL(plasma_rate_band<C>, node_grid, node_block, 0, s, P, pt, B, 0);
     else
       hipLaunchKernelGGL(plasma_rate_tiles<C>, node_grid, node_block, 0, s, P, pt, B);
     check();
@@ -1555,6 +1841,54 @@ void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const Plas
     case 6: launch_node_passes_for<6>(P, R, pt, B, s, marks); break;
     case 7: launch_node_passes_for<7>(P, R, pt, B, s, marks); break;
     default: launch_node_passes_for<8>(P, R, pt, B, s, marks); break;
+  }
+}
+
+namespace {
+template <int C>
+void launch_node_passes_fused_for(const DeviceProblem& P, const AdamsRecord* R,
+                                  const PlasmaTable& pt, FieldBuffers B, hipStream_t s,
+                                  hipEvent_t* marks) {
+  const int tiles = scan_tiles(P.T);
+  const dim3 node_grid(tiles, node_groups(P.Q));
+  const dim3 node_block(kWave * kNodeWaves);
+  auto mark = [&](int i) {
+    if (marks) SP_HIP(hipEventRecord(marks[i], s));
+  };
+  hipLaunchKernelGGL(plasma_rate_band<C>, node_grid, node_block, 0, s, P, pt, B, 1);
+  check();
+  mark(0);
+  hipLaunchKernelGGL(plasma_tile_prefix, dim3(P.Q), dim3(kWave), 0, s, tiles, B.tile_x);
+  check();
+  mark(1);
+  hipLaunchKernelGGL(plasma_current_response<C>, node_grid, node_block, 0, s, P, pt, R, B);
+  check();
+  mark(2);
+  hipLaunchKernelGGL(plasma_response_combine<C>, dim3(P.T / kWave), dim3(kWave * 4), 0, s, P,
+                     R, B);
+  check();
+  mark(3);
+  mark(4);
+}
+}  // namespace
+
+void launch_node_passes_fused(const DeviceProblem& P, const AdamsRecord* R,
+                              const PlasmaTable& pt, FieldBuffers B, hipStream_t s,
+                              hipEvent_t* marks) {
+  SP_CHECK(fused_node_passes(P, B), "the fused node passes need plasma, 1..8 modes, the "
+                                    "exposure buffer and the group streams");
+  SP_CHECK(P.T >= 2 && P.T % kWave == 0, "the fused node passes take 64-sample blocks");
+  SP_CHECK(pt.n_taps >= 1 && pt.n_taps <= kMaxBandHalf + 1,
+           "the running-integral rule has 1 to 9 taps");
+  switch (P.C) {
+    case 1: launch_node_passes_fused_for<1>(P, R, pt, B, s, marks); break;
+    case 2: launch_node_passes_fused_for<2>(P, R, pt, B, s, marks); break;
+    case 3: launch_node_passes_fused_for<3>(P, R, pt, B, s, marks); break;
+    case 4: launch_node_passes_fused_for<4>(P, R, pt, B, s, marks); break;
+    case 5: launch_node_passes_fused_for<5>(P, R, pt, B, s, marks); break;
+    case 6: launch_node_passes_fused_for<6>(P, R, pt, B, s, marks); break;
+    case 7: launch_node_passes_fused_for<7>(P, R, pt, B, s, marks); break;
+    default: launch_node_passes_fused_for<8>(P, R, pt, B, s, marks); break;
   }
 }
 

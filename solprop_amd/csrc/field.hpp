@@ -92,7 +92,22 @@ struct FieldBuffers {
   double* norms;    // (1 + kMaxScored) their totals, in the host's mapped memory
   const cplx* roots_T;  // unit roots of T
   const double* rate_c; // (4, n_intervals) rate table
+  // The fused current-and-response pass (fused-node route with plasma): each node
+  // group's projections of its nodes' polarisation and tile-local current, and each
+  // group's projection of its nodes' free-current tile totals.
+  double* group_streams;  // (G, 2C, T), G = ceil(Q / 32)
+  double* tile_p;         // (G, C, tiles)
+  // (Q, tiles): 1 where the band rate pass found a node's whole filter window zero; its
+  // rate and within-tile exposure are then zero, and a sparse rate pass does not store
+  // them (the fused passes read the flag instead).
+  unsigned char* zero_tiles;
 };
+
+// Nodes one workgroup of the band passes runs (kNodeWaves waves of kNodesPerWave).
+constexpr int kNodesPerGroup = kNodeWaves * kNodesPerWave;
+__host__ __device__ inline int node_groups(int Q) {
+  return (Q + kNodesPerGroup - 1) / kNodesPerGroup;
+}
 
 constexpr int kMaxBandHalf = 8;  // most taps each side of the band rule's filter
 
@@ -190,6 +205,23 @@ void launch_field_scatter_hilbert(const DeviceProblem& P, const cplx* spectrum, 
 // without plasma records the first four back to back).
 void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,
                         FieldBuffers B, hipStream_t s, hipEvent_t* marks = nullptr);
+// Whether the fused node passes apply: the fused-node route (1..8 modes) with plasma and
+// the buffers they write (the band passes' exposure, the group streams and totals).
+__host__ __device__ inline bool fused_node_passes(const DeviceProblem& P, const FieldBuffers& B) {
+  return P.has_plasma && !B.node_field && B.group_streams && B.tile_p && B.x_local &&
+         B.zero_tiles && P.C >= 1 && P.C <= kMaxModes;
+}
+// The same streams from four launches and no (Q, T) array beyond the rate and the
+// within-tile exposure: the rate pass (either rule; the trapezoid is the band rule with
+// one tap), the exposure prefix, the free-current pass fused with the node response
+// (each workgroup's projection of its nodes' polarisation and tile-local current, and of
+// their free-current tile totals), and the combination, which adds every tile's
+// projected free-current offset, sums the groups in order and applies the collars and
+// the density. The stored fraction and free current are not written. `marks` as above
+// (the fourth after the combination, the fifth at once).
+void launch_node_passes_fused(const DeviceProblem& P, const AdamsRecord* R,
+                              const PlasmaTable& pt, FieldBuffers B, hipStream_t s,
+                              hipEvent_t* marks = nullptr);
 void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                          cplx* out, hipStream_t s);
 // The corrector, writing its block sums to B.partials; then their reduction into B.norms.

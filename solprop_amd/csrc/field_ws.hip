@@ -214,7 +214,16 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   B_.free_local = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
   B_.tile_x = alloc<double>(static_cast<std::size_t>(Q) * tiles);
   B_.tile_f = alloc<double>(static_cast<std::size_t>(Q) * tiles);
-  if (d.has_plasma && d.n_taps > 1) B_.x_local = alloc<double>(static_cast<std::size_t>(Q) * T);
+  // The band passes' within-tile exposure; on the fused-node route the fused passes read
+  // it under either rule, and write each node group's projections and projected totals.
+  if (d.has_plasma && (d.n_taps > 1 || !dense_))
+    B_.x_local = alloc<double>(static_cast<std::size_t>(Q) * T);
+  if (d.has_plasma && !dense_) {
+    const std::size_t groups = static_cast<std::size_t>(node_groups(Q));
+    B_.group_streams = alloc<double>(groups * 2 * C * T);
+    B_.tile_p = alloc<double>(groups * C * tiles);
+    B_.zero_tiles = alloc<unsigned char>(static_cast<std::size_t>(Q) * tiles);
+  }
   if (dense_) {
     B_.node_field = alloc<double>(static_cast<std::size_t>(Q) * T);
     B_.node_streams = alloc<double>(static_cast<std::size_t>(Q) * (d.has_plasma ? 2 : 1) * T);
@@ -388,7 +397,10 @@ void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out, const Adams
   mark(4);
   enqueue_raman(spectrum, rec);
   mark(5);
-  launch_node_passes(P_, rec, pt_, B_, stream_, marking_ ? marks_ + 6 : nullptr);
+  if (fused_node_passes(P_, B_))
+    launch_node_passes_fused(P_, rec, pt_, B_, stream_, marking_ ? marks_ + 6 : nullptr);
+  else
+    launch_node_passes(P_, rec, pt_, B_, stream_, marking_ ? marks_ + 6 : nullptr);
   enqueue_projection();
   mark(11);
   enqueue_absorbers();
@@ -494,6 +506,7 @@ bool FieldWorkspace::phase_used(int i) const {
     case 1: return false;  // the packing rides the inverse transform's first pass
     case 4: return P_.n_raman > 0;
     case 5: case 6: case 7: case 8: return P_.has_plasma != 0;
+    case 9: return !fused_node_passes(P_, B_);  // the fused passes have no response of its own
     case 11: return absorbing();
     default: return true;
   }
@@ -501,9 +514,16 @@ bool FieldWorkspace::phase_used(int i) const {
 
 std::vector<std::pair<const char*, double>> FieldWorkspace::phases() const {
   std::vector<std::pair<const char*, double>> out;
-  for (int i = 0; i < kPhases; ++i)
-    if (phase_used(i))
-      out.emplace_back(kPhaseNames[i], timed_ ? phase_ms_[i] * 1e3 / timed_ : 0.0);
+  const bool fused = fused_node_passes(P_, B_);
+  for (int i = 0; i < kPhases; ++i) {
+    if (!phase_used(i)) continue;
+    // The fused passes' third and fourth launches: the free current with the response,
+    // and the combination.
+    const char* name = kPhaseNames[i];
+    if (fused && i == 7) name = "plasma_current_response";
+    if (fused && i == 8) name = "plasma_combine";
+    out.emplace_back(name, timed_ ? phase_ms_[i] * 1e3 / timed_ : 0.0);
+  }
   return out;
 }
 
@@ -876,7 +896,10 @@ void FieldWorkspace::source(double z, const double* lab, double* out, double* st
                         scratch_, +1, stream_);
   enqueue_synthesis();
   enqueue_raman(B_.lab, rec_dev_);
+  // The unfused passes leave every node's rate, fraction and free current for inspection
+  // (buffer()); the streams are then those of the fused passes where the trial runs them.
   launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
+  if (fused_node_passes(P_, B_)) launch_node_passes_fused(P_, rec_dev_, pt_, B_, stream_);
   enqueue_projection();
   enqueue_absorbers();
   SP_HIP(hipStreamSynchronize(stream_));

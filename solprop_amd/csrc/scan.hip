@@ -176,6 +176,7 @@ void Scan::plan_launches() {
   inst_dev_ = device_upload(allocations_, host.data(), host.size());
 
   std::vector<int> first;
+  std::vector<char> fused(static_cast<std::size_t>(n), 0);
   auto table = [&](Group& g, auto blocks_of) {
     g.offset = static_cast<int>(first.size());
     int total = 0;
@@ -196,6 +197,11 @@ void Scan::plan_launches() {
   table(rate_, node_blocks);
   table(current_, node_blocks);
   table(prefix_, [&](const DeviceProblem& P) { return P.has_plasma ? P.Q : 0; });
+  for (int i = 0; i < n; ++i) fused[i] = fused_node_passes(host[i].P, host[i].B);
+  int at = 0;
+  table(prefix_current_, [&](const DeviceProblem& P) {
+    return P.has_plasma && !fused[at++] ? P.Q : 0;
+  });
   table(response_, [&](const DeviceProblem& P) {
     SP_CHECK(P.T % kWave == 0, "the node response takes 64-sample blocks");
     return P.T / kWave;
@@ -382,7 +388,7 @@ void Scan::enqueue_round() {
   launch_batch_rate(grid(rate_), stream_);
   launch_batch_prefix(grid(prefix_), false, stream_);
   launch_batch_current(grid(current_), stream_);
-  launch_batch_prefix(grid(prefix_), true, stream_);
+  launch_batch_prefix(grid(prefix_current_), true, stream_);
   launch_batch_response(grid(response_), stream_);
   for (const FftGroup& g : forward_)
     launch_fft_grouped(g.log2L, g.log2G, g.jobs, g.first, g.n, g.blocks, -1, stream_);

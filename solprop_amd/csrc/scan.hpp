@@ -110,7 +110,10 @@ class Scan {
   BatchInstance* inst_dev_ = nullptr;
   int* first_dev_ = nullptr;
   std::vector<void*> allocations_;
-  Group commit_, predict_, rate_, prefix_, current_, response_, finish_, correct_, norms_;
+  // prefix_current_: the free-current tile prefix of the instances that do not run the
+  // fused node passes (which need none).
+  Group commit_, predict_, rate_, prefix_, current_, prefix_current_, response_, finish_,
+      correct_, norms_;
   std::vector<FftGroup> inverse_, forward_;
   hipGraphExec_t graph_ = nullptr;
   hipEvent_t done_ = nullptr;
