@@ -850,6 +850,8 @@ __device__ inline double wave_total(double v) {
 // total in a fixed order, so every block that needs tile k forms the same value); then at
 // each sample the groups' projections summed in order, the offset added to the current,
 // and the streams written: g P, g ρ J, g' ρ J. Any number of waves; `offsets` holds 2C.
+constexpr int kCombineRun = 8;  // tiles a lane of the combination holds (up to 512 tiles)
+
 template <int C>
 __device__ __forceinline__ void combine_body(const DeviceProblem& P,
                                              const AdamsRecord* __restrict__ R,
@@ -864,17 +866,41 @@ __device__ __forceinline__ void combine_body(const DeviceProblem& P,
   // where the direct sum over j < k0 + 1 adds it: the same value either way.
   const int k1 = min(k0 + 1, tiles - 1);
   for (int m = wave; m < C; m += waves) {
-    double s = 0.0;
-    for (int j = lane; j < k0; j += kWave) {
-      double tj = 0.0;
-      for (int g = 0; g < G; ++g) tj = tj + B.tile_p[(static_cast<long>(g) * C + m) * tiles + j];
-      s = s + tj;
-    }
-    double s1 = s;
-    if (k1 > k0 && lane == k0 % kWave) {
-      double tj = 0.0;
-      for (int g = 0; g < G; ++g) tj = tj + B.tile_p[(static_cast<long>(g) * C + m) * tiles + k0];
-      s1 = s + tj;
+    double s = 0.0, s1;
+    if (tiles <= kCombineRun * kWave) {  // uniform: each lane's tiles held in registers
+      // Every (group, tile) entry is loaded before any is added (clamped addresses, no
+      // branch), a group at a time; then the same additions in the same order.
+      double v[kCombineRun];
+#pragma unroll
+      for (int r = 0; r < kCombineRun; ++r) v[r] = 0.0;
+      for (int g = 0; g < G; ++g) {
+        const double* row = B.tile_p + (static_cast<long>(g) * C + m) * tiles;
+        double u[kCombineRun];
+#pragma unroll
+        for (int r = 0; r < kCombineRun; ++r) u[r] = row[min(lane + r * kWave, tiles - 1)];
+#pragma unroll
+        for (int r = 0; r < kCombineRun; ++r) v[r] = v[r] + u[r];
+      }
+      double extra = 0.0;
+#pragma unroll
+      for (int r = 0; r < kCombineRun; ++r) {
+        const int j = lane + r * kWave;
+        if (j < k0) s = s + v[r];
+        if (j == k0) extra = v[r];
+      }
+      s1 = k1 > k0 && lane == k0 % kWave ? s + extra : s;
+    } else {
+      for (int j = lane; j < k0; j += kWave) {
+        double tj = 0.0;
+        for (int g = 0; g < G; ++g) tj = tj + B.tile_p[(static_cast<long>(g) * C + m) * tiles + j];
+        s = s + tj;
+      }
+      s1 = s;
+      if (k1 > k0 && lane == k0 % kWave) {
+        double tj = 0.0;
+        for (int g = 0; g < G; ++g) tj = tj + B.tile_p[(static_cast<long>(g) * C + m) * tiles + k0];
+        s1 = s + tj;
+      }
     }
     const double o0 = wave_total(s), o1 = wave_total(s1);
     if (lane == 0) {
