@@ -1,4 +1,4 @@
-// Stockham radix-4/2 FFT passes staged in LDS, FP64, for gfx950.
+// Stockham FFT passes staged in LDS (radix-16 stages in registers for large workgroups, radix-4/2 otherwise), FP64, for gfx950.
 #include "fft.hpp"
 #include "fft_lds.hpp"
 
@@ -27,11 +27,109 @@ __device__ inline cplx packed_point(const PassArgs& a, long g) {
 
 // One workgroup's pass: sequence group `group` of batch row `row`, staged in `buf`
 // (G * STRIDE points).
+// The pass's workgroup shape: radix-16 stages on padded rows for sequences of at least
+// 1024 points, radix-4 on plain rows otherwise.
+template <int LOG2L, int G>
+struct PassShape {
+  static constexpr bool r16 = Shape16<LOG2L, G>::use;
+  static constexpr int L = 1 << LOG2L;
+  static constexpr int E = L * G;
+  static constexpr int NT = r16 ? Shape16<LOG2L, G>::NT : Shape<LOG2L, G>::NT;
+  static constexpr int STRIDE = r16 ? Shape16<LOG2L, G>::STRIDE : Shape<LOG2L, G>::STRIDE;
+};
+
+// One radix-R stage of a radix-16 pass (see stage16 in fft_lds.hpp): its inputs from HBM
+// when it is the first stage (point j of sequence s at the pass's input strides, or
+// packed from the half spectrum), else from the padded LDS rows; its outputs to HBM when
+// it is the last stage (point k = i + j p, times the four-step twiddle where asked), else
+// back to LDS. Only LDS-to-LDS stages need a barrier between their reads and writes.
+template <int LOG2L, int G, int R, bool First, bool Last>
+__device__ __forceinline__ void pass_stage(const cplx* src, cplx* dst, const cplx* __restrict__ tw_l,
+                                           const cplx* __restrict__ tw_n, const PassArgs& a,
+                                           int sign, long s0, int row, cplx* buf, int p) {
+  using S = Shape16<LOG2L, G>;
+  constexpr int L = S::L, E = S::E, NT = S::NT, STRIDE = S::STRIDE;
+  constexpr int NB = E / R;
+  constexpr int PER = (NB + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  const int step = L / (p * R);
+  cplx v[PER][R];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int b = tid + r * NT;
+    if (b < NB) {
+      const int s = b / (L / R), i = b % (L / R);
+      const int k = i & (p - 1);
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int point = i + j * (L / R);
+        if constexpr (First) {
+          const long at = point * a.in_sj + (s0 + s) * a.in_ss;
+          v[r][j] = a.spec ? packed_point(a, static_cast<long>(row) * a.batch_in + at) : src[at];
+        } else {
+          v[r][j] = buf[s * STRIDE + padded(point)];
+        }
+      }
+      if (p > 1)
+#pragma unroll
+        for (int j = 1; j < R; ++j) v[r][j] = mul(v[r][j], twiddle(tw_l, j * k * step, sign));
+      dft<R>(v[r], sign);
+    }
+  }
+  if constexpr (!First && !Last) __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int b = tid + r * NT;
+    if (b < NB) {
+      const int s = b / (L / R), i = b % (L / R);
+      const int k = i & (p - 1);
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int point = (i - k) * R + k + j * p;
+        cplx value = v[r][out_at<R>(j)];
+        if constexpr (Last) {
+          if (a.global_twiddle) {
+            const long idx = ((s0 + s) * static_cast<long>(point)) & a.mask_n;
+            value = mul(value, twiddle(tw_n, static_cast<int>(idx), sign));
+          }
+          dst[point * a.out_sk + (s0 + s) * a.out_ss] = value;
+        } else {
+          buf[s * STRIDE + padded(point)] = value;
+        }
+      }
+    }
+  }
+  if constexpr (!Last) __syncthreads();
+}
+
+// A radix-16 pass: stages of 16 points (the first from HBM, the last to HBM) and one
+// radix-2, 4 or 8 stage for what remains of L.
+template <int LOG2L, int G, int Stage>
+__device__ __forceinline__ void r16_stages(const cplx* src, cplx* dst, const cplx* __restrict__ tw_l,
+                                           const cplx* __restrict__ tw_n, const PassArgs& a,
+                                           int sign, long s0, int row, cplx* buf, int p) {
+  constexpr int full = LOG2L / 4;
+  constexpr int rest = (1 << LOG2L) >> (4 * full);
+  constexpr int stages = full + (rest > 1 ? 1 : 0);
+  constexpr int R = Stage < full ? 16 : rest;
+  constexpr bool first = Stage == 0, last = Stage == stages - 1;
+  pass_stage<LOG2L, G, R, first, last>(src, dst, tw_l, tw_n, a, sign, s0, row, buf, p);
+  if constexpr (!last)
+    r16_stages<LOG2L, G, Stage + 1>(src, dst, tw_l, tw_n, a, sign, s0, row, buf, p * R);
+}
+
 template <int LOG2L, int G>
 __device__ __forceinline__ void fft_pass_body(const cplx* in, cplx* out, const cplx* __restrict__ tw_l,
                                      const cplx* __restrict__ tw_n, const PassArgs& a, int sign,
                                      int group, int row, cplx* buf) {
-  using S = Shape<LOG2L, G>;
+  using S = PassShape<LOG2L, G>;
+  if constexpr (S::r16) {
+    const long s0 = static_cast<long>(group) * G;
+    r16_stages<LOG2L, G, 0>(in + static_cast<long>(row) * a.batch_in,
+                            out + static_cast<long>(row) * a.batch_out, tw_l, tw_n, a, sign, s0,
+                            row, buf, 1);
+    return;
+  }
   constexpr int L = S::L, E = S::E, NT = S::NT, STRIDE = S::STRIDE;
   const int tid = threadIdx.x;
   const long s0 = static_cast<long>(group) * G;
@@ -76,11 +174,11 @@ __device__ __forceinline__ void fft_pass_body(const cplx* in, cplx* out, const c
 }
 
 template <int LOG2L, int G>
-__global__ void __launch_bounds__((Shape<LOG2L, G>::NT))
+__global__ void __launch_bounds__((PassShape<LOG2L, G>::NT))
     fft_pass_kernel(const cplx* in, cplx* out,
                     const cplx* __restrict__ tw_l, const cplx* __restrict__ tw_n,
                     PassArgs a, int sign) {
-  using S = Shape<LOG2L, G>;
+  using S = PassShape<LOG2L, G>;
   __shared__ cplx buf[G * S::STRIDE];
   fft_pass_body<LOG2L, G>(in, out, tw_l, tw_n, a, sign, blockIdx.x, blockIdx.y, buf);
 }
@@ -89,10 +187,10 @@ __global__ void __launch_bounds__((Shape<LOG2L, G>::NT))
 // jobs' first workgroups (a binary search, uniform across the workgroup) and runs that
 // job's sequence group and row.
 template <int LOG2L, int G>
-__global__ void __launch_bounds__((Shape<LOG2L, G>::NT))
+__global__ void __launch_bounds__((PassShape<LOG2L, G>::NT))
     fft_pass_grouped(const FftJob* __restrict__ jobs, const int* __restrict__ first, int n_jobs,
                      int sign) {
-  using S = Shape<LOG2L, G>;
+  using S = PassShape<LOG2L, G>;
   __shared__ cplx buf[G * S::STRIDE];
   const int block = blockIdx.x;
   const int i = owner_of(first, n_jobs, block);
@@ -105,9 +203,15 @@ __global__ void __launch_bounds__((Shape<LOG2L, G>::NT))
 
 using GroupedFn = void (*)(const FftJob*, const int*, int, int);
 
+// The pass shapes that exist: workgroups of at most kMaxElems points.
+template <int LOG2L, int G>
+constexpr bool pass_exists() {
+  return (1 << LOG2L) * G <= kMaxElems;
+}
+
 template <int LOG2L, int G>
 constexpr GroupedFn pick_grouped() {
-  if constexpr ((1 << LOG2L) * G <= kMaxElems)
+  if constexpr (pass_exists<LOG2L, G>())
     return fft_pass_grouped<LOG2L, G>;
   else
     return nullptr;
@@ -143,7 +247,7 @@ using KernelFn = void (*)(const cplx*, cplx*, const cplx*, const cplx*, PassArgs
 
 template <int LOG2L, int G>
 constexpr KernelFn pick() {
-  if constexpr ((1 << LOG2L) * G <= kMaxElems)
+  if constexpr (pass_exists<LOG2L, G>())
     return fft_pass_kernel<LOG2L, G>;
   else
     return nullptr;
@@ -182,6 +286,8 @@ KernelFn kernel_for(int log2l, int log2g) {
 
 int threads_for(int log2l, int log2g) {
   const int e = 1 << (log2l + log2g);
+  // The radix-16 shapes (Shape16::use): a thread a 16-point butterfly, at most 256.
+  if (log2l >= 10) return e / 16 >= 256 ? 256 : e / 16;
   const int nt = e / 4;
   return nt >= 256 ? 256 : (nt >= 1 ? nt : 1);
 }

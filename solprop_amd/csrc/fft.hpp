@@ -1,10 +1,12 @@
-// Batched FP64 FFTs for gfx950: Stockham radix-4/2 passes staged in LDS.
+// Batched FP64 FFTs for gfx950: Stockham passes staged in LDS (radix-16 or radix-4/2 stages).
 //
 // A transform of n = 2^m points is one LDS-resident pass when n <= 4096 and the
 // four-step n = n1 * n2 split (column pass with the n-point twiddle, then a row pass
-// with a transposed store) otherwise. Every pass moves whole sequences through LDS
-// with coalesced HBM reads and writes: a workgroup owns G neighbouring sequences, so a
-// strided (column) pass still reads G contiguous complex doubles per row.
+// with a transposed store) otherwise. Every
+// pass moves whole sequences through LDS with coalesced HBM reads and writes: a
+// workgroup owns G neighbouring sequences, so a strided (column) pass still reads G
+// contiguous complex doubles per row. Sequences of 1024 points or more take radix-16
+// stages whose first reads HBM and whose last writes it (fft_lds.hpp).
 //
 // Transforms are unnormalised in both directions (sign -1 forward, +1 inverse), as the
 // reference's native plans are; every scale (1/n, pad and crop) is folded into the

@@ -292,7 +292,7 @@ void Scan::tune(std::vector<FftGroup>& groups, int sign, const char* direction) 
         g.host = open;
         float best_ms = 0.0f;
         for (int cand = 0; cand <= 4; ++cand) {
-          if ((g.log2L + cand) > 12) break;
+          // Shapes with no kernel (more points a workgroup than LDS holds) are skipped.
           bool fits = true;
           for (int sq : g.sequences) fits = fits && sq % (1 << cand) == 0;
           if (!fits || !fft_grouped_available(g.log2L, cand)) continue;
