@@ -482,6 +482,8 @@ def gnlse_problem(
     its rate on the fine grid for the in-stage scheme, on the coarse grid (the collar
     at ``n_t`` samples) for the split-step one.
     """
+    from .translate import dispersion_operator, taylor_beta
+
     n_to = ffac * n_t
     dt = window / n_t
     dt_fine = window / n_to
@@ -489,10 +491,12 @@ def gnlse_problem(
     dw = 2 * np.pi * np.fft.fftfreq(n_t, dt)
     omega = omega0 + dw
     active, omega_window = _band(omega, band)
-    beta_poly = np.zeros_like(dw)
-    for order, b in enumerate(betas, start=2):
-        beta_poly += b * dw**order / math.factorial(order)
-    phi = np.where(active, -1j * beta_poly, 0.0).reshape(1, 1, n_t)
+    # A loss-free Taylor mode in the comoving envelope frame, which removes beta_0 and
+    # beta_1 (so the expansion starts at beta_2).
+    beta = taylor_beta((0.0, 0.0, *betas), omega0, omega[active])
+    phi = dispersion_operator(omega, active, beta, np.zeros(beta.size), kind="envelope",
+                              omega0=omega0, beta0=0.0, beta1=0.0, subtract_beta0=True,
+                              length=length).reshape(1, 1, n_t)
     ratio = omega / omega0 if shock else np.ones_like(omega)
     crop_scale = n_t / n_to
     prefactor = np.where(active, -1j * gamma * ratio * crop_scale, 0.0).reshape(1, 1, n_t)
@@ -821,6 +825,10 @@ def capillary_problem(
     """
     from scipy.special import jn_zeros, j0, j1
 
+    from .translate import dispersion_operator, fill_density, fill_integral, fill_pressure
+    from .translate import ideal_density
+    from .translate import kerr_group as translated_kerr
+
     n_to = n_t
     dt = window / n_t
     n = n_t // 2 + 1
@@ -858,29 +866,31 @@ def capillary_problem(
         refr += np.where(omega > 0, b_coef * lam_um**2 / (lam_um**2 - c_coef), 0.0)
     rho_ref = 1e5 / (KB * 273.15)
     beta_rho = k * refr / rho_ref  # dβ/dρ
-    v_frame = c  # the frame moves at the speed of light
+    # The density expansion of the operator: the vacuum part in a frame moving at the
+    # speed of light (a field grid's delay through zero frequency, beta_1 = 1/c), with
+    # the loss clamped as the reference clamps it, and the part per unit density.
     phi0 = np.zeros((n_modes, n), dtype=np.complex128)
     phi1 = np.zeros((n_modes, n), dtype=np.complex128)
+    # On a field grid the frame's delay runs from zero frequency: its reference frequency
+    # enters only the check that it lies on the grid, which a grid too coarse for the
+    # carrier (a refusal test's) would fail.
+    omega_ref = float(np.clip(2 * np.pi * c / wavelength, omega.min(), omega.max()))
     for m in range(n_modes):
-        phi0[m] = np.where(active, -1j * (beta_vac[m] - omega / v_frame) - 0.5 * alpha[m], 0.0)
+        phi0[m] = dispersion_operator(omega, active, beta_vac[m][active], alpha[m][active],
+                                      kind="field", omega0=omega_ref, beta0=0.0,
+                                      beta1=1.0 / c, subtract_beta0=False, length=length)
         phi1[m] = np.where(active, -1j * beta_rho, 0.0)
     phi_terms = np.ascontiguousarray(np.stack([phi0, phi1]))
-    rho_unit = 1e5 / (KB * temperature)  # density per bar
 
+    # The fill: the steady-flow pressure, its ideal-gas density and ∫ρ dz.
     def pressure(z):
-        return np.sqrt(np.maximum(p0**2 + (z / length) * (p1**2 - p0**2), 0.0))
+        return fill_pressure(z, p0, p1, length)
 
     def density(z):
-        return rho_unit * pressure(z)
+        return fill_density(z, p0, p1, length, temperature)
 
     def cumulative(z):
-        # ∫_0^z p dz' for p = sqrt(p0² + s (p1² - p0²)), s = z/L.
-        d = p1**2 - p0**2
-        if abs(d) < 1e-300:
-            return rho_unit * p0 * z
-        return rho_unit * (2 * length / (3 * d)) * (
-            (p0**2 + (z / length) * d) ** 1.5 - p0**3
-        )
+        return ideal_density(fill_integral(z, p0, p1, length), temperature)
 
     # Geometric refinement towards the end, where sqrt(1 - z/L) is steepest.
     fractions = 1.0 - 0.5 ** np.arange(0, 14)
@@ -909,7 +919,8 @@ def capillary_problem(
     kerr_group = None
     chi3_per_rho = (4.0 / 3.0) * HE_N2_PER_BAR * EPS0 * c / (1e5 / (KB * 273.15))
     if kerr:
-        kerr_group = KerrGroup(EPS0 * chi3_per_rho * (1.0 - raman_share), "real")
+        # The real cubic of a field grid with its harmonic: eps0 chi3 rho (sum E^2) E.
+        kerr_group = translated_kerr(chi3_per_rho * (1.0 - raman_share), "field", True)
     raman_group = None
     if raman_share > 0.0:
         # A molecular response carrying `raman_share` of the third-order response, its
