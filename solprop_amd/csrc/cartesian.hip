@@ -284,7 +284,7 @@ CartesianWorkspace::CartesianWorkspace(const sp_problem_desc& d) {
     rec_dev_ = alloc<AdamsRecord>(3);
     SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&rec_host_), 3 * sizeof(AdamsRecord)));
     std::memset(rec_host_, 0, 3 * sizeof(AdamsRecord));
-    partials_ = alloc<double>(3UL * ls_blocks(P_));
+    partials_ = alloc<double>(ls_partials(P_));
     SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&norms_host_), 4 * sizeof(double),
                          hipHostMallocMapped));
     SP_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&B_.norms), norms_host_, 0));

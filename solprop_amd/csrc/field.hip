@@ -1728,6 +1728,25 @@ __global__ void __launch_bounds__(kLsThreads)
   }
 }
 
+// Above kLsRun workgroup sums a first level sums each run of kLsRun of them (workgroup
+// (run, k)), in the same fixed order, into `out` (3 rows of `runs`).
+__global__ void __launch_bounds__(kLsThreads)
+    ls_runs(const double* __restrict__ partials, int blocks, double* out, int runs) {
+  __shared__ double tree[kLsThreads];
+  const int run = blockIdx.x, k = blockIdx.y;
+  const int b0 = run * kLsRun, b1 = min(b0 + kLsRun, blocks);
+  double total = 0.0;
+  for (int b = b0 + static_cast<int>(threadIdx.x); b < b1; b += kLsThreads)
+    total = total + partials[static_cast<long>(k) * blocks + b];
+  tree[threadIdx.x] = total;
+  __syncthreads();
+  for (int half = kLsThreads / 2; half > 0; half /= 2) {
+    if (threadIdx.x < half) tree[threadIdx.x] = tree[threadIdx.x] + tree[threadIdx.x + half];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[static_cast<long>(k) * runs + run] = tree[0];
+}
+
 // The three sums from the passes' workgroup sums, in a fixed order, into `norms`.
 __global__ void __launch_bounds__(kLsThreads)
     ls_norms(const double* __restrict__ partials, int blocks, double* norms) {
@@ -2062,6 +2081,12 @@ unsigned ls_blocks(const DeviceProblem& P) {
   return blocks_for(static_cast<std::size_t>(P.C) * P.N, kLsThreads);
 }
 
+std::size_t ls_partials(const DeviceProblem& P) {
+  const std::size_t blocks = ls_blocks(P);
+  const std::size_t runs = (blocks + kLsRun - 1) / kLsRun;
+  return 3 * blocks + (blocks > kLsRun ? 3 * runs : 0);
+}
+
 void launch_ls_pass(int pass, const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                     cplx* acc, cplx* prev, double* partials, hipStream_t s) {
   const dim3 grid(ls_blocks(P)), block(kLsThreads);
@@ -2076,8 +2101,18 @@ void launch_ls_pass(int pass, const DeviceProblem& P, const AdamsRecord* R, Fiel
 
 void launch_ls_norms(const DeviceProblem& P, const double* partials, double* norms,
                      hipStream_t s) {
-  hipLaunchKernelGGL(ls_norms, dim3(1), dim3(kLsThreads), 0, s, partials,
-                     static_cast<int>(ls_blocks(P)), norms);
+  const int blocks = static_cast<int>(ls_blocks(P));
+  if (blocks <= kLsRun) {
+    hipLaunchKernelGGL(ls_norms, dim3(1), dim3(kLsThreads), 0, s, partials, blocks, norms);
+  } else {
+    // Two levels: a million workgroup sums would take one workgroup milliseconds.
+    const int runs = (blocks + kLsRun - 1) / kLsRun;
+    double* level = const_cast<double*>(partials) + 3L * blocks;
+    hipLaunchKernelGGL(ls_runs, dim3(runs, 3), dim3(kLsThreads), 0, s, partials, blocks, level,
+                       runs);
+    check();
+    hipLaunchKernelGGL(ls_norms, dim3(1), dim3(kLsThreads), 0, s, level, runs, norms);
+  }
   SP_HIP(hipGetLastError());
 }
 

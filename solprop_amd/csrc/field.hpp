@@ -34,6 +34,7 @@ constexpr int kLdsTableIntervals = 1024;  // rate-table intervals held in LDS
 constexpr int kResponseWaves = 16; // waves per workgroup of the node response
 constexpr int kCorrectThreads = 256;  // the corrector's widest workgroup
 constexpr int kLsThreads = 256;    // workgroup of the low-storage register passes
+constexpr int kLsRun = 4096;       // workgroup sums one workgroup of their reduction reads
 
 // Scan tiles of a fine grid of T samples: ceil((T - 1) / 63).
 __host__ __device__ inline int scan_tiles(int T) { return (T - 1 + kTileSpan - 1) / kTileSpan; }
@@ -231,6 +232,9 @@ void launch_adams_norms(const DeviceProblem& P, FieldBuffers B, hipStream_t s);
 // The low-storage RK4IP step's register passes (1..4, each after its source evaluation)
 // and the three masked sums they leave (|S2 - g|^2, |S3 - S2|^2, |A|^2) into `norms`.
 unsigned ls_blocks(const DeviceProblem& P);
+// Doubles of the passes' partial sums: three rows of ls_blocks, and above kLsRun
+// workgroups three rows of the first level's run sums after them.
+std::size_t ls_partials(const DeviceProblem& P);
 void launch_ls_pass(int pass, const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                     cplx* acc, cplx* prev, double* partials, hipStream_t s);
 void launch_ls_norms(const DeviceProblem& P, const double* partials, double* norms,

@@ -272,7 +272,7 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
     ls_rec_dev_ = alloc<AdamsRecord>(3);
     SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&ls_rec_host_), 3 * sizeof(AdamsRecord)));
     std::memset(ls_rec_host_, 0, 3 * sizeof(AdamsRecord));
-    ls_partials_ = alloc<double>(3UL * ls_blocks(P_));
+    ls_partials_ = alloc<double>(ls_partials(P_));
   }
   for (hipEvent_t& e : marks_) SP_HIP(hipEventCreate(&e));
   SP_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
