@@ -397,10 +397,15 @@ int sp_workspace_statistics(sp_workspace* ws, unsigned long long flags, const do
  * (statistics.cpp:517-603; a_m the modes' analytic signals of the peak powers). On the
  * field routes populations (1 << 11) too: every quadrature node's end-of-window ionised
  * fraction and largest ionisation rate, (n_nodes, 2), one channel
- * (statistics.cpp:636-650). Packed in the native order: ..., electron_density,
+ * (statistics.cpp:636-650), and the plasma integration diagnostics (1 << 12,
+ * statistics.cpp:184-186,684-697; numpy_ref/passes/plasma.py:488-552): six maxima over
+ * the nodes of the column at the committed state — the exposure increments' largest
+ * dip max(-min dS, 0) and rise max dS, the largest |paid - freed| (paid the trapezoid of
+ * W (1 - f), freed f_end - f_0) and freed, and the largest |f E| over the first 2K
+ * samples and anywhere. Packed in the native order: ..., electron_density,
  * peak_ionisation_rate, peak_intensity (1), the fwhm trace (n_t), populations
- * (2 n_nodes). quadrature_error (1 << 10) and the plasma integration diagnostics
- * (1 << 12) are refused with SP_VALUE. */
+ * (2 n_nodes), the integration diagnostics (6). quadrature_error (1 << 10) is refused
+ * with SP_VALUE, and so are the plasma statistics on a problem without a plasma column. */
 typedef struct sp_statistics_request {
   unsigned long long flags;
   const double* windows;

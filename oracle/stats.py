@@ -162,6 +162,52 @@ def ionisation(problem, state, z):
     return float(problem.density(z)) * fraction[densest, -1], float(np.max(rate[densest]))
 
 
+def plasma_integration_parts(problem, state, z) -> np.ndarray:
+    """The plasma column's integration diagnostics at a plane, one channel
+    (``src/solprop/backends/numpy_ref/passes/plasma.py:488-552``): at every node the
+    exposure ``S`` as the rule forms it (before the floor) and its increments ``dS``, the
+    fraction, the work the loss current pays ``int W (1 - f) dt`` (the trapezoid of the
+    raw rate), the electrons freed ``f_end - f_0`` and the free-current integrand
+    ``|f E|``; the six maxima over the nodes: ``[max(-min dS, 0), max dS,
+    max |paid - freed|, max freed, max |f E| over the first 2K samples, max |f E|]``."""
+    from oracle import rk
+
+    src = rk.StageSource(problem)
+    crop = problem.crop_index.astype(np.intp)
+    modal = P.to_time(state.reshape(problem.shape), crop, problem.n_to, problem.kind)
+    nodes = P.synthesise(modal, problem.synthesis, 1.0, 1)  # (Q, 1, T)
+    return integration_parts(nodes, src.plasma, problem.preionisation(z), problem.dt_fine)
+
+
+def integration_parts(nodes, plasma, preionisation, dt) -> np.ndarray:
+    """The six parts of :func:`plasma_integration_parts` over node fields
+    ``(Q, n_pol, T)`` in V/m, one channel."""
+    rule = plasma.integral
+    reach = 2 * rule.half_width
+    magnitude = P.magnitude(nodes)
+    rate = plasma.rate(magnitude)
+    exposure = rule.integrate(rate, dt)
+    steps = np.diff(exposure, axis=-1)
+    fraction = 1.0 - (1.0 - preionisation) * np.exp(-rule.floor(exposure))
+    paid = np.trapezoid(rate * (1.0 - fraction), dx=dt, axis=-1)
+    freed = fraction[..., -1] - fraction[..., 0]
+    integrand = np.abs(fraction[..., np.newaxis, :] * nodes)
+    early = float(np.max(integrand[..., :reach])) if reach else 0.0
+    return np.array([max(-float(np.min(steps)), 0.0), float(np.max(steps)),
+                     float(np.max(np.abs(paid - freed))), float(np.max(freed)), early,
+                     float(np.max(integrand))])
+
+
+def plasma_integration_statistics(parts) -> tuple[float, float, float]:
+    """The three diagnostics of the six parts (``passes/plasma.py:555-575``): the
+    undershoot ``max(-dS) / max(dS)``, the work defect ``max |paid - freed| / max freed``
+    and the boundary level, each zero where its denominator is."""
+    dip, rise, defect, freed, early, peak = (float(part) for part in parts)
+    return (dip / rise if rise > 0.0 else 0.0, defect / freed if freed > 0.0 else 0.0,
+            early / peak if peak > 0.0 else 0.0)
+
+
 __all__ = ["analytic_field", "energy", "energy_window", "monitor_energies", "monitor_power",
-           "ionisation", "omega0", "peak_intensity", "peak_power", "peak_power_total",
-           "peak_power_window", "plasma_column", "populations", "power_trace", "split_sums"]
+           "integration_parts", "ionisation", "omega0", "peak_intensity", "peak_power",
+           "peak_power_total", "peak_power_window", "plasma_column", "plasma_integration_parts",
+           "plasma_integration_statistics", "populations", "power_trace", "split_sums"]

@@ -1821,7 +1821,7 @@ namespace {
 template <int C>
 void launch_node_passes_for(const DeviceProblem& P, const AdamsRecord* R,
                             const PlasmaTable& pt, FieldBuffers B, hipStream_t s,
-                            hipEvent_t* marks) {
+                            hipEvent_t* marks, bool exposure) {
   const int tiles = scan_tiles(P.T);
   const int per_block = kNodeWaves * kNodesPerWave;
   const dim3 node_grid(tiles, (P.Q + per_block - 1) / per_block);
@@ -1829,7 +1829,7 @@ void launch_node_passes_for(const DeviceProblem& P, const AdamsRecord* R,
   auto mark = [&](int i) {
     if (marks) SP_HIP(hipEventRecord(marks[i], s));
   };
-  const bool band = pt.n_taps > 1;
+  const bool band = pt.n_taps > 1 || exposure;
   if (P.has_plasma) {
     SP_CHECK(!band || (B.x_local && pt.n_taps <= kMaxBandHalf + 1 && 64 + 2 * (pt.n_taps - 1) <= kBandSpan),
              "the band rule needs its exposure buffer and at most 8 taps each side");
@@ -1869,23 +1869,134 @@ void launch_node_passes_for(const DeviceProblem& P, const AdamsRecord* R,
 }  // namespace
 
 void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,
-                        FieldBuffers B, hipStream_t s, hipEvent_t* marks) {
+                        FieldBuffers B, hipStream_t s, hipEvent_t* marks, bool exposure) {
   SP_CHECK(P.T >= 2, "the fine grid needs two samples");
   if (B.node_field) {
     SP_CHECK(B.node_streams, "the dense route needs its node streams");
-    launch_node_passes_for<0>(P, R, pt, B, s, marks);
+    launch_node_passes_for<0>(P, R, pt, B, s, marks, exposure);
     return;
   }
   SP_CHECK(P.C >= 1 && P.C <= kMaxModes, "the node passes project 1..8 modes");
   switch (P.C) {
-    case 1: launch_node_passes_for<1>(P, R, pt, B, s, marks); break;
-    case 2: launch_node_passes_for<2>(P, R, pt, B, s, marks); break;
-    case 3: launch_node_passes_for<3>(P, R, pt, B, s, marks); break;
-    case 4: launch_node_passes_for<4>(P, R, pt, B, s, marks); break;
-    case 5: launch_node_passes_for<5>(P, R, pt, B, s, marks); break;
-    case 6: launch_node_passes_for<6>(P, R, pt, B, s, marks); break;
-    case 7: launch_node_passes_for<7>(P, R, pt, B, s, marks); break;
-    default: launch_node_passes_for<8>(P, R, pt, B, s, marks); break;
+    case 1: launch_node_passes_for<1>(P, R, pt, B, s, marks, exposure); break;
+    case 2: launch_node_passes_for<2>(P, R, pt, B, s, marks, exposure); break;
+    case 3: launch_node_passes_for<3>(P, R, pt, B, s, marks, exposure); break;
+    case 4: launch_node_passes_for<4>(P, R, pt, B, s, marks, exposure); break;
+    case 5: launch_node_passes_for<5>(P, R, pt, B, s, marks, exposure); break;
+    case 6: launch_node_passes_for<6>(P, R, pt, B, s, marks, exposure); break;
+    case 7: launch_node_passes_for<7>(P, R, pt, B, s, marks, exposure); break;
+    default: launch_node_passes_for<8>(P, R, pt, B, s, marks, exposure); break;
+  }
+}
+
+// ------------------------------------------ the plasma integration diagnostics
+
+namespace {
+constexpr int kDiagThreads = 256;
+constexpr int kDiagParts = 6;
+
+// One node's parts (see launch_plasma_diagnostics). The sum of the work integrand runs
+// over each thread's samples in order, then the wave's fixed-order total, then the waves
+// in order; minima and maxima need no order.
+template <int C>
+__global__ void __launch_bounds__(kDiagThreads)
+    plasma_diagnose(DeviceProblem P, PlasmaTable pt, FieldBuffers B, double* parts) {
+  __shared__ double lds[kDiagParts][kDiagThreads / kWave];
+  const int q = blockIdx.x;
+  const int T = P.T, tiles = scan_tiles(T), reach = 2 * (pt.n_taps - 1);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const long row = static_cast<long>(q) * T;
+  const double* offsets = B.tile_x + static_cast<long>(q) * tiles;
+  const double dt = P.dt_fine;
+  auto exposure = [&](int t) { return offsets[min(t / kTileSpan, tiles - 1)] + B.x_local[row + t]; };
+  auto work = [&](int t) { return B.rate[row + t] * (1.0 - B.frac[row + t]); };
+  double lo = INFINITY, hi = -INFINITY, paid = 0.0, early = 0.0, peak = 0.0;
+  for (int t = threadIdx.x; t < T; t += kDiagThreads) {
+    if (t + 1 < T) {
+      const double step = exposure(t + 1) - exposure(t);
+      lo = fmin(lo, step);
+      hi = fmax(hi, step);
+      paid = paid + (dt * (work(t + 1) + work(t))) / 2.0;
+    }
+    double am[held(C)];
+    load_modal<C>(B, T, t, am);
+    const double level = fabs(B.frac[row + t] * field_at<C>(P, B, am, q, t));
+    peak = fmax(peak, level);
+    if (t < reach) early = fmax(early, level);
+  }
+  // Wave totals: the sum in a fixed order, the extremes by their own reductions.
+  paid = wave_total(paid);
+  for (int offset = kWave / 2; offset > 0; offset /= 2) {
+    lo = fmin(lo, __shfl_xor(lo, offset));
+    hi = fmax(hi, __shfl_xor(hi, offset));
+    early = fmax(early, __shfl_xor(early, offset));
+    peak = fmax(peak, __shfl_xor(peak, offset));
+  }
+  if (lane == 0) {
+    lds[0][wave] = lo;
+    lds[1][wave] = hi;
+    lds[2][wave] = paid;
+    lds[3][wave] = early;
+    lds[4][wave] = peak;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double l = lds[0][0], h = lds[1][0], w = lds[2][0], e = lds[3][0], k = lds[4][0];
+    for (int v = 1; v < kDiagThreads / kWave; ++v) {
+      l = fmin(l, lds[0][v]);
+      h = fmax(h, lds[1][v]);
+      w = w + lds[2][v];
+      e = fmax(e, lds[3][v]);
+      k = fmax(k, lds[4][v]);
+    }
+    const double freed = B.frac[row + T - 1] - B.frac[row];
+    double* out = parts + static_cast<long>(q) * kDiagParts;
+    out[0] = fmax(-l, 0.0);
+    out[1] = h;
+    out[2] = fabs(w - freed);
+    out[3] = freed;
+    out[4] = e;
+    out[5] = k;
+  }
+}
+
+// The largest of each part over the nodes.
+__global__ void __launch_bounds__(kWave)
+    plasma_diagnose_max(const double* __restrict__ parts, int Q, double* out) {
+  const int part = threadIdx.x;
+  if (part >= kDiagParts) return;
+  double largest = parts[part];
+  for (int q = 1; q < Q; ++q) largest = fmax(largest, parts[static_cast<long>(q) * kDiagParts + part]);
+  out[part] = largest;
+}
+
+template <int C>
+void launch_diagnose_for(const DeviceProblem& P, const PlasmaTable& pt, FieldBuffers B,
+                         double* parts, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(plasma_diagnose<C>, dim3(P.Q), dim3(kDiagThreads), 0, s, P, pt, B, parts);
+  SP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(plasma_diagnose_max, dim3(1), dim3(kWave), 0, s, parts, P.Q, out);
+  SP_HIP(hipGetLastError());
+}
+}  // namespace
+
+void launch_plasma_diagnostics(const DeviceProblem& P, const PlasmaTable& pt, FieldBuffers B,
+                               double* parts, double* out, hipStream_t s) {
+  SP_CHECK(P.has_plasma && B.x_local && B.rate && B.frac && B.tile_x && P.T >= 2 && P.Q >= 1,
+           "the integration diagnostics read a stored plasma column with its exposure");
+  if (B.node_field) {
+    launch_diagnose_for<0>(P, pt, B, parts, out, s);
+    return;
+  }
+  switch (P.C) {
+    case 1: launch_diagnose_for<1>(P, pt, B, parts, out, s); break;
+    case 2: launch_diagnose_for<2>(P, pt, B, parts, out, s); break;
+    case 3: launch_diagnose_for<3>(P, pt, B, parts, out, s); break;
+    case 4: launch_diagnose_for<4>(P, pt, B, parts, out, s); break;
+    case 5: launch_diagnose_for<5>(P, pt, B, parts, out, s); break;
+    case 6: launch_diagnose_for<6>(P, pt, B, parts, out, s); break;
+    case 7: launch_diagnose_for<7>(P, pt, B, parts, out, s); break;
+    default: launch_diagnose_for<8>(P, pt, B, parts, out, s); break;
   }
 }
 

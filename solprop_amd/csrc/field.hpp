@@ -204,8 +204,20 @@ void launch_field_scatter_hilbert(const DeviceProblem& P, const cplx* spectrum, 
 // node streams for the projection GEMM. `marks`, when given, holds five events recorded after the rate and exposure tiles,
 // their prefix, the free-current tiles, their prefix and the node response (a run
 // without plasma records the first four back to back).
+// `exposure` runs the band rate pass under either rule (one tap is the trapezoid, the
+// same bits), so that the within-tile exposure is stored for the integration
+// diagnostics.
 void launch_node_passes(const DeviceProblem& P, const AdamsRecord* R, const PlasmaTable& pt,
-                        FieldBuffers B, hipStream_t s, hipEvent_t* marks = nullptr);
+                        FieldBuffers B, hipStream_t s, hipEvent_t* marks = nullptr,
+                        bool exposure = false);
+// The plasma column's integration diagnostics from the unfused passes' stored column
+// (run with `exposure`): one workgroup a node writes its six parts to `parts` (6 Q) —
+// max(-min dS, 0), max dS, |paid - freed|, freed, max |f E| over the first 2K samples
+// and anywhere, with S = tile offset + within-tile exposure (before the floor), paid the
+// trapezoid of W (1 - f) and freed f(T-1) - f(0) — then their maxima over the nodes go
+// to `out` (6).
+void launch_plasma_diagnostics(const DeviceProblem& P, const PlasmaTable& pt, FieldBuffers B,
+                               double* parts, double* out, hipStream_t s);
 // Whether the fused node passes apply: the fused-node route (1..8 modes) with plasma and
 // the buffers they write (the band passes' exposure, the group streams and totals).
 __host__ __device__ inline bool fused_node_passes(const DeviceProblem& P, const FieldBuffers& B) {

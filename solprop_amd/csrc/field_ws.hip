@@ -214,10 +214,10 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   B_.free_local = d.has_plasma ? alloc<double>(static_cast<std::size_t>(Q) * T) : nullptr;
   B_.tile_x = alloc<double>(static_cast<std::size_t>(Q) * tiles);
   B_.tile_f = alloc<double>(static_cast<std::size_t>(Q) * tiles);
-  // The band passes' within-tile exposure; on the fused-node route the fused passes read
-  // it under either rule, and write each node group's projections and projected totals.
-  if (d.has_plasma && (d.n_taps > 1 || !dense_))
-    B_.x_local = alloc<double>(static_cast<std::size_t>(Q) * T);
+  // The band passes' within-tile exposure: the fused passes read it under either rule,
+  // and so do the integration diagnostics; the fused passes write each node group's
+  // projections and projected totals.
+  if (d.has_plasma) B_.x_local = alloc<double>(static_cast<std::size_t>(Q) * T);
   if (d.has_plasma && !dense_) {
     const std::size_t groups = static_cast<std::size_t>(node_groups(Q));
     B_.group_streams = alloc<double>(groups * 2 * C * T);
@@ -987,10 +987,10 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
   std::vector<double> trace;
   double intensity = 0.0;
   std::vector<double> out =
-      stats_->run(B_.y, flags & ~(kStatPlasma | kStatPopulations), windows, n_windows, stream_,
-                  &trace, &extras, &intensity);
-  std::vector<double> populations;
-  if (flags & (kStatPlasma | kStatPopulations)) {
+      stats_->run(B_.y, flags & ~kStatColumn, windows, n_windows, stream_, &trace, &extras,
+                  &intensity);
+  std::vector<double> populations, diagnostics;
+  if (flags & kStatColumn) {
     SP_CHECK(P_.has_plasma, "the plasma statistics need a compiled plasma response, and this "
                             "problem declares none");
     // The plasma column at the committed state and its position: the node passes on its
@@ -1001,7 +1001,8 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
     launch_field_scatter(P_, B_.y, B_, stream_);
     inv_plan_.exec(B_.modal_t, scratch_, +1, stream_);
     enqueue_synthesis();
-    launch_node_passes(P_, rec_dev_, pt_, B_, stream_);
+    launch_node_passes(P_, rec_dev_, pt_, B_, stream_, nullptr,
+                       (flags & kStatPlasmaIntegration) != 0);
     launch_plasma_densest(B_.frac, B_.rate, P_.Q, P_.T, B_.partials, stream_);
     double densest[2];
     SP_HIP(hipMemcpyAsync(densest, B_.partials, sizeof(densest), hipMemcpyDeviceToHost,
@@ -1015,6 +1016,14 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
                             populations.size() * sizeof(double), hipMemcpyDeviceToHost,
                             stream_));
     }
+    if (flags & kStatPlasmaIntegration) {
+      // Each node's six parts, then their maxima over the nodes.
+      if (!diagnostics_) diagnostics_ = alloc<double>(6 * static_cast<std::size_t>(P_.Q) + 6);
+      launch_plasma_diagnostics(P_, pt_, B_, diagnostics_, diagnostics_ + 6L * P_.Q, stream_);
+      diagnostics.assign(6, 0.0);
+      SP_HIP(hipMemcpyAsync(diagnostics.data(), diagnostics_ + 6L * P_.Q, 6 * sizeof(double),
+                            hipMemcpyDeviceToHost, stream_));
+    }
     SP_HIP(hipStreamSynchronize(stream_));
     if (flags & kStatElectronDensity) out.push_back(density_(z_) * densest[0]);
     if (flags & kStatPeakIonisationRate) out.push_back(densest[1]);
@@ -1022,6 +1031,7 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
   if (flags & kStatPeakIntensity) out.push_back(intensity);
   out.insert(out.end(), trace.begin(), trace.end());
   out.insert(out.end(), populations.begin(), populations.end());
+  out.insert(out.end(), diagnostics.begin(), diagnostics.end());
   return out;
 }
 

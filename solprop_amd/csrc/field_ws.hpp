@@ -157,6 +157,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   int n_t_ = 0;                 // coarse samples (the statistics' grid)
   std::unique_ptr<PlaneStatistics> stats_;  // built at the first statistics call
   double* populations_ = nullptr;            // (2 Q), at the first populations call
+  double* diagnostics_ = nullptr;            // (6 Q + 6), at the first diagnostics call
   MonitorBuffers monitor_;
   double* spatial_matrix_ = nullptr;  // (C, C) row-major
   bool absorbing() const { return in_stage_ || spatial_; }

@@ -36,8 +36,16 @@ enum : std::uint64_t {
   // Each node's end-of-window ionised fraction and largest rate, (Q, 2), one channel
   // (statistics.cpp:636-650); packed last, after the power trace.
   kStatPopulations = 1ULL << 11U,
+  // The plasma column's integration diagnostics: six maxima over the nodes (the exposure
+  // increments' dip and rise, the loss current's work defect and the electrons freed,
+  // the free-current integrand's level in the first 2K samples and its peak;
+  // src/solprop/backends/numpy_ref/passes/plasma.py:488-552, statistics.cpp:184-186,
+  // 684-697); packed last, after the populations.
+  kStatPlasmaIntegration = 1ULL << 12U,
 };
 constexpr std::uint64_t kStatPlasma = kStatElectronDensity | kStatPeakIonisationRate;
+// Every statistic the field routes' plasma column serves.
+constexpr std::uint64_t kStatColumn = kStatPlasma | kStatPopulations | kStatPlasmaIntegration;
 // The node of largest ionised fraction at the window's end (the first such node) and
 // its largest rate: out[0] that fraction, out[1] the rate (numpy_ref/statistics.py:502-546).
 void launch_plasma_densest(const double* frac, const double* rate, int Q, int T, double* out,

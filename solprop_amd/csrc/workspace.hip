@@ -485,7 +485,7 @@ std::vector<double> Workspace::statistics(std::uint64_t flags, const double* win
   if (!stats_)
     stats_ = std::make_unique<PlaneStatistics>(P_.C, P_.N, P_.N, false,
                                                P_.dt_fine * P_.T / P_.N, P_.omega);
-  SP_CHECK(!(flags & (kStatPlasma | kStatPopulations)),
+  SP_CHECK(!(flags & kStatColumn),
            "the plasma statistics read the field-grid plasma column, and the envelope route "
            "carries none");
   std::vector<double> trace;

@@ -318,7 +318,7 @@ class Workspace:
                   "peak_power_total": 1 << 5,
                   "fwhm": 1 << 6, "electron_density": 1 << 7,
                   "peak_ionisation_rate": 1 << 8, "peak_intensity": 1 << 9,
-                  "populations": 1 << 11}
+                  "populations": 1 << 11, "plasma_integration": 1 << 12}
 
     def statistics(self, names, windows=None, on_axis=None, intensity_factor=1.0) -> dict:
         """Plane statistics of the committed state (``stats/definitions.py``): a dict of
@@ -326,7 +326,9 @@ class Workspace:
         ``energy_window``, whose value is (n, n_modes). ``peak_intensity`` reads the
         on-axis synthesis ``on_axis`` ((rows, n_modes), one or two rows) and the
         intensity factor; ``populations`` is (n_nodes, 2): each node's end-of-window
-        ionised fraction and largest rate."""
+        ionised fraction and largest rate; ``plasma_integration`` the column's six
+        integration-diagnostic maxima (:func:`plasma_integration_statistics` forms the
+        three diagnostics of them)."""
         unknown = set(names) - set(self.STATISTICS)
         if unknown:
             raise BackendRefusal(f"statistics not built: {sorted(unknown)}")
@@ -335,7 +337,7 @@ class Workspace:
             flags |= self.STATISTICS[name]
         w = np.zeros((0, 2)) if windows is None else np.ascontiguousarray(windows, np.float64)
         n_modes, n_nodes = self.problem.n_modes, self.problem.n_nodes
-        capacity = 3 * n_modes + 2 * len(w) * n_modes + 4 + self.problem.n_t + 2 * n_nodes
+        capacity = 3 * n_modes + 2 * len(w) * n_modes + 4 + self.problem.n_t + 2 * n_nodes + 6
         out = np.zeros(capacity)
         count = C.c_int()
         axis = None
@@ -351,13 +353,13 @@ class Workspace:
                  "peak_power": n_modes, "peak_power_window": len(w) * n_modes,
                  "peak_power_total": 1, "electron_density": 1,
                  "peak_ionisation_rate": 1, "peak_intensity": 1, "fwhm": self.problem.n_t,
-                 "populations": 2 * n_nodes}
-        # The packing order (native/src/general/statistics.cpp:160-182): the flags'
+                 "populations": 2 * n_nodes, "plasma_integration": 6}
+        # The packing order (native/src/general/statistics.cpp:160-186): the flags'
         # order, except that the power trace (the fwhm flag's) follows the peak
-        # intensity, and the populations come last.
+        # intensity, and the populations and then the integration diagnostics come last.
         order = ["omega0", "energy", "energy_window", "peak_power", "peak_power_window",
                  "peak_power_total", "electron_density", "peak_ionisation_rate",
-                 "peak_intensity", "fwhm", "populations"]
+                 "peak_intensity", "fwhm", "populations", "plasma_integration"]
         scalars = ("peak_power_total", "electron_density", "peak_ionisation_rate",
                    "peak_intensity")
         for name in order:
@@ -448,6 +450,18 @@ class Workspace:
                                              names, 2048), names)
         labels = names.value.decode().split(",") if count.value else []
         return dict(zip(labels, us[: count.value].tolist(), strict=True))
+
+
+def plasma_integration_statistics(parts) -> dict:
+    """The plasma column's three integration diagnostics from the six maxima the
+    ``plasma_integration`` statistic packs (``numpy_ref/passes/plasma.py:555-575``,
+    ``numpy_ref/statistics.py:414-455``): the undershoot of the exposure increments
+    ``max(-dS) / max(dS)``, the work defect ``max |paid - freed| / max freed`` and the
+    boundary level of the free-current integrand, each zero where its denominator is."""
+    dip, rise, defect, freed, early, peak = (float(part) for part in parts)
+    return {"plasma_undershoot": dip / rise if rise > 0.0 else 0.0,
+            "plasma_work_defect": defect / freed if freed > 0.0 else 0.0,
+            "plasma_boundary_level": early / peak if peak > 0.0 else 0.0}
 
 
 def advance_batch(sessions, targets, work_limit: int = 256) -> list:

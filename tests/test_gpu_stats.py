@@ -224,3 +224,40 @@ def test_plasma_statistics_match_oracle(route):
     assert density > 0.0 and rate > 0.0
     assert abs(got["electron_density"] - density) <= 1e-9 * density
     assert abs(got["peak_ionisation_rate"] - rate) <= 1e-12 * rate
+
+
+@pytest.mark.parametrize("route", ["capillary_trapezoid", "capillary_band", "radial"])
+def test_plasma_integration_diagnostics_match_oracle(route):
+    """The column's six integration-diagnostic maxima at the committed state after a short
+    march, against the oracle's over the same state: the exposure increments' dip and rise
+    within 1e-9 of the rise (their differences of a running sum round at eps times it), the
+    electrons freed within 1e-9 and the work defect within 1e-9 of them (a difference of
+    two nearly equal numbers), the free-current integrand's levels within 1e-9 of its peak.
+    """
+    print(f"plasma integration diagnostics {route}", flush=True)
+    if route == "capillary_trapezoid":
+        problem = capillary_problem(n_t=2048, n_nodes=32, length=0.05, integration="trapezoid")
+    elif route == "capillary_band":
+        problem = capillary_problem(n_t=2048, n_nodes=32, length=0.05, integration="band",
+                                    band=(300e-9, 4e-6))
+    else:
+        problem = radial_problem(n_t=1024, window=0.25e-12, n_r=32, length=0.02,
+                                 fwhm=30e-15, w0=0.25e-3, integration="band",
+                                 band=(400e-9, 4e-6))
+    ws = Workspace(problem)
+    session = ws.march(0.0, problem.field0)
+    session.advance(problem.z_end, work_limit=4)
+    state, z = session.state(), session.z
+    got = ws.statistics(["plasma_integration", "populations"])
+    ws.close()
+    parts = np.asarray(got["plasma_integration"])
+    expected = S.plasma_integration_parts(problem, state, z)
+    print("device", parts.tolist(), "oracle", expected.tolist(), flush=True)
+    assert parts.shape == (6,) and expected[1] > 0.0 and expected[3] > 0.0
+    rise, freed, peak = expected[1], expected[3], expected[5]
+    assert abs(parts[0] - expected[0]) <= 1e-9 * rise
+    assert abs(parts[1] - rise) <= 1e-9 * rise
+    assert abs(parts[3] - freed) <= 1e-9 * freed
+    assert abs(parts[2] - expected[2]) <= 1e-9 * freed
+    assert abs(parts[4] - expected[4]) <= 1e-9 * peak
+    assert abs(parts[5] - peak) <= 1e-9 * peak

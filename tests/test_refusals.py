@@ -61,15 +61,15 @@ def test_transverse_collar_off_the_dense_route_is_refused():
 
 def test_unbuilt_statistics_are_refused_by_name():
     """A statistic this library does not form is refused before the library is asked
-    (the quadrature estimate, the plasma integration diagnostics); the refusal names it."""
+    (the quadrature estimate); the refusal names it."""
     from solprop_amd.native import Workspace as W
 
     ws = W.__new__(W)  # no device: the check precedes every library call
     ws.problem = capillary_problem(n_t=256, n_nodes=4, integration="trapezoid")
     with pytest.raises(BackendRefusal, match="quadrature_error"):
         W.statistics(ws, ["energy", "quadrature_error"])
-    with pytest.raises(BackendRefusal, match="plasma_integration"):
-        W.statistics(ws, ["populations", "plasma_integration"])
+    with pytest.raises(BackendRefusal, match="quadrature_error"):
+        W.statistics(ws, ["populations", "plasma_integration", "quadrature_error"])
 
 
 def test_scan_refuses_a_point_it_cannot_batch():
