@@ -271,7 +271,7 @@ __device__ inline cplx step_operator(const DeviceProblem& P, const AdamsRecord& 
 // The predictor for a stencil of width W (moments M_0..M_{W-1}, W - 1 stored rows). One
 // wavefront per workgroup; every global operand is loaded before any arithmetic.
 // Coefficients past the stencil's order are zero in the record.
-template <int W>
+template <int W, bool Commit = false>
 __device__ __forceinline__ void predict_body(const DeviceProblem& P, const AdamsRecord* __restrict__ R,
                                     const FieldBuffers& B, long idx) {
   const long CN = static_cast<long>(P.C) * P.N;
@@ -283,7 +283,20 @@ __device__ __forceinline__ void predict_body(const DeviceProblem& P, const Adams
   cplx phi_t[kMaxTerms], F[W], U[W];
 #pragma unroll
   for (int t = 0; t < kMaxTerms; ++t) phi_t[t] = t < P.n_phi ? P.phi[t * CN + idx] : make(0.0, 0.0);
-  const cplx y = B.y[idx];
+  cplx y;
+  if constexpr (Commit) {
+    // The accepted step's commit for this bin (the commit below, element for element):
+    // its source row and state enter the store at the free slot and its advanced field
+    // becomes y. The stencil's loads below follow the stores in this thread, so they
+    // read the committed rows.
+    const int slot = r.free_slot;
+    B.store[slot * CN + idx] = B.source[idx];
+    if (compensated) B.states[slot * CN + idx] = B.lab[idx];
+    y = P.active[idx % P.N] ? B.advanced[idx] : make(0.0, 0.0);
+    B.y[idx] = y;
+  } else {
+    y = B.y[idx];
+  }
 #pragma unroll
   for (int j = 0; j < W; ++j) {
     F[j] = j < W - 1 ? B.store[r.slots[j] * CN + idx] : make(0.0, 0.0);
@@ -314,10 +327,10 @@ __device__ __forceinline__ void predict_body(const DeviceProblem& P, const Adams
   B.lab[idx] = lab;
 }
 
-template <int W>
+template <int W, bool Commit>
 __global__ void __launch_bounds__(kWave)
     adams_predict(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
-  predict_body<W>(P, R, B, static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x);
+  predict_body<W, Commit>(P, R, B, static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------- the source's transforms
@@ -1785,19 +1798,30 @@ unsigned cn_blocks(const DeviceProblem& P) {
 }
 }  // namespace
 
-void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
-                          int width, hipStream_t s) {
-  SP_CHECK(width >= 2 && width <= kMaxAdamsWidth - 1, "stencil width 2..7");
+namespace {
+template <bool Commit>
+void launch_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B, int width,
+                    hipStream_t s) {
   const dim3 grid(blocks_for(static_cast<std::size_t>(P.C) * P.N, kWave)), block(kWave);
   switch (width) {
-    case 2: hipLaunchKernelGGL(adams_predict<2>, grid, block, 0, s, P, R, B); break;
-    case 3: hipLaunchKernelGGL(adams_predict<3>, grid, block, 0, s, P, R, B); break;
-    case 4: hipLaunchKernelGGL(adams_predict<4>, grid, block, 0, s, P, R, B); break;
-    case 5: hipLaunchKernelGGL(adams_predict<5>, grid, block, 0, s, P, R, B); break;
-    case 6: hipLaunchKernelGGL(adams_predict<6>, grid, block, 0, s, P, R, B); break;
-    default: hipLaunchKernelGGL(adams_predict<7>, grid, block, 0, s, P, R, B); break;
+    case 2: hipLaunchKernelGGL((adams_predict<2, Commit>), grid, block, 0, s, P, R, B); break;
+    case 3: hipLaunchKernelGGL((adams_predict<3, Commit>), grid, block, 0, s, P, R, B); break;
+    case 4: hipLaunchKernelGGL((adams_predict<4, Commit>), grid, block, 0, s, P, R, B); break;
+    case 5: hipLaunchKernelGGL((adams_predict<5, Commit>), grid, block, 0, s, P, R, B); break;
+    case 6: hipLaunchKernelGGL((adams_predict<6, Commit>), grid, block, 0, s, P, R, B); break;
+    default: hipLaunchKernelGGL((adams_predict<7, Commit>), grid, block, 0, s, P, R, B); break;
   }
   check();
+}
+}  // namespace
+
+void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                          int width, hipStream_t s, bool commit) {
+  SP_CHECK(width >= 2 && width <= kMaxAdamsWidth - 1, "stencil width 2..7");
+  if (commit)
+    launch_predict<true>(P, R, B, width, s);
+  else
+    launch_predict<false>(P, R, B, width, s);
 }
 
 void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuffers B,

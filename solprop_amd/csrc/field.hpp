@@ -190,9 +190,10 @@ void launch_field_split_collar(const DeviceProblem& P, const AdamsRecord* R, dou
 void launch_field_split_unpack(const DeviceProblem& P, FieldBuffers B, hipStream_t s);
 
 // The predictor and corrector are compiled per stencil width (2..7); `width` must be
-// the record's.
+// the record's. With `commit` the predictor first applies the accepted step's commit
+// (launch_adams_commit) to each bin it predicts, the record's free slot included.
 void launch_adams_predict(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
-                          int width, hipStream_t s);
+                          int width, hipStream_t s, bool commit = false);
 void launch_field_scatter(const DeviceProblem& P, const cplx* spectrum, FieldBuffers B,
                           hipStream_t s);
 // The modal spectra's Hilbert transforms packed the same way into B.modal_h: -i Y on the

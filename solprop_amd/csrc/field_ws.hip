@@ -413,13 +413,15 @@ void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out, const Adams
 
 void FieldWorkspace::enqueue_trial(bool commit) {
   // An accepted step's commit rides at the head of the next trial (its slot is in the
-  // record), so that the host launches one graph per attempt.
-  if (commit) {
+  // record), so that the host launches one graph per attempt; without the split
+  // absorber, which acts on the committed field, the predictor applies it bin by bin.
+  const bool separate = commit && split_;
+  if (separate) {
     launch_adams_commit(P_, rec_dev_, B_, stream_);
-    if (split_) enqueue_split(rec_dev_, 0.0);
+    enqueue_split(rec_dev_, 0.0);
   }
   mark(0);
-  launch_adams_predict(P_, rec_dev_, B_, width_, stream_);
+  launch_adams_predict(P_, rec_dev_, B_, width_, stream_, commit && !separate);
   mark(1);
   enqueue_source(B_.lab, B_.source);
   launch_adams_correct(P_, rec_dev_, B_, width_, stream_);
