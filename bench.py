@@ -75,9 +75,10 @@ def trial_bytes(problem, order):
     qt2 = 16 * Q * T
     return {
         # The predictor of a trial after an accepted step applies that step's commit: it
-        # reads the source and advanced rows (and the state) in place of y and writes the
-        # store's free row (and state row) and y beside its own output.
-        "adams_predict": cn * (n_phi + 2 + order + (order + 1 if comp else 0))
+        # reads the source and advanced rows (and the state) in place of y and of the
+        # newest stored row and state (which it reads back from cache right after storing
+        # them), and writes the store's free row (and state row) and y beside its output.
+        "adams_predict": cn * (n_phi + 1 + order + (order if comp else 0))
         + cn * (3 + (order + 1 if comp else 0)),
         "field_scatter": cn + 8 * T + ct,
         "inverse_fft": 2 * ct,
