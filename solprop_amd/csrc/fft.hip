@@ -11,9 +11,10 @@ namespace {
 using PassArgs = FftPassArgs;
 
 // The packed point at global index g (row g / M, point g % M) from the half spectrum.
+// M is the transform's length, a power of two: a shift and a mask find the row and point.
 __device__ inline cplx packed_point(const PassArgs& a, long g) {
   const long M = a.spec_m;
-  const long r = g / M, k = g - r * M;
+  const long r = g >> __builtin_ctzl(static_cast<unsigned long>(M)), k = g & (M - 1);
   const cplx* row = a.spec + r * a.spec_n;
   const long kb = M - k;
   cplx xa = k < a.spec_n ? scale(a.spec_scale, row[k]) : make(0.0, 0.0);
@@ -25,10 +26,10 @@ __device__ inline cplx packed_point(const PassArgs& a, long g) {
   return c2r_pre(xa, xb, a.spec_roots[k]);
 }
 
-// One workgroup's pass: sequence group `group` of batch row `row`, staged in `buf`
-// (G * STRIDE points).
 // The pass's workgroup shape: radix-16 stages on padded rows for sequences of at least
-// 1024 points, radix-4 on plain rows otherwise.
+// 1024 points, radix-4 on plain rows otherwise. LDS_POINTS is the complex points of the
+// workgroup's LDS buffer (a whole-point hand-over through half of it, real parts then
+// imaginary ones, let more workgroups share a CU and ran slower).
 template <int LOG2L, int G>
 struct PassShape {
   static constexpr bool r16 = Shape16<LOG2L, G>::use;
@@ -36,86 +37,160 @@ struct PassShape {
   static constexpr int E = L * G;
   static constexpr int NT = r16 ? Shape16<LOG2L, G>::NT : Shape<LOG2L, G>::NT;
   static constexpr int STRIDE = r16 ? Shape16<LOG2L, G>::STRIDE : Shape<LOG2L, G>::STRIDE;
+  static constexpr int LDS_POINTS = G * STRIDE;
 };
 
-// One radix-R stage of a radix-16 pass (see stage16 in fft_lds.hpp): its inputs from HBM
-// when it is the first stage (point j of sequence s at the pass's input strides, or
-// packed from the half spectrum), else from the padded LDS rows; its outputs to HBM when
-// it is the last stage (point k = i + j p, times the four-step twiddle where asked), else
-// back to LDS. Only LDS-to-LDS stages need a barrier between their reads and writes.
-template <int LOG2L, int G, int R, bool First, bool Last>
-__device__ __forceinline__ void pass_stage(const cplx* src, cplx* dst, const cplx* __restrict__ tw_l,
-                                           const cplx* __restrict__ tw_n, const PassArgs& a,
-                                           int sign, long s0, int row, cplx* buf, int p) {
-  using S = Shape16<LOG2L, G>;
-  constexpr int L = S::L, E = S::E, NT = S::NT, STRIDE = S::STRIDE;
-  constexpr int NB = E / R;
-  constexpr int PER = (NB + NT - 1) / NT;
+// A radix-16 pass (see the radix-16 stages in fft_lds.hpp) keeps each stage's values in
+// registers: a thread runs PER butterflies of R points. The first stage reads HBM (point
+// j of sequence s at the pass's input strides, or packed from the half spectrum), the
+// last writes HBM (point k = i + j p, times the four-step twiddle where asked), and each
+// stage hands its outputs to the next through the padded LDS rows.
+template <int LOG2L, int G, int R>
+struct Stage16 {
+  static constexpr int NB = Shape16<LOG2L, G>::E / R;
+  static constexpr int PER = (NB + Shape16<LOG2L, G>::NT - 1) / Shape16<LOG2L, G>::NT;
+};
+
+// A thread's values of one stage.
+template <int LOG2L, int G, int R>
+using Regs16 = cplx[Stage16<LOG2L, G, R>::PER][R];
+
+// The radix of stage `Stage` of a pass over 2^LOG2L points: 16 while whole radix-16
+// stages remain, then 2, 4 or 8 for what remains of L.
+template <int LOG2L, int Stage>
+struct Radix16 {
+  static constexpr int full = LOG2L / 4;
+  static constexpr int rest = (1 << LOG2L) >> (4 * full);
+  static constexpr int stages = full + (rest > 1 ? 1 : 0);
+  static constexpr int R = Stage < full ? 16 : rest;
+  static constexpr bool last = Stage == stages - 1;
+};
+
+template <int LOG2L, int G, int R>
+__device__ __forceinline__ void r16_load(const cplx* src, const PassArgs& a, long s0, int row,
+                                         Regs16<LOG2L, G, R>& v) {
+  constexpr int L = 1 << LOG2L, NT = Shape16<LOG2L, G>::NT;
+  constexpr int NB = Stage16<LOG2L, G, R>::NB, PER = Stage16<LOG2L, G, R>::PER;
   const int tid = threadIdx.x;
-  const int step = L / (p * R);
-  cplx v[PER][R];
 #pragma unroll
   for (int r = 0; r < PER; ++r) {
     const int b = tid + r * NT;
     if (b < NB) {
       const int s = b / (L / R), i = b % (L / R);
-      const int k = i & (p - 1);
 #pragma unroll
       for (int j = 0; j < R; ++j) {
-        const int point = i + j * (L / R);
-        if constexpr (First) {
-          const long at = point * a.in_sj + (s0 + s) * a.in_ss;
-          v[r][j] = a.spec ? packed_point(a, static_cast<long>(row) * a.batch_in + at) : src[at];
-        } else {
-          v[r][j] = buf[s * STRIDE + padded(point)];
-        }
+        const long at = (i + j * (L / R)) * a.in_sj + (s0 + s) * a.in_ss;
+        v[r][j] = a.spec ? packed_point(a, static_cast<long>(row) * a.batch_in + at) : src[at];
       }
+    }
+  }
+}
+
+// The stage's twiddles W_{pR}^{j k} (k = i mod p) and its butterflies.
+template <int LOG2L, int G, int R>
+__device__ __forceinline__ void r16_butterflies(Regs16<LOG2L, G, R>& v,
+                                                const cplx* __restrict__ tw_l, int sign, int p) {
+  constexpr int L = 1 << LOG2L, NT = Shape16<LOG2L, G>::NT;
+  constexpr int NB = Stage16<LOG2L, G, R>::NB, PER = Stage16<LOG2L, G, R>::PER;
+  const int tid = threadIdx.x;
+  const int step = L / (p * R);
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int b = tid + r * NT;
+    if (b < NB) {
+      const int k = (b % (L / R)) & (p - 1);
       if (p > 1)
 #pragma unroll
         for (int j = 1; j < R; ++j) v[r][j] = mul(v[r][j], twiddle(tw_l, j * k * step, sign));
       dft<R>(v[r], sign);
     }
   }
-  if constexpr (!First && !Last) __syncthreads();
+}
+
+// Where output j of butterfly b of a stage with span p lands in its sequence.
+template <int LOG2L, int R>
+__device__ __forceinline__ int r16_out_point(int b, int j, int p) {
+  constexpr int L = 1 << LOG2L;
+  const int i = b % (L / R), k = i & (p - 1);
+  return (i - k) * R + k + j * p;
+}
+
+template <int LOG2L, int G, int R>
+__device__ __forceinline__ void r16_store(cplx* dst, const cplx* __restrict__ tw_n,
+                                          const PassArgs& a, int sign, long s0,
+                                          const Regs16<LOG2L, G, R>& v, int p) {
+  constexpr int L = 1 << LOG2L, NT = Shape16<LOG2L, G>::NT;
+  constexpr int NB = Stage16<LOG2L, G, R>::NB, PER = Stage16<LOG2L, G, R>::PER;
+  const int tid = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < PER; ++r) {
     const int b = tid + r * NT;
     if (b < NB) {
-      const int s = b / (L / R), i = b % (L / R);
-      const int k = i & (p - 1);
+      const int s = b / (L / R);
 #pragma unroll
       for (int j = 0; j < R; ++j) {
-        const int point = (i - k) * R + k + j * p;
+        const int point = r16_out_point<LOG2L, R>(b, j, p);
         cplx value = v[r][out_at<R>(j)];
-        if constexpr (Last) {
-          if (a.global_twiddle) {
-            const long idx = ((s0 + s) * static_cast<long>(point)) & a.mask_n;
-            value = mul(value, twiddle(tw_n, static_cast<int>(idx), sign));
-          }
-          dst[point * a.out_sk + (s0 + s) * a.out_ss] = value;
-        } else {
-          buf[s * STRIDE + padded(point)] = value;
+        if (a.global_twiddle) {
+          const long idx = ((s0 + s) * static_cast<long>(point)) & a.mask_n;
+          value = mul(value, twiddle(tw_n, static_cast<int>(idx), sign));
         }
+        dst[point * a.out_sk + (s0 + s) * a.out_ss] = value;
       }
     }
   }
-  if constexpr (!Last) __syncthreads();
 }
 
-// A radix-16 pass: stages of 16 points (the first from HBM, the last to HBM) and one
-// radix-2, 4 or 8 stage for what remains of L.
+// An LDS hand-over: a stage's outputs `w` (radix RW, span pw) written to the rows, a
+// barrier, and the next stage's inputs `v` (radix RR) read back.
+template <int LOG2L, int G, int RW, int RR>
+__device__ __forceinline__ void r16_move(const Regs16<LOG2L, G, RW>& w, int pw,
+                                         Regs16<LOG2L, G, RR>& v, cplx* lds) {
+  constexpr int L = 1 << LOG2L, NT = Shape16<LOG2L, G>::NT, STRIDE = Shape16<LOG2L, G>::STRIDE;
+  constexpr int NBW = Stage16<LOG2L, G, RW>::NB, PERW = Stage16<LOG2L, G, RW>::PER;
+  constexpr int NBR = Stage16<LOG2L, G, RR>::NB, PERR = Stage16<LOG2L, G, RR>::PER;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < PERW; ++r) {
+    const int b = tid + r * NT;
+    if (b < NBW) {
+      const int s = b / (L / RW);
+#pragma unroll
+      for (int j = 0; j < RW; ++j)
+        lds[s * STRIDE + padded(r16_out_point<LOG2L, RW>(b, j, pw))] = w[r][out_at<RW>(j)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PERR; ++r) {
+    const int b = tid + r * NT;
+    if (b < NBR) {
+      const int s = b / (L / RR), i = b % (L / RR);
+#pragma unroll
+      for (int j = 0; j < RR; ++j) v[r][j] = lds[s * STRIDE + padded(i + j * (L / RR))];
+    }
+  }
+}
+
+// The stages from `Stage` on, its inputs in `v` and its span p; `lds` holds
+// PassShape::LDS_POINTS points. A hand-over after the first starts with a barrier: the
+// previous hand-over's reads are done before the rows are written again.
 template <int LOG2L, int G, int Stage>
-__device__ __forceinline__ void r16_stages(const cplx* src, cplx* dst, const cplx* __restrict__ tw_l,
+__device__ __forceinline__ void r16_stages(Regs16<LOG2L, G, Radix16<LOG2L, Stage>::R>& v,
+                                           cplx* dst, const cplx* __restrict__ tw_l,
                                            const cplx* __restrict__ tw_n, const PassArgs& a,
-                                           int sign, long s0, int row, cplx* buf, int p) {
-  constexpr int full = LOG2L / 4;
-  constexpr int rest = (1 << LOG2L) >> (4 * full);
-  constexpr int stages = full + (rest > 1 ? 1 : 0);
-  constexpr int R = Stage < full ? 16 : rest;
-  constexpr bool first = Stage == 0, last = Stage == stages - 1;
-  pass_stage<LOG2L, G, R, first, last>(src, dst, tw_l, tw_n, a, sign, s0, row, buf, p);
-  if constexpr (!last)
-    r16_stages<LOG2L, G, Stage + 1>(src, dst, tw_l, tw_n, a, sign, s0, row, buf, p * R);
+                                           int sign, long s0, int p, cplx* lds) {
+  constexpr int R = Radix16<LOG2L, Stage>::R;
+  r16_butterflies<LOG2L, G, R>(v, tw_l, sign, p);
+  if constexpr (Radix16<LOG2L, Stage>::last) {
+    r16_store<LOG2L, G, R>(dst, tw_n, a, sign, s0, v, p);
+  } else {
+    constexpr int RN = Radix16<LOG2L, Stage + 1>::R;
+    Regs16<LOG2L, G, RN> next;
+    if constexpr (Stage > 0) __syncthreads();
+    r16_move<LOG2L, G, R, RN>(v, p, next, lds);
+    r16_stages<LOG2L, G, Stage + 1>(next, dst, tw_l, tw_n, a, sign, s0, p * R, lds);
+  }
 }
 
 template <int LOG2L, int G>
@@ -125,9 +200,11 @@ __device__ __forceinline__ void fft_pass_body(const cplx* in, cplx* out, const c
   using S = PassShape<LOG2L, G>;
   if constexpr (S::r16) {
     const long s0 = static_cast<long>(group) * G;
-    r16_stages<LOG2L, G, 0>(in + static_cast<long>(row) * a.batch_in,
-                            out + static_cast<long>(row) * a.batch_out, tw_l, tw_n, a, sign, s0,
-                            row, buf, 1);
+    constexpr int R0 = Radix16<LOG2L, 0>::R;
+    Regs16<LOG2L, G, R0> v;
+    r16_load<LOG2L, G, R0>(in + static_cast<long>(row) * a.batch_in, a, s0, row, v);
+    r16_stages<LOG2L, G, 0>(v, out + static_cast<long>(row) * a.batch_out, tw_l, tw_n, a, sign,
+                            s0, 1, buf);
     return;
   }
   constexpr int L = S::L, E = S::E, NT = S::NT, STRIDE = S::STRIDE;
@@ -179,7 +256,7 @@ __global__ void __launch_bounds__((PassShape<LOG2L, G>::NT))
                     const cplx* __restrict__ tw_l, const cplx* __restrict__ tw_n,
                     PassArgs a, int sign) {
   using S = PassShape<LOG2L, G>;
-  __shared__ cplx buf[G * S::STRIDE];
+  __shared__ cplx buf[S::LDS_POINTS];
   fft_pass_body<LOG2L, G>(in, out, tw_l, tw_n, a, sign, blockIdx.x, blockIdx.y, buf);
 }
 
@@ -191,7 +268,7 @@ __global__ void __launch_bounds__((PassShape<LOG2L, G>::NT))
     fft_pass_grouped(const FftJob* __restrict__ jobs, const int* __restrict__ first, int n_jobs,
                      int sign) {
   using S = PassShape<LOG2L, G>;
-  __shared__ cplx buf[G * S::STRIDE];
+  __shared__ cplx buf[S::LDS_POINTS];
   const int block = blockIdx.x;
   const int i = owner_of(first, n_jobs, block);
   const FftJob& job = jobs[i];
