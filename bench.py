@@ -106,7 +106,11 @@ def trial_bytes(problem, order):
         + (8 * C * C + 2 * ct if spatial else 0),
         "forward_fft": 2 * (n_streams + (in_stage or spatial)) * ct,
         "field_finish": n_streams * ct + cn * len(problem.prefactor_paths) + cn,
-        "adams_correct": cn * (n_phi + 1 + (W - 1) + 1 + (order + 1 if comp else 0)) + cn,
+        # A trial's corrector finishes each bin's source itself: it reads the forward
+        # transform's streams and the prefactor rows in place of the source, and writes
+        # the source beside the advanced field.
+        "adams_correct": cn * (n_phi + (W - 1) + 1 + (order + 1 if comp else 0))
+        + n_streams * ct + cn * len(problem.prefactor_paths) + 2 * cn,
         "adams_norms": 8 * (1 + 3) * ((C * N + (64 if C * N <= 4096 * 64 else 256) - 1)
                                       // (64 if C * N <= 4096 * 64 else 256)),
     }

@@ -1215,9 +1215,9 @@ __global__ void field_raman_drive(DeviceProblem P, FieldBuffers B) {
 // One coarse bin b of mode c from its two packed entries per stream (za at b, zb at its
 // mirror): unpack the stream spectra, assemble the held current on the active bins and
 // normalise, k = (p W) (P~ + G~/(iω) - E~/(iω)^2).
-__device__ inline void finish_bin(const DeviceProblem& P, const AdamsRecord& r,
-                                  const FieldBuffers& B, int c, int b, const cplx (&za)[4],
-                                  const cplx (&zb)[4], int n_streams, cplx* out) {
+__device__ inline cplx finish_value(const DeviceProblem& P, const AdamsRecord& r,
+                                    const FieldBuffers& B, int c, int b, const cplx (&za)[4],
+                                    const cplx (&zb)[4], int n_streams) {
   const long CN = static_cast<long>(P.C) * P.N;
   const long idx = static_cast<long>(c) * P.N + b;
   cplx X[3] = {};
@@ -1238,7 +1238,33 @@ __device__ inline void finish_bin(const DeviceProblem& P, const AdamsRecord& r,
     const cplx a = P.has_plasma ? za[3] : za[1], ab = P.has_plasma ? zb[3] : zb[1];
     value = add(value, r2c_post(a, ab, B.roots_T[b]));
   }
-  out[idx] = value;
+  return value;
+}
+
+__device__ inline void finish_bin(const DeviceProblem& P, const AdamsRecord& r,
+                                  const FieldBuffers& B, int c, int b, const cplx (&za)[4],
+                                  const cplx (&zb)[4], int n_streams, cplx* out) {
+  out[static_cast<long>(c) * P.N + b] = finish_value(P, r, B, c, b, za, zb, n_streams);
+}
+
+// The finish of one kept bin b < N of mode c alone: its two packed entries per stream
+// are row[b mod T/2] and row[(T/2 - b) mod T/2] (the Nyquist bin T/2 reads entry 0
+// twice, as its pair's thread does above).
+__device__ inline cplx finish_at(const DeviceProblem& P, const AdamsRecord& r,
+                                 const FieldBuffers& B, int c, int b) {
+  const int half = P.T / 2;
+  const int ka = b % half, kb = (half - b) % half;
+  const int n_streams = P.has_plasma ? 3 : 1;
+  cplx za[4] = {}, zb[4] = {};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s < n_streams || s == P.absorber_stream) {
+      const cplx* row = B.streams + (static_cast<long>(s) * P.C + c) * half;
+      za[s] = row[ka];
+      zb[s] = row[kb];
+    }
+  }
+  return finish_value(P, r, B, c, b, za, zb, n_streams);
 }
 
 // The finish over bin pairs: the packed real transform holds bins k and T/2 - k in the
@@ -1321,7 +1347,7 @@ __global__ void field_split_unpack(DeviceProblem P, FieldBuffers B) {
 // The corrector's partial sums of one workgroup, in its waves' LDS cells.
 using WaveSums = double[1 + kMaxScored][kCorrectThreads / kWave];
 
-template <int W>
+template <int W, bool Finish = false>
 __device__ __forceinline__ void correct_body(const DeviceProblem& P, const AdamsRecord* __restrict__ R,
                                     const FieldBuffers& B, int block, int blocks,
                                     WaveSums& wave_sums) {
@@ -1336,7 +1362,14 @@ __device__ __forceinline__ void correct_body(const DeviceProblem& P, const Adams
     const bool compensated = r.compensated != 0;
     // Loads.
     cplx X[W], carried[W], phi_t[kMaxTerms];
-    X[0] = B.source[idx];
+    if constexpr (Finish) {
+      // The source evaluation's finish for this bin, stored for the commit as the finish
+      // kernel would store it.
+      X[0] = finish_at(P, r, B, static_cast<int>(idx / P.N), b);
+      B.source[idx] = X[0];
+    } else {
+      X[0] = B.source[idx];
+    }
 #pragma unroll
     for (int k = 0; k < W; ++k) {
       if (k >= 1) X[k] = B.store[r.slots[k - 1] * CN + idx];
@@ -1407,11 +1440,11 @@ __device__ __forceinline__ void correct_body(const DeviceProblem& P, const Adams
   }
 }
 
-template <int W>
+template <int W, bool Finish>
 __global__ void __launch_bounds__(kCorrectThreads)
     adams_correct(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B) {
   __shared__ WaveSums wave_sums;
-  correct_body<W>(P, R, B, blockIdx.x, gridDim.x, wave_sums);
+  correct_body<W, Finish>(P, R, B, blockIdx.x, gridDim.x, wave_sums);
 }
 
 // The corrector's block sums reduced to the 1 + kMaxScored totals in a fixed order, written
@@ -2137,19 +2170,30 @@ void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuff
   check();
 }
 
-void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
-                          int width, hipStream_t s) {
-  SP_CHECK(width >= 2 && width <= kMaxAdamsWidth - 1, "stencil width 2..7");
+namespace {
+template <bool Finish>
+void launch_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B, int width,
+                    hipStream_t s) {
   const dim3 grid(correct_blocks(P)), block(correct_threads(P));
   switch (width) {
-    case 2: hipLaunchKernelGGL(adams_correct<2>, grid, block, 0, s, P, R, B); break;
-    case 3: hipLaunchKernelGGL(adams_correct<3>, grid, block, 0, s, P, R, B); break;
-    case 4: hipLaunchKernelGGL(adams_correct<4>, grid, block, 0, s, P, R, B); break;
-    case 5: hipLaunchKernelGGL(adams_correct<5>, grid, block, 0, s, P, R, B); break;
-    case 6: hipLaunchKernelGGL(adams_correct<6>, grid, block, 0, s, P, R, B); break;
-    default: hipLaunchKernelGGL(adams_correct<7>, grid, block, 0, s, P, R, B); break;
+    case 2: hipLaunchKernelGGL((adams_correct<2, Finish>), grid, block, 0, s, P, R, B); break;
+    case 3: hipLaunchKernelGGL((adams_correct<3, Finish>), grid, block, 0, s, P, R, B); break;
+    case 4: hipLaunchKernelGGL((adams_correct<4, Finish>), grid, block, 0, s, P, R, B); break;
+    case 5: hipLaunchKernelGGL((adams_correct<5, Finish>), grid, block, 0, s, P, R, B); break;
+    case 6: hipLaunchKernelGGL((adams_correct<6, Finish>), grid, block, 0, s, P, R, B); break;
+    default: hipLaunchKernelGGL((adams_correct<7, Finish>), grid, block, 0, s, P, R, B); break;
   }
   check();
+}
+}  // namespace
+
+void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                          int width, hipStream_t s, bool finish) {
+  SP_CHECK(width >= 2 && width <= kMaxAdamsWidth - 1, "stencil width 2..7");
+  if (finish)
+    launch_correct<true>(P, R, B, width, s);
+  else
+    launch_correct<false>(P, R, B, width, s);
 }
 
 void launch_adams_norms(const DeviceProblem& P, FieldBuffers B, hipStream_t s) {

@@ -239,8 +239,10 @@ void launch_node_passes_fused(const DeviceProblem& P, const AdamsRecord* R,
 void launch_field_finish(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                          cplx* out, hipStream_t s);
 // The corrector, writing its block sums to B.partials; then their reduction into B.norms.
+// With `finish` the corrector first finishes its bin's source from the forward transform's
+// streams (launch_field_finish into B.source, bin by bin) instead of reading B.source.
 void launch_adams_correct(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
-                          int width, hipStream_t s);
+                          int width, hipStream_t s, bool finish = false);
 void launch_adams_norms(const DeviceProblem& P, FieldBuffers B, hipStream_t s);
 // The low-storage RK4IP step's register passes (1..4, each after its source evaluation)
 // and the three masked sums they leave (|S2 - g|^2, |S3 - S2|^2, |A|^2) into `norms`.

@@ -386,7 +386,8 @@ void FieldWorkspace::enqueue_raman(const cplx* spectrum, const AdamsRecord* R) {
   raman_plan_.exec(B_.raman_drive, raman_scratch_, +1, stream_);
 }
 
-void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out, const AdamsRecord* R) {
+void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out, const AdamsRecord* R,
+                                    bool finish) {
   const AdamsRecord* rec = R ? R : rec_dev_;
   mark(2);
   // The half spectrum is packed for the real inverse transform as its first pass loads it.
@@ -407,7 +408,7 @@ void FieldWorkspace::enqueue_source(const cplx* spectrum, cplx* out, const Adams
   mark(12);
   fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
   mark(13);
-  launch_field_finish(P_, rec, B_, out, stream_);
+  if (finish) launch_field_finish(P_, rec, B_, out, stream_);
   mark(14);
 }
 
@@ -423,8 +424,9 @@ void FieldWorkspace::enqueue_trial(bool commit) {
   mark(0);
   launch_adams_predict(P_, rec_dev_, B_, width_, stream_, commit && !separate);
   mark(1);
-  enqueue_source(B_.lab, B_.source);
-  launch_adams_correct(P_, rec_dev_, B_, width_, stream_);
+  // The corrector finishes each bin's source itself (into B.source, for the commit).
+  enqueue_source(B_.lab, B_.source, nullptr, false);
+  launch_adams_correct(P_, rec_dev_, B_, width_, stream_, true);
   mark(15);
   launch_adams_norms(P_, B_, stream_);
   mark(16);
@@ -510,6 +512,7 @@ bool FieldWorkspace::phase_used(int i) const {
     case 5: case 6: case 7: case 8: return P_.has_plasma != 0;
     case 9: return !fused_node_passes(P_, B_);  // the fused passes have no response of its own
     case 11: return absorbing();
+    case 13: return false;  // a trial's corrector finishes the source
     default: return true;
   }
 }

@@ -91,7 +91,10 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   void fill_position(double z, AdamsRecord* into = nullptr);
   // The source at a lab-frame spectrum into `out`, its position's scalars read from `R`
   // (the workspace's record when null).
-  void enqueue_source(const cplx* spectrum, cplx* out, const AdamsRecord* R = nullptr);
+  // The source evaluation into `out`; without `finish` it stops at the forward transform's
+  // streams (the corrector then finishes each bin itself).
+  void enqueue_source(const cplx* spectrum, cplx* out, const AdamsRecord* R = nullptr,
+                      bool finish = true);
   void enqueue_synthesis();
   void enqueue_projection();
   void enqueue_raman_kernel(const AdamsRecord* R);
