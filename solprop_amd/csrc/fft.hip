@@ -37,7 +37,10 @@ struct PassShape {
   static constexpr int E = L * G;
   static constexpr int NT = r16 ? Shape16<LOG2L, G>::NT : Shape<LOG2L, G>::NT;
   static constexpr int STRIDE = r16 ? Shape16<LOG2L, G>::STRIDE : Shape<LOG2L, G>::STRIDE;
-  static constexpr int LDS_POINTS = G * STRIDE;
+  // The radix-4 shapes of at least 256 points also stage the pass's L twiddles after the
+  // rows (shorter sequences take theirs from the table: staging them measured slower).
+  static constexpr bool stage_tw = !r16 && L >= 256;
+  static constexpr int LDS_POINTS = G * STRIDE + (stage_tw ? L : 0);
 };
 
 // A radix-16 pass (see the radix-16 stages in fft_lds.hpp) keeps each stage's values in
@@ -213,6 +216,35 @@ __device__ __forceinline__ void fft_pass_body(const cplx* in, cplx* out, const c
   const cplx* src = in + static_cast<long>(row) * a.batch_in;
   cplx* dst = out + static_cast<long>(row) * a.batch_out;
 
+  // Where store e of the workgroup goes: sequence s, point k.
+  auto store_at = [&](int e, int& sq, int& k) {
+    if (a.out_ss == 1) {
+      sq = e % G;
+      k = e / G;
+    } else {
+      k = e % L;
+      sq = e / L;
+    }
+  };
+  // Loads that do not wait on the data are issued with it: the stages' twiddles into LDS
+  // after the rows (a stage would otherwise wait for its own after each barrier) and,
+  // for a few stores a thread, the four-step twiddles into registers.
+  constexpr int PE = (E + NT - 1) / NT;
+  constexpr bool held = PE <= 4;
+  cplx* tw = S::stage_tw ? buf + G * STRIDE : nullptr;
+  cplx gw[held ? PE : 1];
+  if (held && a.global_twiddle) {
+#pragma unroll
+    for (int r = 0; r < PE; ++r) {
+      const int e = tid + r * NT;
+      if (e < E) {
+        int sq, k;
+        store_at(e, sq, k);
+        const long idx = ((s0 + sq) * static_cast<long>(k)) & a.mask_n;
+        gw[r] = twiddle(tw_n, static_cast<int>(idx), sign);
+      }
+    }
+  }
   // Load G sequences, the contiguous axis fastest across the threads.
   for (int e = tid; e < E; e += NT) {
     int s, j;
@@ -227,26 +259,30 @@ __device__ __forceinline__ void fft_pass_body(const cplx* in, cplx* out, const c
     buf[s * STRIDE + j] =
         a.spec ? packed_point(a, static_cast<long>(row) * a.batch_in + at) : src[at];
   }
+  if constexpr (S::stage_tw)
+    for (int e = tid; e < L; e += NT) tw[e] = tw_l[e];
   __syncthreads();
 
-  lds_transform<LOG2L, G>(buf, tw_l, sign);
+  lds_transform<LOG2L, G>(buf, S::stage_tw ? tw : tw_l, sign);
 
   // Store, with the four-step twiddle W_N^{s k} where asked.
-  for (int e = tid; e < E; e += NT) {
-    int s, k;
-    if (a.out_ss == 1) {
-      s = e % G;
-      k = e / G;
-    } else {
-      k = e % L;
-      s = e / L;
+#pragma unroll
+  for (int r = 0; r < PE; ++r) {
+    const int e = tid + r * NT;
+    if (e < E) {
+      int sq, k;
+      store_at(e, sq, k);
+      cplx value = buf[sq * STRIDE + k];
+      if (a.global_twiddle) {
+        if constexpr (held) {
+          value = mul(value, gw[r]);
+        } else {
+          const long idx = ((s0 + sq) * static_cast<long>(k)) & a.mask_n;
+          value = mul(value, twiddle(tw_n, static_cast<int>(idx), sign));
+        }
+      }
+      dst[k * a.out_sk + (s0 + sq) * a.out_ss] = value;
     }
-    cplx value = buf[s * STRIDE + k];
-    if (a.global_twiddle) {
-      const long idx = ((s0 + s) * static_cast<long>(k)) & a.mask_n;
-      value = mul(value, twiddle(tw_n, static_cast<int>(idx), sign));
-    }
-    dst[k * a.out_sk + (s0 + s) * a.out_ss] = value;
   }
 }
 

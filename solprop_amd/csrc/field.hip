@@ -2215,6 +2215,30 @@ void launch_adams_commit(const DeviceProblem& P, const AdamsRecord* R, FieldBuff
   check();
 }
 
+namespace {
+// The record's upload as a kernel node: each thread copies 8-byte words of the pinned
+// host record, read at system scope (no cache can hold an earlier launch's record), into
+// device memory. A copy node would be a blit launch with a wait before the next kernel.
+constexpr int kFetchThreads = 512;
+static_assert(sizeof(AdamsRecord) % sizeof(unsigned long long) == 0,
+              "the record is copied in 8-byte words");
+
+__global__ void __launch_bounds__(kFetchThreads)
+    record_fetch(const unsigned long long* host, unsigned long long* device, int words) {
+  for (int i = threadIdx.x; i < words; i += kFetchThreads)
+    device[i] = __hip_atomic_load(host + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+}  // namespace
+
+void launch_record_fetch(const AdamsRecord* host, AdamsRecord* device, int count,
+                         hipStream_t s) {
+  const int words = static_cast<int>(count * sizeof(AdamsRecord) / sizeof(unsigned long long));
+  hipLaunchKernelGGL(record_fetch, dim3(1), dim3(kFetchThreads), 0, s,
+                     reinterpret_cast<const unsigned long long*>(host),
+                     reinterpret_cast<unsigned long long*>(device), words);
+  check();
+}
+
 void launch_adams_start(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                         hipStream_t s) {
   hipLaunchKernelGGL(adams_start, dim3(cn_blocks(P)), dim3(kThreads), 0, s, P, R, B);

@@ -261,6 +261,10 @@ void launch_adams_accept(const DeviceProblem& P, int slot, bool compensated, Fie
 // The commit with its slot read from the record (`free_slot`).
 void launch_adams_commit(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                          hipStream_t s);
+// The upload of `count` records from pinned host memory (`host` as the device addresses
+// it) to `device`, as a kernel of one workgroup that reads them at system scope.
+void launch_record_fetch(const AdamsRecord* host, AdamsRecord* device, int count,
+                         hipStream_t s);
 void launch_adams_start(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                         hipStream_t s);
 

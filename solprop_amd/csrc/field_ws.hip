@@ -263,6 +263,7 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   rec_dev_ = alloc<AdamsRecord>(1);
   SP_HIP(hipHostMalloc(reinterpret_cast<void**>(&own_rec_host_), sizeof(AdamsRecord)));
   rec_host_ = own_rec_host_;
+  SP_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&rec_host_dev_), rec_host_, 0));
   std::memset(rec_host_, 0, sizeof(AdamsRecord));
   if (low_storage_) {
     const std::size_t CN = static_cast<std::size_t>(C) * N;
@@ -309,6 +310,7 @@ void FieldWorkspace::release() {
   graphs_.clear();
   allocations_.clear();
   rec_host_ = nullptr;
+  rec_host_dev_ = nullptr;
   own_rec_host_ = nullptr;
   norms_host_ = nullptr;
   stream_ = nullptr;
@@ -440,8 +442,7 @@ void FieldWorkspace::enqueue_start() {
 void FieldWorkspace::enqueue(int key) {
   // The record's upload is the graph's first node: the host fills the pinned record,
   // then launches one graph.
-  SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
-                        stream_));
+  launch_record_fetch(rec_host_dev_, rec_dev_, 1, stream_);
   const int kind = key & 15;
   if (kind == kStart)
     enqueue_start();
@@ -1055,6 +1056,7 @@ std::string FieldWorkspace::batch_refusal() const {
 void FieldWorkspace::batch_bind(AdamsRecord* host, AdamsRecord* device) {
   *host = *rec_host_;
   rec_host_ = host;
+  SP_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&rec_host_dev_), rec_host_, 0));
   rec_dev_ = device;
   // The workspace's own graphs read its own record: they are not used again.
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);

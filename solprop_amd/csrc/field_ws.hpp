@@ -138,6 +138,7 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   cplx* scratch_ = nullptr;
   AdamsRecord* rec_dev_ = nullptr;
   AdamsRecord* rec_host_ = nullptr;
+  AdamsRecord* rec_host_dev_ = nullptr;  // rec_host_ as the device addresses it
   AdamsRecord* own_rec_host_ = nullptr;  // the workspace's pinned record (freed at release)
   double* norms_host_ = nullptr;
   int n_blocks_ = 0;
