@@ -529,15 +529,31 @@ __device__ inline int extra_sample(int tile, int K, int T, int h) {
 // rate's trapezoid increments scanned, the within-tile exposure stored and the tile's
 // sum.
 // LDS a band pass's workgroup holds: each wave's window and its nodes' extra samples.
-struct BandLds {
+template <int NPW>
+struct BandLdsN {
   double windows[kNodeWaves][kBandSpan];
-  double halos[kNodeWaves][kNodesPerWave * kBandSlots];
+  double halos[kNodeWaves][NPW * kBandSlots];
 };
+using BandLds = BandLdsN<kNodesPerWave>;
 
+// The nodes a wave of the rate pass runs: on the modal route half as many as the other
+// band passes, for more waves to hide the loads' latency (at 2^14 samples and 128 nodes
+// 15.7 us against 17.5 with four and 19.3 with one); on the dense route (C = 0) as many
+// (26.8 us against 24.5 with two at the radial grid).
 template <int C>
+constexpr int rate_nodes_per_wave() {
+  return C == 0 ? kNodesPerWave : kNodesPerWave / 2;
+}
+template <int C>
+inline int rate_groups(int Q) {
+  constexpr int per_group = kNodeWaves * rate_nodes_per_wave<C>();
+  return (Q + per_group - 1) / per_group;
+}
+
+template <int C, int NPW = kNodesPerWave>
 __device__ __forceinline__ void rate_band_body(const DeviceProblem& P, const PlasmaTable& pt,
                                       const FieldBuffers& B, int tile, int group, double* table,
-                                      BandLds& lds, bool sparse) {
+                                      BandLdsN<NPW>& lds, bool sparse) {
   auto& windows = lds.windows;
   auto& halos = lds.halos;
   (void)table;
@@ -551,13 +567,13 @@ __device__ __forceinline__ void rate_band_body(const DeviceProblem& P, const Pla
   const bool step = lane < kTileSpan && t + 1 < T;
   double* window = windows[wave];
   double* halo = halos[wave];
-  const int q0 = group * kNodeWaves * kNodesPerWave + wave;
+  const int q0 = group * kNodeWaves * NPW + wave;
   const int S = H + (tile == tiles - 1 ? K : 0);  // extra samples a node needs
   // The lane's own modal fields first, so that their loads overlap the halo's (up to four
   // modes: more held across the halo would spill).
   double am[held(C)];
   if constexpr (C <= 4) load_modal<C>(B, T, t, am);
-  for (int idx = lane; idx < kNodesPerWave * S; idx += kWave) {
+  for (int idx = lane; idx < NPW * S; idx += kWave) {
     const int i = idx / S, h = idx - i * S;
     const int q = q0 + i * kNodeWaves;
     double v = 0.0;
@@ -569,7 +585,7 @@ __device__ __forceinline__ void rate_band_body(const DeviceProblem& P, const Pla
   }
   wave_sync();
   if constexpr (C > 4) load_modal<C>(B, T, t, am);
-  for (int i = 0; i < kNodesPerWave; ++i) {
+  for (int i = 0; i < NPW; ++i) {
     const int q = q0 + i * kNodeWaves;
     if (q >= P.Q) break;  // uniform across the wave, and so for every later node
     double w = 0.0;
@@ -606,8 +622,9 @@ template <int C>
 __global__ void __launch_bounds__(kWave * kNodeWaves)
     __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4)))
     plasma_rate_band(DeviceProblem P, PlasmaTable pt, FieldBuffers B, int sparse) {
-  __shared__ BandLds lds;
-  rate_band_body<C>(P, pt, B, blockIdx.x, blockIdx.y, nullptr, lds, sparse != 0);
+  __shared__ BandLdsN<rate_nodes_per_wave<C>()> lds;
+  rate_band_body<C, rate_nodes_per_wave<C>()>(P, pt, B, blockIdx.x, blockIdx.y, nullptr, lds,
+                                              sparse != 0);
 }
 
 // Pass C under the band rule: the free-current integrand (the fraction from the stored
@@ -1891,7 +1908,8 @@ void launch_node_passes_for(const DeviceProblem& P, const AdamsRecord* R,
     SP_CHECK(!band || (B.x_local && pt.n_taps <= kMaxBandHalf + 1 && 64 + 2 * (pt.n_taps - 1) <= kBandSpan),
              "the band rule needs its exposure buffer and at most 8 taps each side");
     if (band)
-      hipLaunchKernelGGL(plasma_rate_band<C>, node_grid, node_block, 0, s, P, pt, B, 0);
+      hipLaunchKernelGGL(plasma_rate_band<C>, dim3(tiles, rate_groups<C>(P.Q)), node_block, 0, s,
+                         P, pt, B, 0);
     else
       hipLaunchKernelGGL(plasma_rate_tiles<C>, node_grid, node_block, 0, s, P, pt, B);
     check();
@@ -2068,7 +2086,8 @@ void launch_node_passes_fused_for(const DeviceProblem& P, const AdamsRecord* R,
   auto mark = [&](int i) {
     if (marks) SP_HIP(hipEventRecord(marks[i], s));
   };
-  hipLaunchKernelGGL(plasma_rate_band<C>, node_grid, node_block, 0, s, P, pt, B, 1);
+  hipLaunchKernelGGL(plasma_rate_band<C>, dim3(tiles, rate_groups<C>(P.Q)), node_block, 0, s, P,
+                     pt, B, 1);
   check();
   mark(0);
   hipLaunchKernelGGL(plasma_tile_prefix, dim3(P.Q), dim3(kWave), 0, s, tiles, B.tile_x);
