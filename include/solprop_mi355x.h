@@ -149,6 +149,12 @@ typedef struct sp_problem_desc {
    * (n_phi = 1, W(z) = z), one prefactor table, the analytic Kerr response, method 2.
    * 0 otherwise. */
   int grid_nx, grid_ny;
+  /* The embedded coarse rule's projection (Problem.Transform.projection_coarse_matrix,
+   * _core.pyi:222; src/solprop/modes/quadrature.py:265-332): (C, Q), the Gauss weights of
+   * the rule embedded in a Gauss-Kronrod radial rule on the nodes the two share, zero on
+   * the others. It serves the quadrature estimate (statistics flag 1 << 10) alone. NULL
+   * when the rule has no embedded pair (the estimate is then NaN). */
+  const double* projection_coarse;
 } sp_problem_desc;
 
 typedef struct sp_workspace sp_workspace;
@@ -402,10 +408,16 @@ int sp_workspace_statistics(sp_workspace* ws, unsigned long long flags, const do
  * the nodes of the column at the committed state — the exposure increments' largest
  * dip max(-min dS, 0) and rise max dS, the largest |paid - freed| (paid the trapezoid of
  * W (1 - f), freed f_end - f_0) and freed, and the largest |f E| over the first 2K
- * samples and anywhere. Packed in the native order: ..., electron_density,
- * peak_ionisation_rate, peak_intensity (1), the fwhm trace (n_t), populations
- * (2 n_nodes), the integration diagnostics (6). quadrature_error (1 << 10) is refused
- * with SP_VALUE, and so are the plasma statistics on a problem without a plasma column. */
+ * samples and anywhere. And quadrature_error (1 << 10; trial_finish.inc:921-1028): the
+ * last source evaluation of the march's last accepted step (its field and position)
+ * repeated with the projection and with projection_coarse, each response spectrum
+ * weighted by the prefactor times the spectral window, sqrt(sum |w (coarse - fine)|^2 /
+ * sum |w fine|^2) over modes and bins; NaN without projection_coarse, before the first
+ * accepted step, on the envelope route or when the fine spectrum vanishes. Packed in the
+ * native order: ..., electron_density, peak_ionisation_rate, peak_intensity (1),
+ * quadrature_error (1), the fwhm trace (n_t), populations (2 n_nodes), the integration
+ * diagnostics (6). The plasma statistics on a problem without a plasma column are
+ * refused with SP_VALUE. */
 typedef struct sp_statistics_request {
   unsigned long long flags;
   const double* windows;

@@ -94,7 +94,26 @@ class StageSource:
     def evaluate(self, lab, z, inverse_propagator):
         """The stage derivative in the anchor's frame, from a lab-frame field at z."""
         p = self.p
+        crop = p.crop_index.astype(np.intp)
+        spectrum, modal_t = self.projected_spectrum(lab, z)
+        absorber = None
+        if p.absorber_scheme == "in_stage" or p.spatial_absorber is not None:
+            # ADR-025's in-stage absorber, a term of its own (``source.py:391-401``),
+            # with the radial collar's matrix product added to it (``source.py:133``).
+            absorber = P.absorber_term(
+                modal_t, p.absorber_rate if p.absorber_scheme == "in_stage" else None,
+                crop, p.active_mask, kind=p.kind, radial=p.spatial_absorber)
+        return P.stage_output(
+            spectrum, self.prefactor(z), p.omega_window, inverse_propagator, absorber
+        )
+
+    def projected_spectrum(self, lab, z, projection=None):
+        """The response's spectrum on the coarse bins before the prefactor: the projected
+        polarisation's transform plus the held plasma currents' (with `projection` in
+        place of the problem's when given); and the modal time fields."""
+        p = self.p
         kind = p.kind
+        projection = p.projection if projection is None else projection
         crop = p.crop_index.astype(np.intp)
         modal_t = P.to_time(lab, crop, p.n_to, kind)
         raman = self.raman_kernel(z)
@@ -114,26 +133,17 @@ class StageSource:
             preionisation=p.preionisation(z),
             dt=p.dt_fine,
         )
-        projected = P.project(sources.polarisation, p.projection, 1.0, 1)
+        projected = P.project(sources.polarisation, projection, 1.0, 1)
         spectrum = P.to_frequency(projected, crop, kind)
         if sources.current is not None:
-            gated = P.project(p.time_window * sources.current, p.projection, 1.0, 1)
+            gated = P.project(p.time_window * sources.current, projection, 1.0, 1)
             edge = P.project(
-                p.time_window_derivative * sources.current, p.projection, 1.0, 1
+                p.time_window_derivative * sources.current, projection, 1.0, 1
             )
             spectrum = spectrum + P.held_plasma_spectrum(
                 gated, edge, p.omega, crop, p.active_mask
             )
-        absorber = None
-        if p.absorber_scheme == "in_stage" or p.spatial_absorber is not None:
-            # ADR-025's in-stage absorber, a term of its own (``source.py:391-401``),
-            # with the radial collar's matrix product added to it (``source.py:133``).
-            absorber = P.absorber_term(
-                modal_t, p.absorber_rate if p.absorber_scheme == "in_stage" else None,
-                crop, p.active_mask, kind, radial=p.spatial_absorber)
-        return P.stage_output(
-            spectrum, self.prefactor(z), p.omega_window, inverse_propagator, absorber
-        )
+        return spectrum, modal_t
 
 
 def delta_phi(problem, z0, z1):

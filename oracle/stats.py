@@ -20,13 +20,21 @@ of shape ``(n_modes, n_omega)`` (one polarisation), unit modal power weight:
   (``statistics.cpp:517-603``).
 - :func:`monitor_energies` and :func:`split_sums` restate the boundary monitor's
   reductions (``src/solprop/monitor.py:493-606``).
+- :func:`quadrature_error` restates the quadrature estimate of a source evaluation
+  (``native/src/general/trial_finish.inc:921-1028``, ``numpy_ref/source.py:929-995``):
+  ``|w (F_coarse - F_fine)| / |w F_fine|`` over every mode and coarse bin, ``F`` the
+  response's spectrum projected with the Kronrod and the embedded Gauss weights, ``w``
+  the prefactor at the evaluation's position times the spectral window; NaN without an
+  embedded rule or where the fine spectrum vanishes.
 
 Pins (``tests/test_oracle.py``): Parseval's identity with the analytic signal's time
 sum, the window tiling summing to the total, the centroid of a single bin, a sampled
 envelope's peak, and an empty column's NaN; the monitor's total against the reference's
 docstring identity, and its removal against the split collar's own factor; the peak
 intensity of a one-hot row against the peak power of that mode and of a row of ones
-against the coherent sum's peak; the populations' densest row against the plasma pair.
+against the coherent sum's peak; the populations' densest row against the plasma pair;
+the quadrature estimate of a Kerr response launched in one mode against the closed form
+its separable radial integrals give (``tests/test_quadrature.py``).
 """
 
 from __future__ import annotations
@@ -210,4 +218,26 @@ def plasma_integration_statistics(parts) -> tuple[float, float, float]:
 __all__ = ["analytic_field", "energy", "energy_window", "monitor_energies", "monitor_power",
            "integration_parts", "ionisation", "omega0", "peak_intensity", "peak_power",
            "peak_power_total", "peak_power_window", "plasma_column", "plasma_integration_parts",
-           "plasma_integration_statistics", "populations", "power_trace", "split_sums"]
+           "plasma_integration_statistics", "populations", "power_trace",
+           "quadrature_error", "split_sums"]
+
+
+def quadrature_error(problem, lab, z) -> float:
+    """The quadrature estimate of the source evaluation at the lab-frame field ``lab``
+    (``(n_modes, n_omega)``) and position ``z``: the response's spectrum projected with
+    the problem's projection and with its embedded coarse one, both weighted by the
+    prefactor at z times the spectral window, ``sqrt(sum |w (c - f)|^2 / sum |w f|^2)``;
+    NaN without an embedded rule or when the fine spectrum is zero."""
+    from oracle import rk
+
+    if problem.projection_coarse is None:
+        return float("nan")
+    src = rk.StageSource(problem)
+    lab = np.asarray(lab).reshape(problem.shape)
+    fine, _ = src.projected_spectrum(lab, z)
+    coarse, _ = src.projected_spectrum(lab, z, problem.projection_coarse)
+    weight = src.prefactor(z) * problem.omega_window
+    scale = float(np.sum(np.abs(weight * fine) ** 2))
+    if scale == 0.0:
+        return float("nan")
+    return float(np.sqrt(np.sum(np.abs(weight * (coarse - fine)) ** 2) / scale))

@@ -75,6 +75,7 @@ class ProblemDescC(C.Structure):
         ("absorber_scheme", _I), ("absorber_rate", _PD), ("spatial_absorber", _PD),
         ("n_schedule", _I), ("schedule_edges", _PD),
         ("grid_nx", _I), ("grid_ny", _I),
+        ("projection_coarse", _PD),
     ]
 
 

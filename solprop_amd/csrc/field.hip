@@ -2249,6 +2249,43 @@ __global__ void __launch_bounds__(kFetchThreads)
 }
 }  // namespace
 
+namespace {
+constexpr int kScoreThreads = 256;
+
+// Each thread sums its strided bins in order, then a fixed pairwise tree in LDS.
+__global__ void __launch_bounds__(kScoreThreads)
+    quadrature_score(const cplx* __restrict__ fine, const cplx* __restrict__ coarse, long n,
+                     double* out) {
+  __shared__ double diff[kScoreThreads], base[kScoreThreads];
+  const int tid = threadIdx.x;
+  double d = 0.0, f = 0.0;
+  for (long i = tid; i < n; i += kScoreThreads) {
+    const cplx a = fine[i], b = coarse[i];
+    const double dx = b.x - a.x, dy = b.y - a.y;
+    d = d + (dx * dx + dy * dy);
+    f = f + (a.x * a.x + a.y * a.y);
+  }
+  diff[tid] = d;
+  base[tid] = f;
+  __syncthreads();
+  for (int half = kScoreThreads / 2; half > 0; half /= 2) {
+    if (tid < half) {
+      diff[tid] = diff[tid] + diff[tid + half];
+      base[tid] = base[tid] + base[tid + half];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) out[0] = base[0] != 0.0 ? sqrt(diff[0] / base[0]) : __builtin_nan("");
+}
+}  // namespace
+
+void launch_quadrature_score(const cplx* fine, const cplx* coarse, long n, double* out,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(quadrature_score, dim3(1), dim3(kScoreThreads), 0, s, fine, coarse, n,
+                     out);
+  check();
+}
+
 void launch_record_fetch(const AdamsRecord* host, AdamsRecord* device, int count,
                          hipStream_t s) {
   const int words = static_cast<int>(count * sizeof(AdamsRecord) / sizeof(unsigned long long));

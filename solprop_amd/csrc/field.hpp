@@ -261,6 +261,11 @@ void launch_adams_accept(const DeviceProblem& P, int slot, bool compensated, Fie
 // The commit with its slot read from the record (`free_slot`).
 void launch_adams_commit(const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
                          hipStream_t s);
+// The quadrature estimate from two finished response spectra of n bins:
+// sqrt(sum |coarse - fine|^2 / sum |fine|^2) into out[0] (NaN when the fine sum is zero),
+// one workgroup summing in a fixed order.
+void launch_quadrature_score(const cplx* fine, const cplx* coarse, long n, double* out,
+                             hipStream_t s);
 // The upload of `count` records from pinned host memory (`host` as the device addresses
 // it) to `device`, as a kernel of one workgroup that reads them at system scope.
 void launch_record_fetch(const AdamsRecord* host, AdamsRecord* device, int count,

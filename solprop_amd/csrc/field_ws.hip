@@ -178,6 +178,8 @@ void FieldWorkspace::allocate(const sp_problem_desc& d) {
   P_.pref = upload(reinterpret_cast<const cplx*>(d.prefactor_terms), d.n_prefactor * CN);
   P_.synthesis = upload(d.synthesis, static_cast<std::size_t>(Q) * C);
   P_.projection = upload(d.projection, static_cast<std::size_t>(Q) * C);
+  if (d.projection_coarse)
+    projection_coarse_ = upload(d.projection_coarse, static_cast<std::size_t>(Q) * C);
   if (d.has_plasma) {
     pt_.log_field_min = d.log_field_min;
     pt_.d_log_field = d.d_log_field;
@@ -637,6 +639,9 @@ std::vector<double> FieldWorkspace::trial_norms() {
 
 std::vector<double> FieldWorkspace::trial(double z, double dz, const adams::Stencil& s) {
   fill_trial(z, dz, s);
+  // The trial's source evaluation replaces B.lab: the last accepted one's is gone.
+  quad_ready_ = false;
+  trial_end_ = z + dz;
   run((pending_slot_ >= 0 ? kCommitTrial : kTrial) + 16 * width_);
   pending_slot_ = -1;
   if (marking_)
@@ -664,6 +669,8 @@ void FieldWorkspace::accept(double dz, int free, const adams::Stencil&,
   rec_host_->accepted_dz = dz;
   slot_rates_[free] = held_rate_end_;
   accepted_ += 1;
+  quad_ready_ = true;
+  quad_z_ = trial_end_;
 }
 
 // ------------------------------------------------ the low-storage RK4IP replay
@@ -796,6 +803,7 @@ void FieldWorkspace::load(double z, const double* field) {
   }
   z_ = z;
   loaded_ = true;
+  quad_ready_ = false;
 }
 
 double FieldWorkspace::probe_trial(double z, double dz) {
@@ -993,7 +1001,8 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
   std::vector<double> trace;
   double intensity = 0.0;
   std::vector<double> out =
-      stats_->run(B_.y, flags & ~kStatColumn, windows, n_windows, stream_, &trace, &extras,
+      stats_->run(B_.y, flags & ~(kStatColumn | kStatQuadrature), windows, n_windows, stream_,
+                  &trace, &extras,
                   &intensity);
   std::vector<double> populations, diagnostics;
   if (flags & kStatColumn) {
@@ -1035,10 +1044,54 @@ std::vector<double> FieldWorkspace::statistics(std::uint64_t flags, const double
     if (flags & kStatPeakIonisationRate) out.push_back(densest[1]);
   }
   if (flags & kStatPeakIntensity) out.push_back(intensity);
+  if (flags & kStatQuadrature) out.push_back(quadrature_estimate());
   out.insert(out.end(), trace.begin(), trace.end());
   out.insert(out.end(), populations.begin(), populations.end());
   out.insert(out.end(), diagnostics.begin(), diagnostics.end());
   return out;
+}
+
+void FieldWorkspace::enqueue_response_spectrum(const DeviceProblem& P, cplx* out) {
+  inv_plan_.exec_packed(PackedSpectrum{B_.lab, B_.roots_T, P_.N, P_.inv_T}, B_.modal_t,
+                        scratch_, +1, stream_);
+  enqueue_synthesis();
+  enqueue_raman(B_.lab, rec_dev_);
+  if (fused_node_passes(P, B_))
+    launch_node_passes_fused(P, rec_dev_, pt_, B_, stream_);
+  else
+    launch_node_passes(P, rec_dev_, pt_, B_, stream_);
+  if (dense_)
+    launch_collar_projection(P.C, P.T, P.Q, P.projection, B_.node_streams, P.has_plasma != 0,
+                             P.t_window, P.t_deriv, reinterpret_cast<double*>(B_.streams),
+                             stream_);
+  fwd_plan_.exec(B_.streams, scratch_, -1, stream_);
+  launch_field_finish(P, rec_dev_, B_, out, stream_);
+}
+
+double FieldWorkspace::quadrature_estimate() {
+  // NaN without an embedded rule, before an accepted step since the state was loaded,
+  // and on the low-storage replay (whose last stage field this estimate does not keep).
+  if (!projection_coarse_ || !quad_ready_ || low_storage_) return std::nan("");
+  const std::size_t CN = static_cast<std::size_t>(P_.C) * P_.N;
+  if (!quad_fine_) {
+    quad_fine_ = alloc<cplx>(CN);
+    quad_coarse_ = alloc<cplx>(CN);
+    quad_out_ = alloc<double>(1);
+  }
+  // The accepted trial's position terms (prefactor, density, preionisation, Raman rates).
+  fill_position(quad_z_);
+  SP_HIP(hipMemcpyAsync(rec_dev_, rec_host_, sizeof(AdamsRecord), hipMemcpyHostToDevice,
+                        stream_));
+  DeviceProblem P = P_;
+  P.absorber_stream = -1;  // the responses alone: no absorber term in either finish
+  enqueue_response_spectrum(P, quad_fine_);
+  P.projection = projection_coarse_;
+  enqueue_response_spectrum(P, quad_coarse_);
+  launch_quadrature_score(quad_fine_, quad_coarse_, static_cast<long>(CN), quad_out_, stream_);
+  double value = 0.0;
+  SP_HIP(hipMemcpyAsync(&value, quad_out_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+  SP_HIP(hipStreamSynchronize(stream_));
+  return value;
 }
 
 // ------------------------------------------------------------ batched marching

@@ -93,6 +93,11 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   // (the workspace's record when null).
   // The source evaluation into `out`; without `finish` it stops at the forward transform's
   // streams (the corrector then finishes each bin itself).
+  // The source evaluation's response spectrum at B.lab (the finish without the absorber
+  // term) into `out`, under `P` (whose projection may be the coarse rule's).
+  void enqueue_response_spectrum(const DeviceProblem& P, cplx* out);
+  // The quadrature estimate of the last accepted step's source evaluation, or NaN.
+  double quadrature_estimate();
   void enqueue_source(const cplx* spectrum, cplx* out, const AdamsRecord* R = nullptr,
                       bool finish = true);
   void enqueue_synthesis();
@@ -162,6 +167,16 @@ class FieldWorkspace final : public Propagator, private adams::Engine {
   std::unique_ptr<PlaneStatistics> stats_;  // built at the first statistics call
   double* populations_ = nullptr;            // (2 Q), at the first populations call
   double* diagnostics_ = nullptr;            // (6 Q + 6), at the first diagnostics call
+  // The embedded coarse rule's projection (C, Q), or null; the quadrature estimate's
+  // spectra (C N each) and value, at its first call. The estimate repeats the source
+  // evaluation of the last accepted trial (its field stays in B.lab until the next
+  // trial) at that trial's end, quad_z_.
+  double* projection_coarse_ = nullptr;
+  cplx* quad_fine_ = nullptr;
+  cplx* quad_coarse_ = nullptr;
+  double* quad_out_ = nullptr;
+  bool quad_ready_ = false;
+  double trial_end_ = 0.0, quad_z_ = 0.0;
   MonitorBuffers monitor_;
   double* spatial_matrix_ = nullptr;  // (C, C) row-major
   bool absorbing() const { return in_stage_ || spatial_; }

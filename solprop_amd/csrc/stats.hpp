@@ -33,6 +33,10 @@ enum : std::uint64_t {
   // signals, max over t of sum over its rows of the squared modulus, times the intensity
   // factor (native/src/general/statistics.cpp:517-603); packed after the plasma pair.
   kStatPeakIntensity = 1ULL << 9U,
+  // The quadrature estimate of the last accepted step's source evaluation: its response
+  // projected with the embedded coarse rule against the full one (trial_finish.inc:
+  // 921-1028), read by the workspace; packed after the peak intensity.
+  kStatQuadrature = 1ULL << 10U,
   // Each node's end-of-window ionised fraction and largest rate, (Q, 2), one channel
   // (statistics.cpp:636-650); packed last, after the power trace.
   kStatPopulations = 1ULL << 11U,

@@ -490,9 +490,11 @@ std::vector<double> Workspace::statistics(std::uint64_t flags, const double* win
            "carries none");
   std::vector<double> trace;
   double intensity = 0.0;
-  std::vector<double> out =
-      stats_->run(y_, flags, windows, n_windows, stream_, &trace, &extras, &intensity);
+  std::vector<double> out = stats_->run(y_, flags & ~kStatQuadrature, windows, n_windows,
+                                        stream_, &trace, &extras, &intensity);
   if (flags & kStatPeakIntensity) out.push_back(intensity);
+  // One mode on one node: no radial rule, so no embedded pair to compare against.
+  if (flags & kStatQuadrature) out.push_back(std::nan(""));
   out.insert(out.end(), trace.begin(), trace.end());
   return out;
 }

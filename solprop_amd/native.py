@@ -154,6 +154,8 @@ def lower(problem: Problem, settings: Settings | None = None) -> tuple[_lib.Prob
         d.prefactor_paths[i] = k.path(p)
     d.synthesis = k.f64(problem.synthesis)
     d.projection = k.f64(problem.projection)
+    if problem.projection_coarse is not None:
+        d.projection_coarse = k.f64(problem.projection_coarse)
     d.density = k.path(problem.density)
     d.preionisation = k.path(problem.preionisation)
     if problem.kerr is not None:
@@ -318,7 +320,8 @@ class Workspace:
                   "peak_power_total": 1 << 5,
                   "fwhm": 1 << 6, "electron_density": 1 << 7,
                   "peak_ionisation_rate": 1 << 8, "peak_intensity": 1 << 9,
-                  "populations": 1 << 11, "plasma_integration": 1 << 12}
+                  "quadrature_error": 1 << 10, "populations": 1 << 11,
+                  "plasma_integration": 1 << 12}
 
     def statistics(self, names, windows=None, on_axis=None, intensity_factor=1.0) -> dict:
         """Plane statistics of the committed state (``stats/definitions.py``): a dict of
@@ -328,7 +331,9 @@ class Workspace:
         intensity factor; ``populations`` is (n_nodes, 2): each node's end-of-window
         ionised fraction and largest rate; ``plasma_integration`` the column's six
         integration-diagnostic maxima (:func:`plasma_integration_statistics` forms the
-        three diagnostics of them)."""
+        three diagnostics of them); ``quadrature_error`` the last accepted step's source
+        evaluation repeated with the embedded coarse projection, their relative
+        difference (NaN without an embedded rule or before the first accepted step)."""
         unknown = set(names) - set(self.STATISTICS)
         if unknown:
             raise BackendRefusal(f"statistics not built: {sorted(unknown)}")
@@ -337,7 +342,7 @@ class Workspace:
             flags |= self.STATISTICS[name]
         w = np.zeros((0, 2)) if windows is None else np.ascontiguousarray(windows, np.float64)
         n_modes, n_nodes = self.problem.n_modes, self.problem.n_nodes
-        capacity = 3 * n_modes + 2 * len(w) * n_modes + 4 + self.problem.n_t + 2 * n_nodes + 6
+        capacity = 3 * n_modes + 2 * len(w) * n_modes + 5 + self.problem.n_t + 2 * n_nodes + 6
         out = np.zeros(capacity)
         count = C.c_int()
         axis = None
@@ -352,16 +357,18 @@ class Workspace:
         sizes = {"omega0": n_modes, "energy": n_modes, "energy_window": len(w) * n_modes,
                  "peak_power": n_modes, "peak_power_window": len(w) * n_modes,
                  "peak_power_total": 1, "electron_density": 1,
-                 "peak_ionisation_rate": 1, "peak_intensity": 1, "fwhm": self.problem.n_t,
+                 "peak_ionisation_rate": 1, "peak_intensity": 1, "quadrature_error": 1,
+                 "fwhm": self.problem.n_t,
                  "populations": 2 * n_nodes, "plasma_integration": 6}
         # The packing order (native/src/general/statistics.cpp:160-186): the flags'
-        # order, except that the power trace (the fwhm flag's) follows the peak
-        # intensity, and the populations and then the integration diagnostics come last.
+        # order, except that the power trace (the fwhm flag's) follows the quadrature
+        # estimate, and the populations and then the integration diagnostics come last.
         order = ["omega0", "energy", "energy_window", "peak_power", "peak_power_window",
                  "peak_power_total", "electron_density", "peak_ionisation_rate",
-                 "peak_intensity", "fwhm", "populations", "plasma_integration"]
+                 "peak_intensity", "quadrature_error", "fwhm", "populations",
+                 "plasma_integration"]
         scalars = ("peak_power_total", "electron_density", "peak_ionisation_rate",
-                   "peak_intensity")
+                   "peak_intensity", "quadrature_error")
         for name in order:
             if name in names:
                 chunk = values[at: at + sizes[name]]

@@ -248,6 +248,10 @@ class Problem:
     spatial_absorber: np.ndarray | None = None
     # The Cartesian route's transverse grid (nx, ny); None on the other routes.
     transverse: tuple | None = None
+    # The embedded coarse rule's projection (``Problem.Transform.projection_coarse_matrix``,
+    # _core.pyi:222): (M, Q) float64 with the Gauss weights on the Kronrod nodes they share
+    # and zero elsewhere; None when the rule has no embedded pair.
+    projection_coarse: np.ndarray | None = None
     description: dict = field(default_factory=dict)
 
     @property
@@ -353,6 +357,12 @@ class Problem:
                 raise ValueError("spatial_absorber: expected (n_modes, n_modes) float64")
             if not np.all(np.isfinite(m)):
                 raise ValueError("spatial_absorber must be finite")
+        if self.projection_coarse is not None:
+            m = self.projection_coarse
+            if m.shape != (c, self.n_nodes) or m.dtype != np.float64:
+                raise ValueError(f"projection_coarse: expected {(c, self.n_nodes)} float64")
+            if not np.all(np.isfinite(m)):
+                raise ValueError("projection_coarse must be finite")
 
     def _validate_cartesian(self) -> None:
         """The free-space route's refusals: a power-of-two (ny, nx, nt) grid of at most
@@ -809,6 +819,7 @@ def capillary_problem(
     absorber: str = "none",
     absorber_length: float | None = None,
     integration: str = "band",
+    radial_rule: str = "gauss",
 ) -> Problem:
     """Gas-filled capillary on a real field grid: HE_1m modes, a pressure gradient.
 
@@ -897,15 +908,30 @@ def capillary_problem(
     edges = np.unique(np.concatenate([np.linspace(0, 0.5, 5), fractions, [1.0]])) * length
     w_path = ScalarPath.chebyshev(lambda z: cumulative(z), 0.0, length, degree=24, edges=edges)
     rho_path = ScalarPath.chebyshev(lambda z: density(z), 0.0, length, degree=24, edges=edges)
-    # Synthesis and projection on a Gauss-Legendre rule in r.
-    x, wq = np.polynomial.legendre.leggauss(n_nodes)
-    r = 0.5 * radius * (x + 1.0)
-    wr = 0.5 * radius * wq * 2 * np.pi * r
+    # Synthesis and projection on a Gauss-Legendre rule in r, or on the Kronrod extension
+    # of one (rounded up to 2 n + 1 nodes) with the Gauss rule embedded in it.
+    if radial_rule not in ("gauss", "gauss_kronrod"):
+        raise ValueError(f"unknown radial rule {radial_rule!r}")
+    if radial_rule == "gauss":
+        x, wq = np.polynomial.legendre.leggauss(n_nodes)
+        r = 0.5 * radius * (x + 1.0)
+        wr = 0.5 * radius * wq * 2 * np.pi * r
+    else:
+        from .quadrature import radial_rule as radial_nodes
+
+        xi, w_fine, w_coarse = radial_nodes(n_nodes, True)
+        n_nodes = xi.size
+        r = radius * xi
+        wr = radius * w_fine * 2 * np.pi * r
     synthesis = np.zeros((n_nodes, n_modes))
     for m in range(n_modes):
         norm = math.sqrt(2.0 / (EPS0 * c)) / (math.sqrt(math.pi) * radius * abs(j1(u[m])))
         synthesis[:, m] = norm * j0(u[m] * r / radius)
     projection = np.ascontiguousarray((synthesis * wr[:, None]).T)
+    projection_coarse = None
+    if radial_rule == "gauss_kronrod":
+        wr_coarse = radius * w_coarse * 2 * np.pi * r
+        projection_coarse = np.ascontiguousarray((synthesis * wr_coarse[:, None]).T)
     prefactor = np.zeros((1, n_modes, n), dtype=np.complex128)
     for m in range(n_modes):
         prefactor[0, m] = np.where(active, -1j * omega / 4.0, 0.0)
@@ -960,6 +986,7 @@ def capillary_problem(
         prefactor_paths=(ScalarPath.constant(1.0),),
         synthesis=np.ascontiguousarray(synthesis),
         projection=projection,
+        projection_coarse=projection_coarse,
         density=rho_path,
         preionisation=ScalarPath.constant(0.0),
         kerr=kerr_group,

@@ -261,3 +261,46 @@ def test_plasma_integration_diagnostics_match_oracle(route):
     assert abs(parts[2] - expected[2]) <= 1e-9 * freed
     assert abs(parts[4] - expected[4]) <= 1e-9 * peak
     assert abs(parts[5] - peak) <= 1e-9 * peak
+
+
+@pytest.mark.parametrize("route", ["kerr", "plasma_band"])
+def test_quadrature_estimate_matches_oracle(route):
+    """The quadrature estimate after a march lands on a plane: the last accepted trial's
+    source evaluation (its field, read back from the device, at the plane) repeated by
+    the oracle with the Kronrod and the embedded Gauss projections. The estimate is a
+    relative difference near 1e-2 of two spectra that agree with the oracle's to about
+    1e-12, so it agrees to 1e-7 of itself."""
+    print(f"quadrature estimate {route}", flush=True)
+    if route == "kerr":
+        problem = capillary_problem(n_t=2048, n_nodes=16, n_modes=6, length=0.05,
+                                    plasma=False, integration="trapezoid",
+                                    radial_rule="gauss_kronrod")
+    else:
+        problem = capillary_problem(n_t=2048, n_nodes=16, n_modes=6, length=0.05,
+                                    integration="band", band=(300e-9, 4e-6),
+                                    radial_rule="gauss_kronrod")
+    ws = Workspace(problem)
+    session = ws.march(0.0, problem.field0)
+    assert np.isnan(ws.statistics(["quadrature_error"])["quadrature_error"])  # no step yet
+    session.run(0.004, work_limit=64)
+    assert session.z == 0.004
+    got = ws.statistics(["peak_intensity", "quadrature_error", "energy"],
+                        on_axis=problem.synthesis[:1])
+    lab = ws.buffer("lab").reshape(problem.n_modes, problem.n_omega)
+    ws.close()
+    expected = S.quadrature_error(problem, lab, 0.004)
+    print("device", got["quadrature_error"], "oracle", expected, flush=True)
+    assert 1e-4 < expected < 1.0
+    assert abs(got["quadrature_error"] - expected) <= 1e-7 * expected
+    assert got["energy"].shape == (problem.n_modes,)
+
+
+def test_quadrature_estimate_without_an_embedded_rule_is_nan():
+    problem = capillary_problem(n_t=2048, n_nodes=16, length=0.05, plasma=False,
+                                integration="trapezoid")
+    ws = Workspace(problem)
+    session = ws.march(0.0, problem.field0)
+    session.run(0.002, work_limit=64)
+    value = ws.statistics(["quadrature_error"])["quadrature_error"]
+    ws.close()
+    assert np.isnan(value)

@@ -59,17 +59,17 @@ def test_transverse_collar_off_the_dense_route_is_refused():
             "dense route")
 
 
-def test_unbuilt_statistics_are_refused_by_name():
-    """A statistic this library does not form is refused before the library is asked
-    (the quadrature estimate); the refusal names it."""
+def test_unknown_statistics_are_refused_by_name():
+    """A statistic the native flags do not name is refused before the library is asked;
+    the refusal names it. Every flag of native/include/solprop_native/problem/
+    problem.hpp:44-60 is built."""
     from solprop_amd.native import Workspace as W
 
     ws = W.__new__(W)  # no device: the check precedes every library call
     ws.problem = capillary_problem(n_t=256, n_nodes=4, integration="trapezoid")
-    with pytest.raises(BackendRefusal, match="quadrature_error"):
-        W.statistics(ws, ["energy", "quadrature_error"])
-    with pytest.raises(BackendRefusal, match="quadrature_error"):
-        W.statistics(ws, ["populations", "plasma_integration", "quadrature_error"])
+    with pytest.raises(BackendRefusal, match="walk_off"):
+        W.statistics(ws, ["energy", "walk_off"])
+    assert set(W.STATISTICS.values()) == {1 << k for k in range(13)}
 
 
 def test_scan_refuses_a_point_it_cannot_batch():
