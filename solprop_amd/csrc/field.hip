@@ -1740,7 +1740,10 @@ __device__ inline cplx ls_flow(const DeviceProblem& P, const double* w, long idx
   return cexp(x);
 }
 
-template <int Pass>
+// With Finish (the field route) a pass finishes its bin's source from the forward
+// transform's streams itself, at its evaluation's record (R[0] for pass 1, R[1] for passes
+// 2 and 3, R[2] for pass 4), instead of reading B.source.
+template <int Pass, bool Finish>
 __global__ void __launch_bounds__(kLsThreads)
     ls_pass(DeviceProblem P, const AdamsRecord* __restrict__ R, FieldBuffers B, cplx* acc,
             cplx* prev, double* partials) {
@@ -1750,7 +1753,10 @@ __global__ void __launch_bounds__(kLsThreads)
   const double h = R[0].dz;
   double sq = 0.0;
   if (idx < CN) {
-    const cplx s = B.source[idx];
+    constexpr int at = Pass == 1 ? 0 : (Pass == 4 ? 2 : 1);
+    const cplx s = Finish ? finish_at(P, R[at], B, static_cast<int>(idx / P.N),
+                                      static_cast<int>(idx % P.N))
+                          : B.source[idx];
     const bool scored = P.normmask[idx % P.N] != 0;
     if constexpr (Pass == 1) {
       const cplx flow = ls_flow(P, R[0].x_w, idx, CN);
@@ -2346,16 +2352,27 @@ std::size_t ls_partials(const DeviceProblem& P) {
   return 3 * blocks + (blocks > kLsRun ? 3 * runs : 0);
 }
 
-void launch_ls_pass(int pass, const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
-                    cplx* acc, cplx* prev, double* partials, hipStream_t s) {
+namespace {
+template <bool Finish>
+void launch_ls_pass_for(int pass, const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                        cplx* acc, cplx* prev, double* partials, hipStream_t s) {
   const dim3 grid(ls_blocks(P)), block(kLsThreads);
   switch (pass) {
-    case 1: hipLaunchKernelGGL(ls_pass<1>, grid, block, 0, s, P, R, B, acc, prev, partials); break;
-    case 2: hipLaunchKernelGGL(ls_pass<2>, grid, block, 0, s, P, R, B, acc, prev, partials); break;
-    case 3: hipLaunchKernelGGL(ls_pass<3>, grid, block, 0, s, P, R, B, acc, prev, partials); break;
-    default: hipLaunchKernelGGL(ls_pass<4>, grid, block, 0, s, P, R, B, acc, prev, partials); break;
+    case 1: hipLaunchKernelGGL((ls_pass<1, Finish>), grid, block, 0, s, P, R, B, acc, prev, partials); break;
+    case 2: hipLaunchKernelGGL((ls_pass<2, Finish>), grid, block, 0, s, P, R, B, acc, prev, partials); break;
+    case 3: hipLaunchKernelGGL((ls_pass<3, Finish>), grid, block, 0, s, P, R, B, acc, prev, partials); break;
+    default: hipLaunchKernelGGL((ls_pass<4, Finish>), grid, block, 0, s, P, R, B, acc, prev, partials); break;
   }
   SP_HIP(hipGetLastError());
+}
+}  // namespace
+
+void launch_ls_pass(int pass, const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
+                    cplx* acc, cplx* prev, double* partials, hipStream_t s, bool finish) {
+  if (finish)
+    launch_ls_pass_for<true>(pass, P, R, B, acc, prev, partials, s);
+  else
+    launch_ls_pass_for<false>(pass, P, R, B, acc, prev, partials, s);
 }
 
 void launch_ls_norms(const DeviceProblem& P, const double* partials, double* norms,

@@ -250,8 +250,11 @@ unsigned ls_blocks(const DeviceProblem& P);
 // Doubles of the passes' partial sums: three rows of ls_blocks, and above kLsRun
 // workgroups three rows of the first level's run sums after them.
 std::size_t ls_partials(const DeviceProblem& P);
+// With `finish` (the field route) each pass finishes its bin's source from the forward
+// transform's streams instead of reading B.source.
 void launch_ls_pass(int pass, const DeviceProblem& P, const AdamsRecord* R, FieldBuffers B,
-                    cplx* acc, cplx* prev, double* partials, hipStream_t s);
+                    cplx* acc, cplx* prev, double* partials, hipStream_t s,
+                    bool finish = false);
 void launch_ls_norms(const DeviceProblem& P, const double* partials, double* norms,
                      hipStream_t s);
 // Commits the held trial into store slot `slot`; reads no record, so the host may go on

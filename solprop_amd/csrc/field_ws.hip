@@ -680,14 +680,15 @@ void FieldWorkspace::enqueue_ls_step() {
                         hipMemcpyHostToDevice, stream_));
   // Pass 1 reads the source at the committed state, z; passes 2 and 3 the stage field at
   // the midpoint; pass 4 at the end.
-  enqueue_source(B_.y, B_.source, ls_rec_dev_);
-  launch_ls_pass(1, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_);
-  enqueue_source(B_.lab, B_.source, ls_rec_dev_ + 1);
-  launch_ls_pass(2, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_);
-  enqueue_source(B_.lab, B_.source, ls_rec_dev_ + 1);
-  launch_ls_pass(3, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_);
-  enqueue_source(B_.lab, B_.source, ls_rec_dev_ + 2);
-  launch_ls_pass(4, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_);
+  // Each pass finishes its own bins' source (no finish launch between them).
+  enqueue_source(B_.y, B_.source, ls_rec_dev_, false);
+  launch_ls_pass(1, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_, true);
+  enqueue_source(B_.lab, B_.source, ls_rec_dev_ + 1, false);
+  launch_ls_pass(2, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_, true);
+  enqueue_source(B_.lab, B_.source, ls_rec_dev_ + 1, false);
+  launch_ls_pass(3, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_, true);
+  enqueue_source(B_.lab, B_.source, ls_rec_dev_ + 2, false);
+  launch_ls_pass(4, P_, ls_rec_dev_, B_, ls_acc_, ls_prev_, ls_partials_, stream_, true);
   launch_ls_norms(P_, ls_partials_, B_.norms, stream_);
 }
 
