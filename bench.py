@@ -709,7 +709,9 @@ def run_gpu(args):
             "avg_launch_us": phases[dominant],
             "trial_us": trial_us, "kernels": kernels, "profiled_trials": trials,
             "timing": "HIP events between consecutive kernels on the workspace stream, "
-                      "over the profiled trials that follow the timed region",
+                      "over the profiled trials that follow the timed region; the fused "
+                      "plasma column's current-and-response pass by its own start and end "
+                      "events (hipExtLaunchKernel)",
         }
     else:
         roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
