@@ -409,7 +409,8 @@ int sp_workspace_statistics(sp_workspace* ws, unsigned long long flags, const do
  * dip max(-min dS, 0) and rise max dS, the largest |paid - freed| (paid the trapezoid of
  * W (1 - f), freed f_end - f_0) and freed, and the largest |f E| over the first 2K
  * samples and anywhere. And quadrature_error (1 << 10; trial_finish.inc:921-1028): the
- * last source evaluation of the march's last accepted step (its field and position)
+ * last source evaluation of the march's last accepted step (its field and position; the
+ * low-storage replay's last stage)
  * repeated with the projection and with projection_coarse, each response spectrum
  * weighted by the prefactor times the spectral window, sqrt(sum |w (coarse - fine)|^2 /
  * sum |w fine|^2) over modes and bins; NaN without projection_coarse, before the first

@@ -694,6 +694,10 @@ void FieldWorkspace::enqueue_ls_step() {
 
 double FieldWorkspace::ls_trial(double z, double dz) {
   const double mid = z + 0.5 * dz, end = z + dz;
+  // The step's last source evaluation (its field in B.lab, at `end`) is the one a
+  // quadrature estimate repeats once the step is accepted.
+  quad_ready_ = false;
+  trial_end_ = end;
   // The three positions' scalars (each evaluation reads its own position), and the
   // flows' weights: Phi(mid) - Phi(z) in record 0, Phi(end) - Phi(mid) in record 1.
   const double at[3] = {z, mid, end};
@@ -725,6 +729,8 @@ void FieldWorkspace::ls_accept(double dz) {
   // The hard mask was applied by the step's last pass; nothing else finishes a step here.
   (void)dz;
   accepted_ += 1;
+  quad_ready_ = true;
+  quad_z_ = trial_end_;
 }
 
 namespace {
@@ -1070,9 +1076,8 @@ void FieldWorkspace::enqueue_response_spectrum(const DeviceProblem& P, cplx* out
 }
 
 double FieldWorkspace::quadrature_estimate() {
-  // NaN without an embedded rule, before an accepted step since the state was loaded,
-  // and on the low-storage replay (whose last stage field this estimate does not keep).
-  if (!projection_coarse_ || !quad_ready_ || low_storage_) return std::nan("");
+  // NaN without an embedded rule or before an accepted step since the state was loaded.
+  if (!projection_coarse_ || !quad_ready_) return std::nan("");
   const std::size_t CN = static_cast<std::size_t>(P_.C) * P_.N;
   if (!quad_fine_) {
     quad_fine_ = alloc<cplx>(CN);

@@ -131,3 +131,25 @@ def test_session_over_the_replay():
     state, _, _ = device_replay(problem)
     expected, _, _ = ls.replay(problem, problem.field0.reshape(problem.shape), problem.schedule)
     assert rel_l2(state, expected.reshape(state.shape)) < 1e-12
+
+
+def test_quadrature_estimate_of_the_last_stage():
+    """After a replay lands on an edge, the quadrature estimate repeats the step's last
+    source evaluation (its stage field, read back from the device, at the step's end)
+    with both projections, as ``low_storage.cpp:219`` scores the last stage; the oracle's
+    estimate of that field within 1e-7 of itself."""
+    from oracle import stats as S
+
+    print("low-storage quadrature estimate", flush=True)
+    problem = replay_problem(steps=10, n_nodes=16, n_modes=6, radial_rule="gauss_kronrod")
+    ws = Workspace(problem)
+    march = ws.march(0.0, problem.field0)
+    target = float(problem.schedule[3])
+    march.advance(target, work_limit=100000)
+    got = ws.statistics(["quadrature_error"])["quadrature_error"]
+    lab = ws.buffer("lab").reshape(problem.n_modes, problem.n_omega)
+    ws.close()
+    expected = S.quadrature_error(problem, lab, target)
+    print("device", got, "oracle", expected, flush=True)
+    assert 1e-4 < expected < 1.0
+    assert abs(got - expected) <= 1e-7 * expected
