@@ -25,6 +25,9 @@ void launch_raman_kernel_post(const cplx* z, cplx* H, const cplx* roots2T, int T
                               hipStream_t s);
 void launch_env_response(const DeviceProblem& P, const TrialRecord* R, cplx* fine,
                          const cplx* conv, int stage, hipStream_t s);
+// Stage `stage - 1`'s finish and stage `stage`'s entry in one launch (stage >= 1).
+void launch_rk_finish_entry(const DeviceProblem& P, const TrialRecord* R, const cplx* y,
+                            cplx* k, cplx* fine, cplx* einv, int stage, hipStream_t s);
 void launch_rk_finish(const DeviceProblem& P, const TrialRecord* R, const cplx* fine,
                       const cplx* einv, cplx* k, int stage, hipStream_t s);
 void launch_rk_accept(const DeviceProblem& P, const TrialRecord* R, const cplx* y,

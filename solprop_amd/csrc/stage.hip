@@ -161,6 +161,23 @@ __global__ void env_response(DeviceProblem P, const TrialRecord* R, cplx* fine,
 
 // ------------------------------------------------------------------ stage finish
 
+// Stage `stage`'s derivative at coarse bin b of mode c from its transformed response:
+// k = einv (p W P~ + A).
+__device__ inline cplx stage_finish(const DeviceProblem& P, const TrialRecord* R,
+                                    const cplx* fine, const cplx* einv, int stage, int c,
+                                    int b) {
+  const long CN = static_cast<long>(P.C) * P.N;
+  const long cb = static_cast<long>(c) * P.N + b;
+  const cplx pref = phase_sum(R->stage[stage].pref_w, P.pref, P.n_pref, CN, cb);
+  const cplx pt = fine[static_cast<long>(c) * P.NF + P.crop[b]];
+  cplx normalised = mul(scale(P.omega_window[b], pref), pt);
+  // The absorber's term joins unscaled, on the active band (stepping.py:88-145).
+  if (P.absorber_stream >= 0 && P.active[b])
+    normalised = add(normalised,
+                     fine[static_cast<long>(P.absorber_stream + c) * P.NF + P.crop[b]]);
+  return mul(einv[cb], normalised);
+}
+
 __global__ void rk_finish(DeviceProblem P, const TrialRecord* R, const cplx* fine,
                           const cplx* einv, cplx* k, int stage) {
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -168,14 +185,32 @@ __global__ void rk_finish(DeviceProblem P, const TrialRecord* R, const cplx* fin
   if (idx >= CN) return;
   const int c = static_cast<int>(idx / P.N);
   const int b = static_cast<int>(idx % P.N);
-  const cplx pref = phase_sum(R->stage[stage].pref_w, P.pref, P.n_pref, CN, idx);
-  const cplx pt = fine[static_cast<long>(c) * P.NF + P.crop[b]];
-  cplx normalised = mul(scale(P.omega_window[b], pref), pt);
-  // The absorber's term joins unscaled, on the active band (stepping.py:88-145).
-  if (P.absorber_stream >= 0 && P.active[b])
-    normalised = add(normalised,
-                     fine[static_cast<long>(P.absorber_stream + c) * P.NF + P.crop[b]]);
-  k[stage * CN + idx] = mul(einv[idx], normalised);
+  k[stage * CN + idx] = stage_finish(P, R, fine, einv, stage, c, b);
+}
+
+// The previous stage's finish fused into this stage's entry. Each coarse bin b is the
+// crop of exactly one fine bin f, whose thread reads the previous stage's transformed
+// response at f and that stage's einv at b before it writes this stage's input at f and
+// einv at b; the stage input then reads the derivative it just stored.
+__global__ void rk_finish_entry(DeviceProblem P, const TrialRecord* R, const cplx* y, cplx* k,
+                                cplx* fine, cplx* einv, int stage) {
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long total = static_cast<long>(P.C) * P.T;
+  if (idx >= total) return;
+  const int c = static_cast<int>(idx / P.T);
+  const int f = static_cast<int>(idx % P.T);
+  const int b = P.fine_to_coarse[f];
+  if (b < 0) {
+    fine[idx] = make(0.0, 0.0);
+    return;
+  }
+  const long CN = static_cast<long>(P.C) * P.N;
+  const long cb = static_cast<long>(c) * P.N + b;
+  k[(stage - 1) * CN + cb] = stage_finish(P, R, fine, einv, stage - 1, c, b);
+  cplx back;
+  const cplx u = stage_lab(P, R, y, k, stage, c, b, &back);
+  fine[idx] = scale(P.inv_T, u);
+  einv[cb] = back;
 }
 
 // ----------------------------------------------------------------- step accept
@@ -309,6 +344,14 @@ void launch_env_response(const DeviceProblem& P, const TrialRecord* R, cplx* fin
                          const cplx* conv, int stage, hipStream_t s) {
   hipLaunchKernelGGL(env_response, dim3(blocks_for(P.T, kThreads)), dim3(kThreads), 0, s, P, R,
                      fine, conv, stage);
+  check_launch();
+}
+
+void launch_rk_finish_entry(const DeviceProblem& P, const TrialRecord* R, const cplx* y,
+                            cplx* k, cplx* fine, cplx* einv, int stage, hipStream_t s) {
+  const std::size_t n = static_cast<std::size_t>(P.C) * P.T;
+  hipLaunchKernelGGL(rk_finish_entry, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, P,
+                     R, y, k, fine, einv, stage);
   check_launch();
 }
 

@@ -225,8 +225,13 @@ void Workspace::enqueue_raman_kernel(int stage) {
   launch_raman_kernel_post(drive_, H_, roots2T_, P_.T, P_.dt_fine, stream_);
 }
 
-void Workspace::enqueue_stage(int stage) {
-  launch_rk_entry_envelope(P_, rec_dev_, y_, k_, fine_, einv_, stage, stream_);
+void Workspace::enqueue_stage(int stage, bool finish_previous) {
+  // The previous stage's finish rides in this stage's entry; the last stage finishes on
+  // its own.
+  if (finish_previous)
+    launch_rk_finish_entry(P_, rec_dev_, y_, k_, fine_, einv_, stage, stream_);
+  else
+    launch_rk_entry_envelope(P_, rec_dev_, y_, k_, fine_, einv_, stage, stream_);
   fine_plan_.exec(fine_, scratch_, +1, stream_);
   if (P_.n_raman) {
     if (raman_dynamic_) enqueue_raman_kernel(stage);
@@ -237,7 +242,7 @@ void Workspace::enqueue_stage(int stage) {
   }
   launch_env_response(P_, rec_dev_, fine_, drive_, stage, stream_);
   (in_stage_ ? forward_plan_ : fine_plan_).exec(fine_, scratch_, -1, stream_);
-  launch_rk_finish(P_, rec_dev_, fine_, einv_, k_, stage, stream_);
+  if (stage == P_.n_stages - 1) launch_rk_finish(P_, rec_dev_, fine_, einv_, k_, stage, stream_);
 }
 
 void Workspace::enqueue_commit(const TrialRecord* R, double dz) {
@@ -261,7 +266,8 @@ void Workspace::enqueue_trial(bool commit, Stage0 mode) {
   const long CN = static_cast<long>(P_.C) * P_.N;
   if (commit) enqueue_commit(rec_dev_, 0.0);
   if (mode == kCopyRecycled) launch_copy_points(recycled_, k_, CN, stream_);
-  for (int i = (mode == kEvaluate ? 0 : 1); i < P_.n_stages; ++i) enqueue_stage(i);
+  const int first = mode == kEvaluate ? 0 : 1;
+  for (int i = first; i < P_.n_stages; ++i) enqueue_stage(i, i > first);
   launch_rk_accept(P_, rec_dev_, y_, k_, adv_, recycled_, partials_, stream_);
 }
 

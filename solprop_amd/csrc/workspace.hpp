@@ -52,7 +52,8 @@ class Workspace final : public Propagator {
   T* device_alloc(std::size_t n);
   void fill_record(double z, double dz);
   void enqueue_trial(bool commit, Stage0 mode);
-  void enqueue_stage(int stage);
+  // One stage; with `finish_previous` its entry also finishes the stage before it.
+  void enqueue_stage(int stage, bool finish_previous);
   void enqueue_raman_kernel(int stage);
   void ensure_committed();
   // The commit of the held trial: the band mask and, under the split-step absorber, the
