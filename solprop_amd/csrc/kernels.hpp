@@ -28,11 +28,11 @@ void launch_env_response(const DeviceProblem& P, const TrialRecord* R, cplx* fin
 // Stage `stage - 1`'s finish and stage `stage`'s entry in one launch (stage >= 1).
 void launch_rk_finish_entry(const DeviceProblem& P, const TrialRecord* R, const cplx* y,
                             cplx* k, cplx* fine, cplx* einv, int stage, hipStream_t s);
-void launch_rk_finish(const DeviceProblem& P, const TrialRecord* R, const cplx* fine,
-                      const cplx* einv, cplx* k, int stage, hipStream_t s);
-void launch_rk_accept(const DeviceProblem& P, const TrialRecord* R, const cplx* y,
-                      const cplx* k, cplx* adv, cplx* recycled, double* partials,
-                      hipStream_t s);
+// The step's acceptance; it first finishes the last stage from its transformed response
+// (`fine`, `einv`), bin by bin.
+void launch_rk_accept(const DeviceProblem& P, const TrialRecord* R, const cplx* y, cplx* k,
+                      cplx* adv, cplx* recycled, double* partials, const cplx* fine,
+                      const cplx* einv, hipStream_t s);
 void launch_commit_masked(const DeviceProblem& P, const cplx* adv, cplx* y, hipStream_t s);
 void launch_copy_points(const cplx* src, cplx* dst, long n, hipStream_t s);
 // The envelope route's split-step absorber, around the caller's inverse and forward

@@ -178,16 +178,6 @@ __device__ inline cplx stage_finish(const DeviceProblem& P, const TrialRecord* R
   return mul(einv[cb], normalised);
 }
 
-__global__ void rk_finish(DeviceProblem P, const TrialRecord* R, const cplx* fine,
-                          const cplx* einv, cplx* k, int stage) {
-  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const long CN = static_cast<long>(P.C) * P.N;
-  if (idx >= CN) return;
-  const int c = static_cast<int>(idx / P.N);
-  const int b = static_cast<int>(idx % P.N);
-  k[stage * CN + idx] = stage_finish(P, R, fine, einv, stage, c, b);
-}
-
 // The previous stage's finish fused into this stage's entry. Each coarse bin b is the
 // crop of exactly one fine bin f, whose thread reads the previous stage's transformed
 // response at f and that stage's einv at b before it writes this stage's input at f and
@@ -239,13 +229,16 @@ __device__ inline void block_sum3(double v0, double v1, double v2, double* out, 
 }
 
 __global__ void __launch_bounds__(kAcceptThreads)
-    rk_accept(DeviceProblem P, const TrialRecord* R, const cplx* y, const cplx* k, cplx* adv,
-              cplx* recycled, double* partials) {
+    rk_accept(DeviceProblem P, const TrialRecord* R, const cplx* y, cplx* k, cplx* adv,
+              cplx* recycled, double* partials, const cplx* fine, const cplx* einv) {
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const long CN = static_cast<long>(P.C) * P.N;
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
   if (idx < CN) {
     const int b = static_cast<int>(idx % P.N);
+    // The last stage's finish first (stage_finish, bin by bin), its derivative stored.
+    const int last = P.n_stages - 1;
+    k[last * CN + idx] = stage_finish(P, R, fine, einv, last, static_cast<int>(idx / P.N), b);
     cplx sol = make(0.0, 0.0), err = make(0.0, 0.0);
     for (int i = 0; i < P.n_stages; ++i) {
       const cplx ki = k[i * CN + idx];
@@ -355,19 +348,11 @@ void launch_rk_finish_entry(const DeviceProblem& P, const TrialRecord* R, const 
   check_launch();
 }
 
-void launch_rk_finish(const DeviceProblem& P, const TrialRecord* R, const cplx* fine,
-                      const cplx* einv, cplx* k, int stage, hipStream_t s) {
-  const std::size_t n = static_cast<std::size_t>(P.C) * P.N;
-  hipLaunchKernelGGL(rk_finish, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, P, R, fine,
-                     einv, k, stage);
-  check_launch();
-}
-
-void launch_rk_accept(const DeviceProblem& P, const TrialRecord* R, const cplx* y,
-                      const cplx* k, cplx* adv, cplx* recycled, double* partials,
-                      hipStream_t s) {
+void launch_rk_accept(const DeviceProblem& P, const TrialRecord* R, const cplx* y, cplx* k,
+                      cplx* adv, cplx* recycled, double* partials, const cplx* fine,
+                      const cplx* einv, hipStream_t s) {
   hipLaunchKernelGGL(rk_accept, dim3(accept_blocks(P)), dim3(kAcceptThreads), 0, s, P, R, y, k,
-                     adv, recycled, partials);
+                     adv, recycled, partials, fine, einv);
   check_launch();
 }
 

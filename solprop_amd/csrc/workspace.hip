@@ -226,8 +226,8 @@ void Workspace::enqueue_raman_kernel(int stage) {
 }
 
 void Workspace::enqueue_stage(int stage, bool finish_previous) {
-  // The previous stage's finish rides in this stage's entry; the last stage finishes on
-  // its own.
+  // The previous stage's finish rides in this stage's entry, and the last stage's in the
+  // step's acceptance.
   if (finish_previous)
     launch_rk_finish_entry(P_, rec_dev_, y_, k_, fine_, einv_, stage, stream_);
   else
@@ -242,7 +242,6 @@ void Workspace::enqueue_stage(int stage, bool finish_previous) {
   }
   launch_env_response(P_, rec_dev_, fine_, drive_, stage, stream_);
   (in_stage_ ? forward_plan_ : fine_plan_).exec(fine_, scratch_, -1, stream_);
-  if (stage == P_.n_stages - 1) launch_rk_finish(P_, rec_dev_, fine_, einv_, k_, stage, stream_);
 }
 
 void Workspace::enqueue_commit(const TrialRecord* R, double dz) {
@@ -268,7 +267,7 @@ void Workspace::enqueue_trial(bool commit, Stage0 mode) {
   if (mode == kCopyRecycled) launch_copy_points(recycled_, k_, CN, stream_);
   const int first = mode == kEvaluate ? 0 : 1;
   for (int i = first; i < P_.n_stages; ++i) enqueue_stage(i, i > first);
-  launch_rk_accept(P_, rec_dev_, y_, k_, adv_, recycled_, partials_, stream_);
+  launch_rk_accept(P_, rec_dev_, y_, k_, adv_, recycled_, partials_, fine_, einv_, stream_);
 }
 
 double Workspace::finish_trial() {
